@@ -1,0 +1,324 @@
+/*
+ * pt_api.h -- C ABI of the MI355X path tracer (libpt_hip.so).
+ *
+ * This is the drop-in boundary for the reference's per-sample render path:
+ * everything the reference hands to OptiX 6 (scene graph, acceleration
+ * structures, camera variables, the per-frame launch and the output-buffer
+ * maps) crosses this interface as plain structs, pointers and sizes.  No C++,
+ * no HIP and no torch types appear in these signatures.
+ *
+ * Ownership: every input is deep-copied; the context owns all device memory.
+ * Errors: every call returns a pt_status; nothing throws or exits across the
+ *         ABI.  pt_last_error() gives a human-readable message.
+ * Threading: a context must not be used from two threads at once; distinct
+ *         contexts are independent.  pt_render is asynchronous on the
+ *         context's stream; pt_sync and pt_read_* synchronise.
+ *
+ * Reference semantics: integrator defaults reproduce the reference
+ * (programs/raygen.cu:66-123) unless a PT_FLAG_* switch says otherwise.
+ */
+#ifndef PT_API_H
+#define PT_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PT_API_VERSION 1
+
+/* ------------------------------------------------------------------ status */
+typedef int32_t pt_status;
+#define PT_OK 0
+#define PT_ERR_INVALID_ARG (-1)  /* bad pointer, size or enum value          */
+#define PT_ERR_DEVICE (-2)       /* HIP runtime error (message in last_error) */
+#define PT_ERR_STATE (-3)        /* call out of order (e.g. render w/o accel) */
+#define PT_ERR_OUT_OF_MEMORY (-4)
+#define PT_ERR_UNSUPPORTED (-5)
+
+/* ------------------------------------------------------------ enumerations */
+/* Axis of an axis-aligned rectangle (programs/common.hpp:26 AXIS). */
+#define PT_AXIS_X 0
+#define PT_AXIS_Y 1
+#define PT_AXIS_Z 2
+
+/* Primitive kinds of a geometry group (the BLAS an instance refers to). */
+#define PT_GEOM_SPHERES 0   /* programs/hitables/sphere.cu              */
+#define PT_GEOM_RECTS 1     /* programs/hitables/aarect.cu              */
+#define PT_GEOM_BOXES 2     /* programs/hitables/box.cu                 */
+#define PT_GEOM_MESH 3      /* programs/hitables/triangle.cu, mesh.hpp  */
+
+/* Materials (host_includes/materials.hpp). */
+#define PT_MAT_LAMBERTIAN 0        /* tex[0]=albedo                            */
+#define PT_MAT_METAL 1             /* tex[0]=albedo, param[0]=fuzz             */
+#define PT_MAT_DIELECTRIC 2        /* tex[0]=base, tex[1]=extinction,
+                                      param[0]=ior, param[1]=density           */
+#define PT_MAT_DIFFUSE_LIGHT 3     /* tex[0]=emission                          */
+#define PT_MAT_ASHIKHMIN_SHIRLEY 4 /* tex[0]=diffuse, tex[1]=specular,
+                                      param[0]=nu, param[1]=nv (already >= 1)  */
+#define PT_MAT_TORRANCE_SPARROW 5  /* tex[0]=Rs, param[0]=alpha_u,
+                                      param[1]=alpha_v (already roughness^2)   */
+#define PT_MAT_OREN_NAYAR 6        /* tex[0]=albedo, param[0]=A, param[1]=B    */
+#define PT_MAT_COUNT 7
+
+/* Textures (host_includes/textures.hpp). */
+#define PT_TEX_CONSTANT 0  /* rgb                                           */
+#define PT_TEX_IMAGE 1     /* width x height float4 texels, bilinear, repeat */
+
+/* Lights (host_includes/pdfs.hpp + Light_Sampler, host_common.hpp:65). */
+#define PT_LIGHT_RECT 0    /* axis, a0, a1, b0, b1, k                   */
+#define PT_LIGHT_SPHERE 1  /* center, radius                            */
+#define PT_LIGHT_ENV 2     /* importance-sampled environment (miss texture) */
+
+/* Miss programs (host_includes/programs.hpp:32 Miss_Programs). */
+#define PT_MISS_GRADIENT 0
+#define PT_MISS_CONSTANT 1
+#define PT_MISS_IMAGE 2    /* LDR lat-long background (miss.cu:51)      */
+#define PT_MISS_HDR 3      /* environment map (miss.cu:64)              */
+
+/* Render flags. */
+#define PT_FLAG_MATERIAL_SORT 0x1u   /* sort shading by material key         */
+#define PT_FLAG_KEEP_PATHS 0x2u      /* keep last frame's per-path radiance  */
+#define PT_FLAG_ENV_MIS 0x4u         /* env NEE + MIS, no miss clamp (new)   */
+#define PT_FLAG_COUNT_TRAVERSAL 0x8u /* count node/primitive fetches         */
+
+/* ------------------------------------------------------------ scene input */
+typedef struct pt_sphere_desc {
+  float center[3];
+  float radius;
+  int32_t material;
+} pt_sphere_desc;
+
+typedef struct pt_rect_desc {
+  int32_t axis; /* PT_AXIS_* */
+  int32_t flip; /* non-zero: normal points along -axis */
+  float a0, a1, b0, b1, k;
+  int32_t material;
+} pt_rect_desc;
+
+typedef struct pt_box_desc {
+  float pmin[3];
+  float pmax[3];
+  int32_t material;
+} pt_box_desc;
+
+typedef struct pt_mesh_desc {
+  uint32_t num_vertices;
+  const float* vertices; /* 3 * num_vertices                                */
+  const float* normals;  /* optional, 3 * num_vertices (NULL: geometric N)  */
+  const float* uvs;      /* optional, 2 * num_vertices (NULL: u = v = 0)    */
+  uint32_t num_triangles;
+  const uint32_t* indices;       /* 3 * num_triangles                       */
+  const int32_t* face_materials; /* optional per-face material (NULL: use
+                                    `material` for every face)               */
+  int32_t material;
+} pt_mesh_desc;
+
+/* One instance = one group of primitives of a single kind (a bottom-level
+ * acceleration structure) placed in the world by an affine transform.  The
+ * matrices are row-major 3x4: p_world = M * (p_obj, 1).  Instances with the
+ * same (kind, first, count) share one bottom-level structure. */
+typedef struct pt_instance_desc {
+  int32_t kind;   /* PT_GEOM_* */
+  uint32_t first; /* first primitive (spheres/rects/boxes) or mesh index */
+  uint32_t count; /* primitive count (ignored for PT_GEOM_MESH)          */
+  int32_t has_transform;
+  float obj_to_world[12];
+  float world_to_obj[12];
+} pt_instance_desc;
+
+typedef struct pt_material_desc {
+  int32_t type; /* PT_MAT_* */
+  int32_t tex[2];
+  float param[4];
+} pt_material_desc;
+
+typedef struct pt_texture_desc {
+  int32_t type; /* PT_TEX_* */
+  float rgb[3];
+  uint32_t width, height;
+  const float* texels; /* width * height * 4 floats, row 0 = v in [0, 1/h) */
+} pt_texture_desc;
+
+typedef struct pt_light_desc {
+  int32_t type; /* PT_LIGHT_* */
+  int32_t axis; /* rect only */
+  float a0, a1, b0, b1, k;
+  float center[3]; /* sphere only */
+  float radius;
+  float emission[3]; /* Light_Emissions entry (programs.hpp:26) */
+} pt_light_desc;
+
+typedef struct pt_miss_desc {
+  int32_t type; /* PT_MISS_* */
+  float color1[3];
+  float color2[3];
+  int32_t texture;   /* PT_MISS_IMAGE / PT_MISS_HDR: texture index */
+  int32_t spherical; /* PT_MISS_HDR: 1 spherical, 0 cylindrical   */
+} pt_miss_desc;
+
+typedef struct pt_scene_desc {
+  uint32_t num_spheres;
+  const pt_sphere_desc* spheres;
+  uint32_t num_rects;
+  const pt_rect_desc* rects;
+  uint32_t num_boxes;
+  const pt_box_desc* boxes;
+  uint32_t num_meshes;
+  const pt_mesh_desc* meshes;
+  uint32_t num_instances;
+  const pt_instance_desc* instances;
+  uint32_t num_materials;
+  const pt_material_desc* materials;
+  uint32_t num_textures;
+  const pt_texture_desc* textures;
+  uint32_t num_lights;
+  const pt_light_desc* lights;
+  pt_miss_desc miss;
+} pt_scene_desc;
+
+/* Camera variables exactly as Camera::set uploads them (camera.hpp:40-51). */
+typedef struct pt_camera {
+  float lower_left[3];
+  float horizontal[3];
+  float vertical[3];
+  float origin[3];
+  float u[3], v[3], w[3];
+  float lens_radius;
+  float time0, time1;
+} pt_camera;
+
+typedef struct pt_render_params {
+  uint32_t width, height;   /* launch dimensions (main.cpp:33)             */
+  uint32_t max_depth;       /* maxDepth (main.cpp:63)                      */
+  int32_t russian;          /* russian (main.cpp:62)                       */
+  uint32_t rr_min_depth;    /* RR applies when depth >= this; reference 11 */
+  uint32_t frames_per_batch;/* frames traced together (0 = automatic)      */
+  uint32_t flags;           /* PT_FLAG_*                                   */
+  uint32_t reserved;
+} pt_render_params;
+
+typedef struct pt_render_stats {
+  uint64_t paths;           /* camera paths traced                          */
+  uint64_t extension_rays;  /* closest-hit queries                          */
+  uint64_t shadow_rays;     /* occlusion queries                            */
+  uint64_t node_fetches;    /* only with PT_FLAG_COUNT_TRAVERSAL            */
+  uint64_t prim_fetches;    /* only with PT_FLAG_COUNT_TRAVERSAL            */
+  uint64_t instance_fetches;/* only with PT_FLAG_COUNT_TRAVERSAL            */
+  double gpu_ms;            /* render wall time on the stream (events)      */
+  double extend_ms;         /* summed closest-hit kernel time               */
+  double shadow_ms;         /* summed occlusion kernel time                 */
+  uint32_t extend_launches;
+  uint32_t shadow_launches;
+} pt_render_stats;
+
+typedef struct pt_accel_stats {
+  uint32_t num_blas;
+  uint32_t num_instances;
+  uint64_t num_primitives;
+  uint64_t num_nodes;       /* 128-byte BVH4 nodes (all levels)             */
+  uint64_t bytes;           /* device bytes of nodes + primitive records    */
+  double build_ms;
+} pt_accel_stats;
+
+/* Ray and hit records of the query entry points. */
+typedef struct pt_ray {
+  float org[3];
+  float tmin;
+  float dir[3]; /* not normalised; t is in units of |dir| */
+  float tmax;
+} pt_ray;
+
+typedef struct pt_hit {
+  float t;      /* +inf on a miss                                 */
+  int32_t inst; /* instance index, -1 on a miss                   */
+  int32_t prim; /* primitive index of that kind (sphere/rect/box
+                   index in the scene arrays, or triangle index in
+                   the mesh); -1 on a miss                         */
+  uint32_t reserved;
+} pt_hit;
+
+/* ------------------------------------------------------------ entry points */
+/* Context::create() (host_includes/host_common.hpp:24). */
+pt_status pt_create(int32_t device, struct pt_context** out);
+/* Context destruction; frees all device memory. */
+void pt_destroy(struct pt_context* ctx);
+/* Message of the last failing call on ctx (or of a failed pt_create when
+ * ctx is NULL).  Valid until the next call on ctx. */
+const char* pt_last_error(const struct pt_context* ctx);
+
+/* Scene graph assembly (hitables.hpp:518 addElementsTo, transforms.hpp:400
+ * addAndTransform, programs.hpp:17-92 light table and miss program).
+ * Deep copy; invalidates the acceleration structure. */
+pt_status pt_set_scene(struct pt_context* ctx, const pt_scene_desc* scene);
+
+/* createAcceleration("Trbvh") + the warm-up launch renderFrame(ctx,0,0)
+ * (scenes.hpp:31-32, main.cpp:99): GPU LBVH build of every bottom-level
+ * structure and the top level. */
+pt_status pt_build_accel(struct pt_context* ctx, pt_accel_stats* out_or_null);
+
+/* Camera::set (camera.hpp:40-51). */
+pt_status pt_set_camera(struct pt_context* ctx, const pt_camera* cam);
+
+/* renderFrame / Context::launch(0, W, H) for frames
+ * [frame_begin, frame_begin + frame_count) (main.cpp:26-39, :264-265).
+ * frame_begin == 0 clears the accumulation (raygen.cu:154).  Asynchronous;
+ * stats (if given) are filled after an internal synchronisation. */
+pt_status pt_render(struct pt_context* ctx, const pt_render_params* params,
+                    uint32_t frame_begin, uint32_t frame_count,
+                    pt_render_stats* out_or_null);
+
+pt_status pt_sync(struct pt_context* ctx);
+
+/* accBuffer->map() (image_save.hpp:11): W*H*3 doubles, row 0 = top of the
+ * image; *frames = number of accumulated frames. */
+pt_status pt_read_accum(struct pt_context* ctx, double* rgb, uint32_t* frames);
+
+/* displayBuffer->map() (main.cpp:269): W*H*4 bytes, make_Color of the
+ * accumulation (raygen.cu:136-146). */
+pt_status pt_read_display(struct pt_context* ctx, uint8_t* rgba);
+
+/* Per-path result of the last rendered frame (needs PT_FLAG_KEEP_PATHS):
+ * rgb = W*H*3 floats of color() before de_nan, indexed by launch index
+ * (x + W*y); signature = W*H hashes of the path's discrete decisions;
+ * bounces = W*H bounce counts.  Any pointer may be NULL. */
+pt_status pt_read_paths(struct pt_context* ctx, float* rgb,
+                        uint32_t* signature, uint32_t* bounces);
+
+/* rtTrace of a radiance ray (raygen.cu:77): closest hit in (tmin, tmax),
+ * ties broken by (instance, primitive).  Host arrays. */
+pt_status pt_trace_closest(struct pt_context* ctx, const pt_ray* rays,
+                           uint64_t n, pt_hit* hits);
+/* Same on device-resident arrays, asynchronous on the context stream. */
+pt_status pt_trace_closest_device(struct pt_context* ctx, const pt_ray* d_rays,
+                                  uint64_t n, pt_hit* d_hits);
+/* Counted variant: per-ray node, primitive and instance fetch counts
+ * (3 uint32 per ray), host arrays. */
+pt_status pt_trace_closest_counted(struct pt_context* ctx, const pt_ray* rays,
+                                   uint64_t n, pt_hit* hits, uint32_t* counts);
+
+/* Shadow rtTrace (light_sample.cuh:61-72) under the deterministic rule:
+ * lit (0) iff the closest hit in (tmin, tmax) is a diffuse light or there is
+ * no hit; occluded (1) otherwise.  Host arrays. */
+pt_status pt_trace_occluded(struct pt_context* ctx, const pt_ray* rays,
+                            uint64_t n, uint8_t* occluded);
+
+/* Copies of the built acceleration structure for bit-exact checks.
+ * Pass NULL buffers to query sizes.  nodes: 32 uint32 per 128-byte node;
+ * prim_order: sorted primitive order of every bottom-level structure
+ * concatenated; morton: the 30-bit codes in sorted order. */
+pt_status pt_read_accel(struct pt_context* ctx, uint32_t* nodes,
+                        uint64_t* num_nodes, uint32_t* prim_order,
+                        uint32_t* morton, uint64_t* num_prims,
+                        uint32_t* blas_info, uint32_t* num_blas);
+
+/* Version of this ABI (PT_API_VERSION). */
+int32_t pt_api_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PT_API_H */

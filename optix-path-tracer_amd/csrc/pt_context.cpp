@@ -1,0 +1,957 @@
+// pt_context.cpp -- implementation of the C ABI declared in include/pt_api.h.
+//
+// Host orchestration only: validation, HBM allocation, uploads, the LBVH
+// build sequence and the wavefront bounce loop.  All per-ray and
+// per-primitive work runs in the gfx950 kernels of pt_render.hip and
+// pt_bvh.hip; there is no CPU fallback.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/pt_api.h"
+#include "pt_kernels.hpp"
+#include "pt_layout.hpp"
+
+using namespace pt;
+
+namespace {
+
+constexpr int MAX_GRID = 2048;         // blocks of the grid-stride kernels
+constexpr uint32_t PATHS_TARGET = 1u << 21;  // paths in flight per batch (automatic)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct BlasInfo {
+  int kind;
+  uint32_t first, count;
+  uint32_t node_base, node_count;
+  uint32_t prim_base;
+};
+
+}  // namespace
+
+struct pt_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // ---- scene (host copies)
+  bool has_scene = false;
+  std::vector<pt_sphere_desc> spheres;
+  std::vector<pt_rect_desc> rects;
+  std::vector<pt_box_desc> boxes;
+  struct MeshHost {
+    std::vector<float> v, n, uv;
+    std::vector<uint32_t> idx;
+    std::vector<int32_t> fmat;
+    int32_t mat;
+  };
+  std::vector<MeshHost> meshes;
+  std::vector<pt_instance_desc> instances;
+  std::vector<pt_material_desc> materials;
+  std::vector<pt_texture_desc> textures;
+  std::vector<std::vector<float>> texels;
+  std::vector<pt_light_desc> lights;
+  pt_miss_desc miss{};
+
+  // ---- device scene buffers
+  std::vector<DevBuf> scene_bufs;  // original-order data, freed on set_scene
+  std::vector<DevBuf> accel_bufs;  // acceleration data, freed on rebuild
+  DevScene dsc{};
+  bool accel_ready = false;
+  std::vector<BlasInfo> blas;
+  uint64_t total_nodes = 0, total_prims_sorted = 0;
+  uint32_t tlas_nodes = 0;
+  uint4* nodes_dev = nullptr;
+  uint32_t* order_dev = nullptr;   // every BLAS order, then the TLAS order
+  uint32_t* morton_dev = nullptr;
+  uint64_t accel_bytes = 0;
+
+  // ---- camera and render state
+  bool has_camera = false;
+  pt_camera cam{};
+  uint32_t W = 0, H = 0;
+  uint32_t frames = 0;
+  double* accum = nullptr;
+  uchar4* rgba = nullptr;
+  uint32_t last_fpb = 0;        // frames of the last batch
+  uint32_t last_frame_slot = 0; // slot base of the last frame in its batch
+  bool paths_valid = false;
+
+  // ---- path queues
+  uint64_t path_capacity = 0;
+  std::vector<DevBuf> path_bufs;
+  DevPaths paths{};
+  uint2* spill = nullptr;
+  uint64_t* stats = nullptr;
+  uint32_t* qcounters = nullptr;  // counters of the query kernels
+};
+
+namespace {
+
+pt_status fail(pt_context* c, pt_status s, const std::string& m) {
+  if (c) c->err = m;
+  return s;
+}
+pt_status hip_fail(pt_context* c, hipError_t e, const char* what) {
+  return fail(c, PT_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPCHK(ctx, call)                                  \
+  do {                                                     \
+    hipError_t e_ = (call);                                \
+    if (e_ != hipSuccess) return hip_fail(ctx, e_, #call); \
+  } while (0)
+
+hipError_t dalloc(std::vector<DevBuf>& list, void** out, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) return e;
+  list.push_back({p, bytes});
+  *out = p;
+  return hipSuccess;
+}
+void dfree(std::vector<DevBuf>& list) {
+  for (auto& b : list) (void)hipFree(b.p);
+  list.clear();
+}
+template <class T>
+hipError_t upload(pt_context* c, std::vector<DevBuf>& list, T** out, const void* src, size_t bytes) {
+  void* p = nullptr;
+  hipError_t e = dalloc(list, &p, bytes);
+  if (e != hipSuccess) return e;
+  if (bytes) {
+    e = hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) return e;
+  }
+  *out = static_cast<T*>(p);
+  return hipSuccess;
+}
+
+inline uint32_t fb(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  return u;
+}
+inline float bf(uint32_t u) {
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+int grid_for(uint64_t n) { return (int)std::max<uint64_t>(1, std::min<uint64_t>((n + BLOCK - 1) / BLOCK, MAX_GRID)); }
+}  // namespace
+
+// gather kernels of pt_bvh.hip: BVH-ordered primitive and instance records
+namespace pt {
+void launch_gather_records(const float4* src, const uint32_t* order, uint32_t n, uint32_t stride, float4* dst,
+                           uint32_t first, int mode, hipStream_t st);
+void launch_gather_tris(const float4* vtx, const uint4* idx, const uint32_t* order, uint32_t n, float4* dst,
+                        hipStream_t st);
+void launch_gather_instances(const uint4* src, const uint32_t* order, uint32_t n, uint4* dst, hipStream_t st);
+}  // namespace pt
+
+extern "C" {
+
+int32_t pt_api_version(void) { return PT_API_VERSION; }
+
+static std::string g_create_err;
+
+pt_status pt_create(int32_t device, pt_context** out) {
+  if (!out) return PT_ERR_INVALID_ARG;
+  *out = nullptr;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) {
+    g_create_err = std::string("no HIP device available: ") + hipGetErrorString(e);
+    return PT_ERR_DEVICE;
+  }
+  if (device < 0 || device >= n) {
+    g_create_err = "device index out of range";
+    return PT_ERR_INVALID_ARG;
+  }
+  e = hipSetDevice(device);
+  if (e != hipSuccess) {
+    g_create_err = std::string("hipSetDevice: ") + hipGetErrorString(e);
+    return PT_ERR_DEVICE;
+  }
+  pt_context* c = new pt_context();
+  c->device = device;
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    g_create_err = std::string("hipStreamCreate: ") + hipGetErrorString(e);
+    delete c;
+    return PT_ERR_DEVICE;
+  }
+  size_t spill_bytes = (size_t)MAX_GRID * BLOCK * STK_SPILL * sizeof(uint2);
+  if ((e = hipMalloc(&c->spill, spill_bytes)) != hipSuccess ||
+      (e = hipMalloc(&c->stats, STAT_WORDS * sizeof(uint64_t))) != hipSuccess ||
+      (e = hipMalloc(&c->qcounters, CTR_WORDS * sizeof(uint32_t))) != hipSuccess) {
+    g_create_err = std::string("hipMalloc: ") + hipGetErrorString(e);
+    pt_destroy(c);
+    return PT_ERR_OUT_OF_MEMORY;
+  }
+  launch_fill_u32(c->qcounters, CTR_WORDS, 0u, c->stream);
+  *out = c;
+  return PT_OK;
+}
+
+void pt_destroy(pt_context* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  dfree(c->scene_bufs);
+  dfree(c->accel_bufs);
+  dfree(c->path_bufs);
+  if (c->accum) (void)hipFree(c->accum);
+  if (c->rgba) (void)hipFree(c->rgba);
+  if (c->spill) (void)hipFree(c->spill);
+  if (c->stats) (void)hipFree(c->stats);
+  if (c->qcounters) (void)hipFree(c->qcounters);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* pt_last_error(const pt_context* c) { return c ? c->err.c_str() : g_create_err.c_str(); }
+
+pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
+  if (!c || !d) return PT_ERR_INVALID_ARG;
+  (void)hipSetDevice(c->device);
+  // ---- validation
+  auto bad = [&](const std::string& m) { return fail(c, PT_ERR_INVALID_ARG, m); };
+  if ((d->num_spheres && !d->spheres) || (d->num_rects && !d->rects) || (d->num_boxes && !d->boxes) ||
+      (d->num_meshes && !d->meshes) || (d->num_instances && !d->instances) || (d->num_materials && !d->materials) ||
+      (d->num_textures && !d->textures) || (d->num_lights && !d->lights))
+    return bad("scene array pointer is NULL with a non-zero count");
+  if (d->num_instances == 0) return bad("scene has no instances");
+  auto mat_ok = [&](int32_t m) { return m >= 0 && (uint32_t)m < d->num_materials; };
+  auto tex_ok = [&](int32_t t) { return t >= 0 && (uint32_t)t < d->num_textures; };
+  for (uint32_t i = 0; i < d->num_spheres; i++)
+    if (!mat_ok(d->spheres[i].material)) return bad("sphere material index out of range");
+  for (uint32_t i = 0; i < d->num_rects; i++) {
+    if (!mat_ok(d->rects[i].material)) return bad("rect material index out of range");
+    if (d->rects[i].axis < 0 || d->rects[i].axis > 2) return bad("invalid rect axis");
+  }
+  for (uint32_t i = 0; i < d->num_boxes; i++)
+    if (!mat_ok(d->boxes[i].material)) return bad("box material index out of range");
+  for (uint32_t i = 0; i < d->num_meshes; i++) {
+    const pt_mesh_desc& m = d->meshes[i];
+    if (m.num_triangles == 0 || !m.vertices || !m.indices) return bad("mesh without vertices or triangles");
+    for (uint64_t k = 0; k < 3ull * m.num_triangles; k++)
+      if (m.indices[k] >= m.num_vertices) return bad("mesh index out of range");
+    if (m.face_materials) {
+      for (uint32_t k = 0; k < m.num_triangles; k++)
+        if (!mat_ok(m.face_materials[k])) return bad("mesh face material out of range");
+    } else if (!mat_ok(m.material)) {
+      return bad("mesh material index out of range");
+    }
+  }
+  for (uint32_t i = 0; i < d->num_instances; i++) {
+    const pt_instance_desc& in = d->instances[i];
+    uint64_t lim = in.kind == PT_GEOM_SPHERES ? d->num_spheres
+                   : in.kind == PT_GEOM_RECTS ? d->num_rects
+                   : in.kind == PT_GEOM_BOXES ? d->num_boxes
+                   : in.kind == PT_GEOM_MESH ? d->num_meshes
+                                             : 0;
+    if (in.kind < 0 || in.kind > 3) return bad("invalid instance kind");
+    if (in.kind == PT_GEOM_MESH) {
+      if (in.first >= lim) return bad("instance mesh index out of range");
+    } else if (in.count == 0 || (uint64_t)in.first + in.count > lim) {
+      return bad("instance primitive range out of range");
+    }
+  }
+  for (uint32_t i = 0; i < d->num_materials; i++) {
+    const pt_material_desc& m = d->materials[i];
+    if (m.type < 0 || m.type >= PT_MAT_COUNT) return bad("invalid material type");
+    if (!tex_ok(m.tex[0])) return bad("material texture index out of range");
+    if ((m.type == PT_MAT_ASHIKHMIN_SHIRLEY || m.type == PT_MAT_DIELECTRIC) && !tex_ok(m.tex[1]))
+      return bad("material second texture index out of range");
+  }
+  for (uint32_t i = 0; i < d->num_textures; i++) {
+    const pt_texture_desc& t = d->textures[i];
+    if (t.type == PT_TEX_IMAGE && (t.width == 0 || t.height == 0 || !t.texels)) return bad("empty image texture");
+    if (t.type != PT_TEX_IMAGE && t.type != PT_TEX_CONSTANT) return bad("invalid texture type");
+  }
+  for (uint32_t i = 0; i < d->num_lights; i++)
+    if (d->lights[i].type != PT_LIGHT_RECT && d->lights[i].type != PT_LIGHT_SPHERE)
+      return fail(c, PT_ERR_UNSUPPORTED, "light type not supported");
+  if (d->miss.type < 0 || d->miss.type > 3) return bad("invalid miss program");
+  if ((d->miss.type == PT_MISS_IMAGE || d->miss.type == PT_MISS_HDR) && !tex_ok(d->miss.texture))
+    return bad("miss texture index out of range");
+
+  // ---- host copies
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  dfree(c->scene_bufs);
+  dfree(c->accel_bufs);
+  c->accel_ready = false;
+  c->paths_valid = false;
+  c->spheres.assign(d->spheres, d->spheres + d->num_spheres);
+  c->rects.assign(d->rects, d->rects + d->num_rects);
+  c->boxes.assign(d->boxes, d->boxes + d->num_boxes);
+  c->meshes.clear();
+  for (uint32_t i = 0; i < d->num_meshes; i++) {
+    const pt_mesh_desc& m = d->meshes[i];
+    pt_context::MeshHost h;
+    h.v.assign(m.vertices, m.vertices + 3ull * m.num_vertices);
+    if (m.normals) h.n.assign(m.normals, m.normals + 3ull * m.num_vertices);
+    if (m.uvs) h.uv.assign(m.uvs, m.uvs + 2ull * m.num_vertices);
+    h.idx.assign(m.indices, m.indices + 3ull * m.num_triangles);
+    if (m.face_materials) h.fmat.assign(m.face_materials, m.face_materials + m.num_triangles);
+    h.mat = m.material;
+    c->meshes.push_back(std::move(h));
+  }
+  c->instances.assign(d->instances, d->instances + d->num_instances);
+  c->materials.assign(d->materials, d->materials + d->num_materials);
+  c->textures.assign(d->textures, d->textures + d->num_textures);
+  c->texels.clear();
+  for (auto& t : c->textures) {
+    std::vector<float> tx;
+    if (t.type == PT_TEX_IMAGE) tx.assign(t.texels, t.texels + 4ull * t.width * t.height);
+    c->texels.push_back(std::move(tx));
+    t.texels = nullptr;
+  }
+  c->lights.assign(d->lights, d->lights + d->num_lights);
+  c->miss = d->miss;
+
+  // ---- original-order device data
+  DevScene& s = c->dsc;
+  s = DevScene{};
+  std::vector<float4> sph_o(c->spheres.size());
+  std::vector<int32_t> sph_mat(c->spheres.size());
+  for (size_t i = 0; i < c->spheres.size(); i++) {
+    auto& q = c->spheres[i];
+    sph_o[i] = make_float4(q.center[0], q.center[1], q.center[2], q.radius);
+    sph_mat[i] = q.material;
+  }
+  std::vector<float4> rect_o(2 * c->rects.size());
+  std::vector<int32_t> rect_mat(c->rects.size());
+  for (size_t i = 0; i < c->rects.size(); i++) {
+    auto& q = c->rects[i];
+    rect_o[2 * i] = make_float4(q.a0, q.a1, q.b0, q.b1);
+    rect_o[2 * i + 1] = make_float4(q.k, bf((uint32_t)q.axis), bf(q.flip ? 1u : 0u), bf((uint32_t)i));
+    rect_mat[i] = q.material;
+  }
+  std::vector<float4> box_o(2 * c->boxes.size());
+  std::vector<int32_t> box_mat(c->boxes.size());
+  for (size_t i = 0; i < c->boxes.size(); i++) {
+    auto& q = c->boxes[i];
+    box_o[2 * i] = make_float4(q.pmin[0], q.pmin[1], q.pmin[2], bf((uint32_t)i));
+    box_o[2 * i + 1] = make_float4(q.pmax[0], q.pmax[1], q.pmax[2], 0.f);
+    box_mat[i] = q.material;
+  }
+  std::vector<DevMesh> dmesh;
+  std::vector<float4> vtx, nrm;
+  std::vector<float2> uv;
+  std::vector<uint4> idx;
+  std::vector<int32_t> fmat;
+  for (auto& m : c->meshes) {
+    DevMesh dm{};
+    uint32_t nv = (uint32_t)(m.v.size() / 3);
+    dm.vbase = (uint32_t)vtx.size();
+    dm.ibase = (uint32_t)idx.size();
+    dm.ntri = (uint32_t)(m.idx.size() / 3);
+    dm.has_n = !m.n.empty();
+    dm.has_uv = !m.uv.empty();
+    dm.mat = m.mat;
+    dm.fmat_base = m.fmat.empty() ? -1 : (int32_t)fmat.size();
+    for (uint32_t k = 0; k < nv; k++) {
+      vtx.push_back(make_float4(m.v[3 * k], m.v[3 * k + 1], m.v[3 * k + 2], 0.f));
+      nrm.push_back(dm.has_n ? make_float4(m.n[3 * k], m.n[3 * k + 1], m.n[3 * k + 2], 0.f) : make_float4(0, 0, 0, 0));
+      uv.push_back(dm.has_uv ? make_float2(m.uv[2 * k], m.uv[2 * k + 1]) : make_float2(0, 0));
+    }
+    for (uint32_t k = 0; k < dm.ntri; k++) idx.push_back(make_uint4(m.idx[3 * k], m.idx[3 * k + 1], m.idx[3 * k + 2], 0));
+    fmat.insert(fmat.end(), m.fmat.begin(), m.fmat.end());
+    dmesh.push_back(dm);
+  }
+  std::vector<DevMaterial> mats;
+  for (auto& m : c->materials) {
+    DevMaterial dm{m.type, m.tex[0], m.tex[1], m.param[0], m.param[1], m.param[2], m.param[3], 0};
+    mats.push_back(dm);
+  }
+  std::vector<DevTexture> texs;
+  std::vector<float4> texel_pool;
+  for (size_t i = 0; i < c->textures.size(); i++) {
+    auto& t = c->textures[i];
+    DevTexture dt{t.type, t.rgb[0], t.rgb[1], t.rgb[2], t.width, t.height, (uint32_t)texel_pool.size(), 0};
+    const auto& tx = c->texels[i];
+    for (size_t k = 0; k + 3 < tx.size(); k += 4) texel_pool.push_back(make_float4(tx[k], tx[k + 1], tx[k + 2], tx[k + 3]));
+    texs.push_back(dt);
+  }
+  std::vector<DevLight> lts;
+  for (auto& l : c->lights) {
+    DevLight dl{l.type, l.axis, l.a0, l.a1, l.b0, l.b1, l.k, l.center[0], l.center[1], l.center[2], l.radius,
+                l.emission[0], l.emission[1], l.emission[2], {0.f, 0.f}};
+    lts.push_back(dl);
+  }
+  auto& L = c->scene_bufs;
+  HIPCHK(c, upload(c, L, const_cast<float4**>(&s.sph_o), sph_o.data(), sph_o.size() * sizeof(float4)));
+  HIPCHK(c, upload(c, L, const_cast<int32_t**>(&s.sph_mat), sph_mat.data(), sph_mat.size() * 4));
+  HIPCHK(c, upload(c, L, const_cast<float4**>(&s.rect_o), rect_o.data(), rect_o.size() * sizeof(float4)));
+  HIPCHK(c, upload(c, L, const_cast<int32_t**>(&s.rect_mat), rect_mat.data(), rect_mat.size() * 4));
+  HIPCHK(c, upload(c, L, const_cast<float4**>(&s.box_o), box_o.data(), box_o.size() * sizeof(float4)));
+  HIPCHK(c, upload(c, L, const_cast<int32_t**>(&s.box_mat), box_mat.data(), box_mat.size() * 4));
+  HIPCHK(c, upload(c, L, const_cast<DevMesh**>(&s.meshes), dmesh.data(), dmesh.size() * sizeof(DevMesh)));
+  HIPCHK(c, upload(c, L, const_cast<float4**>(&s.vtx), vtx.data(), vtx.size() * sizeof(float4)));
+  HIPCHK(c, upload(c, L, const_cast<float4**>(&s.nrm), nrm.data(), nrm.size() * sizeof(float4)));
+  HIPCHK(c, upload(c, L, const_cast<float2**>(&s.uv), uv.data(), uv.size() * sizeof(float2)));
+  HIPCHK(c, upload(c, L, const_cast<uint4**>(&s.idx), idx.data(), idx.size() * sizeof(uint4)));
+  HIPCHK(c, upload(c, L, const_cast<int32_t**>(&s.fmat), fmat.data(), fmat.size() * 4));
+  HIPCHK(c, upload(c, L, const_cast<DevMaterial**>(&s.mats), mats.data(), mats.size() * sizeof(DevMaterial)));
+  HIPCHK(c, upload(c, L, const_cast<DevTexture**>(&s.texs), texs.data(), texs.size() * sizeof(DevTexture)));
+  HIPCHK(c, upload(c, L, const_cast<float4**>(&s.texels), texel_pool.data(), texel_pool.size() * sizeof(float4)));
+  HIPCHK(c, upload(c, L, const_cast<DevLight**>(&s.lights), lts.data(), lts.size() * sizeof(DevLight)));
+  s.num_lights = (int32_t)lts.size();
+  s.miss = c->miss;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->has_scene = true;
+  return PT_OK;
+}
+
+pt_status pt_build_accel(pt_context* c, pt_accel_stats* out) {
+  if (!c) return PT_ERR_INVALID_ARG;
+  if (!c->has_scene) return fail(c, PT_ERR_STATE, "pt_build_accel before pt_set_scene");
+  (void)hipSetDevice(c->device);
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipEventRecord(e0, c->stream));
+  dfree(c->accel_bufs);
+  c->accel_ready = false;
+  DevScene& s = c->dsc;
+  auto& A = c->accel_bufs;
+
+  // ---- bottom-level structures, one per distinct (kind, first, count)
+  std::map<std::tuple<int, uint32_t, uint32_t>, int> blas_of;
+  std::vector<int> inst_blas(c->instances.size());
+  c->blas.clear();
+  uint64_t maxn = c->instances.size();
+  uint32_t pool_n[4] = {0, 0, 0, 0};
+  for (size_t i = 0; i < c->instances.size(); i++) {
+    const auto& in = c->instances[i];
+    uint32_t count = in.kind == PT_GEOM_MESH ? (uint32_t)(c->meshes[in.first].idx.size() / 3) : in.count;
+    auto key = std::make_tuple((int)in.kind, in.first, count);
+    auto it = blas_of.find(key);
+    if (it == blas_of.end()) {
+      BlasInfo b{in.kind, in.first, count, 0, 0, pool_n[in.kind]};
+      pool_n[in.kind] += count;
+      it = blas_of.emplace(key, (int)c->blas.size()).first;
+      c->blas.push_back(b);
+      maxn = std::max<uint64_t>(maxn, count);
+    }
+    inst_blas[i] = it->second;
+  }
+  // ---- build scratch
+  BuildScratch sb{};
+  std::vector<DevBuf> S;
+  uint64_t cap = maxn + 1;
+  uint64_t nblk = (cap + 2047) / 2048;
+  auto A32 = [&](uint32_t** p, uint64_t n) { return dalloc(S, (void**)p, n * 4); };
+  auto AF4 = [&](float4** p, uint64_t n) { return dalloc(S, (void**)p, n * sizeof(float4)); };
+  hipError_t e = hipSuccess;
+  if ((e = AF4(&sb.lo, cap)) || (e = AF4(&sb.hi, cap)) || (e = A32(&sb.code_a, cap)) || (e = A32(&sb.code_b, cap)) ||
+      (e = A32(&sb.idx_a, cap)) || (e = A32(&sb.idx_b, cap)) || (e = A32(&sb.hist, 256 * nblk + 64)) ||
+      (e = A32(&sb.left, cap)) || (e = A32(&sb.right, cap)) || (e = A32(&sb.first, cap)) ||
+      (e = A32(&sb.last, cap)) || (e = A32(&sb.parent, 2 * cap)) || (e = A32(&sb.flags, cap)) ||
+      (e = AF4(&sb.nlo, cap)) || (e = AF4(&sb.nhi, cap)) || (e = A32(&sb.frontier_a, cap)) ||
+      (e = A32(&sb.frontier_b, cap)) || (e = A32(&sb.slots, 4 * cap)) || (e = A32(&sb.nchild, cap + 64)) ||
+      (e = A32(&sb.scan_tmp, (256 * nblk + cap) / 256 + 1024)) || (e = A32(&sb.cbounds, 8)) ||
+      (e = A32(&sb.misc, 64))) {
+    dfree(S);
+    return hip_fail(c, e, "hipMalloc (build scratch)");
+  }
+  sb.capacity = cap;
+
+  // per-BLAS outputs into temporaries, then compacted
+  std::vector<uint4*> bnodes(c->blas.size());
+  std::vector<uint32_t> bcount(c->blas.size());
+  float4* roots = nullptr;  // 2 per BLAS
+  if ((e = dalloc(A, (void**)&roots, 2 * sizeof(float4) * (c->blas.size() + 1)))) {
+    dfree(S);
+    return hip_fail(c, e, "hipMalloc");
+  }
+  uint64_t total_sorted = 0;
+  for (auto& b : c->blas) total_sorted += b.count;
+  total_sorted += c->instances.size();
+  if ((e = dalloc(A, (void**)&c->order_dev, total_sorted * 4)) ||
+      (e = dalloc(A, (void**)&c->morton_dev, total_sorted * 4))) {
+    dfree(S);
+    return hip_fail(c, e, "hipMalloc");
+  }
+  // sorted primitive pools
+  float4 *sph = nullptr, *rect = nullptr, *box = nullptr, *tri = nullptr;
+  uint32_t* sph_idx = nullptr;
+  if ((e = dalloc(A, (void**)&sph, pool_n[0] * sizeof(float4))) ||
+      (e = dalloc(A, (void**)&sph_idx, pool_n[0] * 4)) ||
+      (e = dalloc(A, (void**)&rect, 2ull * pool_n[1] * sizeof(float4))) ||
+      (e = dalloc(A, (void**)&box, 2ull * pool_n[2] * sizeof(float4))) ||
+      (e = dalloc(A, (void**)&tri, 3ull * pool_n[3] * sizeof(float4)))) {
+    dfree(S);
+    return hip_fail(c, e, "hipMalloc (primitive pools)");
+  }
+  uint64_t order_off = 0;
+  std::vector<uint64_t> order_base(c->blas.size());
+  for (size_t b = 0; b < c->blas.size(); b++) {
+    BlasInfo& bi = c->blas[b];
+    uint32_t n = bi.count;
+    switch (bi.kind) {
+      case PT_GEOM_SPHERES: launch_bounds_spheres(s.sph_o + bi.first, n, sb.lo, sb.hi, c->stream); break;
+      case PT_GEOM_RECTS: launch_bounds_rects(s.rect_o + 2ull * bi.first, n, sb.lo, sb.hi, c->stream); break;
+      case PT_GEOM_BOXES: launch_bounds_boxes(s.box_o + 2ull * bi.first, n, sb.lo, sb.hi, c->stream); break;
+      default: {
+        // device mesh table offsets are computed in set_scene in mesh order
+        uint32_t vbase = 0, ibase = 0;
+        for (uint32_t m = 0; m < bi.first; m++) {
+          vbase += (uint32_t)(c->meshes[m].v.size() / 3);
+          ibase += (uint32_t)(c->meshes[m].idx.size() / 3);
+        }
+        launch_bounds_tris(s.vtx + vbase, s.idx + ibase, n, sb.lo, sb.hi, c->stream);
+      }
+    }
+    uint4* nodes_tmp = nullptr;
+    if ((e = dalloc(S, (void**)&nodes_tmp, (size_t)std::max<uint32_t>(1, n) * 128))) {
+      dfree(S);
+      return hip_fail(c, e, "hipMalloc (nodes)");
+    }
+    uint32_t nn = build_bvh4(sb, n, BLAS_LEAF_MAX, nodes_tmp, c->order_dev + order_off, c->morton_dev + order_off,
+                             roots + 2 * b, c->stream);
+    if (nn == 0) {
+      dfree(S);
+      return fail(c, PT_ERR_DEVICE, "bottom-level build failed");
+    }
+    bnodes[b] = nodes_tmp;
+    bcount[b] = nn;
+    order_base[b] = order_off;
+    // sorted primitive records of this structure
+    const uint32_t* ord = c->order_dev + order_off;
+    switch (bi.kind) {
+      case PT_GEOM_SPHERES:
+        launch_gather_records(s.sph_o + bi.first, ord, n, 1, sph + bi.prim_base, bi.first, 0, c->stream);
+        launch_gather_records(s.sph_o + bi.first, ord, n, 1, (float4*)(sph_idx + bi.prim_base), bi.first, 1,
+                              c->stream);
+        break;
+      case PT_GEOM_RECTS:
+        launch_gather_records(s.rect_o + 2ull * bi.first, ord, n, 2, rect + 2ull * bi.prim_base, bi.first, 0,
+                              c->stream);
+        break;
+      case PT_GEOM_BOXES:
+        launch_gather_records(s.box_o + 2ull * bi.first, ord, n, 2, box + 2ull * bi.prim_base, bi.first, 0,
+                              c->stream);
+        break;
+      default: {
+        uint32_t vbase = 0, ibase = 0;
+        for (uint32_t m = 0; m < bi.first; m++) {
+          vbase += (uint32_t)(c->meshes[m].v.size() / 3);
+          ibase += (uint32_t)(c->meshes[m].idx.size() / 3);
+        }
+        launch_gather_tris(s.vtx + vbase, s.idx + ibase, ord, n, tri + 3ull * bi.prim_base, c->stream);
+      }
+    }
+    order_off += n;
+  }
+  uint64_t blas_nodes = 0;
+  for (size_t b = 0; b < c->blas.size(); b++) {
+    c->blas[b].node_base = (uint32_t)blas_nodes;
+    c->blas[b].node_count = bcount[b];
+    blas_nodes += bcount[b];
+  }
+  // ---- instances (unsorted) and the top level
+  const uint32_t ni = (uint32_t)c->instances.size();
+  std::vector<uint4> irec(8ull * ni);
+  std::vector<uint32_t> mesh_of(ni, 0);
+  for (uint32_t i = 0; i < ni; i++) {
+    const auto& in = c->instances[i];
+    const BlasInfo& bi = c->blas[inst_blas[i]];
+    const float* w = in.world_to_obj;
+    const float* m = in.obj_to_world;
+    uint4* r = &irec[8ull * i];
+    r[0] = make_uint4(fb(w[0]), fb(w[1]), fb(w[2]), fb(w[3]));
+    r[1] = make_uint4(fb(w[4]), fb(w[5]), fb(w[6]), fb(w[7]));
+    r[2] = make_uint4(fb(w[8]), fb(w[9]), fb(w[10]), fb(w[11]));
+    r[3] = make_uint4(bi.node_base, bi.prim_base, (uint32_t)in.kind | ((in.has_transform ? 1u : 0u) << 8), i);
+    r[4] = make_uint4(fb(m[0]), fb(m[1]), fb(m[2]), fb(m[3]));
+    r[5] = make_uint4(fb(m[4]), fb(m[5]), fb(m[6]), fb(m[7]));
+    r[6] = make_uint4(fb(m[8]), fb(m[9]), fb(m[10]), fb(m[11]));
+    r[7] = make_uint4(in.first, bi.count, in.kind == PT_GEOM_MESH ? in.first : 0u, (uint32_t)inst_blas[i]);
+  }
+  uint4* irec_dev = nullptr;
+  if ((e = upload(c, S, &irec_dev, irec.data(), irec.size() * sizeof(uint4)))) {
+    dfree(S);
+    return hip_fail(c, e, "upload instances");
+  }
+  launch_bounds_instances(irec_dev, roots, ni, sb.lo, sb.hi, c->stream);
+  uint4* tnodes = nullptr;
+  if ((e = dalloc(S, (void**)&tnodes, (size_t)std::max<uint32_t>(1, ni) * 128))) {
+    dfree(S);
+    return hip_fail(c, e, "hipMalloc (tlas)");
+  }
+  uint32_t tn = build_bvh4(sb, ni, TLAS_LEAF_MAX, tnodes, c->order_dev + order_off, c->morton_dev + order_off,
+                           roots + 2 * c->blas.size(), c->stream);
+  if (tn == 0) {
+    dfree(S);
+    return fail(c, PT_ERR_DEVICE, "top-level build failed");
+  }
+  uint4* insts_sorted = nullptr;
+  if ((e = dalloc(A, (void**)&insts_sorted, irec.size() * sizeof(uint4)))) {
+    dfree(S);
+    return hip_fail(c, e, "hipMalloc");
+  }
+  launch_gather_instances(irec_dev, c->order_dev + order_off, ni, insts_sorted, c->stream);
+  // ---- compact all nodes into one array (BLAS after BLAS, then TLAS)
+  c->total_nodes = blas_nodes + tn;
+  if ((e = dalloc(A, (void**)&c->nodes_dev, c->total_nodes * 128))) {
+    dfree(S);
+    return hip_fail(c, e, "hipMalloc (nodes)");
+  }
+  for (size_t b = 0; b < c->blas.size(); b++)
+    HIPCHK(c, hipMemcpyAsync(c->nodes_dev + 8ull * c->blas[b].node_base, bnodes[b], (size_t)bcount[b] * 128,
+                             hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->nodes_dev + 8ull * blas_nodes, tnodes, (size_t)tn * 128, hipMemcpyDeviceToDevice,
+                           c->stream));
+  HIPCHK(c, hipEventRecord(e1, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  dfree(S);
+  c->tlas_nodes = tn;
+  c->total_prims_sorted = total_sorted;
+  s.nodes = c->nodes_dev;
+  s.tlas_root = (uint32_t)blas_nodes;
+  s.num_instances = ni;
+  s.insts = insts_sorted;
+  s.sph = sph;
+  s.sph_idx = sph_idx;
+  s.rect = rect;
+  s.box = box;
+  s.tri = tri;
+  c->accel_bytes = c->total_nodes * 128 + pool_n[0] * 20ull + pool_n[1] * 32ull + pool_n[2] * 32ull +
+                   pool_n[3] * 48ull + ni * 128ull;
+  c->accel_ready = true;
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (out) {
+    out->num_blas = (uint32_t)c->blas.size();
+    out->num_instances = ni;
+    out->num_primitives = total_sorted - ni;
+    out->num_nodes = c->total_nodes;
+    out->bytes = c->accel_bytes;
+    out->build_ms = ms;
+  }
+  return PT_OK;
+}
+
+pt_status pt_read_accel(pt_context* c, uint32_t* nodes, uint64_t* num_nodes, uint32_t* prim_order, uint32_t* morton,
+                        uint64_t* num_prims, uint32_t* blas_info, uint32_t* num_blas) {
+  if (!c) return PT_ERR_INVALID_ARG;
+  if (!c->accel_ready) return fail(c, PT_ERR_STATE, "acceleration structure not built");
+  (void)hipSetDevice(c->device);
+  if (num_nodes) *num_nodes = c->total_nodes;
+  if (num_prims) *num_prims = c->total_prims_sorted;
+  if (num_blas) *num_blas = (uint32_t)c->blas.size();
+  if (nodes) HIPCHK(c, hipMemcpy(nodes, c->nodes_dev, c->total_nodes * 128, hipMemcpyDeviceToHost));
+  if (prim_order) HIPCHK(c, hipMemcpy(prim_order, c->order_dev, c->total_prims_sorted * 4, hipMemcpyDeviceToHost));
+  if (morton) HIPCHK(c, hipMemcpy(morton, c->morton_dev, c->total_prims_sorted * 4, hipMemcpyDeviceToHost));
+  if (blas_info) {
+    uint64_t pb = 0;
+    for (size_t b = 0; b < c->blas.size(); b++) {
+      blas_info[6 * b + 0] = c->blas[b].node_base;
+      blas_info[6 * b + 1] = c->blas[b].node_count;
+      blas_info[6 * b + 2] = (uint32_t)pb;
+      blas_info[6 * b + 3] = c->blas[b].count;
+      blas_info[6 * b + 4] = (uint32_t)c->blas[b].kind;
+      blas_info[6 * b + 5] = c->blas[b].first;
+      pb += c->blas[b].count;
+    }
+  }
+  return PT_OK;
+}
+
+pt_status pt_set_camera(pt_context* c, const pt_camera* cam) {
+  if (!c || !cam) return PT_ERR_INVALID_ARG;
+  c->cam = *cam;
+  c->has_camera = true;
+  return PT_OK;
+}
+
+static pt_status ensure_paths(pt_context* c, uint64_t cap) {
+  if (cap <= c->path_capacity) return PT_OK;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  dfree(c->path_bufs);
+  c->path_capacity = 0;
+  DevPaths& P = c->paths;
+  auto& B = c->path_bufs;
+  hipError_t e;
+  auto F4 = [&](float4** p) { return dalloc(B, (void**)p, cap * sizeof(float4)); };
+  auto U32 = [&](uint32_t** p) { return dalloc(B, (void**)p, cap * 4); };
+  if ((e = F4(&P.ray_o)) || (e = F4(&P.ray_d)) || (e = F4(&P.thr)) || (e = F4(&P.rad)) || (e = U32(&P.seed)) ||
+      (e = U32(&P.sig)) || (e = F4(&P.hit)) || (e = F4(&P.nee_p)) || (e = F4(&P.nee_n)) || (e = F4(&P.nee_wo)) ||
+      (e = F4(&P.nee_wi)) || (e = F4(&P.nee_c)) || (e = dalloc(B, (void**)&P.occl, cap)) || (e = F4(&P.result)) ||
+      (e = U32(&P.list_a)) || (e = U32(&P.list_b)) || (e = U32(&P.list_shadow)) || (e = U32(&P.list_sorted)) ||
+      (e = U32(&P.keys)) || (e = dalloc(B, (void**)&P.counters, CTR_WORDS * 4))) {
+    dfree(B);
+    return hip_fail(c, e, "hipMalloc (path queues)");
+  }
+  launch_fill_u32(P.counters, CTR_WORDS, 0u, c->stream);
+  c->path_capacity = cap;
+  return PT_OK;
+}
+
+pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_begin, uint32_t frame_count,
+                    pt_render_stats* out) {
+  if (!c || !rp) return PT_ERR_INVALID_ARG;
+  if (!c->accel_ready) return fail(c, PT_ERR_STATE, "pt_render before pt_build_accel");
+  if (!c->has_camera) return fail(c, PT_ERR_STATE, "pt_render before pt_set_camera");
+  if (rp->width == 0 || rp->height == 0) return fail(c, PT_ERR_INVALID_ARG, "width and height must be positive");
+  if ((uint64_t)rp->width * rp->height > (1ull << 28)) return fail(c, PT_ERR_INVALID_ARG, "image too large");
+  if (rp->max_depth == 0) return fail(c, PT_ERR_INVALID_ARG, "max_depth must be positive");
+  (void)hipSetDevice(c->device);
+  const uint32_t WH = rp->width * rp->height;
+  // accumulation buffers
+  if (frame_begin == 0 || rp->width != c->W || rp->height != c->H || !c->accum) {
+    if (frame_begin != 0 && (rp->width != c->W || rp->height != c->H))
+      return fail(c, PT_ERR_STATE, "resolution changed without restarting at frame 0");
+    if (rp->width != c->W || rp->height != c->H || !c->accum) {
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      if (c->accum) (void)hipFree(c->accum);
+      if (c->rgba) (void)hipFree(c->rgba);
+      c->accum = nullptr;
+      c->rgba = nullptr;
+      HIPCHK(c, hipMalloc(&c->accum, (size_t)WH * 3 * sizeof(double)));
+      HIPCHK(c, hipMalloc(&c->rgba, (size_t)WH * sizeof(uchar4)));
+      c->W = rp->width;
+      c->H = rp->height;
+    }
+    launch_fill_f64(c->accum, (uint64_t)WH * 3, 0.0, c->stream);
+    c->frames = 0;
+  }
+  if (frame_count == 0) return PT_OK;
+  uint32_t fpb = rp->frames_per_batch ? rp->frames_per_batch : std::max<uint32_t>(1, PATHS_TARGET / WH);
+  fpb = std::min(fpb, frame_count);
+  pt_status ps = ensure_paths(c, (uint64_t)fpb * WH);
+  if (ps != PT_OK) return ps;
+
+  RenderConsts rc{};
+  rc.W = rp->width;
+  rc.H = rp->height;
+  rc.WH = WH;
+  rc.max_depth = rp->max_depth;
+  rc.russian = rp->russian;
+  rc.rr_min_depth = rp->rr_min_depth;
+  rc.flags = rp->flags;
+  const pt_camera& cm = c->cam;
+  for (int k = 0; k < 3; k++) {
+    rc.cam.llc[k] = cm.lower_left[k];
+    rc.cam.hor[k] = cm.horizontal[k];
+    rc.cam.ver[k] = cm.vertical[k];
+    rc.cam.org[k] = cm.origin[k];
+    rc.cam.u[k] = cm.u[k];
+    rc.cam.v[k] = cm.v[k];
+  }
+  rc.cam.lens_radius = cm.lens_radius;
+  rc.cam.time0 = cm.time0;
+  rc.cam.time1 = cm.time1;
+
+  const bool want_stats = out != nullptr;
+  const bool count_trav = (rp->flags & PT_FLAG_COUNT_TRAVERSAL) != 0;
+  const bool sort = (rp->flags & PT_FLAG_MATERIAL_SORT) != 0;
+  std::vector<hipEvent_t> ev;
+  auto mark = [&]() -> hipEvent_t {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    (void)hipEventRecord(e, c->stream);
+    ev.push_back(e);
+    return e;
+  };
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ext_ev, sh_ev;
+  launch_fill_u32((uint32_t*)c->stats, STAT_WORDS * 2, 0u, c->stream);
+  hipEvent_t t_begin = want_stats ? mark() : nullptr;
+  DevPaths& P = c->paths;
+  for (uint32_t f0 = 0; f0 < frame_count; f0 += fpb) {
+    uint32_t nf = std::min(fpb, frame_count - f0);
+    uint32_t nslots = nf * WH;
+    rc.frame0 = frame_begin + f0;
+    const int grid = grid_for(nslots);
+    launch_raygen(P, rc, nslots, grid, c->stream);
+    uint32_t* lin = P.list_a;
+    uint32_t* lout = P.list_b;
+    int cin = CTR_ACTIVE_A, cout = CTR_ACTIVE_B;
+    for (uint32_t depth = 0; depth < rp->max_depth; depth++) {
+      hipEvent_t a = want_stats ? mark() : nullptr;
+      launch_extend(c->dsc, P, lin, P.counters + cin, c->spill, c->stats, count_trav, grid, c->stream);
+      hipEvent_t b = want_stats ? mark() : nullptr;
+      if (want_stats) ext_ev.push_back({a, b});
+      const uint32_t* shade_list = lin;
+      if (sort) {
+        launch_sort(c->dsc, P, lin, P.counters + cin, grid, c->stream);
+        shade_list = P.list_sorted;
+      }
+      launch_shade(c->dsc, P, shade_list, P.counters + cin, lout, P.counters + cout, depth, rc, grid, c->stream);
+      hipEvent_t a2 = want_stats ? mark() : nullptr;
+      launch_occlusion(c->dsc, P, c->spill, c->stats, grid, c->stream);
+      hipEvent_t b2 = want_stats ? mark() : nullptr;
+      if (want_stats) sh_ev.push_back({a2, b2});
+      launch_connect(c->dsc, P, lout, P.counters + cout, depth, rc, grid, c->stream);
+      launch_bounce_reset(P, cin, c->stream);
+      std::swap(lin, lout);
+      std::swap(cin, cout);
+    }
+    launch_accumulate(P, rc, nf, c->accum, grid_for(WH), c->stream);
+    c->last_fpb = nf;
+  }
+  hipEvent_t t_end = want_stats ? mark() : nullptr;
+  c->frames = frame_begin + frame_count;
+  c->paths_valid = true;
+  if (want_stats) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    uint64_t st[STAT_WORDS];
+    uint32_t ctr[CTR_WORDS];
+    HIPCHK(c, hipMemcpy(st, c->stats, sizeof(st), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(ctr, P.counters, sizeof(ctr), hipMemcpyDeviceToHost));
+    std::memset(out, 0, sizeof(*out));
+    out->paths = (uint64_t)frame_count * WH;
+    out->extension_rays = st[STAT_EXT];
+    out->shadow_rays = st[STAT_SHADOW];
+    out->node_fetches = st[STAT_NODES];
+    out->prim_fetches = st[STAT_PRIMS];
+    out->instance_fetches = st[STAT_INSTS];
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, t_begin, t_end);
+    out->gpu_ms = ms;
+    for (auto& p : ext_ev) {
+      float m = 0.f;
+      (void)hipEventElapsedTime(&m, p.first, p.second);
+      out->extend_ms += m;
+    }
+    for (auto& p : sh_ev) {
+      float m = 0.f;
+      (void)hipEventElapsedTime(&m, p.first, p.second);
+      out->shadow_ms += m;
+    }
+    out->extend_launches = (uint32_t)ext_ev.size();
+    out->shadow_launches = (uint32_t)sh_ev.size();
+    for (auto e : ev) (void)hipEventDestroy(e);
+    if (ctr[CTR_STACK_OVERFLOW]) return fail(c, PT_ERR_DEVICE, "traversal stack overflow");
+  }
+  return PT_OK;
+}
+
+pt_status pt_sync(pt_context* c) {
+  if (!c) return PT_ERR_INVALID_ARG;
+  (void)hipSetDevice(c->device);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return PT_OK;
+}
+
+pt_status pt_read_accum(pt_context* c, double* rgb, uint32_t* frames) {
+  if (!c) return PT_ERR_INVALID_ARG;
+  if (!c->accum) return fail(c, PT_ERR_STATE, "nothing rendered");
+  (void)hipSetDevice(c->device);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (rgb) HIPCHK(c, hipMemcpy(rgb, c->accum, (size_t)c->W * c->H * 3 * sizeof(double), hipMemcpyDeviceToHost));
+  if (frames) *frames = c->frames;
+  return PT_OK;
+}
+
+pt_status pt_read_display(pt_context* c, uint8_t* rgba) {
+  if (!c || !rgba) return PT_ERR_INVALID_ARG;
+  if (!c->accum || c->frames == 0) return fail(c, PT_ERR_STATE, "nothing rendered");
+  (void)hipSetDevice(c->device);
+  uint32_t n = c->W * c->H;
+  launch_display(c->accum, n, c->frames, c->rgba, grid_for(n), c->stream);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(rgba, c->rgba, (size_t)n * 4, hipMemcpyDeviceToHost));
+  return PT_OK;
+}
+
+pt_status pt_read_paths(pt_context* c, float* rgb, uint32_t* signature, uint32_t* bounces) {
+  if (!c) return PT_ERR_INVALID_ARG;
+  if (!c->paths_valid) return fail(c, PT_ERR_STATE, "no frame rendered");
+  (void)hipSetDevice(c->device);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const uint64_t WH = (uint64_t)c->W * c->H;
+  const uint64_t base = (uint64_t)(c->last_fpb - 1) * WH;
+  std::vector<float4> res(WH);
+  HIPCHK(c, hipMemcpy(res.data(), c->paths.result + base, WH * sizeof(float4), hipMemcpyDeviceToHost));
+  if (rgb)
+    for (uint64_t i = 0; i < WH; i++) {
+      rgb[3 * i] = res[i].x;
+      rgb[3 * i + 1] = res[i].y;
+      rgb[3 * i + 2] = res[i].z;
+    }
+  if (bounces)
+    for (uint64_t i = 0; i < WH; i++) bounces[i] = fb(res[i].w);
+  if (signature) HIPCHK(c, hipMemcpy(signature, c->paths.sig + base, WH * 4, hipMemcpyDeviceToHost));
+  return PT_OK;
+}
+
+static pt_status trace_common(pt_context* c, const pt_ray* rays, uint64_t n, void* out, uint32_t* counts, int mode) {
+  if (!c || (n && (!rays || !out))) return PT_ERR_INVALID_ARG;
+  if (!c->accel_ready) return fail(c, PT_ERR_STATE, "acceleration structure not built");
+  if (n == 0) return PT_OK;
+  (void)hipSetDevice(c->device);
+  std::vector<DevBuf> T;
+  pt_ray* dr = nullptr;
+  void* dout = nullptr;
+  uint32_t* dcnt = nullptr;
+  hipError_t e;
+  size_t out_bytes = mode == 2 ? n : n * sizeof(pt_hit);
+  if ((e = dalloc(T, (void**)&dr, n * sizeof(pt_ray))) || (e = dalloc(T, &dout, out_bytes)) ||
+      (counts && (e = dalloc(T, (void**)&dcnt, n * 12)))) {
+    dfree(T);
+    return hip_fail(c, e, "hipMalloc (rays)");
+  }
+  e = hipMemcpyAsync(dr, rays, n * sizeof(pt_ray), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    launch_fill_u32(c->qcounters, CTR_WORDS, 0u, c->stream);
+    if (mode == 2)
+      launch_trace_occluded(c->dsc, dr, n, (uint8_t*)dout, c->spill, c->qcounters + CTR_STACK_OVERFLOW, grid_for(n),
+                            c->stream);
+    else
+      launch_trace_closest(c->dsc, dr, n, (pt_hit*)dout, dcnt, c->spill, c->qcounters + CTR_STACK_OVERFLOW,
+                           grid_for(n), c->stream);
+    e = hipMemcpyAsync(out, dout, out_bytes, hipMemcpyDeviceToHost, c->stream);
+  }
+  if (e == hipSuccess && counts) e = hipMemcpyAsync(counts, dcnt, n * 12, hipMemcpyDeviceToHost, c->stream);
+  uint32_t ovf = 0;
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(&ovf, c->qcounters + CTR_STACK_OVERFLOW, 4, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  dfree(T);
+  if (e != hipSuccess) return hip_fail(c, e, "trace");
+  if (ovf) return fail(c, PT_ERR_DEVICE, "traversal stack overflow");
+  return PT_OK;
+}
+
+pt_status pt_trace_closest(pt_context* c, const pt_ray* rays, uint64_t n, pt_hit* hits) {
+  return trace_common(c, rays, n, hits, nullptr, 0);
+}
+pt_status pt_trace_closest_counted(pt_context* c, const pt_ray* rays, uint64_t n, pt_hit* hits, uint32_t* counts) {
+  if (!counts) return PT_ERR_INVALID_ARG;
+  return trace_common(c, rays, n, hits, counts, 1);
+}
+pt_status pt_trace_occluded(pt_context* c, const pt_ray* rays, uint64_t n, uint8_t* occ) {
+  return trace_common(c, rays, n, occ, nullptr, 2);
+}
+pt_status pt_trace_closest_device(pt_context* c, const pt_ray* d_rays, uint64_t n, pt_hit* d_hits) {
+  if (!c || (n && (!d_rays || !d_hits))) return PT_ERR_INVALID_ARG;
+  if (!c->accel_ready) return fail(c, PT_ERR_STATE, "acceleration structure not built");
+  if (n == 0) return PT_OK;
+  (void)hipSetDevice(c->device);
+  launch_trace_closest(c->dsc, d_rays, n, d_hits, nullptr, c->spill, c->qcounters + CTR_STACK_OVERFLOW, grid_for(n),
+                       c->stream);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(c, e, "trace launch");
+  return PT_OK;
+}
+
+}  // extern "C"

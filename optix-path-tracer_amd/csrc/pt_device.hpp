@@ -1,0 +1,471 @@
+// pt_device.hpp -- device-side math, RNG, intersection, BRDF, light and miss
+// functions of the gfx950 path tracer.
+//
+// Every function cites the reference program it replaces.  Floating-point
+// expressions keep the reference's evaluation order and the library is
+// built with -ffp-contract=off, so the results are the same IEEE fp32 values
+// a scalar CPU build of the reference logic produces (transcendentals
+// excepted: ocml vs. the host libm).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pt_layout.hpp"
+
+namespace pt {
+
+#define PTD __device__ __forceinline__
+
+constexpr float PI_F = 3.141592654f;  // programs/common.hpp:18
+constexpr float FINF = __builtin_huge_valf();
+constexpr float SLAB_FAR_SCALE = 1.00000024f;  // 1 + 2^-22, conservative slab test
+
+struct v3 {
+  float x, y, z;
+};
+PTD v3 mk3(float x, float y, float z) { return v3{x, y, z}; }
+PTD v3 mk3(float s) { return v3{s, s, s}; }
+PTD v3 mk3(float4 a) { return v3{a.x, a.y, a.z}; }
+PTD v3 operator+(v3 a, v3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+PTD v3 operator-(v3 a, v3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+PTD v3 operator-(v3 a) { return {-a.x, -a.y, -a.z}; }
+PTD v3 operator*(v3 a, v3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+PTD v3 operator*(v3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+PTD v3 operator*(float s, v3 a) { return {s * a.x, s * a.y, s * a.z}; }
+PTD v3 operator/(v3 a, v3 b) { return {a.x / b.x, a.y / b.y, a.z / b.z}; }
+PTD v3 operator/(v3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+PTD float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+PTD v3 cross(v3 a, v3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+PTD float length(v3 a) { return sqrtf(dot(a, a)); }
+PTD v3 normalize(v3 v) {
+  float inv = 1.0f / sqrtf(dot(v, v));
+  return v * inv;
+}
+PTD float squared_length(v3 a) { return length(a) * length(a); }        // vec.hpp:21
+PTD bool isNull(v3 a) { return a.x == 0.f && a.y == 0.f && a.z == 0.f; }  // vec.hpp:26
+PTD v3 unit_vector(v3 v) { return v / length(v); }                        // vec.hpp:47
+PTD float ffmin(float a, float b) { return a < b ? a : b; }               // vec.hpp:57
+PTD float ffmax(float a, float b) { return a > b ? a : b; }               // vec.hpp:62
+PTD v3 min_vec(v3 a, v3 b) { return {ffmin(a.x, b.x), ffmin(a.y, b.y), ffmin(a.z, b.z)}; }
+PTD v3 max_vec(v3 a, v3 b) { return {ffmax(a.x, b.x), ffmax(a.y, b.y), ffmax(a.z, b.z)}; }
+PTD float max_component(v3 a) { return ffmax(ffmax(a.x, a.y), a.z); }
+PTD float min_component(v3 a) { return ffmin(ffmin(a.x, a.y), a.z); }
+PTD float clampf(float x, float lo, float hi) { return fmaxf(lo, fminf(x, hi)); }
+PTD v3 clamp3(v3 v, float lo, float hi) { return {clampf(v.x, lo, hi), clampf(v.y, lo, hi), clampf(v.z, lo, hi)}; }
+PTD v3 reflect(v3 i, v3 n) { return i - n * (2.0f * dot(n, i)); }
+PTD float comp(v3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+// ------------------------------------------------------------------ RNG
+// programs/random.cuh:33-47
+PTD uint32_t tea64(uint32_t val0, uint32_t val1) {
+  uint32_t v0 = val0, v1 = val1, s0 = 0;
+#pragma unroll 16
+  for (int n = 0; n < 64; n++) {
+    s0 += 0x9e3779b9u;
+    v0 += ((v1 << 4) + 0xa341316cu) ^ (v1 + s0) ^ ((v1 >> 5) + 0xc8013ea4u);
+    v1 += ((v0 << 4) + 0xad90777du) ^ (v0 + s0) ^ ((v0 >> 5) + 0x7e95761eu);
+  }
+  return v0;
+}
+// programs/random.cuh:50-65: 24-bit LCG output / 2^24 (exact in fp32)
+PTD float rnd(uint32_t& prev) {
+  prev = 1664525u * prev + 1013904223u;
+  return (float)(prev & 0x00FFFFFFu) * (1.0f / 16777216.0f);
+}
+
+// programs/sampling.cuh:22-29
+PTD v3 random_in_unit_disk(uint32_t& seed) {
+  float a = rnd(seed) * 2.f * PI_F;
+  v3 xy = mk3(sinf(a), cosf(a), 0.f);
+  return xy * sqrtf(rnd(seed));
+}
+// programs/sampling.cuh:31-44
+PTD v3 random_in_unit_sphere(uint32_t& seed) {
+  float z = rnd(seed) * 2.f - 1.f;
+  float t = rnd(seed) * 2.f * PI_F;
+  float r = sqrtf((0.f > (1.f - z * z) ? 0.f : (1.f - z * z)));
+  v3 res = mk3(r * cosf(t), r * sinf(t), z);
+  return res * powf(rnd(seed), 1.f / 3.f);
+}
+// programs/sampling.cuh:46-59
+PTD v3 random_on_unit_sphere(uint32_t& seed) {
+  float z = rnd(seed) * 2.f - 1.f;
+  float t = rnd(seed) * 2.f * PI_F;
+  float r = sqrtf((0.f > (1.f - z * z) ? 0.f : (1.f - z * z)));
+  v3 res = mk3(r * cosf(t), r * sinf(t), z);
+  res = res * powf(rnd(seed), 1.f / 3.f);
+  return unit_vector(res);
+}
+// Malley's method: uniform disk point (r = sqrt(u1), phi = 2 pi u2) lifted to
+// the +z hemisphere (replaces the OptiX SDK cosine_sample_hemisphere).
+PTD v3 cosine_sample_hemisphere(float u1, float u2) {
+  float r = sqrtf(u1);
+  float phi = 2.0f * PI_F * u2;
+  float x = r * cosf(phi);
+  float y = r * sinf(phi);
+  float z = sqrtf(fmaxf(0.0f, 1.0f - x * x - y * y));
+  return mk3(x, y, z);
+}
+// Orthonormal basis about a unit normal (Duff et al., JCGT 6(1), 2017);
+// replaces the OptiX SDK Onb.  Local (x, y, z) -> x*t + y*b + z*n.
+PTD v3 onb_to_world(v3 n, v3 p) {
+  float sign = copysignf(1.0f, n.z);
+  float a = -1.0f / (sign + n.z);
+  float bb = n.x * n.y * a;
+  v3 t = mk3(1.0f + sign * n.x * n.x * a, sign * bb, -sign * n.x);
+  v3 b = mk3(bb, sign + n.y * n.y * a, -n.y);
+  return t * p.x + b * p.y + n * p.z;
+}
+
+// programs/math/trigonometric.cuh:5-58 (angles against world +Y)
+PTD float Cos2Theta(v3 w) { return w.y * w.y; }
+PTD float AbsCosTheta(v3 w) { return fabsf(w.y); }
+PTD float SinTheta(v3 w) { return sqrtf(fmaxf(0.f, 1.f - Cos2Theta(w))); }
+PTD float TanTheta(v3 w) { return SinTheta(w) / w.y; }
+PTD float CosPhi(v3 w) {
+  float s = SinTheta(w);
+  return (s == 0) ? 1.f : clampf(w.x / s, -1.f, 1.f);
+}
+PTD float SinPhi(v3 w) {
+  float s = SinTheta(w);
+  return (s == 0) ? 0.f : clampf(w.z / s, -1.f, 1.f);
+}
+PTD float Cos2Phi(v3 w) { float c = CosPhi(w); return c * c; }
+PTD float Sin2Phi(v3 w) { float s = SinPhi(w); return s * s; }
+
+// ------------------------------------------------------------ transforms
+// rows r0..r2 of a 3x4 matrix
+PTD v3 xf_point(float4 r0, float4 r1, float4 r2, v3 p) {
+  return mk3(r0.x * p.x + r0.y * p.y + r0.z * p.z + r0.w, r1.x * p.x + r1.y * p.y + r1.z * p.z + r1.w,
+             r2.x * p.x + r2.y * p.y + r2.z * p.z + r2.w);
+}
+PTD v3 xf_vector(float4 r0, float4 r1, float4 r2, v3 v) {
+  return mk3(r0.x * v.x + r0.y * v.y + r0.z * v.z, r1.x * v.x + r1.y * v.y + r1.z * v.z,
+             r2.x * v.x + r2.y * v.y + r2.z * v.z);
+}
+// normal by the transpose of world-to-object
+PTD v3 xf_normal(float4 w0, float4 w1, float4 w2, v3 n) {
+  return mk3(w0.x * n.x + w1.x * n.y + w2.x * n.z, w0.y * n.x + w1.y * n.y + w2.y * n.z,
+             w0.z * n.x + w1.z * n.y + w2.z * n.z);
+}
+
+// ------------------------------------------------------------ intersectors
+// sphere.cu:32-62: the nearer root above tmin is the only one that can win
+PTD float isect_sphere(float4 s, v3 o, v3 d, float tmin) {
+  v3 oc = o - mk3(s);
+  float a = dot(d, d);
+  float b = dot(oc, d);
+  float c = dot(oc, oc) - s.w * s.w;
+  float disc = b * b - a * c;
+  if (disc < 0.f) return FINF;
+  float sq = sqrtf(disc);
+  float t1 = (-b - sq) / a;
+  if (t1 > tmin) return t1;
+  return (-b + sq) / a;
+}
+// aarect.cu:20-69 (inclusive bounds); A = (a0, a1, b0, b1), k, axis
+PTD float isect_rect(float4 A, float k, int axis, v3 o, v3 d) {
+  float t, a, b;
+  if (axis == 0) { t = (k - o.x) / d.x; a = o.y + t * d.y; b = o.z + t * d.z; }
+  else if (axis == 1) { t = (k - o.y) / d.y; a = o.x + t * d.x; b = o.z + t * d.z; }
+  else { t = (k - o.z) / d.z; a = o.x + t * d.x; b = o.y + t * d.y; }
+  if (a < A.x || a > A.y || b < A.z || b > A.w) return FINF;
+  return t;
+}
+// box.cu:36-53: tmin offered first, tmax when tmin is not above the ray tmin
+PTD float isect_box(v3 lo, v3 hi, v3 o, v3 d, float tmin) {
+  v3 t0 = (lo - o) / d;
+  v3 t1 = (hi - o) / d;
+  float tn = max_component(min_vec(t0, t1));
+  float tf = min_component(max_vec(t0, t1));
+  if (!(tn <= tf)) return FINF;
+  return tn > tmin ? tn : tf;
+}
+
+// Watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2(1), 2013).
+struct TriRay {
+  int kx, ky, kz;
+  float Sx, Sy, Sz;
+};
+PTD TriRay tri_setup(v3 d) {
+  TriRay r;
+  float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+  r.kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+  r.kx = r.kz + 1 == 3 ? 0 : r.kz + 1;
+  r.ky = r.kx + 1 == 3 ? 0 : r.kx + 1;
+  if (comp(d, r.kz) < 0.0f) { int t = r.kx; r.kx = r.ky; r.ky = t; }
+  r.Sx = comp(d, r.kx) / comp(d, r.kz);
+  r.Sy = comp(d, r.ky) / comp(d, r.kz);
+  r.Sz = 1.0f / comp(d, r.kz);
+  return r;
+}
+PTD bool tri_test(const TriRay& r, v3 o, v3 v0, v3 v1, v3 v2, float& t, float& b1, float& b2) {
+  v3 A = v0 - o, B = v1 - o, C = v2 - o;
+  float Ax = comp(A, r.kx) - r.Sx * comp(A, r.kz);
+  float Ay = comp(A, r.ky) - r.Sy * comp(A, r.kz);
+  float Bx = comp(B, r.kx) - r.Sx * comp(B, r.kz);
+  float By = comp(B, r.ky) - r.Sy * comp(B, r.kz);
+  float Cx = comp(C, r.kx) - r.Sx * comp(C, r.kz);
+  float Cy = comp(C, r.ky) - r.Sy * comp(C, r.kz);
+  float U = Cx * By - Cy * Bx;
+  float V = Ax * Cy - Ay * Cx;
+  float W = Bx * Ay - By * Ax;
+  if (U == 0.0f || V == 0.0f || W == 0.0f) {
+    double CxBy = (double)Cx * (double)By, CyBx = (double)Cy * (double)Bx;
+    U = (float)(CxBy - CyBx);
+    double AxCy = (double)Ax * (double)Cy, AyCx = (double)Ay * (double)Cx;
+    V = (float)(AxCy - AyCx);
+    double BxAy = (double)Bx * (double)Ay, ByAx = (double)By * (double)Ax;
+    W = (float)(BxAy - ByAx);
+  }
+  if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+  float det = U + V + W;
+  if (det == 0.0f) return false;
+  float Az = r.Sz * comp(A, r.kz);
+  float Bz = r.Sz * comp(B, r.kz);
+  float Cz = r.Sz * comp(C, r.kz);
+  float T = U * Az + V * Bz + W * Cz;
+  float rcp = 1.0f / det;
+  t = T * rcp;
+  b1 = V * rcp;
+  b2 = W * rcp;
+  return true;
+}
+
+// ------------------------------------------------------------ materials
+// material.cuh:30-39
+PTD float schlick(float cosine, float ref_idx) {
+  float r0 = (1.f - ref_idx) / (1.f + ref_idx);
+  r0 = r0 * r0;
+  return r0 + (1.f - r0) * powf((1.f - cosine), 5.f);
+}
+PTD v3 schlick3(v3 r0, float cosine) {
+  float e = powf(1.f - cosine, 5.f);
+  return r0 + (mk3(1.f) - r0) * e;
+}
+// material.cuh:50-63
+PTD bool Refract(v3 v, v3 n, float ni_over_nt, v3& refracted) {
+  v3 uv = normalize(v);
+  float dt = dot(uv, n);
+  float disc = 1.f - ni_over_nt * ni_over_nt * (1.f - dt * dt);
+  if (disc > 0.f) {
+    refracted = ni_over_nt * (uv - n * dt) - n * sqrtf(disc);
+    return true;
+  }
+  return false;
+}
+// microfacets.cuh:11-20
+PTD void Sample_Quadrant(float nu, float nv, float rx, float ry, float& phi, float& theta) {
+  phi = atanf(sqrtf((nu + 1.f) / (nv + 1.f)) * tanf(2.f * PI_F * rx));
+  float cp = cosf(phi), sp = sinf(phi);
+  theta = powf(ry, 1.f / (nu * cp * cp + nv * sp * sp + 1.f));
+}
+// microfacets.cuh:85-99
+PTD float GGX_D(v3 H, float nu, float nv) {
+  float c2 = Cos2Theta(H);
+  if (c2 <= 0.0f) return 0.f;
+  if (nu == nv) {
+    float beta = c2 * ((nu * nu) - 1) + 1;
+    return (nu * nu) / (PI_F * beta * beta);
+  }
+  float s2nv = Sin2Phi(H) / (nv * nv);
+  float c2nu = (1.f - Sin2Phi(H)) / (nu * nu);
+  float beta = (c2 + (1.f - c2) * (c2nu + s2nv));
+  return 1.f / (PI_F * nu * nv * beta * beta);
+}
+// microfacets.cuh:102-168 (visible-slope GGX sample)
+PTD v3 GGX_Sample(v3 origin, float rx, float ry, float nu, float nv) {
+  bool flip = origin.y < 0;
+  v3 so = normalize(mk3(origin.x * nu, origin.y, origin.z * nv));
+  float slopeX, slopeY;
+  float cosTheta = so.y;
+  if (cosTheta > 0.9999f) {
+    float r = sqrtf(rx / (1 - rx));
+    float phi = (float)(6.28318530718 * (double)ry);
+    slopeX = r * cosf(phi);
+    slopeY = r * sinf(phi);
+  } else {
+    float sinTheta = sqrtf(fmaxf(0.f, 1.f - cosTheta * cosTheta));
+    float tanTheta = sinTheta / cosTheta;
+    float a = 1.f / tanTheta;
+    float G1 = 2.f / (1.f + sqrtf(1.f + 1.f / (a * a)));
+    float A = 2.f * rx / G1 - 1.f;
+    float tmp = 1.f / (A * A - 1.f);
+    if (tmp > 1e10f) tmp = 1e10f;
+    float B = tanTheta;
+    float D = sqrtf(fmaxf(B * B * tmp * tmp - (A * A - B * B) * tmp, 0.f));
+    float sx1 = B * tmp - D;
+    float sx2 = B * tmp + D;
+    slopeX = (A < 0 || sx2 > 1.f / tanTheta) ? sx1 : sx2;
+    float S;
+    if (ry > 0.5f) { S = 1.f; ry = 2.f * (ry - .5f); }
+    else { S = -1.f; ry = 2.f * (.5f - ry); }
+    float z = (ry * (ry * (ry * 0.27385f - 0.73369f) + 0.46341f)) /
+              (ry * (ry * (ry * 0.093073f + 0.309420f) - 1.f) + 0.597999f);
+    slopeY = S * z * sqrtf(1.f + slopeX * slopeX);
+  }
+  float cp = CosPhi(so), sp = SinPhi(so);
+  float t = cp * slopeX - sp * slopeY;
+  slopeY = sp * slopeX + cp * slopeY;
+  slopeX = t;
+  slopeX = nu * slopeX;
+  slopeY = nv * slopeY;
+  v3 H = normalize(mk3(-slopeX, 1.f, -slopeY));
+  if (flip) H = H * -1.f;
+  return H;
+}
+// microfacets.cuh:170-195
+PTD float GGX_Lambda(v3 V, float nu, float nv) {
+  float at = fabsf(TanTheta(V));
+  if (__builtin_isinf(at)) return 0.f;
+  float alpha = sqrtf(Cos2Phi(V) * nu * nu + Sin2Phi(V) * nv * nv);
+  float a2t2 = (alpha * at) * (alpha * at);
+  return (-1.f + sqrtf(1.f + a2t2)) / 2.f;
+}
+PTD float GGX_G(v3 Wo, v3 Wi, float nu, float nv) {
+  return 1.f / (1.f + GGX_Lambda(Wo, nu, nv) + GGX_Lambda(Wi, nu, nv));
+}
+
+struct Surface {
+  int type;
+  v3 c0, c1;
+  float p0, p1;
+};
+
+// Sample(): lambertian.cuh:7-19, oren_nayar.cuh:8-20, diffuse_light.cuh:7-13,
+// ashikhmin_shirley.cuh:11-82, torrance_sparrow.cuh:10-35
+PTD v3 brdf_sample(const Surface& f, v3 Wo, v3 N, uint32_t& seed) {
+  if (f.type == PT_MAT_LAMBERTIAN || f.type == PT_MAT_OREN_NAYAR) {
+    float u1 = rnd(seed);
+    float u2 = rnd(seed);
+    return onb_to_world(N, cosine_sample_hemisphere(u1, u2));
+  }
+  if (f.type == PT_MAT_DIFFUSE_LIGHT) return random_on_unit_sphere(seed);
+  float nu = f.p0, nv = f.p1;
+  v3 Nn = normalize(N);
+  v3 T = normalize(cross(Nn, mk3(0.f, 1.f, 0.f)));
+  v3 B = cross(T, Nn);
+  float rx = rnd(seed);
+  float ry = rnd(seed);
+  v3 H;
+  if (f.type == PT_MAT_ASHIKHMIN_SHIRLEY) {
+    if ((double)rx < 0.5) {
+      rx = fminf(2 * rx, 1.f - 1e-6f);
+      return onb_to_world(Nn, cosine_sample_hemisphere(rx, ry));
+    }
+    rx = fminf(2 * (rx - 0.5f), 1.f - 1e-6f);
+    float phi, cos_theta;
+    if (nu == nv) {
+      phi = 2.f * PI_F * rx;
+      cos_theta = powf(ry, 1.f / (nu + 1.f));
+    } else {
+      if (rx < 0.25f) { Sample_Quadrant(nu, nv, 4.f * rx, ry, phi, cos_theta); }
+      else if (rx < 0.5f) { Sample_Quadrant(nu, nv, 4.f * (0.5f - rx), ry, phi, cos_theta); phi = PI_F - phi; }
+      else if (rx < 0.75f) { Sample_Quadrant(nu, nv, 4.f * (rx - 0.5f), ry, phi, cos_theta); phi = PI_F + phi; }
+      else { Sample_Quadrant(nu, nv, 4.f * (1.f - rx), ry, phi, cos_theta); phi = 2.f * PI_F - phi; }
+    }
+    float sin_theta = sqrtf(fmaxf(0.f, 1.f - cos_theta * cos_theta));
+    H = normalize(mk3(sin_theta * cosf(phi), cos_theta, sin_theta * sinf(phi)));
+  } else {
+    H = normalize(GGX_Sample(Wo, rx, ry, nu, nv));
+  }
+  H = H.x * B + H.y * Nn + H.z * T;
+  if (dot(H, Wo) < 0.f) H = -H;
+  return normalize(-Wo + 2.f * dot(Wo, H) * H);
+}
+
+// Evaluate(): lambertian.cuh:21-42, oren_nayar.cuh:22-65,
+// diffuse_light.cuh:15-22, ashikhmin_shirley.cuh:84-151,
+// torrance_sparrow.cuh:37-64.  pdf keeps its incoming value on early returns.
+PTD v3 brdf_eval(const Surface& f, v3 Wo, v3 Wi, v3 N, float& pdf) {
+  switch (f.type) {
+    case PT_MAT_LAMBERTIAN: {
+      float cosine = dot(normalize(Wi), normalize(N));
+      if (cosine < 0.f) return mk3(0.f);
+      pdf = dot(normalize(Wi), normalize(N)) / PI_F;
+      return (cosine * f.c0) / PI_F;
+    }
+    case PT_MAT_OREN_NAYAR: {
+      v3 WiN = normalize(Wi);
+      float sI = SinTheta(WiN), sO = SinTheta(Wo);
+      float maxCos = 0;
+      if ((double)sI > 1e-4 && (double)sO > 1e-4) {
+        float sPI = SinPhi(WiN), cPI = CosPhi(WiN);
+        float sPO = SinPhi(Wo), cPO = CosPhi(Wo);
+        float dCos = cPI * cPO + sPI * sPO;
+        maxCos = fmaxf(0.f, dCos);
+      }
+      float sinAlpha, tanBeta;
+      if (AbsCosTheta(WiN) > AbsCosTheta(Wo)) { sinAlpha = sO; tanBeta = sI / AbsCosTheta(WiN); }
+      else { sinAlpha = sI; tanBeta = sO / AbsCosTheta(Wo); }
+      pdf = dot(normalize(Wi), normalize(N)) / PI_F;
+      return f.c0 * (1.f / PI_F) * (f.p0 + f.p1 * maxCos * sinAlpha * tanBeta);
+    }
+    case PT_MAT_DIFFUSE_LIGHT:
+      pdf = 1.f;
+      return f.c0;
+    case PT_MAT_ASHIKHMIN_SHIRLEY: {
+      v3 Rd = f.c0, Rs = f.c1;
+      float nu = f.p0, nv = f.p1;
+      v3 Up = mk3(0.f, 1.f, 0.f);
+      v3 Nn = normalize(N);
+      v3 T = normalize(cross(Nn, Up));
+      v3 B = cross(T, Nn);
+      float NdotI = fabsf(dot(Up, Wi)), NdotO = fabsf(dot(Up, Wo));
+      v3 diff = Rd * (28.f / (23.f * PI_F));
+      diff = diff * (mk3(1.f) - Rs);
+      diff = diff * (1.f - powf(1.f - NdotI * 0.5f, 5.f));
+      diff = diff * (1.f - powf(1.f - NdotO * 0.5f, 5.f));
+      float dpdf = dot(Wi, Nn) / PI_F;
+      if (dpdf < 0.f) return mk3(0.f);
+      v3 H = Wo + Wi;
+      if (isNull(H)) return mk3(0.f);
+      H = normalize(H);
+      float HdotI = fabsf(dot(H, Wi));
+      float HdotN = fabsf(dot(H, Nn)), HdotT = fabsf(dot(H, T)), HdotB = fabsf(dot(H, B));
+      float norm, lobe;
+      if (nu == nv) {
+        norm = (nu + 1.f) / (8.f * PI_F);
+        lobe = powf(HdotN, nu);
+      } else {
+        norm = sqrtf((nu + 1.f) * (nv + 1.f)) / (8.f * PI_F);
+        if (HdotN < 1.f)
+          lobe = powf(HdotN, (nu * HdotT * HdotT + nv * HdotB * HdotB) / (1.f - HdotN * HdotN));
+        else
+          lobe = 1.f;
+      }
+      v3 spec = schlick3(Rs, HdotI);
+      spec = spec * norm;
+      spec = spec * (lobe / (HdotI * fmaxf(NdotI, NdotO)));
+      float spdf = HdotI / (norm * lobe);
+      pdf = (dpdf + spdf) / 2.f;
+      return (diff / dpdf) + (spec / spdf);
+    }
+    default: {  // Torrance-Sparrow
+      v3 Rs = f.c0;
+      float nu = f.p0, nv = f.p1;
+      v3 Up = mk3(0.f, 1.f, 0.f);
+      float NdotI = fmaxf(dot(Up, Wi), 1e-6f), NdotO = fmaxf(dot(Up, Wo), 1e-6f);
+      v3 H = Wo + Wi;
+      if (isNull(H)) return mk3(0.f);
+      H = normalize(H);
+      float HdotI = fabsf(dot(H, Wi));
+      v3 F = schlick3(Rs, HdotI);
+      float G = GGX_G(Wo, Wi, nu, nv);
+      float D = GGX_D(H, nu, nv);
+      pdf = (D * AbsCosTheta(H)) / (4.f * dot(Wo, H));
+      return Rs * D * G * F / (4.f * NdotI * NdotO);
+    }
+  }
+}
+
+// light_sample.cuh:27-33 with numf = numg = 1
+PTD float power_heuristic(float fPdf, float gPdf) {
+  float f = 1u * fPdf, g = 1u * gPdf;
+  return (f * f) / (f * f + g * g);
+}
+
+PTD uint32_t sig_mix(uint32_t h, uint32_t v) { return (h ^ v) * 16777619u; }
+
+}  // namespace pt

@@ -1,0 +1,89 @@
+// pt_kernels.hpp -- host-side launchers of the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+
+#include "pt_layout.hpp"
+
+namespace pt {
+
+struct RenderConsts {
+  uint32_t W, H, WH;
+  uint32_t max_depth;
+  int32_t russian;
+  uint32_t rr_min_depth;
+  uint32_t frame0;
+  uint32_t flags;
+  DevCamera cam;
+};
+
+// render path (pt_render.hip)
+void launch_trace_closest(const DevScene& sc, const pt_ray* rays, uint64_t n, pt_hit* hits, uint32_t* counts,
+                          uint2* spill, uint32_t* overflow, int grid, hipStream_t st);
+void launch_trace_occluded(const DevScene& sc, const pt_ray* rays, uint64_t n, uint8_t* occ, uint2* spill,
+                           uint32_t* overflow, int grid, hipStream_t st);
+void launch_raygen(const DevPaths& P, const RenderConsts& rc, uint32_t nslots, int grid, hipStream_t st);
+void launch_extend(const DevScene& sc, const DevPaths& P, const uint32_t* list, const uint32_t* count, uint2* spill,
+                   uint64_t* stats, bool count_traversal, int grid, hipStream_t st);
+void launch_sort(const DevScene& sc, const DevPaths& P, const uint32_t* list, const uint32_t* count, int grid,
+                 hipStream_t st);
+void launch_shade(const DevScene& sc, const DevPaths& P, const uint32_t* list, const uint32_t* count,
+                  uint32_t* next_list, uint32_t* next_count, uint32_t depth, const RenderConsts& rc, int grid,
+                  hipStream_t st);
+void launch_occlusion(const DevScene& sc, const DevPaths& P, uint2* spill, uint64_t* stats, int grid,
+                      hipStream_t st);
+void launch_connect(const DevScene& sc, const DevPaths& P, uint32_t* next_list, uint32_t* next_count, uint32_t depth,
+                    const RenderConsts& rc, int grid, hipStream_t st);
+void launch_bounce_reset(const DevPaths& P, int zero_ctr, hipStream_t st);
+void launch_accumulate(const DevPaths& P, const RenderConsts& rc, uint32_t nframes, double* accum, int grid,
+                       hipStream_t st);
+void launch_display(const double* accum, uint32_t npix, uint32_t frames, uchar4* rgba, int grid, hipStream_t st);
+void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st);
+void launch_fill_f64(double* p, uint64_t n, double v, hipStream_t st);
+
+// acceleration-structure build (pt_bvh.hip)
+struct BuildScratch {
+  // sized for the largest structure of the scene
+  float4* lo;         // primitive bounds
+  float4* hi;
+  uint32_t* code_a;   // Morton codes (ping-pong)
+  uint32_t* code_b;
+  uint32_t* idx_a;    // primitive order (ping-pong)
+  uint32_t* idx_b;
+  uint32_t* hist;     // radix-sort digit counts
+  uint32_t* left;     // Karras binary hierarchy
+  uint32_t* right;
+  uint32_t* first;
+  uint32_t* last;
+  uint32_t* parent;   // parent of internal nodes [0, n-1) and leaves [n-1, 2n-1)
+  uint32_t* flags;    // refit arrival counters
+  float4* nlo;        // binary node bounds
+  float4* nhi;
+  uint32_t* frontier_a;  // collapse frontier (binary node per wide node)
+  uint32_t* frontier_b;
+  uint32_t* slots;    // 4 binary refs per frontier node
+  uint32_t* nchild;   // internal children per frontier node (scanned)
+  uint32_t* scan_tmp; // block sums of the scans
+  uint32_t* cbounds;  // centroid bounds (6 ordered uints)
+  uint32_t* misc;     // device scalars
+  uint64_t capacity;
+};
+
+// Build one BVH4 over `n` primitive boxes already in scratch.lo/hi.
+// Writes nodes to `nodes_out` (relative child indices), the sorted order to
+// `order_out` and the sorted Morton codes to `morton_out`; returns the node
+// count (host-synchronous), or 0 on error.
+uint32_t build_bvh4(BuildScratch& sb, uint32_t n, int leaf_max, uint4* nodes_out, uint32_t* order_out,
+                    uint32_t* morton_out, float4* root_lo_hi, hipStream_t st);
+// Primitive bounds kernels (fill scratch.lo/hi)
+void launch_bounds_spheres(const float4* sph, uint32_t n, float4* lo, float4* hi, hipStream_t st);
+void launch_bounds_rects(const float4* rect, uint32_t n, float4* lo, float4* hi, hipStream_t st);
+void launch_bounds_boxes(const float4* box, uint32_t n, float4* lo, float4* hi, hipStream_t st);
+void launch_bounds_tris(const float4* vtx, const uint4* idx, uint32_t n, float4* lo, float4* hi, hipStream_t st);
+void launch_bounds_instances(const uint4* insts_unsorted, const float4* blas_root, uint32_t n, float4* lo,
+                             float4* hi, hipStream_t st);
+
+}  // namespace pt

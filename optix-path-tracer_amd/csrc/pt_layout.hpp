@@ -1,0 +1,154 @@
+// pt_layout.hpp -- HBM layout of the scene, acceleration structure and path
+// queues shared by the host orchestration and the gfx950 kernels.
+#pragma once
+
+#include <stdint.h>
+
+#include "../../include/pt_api.h"
+
+namespace pt {
+
+// ---------------------------------------------------------------- BVH4
+// One node = 128 bytes = 32 words, one cache line:
+//   words  0.. 3  lo.x[4]     words  4.. 7  hi.x[4]
+//   words  8..11  lo.y[4]     words 12..15  hi.y[4]
+//   words 16..19  lo.z[4]     words 20..23  hi.z[4]
+//   words 24..27  child[4]: internal -> node index within its structure,
+//                 leaf -> first sorted primitive, empty -> 0xFFFFFFFF
+//   words 28..31  count[4]: 0 internal / empty, else leaf primitive count
+constexpr int NODE_WORDS = 32;
+constexpr int BLAS_LEAF_MAX = 4;
+constexpr int TLAS_LEAF_MAX = 1;
+// Traversal stack references: internal node index, or
+// LEAF_BIT | (count - 1) << 28 | first.
+constexpr uint32_t LEAF_BIT = 1u << 31;
+constexpr uint32_t EMPTY_CHILD = 0xFFFFFFFFu;
+
+// ------------------------------------------------------------ records
+// Sorted (BVH-order) primitive records used by traversal:
+//   sphere   float4 (c.xyz, r)                           16 B  + uint32 index
+//   rect     float4 (a0, a1, b0, b1), float4 (k, axis, flip, index)  32 B
+//   box      float4 (lo.xyz, index),  float4 (hi.xyz, 0)            32 B
+//   triangle float4 (v0.xyz, index), float4 (v1.xyz, 0), float4 (v2, 0) 48 B
+// Instance record, 8 x 16 B (sorted in top-level order):
+//   [0..2] world-to-object rows, [3] (root node, prim base, kind | xf << 8,
+//   instance id), [4..6] object-to-world rows, [7] (first, count, mesh,
+//   bottom-level structure)
+constexpr int INST_WORDS = 32;
+
+// Block size of the path kernels and the traversal stack: STK_LDS entries per
+// lane in LDS, STK_SPILL more per thread in HBM.
+constexpr int BLOCK = 256;
+constexpr int STK_LDS = 16;
+constexpr int STK_SPILL = 48;
+
+struct DevMaterial {
+  int32_t type;
+  int32_t tex0, tex1;
+  float p0, p1, p2, p3;
+  int32_t pad;
+};
+
+struct DevTexture {
+  int32_t type;
+  float r, g, b;
+  uint32_t w, h;
+  uint32_t offset;  // first float4 texel in the texel pool
+  uint32_t pad;
+};
+
+struct DevLight {
+  int32_t type, axis;
+  float a0, a1, b0, b1, k;
+  float cx, cy, cz, r;
+  float ex, ey, ez;
+  float pad[2];
+};
+
+struct DevMesh {
+  uint32_t vbase;   // first vertex in the vertex pool
+  uint32_t ibase;   // first index triple (triangle) in the index pool
+  uint32_t ntri;
+  int32_t has_n, has_uv;
+  int32_t mat;      // mesh material when fmat_base < 0
+  int32_t fmat_base;  // first per-face material, or -1
+  uint32_t pad;
+};
+
+// Everything a kernel needs to see of the scene, passed by value.
+struct DevScene {
+  const uint4* nodes;  // all BVH4 nodes (every BLAS, then the TLAS)
+  uint32_t tlas_root;  // absolute node index of the TLAS root
+  uint32_t num_instances;
+  const uint4* insts;  // instance records in top-level order
+  // sorted primitive records (per-kind pools, BLAS after BLAS)
+  const float4* sph;
+  const uint32_t* sph_idx;
+  const float4* rect;
+  const float4* box;
+  const float4* tri;
+  // original-order data for hit records and materials
+  const float4* sph_o;   // (c, r)
+  const float4* rect_o;  // 2 per rect, as the sorted record
+  const float4* box_o;   // 2 per box
+  const int32_t* sph_mat;
+  const int32_t* rect_mat;
+  const int32_t* box_mat;
+  const DevMesh* meshes;
+  const float4* vtx;     // vertex pool (xyz)
+  const float4* nrm;     // normal pool (xyz)
+  const float2* uv;      // uv pool
+  const uint4* idx;      // index pool (x, y, z: mesh-local vertex ids)
+  const int32_t* fmat;   // per-face materials
+  const DevMaterial* mats;
+  const DevTexture* texs;
+  const float4* texels;
+  const DevLight* lights;
+  int32_t num_lights;
+  pt_miss_desc miss;
+};
+
+struct DevCamera {
+  float llc[3], hor[3], ver[3], org[3], u[3], v[3];
+  float lens_radius, time0, time1;
+};
+
+// Path queues in HBM (structure of arrays, one slot per camera path of a
+// batch; queue lists hold slot indices).
+struct DevPaths {
+  float4* ray_o;   // (o.xyz, tmin)
+  float4* ray_d;   // (d.xyz, 0)
+  float4* thr;     // (T.xyz, flags: bit0 previous bounce specular)
+  float4* rad;     // (L.xyz, 0)
+  uint32_t* seed;
+  uint32_t* sig;
+  float4* hit;     // (t, inst bits, prim bits, 0)
+  float4* nee_p;   // (P.xyz, light index bits)
+  float4* nee_n;   // (N.xyz, material bits)
+  float4* nee_wo;  // (Wo.xyz, light pdf)
+  float4* nee_wi;  // (Wi.xyz, flags: bit0 emitter)
+  float4* nee_c;   // (surface c0.xyz, 0)
+  uint8_t* occl;
+  float4* result;  // (L.xyz, bounces bits)
+  uint32_t* list_a;
+  uint32_t* list_b;
+  uint32_t* list_shadow;
+  uint32_t* list_sorted;
+  uint32_t* keys;
+  uint32_t* counters;  // see CTR_*
+};
+
+// device counters (uint32 unless noted)
+constexpr int CTR_ACTIVE_A = 0;
+constexpr int CTR_ACTIVE_B = 1;
+constexpr int CTR_SHADOW = 2;
+constexpr int CTR_BUCKETS = 4;        // 8 material buckets
+constexpr int CTR_BUCKET_CURSOR = 12; // 8 scatter cursors
+constexpr int CTR_STACK_OVERFLOW = 20;
+constexpr int CTR_WORDS = 32;
+// 64-bit statistics: [0] extension rays, [1] shadow rays, [2] nodes,
+// [3] prims, [4] instances
+constexpr int STAT_EXT = 0, STAT_SHADOW = 1, STAT_NODES = 2, STAT_PRIMS = 3, STAT_INSTS = 4;
+constexpr int STAT_WORDS = 8;
+
+}  // namespace pt

@@ -1,0 +1,235 @@
+// pt_shade.hpp -- hit records, textures, miss programs and light sampling of
+// the gfx950 path tracer.
+#pragma once
+
+#include "pt_trace.hpp"
+
+namespace pt {
+
+struct HitRec {
+  v3 P, N, Wo;
+  float u, v;
+  int mat;
+};
+
+// Get_HitRecord of sphere.cu:65-93, aarect.cu:71-115, box.cu:24-33 + 56-81,
+// triangle.cu:85-149, rebuilt in object space from the object-space ray and
+// then transformed to world space.  u, v are only produced when `need_uv`.
+PTD HitRec hit_record(const DevScene& sc, int inst_pos, int prim, float t, v3 o, v3 d, bool need_uv_hint) {
+  const uint4* ir = sc.insts + (size_t)inst_pos * 8;
+  float4 w0 = as_f4(ir[0]), w1 = as_f4(ir[1]), w2 = as_f4(ir[2]);
+  uint4 info = ir[3];
+  uint32_t kind = info.z & 0xFFu;
+  bool xf = (info.z >> 8) != 0;
+  v3 ro = o, rd = d;
+  if (xf) {
+    ro = xf_point(w0, w1, w2, o);
+    rd = xf_vector(w0, w1, w2, d);
+  }
+  HitRec r;
+  r.Wo = normalize(-d);
+  r.u = 0.f;
+  r.v = 0.f;
+  v3 hp, n;
+  if (kind == PT_GEOM_SPHERES) {
+    float4 s = sc.sph_o[prim];
+    r.mat = sc.sph_mat[prim];
+    hp = ro + t * rd;
+    n = (hp - mk3(s)) / s.w;
+    if (need_uv_hint) {
+      float phi = atan2f(n.z, n.x);
+      float theta = asinf(n.y);
+      r.u = 1.f - (phi + PI_F) / (2.f * PI_F);
+      r.v = (theta + PI_F / 2.f) / PI_F;
+    }
+  } else if (kind == PT_GEOM_RECTS) {
+    float4 A = sc.rect_o[2 * prim], B = sc.rect_o[2 * prim + 1];
+    r.mat = sc.rect_mat[prim];
+    int axis = (int)__float_as_uint(B.y);
+    hp = ro + t * rd;
+    if (axis == 0) { n = mk3(1.f, 0.f, 0.f); r.u = (hp.y - A.x) / (A.y - A.x); r.v = (hp.z - A.z) / (A.w - A.z); }
+    else if (axis == 1) { n = mk3(0.f, 1.f, 0.f); r.u = (hp.x - A.x) / (A.y - A.x); r.v = (hp.z - A.z) / (A.w - A.z); }
+    else { n = mk3(0.f, 0.f, 1.f); r.u = (hp.x - A.x) / (A.y - A.x); r.v = (hp.y - A.z) / (A.w - A.z); }
+    if (__float_as_uint(B.z)) n = -n;
+  } else if (kind == PT_GEOM_BOXES) {
+    float4 lo = sc.box_o[2 * prim], hi = sc.box_o[2 * prim + 1];
+    r.mat = sc.box_mat[prim];
+    hp = ro + t * rd;
+    v3 t0 = (mk3(lo) - ro) / rd;
+    v3 t1 = (mk3(hi) - ro) / rd;
+    v3 neg = mk3(t == t0.x ? 1.f : 0.f, t == t0.y ? 1.f : 0.f, t == t0.z ? 1.f : 0.f);
+    v3 pos = mk3(t == t1.x ? 1.f : 0.f, t == t1.y ? 1.f : 0.f, t == t1.z ? 1.f : 0.f);
+    n = pos - neg;
+  } else {
+    const DevMesh& m = sc.meshes[ir[7].z];
+    uint4 ix = sc.idx[m.ibase + prim];
+    v3 a = mk3(sc.vtx[m.vbase + ix.x]), b = mk3(sc.vtx[m.vbase + ix.y]), c = mk3(sc.vtx[m.vbase + ix.z]);
+    TriRay tr = tri_setup(rd);
+    float tt = 0.f, b1 = 0.f, b2 = 0.f;
+    tri_test(tr, ro, a, b, c, tt, b1, b2);
+    float b0 = 1.f - b1 - b2;
+    hp = a * b0 + b * b1 + c * b2;
+    n = cross(b - a, c - a);
+    if (m.has_n) {
+      n = mk3(sc.nrm[m.vbase + ix.x]) * b0 + mk3(sc.nrm[m.vbase + ix.y]) * b1 + mk3(sc.nrm[m.vbase + ix.z]) * b2;
+    }
+    if (m.has_uv) {
+      float2 ua = sc.uv[m.vbase + ix.x], ub = sc.uv[m.vbase + ix.y], uc = sc.uv[m.vbase + ix.z];
+      r.u = ua.x * b0 + ub.x * b1 + uc.x * b2;
+      r.v = ua.y * b0 + ub.y * b1 + uc.y * b2;
+    }
+    r.mat = m.fmat_base >= 0 ? sc.fmat[m.fmat_base + prim] : m.mat;
+  }
+  if (xf) {
+    float4 m0 = as_f4(ir[4]), m1 = as_f4(ir[5]), m2 = as_f4(ir[6]);
+    r.P = xf_point(m0, m1, m2, hp);
+    r.N = normalize(xf_normal(w0, w1, w2, n));
+  } else {
+    r.P = hp;
+    r.N = normalize(n);
+  }
+  return r;
+}
+
+// image_texture.cu:5-7 with the sampler of textures.hpp:177-230: bilinear,
+// repeat wrap, texel centres at (i + 0.5) / w, row 0 at v = 0.
+PTD v3 tex_image(const DevScene& sc, const DevTexture& t, float u, float v) {
+  float x = u * (float)t.w - 0.5f, y = v * (float)t.h - 0.5f;
+  float x0 = floorf(x), y0 = floorf(y);
+  float fx = x - x0, fy = y - y0;
+  long long W = t.w, H = t.h;
+  long long xi = (long long)x0, yi = (long long)y0;
+  long long i0 = xi % W; if (i0 < 0) i0 += W;
+  long long i1 = (xi + 1) % W; if (i1 < 0) i1 += W;
+  long long j0 = yi % H; if (j0 < 0) j0 += H;
+  long long j1 = (yi + 1) % H; if (j1 < 0) j1 += H;
+  const float4* p = sc.texels + t.offset;
+  v3 c00 = mk3(p[j0 * W + i0]), c10 = mk3(p[j0 * W + i1]);
+  v3 c01 = mk3(p[j1 * W + i0]), c11 = mk3(p[j1 * W + i1]);
+  v3 top = c00 * (1.f - fx) + c10 * fx;
+  v3 bot = c01 * (1.f - fx) + c11 * fx;
+  return top * (1.f - fy) + bot * fy;
+}
+PTD v3 tex_eval(const DevScene& sc, int ti, float u, float v) {
+  const DevTexture& t = sc.texs[ti];
+  if (t.type == PT_TEX_CONSTANT) return mk3(t.r, t.g, t.b);
+  return tex_image(sc, t, u, v);
+}
+PTD bool tex_needs_uv(const DevScene& sc, int ti) { return ti >= 0 && sc.texs[ti].type != PT_TEX_CONSTANT; }
+
+PTD Surface make_surface(const DevScene& sc, const HitRec& r) {
+  const DevMaterial& m = sc.mats[r.mat];
+  Surface f;
+  f.type = m.type;
+  f.c0 = tex_eval(sc, m.tex0, r.u, r.v);
+  f.c1 = (m.type == PT_MAT_ASHIKHMIN_SHIRLEY) ? tex_eval(sc, m.tex1, r.u, r.v) : mk3(0.f);
+  f.p0 = m.p0;
+  f.p1 = m.p1;
+  return f;
+}
+
+// miss.cu:30-95
+PTD v3 miss_color(const DevScene& sc, v3 d) {
+  const pt_miss_desc& m = sc.miss;
+  if (m.type == PT_MISS_GRADIENT) {
+    v3 ud = normalize(d);
+    float t = 0.5f * (ud.y + 1.f);
+    v3 c = (1.f - t) * mk3(m.color1[0], m.color1[1], m.color1[2]);
+    c = c + t * mk3(m.color2[0], m.color2[1], m.color2[2]);
+    return c;
+  }
+  if (m.type == PT_MISS_CONSTANT) return mk3(m.color1[0], m.color1[1], m.color1[2]);
+  if (m.type == PT_MISS_IMAGE) {
+    const float PI = 3.14159265358979323846f, INV_PI = 0.318309886183790671538f;
+    float theta = atan2f(d.x, d.z);
+    float phi = PI * 0.5f - acosf(d.y);
+    float u = (theta + PI) * (0.5f * INV_PI);
+    float v = 0.5f * (1.f + sinf(phi));
+    return tex_eval(sc, m.texture, u, v);
+  }
+  float u, v;
+  if (m.spherical) {
+    float r = length(d);
+    float lon = atan2f(d.z, d.x);
+    float lat = acosf(d.y / r);
+    u = lon * (1.f / (PI_F * 2.f));
+    v = lat * (1.f / PI_F);
+  } else {
+    float theta = atan2f(d.x, d.z);
+    theta = theta < 0.f ? theta + (2.f * PI_F) : theta;
+    float phi = acosf(d.y);
+    u = 1.f - (theta / (2.f * PI_F));
+    v = phi / PI_F;
+  }
+  return 2.f * tex_eval(sc, m.texture, u, v);
+}
+
+// rect_pdf.cu:115-146 and sphere_pdf.cu:53-74
+PTD v3 light_sample(const DevLight& l, v3 P, uint32_t& seed) {
+  if (l.type == PT_LIGHT_RECT) {
+    float ra = rnd(seed);
+    float ca = l.a0 + ra * (l.a1 - l.a0);
+    float rb = rnd(seed);
+    float cb = l.b0 + rb * (l.b1 - l.b0);
+    v3 pt = l.axis == 0 ? mk3(l.k, ca, cb) : (l.axis == 1 ? mk3(ca, l.k, cb) : mk3(ca, cb, l.k));
+    return pt - P;
+  }
+  float r1 = rnd(seed);
+  float r2 = rnd(seed);
+  v3 c = mk3(l.cx, l.cy, l.cz);
+  float dsq = squared_length(c - P);
+  float z = 1.f + r2 * (sqrtf(1.f - l.r * l.r / dsq) - 1.f);
+  float phi = 2.f * PI_F * r1;
+  float x = cosf(phi) * sqrtf(1.f - z * z);
+  float y = sinf(phi) * sqrtf(1.f - z * z);
+  v3 Wi = mk3(x, y, z);
+  return onb_to_world(normalize(Wi), Wi);
+}
+// rect_pdf.cu:64-113 and sphere_pdf.cu:7-50
+PTD float light_pdf(const DevLight& l, v3 P, v3 Wi) {
+  const float FLTMAX = 3.402823466e+38f;
+  if (l.type == PT_LIGHT_RECT) {
+    float t, a, b;
+    v3 rn;
+    if (l.axis == 0) { t = (l.k - P.x) / Wi.x; a = P.y + t * Wi.y; b = P.z + t * Wi.z; rn = mk3(1.f, 0.f, 0.f); }
+    else if (l.axis == 1) { t = (l.k - P.y) / Wi.y; a = P.x + t * Wi.x; b = P.z + t * Wi.z; rn = mk3(0.f, 1.f, 0.f); }
+    else { t = (l.k - P.z) / Wi.z; a = P.x + t * Wi.x; b = P.y + t * Wi.y; rn = mk3(0.f, 0.f, 1.f); }
+    if (a < l.a0 || a > l.a1 || b < l.b0 || b > l.b1) return 0.f;
+    if (!(t < FLTMAX && t > 0.001f)) return 0.f;
+    float dsq = t * t * squared_length(Wi);
+    float cosine = fabsf(dot(Wi, rn) / length(Wi));
+    float area = (l.a1 - l.a0) * (l.b1 - l.b0);
+    return dsq / (cosine * area);
+  }
+  v3 c = mk3(l.cx, l.cy, l.cz);
+  v3 oc = P - c;
+  float a = dot(Wi, Wi), b = dot(oc, Wi), cc = dot(oc, oc) - l.r * l.r;
+  float disc = b * b - a * cc;
+  if (disc < 0.f) return 0.f;
+  float tt = (-b - sqrtf(disc)) / a;
+  bool hit = (tt < FLTMAX && tt > 0.001f);
+  if (!hit) {
+    tt = (-b + sqrtf(disc)) / a;
+    hit = (tt < FLTMAX && tt > 0.001f);
+  }
+  if (!hit) return 0.f;
+  float dsq = squared_length(c - P);
+  float ctm = sqrtf(1.f - l.r * l.r / dsq);
+  float sa = 2.f * PI_F * (1.f - ctm);
+  return 1.f / sa;
+}
+
+PTD v3 light_emission(const DevLight& l) { return mk3(l.ex, l.ey, l.ez); }
+
+// material of a hit (for the shadow-ray rule and the material sort)
+PTD int hit_material(const DevScene& sc, int inst_pos, int prim) {
+  const uint4* ir = sc.insts + (size_t)inst_pos * 8;
+  uint32_t kind = ir[3].z & 0xFFu;
+  if (kind == PT_GEOM_SPHERES) return sc.sph_mat[prim];
+  if (kind == PT_GEOM_RECTS) return sc.rect_mat[prim];
+  if (kind == PT_GEOM_BOXES) return sc.box_mat[prim];
+  const DevMesh& m = sc.meshes[ir[7].z];
+  return m.fmat_base >= 0 ? sc.fmat[m.fmat_base + prim] : m.mat;
+}
+
+}  // namespace pt

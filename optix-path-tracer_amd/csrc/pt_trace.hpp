@@ -1,0 +1,197 @@
+// pt_trace.hpp -- two-level BVH4 closest-hit traversal (replaces OptiX's
+// rtTrace: programs/raygen.cu:77, programs/materials/light_sample.cuh:69).
+//
+// One ray per lane.  The traversal stack keeps its top STK_LDS entries in LDS
+// (one column per lane, bank-conflict free) and spills deeper entries to a
+// per-thread slice of HBM.  Children are visited nearest first by entry
+// distance (ties by slot); popped entries farther than the current hit are
+// skipped.  Candidates win on smaller t, ties on the smaller
+// (instance, primitive) key, so the result does not depend on visit order.
+#pragma once
+
+#include "pt_device.hpp"
+
+namespace pt {
+
+struct Stack {
+  uint32_t* ref;  // LDS column, stride BLOCK
+  float* tn;
+  uint2* spill;   // this thread's HBM slice (STK_SPILL entries)
+  int sp;
+  uint32_t* overflow;
+  PTD void put(int pos, uint32_t r, float t) {
+    if (pos < STK_LDS) {
+      ref[pos * BLOCK] = r;
+      tn[pos * BLOCK] = t;
+    } else if (pos < STK_LDS + STK_SPILL) {
+      spill[pos - STK_LDS] = make_uint2(r, __float_as_uint(t));
+    } else {
+      atomicAdd(overflow, 1u);
+    }
+  }
+  PTD void pop(uint32_t& r, float& t) {
+    sp--;
+    if (sp < STK_LDS) {
+      r = ref[sp * BLOCK];
+      t = tn[sp * BLOCK];
+    } else if (sp < STK_LDS + STK_SPILL) {
+      uint2 e = spill[sp - STK_LDS];
+      r = e.x;
+      t = __uint_as_float(e.y);
+    } else {
+      r = EMPTY_CHILD;
+      t = FINF;
+    }
+  }
+};
+
+PTD float4 as_f4(uint4 u) {
+  return make_float4(__uint_as_float(u.x), __uint_as_float(u.y), __uint_as_float(u.z), __uint_as_float(u.w));
+}
+
+struct HitOut {
+  float t;
+  int inst, prim;  // original instance id and primitive index (-1: miss)
+  int pos;         // instance position in top-level order
+};
+
+struct TraceCounts {
+  uint32_t nodes, prims, insts;
+};
+
+// entry distance of one child slot or +inf
+PTD float slab_test(float lox, float hix, float loy, float hiy, float loz, float hiz, v3 o, v3 inv, float tmin,
+                    float tmax) {
+  float t0x = (lox - o.x) * inv.x, t1x = (hix - o.x) * inv.x;
+  float t0y = (loy - o.y) * inv.y, t1y = (hiy - o.y) * inv.y;
+  float t0z = (loz - o.z) * inv.z, t1z = (hiz - o.z) * inv.z;
+  float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
+  float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
+  return (tn <= tf * SLAB_FAR_SCALE) ? tn : FINF;
+}
+
+template <bool COUNT>
+PTD HitOut trace_closest(const DevScene& sc, v3 o, v3 d, float tmin, float tmax, Stack& st, TraceCounts& cnt) {
+  HitOut h{tmax, -1, -1, -1};
+  const v3 winv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+  v3 ro = o, rd = d, rinv = winv;
+  TriRay tr{0, 1, 2, 0.f, 0.f, 0.f};
+  int cur = -1;     // instance slot in top-level order, -1 = top level
+  int base_sp = 0;  // stack depth when the instance was entered
+  uint32_t node_base = sc.tlas_root, prim_base = 0, kind = 0, inst_id = 0;
+  st.sp = 0;
+  st.put(0, 0u, tmin);
+  st.sp = 1;
+  while (true) {
+    if (cur >= 0 && st.sp == base_sp) {
+      cur = -1;
+      node_base = sc.tlas_root;
+      ro = o; rd = d; rinv = winv;
+    }
+    if (st.sp == 0) break;
+    uint32_t ref;
+    float etn;
+    st.pop(ref, etn);
+    if (etn > h.t) continue;
+    if (ref & LEAF_BIT) {
+      uint32_t first = ref & 0x0FFFFFFFu;
+      uint32_t count = ((ref >> 28) & 7u) + 1u;
+      if (cur < 0) {
+        // enter the instance at top-level position `first`
+        const uint4* ir = sc.insts + (size_t)first * 8;
+        float4 w0 = as_f4(ir[0]);
+        float4 w1 = as_f4(ir[1]);
+        float4 w2 = as_f4(ir[2]);
+        uint4 info = ir[3];
+        if (COUNT) cnt.insts++;
+        node_base = info.x;
+        prim_base = info.y;
+        kind = info.z & 0xFFu;
+        inst_id = info.w;
+        if (info.z >> 8) {
+          ro = xf_point(w0, w1, w2, o);
+          rd = xf_vector(w0, w1, w2, d);
+          rinv = mk3(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+        }
+        if (kind == PT_GEOM_MESH) tr = tri_setup(rd);
+        cur = (int)first;
+        base_sp = st.sp;
+        st.put(st.sp, 0u, etn);
+        st.sp++;
+        continue;
+      }
+      for (uint32_t k = 0; k < count; k++) {
+        uint32_t p = prim_base + first + k;
+        if (COUNT) cnt.prims++;
+        float t;
+        uint32_t orig = 0xFFFFFFFFu;
+        if (kind == PT_GEOM_SPHERES) {
+          t = isect_sphere(sc.sph[p], ro, rd, tmin);
+        } else if (kind == PT_GEOM_RECTS) {
+          float4 A = sc.rect[2 * p], B = sc.rect[2 * p + 1];
+          t = isect_rect(A, B.x, (int)__float_as_uint(B.y), ro, rd);
+          orig = __float_as_uint(B.w);
+        } else if (kind == PT_GEOM_BOXES) {
+          float4 lo = sc.box[2 * p], hi = sc.box[2 * p + 1];
+          t = isect_box(mk3(lo), mk3(hi), ro, rd, tmin);
+          orig = __float_as_uint(lo.w);
+        } else {
+          float4 a = sc.tri[3 * p], b = sc.tri[3 * p + 1], c = sc.tri[3 * p + 2];
+          float b1, b2;
+          if (!tri_test(tr, ro, mk3(a), mk3(b), mk3(c), t, b1, b2)) t = FINF;
+          orig = __float_as_uint(a.w);
+        }
+        if (!(t > tmin)) continue;
+        bool take = t < h.t;
+        if (!take && t == h.t && h.inst >= 0) {
+          if (kind == PT_GEOM_SPHERES) orig = sc.sph_idx[p];
+          take = ((int)inst_id < h.inst) || ((int)inst_id == h.inst && (int)orig < h.prim);
+        }
+        if (take) {
+          if (kind == PT_GEOM_SPHERES && orig == 0xFFFFFFFFu) orig = sc.sph_idx[p];
+          h.t = t;
+          h.inst = (int)inst_id;
+          h.prim = (int)orig;
+          h.pos = cur;
+        }
+      }
+      continue;
+    }
+    // internal node
+    if (COUNT) cnt.nodes++;
+    const uint4* nd = sc.nodes + (size_t)(node_base + ref) * 8;
+    uint4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3], q4 = nd[4], q5 = nd[5], q6 = nd[6], q7 = nd[7];
+    const uint32_t lox[4] = {q0.x, q0.y, q0.z, q0.w}, hix[4] = {q1.x, q1.y, q1.z, q1.w};
+    const uint32_t loy[4] = {q2.x, q2.y, q2.z, q2.w}, hiy[4] = {q3.x, q3.y, q3.z, q3.w};
+    const uint32_t loz[4] = {q4.x, q4.y, q4.z, q4.w}, hiz[4] = {q5.x, q5.y, q5.z, q5.w};
+    const uint32_t chd[4] = {q6.x, q6.y, q6.z, q6.w}, cnc[4] = {q7.x, q7.y, q7.z, q7.w};
+    float tn[4];
+    uint32_t cref[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      tn[c] = FINF;
+      if (chd[c] != EMPTY_CHILD)
+        tn[c] = slab_test(__uint_as_float(lox[c]), __uint_as_float(hix[c]), __uint_as_float(loy[c]),
+                          __uint_as_float(hiy[c]), __uint_as_float(loz[c]), __uint_as_float(hiz[c]), ro, rinv, tmin,
+                          h.t);
+      cref[c] = cnc[c] ? (LEAF_BIT | ((cnc[c] - 1u) << 28) | chd[c]) : chd[c];
+    }
+    // rank by (tn, slot); nearest ends on top of the stack
+    int nh = 0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) nh += (tn[c] != FINF);
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      if (tn[c] == FINF) continue;
+      int rank = 0;
+#pragma unroll
+      for (int e = 0; e < 4; e++)
+        if (e != c && tn[e] != FINF && (tn[e] < tn[c] || (tn[e] == tn[c] && e < c))) rank++;
+      st.put(st.sp + nh - 1 - rank, cref[c], tn[c]);
+    }
+    st.sp += nh;
+  }
+  return h;
+}
+
+}  // namespace pt

@@ -1,0 +1,680 @@
+// ptref_render.hpp -- TEST INFRASTRUCTURE ONLY (the oracle).
+//
+// Scalar CPU restatement of the reference's shading and integrator programs:
+// hit-record reconstruction (programs/hitables/*.cu Get_HitRecord), the BRDF
+// sample/eval/pdf of programs/materials/*, light sampling and MIS
+// (programs/materials/light_sample.cuh, programs/pdfs/*.cu), miss programs
+// (programs/miss.cu), the bounce loop (programs/raygen.cu:66-123), per-pixel
+// accumulation and the display mapping (raygen.cu:126-168).
+//
+// Decisions shared with the GPU path where the reference is unspecified:
+//  * rnd() calls inside one argument list are evaluated left to right.
+//  * Hit records are rebuilt in object space from the object-space ray and
+//    then transformed (fixes the double transform of box.cu / sphere.cu).
+//  * Shadow rays: lit iff the closest hit in (1e-3, inf) is a diffuse light
+//    (and dot(N, Wi) > 0), or there is no hit (light_sample.cuh:61-72,
+//    hit.cu:16-24 made order-independent).
+//  * A pdf the reference leaves unset (Evaluate returning early) reads as 1.
+//  * Russian roulette applies when depth >= rr_min_depth (reference: 11).
+#pragma once
+
+#include "ptref_scene.hpp"
+
+namespace ptref {
+
+struct HitRec {
+  f3 P, N, Wo;
+  float u = 0.f, v = 0.f;
+  int mat;
+};
+
+// sphere.cu:65-93, aarect.cu:71-115, box.cu:24-33 + 56-81, triangle.cu:85-149
+inline HitRec hit_record(const Scene& s, int inst, int prim, float t, f3 o, f3 d) {
+  const Instance& I = s.insts[inst];
+  f3 ro, rd;
+  s.to_object(I, o, d, ro, rd);
+  HitRec r;
+  r.Wo = normalize(-d);
+  r.mat = s.material_of(inst, prim);
+  f3 hp, n;
+  switch (I.kind) {
+    case PT_GEOM_SPHERES: {
+      const Sphere& sp = s.spheres[prim];
+      hp = ro + t * rd;
+      n = (hp - sp.c) / sp.r;
+      float phi = std::atan2(n.z, n.x);
+      float theta = std::asin(n.y);
+      r.u = 1.f - (phi + PI_F) / (2.f * PI_F);
+      r.v = (theta + PI_F / 2.f) / PI_F;
+      break;
+    }
+    case PT_GEOM_RECTS: {
+      const Rect& q = s.rects[prim];
+      hp = ro + t * rd;
+      if (q.axis == PT_AXIS_X) { n = mk(1.f, 0.f, 0.f); r.u = (hp.y - q.a0) / (q.a1 - q.a0); r.v = (hp.z - q.b0) / (q.b1 - q.b0); }
+      else if (q.axis == PT_AXIS_Y) { n = mk(0.f, 1.f, 0.f); r.u = (hp.x - q.a0) / (q.a1 - q.a0); r.v = (hp.z - q.b0) / (q.b1 - q.b0); }
+      else { n = mk(0.f, 0.f, 1.f); r.u = (hp.x - q.a0) / (q.a1 - q.a0); r.v = (hp.y - q.b0) / (q.b1 - q.b0); }
+      if (q.flip) n = -n;
+      break;
+    }
+    case PT_GEOM_BOXES: {
+      const Box& b = s.boxes[prim];
+      hp = ro + t * rd;
+      f3 t0 = (b.lo - ro) / rd;
+      f3 t1 = (b.hi - ro) / rd;
+      f3 neg = mk(t == t0.x ? 1.f : 0.f, t == t0.y ? 1.f : 0.f, t == t0.z ? 1.f : 0.f);
+      f3 pos = mk(t == t1.x ? 1.f : 0.f, t == t1.y ? 1.f : 0.f, t == t1.z ? 1.f : 0.f);
+      n = pos - neg;
+      break;
+    }
+    default: {
+      const Mesh& m = s.meshes[I.first];
+      uint32_t i0 = m.idx[3 * prim], i1 = m.idx[3 * prim + 1], i2 = m.idx[3 * prim + 2];
+      f3 a = m.v[i0], b = m.v[i1], c = m.v[i2];
+      TriRay tr = tri_ray_setup(rd);
+      float tt = 0.f, b1 = 0.f, b2 = 0.f;
+      tri_test(tr, ro, a, b, c, tt, b1, b2);
+      float b0 = 1.f - b1 - b2;
+      hp = a * b0 + b * b1 + c * b2;
+      n = cross(b - a, c - a);
+      if (!m.n.empty()) {
+        f3 ns = m.n[i0] * b0 + m.n[i1] * b1 + m.n[i2] * b2;
+        n = ns;
+      }
+      if (!m.uv.empty()) {
+        r.u = m.uv[i0].x * b0 + m.uv[i1].x * b1 + m.uv[i2].x * b2;
+        r.v = m.uv[i0].y * b0 + m.uv[i1].y * b1 + m.uv[i2].y * b2;
+      }
+      break;
+    }
+  }
+  if (I.xf) {
+    r.P = xform_point(I.o2w, hp);
+    r.N = normalize(xform_normal(I.w2o, n));
+  } else {
+    r.P = hp;
+    r.N = normalize(n);
+  }
+  return r;
+}
+
+// Bilinear lookup with repeat wrap, texel centres at (i + 0.5) / w
+// (textures.hpp:177-230 sampler settings; image_texture.cu:5-7).
+inline f3 tex_image(const Texture& t, float u, float v) {
+  float x = u * (float)t.w - 0.5f, y = v * (float)t.h - 0.5f;
+  float x0 = std::floor(x), y0 = std::floor(y);
+  float fx = x - x0, fy = y - y0;
+  auto wrap = [](int64_t i, int64_t n) { int64_t r = i % n; return r < 0 ? r + n : r; };
+  int64_t i0 = wrap((int64_t)x0, t.w), i1 = wrap((int64_t)x0 + 1, t.w);
+  int64_t j0 = wrap((int64_t)y0, t.h), j1 = wrap((int64_t)y0 + 1, t.h);
+  auto px = [&](int64_t i, int64_t j) {
+    const float* p = &t.texels[4 * (size_t)(j * t.w + i)];
+    return mk(p[0], p[1], p[2]);
+  };
+  f3 top = px(i0, j0) * (1.f - fx) + px(i1, j0) * fx;
+  f3 bot = px(i0, j1) * (1.f - fx) + px(i1, j1) * fx;
+  return top * (1.f - fy) + bot * fy;
+}
+inline f3 tex_eval(const Scene& s, int tex, float u, float v) {
+  const Texture& t = s.texs[tex];
+  if (t.type == PT_TEX_CONSTANT) return t.rgb;
+  return tex_image(t, u, v);
+}
+
+// ----------------------------------------------------------------- BRDFs
+struct Surface {
+  int type;
+  f3 c0, c1;    // albedo / diffuse ; specular (AS)
+  float p0, p1; // fuzz | ior | nu,nv | A,B
+};
+
+inline Surface make_surface(const Scene& s, const HitRec& r) {
+  const Material& m = s.mats[r.mat];
+  Surface f;
+  f.type = m.type;
+  f.c0 = tex_eval(s, m.tex[0], r.u, r.v);
+  f.c1 = (m.type == PT_MAT_ASHIKHMIN_SHIRLEY) ? tex_eval(s, m.tex[1], r.u, r.v) : mk(0.f);
+  f.p0 = m.p[0];
+  f.p1 = m.p[1];
+  return f;
+}
+
+// material.cuh:30-39
+inline float schlick(float cosine, float ref_idx) {
+  float r0 = (1.f - ref_idx) / (1.f + ref_idx);
+  r0 = r0 * r0;
+  return r0 + (1.f - r0) * std::pow((1.f - cosine), 5.f);
+}
+inline f3 schlick3(f3 r0, float cosine) {
+  float e = std::pow(1.f - cosine, 5.f);
+  return r0 + (mk(1.f) - r0) * e;
+}
+// material.cuh:50-63
+inline bool Refract(f3 v, f3 n, float ni_over_nt, f3& refracted) {
+  f3 uv = normalize(v);
+  float dt = dot(uv, n);
+  float disc = 1.f - ni_over_nt * ni_over_nt * (1.f - dt * dt);
+  if (disc > 0.f) {
+    refracted = ni_over_nt * (uv - n * dt) - n * std::sqrt(disc);
+    return true;
+  }
+  return false;
+}
+// microfacets.cuh:11-20
+inline void Sample_Quadrant(float nu, float nv, float rx, float ry, float& phi, float& theta) {
+  phi = std::atan(std::sqrt((nu + 1.f) / (nv + 1.f)) * std::tan(2.f * PI_F * rx));
+  float cp = std::cos(phi), sp = std::sin(phi);
+  theta = std::pow(ry, 1.f / (nu * cp * cp + nv * sp * sp + 1.f));
+}
+// microfacets.cuh:85-99
+inline float GGX_D(f3 H, float nu, float nv) {
+  float c2 = Cos2Theta(H);
+  if (c2 <= 0.0f) return 0.f;
+  if (nu == nv) {
+    float beta = c2 * ((nu * nu) - 1) + 1;
+    return (nu * nu) / (PI_F * beta * beta);
+  }
+  float s2nv = Sin2Phi(H) / (nv * nv);
+  float c2nu = (1.f - Sin2Phi(H)) / (nu * nu);
+  float beta = (c2 + (1.f - c2) * (c2nu + s2nv));
+  return 1.f / (PI_F * nu * nv * beta * beta);
+}
+// microfacets.cuh:102-168
+inline f3 GGX_Sample(f3 origin, float rx, float ry, float nu, float nv) {
+  bool flip = origin.y < 0;
+  f3 so = normalize(mk(origin.x * nu, origin.y, origin.z * nv));
+  float slopeX, slopeY;
+  float cosTheta = CosTheta(so);
+  if (cosTheta > 0.9999f) {
+    float r = std::sqrt(rx / (1 - rx));
+    float phi = (float)(6.28318530718 * (double)ry);
+    slopeX = r * std::cos(phi);
+    slopeY = r * std::sin(phi);
+  } else {
+    float sinTheta = std::sqrt(std::fmax(0.f, 1.f - cosTheta * cosTheta));
+    float tanTheta = sinTheta / cosTheta;
+    float a = 1.f / tanTheta;
+    float G1 = 2.f / (1.f + std::sqrt(1.f + 1.f / (a * a)));
+    float A = 2.f * rx / G1 - 1.f;
+    float tmp = 1.f / (A * A - 1.f);
+    if (tmp > 1e10f) tmp = 1e10f;
+    float B = tanTheta;
+    float D = std::sqrt(std::fmax(B * B * tmp * tmp - (A * A - B * B) * tmp, 0.f));
+    float sx1 = B * tmp - D;
+    float sx2 = B * tmp + D;
+    slopeX = (A < 0 || sx2 > 1.f / tanTheta) ? sx1 : sx2;
+    float S;
+    if (ry > 0.5f) { S = 1.f; ry = 2.f * (ry - .5f); }
+    else { S = -1.f; ry = 2.f * (.5f - ry); }
+    float z = (ry * (ry * (ry * 0.27385f - 0.73369f) + 0.46341f)) /
+              (ry * (ry * (ry * 0.093073f + 0.309420f) - 1.f) + 0.597999f);
+    slopeY = S * z * std::sqrt(1.f + slopeX * slopeX);
+  }
+  float t = CosPhi(so) * slopeX - SinPhi(so) * slopeY;
+  slopeY = SinPhi(so) * slopeX + CosPhi(so) * slopeY;
+  slopeX = t;
+  slopeX = nu * slopeX;
+  slopeY = nv * slopeY;
+  f3 H = normalize(mk(-slopeX, 1.f, -slopeY));
+  if (flip) H *= -1.f;
+  return H;
+}
+// microfacets.cuh:170-195
+inline float GGX_Lambda(f3 V, float nu, float nv) {
+  float at = std::fabs(TanTheta(V));
+  if (std::isinf(at)) return 0.f;
+  float alpha = std::sqrt(Cos2Phi(V) * nu * nu + Sin2Phi(V) * nv * nv);
+  float a2t2 = (alpha * at) * (alpha * at);
+  return (-1.f + std::sqrt(1.f + a2t2)) / 2.f;
+}
+inline float GGX_G(f3 Wo, f3 Wi, float nu, float nv) {
+  return 1.f / (1.f + GGX_Lambda(Wo, nu, nv) + GGX_Lambda(Wi, nu, nv));
+}
+
+// Sample(surface, P, Wo, N, seed) of the materials with NEE or a pdf:
+// lambertian.cuh:7-19, oren_nayar.cuh:8-20, diffuse_light.cuh:7-13,
+// ashikhmin_shirley.cuh:11-82, torrance_sparrow.cuh:10-35
+inline f3 brdf_sample(const Surface& f, f3 Wo, f3 N, uint32_t& seed) {
+  switch (f.type) {
+    case PT_MAT_LAMBERTIAN:
+    case PT_MAT_OREN_NAYAR: {
+      float u1 = rnd(seed);
+      float u2 = rnd(seed);
+      return Onb(N).to_world(cosine_sample_hemisphere(u1, u2));
+    }
+    case PT_MAT_DIFFUSE_LIGHT:
+      return random_on_unit_sphere(seed);
+    case PT_MAT_ASHIKHMIN_SHIRLEY: {
+      float nu = f.p0, nv = f.p1;
+      f3 Nn = normalize(N);
+      f3 T = normalize(cross(Nn, mk(0.f, 1.f, 0.f)));
+      f3 B = cross(T, Nn);
+      float rx = rnd(seed);
+      float ry = rnd(seed);
+      if ((double)rx < 0.5) {
+        rx = std::fmin(2 * rx, 1.f - 1e-6f);
+        return Onb(Nn).to_world(cosine_sample_hemisphere(rx, ry));
+      }
+      rx = std::fmin(2 * (rx - 0.5f), 1.f - 1e-6f);
+      float phi, cos_theta;
+      if (nu == nv) {
+        phi = 2.f * PI_F * rx;
+        cos_theta = std::pow(ry, 1.f / (nu + 1.f));
+      } else {
+        if (rx < 0.25f) { Sample_Quadrant(nu, nv, 4.f * rx, ry, phi, cos_theta); }
+        else if (rx < 0.5f) { Sample_Quadrant(nu, nv, 4.f * (0.5f - rx), ry, phi, cos_theta); phi = PI_F - phi; }
+        else if (rx < 0.75f) { Sample_Quadrant(nu, nv, 4.f * (rx - 0.5f), ry, phi, cos_theta); phi = PI_F + phi; }
+        else { Sample_Quadrant(nu, nv, 4.f * (1.f - rx), ry, phi, cos_theta); phi = 2.f * PI_F - phi; }
+      }
+      float sin_theta = std::sqrt(std::fmax(0.f, 1.f - cos_theta * cos_theta));
+      f3 H = normalize(Spherical_Vector(sin_theta, cos_theta, phi));
+      H = H.x * B + H.y * Nn + H.z * T;
+      if (dot(H, Wo) < 0.f) H = -H;
+      return normalize(-Wo + 2.f * dot(Wo, H) * H);
+    }
+    default: {  // Torrance-Sparrow
+      float nu = f.p0, nv = f.p1;
+      f3 Nn = normalize(N);
+      f3 T = normalize(cross(Nn, mk(0.f, 1.f, 0.f)));
+      f3 B = cross(T, Nn);
+      float rx = rnd(seed);
+      float ry = rnd(seed);
+      f3 H = normalize(GGX_Sample(Wo, rx, ry, nu, nv));
+      H = H.x * B + H.y * Nn + H.z * T;
+      if (dot(H, Wo) < 0.f) H = -H;
+      return normalize(-Wo + 2.f * dot(Wo, H) * H);
+    }
+  }
+}
+
+// Evaluate(surface, P, Wo, Wi, N, pdf): lambertian.cuh:21-42,
+// oren_nayar.cuh:22-65, diffuse_light.cuh:15-22, ashikhmin_shirley.cuh:84-151,
+// torrance_sparrow.cuh:37-64.  pdf keeps its incoming value on early returns.
+inline f3 brdf_eval(const Surface& f, f3 Wo, f3 Wi, f3 N, float& pdf) {
+  switch (f.type) {
+    case PT_MAT_LAMBERTIAN: {
+      float cosine = dot(normalize(Wi), normalize(N));
+      if (cosine < 0.f) return mk(0.f);
+      pdf = dot(normalize(Wi), normalize(N)) / PI_F;
+      return (cosine * f.c0) / PI_F;
+    }
+    case PT_MAT_OREN_NAYAR: {
+      f3 WiN = normalize(Wi);
+      float sI = SinTheta(WiN), sO = SinTheta(Wo);
+      float maxCos = 0;
+      if ((double)sI > 1e-4 && (double)sO > 1e-4) {
+        float sPI = SinPhi(WiN), cPI = CosPhi(WiN);
+        float sPO = SinPhi(Wo), cPO = CosPhi(Wo);
+        float dCos = cPI * cPO + sPI * sPO;
+        maxCos = std::fmax(0.f, dCos);
+      }
+      float sinAlpha, tanBeta;
+      if (AbsCosTheta(WiN) > AbsCosTheta(Wo)) { sinAlpha = sO; tanBeta = sI / AbsCosTheta(WiN); }
+      else { sinAlpha = sI; tanBeta = sO / AbsCosTheta(Wo); }
+      pdf = dot(normalize(Wi), normalize(N)) / PI_F;
+      return f.c0 * (1.f / PI_F) * (f.p0 + f.p1 * maxCos * sinAlpha * tanBeta);
+    }
+    case PT_MAT_DIFFUSE_LIGHT:
+      pdf = 1.f;
+      return f.c0;
+    case PT_MAT_ASHIKHMIN_SHIRLEY: {
+      f3 Rd = f.c0, Rs = f.c1;
+      float nu = f.p0, nv = f.p1;
+      f3 Up = mk(0.f, 1.f, 0.f);
+      f3 Nn = normalize(N);
+      f3 T = normalize(cross(Nn, Up));
+      f3 B = cross(T, Nn);
+      float NdotI = std::fabs(dot(Up, Wi)), NdotO = std::fabs(dot(Up, Wo));
+      f3 diff = Rd * (28.f / (23.f * PI_F));
+      diff *= (mk(1.f) - Rs);
+      diff *= (1.f - std::pow(1.f - NdotI * 0.5f, 5.f));
+      diff *= (1.f - std::pow(1.f - NdotO * 0.5f, 5.f));
+      float dpdf = dot(Wi, Nn) / PI_F;
+      if (dpdf < 0.f) return mk(0.f);
+      f3 H = Wo + Wi;
+      if (isNull(H)) return mk(0.f);
+      H = normalize(H);
+      float HdotI = std::fabs(dot(H, Wi));
+      float HdotN = std::fabs(dot(H, Nn)), HdotT = std::fabs(dot(H, T)), HdotB = std::fabs(dot(H, B));
+      float norm, lobe;
+      if (nu == nv) {
+        norm = (nu + 1.f) / (8.f * PI_F);
+        lobe = std::pow(HdotN, nu);
+      } else {
+        norm = std::sqrt((nu + 1.f) * (nv + 1.f)) / (8.f * PI_F);
+        if (HdotN < 1.f)
+          lobe = std::pow(HdotN, (nu * HdotT * HdotT + nv * HdotB * HdotB) / (1.f - HdotN * HdotN));
+        else
+          lobe = 1.f;
+      }
+      f3 spec = schlick3(Rs, HdotI);
+      spec *= norm;
+      spec *= lobe / (HdotI * std::fmax(NdotI, NdotO));
+      float spdf = HdotI / (norm * lobe);
+      pdf = (dpdf + spdf) / 2.f;
+      return (diff / dpdf) + (spec / spdf);
+    }
+    default: {  // Torrance-Sparrow
+      f3 Rs = f.c0;
+      float nu = f.p0, nv = f.p1;
+      f3 Up = mk(0.f, 1.f, 0.f);
+      float NdotI = std::fmax(dot(Up, Wi), 1e-6f), NdotO = std::fmax(dot(Up, Wo), 1e-6f);
+      f3 H = Wo + Wi;
+      if (isNull(H)) return mk(0.f);
+      H = normalize(H);
+      float HdotI = std::fabs(dot(H, Wi));
+      f3 F = schlick3(Rs, HdotI);
+      float G = GGX_G(Wo, Wi, nu, nv);
+      float D = GGX_D(H, nu, nv);
+      pdf = (D * AbsCosTheta(H)) / (4.f * dot(Wo, H));
+      return Rs * D * G * F / (4.f * NdotI * NdotO);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- lights
+// rect_pdf.cu:11-146, sphere_pdf.cu:8-74
+inline f3 light_sample(const Light& l, f3 P, uint32_t& seed) {
+  if (l.type == PT_LIGHT_RECT) {
+    float ra = rnd(seed);
+    float ca = l.a0 + ra * (l.a1 - l.a0);
+    float rb = rnd(seed);
+    float cb = l.b0 + rb * (l.b1 - l.b0);
+    f3 pt = l.axis == PT_AXIS_X ? mk(l.k, ca, cb) : (l.axis == PT_AXIS_Y ? mk(ca, l.k, cb) : mk(ca, cb, l.k));
+    return pt - P;
+  }
+  // sphere: cone sample, then the reference's basis built around Wi itself
+  float r1 = rnd(seed);
+  float r2 = rnd(seed);
+  float dsq = squared_length(l.c - P);
+  float z = 1.f + r2 * (std::sqrt(1.f - l.r * l.r / dsq) - 1.f);
+  float phi = 2.f * PI_F * r1;
+  float x = std::cos(phi) * std::sqrt(1.f - z * z);
+  float y = std::sin(phi) * std::sqrt(1.f - z * z);
+  f3 Wi = mk(x, y, z);
+  return Onb(normalize(Wi)).to_world(Wi);
+}
+inline float light_pdf(const Light& l, f3 P, f3 Wi) {
+  if (l.type == PT_LIGHT_RECT) {
+    float t, a, b;
+    f3 rn;
+    if (l.axis == PT_AXIS_X) { t = (l.k - P.x) / Wi.x; a = P.y + t * Wi.y; b = P.z + t * Wi.z; rn = mk(1.f, 0.f, 0.f); }
+    else if (l.axis == PT_AXIS_Y) { t = (l.k - P.y) / Wi.y; a = P.x + t * Wi.x; b = P.z + t * Wi.z; rn = mk(0.f, 1.f, 0.f); }
+    else { t = (l.k - P.z) / Wi.z; a = P.x + t * Wi.x; b = P.y + t * Wi.y; rn = mk(0.f, 0.f, 1.f); }
+    if (a < l.a0 || a > l.a1 || b < l.b0 || b > l.b1) return 0.f;
+    if (!(t < FLT_MAX && t > 0.001f)) return 0.f;
+    float dsq = t * t * squared_length(Wi);
+    float cosine = std::fabs(dot(Wi, rn) / length(Wi));
+    float area = (l.a1 - l.a0) * (l.b1 - l.b0);
+    return dsq / (cosine * area);
+  }
+  // sphere
+  f3 oc = P - l.c;
+  float a = dot(Wi, Wi), b = dot(oc, Wi), c = dot(oc, oc) - l.r * l.r;
+  float disc = b * b - a * c;
+  if (disc < 0.f) return 0.f;
+  float tt = (-b - std::sqrt(disc)) / a;
+  bool hit = (tt < FLT_MAX && tt > 0.001f);
+  if (!hit) {
+    tt = (-b + std::sqrt(disc)) / a;
+    hit = (tt < FLT_MAX && tt > 0.001f);
+  }
+  if (!hit) return 0.f;
+  float dsq = squared_length(l.c - P);
+  float ctm = std::sqrt(1.f - l.r * l.r / dsq);
+  float sa = 2.f * PI_F * (1.f - ctm);
+  return 1.f / sa;
+}
+
+// light_sample.cuh:27-33
+inline float PowerHeuristic(unsigned int nf, float fPdf, unsigned int ng, float gPdf) {
+  float f = nf * fPdf, g = ng * gPdf;
+  return (f * f) / (f * f + g * g);
+}
+
+inline uint32_t mix(uint32_t h, uint32_t v) { return (h ^ v) * 16777619u; }
+
+struct PathCounters {
+  uint64_t ext = 0, shadow = 0;
+};
+
+// light_sample.cuh:35-103 with the shadow ray resolved by the shared rule
+inline f3 direct_light(const Scene& s, const Surface& f, f3 P, f3 Wo, f3 N, bool isLight, uint32_t& seed,
+                       uint32_t& sig, PathCounters& pc) {
+  f3 direct = mk(0.f);
+  int nl = (int)s.lights.size();
+  if (nl == 0) return mk(0.f);
+  int index = ((int)(rnd(seed) * nl)) % nl;
+  if (isLight && nl == 1) return mk(0.f);
+  const Light& L = s.lights[index];
+  f3 emission = L.e;
+  f3 Wi = light_sample(L, P, seed);
+  float lightPDF = light_pdf(L, P, Wi);
+  if (dot(Wi, N) < 0.f) { sig = mix(sig, 0x33333333u); return mk(0.f); }
+  pc.shadow++;
+  HitState h = s.closest(P, Wi, 1e-3f, INF);
+  bool occluded = false;
+  if (h.found()) {
+    int m = s.material_of(h.inst, h.prim);
+    occluded = !(s.mats[m].type == PT_MAT_DIFFUSE_LIGHT && dot(N, Wi) > 0.f);
+  }
+  sig = mix(sig, occluded ? 0xA5A5A5A5u : 0x5A5A5A5Au);
+  if (occluded) return mk(0.f);
+  if (lightPDF != 0.f && !isNull(emission)) {
+    float matPDF = 1.f;
+    f3 mv = brdf_eval(f, Wo, Wi, N, matPDF);
+    if (matPDF != 0.f && !isNull(mv)) {
+      float w = PowerHeuristic(1, lightPDF, 1, matPDF);
+      direct += mv * emission * w / lightPDF;
+    }
+  }
+  Wi = brdf_sample(f, Wo, N, seed);
+  float matPDF = 1.f;
+  f3 mv = brdf_eval(f, Wo, Wi, N, matPDF);
+  if (matPDF != 0.f && !isNull(mv)) {
+    lightPDF = light_pdf(L, P, Wi);
+    if (!lightPDF || isNull(emission)) return direct;
+    float w = PowerHeuristic(1, matPDF, 1, lightPDF);
+    direct += mv * emission * w / matPDF;
+  }
+  return direct;
+}
+
+// miss.cu:30-95
+inline f3 miss_color(const Scene& s, f3 d) {
+  const pt_miss_desc& m = s.miss;
+  switch (m.type) {
+    case PT_MISS_GRADIENT: {
+      f3 ud = normalize(d);
+      float t = 0.5f * (ud.y + 1.f);
+      f3 c = (1.f - t) * mk(m.color1[0], m.color1[1], m.color1[2]);
+      c += t * mk(m.color2[0], m.color2[1], m.color2[2]);
+      return c;
+    }
+    case PT_MISS_CONSTANT:
+      return mk(m.color1[0], m.color1[1], m.color1[2]);
+    case PT_MISS_IMAGE: {
+      const float PI = 3.14159265358979323846f, INV_PI = 0.318309886183790671538f;
+      float theta = std::atan2(d.x, d.z);
+      float phi = PI * 0.5f - std::acos(d.y);
+      float u = (theta + PI) * (0.5f * INV_PI);
+      float v = 0.5f * (1.f + std::sin(phi));
+      return tex_eval(s, m.texture, u, v);
+    }
+    default: {
+      float u, v;
+      if (m.spherical) {
+        float r = length(d);
+        float lon = std::atan2(d.z, d.x);
+        float lat = std::acos(d.y / r);
+        u = lon * (1.f / (PI_F * 2.f));
+        v = lat * (1.f / PI_F);
+      } else {
+        float theta = std::atan2(d.x, d.z);
+        theta = theta < 0.f ? theta + (2.f * PI_F) : theta;
+        float phi = std::acos(d.y);
+        u = 1.f - (theta / (2.f * PI_F));
+        v = phi / PI_F;
+      }
+      return 2.f * tex_eval(s, m.texture, u, v);
+    }
+  }
+}
+
+struct Params {
+  uint32_t W, H, max_depth;
+  bool russian;
+  uint32_t rr_min_depth;
+};
+
+struct PathResult {
+  f3 L;
+  uint32_t sig;
+  uint32_t bounces;
+};
+
+// raygen.cu:148-168 (seed, jitter, camera ray) and raygen.cu:66-123 (color)
+inline PathResult trace_path(const Scene& s, const pt_camera& cam, const Params& p, uint32_t x, uint32_t y,
+                             uint32_t frame, PathCounters& pc) {
+  PathResult res{mk(0.f), 0x811C9DC5u, 0};
+  uint32_t seed = tea(p.W * y + x, frame, 64);
+  float u = ((float)x + rnd(seed)) / (float)p.W;
+  float v = ((float)y + rnd(seed)) / (float)p.H;
+  f3 cu = mk(cam.u[0], cam.u[1], cam.u[2]), cv = mk(cam.v[0], cam.v[1], cam.v[2]);
+  f3 rd = cam.lens_radius * random_in_unit_disk(seed);
+  f3 lens = cu * rd.x + cv * rd.y;
+  f3 origin = mk(cam.origin[0], cam.origin[1], cam.origin[2]) + lens;
+  f3 dir = mk(cam.lower_left[0], cam.lower_left[1], cam.lower_left[2]) +
+           u * mk(cam.horizontal[0], cam.horizontal[1], cam.horizontal[2]) +
+           v * mk(cam.vertical[0], cam.vertical[1], cam.vertical[2]) - origin;
+  float tmin = 1e-6f;
+  (void)(cam.time0 + rnd(seed) * (cam.time1 - cam.time0));  // prd.time, unused
+  f3 T = mk(1.f), L = mk(0.f);
+  bool prevSpec = false;
+  for (uint32_t depth = 0; depth < p.max_depth; depth++) {
+    pc.ext++;
+    res.bounces = depth + 1;
+    HitState h = s.closest(origin, dir, tmin, INF);
+    res.sig = mix(res.sig, h.found() ? (((uint32_t)h.inst << 20) ^ (uint32_t)h.prim) : 0xFFFFFFFFu);
+    if (!h.found()) {
+      T *= miss_color(s, dir);
+      res.L = L + clamp3(T, 0.f, 1.f);
+      return res;
+    }
+    HitRec r = hit_record(s, h.inst, h.prim, h.t, origin, dir);
+    Surface f = make_surface(s, r);
+    f3 P = r.P, Wo = r.Wo, N = r.N;
+    bool spec = false;
+    switch (f.type) {
+      case PT_MAT_DIFFUSE_LIGHT: {  // diffuse_light.cu:30-48
+        f3 dl = direct_light(s, f, P, Wo, N, true, seed, res.sig, pc);
+        L += T * dl;
+        if (dot(N, Wo) < 0.f) T *= f.c0;
+        if (depth == 0 || prevSpec) L += T;
+        res.L = L;
+        return res;
+      }
+      case PT_MAT_LAMBERTIAN:
+      case PT_MAT_OREN_NAYAR: {  // lambertian.cu:49-73, oren_nayar.cu:46-69
+        f3 dl = direct_light(s, f, P, Wo, N, false, seed, res.sig, pc);
+        L += T * dl;
+        f3 Wi = brdf_sample(f, Wo, N, seed);
+        float pdf = 1.f;
+        f3 att = brdf_eval(f, Wo, Wi, N, pdf);
+        origin = P;
+        dir = Wi;
+        T *= clamp3(att / pdf, 0.f, 1.f);
+        break;
+      }
+      case PT_MAT_METAL: {  // metal.cu:38-56
+        f3 refl = reflect(-Wo, N);
+        dir = refl + f.p0 * random_in_unit_sphere(seed);
+        origin = P;
+        T *= f.c0;
+        spec = true;
+        break;
+      }
+      case PT_MAT_DIELECTRIC: {  // dielectric.cu:46-96
+        float ior = f.p0;
+        f3 W = -Wo;
+        f3 Nd = N;
+        float ni_over_nt;
+        float cosine = dot(W, Nd);
+        if (cosine > 0.f) {
+          Nd = -Nd;
+          ni_over_nt = ior;
+          cosine = ior * cosine / length(W);
+        } else {
+          ni_over_nt = 1.f / ior;
+          cosine = -cosine / length(W);
+        }
+        f3 refracted = mk(0.f);
+        float reflect_prob;
+        if (Refract(W, Nd, ni_over_nt, refracted)) reflect_prob = schlick(cosine, ior);
+        else reflect_prob = 1.f;
+        bool refl = rnd(seed) < reflect_prob;
+        res.sig = mix(res.sig, refl ? 0x11111111u : 0x22222222u);
+        if (refl) dir = reflect(W, Nd);
+        else dir = normalize(refracted);
+        origin = P;
+        T *= f.c0;  // absorption disabled (dielectric.cu:67)
+        spec = true;
+        break;
+      }
+      case PT_MAT_ASHIKHMIN_SHIRLEY: {  // ashikhmin_shirley.cu:48-68
+        f3 Wi = brdf_sample(f, Wo, N, seed);
+        float pdf = 1.f;
+        f3 att = brdf_eval(f, Wo, Wi, N, pdf);
+        origin = P;
+        dir = Wi;
+        T *= clamp3(att, 0.f, 1.f);
+        spec = true;
+        break;
+      }
+      default: {  // Torrance-Sparrow, torrance_sparrow.cu:39-59
+        f3 Wi = brdf_sample(f, Wo, N, seed);
+        float pdf = 1.f;
+        f3 att = brdf_eval(f, Wo, Wi, N, pdf);
+        origin = P;
+        dir = Wi;
+        T *= clamp3(att / pdf, 0.f, 1.f);
+        spec = true;
+        break;
+      }
+    }
+    tmin = 1e-3f;
+    prevSpec = spec;
+    if (p.russian) {  // raygen.cu:110-118
+      float prob = max_component(T);
+      if (depth >= p.rr_min_depth) {
+        bool term = rnd(seed) >= prob;
+        res.sig = mix(res.sig, term ? 0x44444444u : 0x55555555u);
+        if (term) {
+          res.L = L + T;
+          return res;
+        }
+        T *= 1.f / prob;
+      }
+    }
+  }
+  res.L = mk(0.f);  // raygen.cu:121-122
+  return res;
+}
+
+// raygen.cu:126-134
+inline f3 de_nan(f3 c) {
+  if (!(c.x == c.x)) c.x = 0.f;
+  if (!(c.y == c.y)) c.y = 0.f;
+  if (!(c.z == c.z)) c.z = 0.f;
+  return c;
+}
+
+// raygen.cu:136-146 on the FP64 accumulation
+inline uint8_t display_channel(double acc, uint32_t frames) {
+  float avg = (float)(acc / (double)frames);
+  float s = std::sqrt(avg);
+  s = clampf(s, 0.f, 1.f);
+  return (uint8_t)(int)(255.99 * (double)s);
+}
+
+}  // namespace ptref
