@@ -1,0 +1,48 @@
+/*
+ * pt_scene.h -- synthetic benchmark scenes built through the host scene API
+ * (the C++ facade over pt_api.h that mirrors the reference's host_includes/
+ * builders), exported as flattened pt_scene_desc + pt_camera for callers
+ * that are not C++.  Part of libpt_hip.so; no GPU is touched.
+ *
+ * No counterpart in the reference: its scenes are C++ functions
+ * (host_includes/scenes.hpp) run inside the GUI application.
+ */
+#ifndef PT_SCENE_H
+#define PT_SCENE_H
+
+#include "pt_api.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PT_SCENE_RANDOM_SPHERES 0 /* random-spheres scene  */
+#define PT_SCENE_CORNELL 1        /* Cornell box with MIS  */
+
+typedef struct pt_scene_handle pt_scene_handle;
+
+typedef struct pt_scene_info {
+  uint32_t width, height;  /* image size the camera was built for */
+  uint32_t spp;            /* samples per pixel of the workload   */
+  uint32_t max_depth;
+  int32_t russian;
+  uint32_t rr_min_depth;
+  uint32_t reserved[2];
+} pt_scene_info;
+
+/* Build scene `config` for a width x height image (0 = the workload's size);
+ * `variant` seeds the scene RNG where the scene is random. */
+pt_status pt_scene_build(int32_t config, uint32_t width, uint32_t height, uint32_t variant,
+                         pt_scene_handle** out);
+/* Views owned by the handle, valid until pt_scene_free. */
+const pt_scene_desc* pt_scene_get_desc(const pt_scene_handle* h);
+const pt_camera* pt_scene_get_camera(const pt_scene_handle* h);
+const pt_scene_info* pt_scene_get_info(const pt_scene_handle* h);
+void pt_scene_free(pt_scene_handle* h);
+const char* pt_scene_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PT_SCENE_H */

@@ -1,0 +1,159 @@
+// hitables.hpp -- geometry builders of the host scene API
+// (host_includes/hitables.hpp:20-528): Sphere, AARect, Box and
+// Hitable_List with the reference's transform-order semantics
+// (Hitable::addTo reverses its list, Hitable_List::addElementsTo and
+// addListTo do not).
+#pragma once
+
+#include <algorithm>
+
+#include "host_common.hpp"
+#include "materials.hpp"
+#include "programs.hpp"
+#include "transforms.hpp"
+
+// hitables.hpp:21-86
+class Hitable {
+ public:
+  std::vector<TransformParameter> transforms;
+  Hitable(BRDF* material) : material(material) {}
+  virtual ~Hitable() = default;
+  virtual GeometryInstance getGeometryInstance(Context& g_context) = 0;
+  virtual void rotate(float angle, AXIS axis) {
+    transforms.push_back(TransformParameter(Rotate_Transform, angle, axis, make_float3(0.f), make_float3(0.f)));
+  }
+  virtual void scale(float3 s) {
+    transforms.push_back(TransformParameter(Scale_Transform, 0.f, X_AXIS, s, make_float3(0.f)));
+  }
+  virtual void translate(float3 pos) {
+    transforms.push_back(TransformParameter(Translate_Transform, 0.f, X_AXIS, make_float3(0.f), pos));
+  }
+  virtual void addTo(Group& d_world, Context& g_context) {
+    std::reverse(transforms.begin(), transforms.end());
+    GeometryInstance gi = getGeometryInstance(g_context);
+    addAndTransform(gi, d_world, g_context, transforms);
+  }
+
+ protected:
+  BRDF* material;
+  GeometryInstance make_instance(Context& g_context, const ptfacade::Prim& p) {
+    if (!material) throw "Hitable has no material";
+    GeometryInstance gi = g_context->createGeometryInstance();
+    gi->setMaterialCount(1);
+    gi->setMaterial(0, material->assignTo(g_context));
+    gi->prims.push_back(p);
+    return gi;
+  }
+};
+
+// hitables.hpp:89-128
+class Sphere : public Hitable {
+ public:
+  Sphere(const float3& c, const float r, BRDF* material) : Hitable(material), center(c), radius(r) {}
+  GeometryInstance getGeometryInstance(Context& g_context) override {
+    ptfacade::Prim p;
+    p.kind = PT_GEOM_SPHERES;
+    p.s = pt_sphere_desc{{center.x, center.y, center.z}, radius, -1};
+    return make_instance(g_context, p);
+  }
+
+ protected:
+  const float3 center;
+  const float radius;
+};
+
+// hitables.hpp:211-272: a0..a1 and b0..b1 span the two axes other than
+// `ax` (Y,Z for X; X,Z for Y; X,Y for Z), k lies on `ax`.
+class AARect : public Hitable {
+ public:
+  AARect(const float a0, const float a1, const float b0, const float b1, const float k, const bool flip,
+         const AXIS ax, BRDF* material)
+      : Hitable(material), a0(a0), a1(a1), b0(b0), b1(b1), k(k), axis(ax), flip(flip) {}
+  GeometryInstance getGeometryInstance(Context& g_context) override {
+    ptfacade::Prim p;
+    p.kind = PT_GEOM_RECTS;
+    p.r = pt_rect_desc{(int32_t)axis, flip ? 1 : 0, a0, a1, b0, b1, k, -1};
+    return make_instance(g_context, p);
+  }
+
+ protected:
+  const float a0, a1, b0, b1, k;
+  const AXIS axis;
+  const bool flip;
+};
+
+// hitables.hpp:275-313
+class Box : public Hitable {
+ public:
+  Box(const float3& p0, const float3& p1, BRDF* material) : Hitable(material), p0(p0), p1(p1) {}
+  GeometryInstance getGeometryInstance(Context& g_context) override {
+    ptfacade::Prim p;
+    p.kind = PT_GEOM_BOXES;
+    p.b = pt_box_desc{{p0.x, p0.y, p0.z}, {p1.x, p1.y, p1.z}, -1};
+    return make_instance(g_context, p);
+  }
+
+ protected:
+  const float3 p0, p1;
+};
+
+// Hitables of the reference without a device program yet
+// (moving_sphere.cu, volume_sphere.cu, volume_box.cu, cylinder.cu).
+class Moving_Sphere : public Hitable {
+ public:
+  Moving_Sphere(const float3&, const float3&, const float, const float, const float, BRDF* m) : Hitable(m) {}
+  GeometryInstance getGeometryInstance(Context&) override { throw "Moving_Sphere is not supported by the GPU path yet"; }
+};
+class Volumetric_Sphere : public Hitable {
+ public:
+  Volumetric_Sphere(const float3&, const float, const float, BRDF* m) : Hitable(m) {}
+  GeometryInstance getGeometryInstance(Context&) override { throw "Volumetric_Sphere is not supported by the GPU path yet"; }
+};
+class Volumetric_Box : public Hitable {
+ public:
+  Volumetric_Box(const float3&, const float3&, const float, BRDF* m) : Hitable(m) {}
+  GeometryInstance getGeometryInstance(Context&) override { throw "Volumetric_Box is not supported by the GPU path yet"; }
+};
+class Cylinder : public Hitable {
+ public:
+  Cylinder(const float3&, const float&, const float, BRDF* m) : Hitable(m) {}
+  GeometryInstance getGeometryInstance(Context&) override { throw "Cylinder is not supported by the GPU path yet"; }
+};
+
+// hitables.hpp:457-528
+class Hitable_List {
+ public:
+  Hitable_List() {}
+  int push(Hitable* hit) {
+    int index = (int)hitList.size();
+    hitList.push_back(hit);
+    return index;
+  }
+  void rotate(float angle, AXIS axis) {
+    transforms.push_back(TransformParameter(Rotate_Transform, angle, axis, make_float3(0.f), make_float3(0.f)));
+  }
+  void scale(float3 s) { transforms.push_back(TransformParameter(Scale_Transform, 0.f, X_AXIS, s, make_float3(0.f))); }
+  void translate(float3 pos) {
+    transforms.push_back(TransformParameter(Translate_Transform, 0.f, X_AXIS, make_float3(0.f), pos));
+  }
+  GeometryGroup getGroup(Context& g_context) {
+    GeometryGroup gg = g_context->createGeometryGroup();
+    gg->setAcceleration(g_context->createAcceleration("Trbvh"));
+    for (size_t i = 0; i < hitList.size(); i++) gg->addChild(hitList[i]->getGeometryInstance(g_context));
+    return gg;
+  }
+  void addListTo(Group& d_world, Context& g_context) {
+    GeometryGroup gg = getGroup(g_context);
+    addAndTransform(gg, d_world, g_context, transforms);
+  }
+  void addElementsTo(Group& d_world, Context& g_context) {
+    for (int i = 0; i < (int)hitList.size(); i++) {
+      GeometryInstance gi = hitList[i]->getGeometryInstance(g_context);
+      addAndTransform(gi, d_world, g_context, hitList[i]->transforms);
+    }
+  }
+
+ protected:
+  std::vector<Hitable*> hitList;
+  std::vector<TransformParameter> transforms;
+};

@@ -1,0 +1,326 @@
+"""ctypes binding of libpt_hip.so (include/pt_api.h, include/pt_scene.h).
+
+The library is the MI355X path tracer: the scene, acceleration structure,
+render and query entry points of the C ABI.  This module only loads it and
+mirrors its structs; it has no compute of its own and no fallback: if the
+library is missing, loading fails with an error.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent.parent
+REPO = PKG_DIR.parent
+LIB_PATH = PKG_DIR / "libpt_hip.so"
+
+PT_OK = 0
+
+PT_GEOM_SPHERES, PT_GEOM_RECTS, PT_GEOM_BOXES, PT_GEOM_MESH = 0, 1, 2, 3
+PT_MAT_LAMBERTIAN, PT_MAT_METAL, PT_MAT_DIELECTRIC, PT_MAT_DIFFUSE_LIGHT = 0, 1, 2, 3
+PT_MAT_ASHIKHMIN_SHIRLEY, PT_MAT_TORRANCE_SPARROW, PT_MAT_OREN_NAYAR = 4, 5, 6
+PT_TEX_CONSTANT, PT_TEX_IMAGE = 0, 1
+PT_LIGHT_RECT, PT_LIGHT_SPHERE, PT_LIGHT_ENV = 0, 1, 2
+PT_MISS_GRADIENT, PT_MISS_CONSTANT, PT_MISS_IMAGE, PT_MISS_HDR = 0, 1, 2, 3
+PT_FLAG_MATERIAL_SORT = 0x1
+PT_FLAG_KEEP_PATHS = 0x2
+PT_FLAG_ENV_MIS = 0x4
+PT_FLAG_COUNT_TRAVERSAL = 0x8
+PT_SCENE_RANDOM_SPHERES, PT_SCENE_CORNELL = 0, 1
+
+
+class SphereDesc(C.Structure):
+    _fields_ = [("center", C.c_float * 3), ("radius", C.c_float), ("material", C.c_int32)]
+
+
+class RectDesc(C.Structure):
+    _fields_ = [("axis", C.c_int32), ("flip", C.c_int32), ("a0", C.c_float), ("a1", C.c_float),
+                ("b0", C.c_float), ("b1", C.c_float), ("k", C.c_float), ("material", C.c_int32)]
+
+
+class BoxDesc(C.Structure):
+    _fields_ = [("pmin", C.c_float * 3), ("pmax", C.c_float * 3), ("material", C.c_int32)]
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("num_vertices", C.c_uint32), ("vertices", C.POINTER(C.c_float)),
+                ("normals", C.POINTER(C.c_float)), ("uvs", C.POINTER(C.c_float)),
+                ("num_triangles", C.c_uint32), ("indices", C.POINTER(C.c_uint32)),
+                ("face_materials", C.POINTER(C.c_int32)), ("material", C.c_int32)]
+
+
+class InstanceDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("first", C.c_uint32), ("count", C.c_uint32),
+                ("has_transform", C.c_int32), ("obj_to_world", C.c_float * 12),
+                ("world_to_obj", C.c_float * 12)]
+
+
+class MaterialDesc(C.Structure):
+    _fields_ = [("type", C.c_int32), ("tex", C.c_int32 * 2), ("param", C.c_float * 4)]
+
+
+class TextureDesc(C.Structure):
+    _fields_ = [("type", C.c_int32), ("rgb", C.c_float * 3), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("texels", C.POINTER(C.c_float))]
+
+
+class LightDesc(C.Structure):
+    _fields_ = [("type", C.c_int32), ("axis", C.c_int32), ("a0", C.c_float), ("a1", C.c_float),
+                ("b0", C.c_float), ("b1", C.c_float), ("k", C.c_float), ("center", C.c_float * 3),
+                ("radius", C.c_float), ("emission", C.c_float * 3)]
+
+
+class MissDesc(C.Structure):
+    _fields_ = [("type", C.c_int32), ("color1", C.c_float * 3), ("color2", C.c_float * 3),
+                ("texture", C.c_int32), ("spherical", C.c_int32)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("num_spheres", C.c_uint32), ("spheres", C.POINTER(SphereDesc)),
+                ("num_rects", C.c_uint32), ("rects", C.POINTER(RectDesc)),
+                ("num_boxes", C.c_uint32), ("boxes", C.POINTER(BoxDesc)),
+                ("num_meshes", C.c_uint32), ("meshes", C.POINTER(MeshDesc)),
+                ("num_instances", C.c_uint32), ("instances", C.POINTER(InstanceDesc)),
+                ("num_materials", C.c_uint32), ("materials", C.POINTER(MaterialDesc)),
+                ("num_textures", C.c_uint32), ("textures", C.POINTER(TextureDesc)),
+                ("num_lights", C.c_uint32), ("lights", C.POINTER(LightDesc)),
+                ("miss", MissDesc)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("lower_left", C.c_float * 3), ("horizontal", C.c_float * 3),
+                ("vertical", C.c_float * 3), ("origin", C.c_float * 3), ("u", C.c_float * 3),
+                ("v", C.c_float * 3), ("w", C.c_float * 3), ("lens_radius", C.c_float),
+                ("time0", C.c_float), ("time1", C.c_float)]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("max_depth", C.c_uint32),
+                ("russian", C.c_int32), ("rr_min_depth", C.c_uint32),
+                ("frames_per_batch", C.c_uint32), ("flags", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("paths", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("node_fetches", C.c_uint64), ("prim_fetches", C.c_uint64),
+                ("instance_fetches", C.c_uint64), ("gpu_ms", C.c_double), ("extend_ms", C.c_double),
+                ("shadow_ms", C.c_double), ("extend_launches", C.c_uint32),
+                ("shadow_launches", C.c_uint32)]
+
+
+class AccelStats(C.Structure):
+    _fields_ = [("num_blas", C.c_uint32), ("num_instances", C.c_uint32),
+                ("num_primitives", C.c_uint64), ("num_nodes", C.c_uint64), ("bytes", C.c_uint64),
+                ("build_ms", C.c_double)]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
+                ("max_depth", C.c_uint32), ("russian", C.c_int32), ("rr_min_depth", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
+
+
+RAY_DTYPE = np.dtype([("org", np.float32, 3), ("tmin", np.float32), ("dir", np.float32, 3),
+                      ("tmax", np.float32)])
+HIT_DTYPE = np.dtype([("t", np.float32), ("inst", np.int32), ("prim", np.int32),
+                      ("reserved", np.uint32)])
+
+_lib = None
+
+
+def _declare(lib):
+    P = C.c_void_p
+    sigs = {
+        "pt_api_version": (C.c_int32, []),
+        "pt_create": (C.c_int32, [C.c_int32, C.POINTER(C.c_void_p)]),
+        "pt_destroy": (None, [P]),
+        "pt_last_error": (C.c_char_p, [P]),
+        "pt_set_scene": (C.c_int32, [P, C.POINTER(SceneDesc)]),
+        "pt_build_accel": (C.c_int32, [P, C.POINTER(AccelStats)]),
+        "pt_set_camera": (C.c_int32, [P, C.POINTER(Camera)]),
+        "pt_render": (C.c_int32, [P, C.POINTER(RenderParams), C.c_uint32, C.c_uint32,
+                                  C.POINTER(RenderStats)]),
+        "pt_sync": (C.c_int32, [P]),
+        "pt_read_accum": (C.c_int32, [P, P, C.POINTER(C.c_uint32)]),
+        "pt_read_display": (C.c_int32, [P, P]),
+        "pt_read_paths": (C.c_int32, [P, P, P, P]),
+        "pt_trace_closest": (C.c_int32, [P, P, C.c_uint64, P]),
+        "pt_trace_closest_device": (C.c_int32, [P, P, C.c_uint64, P]),
+        "pt_trace_closest_counted": (C.c_int32, [P, P, C.c_uint64, P, P]),
+        "pt_trace_occluded": (C.c_int32, [P, P, C.c_uint64, P]),
+        "pt_read_accel": (C.c_int32, [P, P, C.POINTER(C.c_uint64), P, P, C.POINTER(C.c_uint64), P,
+                                      C.POINTER(C.c_uint32)]),
+        "pt_scene_build": (C.c_int32, [C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                       C.POINTER(C.c_void_p)]),
+        "pt_scene_get_desc": (C.POINTER(SceneDesc), [P]),
+        "pt_scene_get_camera": (C.POINTER(Camera), [P]),
+        "pt_scene_get_info": (C.POINTER(SceneInfo), [P]),
+        "pt_scene_free": (None, [P]),
+        "pt_scene_last_error": (C.c_char_p, []),
+    }
+    for name, (res, args) in sigs.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+def load():
+    """Load libpt_hip.so (built in-tree); raises if it is missing."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        _lib = _declare(C.CDLL(str(LIB_PATH)))
+    return _lib
+
+
+def ptr(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class PtError(RuntimeError):
+    pass
+
+
+class Scene:
+    """A synthetic scene built by the host scene API inside the library."""
+
+    def __init__(self, config: int, width: int = 0, height: int = 0, variant: int = 0):
+        lib = load()
+        h = C.c_void_p()
+        st = lib.pt_scene_build(config, width, height, variant, C.byref(h))
+        if st != PT_OK:
+            raise PtError(f"pt_scene_build: {lib.pt_scene_last_error().decode()}")
+        self._lib = lib
+        self.handle = h
+        self.desc = lib.pt_scene_get_desc(h).contents
+        self.camera = lib.pt_scene_get_camera(h).contents
+        self.info = lib.pt_scene_get_info(h).contents
+
+    def close(self):
+        if self.handle:
+            self._lib.pt_scene_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def render_params(self, flags=PT_FLAG_MATERIAL_SORT, frames_per_batch=0):
+        i = self.info
+        return RenderParams(i.width, i.height, i.max_depth, i.russian, i.rr_min_depth,
+                            frames_per_batch, flags, 0)
+
+
+class Context:
+    """One pt_context on a GPU."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = C.c_void_p()
+        st = self.lib.pt_create(device, C.byref(h))
+        if st != PT_OK:
+            raise PtError(f"pt_create({device}): {self.lib.pt_last_error(None).decode()}")
+        self.h = h
+
+    def _check(self, st, what):
+        if st != PT_OK:
+            raise PtError(f"{what} -> {st}: {self.lib.pt_last_error(self.h).decode()}")
+
+    def close(self):
+        if self.h:
+            self.lib.pt_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_scene(self, desc: SceneDesc):
+        self._check(self.lib.pt_set_scene(self.h, C.byref(desc)), "pt_set_scene")
+
+    def build_accel(self) -> AccelStats:
+        s = AccelStats()
+        self._check(self.lib.pt_build_accel(self.h, C.byref(s)), "pt_build_accel")
+        return s
+
+    def set_camera(self, cam: Camera):
+        self._check(self.lib.pt_set_camera(self.h, C.byref(cam)), "pt_set_camera")
+
+    def render(self, params: RenderParams, frame_begin: int, frame_count: int, stats=True):
+        s = RenderStats() if stats else None
+        self._check(self.lib.pt_render(self.h, C.byref(params), frame_begin, frame_count,
+                                       C.byref(s) if s is not None else None), "pt_render")
+        return s
+
+    def sync(self):
+        self._check(self.lib.pt_sync(self.h), "pt_sync")
+
+    def read_accum(self, w: int, h: int):
+        a = np.zeros((h, w, 3), np.float64)
+        fr = C.c_uint32()
+        self._check(self.lib.pt_read_accum(self.h, ptr(a), C.byref(fr)), "pt_read_accum")
+        return a, fr.value
+
+    def read_display(self, w: int, h: int):
+        a = np.zeros((h, w, 4), np.uint8)
+        self._check(self.lib.pt_read_display(self.h, ptr(a)), "pt_read_display")
+        return a
+
+    def read_paths(self, w: int, h: int):
+        rgb = np.zeros((w * h, 3), np.float32)
+        sig = np.zeros(w * h, np.uint32)
+        b = np.zeros(w * h, np.uint32)
+        self._check(self.lib.pt_read_paths(self.h, ptr(rgb), ptr(sig), ptr(b)), "pt_read_paths")
+        return rgb, sig, b
+
+    def trace_closest(self, rays: np.ndarray, counted=False):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        hits = np.zeros(len(rays), HIT_DTYPE)
+        if counted:
+            cnt = np.zeros((len(rays), 3), np.uint32)
+            self._check(self.lib.pt_trace_closest_counted(self.h, ptr(rays), len(rays), ptr(hits),
+                                                          ptr(cnt)), "pt_trace_closest_counted")
+            return hits, cnt
+        self._check(self.lib.pt_trace_closest(self.h, ptr(rays), len(rays), ptr(hits)),
+                    "pt_trace_closest")
+        return hits
+
+    def trace_occluded(self, rays: np.ndarray):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        occ = np.zeros(len(rays), np.uint8)
+        self._check(self.lib.pt_trace_occluded(self.h, ptr(rays), len(rays), ptr(occ)),
+                    "pt_trace_occluded")
+        return occ
+
+    def read_accel(self):
+        nn, npm, nb = C.c_uint64(), C.c_uint64(), C.c_uint32()
+        self._check(self.lib.pt_read_accel(self.h, None, C.byref(nn), None, None, C.byref(npm),
+                                           None, C.byref(nb)), "pt_read_accel")
+        nodes = np.zeros((nn.value, 32), np.uint32)
+        order = np.zeros(npm.value, np.uint32)
+        morton = np.zeros(npm.value, np.uint32)
+        info = np.zeros((nb.value, 6), np.uint32)
+        self._check(self.lib.pt_read_accel(self.h, ptr(nodes), C.byref(nn), ptr(order), ptr(morton),
+                                           C.byref(npm), ptr(info), C.byref(nb)), "pt_read_accel")
+        return nodes, order, morton, info
+
+
+def header_symbols():
+    """Function names declared in include/*.h."""
+    import re
+    names = []
+    for h in sorted((REPO / "include").glob("*.h")):
+        text = h.read_text()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*[A-Za-z_][\w\s\*]*?\b(pt_\w+)\s*\(", text, flags=re.M):
+            names.append(m.group(1))
+    return sorted(set(names))
