@@ -299,7 +299,7 @@ __global__ void k_refit(uint32_t n, const uint32_t* idx, const float4* plo, cons
                         const uint32_t* right, const uint32_t* parent, uint32_t* flags, float4* nlo, float4* nhi) {
   for (uint32_t l = blockIdx.x * BB + threadIdx.x; l < n; l += gridDim.x * BB) {
     uint32_t node = parent[n - 1 + l];
-    while (true) {
+    for (int climb = 0; climb < 4096 && node < n - 1; climb++) {
       __threadfence();
       if (atomicAdd(&flags[node], 1u) == 0) break;
       __threadfence();
@@ -479,9 +479,10 @@ uint32_t build_bvh4(BuildScratch& sb, uint32_t n, int leaf_max, uint4* nodes_out
                        root_lo_hi);
     return 1;
   }
+  launch_fill_u32(sb.flags, n - 1, 0u, st);
+  launch_fill_u32(sb.parent, 2ull * n - 1, 0xFFFFFFFFu, st);
   hipLaunchKernelGGL(k_karras, dim3(grid_for(n - 1)), dim3(BB), 0, st, ci, n, sb.left, sb.right, sb.first, sb.last,
                      sb.parent);
-  launch_fill_u32(sb.flags, n - 1, 0u, st);
   hipLaunchKernelGGL(k_refit, dim3(g), dim3(BB), 0, st, n, ii, sb.lo, sb.hi, sb.left, sb.right, sb.parent, sb.flags,
                      sb.nlo, sb.nhi);
   hipLaunchKernelGGL(k_copy_root, dim3(1), dim3(64), 0, st, sb.nlo, sb.nhi, root_lo_hi);
@@ -494,7 +495,9 @@ uint32_t build_bvh4(BuildScratch& sb, uint32_t n, int leaf_max, uint4* nodes_out
   uint32_t* fb = sb.frontier_b;
   launch_fill_u32(fa, 1, 0u, st);
   uint32_t F = 1, level_base = 0, next_base = 1;
+  int levels = 0;
   while (F > 0) {
+    if (++levels > 128 || next_base > n) return 0;  // malformed hierarchy
     const int gf = grid_for(F);
     hipLaunchKernelGGL(k_collapse_expand, dim3(gf), dim3(BB), 0, st, t, fa, F, leaf_max, sb.slots, sb.nchild);
     // total = last exclusive prefix + last count

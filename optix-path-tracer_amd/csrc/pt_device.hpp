@@ -3,15 +3,16 @@
 //
 // Every function cites the reference program it replaces.  Floating-point
 // expressions keep the reference's evaluation order and the library is
-// built with -ffp-contract=off, so the results are the same IEEE fp32 values
-// a scalar CPU build of the reference logic produces (transcendentals
-// excepted: ocml vs. the host libm).
+// built with -ffp-contract=off; transcendentals come from pt_libm.hpp.  The
+// results are the same IEEE fp32 values a scalar CPU evaluation of the same
+// expressions produces.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "pt_layout.hpp"
+#include "pt_libm.hpp"
 
 namespace pt {
 
@@ -77,7 +78,7 @@ PTD float rnd(uint32_t& prev) {
 // programs/sampling.cuh:22-29
 PTD v3 random_in_unit_disk(uint32_t& seed) {
   float a = rnd(seed) * 2.f * PI_F;
-  v3 xy = mk3(sinf(a), cosf(a), 0.f);
+  v3 xy = mk3(psin(a), pcos(a), 0.f);
   return xy * sqrtf(rnd(seed));
 }
 // programs/sampling.cuh:31-44
@@ -85,16 +86,16 @@ PTD v3 random_in_unit_sphere(uint32_t& seed) {
   float z = rnd(seed) * 2.f - 1.f;
   float t = rnd(seed) * 2.f * PI_F;
   float r = sqrtf((0.f > (1.f - z * z) ? 0.f : (1.f - z * z)));
-  v3 res = mk3(r * cosf(t), r * sinf(t), z);
-  return res * powf(rnd(seed), 1.f / 3.f);
+  v3 res = mk3(r * pcos(t), r * psin(t), z);
+  return res * ppow(rnd(seed), 1.f / 3.f);
 }
 // programs/sampling.cuh:46-59
 PTD v3 random_on_unit_sphere(uint32_t& seed) {
   float z = rnd(seed) * 2.f - 1.f;
   float t = rnd(seed) * 2.f * PI_F;
   float r = sqrtf((0.f > (1.f - z * z) ? 0.f : (1.f - z * z)));
-  v3 res = mk3(r * cosf(t), r * sinf(t), z);
-  res = res * powf(rnd(seed), 1.f / 3.f);
+  v3 res = mk3(r * pcos(t), r * psin(t), z);
+  res = res * ppow(rnd(seed), 1.f / 3.f);
   return unit_vector(res);
 }
 // Malley's method: uniform disk point (r = sqrt(u1), phi = 2 pi u2) lifted to
@@ -102,8 +103,8 @@ PTD v3 random_on_unit_sphere(uint32_t& seed) {
 PTD v3 cosine_sample_hemisphere(float u1, float u2) {
   float r = sqrtf(u1);
   float phi = 2.0f * PI_F * u2;
-  float x = r * cosf(phi);
-  float y = r * sinf(phi);
+  float x = r * pcos(phi);
+  float y = r * psin(phi);
   float z = sqrtf(fmaxf(0.0f, 1.0f - x * x - y * y));
   return mk3(x, y, z);
 }
@@ -238,10 +239,10 @@ PTD bool tri_test(const TriRay& r, v3 o, v3 v0, v3 v1, v3 v2, float& t, float& b
 PTD float schlick(float cosine, float ref_idx) {
   float r0 = (1.f - ref_idx) / (1.f + ref_idx);
   r0 = r0 * r0;
-  return r0 + (1.f - r0) * powf((1.f - cosine), 5.f);
+  return r0 + (1.f - r0) * ppow5((1.f - cosine));
 }
 PTD v3 schlick3(v3 r0, float cosine) {
-  float e = powf(1.f - cosine, 5.f);
+  float e = ppow5(1.f - cosine);
   return r0 + (mk3(1.f) - r0) * e;
 }
 // material.cuh:50-63
@@ -257,9 +258,9 @@ PTD bool Refract(v3 v, v3 n, float ni_over_nt, v3& refracted) {
 }
 // microfacets.cuh:11-20
 PTD void Sample_Quadrant(float nu, float nv, float rx, float ry, float& phi, float& theta) {
-  phi = atanf(sqrtf((nu + 1.f) / (nv + 1.f)) * tanf(2.f * PI_F * rx));
-  float cp = cosf(phi), sp = sinf(phi);
-  theta = powf(ry, 1.f / (nu * cp * cp + nv * sp * sp + 1.f));
+  phi = patan(sqrtf((nu + 1.f) / (nv + 1.f)) * ptan(2.f * PI_F * rx));
+  float cp = pcos(phi), sp = psin(phi);
+  theta = ppow(ry, 1.f / (nu * cp * cp + nv * sp * sp + 1.f));
 }
 // microfacets.cuh:85-99
 PTD float GGX_D(v3 H, float nu, float nv) {
@@ -283,8 +284,8 @@ PTD v3 GGX_Sample(v3 origin, float rx, float ry, float nu, float nv) {
   if (cosTheta > 0.9999f) {
     float r = sqrtf(rx / (1 - rx));
     float phi = (float)(6.28318530718 * (double)ry);
-    slopeX = r * cosf(phi);
-    slopeY = r * sinf(phi);
+    slopeX = r * pcos(phi);
+    slopeY = r * psin(phi);
   } else {
     float sinTheta = sqrtf(fmaxf(0.f, 1.f - cosTheta * cosTheta));
     float tanTheta = sinTheta / cosTheta;
@@ -358,7 +359,7 @@ PTD v3 brdf_sample(const Surface& f, v3 Wo, v3 N, uint32_t& seed) {
     float phi, cos_theta;
     if (nu == nv) {
       phi = 2.f * PI_F * rx;
-      cos_theta = powf(ry, 1.f / (nu + 1.f));
+      cos_theta = ppow(ry, 1.f / (nu + 1.f));
     } else {
       if (rx < 0.25f) { Sample_Quadrant(nu, nv, 4.f * rx, ry, phi, cos_theta); }
       else if (rx < 0.5f) { Sample_Quadrant(nu, nv, 4.f * (0.5f - rx), ry, phi, cos_theta); phi = PI_F - phi; }
@@ -366,7 +367,7 @@ PTD v3 brdf_sample(const Surface& f, v3 Wo, v3 N, uint32_t& seed) {
       else { Sample_Quadrant(nu, nv, 4.f * (1.f - rx), ry, phi, cos_theta); phi = 2.f * PI_F - phi; }
     }
     float sin_theta = sqrtf(fmaxf(0.f, 1.f - cos_theta * cos_theta));
-    H = normalize(mk3(sin_theta * cosf(phi), cos_theta, sin_theta * sinf(phi)));
+    H = normalize(mk3(sin_theta * pcos(phi), cos_theta, sin_theta * psin(phi)));
   } else {
     H = normalize(GGX_Sample(Wo, rx, ry, nu, nv));
   }
@@ -415,8 +416,8 @@ PTD v3 brdf_eval(const Surface& f, v3 Wo, v3 Wi, v3 N, float& pdf) {
       float NdotI = fabsf(dot(Up, Wi)), NdotO = fabsf(dot(Up, Wo));
       v3 diff = Rd * (28.f / (23.f * PI_F));
       diff = diff * (mk3(1.f) - Rs);
-      diff = diff * (1.f - powf(1.f - NdotI * 0.5f, 5.f));
-      diff = diff * (1.f - powf(1.f - NdotO * 0.5f, 5.f));
+      diff = diff * (1.f - ppow5(1.f - NdotI * 0.5f));
+      diff = diff * (1.f - ppow5(1.f - NdotO * 0.5f));
       float dpdf = dot(Wi, Nn) / PI_F;
       if (dpdf < 0.f) return mk3(0.f);
       v3 H = Wo + Wi;
@@ -427,11 +428,11 @@ PTD v3 brdf_eval(const Surface& f, v3 Wo, v3 Wi, v3 N, float& pdf) {
       float norm, lobe;
       if (nu == nv) {
         norm = (nu + 1.f) / (8.f * PI_F);
-        lobe = powf(HdotN, nu);
+        lobe = ppow(HdotN, nu);
       } else {
         norm = sqrtf((nu + 1.f) * (nv + 1.f)) / (8.f * PI_F);
         if (HdotN < 1.f)
-          lobe = powf(HdotN, (nu * HdotT * HdotT + nv * HdotB * HdotB) / (1.f - HdotN * HdotN));
+          lobe = ppow(HdotN, (nu * HdotT * HdotT + nv * HdotB * HdotB) / (1.f - HdotN * HdotN));
         else
           lobe = 1.f;
       }

@@ -37,8 +37,8 @@ PTD HitRec hit_record(const DevScene& sc, int inst_pos, int prim, float t, v3 o,
     hp = ro + t * rd;
     n = (hp - mk3(s)) / s.w;
     if (need_uv_hint) {
-      float phi = atan2f(n.z, n.x);
-      float theta = asinf(n.y);
+      float phi = patan2(n.z, n.x);
+      float theta = pasin(n.y);
       r.u = 1.f - (phi + PI_F) / (2.f * PI_F);
       r.v = (theta + PI_F / 2.f) / PI_F;
     }
@@ -141,23 +141,23 @@ PTD v3 miss_color(const DevScene& sc, v3 d) {
   if (m.type == PT_MISS_CONSTANT) return mk3(m.color1[0], m.color1[1], m.color1[2]);
   if (m.type == PT_MISS_IMAGE) {
     const float PI = 3.14159265358979323846f, INV_PI = 0.318309886183790671538f;
-    float theta = atan2f(d.x, d.z);
-    float phi = PI * 0.5f - acosf(d.y);
+    float theta = patan2(d.x, d.z);
+    float phi = PI * 0.5f - pacos(d.y);
     float u = (theta + PI) * (0.5f * INV_PI);
-    float v = 0.5f * (1.f + sinf(phi));
+    float v = 0.5f * (1.f + psin(phi));
     return tex_eval(sc, m.texture, u, v);
   }
   float u, v;
   if (m.spherical) {
     float r = length(d);
-    float lon = atan2f(d.z, d.x);
-    float lat = acosf(d.y / r);
+    float lon = patan2(d.z, d.x);
+    float lat = pacos(d.y / r);
     u = lon * (1.f / (PI_F * 2.f));
     v = lat * (1.f / PI_F);
   } else {
-    float theta = atan2f(d.x, d.z);
+    float theta = patan2(d.x, d.z);
     theta = theta < 0.f ? theta + (2.f * PI_F) : theta;
-    float phi = acosf(d.y);
+    float phi = pacos(d.y);
     u = 1.f - (theta / (2.f * PI_F));
     v = phi / PI_F;
   }
@@ -180,8 +180,8 @@ PTD v3 light_sample(const DevLight& l, v3 P, uint32_t& seed) {
   float dsq = squared_length(c - P);
   float z = 1.f + r2 * (sqrtf(1.f - l.r * l.r / dsq) - 1.f);
   float phi = 2.f * PI_F * r1;
-  float x = cosf(phi) * sqrtf(1.f - z * z);
-  float y = sinf(phi) * sqrtf(1.f - z * z);
+  float x = pcos(phi) * sqrtf(1.f - z * z);
+  float y = psin(phi) * sqrtf(1.f - z * z);
   v3 Wi = mk3(x, y, z);
   return onb_to_world(normalize(Wi), Wi);
 }

@@ -82,7 +82,9 @@ PTD HitOut trace_closest(const DevScene& sc, v3 o, v3 d, float tmin, float tmax,
   st.sp = 0;
   st.put(0, 0u, tmin);
   st.sp = 1;
-  while (true) {
+  // bound on loop trips: a well-formed structure never comes close
+  uint32_t trips = 0;
+  while (++trips < (1u << 22)) {
     if (cur >= 0 && st.sp == base_sp) {
       cur = -1;
       node_base = sc.tlas_root;
@@ -92,7 +94,7 @@ PTD HitOut trace_closest(const DevScene& sc, v3 o, v3 d, float tmin, float tmax,
     uint32_t ref;
     float etn;
     st.pop(ref, etn);
-    if (etn > h.t) continue;
+    if (ref == EMPTY_CHILD || etn > h.t) continue;
     if (ref & LEAF_BIT) {
       uint32_t first = ref & 0x0FFFFFFFu;
       uint32_t count = ((ref >> 28) & 7u) + 1u;

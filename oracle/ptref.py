@@ -54,6 +54,7 @@ def load():
             "ptref_light_pdf": (C.c_float, [P, C.c_int32, P, P]),
             "ptref_miss": (None, [P, P, P]),
             "ptref_hit_record": (None, [P, C.c_int32, C.c_int32, C.c_float, P, P, P, P]),
+            "ptref_libm": (None, [C.c_int32, P, P, P, C.c_uint64]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(lib, name)
@@ -185,6 +186,17 @@ def brdf_eval(mtype, c0, c1, p0, p1, wo, wi, n):
     load().ptref_brdf_eval(mtype, p(f3(c0)), p(f3(c1)), p0, p1, p(f3(wo)), p(f3(wi)), p(f3(n)), p(f),
                            p(pdf))
     return f, float(pdf[0])
+
+
+LIBM = {"sin": 0, "cos": 1, "tan": 2, "atan": 3, "atan2": 4, "asin": 5, "acos": 6, "pow": 7, "pow5": 8}
+
+
+def libm(fn, x, y=None):
+    x = np.ascontiguousarray(x, np.float32)
+    yy = None if y is None else np.ascontiguousarray(np.broadcast_to(y, x.shape), np.float32)
+    out = np.zeros_like(x)
+    load().ptref_libm(LIBM[fn], p(x), p(yy), p(out), x.size)
+    return out
 
 
 def cosine_sample(u1, u2, n):

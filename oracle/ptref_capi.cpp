@@ -252,4 +252,23 @@ void ptref_hit_record(const void* sp, int32_t inst, int32_t prim, float t, const
   *mat = r.mat;
 }
 
+// elementary functions of ptref_libm.hpp on arrays: fn 0 sin, 1 cos, 2 tan,
+// 3 atan, 4 atan2(x, y), 5 asin, 6 acos, 7 pow(x, y), 8 pow5
+void ptref_libm(int32_t fn, const float* x, const float* y, float* out, uint64_t n) {
+  for (uint64_t i = 0; i < n; i++) {
+    float a = x[i], b = y ? y[i] : 0.f;
+    switch (fn) {
+      case 0: out[i] = psin(a); break;
+      case 1: out[i] = pcos(a); break;
+      case 2: out[i] = ptan(a); break;
+      case 3: out[i] = patan(a); break;
+      case 4: out[i] = patan2(a, b); break;
+      case 5: out[i] = pasin(a); break;
+      case 6: out[i] = pacos(a); break;
+      case 7: out[i] = ppow(a, b); break;
+      default: out[i] = ppow5(a); break;
+    }
+  }
+}
+
 }  // extern "C"

@@ -12,6 +12,8 @@
 #include <cmath>
 #include <cstdint>
 
+#include "ptref_libm.hpp"
+
 namespace ptref {
 
 constexpr float PI_F = 3.141592654f;  // programs/common.hpp:18
@@ -93,7 +95,7 @@ inline float rnd(uint32_t& prev) { return (float)lcg(prev) / (float)0x01000000; 
 // programs/sampling.cuh:22-29
 inline f3 random_in_unit_disk(uint32_t& seed) {
   float a = rnd(seed) * 2.f * PI_F;
-  f3 xy = mk(std::sin(a), std::cos(a), 0.f);
+  f3 xy = mk(psin(a), pcos(a), 0.f);
   xy *= std::sqrt(rnd(seed));
   return xy;
 }
@@ -102,10 +104,10 @@ inline f3 random_in_unit_sphere(uint32_t& seed) {
   float z = rnd(seed) * 2.f - 1.f;
   float t = rnd(seed) * 2.f * PI_F;
   float r = std::sqrt((0.f > (1.f - z * z) ? 0.f : (1.f - z * z)));
-  float x = r * std::cos(t);
-  float y = r * std::sin(t);
+  float x = r * pcos(t);
+  float y = r * psin(t);
   f3 res = mk(x, y, z);
-  res *= std::pow(rnd(seed), 1.f / 3.f);
+  res *= ppow(rnd(seed), 1.f / 3.f);
   return res;
 }
 // programs/sampling.cuh:46-59
@@ -113,10 +115,10 @@ inline f3 random_on_unit_sphere(uint32_t& seed) {
   float z = rnd(seed) * 2.f - 1.f;
   float t = rnd(seed) * 2.f * PI_F;
   float r = std::sqrt((0.f > (1.f - z * z) ? 0.f : (1.f - z * z)));
-  float x = r * std::cos(t);
-  float y = r * std::sin(t);
+  float x = r * pcos(t);
+  float y = r * psin(t);
   f3 res = mk(x, y, z);
-  res *= std::pow(rnd(seed), 1.f / 3.f);
+  res *= ppow(rnd(seed), 1.f / 3.f);
   return unit_vector(res);
 }
 
@@ -128,8 +130,8 @@ inline f3 random_on_unit_sphere(uint32_t& seed) {
 inline f3 cosine_sample_hemisphere(float u1, float u2) {
   float r = std::sqrt(u1);
   float phi = 2.0f * PI_F * u2;
-  float x = r * std::cos(phi);
-  float y = r * std::sin(phi);
+  float x = r * pcos(phi);
+  float y = r * psin(phi);
   float z = std::sqrt(std::fmax(0.0f, 1.0f - x * x - y * y));
   return mk(x, y, z);
 }
@@ -170,7 +172,7 @@ inline float Cos2Phi(f3 w) { float c = CosPhi(w); return c * c; }
 inline float Sin2Phi(f3 w) { float s = SinPhi(w); return s * s; }
 // trigonometric.cuh:124-126
 inline f3 Spherical_Vector(float sintheta, float costheta, float phi) {
-  return mk(sintheta * std::cos(phi), costheta, sintheta * std::sin(phi));
+  return mk(sintheta * pcos(phi), costheta, sintheta * psin(phi));
 }
 
 }  // namespace ptref

@@ -42,8 +42,8 @@ inline HitRec hit_record(const Scene& s, int inst, int prim, float t, f3 o, f3 d
       const Sphere& sp = s.spheres[prim];
       hp = ro + t * rd;
       n = (hp - sp.c) / sp.r;
-      float phi = std::atan2(n.z, n.x);
-      float theta = std::asin(n.y);
+      float phi = patan2(n.z, n.x);
+      float theta = pasin(n.y);
       r.u = 1.f - (phi + PI_F) / (2.f * PI_F);
       r.v = (theta + PI_F / 2.f) / PI_F;
       break;
@@ -143,10 +143,10 @@ inline Surface make_surface(const Scene& s, const HitRec& r) {
 inline float schlick(float cosine, float ref_idx) {
   float r0 = (1.f - ref_idx) / (1.f + ref_idx);
   r0 = r0 * r0;
-  return r0 + (1.f - r0) * std::pow((1.f - cosine), 5.f);
+  return r0 + (1.f - r0) * ppow5((1.f - cosine));
 }
 inline f3 schlick3(f3 r0, float cosine) {
-  float e = std::pow(1.f - cosine, 5.f);
+  float e = ppow5(1.f - cosine);
   return r0 + (mk(1.f) - r0) * e;
 }
 // material.cuh:50-63
@@ -162,9 +162,9 @@ inline bool Refract(f3 v, f3 n, float ni_over_nt, f3& refracted) {
 }
 // microfacets.cuh:11-20
 inline void Sample_Quadrant(float nu, float nv, float rx, float ry, float& phi, float& theta) {
-  phi = std::atan(std::sqrt((nu + 1.f) / (nv + 1.f)) * std::tan(2.f * PI_F * rx));
-  float cp = std::cos(phi), sp = std::sin(phi);
-  theta = std::pow(ry, 1.f / (nu * cp * cp + nv * sp * sp + 1.f));
+  phi = patan(std::sqrt((nu + 1.f) / (nv + 1.f)) * ptan(2.f * PI_F * rx));
+  float cp = pcos(phi), sp = psin(phi);
+  theta = ppow(ry, 1.f / (nu * cp * cp + nv * sp * sp + 1.f));
 }
 // microfacets.cuh:85-99
 inline float GGX_D(f3 H, float nu, float nv) {
@@ -188,8 +188,8 @@ inline f3 GGX_Sample(f3 origin, float rx, float ry, float nu, float nv) {
   if (cosTheta > 0.9999f) {
     float r = std::sqrt(rx / (1 - rx));
     float phi = (float)(6.28318530718 * (double)ry);
-    slopeX = r * std::cos(phi);
-    slopeY = r * std::sin(phi);
+    slopeX = r * pcos(phi);
+    slopeY = r * psin(phi);
   } else {
     float sinTheta = std::sqrt(std::fmax(0.f, 1.f - cosTheta * cosTheta));
     float tanTheta = sinTheta / cosTheta;
@@ -259,7 +259,7 @@ inline f3 brdf_sample(const Surface& f, f3 Wo, f3 N, uint32_t& seed) {
       float phi, cos_theta;
       if (nu == nv) {
         phi = 2.f * PI_F * rx;
-        cos_theta = std::pow(ry, 1.f / (nu + 1.f));
+        cos_theta = ppow(ry, 1.f / (nu + 1.f));
       } else {
         if (rx < 0.25f) { Sample_Quadrant(nu, nv, 4.f * rx, ry, phi, cos_theta); }
         else if (rx < 0.5f) { Sample_Quadrant(nu, nv, 4.f * (0.5f - rx), ry, phi, cos_theta); phi = PI_F - phi; }
@@ -327,8 +327,8 @@ inline f3 brdf_eval(const Surface& f, f3 Wo, f3 Wi, f3 N, float& pdf) {
       float NdotI = std::fabs(dot(Up, Wi)), NdotO = std::fabs(dot(Up, Wo));
       f3 diff = Rd * (28.f / (23.f * PI_F));
       diff *= (mk(1.f) - Rs);
-      diff *= (1.f - std::pow(1.f - NdotI * 0.5f, 5.f));
-      diff *= (1.f - std::pow(1.f - NdotO * 0.5f, 5.f));
+      diff *= (1.f - ppow5(1.f - NdotI * 0.5f));
+      diff *= (1.f - ppow5(1.f - NdotO * 0.5f));
       float dpdf = dot(Wi, Nn) / PI_F;
       if (dpdf < 0.f) return mk(0.f);
       f3 H = Wo + Wi;
@@ -339,11 +339,11 @@ inline f3 brdf_eval(const Surface& f, f3 Wo, f3 Wi, f3 N, float& pdf) {
       float norm, lobe;
       if (nu == nv) {
         norm = (nu + 1.f) / (8.f * PI_F);
-        lobe = std::pow(HdotN, nu);
+        lobe = ppow(HdotN, nu);
       } else {
         norm = std::sqrt((nu + 1.f) * (nv + 1.f)) / (8.f * PI_F);
         if (HdotN < 1.f)
-          lobe = std::pow(HdotN, (nu * HdotT * HdotT + nv * HdotB * HdotB) / (1.f - HdotN * HdotN));
+          lobe = ppow(HdotN, (nu * HdotT * HdotT + nv * HdotB * HdotB) / (1.f - HdotN * HdotN));
         else
           lobe = 1.f;
       }
@@ -389,8 +389,8 @@ inline f3 light_sample(const Light& l, f3 P, uint32_t& seed) {
   float dsq = squared_length(l.c - P);
   float z = 1.f + r2 * (std::sqrt(1.f - l.r * l.r / dsq) - 1.f);
   float phi = 2.f * PI_F * r1;
-  float x = std::cos(phi) * std::sqrt(1.f - z * z);
-  float y = std::sin(phi) * std::sqrt(1.f - z * z);
+  float x = pcos(phi) * std::sqrt(1.f - z * z);
+  float y = psin(phi) * std::sqrt(1.f - z * z);
   f3 Wi = mk(x, y, z);
   return Onb(normalize(Wi)).to_world(Wi);
 }
@@ -495,24 +495,24 @@ inline f3 miss_color(const Scene& s, f3 d) {
       return mk(m.color1[0], m.color1[1], m.color1[2]);
     case PT_MISS_IMAGE: {
       const float PI = 3.14159265358979323846f, INV_PI = 0.318309886183790671538f;
-      float theta = std::atan2(d.x, d.z);
-      float phi = PI * 0.5f - std::acos(d.y);
+      float theta = patan2(d.x, d.z);
+      float phi = PI * 0.5f - pacos(d.y);
       float u = (theta + PI) * (0.5f * INV_PI);
-      float v = 0.5f * (1.f + std::sin(phi));
+      float v = 0.5f * (1.f + psin(phi));
       return tex_eval(s, m.texture, u, v);
     }
     default: {
       float u, v;
       if (m.spherical) {
         float r = length(d);
-        float lon = std::atan2(d.z, d.x);
-        float lat = std::acos(d.y / r);
+        float lon = patan2(d.z, d.x);
+        float lat = pacos(d.y / r);
         u = lon * (1.f / (PI_F * 2.f));
         v = lat * (1.f / PI_F);
       } else {
-        float theta = std::atan2(d.x, d.z);
+        float theta = patan2(d.x, d.z);
         theta = theta < 0.f ? theta + (2.f * PI_F) : theta;
-        float phi = std::acos(d.y);
+        float phi = pacos(d.y);
         u = 1.f - (theta / (2.f * PI_F));
         v = phi / PI_F;
       }
