@@ -1,0 +1,203 @@
+"""Benchmark: Mrays/s of the per-sample render path on one MI355X.
+
+Workload (one step): the Cornell box with light + BRDF MIS, 512x512, 64
+samples per pixel, max depth 16, Russian roulette from depth 3, rendered
+through the C ABI (pt_render, frames 0..63) with the scene, acceleration
+structure and path queues resident in HBM before the timed region.
+
+  value    = (closest-hit + shadow queries) / wall time of the timed steps
+  roofline = closest-hit traversal kernel: algorithmic bytes
+             (32 B ray + 16 B hit + 128 B per BVH4 node + record bytes per
+             primitive + 64 B per instance entered, counted by a traversal-
+             counting pass over the identical deterministic rays) divided by
+             the kernel's summed duration (HIP events on the render stream)
+  cpu_baseline = the scalar CPU restatement (oracle) on the host cores, on
+             all 64 frames on 16 threads, in Mrays/s.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO / "optix-path-tracer_amd"))
+
+HBM_PEAK_GBS = 8000.0
+# stored record bytes per primitive kind (pt_layout.hpp)
+PRIM_BYTES = {0: 16, 1: 32, 2: 32, 3: 48}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def copy_calibration(mib: int):
+    """A plain device-to-device copy of known size (reads and writes `mib`
+    MiB once), for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"[bench] copied {mib} MiB three times")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--spp", type=int, default=0, help="override samples per pixel (0: workload's 64)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-frames", type=int, default=64)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (counter calibration) and exit")
+    args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(samples would shard by frame with one reduction of the FP64 accumulation)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import ptamd
+
+    scene = ptamd.Scene(ptamd.PT_SCENE_CORNELL)
+    info = scene.info
+    W, H = info.width, info.height
+    spp = args.spp or info.spp
+    params = scene.render_params(flags=ptamd.PT_FLAG_MATERIAL_SORT)
+    ctx = ptamd.Context(0)
+    ctx.set_scene(scene.desc)
+    acc = ctx.build_accel()
+    ctx.set_camera(scene.camera)
+    log(f"[bench] accel: {acc.num_nodes} nodes, {acc.num_instances} instances, build {acc.build_ms:.2f} ms")
+
+    # traversal-counting pass (identical rays: the render is deterministic)
+    cparams = scene.render_params(flags=ptamd.PT_FLAG_MATERIAL_SORT | ptamd.PT_FLAG_COUNT_TRAVERSAL)
+    cst = ctx.render(cparams, 0, spp)
+    log(f"[bench] counted pass: {cst.extension_rays} closest-hit, {cst.shadow_rays} shadow rays")
+
+    for _ in range(args.warmup):
+        ctx.render(params, 0, spp)
+    ctx.sync()
+
+    steps = []
+    ctx.sync()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        steps.append(ctx.render(params, 0, spp))
+        log(f"[bench] step {k}: {steps[-1].gpu_ms:.2f} ms")
+    ctx.sync()
+    wall = time.perf_counter() - t0
+
+    rays = sum(s.extension_rays + s.shadow_rays for s in steps)
+    ext_rays = sum(s.extension_rays for s in steps)
+    mrays = rays / wall / 1e6
+    ms_per_step = wall * 1e3 / args.steps
+    samples_per_s = args.steps * W * H * spp / wall
+    # the counted pass saw exactly the rays of one step
+    assert all(s.extension_rays == cst.extension_rays and s.shadow_rays == cst.shadow_rays for s in steps)
+
+    # roofline of the closest-hit traversal kernel
+    prim_b = PRIM_BYTES[1]  # rects and boxes (32 B records)
+    alg_bytes_step = (cst.extension_rays * (32 + 16) + cst.node_fetches * 128 + cst.prim_fetches * prim_b +
+                      cst.instance_fetches * 64)
+    ext_ms = sum(s.extend_ms for s in steps) / len(steps)
+    launches = steps[0].extend_launches
+    achieved = alg_bytes_step / (ext_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = REPO / "profiles" / "traffic_extend_cornell.json"
+    if tfile.exists():
+        try:
+            traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+    roofline = {
+        "kernel": "k_extend (closest-hit BVH4 traversal)",
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "algorithmic_bytes_per_launch": round(alg_bytes_step / launches),
+        "avg_launch_ms": round(ext_ms / launches, 4),
+        "per_ray": {
+            "nodes": round(cst.node_fetches / cst.extension_rays, 3),
+            "prims": round(cst.prim_fetches / cst.extension_rays, 3),
+            "instances": round(cst.instance_fetches / cst.extension_rays, 3),
+        },
+    }
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            sys.path.insert(0, str(REPO / "oracle"))
+            import ptref
+
+            orc = ptref.OracleScene(scene.desc)
+            nthreads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+            fr = args.cpu_frames
+            t1 = time.perf_counter()
+            _, _, st = orc.render(scene.camera, params, 0, fr, threads=nthreads)
+            dt = time.perf_counter() - t1
+            cpu = {
+                "value": round((int(st[1]) + int(st[2])) / dt / 1e6, 3),
+                "unit": "Mrays/s",
+                "cores": nthreads,
+                "kind": "port",
+                "sample": f"Cornell 512x512, frames 0-{fr - 1} of 64 ({W * H * fr} paths, "
+                          f"{int(st[1]) + int(st[2])} rays, {dt:.1f} s)",
+            }
+        except Exception as e:  # the baseline is reported, never the target
+            log(f"[bench] cpu baseline failed: {e}")
+
+    ctx.close()
+    out = {
+        "metric": "Mrays/s",
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"cornell_box_mis_{W}x{H}_{spp}spp_depth{info.max_depth}_rr{info.rr_min_depth}",
+            "width": W, "height": H, "spp": spp, "max_depth": info.max_depth,
+            "rr_min_depth": info.rr_min_depth, "material_sort": True,
+        },
+        "samples_per_s": round(samples_per_s),
+        "rays_per_step": {"closest_hit": cst.extension_rays, "shadow": cst.shadow_rays},
+        "kernel_ms_per_step": {
+            "extend": round(ext_ms, 3),
+            "occlusion": round(sum(s.shadow_ms for s in steps) / len(steps), 3),
+            "render_events": round(sum(s.gpu_ms for s in steps) / len(steps), 3),
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

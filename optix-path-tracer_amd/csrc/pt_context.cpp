@@ -24,7 +24,7 @@ using namespace pt;
 namespace {
 
 constexpr int MAX_GRID = 2048;         // blocks of the grid-stride kernels
-constexpr uint32_t PATHS_TARGET = 1u << 21;  // paths in flight per batch (automatic)
+constexpr uint32_t PATHS_TARGET = 1u << 22;  // paths in flight per batch (automatic)
 
 struct DevBuf {
   void* p = nullptr;
@@ -87,6 +87,7 @@ struct pt_context {
   uint32_t last_fpb = 0;        // frames of the last batch
   uint32_t last_frame_slot = 0; // slot base of the last frame in its batch
   bool paths_valid = false;
+  int shading_kinds = 0;  // distinct non-emitter material types of the scene
 
   // ---- path queues
   uint64_t path_capacity = 0;
@@ -322,6 +323,15 @@ pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
   }
   c->lights.assign(d->lights, d->lights + d->num_lights);
   c->miss = d->miss;
+  {
+    bool seen[PT_MAT_COUNT] = {};
+    c->shading_kinds = 0;
+    for (auto& m : c->materials)
+      if (m.type != PT_MAT_DIFFUSE_LIGHT && !seen[m.type]) {
+        seen[m.type] = true;
+        c->shading_kinds++;
+      }
+  }
 
   // ---- original-order device data
   DevScene& s = c->dsc;
@@ -695,7 +705,7 @@ static pt_status ensure_paths(pt_context* c, uint64_t cap) {
   auto U32 = [&](uint32_t** p) { return dalloc(B, (void**)p, cap * 4); };
   if ((e = F4(&P.ray_o)) || (e = F4(&P.ray_d)) || (e = F4(&P.thr)) || (e = F4(&P.rad)) || (e = U32(&P.seed)) ||
       (e = U32(&P.sig)) || (e = F4(&P.hit)) || (e = F4(&P.nee_p)) || (e = F4(&P.nee_n)) || (e = F4(&P.nee_wo)) ||
-      (e = F4(&P.nee_wi)) || (e = F4(&P.nee_c)) || (e = dalloc(B, (void**)&P.occl, cap)) || (e = F4(&P.result)) ||
+      (e = F4(&P.nee_wi)) || (e = F4(&P.nee_c)) || (e = F4(&P.result)) ||
       (e = U32(&P.list_a)) || (e = U32(&P.list_b)) || (e = U32(&P.list_shadow)) || (e = U32(&P.list_sorted)) ||
       (e = U32(&P.keys)) || (e = dalloc(B, (void**)&P.counters, CTR_WORDS * 4))) {
     dfree(B);
@@ -763,7 +773,8 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
 
   const bool want_stats = out != nullptr;
   const bool count_trav = (rp->flags & PT_FLAG_COUNT_TRAVERSAL) != 0;
-  const bool sort = (rp->flags & PT_FLAG_MATERIAL_SORT) != 0;
+  // sorting pays only when more than one shading program can meet in a wave
+  const bool sort = (rp->flags & PT_FLAG_MATERIAL_SORT) != 0 && c->shading_kinds > 1;
   std::vector<hipEvent_t> ev;
   auto mark = [&]() -> hipEvent_t {
     hipEvent_t e;
@@ -797,10 +808,9 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
       }
       launch_shade(c->dsc, P, shade_list, P.counters + cin, lout, P.counters + cout, depth, rc, grid, c->stream);
       hipEvent_t a2 = want_stats ? mark() : nullptr;
-      launch_occlusion(c->dsc, P, c->spill, c->stats, grid, c->stream);
+      launch_shadow_connect(c->dsc, P, lout, P.counters + cout, depth, rc, c->spill, c->stats, grid, c->stream);
       hipEvent_t b2 = want_stats ? mark() : nullptr;
       if (want_stats) sh_ev.push_back({a2, b2});
-      launch_connect(c->dsc, P, lout, P.counters + cout, depth, rc, grid, c->stream);
       launch_bounce_reset(P, cin, c->stream);
       std::swap(lin, lout);
       std::swap(cin, cout);

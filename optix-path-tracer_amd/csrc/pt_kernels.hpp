@@ -33,10 +33,9 @@ void launch_sort(const DevScene& sc, const DevPaths& P, const uint32_t* list, co
 void launch_shade(const DevScene& sc, const DevPaths& P, const uint32_t* list, const uint32_t* count,
                   uint32_t* next_list, uint32_t* next_count, uint32_t depth, const RenderConsts& rc, int grid,
                   hipStream_t st);
-void launch_occlusion(const DevScene& sc, const DevPaths& P, uint2* spill, uint64_t* stats, int grid,
-                      hipStream_t st);
-void launch_connect(const DevScene& sc, const DevPaths& P, uint32_t* next_list, uint32_t* next_count, uint32_t depth,
-                    const RenderConsts& rc, int grid, hipStream_t st);
+void launch_shadow_connect(const DevScene& sc, const DevPaths& P, uint32_t* next_list, uint32_t* next_count,
+                           uint32_t depth, const RenderConsts& rc, uint2* spill, uint64_t* stats, int grid,
+                           hipStream_t st);
 void launch_bounce_reset(const DevPaths& P, int zero_ctr, hipStream_t st);
 void launch_accumulate(const DevPaths& P, const RenderConsts& rc, uint32_t nframes, double* accum, int grid,
                        hipStream_t st);

@@ -128,7 +128,6 @@ struct DevPaths {
   float4* nee_wo;  // (Wo.xyz, light pdf)
   float4* nee_wi;  // (Wi.xyz, flags: bit0 emitter)
   float4* nee_c;   // (surface c0.xyz, 0)
-  uint8_t* occl;
   float4* result;  // (L.xyz, bounces bits)
   uint32_t* list_a;
   uint32_t* list_b;

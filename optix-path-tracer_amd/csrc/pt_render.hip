@@ -5,7 +5,7 @@
 // programs/materials/*.cu, miss programs, shadow any-hit).
 //
 // Wavefront, one bounce at a time over SoA path state in HBM:
-//   raygen -> [extend -> (material sort) -> shade -> occlusion -> connect]*
+//   raygen -> [extend -> (material sort) -> shade -> shadow+connect]*
 //   -> accumulate
 // Queues are slot lists compacted with wave64 ballots: one atomic per wave.
 // All kernels are grid-stride loops that read their item count from device
@@ -136,32 +136,49 @@ __global__ __launch_bounds__(BLOCK) void k_extend(DevScene sc, DevPaths P, const
     P.hit[s] = make_float4(h.t, __int_as_float(h.pos), __int_as_float(h.prim), __int_as_float(h.inst));
   }
   if (COUNT) {
-    atomicAdd((unsigned long long*)&stats[STAT_NODES], (unsigned long long)c.nodes);
-    atomicAdd((unsigned long long*)&stats[STAT_PRIMS], (unsigned long long)c.prims);
-    atomicAdd((unsigned long long*)&stats[STAT_INSTS], (unsigned long long)c.insts);
+    uint32_t a = c.nodes, b = c.prims, d = c.insts;
+    for (int off = 32; off > 0; off >>= 1) {
+      a += __shfl_down(a, off);
+      b += __shfl_down(b, off);
+      d += __shfl_down(d, off);
+    }
+    if (__lane_id() == 0) {
+      atomicAdd((unsigned long long*)&stats[STAT_NODES], (unsigned long long)a);
+      atomicAdd((unsigned long long*)&stats[STAT_PRIMS], (unsigned long long)b);
+      atomicAdd((unsigned long long*)&stats[STAT_INSTS], (unsigned long long)d);
+    }
   }
 }
 
-// material bucket: 0 = miss, 1 + material type otherwise
+// Material sort: bucket 0 = miss, 1 + material type otherwise.  Counts and
+// scatter cursors are aggregated per wave (ballot) and per block (LDS), so
+// each block issues one global atomic per bucket.
+PTD uint32_t material_key(const DevScene& sc, const DevPaths& P, uint32_t s) {
+  float4 h = P.hit[s];
+  int pos = __float_as_int(h.y);
+  return pos < 0 ? 0u : 1u + (uint32_t)sc.mats[hit_material(sc, pos, __float_as_int(h.z))].type;
+}
 __global__ __launch_bounds__(BLOCK) void k_sort_count(DevScene sc, DevPaths P, const uint32_t* __restrict__ list,
                                                       const uint32_t* __restrict__ count_ptr) {
+  __shared__ uint32_t cnt[8];
   const uint32_t n = *count_ptr;
+  if (threadIdx.x < 8) cnt[threadIdx.x] = 0;
+  __syncthreads();
   for (uint32_t i0 = blockIdx.x * BLOCK; i0 < n; i0 += gridDim.x * BLOCK) {
     uint32_t i = i0 + threadIdx.x;
     uint32_t key = 0xFFu;
     if (i < n) {
-      uint32_t s = list[i];
-      float4 h = P.hit[s];
-      int pos = __float_as_int(h.y);
-      key = pos < 0 ? 0u : 1u + (uint32_t)sc.mats[hit_material(sc, pos, __float_as_int(h.z))].type;
+      key = material_key(sc, P, list[i]);
       P.keys[i] = key;
     }
 #pragma unroll
     for (uint32_t b = 0; b < 8; b++) {
       uint64_t m = __ballot(key == b);
-      if (m && lane_rank(m) == 0 && key == b) atomicAdd(&P.counters[CTR_BUCKETS + b], (uint32_t)__popcll(m));
+      if (m && key == b && lane_rank(m) == 0) atomicAdd(&cnt[b], (uint32_t)__popcll(m));
     }
   }
+  __syncthreads();
+  if (threadIdx.x < 8 && cnt[threadIdx.x]) atomicAdd(&P.counters[CTR_BUCKETS + threadIdx.x], cnt[threadIdx.x]);
 }
 __global__ void k_sort_offsets(DevPaths P) {
   uint32_t acc = 0;
@@ -172,21 +189,30 @@ __global__ void k_sort_offsets(DevPaths P) {
 }
 __global__ __launch_bounds__(BLOCK) void k_sort_scatter(DevPaths P, const uint32_t* __restrict__ list,
                                                         const uint32_t* __restrict__ count_ptr) {
+  __shared__ uint32_t cnt[8], base[8];
   const uint32_t n = *count_ptr;
   for (uint32_t i0 = blockIdx.x * BLOCK; i0 < n; i0 += gridDim.x * BLOCK) {
     uint32_t i = i0 + threadIdx.x;
     uint32_t key = i < n ? P.keys[i] : 0xFFu;
     uint32_t s = i < n ? list[i] : 0;
+    if (threadIdx.x < 8) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t wave_off = 0;
+    uint64_t mine = 0;
 #pragma unroll
     for (uint32_t b = 0; b < 8; b++) {
       uint64_t m = __ballot(key == b);
-      if (m == 0) continue;
-      int leader = __ffsll((unsigned long long)m) - 1;
-      uint32_t base = 0;
-      if ((int)__lane_id() == leader) base = atomicAdd(&P.counters[CTR_BUCKET_CURSOR + b], (uint32_t)__popcll(m));
-      base = __shfl(base, leader);
-      if (key == b) P.list_sorted[base + lane_rank(m)] = s;
+      uint32_t o = 0;
+      if (m && key == b && lane_rank(m) == 0) o = atomicAdd(&cnt[b], (uint32_t)__popcll(m));
+      if (key == b) mine = m;
+      o = __shfl(o, m ? (int)(__ffsll((unsigned long long)m) - 1) : 0);
+      if (key == b) wave_off = o;
     }
+    __syncthreads();
+    if (threadIdx.x < 8) base[threadIdx.x] = cnt[threadIdx.x] ? atomicAdd(&P.counters[CTR_BUCKET_CURSOR + threadIdx.x], cnt[threadIdx.x]) : 0u;
+    __syncthreads();
+    if (key < 8) P.list_sorted[base[key] + wave_off + lane_rank(mine)] = s;
+    __syncthreads();
   }
 }
 
@@ -358,35 +384,20 @@ __global__ __launch_bounds__(BLOCK) void k_shade(DevScene sc, DevPaths P, const 
   }
 }
 
-// Shadow rays of Direct_Light (light_sample.cuh:61-72) under the rule: lit
+// Shadow ray of Direct_Light (light_sample.cuh:61-72) under the rule: lit
 // iff the closest hit in (1e-3, inf) is a diffuse light with dot(N, Wi) > 0,
-// or nothing is hit.
-__global__ __launch_bounds__(BLOCK) void k_occlusion(DevScene sc, DevPaths P, uint2* spill, uint64_t* stats) {
+// or nothing is hit; then the second half of Direct_Light
+// (light_sample.cuh:74-102), the radiance update of the closest-hit program
+// and the continuation bounce.
+__global__ __launch_bounds__(BLOCK) void k_shadow_connect(DevScene sc, DevPaths P, uint32_t* next_list,
+                                                          uint32_t* next_count, uint32_t depth, RenderConsts rc,
+                                                          uint2* spill, uint64_t* stats) {
   __shared__ uint32_t s_ref[STK_LDS * BLOCK];
   __shared__ float s_tn[STK_LDS * BLOCK];
   Stack st = make_stack(s_ref, s_tn, spill, P.counters + CTR_STACK_OVERFLOW);
   const uint32_t n = P.counters[CTR_SHADOW];
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd((unsigned long long*)&stats[STAT_SHADOW], (unsigned long long)n);
-  TraceCounts c{0, 0, 0};
-  for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
-    uint32_t s = P.list_shadow[i];
-    float4 p = P.nee_p[s], nn = P.nee_n[s], wi = P.nee_wi[s];
-    v3 Wi = mk3(wi), N = mk3(nn);
-    HitOut h = trace_closest<false>(sc, mk3(p), Wi, 1e-3f, FINF, st, c);
-    uint8_t occ = 0;
-    if (h.inst >= 0) {
-      int m = hit_material(sc, h.pos, h.prim);
-      occ = (sc.mats[m].type == PT_MAT_DIFFUSE_LIGHT && dot(N, Wi) > 0.f) ? 0 : 1;
-    }
-    P.occl[s] = occ;
-  }
-}
-
-// Second half of Direct_Light (light_sample.cuh:74-102), radiance update of
-// the closest-hit program and the continuation bounce.
-__global__ __launch_bounds__(BLOCK) void k_connect(DevScene sc, DevPaths P, uint32_t* next_list, uint32_t* next_count,
-                                                   uint32_t depth, RenderConsts rc) {
-  const uint32_t n = P.counters[CTR_SHADOW];
+  TraceCounts tc{0, 0, 0};
   for (uint32_t i0 = blockIdx.x * BLOCK; i0 < n; i0 += gridDim.x * BLOCK) {
     uint32_t i = i0 + threadIdx.x;
     bool alive = false;
@@ -394,12 +405,16 @@ __global__ __launch_bounds__(BLOCK) void k_connect(DevScene sc, DevPaths P, uint
     if (i < n) {
       s = P.list_shadow[i];
       float4 p4 = P.nee_p[s], n4 = P.nee_n[s], wo4 = P.nee_wo[s], wi4 = P.nee_wi[s], c4 = P.nee_c[s];
+      v3 Pp = mk3(p4), N = mk3(n4), Wo = mk3(wo4), Wi = mk3(wi4);
+      HitOut sh = trace_closest<false>(sc, Pp, Wi, 1e-3f, FINF, st, tc);
+      bool occluded = false;
+      if (sh.inst >= 0)
+        occluded = !(sc.mats[hit_material(sc, sh.pos, sh.prim)].type == PT_MAT_DIFFUSE_LIGHT && dot(N, Wi) > 0.f);
       float4 t4 = P.thr[s], l4 = P.rad[s];
-      v3 Pp = mk3(p4), N = mk3(n4), Wo = mk3(wo4), Wi = mk3(wi4), T = mk3(t4), L = mk3(l4);
+      v3 T = mk3(t4), L = mk3(l4);
       uint32_t flags = __float_as_uint(t4.w);
       uint32_t seed = P.seed[s];
       uint32_t sig = P.sig[s];
-      bool occluded = P.occl[s] != 0;
       int li = __float_as_int(p4.w), mat = __float_as_int(n4.w);
       bool is_light = (__float_as_uint(wi4.w) & 1u) != 0;
       float lightPDF = wo4.w;
@@ -543,13 +558,11 @@ void launch_shade(const DevScene& sc, const DevPaths& P, const uint32_t* list, c
                   hipStream_t st) {
   hipLaunchKernelGGL(k_shade, dim3(grid), dim3(BLOCK), 0, st, sc, P, list, count, next_list, next_count, depth, rc);
 }
-void launch_occlusion(const DevScene& sc, const DevPaths& P, uint2* spill, uint64_t* stats, int grid,
-                      hipStream_t st) {
-  hipLaunchKernelGGL(k_occlusion, dim3(grid), dim3(BLOCK), 0, st, sc, P, spill, stats);
-}
-void launch_connect(const DevScene& sc, const DevPaths& P, uint32_t* next_list, uint32_t* next_count, uint32_t depth,
-                    const RenderConsts& rc, int grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_connect, dim3(grid), dim3(BLOCK), 0, st, sc, P, next_list, next_count, depth, rc);
+void launch_shadow_connect(const DevScene& sc, const DevPaths& P, uint32_t* next_list, uint32_t* next_count,
+                           uint32_t depth, const RenderConsts& rc, uint2* spill, uint64_t* stats, int grid,
+                           hipStream_t st) {
+  hipLaunchKernelGGL(k_shadow_connect, dim3(grid), dim3(BLOCK), 0, st, sc, P, next_list, next_count, depth, rc, spill,
+                     stats);
 }
 void launch_bounce_reset(const DevPaths& P, int zero_ctr, hipStream_t st) {
   hipLaunchKernelGGL(k_bounce_reset, dim3(1), dim3(64), 0, st, P, zero_ctr);
