@@ -6,11 +6,16 @@ through the C ABI (pt_render, frames 0..63) with the scene, acceleration
 structure and path queues resident in HBM before the timed region.
 
   value    = (closest-hit + shadow queries) / wall time of the timed steps
-  roofline = closest-hit traversal kernel: algorithmic bytes
-             (32 B ray + 16 B hit + 128 B per BVH4 node + record bytes per
-             primitive + 64 B per instance entered, counted by a traversal-
-             counting pass over the identical deterministic rays) divided by
-             the kernel's summed duration (HIP events on the render stream)
+  roofline = the dominant kernel, k_shade (hit/miss programs, light sample,
+             shadow-ray traversal, continuation, compaction of survivors):
+             algorithmic bytes = per shaded path 64 B state + 16 B hit read,
+             64 B survivor state or 20 B result written; per shadow ray
+             128 B per BVH4 node + record bytes per primitive + 64 B per
+             instance entered (counted by a traversal-counting pass over the
+             identical deterministic rays).  Material, light and texture
+             tables are not counted.  Divided by the kernel's summed duration
+             (HIP events on the render stream).
+  roofline_extend = the same for k_extend (32 B ray + 16 B hit + traversal).
   cpu_baseline = the scalar CPU restatement (oracle) on the host cores, on
              all 64 frames on 16 threads, in Mrays/s.
 """
@@ -51,6 +56,16 @@ def copy_calibration(mib: int):
     hip.hipFree(a)
     hip.hipFree(b)
     log(f"[bench] copied {mib} MiB three times")
+
+
+def load_traffic():
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on
+    this workload, scaled by the same counters over a plain copy."""
+    f = REPO / "profiles" / "traffic_cornell_512x512_64spp.json"
+    try:
+        return {k: v["hbm_bytes_per_launch"] for k, v in json.loads(f.read_text())["kernels"].items()}
+    except Exception:
+        return {}
 
 
 def main():
@@ -113,36 +128,47 @@ def main():
     # the counted pass saw exactly the rays of one step
     assert all(s.extension_rays == cst.extension_rays and s.shadow_rays == cst.shadow_rays for s in steps)
 
-    # roofline of the closest-hit traversal kernel
+    # Roofline of the dominant kernel (k_shade: hit and miss programs, light
+    # sampling, the shadow ray's traversal, continuation and compaction) and
+    # of the closest-hit traversal kernel (k_extend).
     prim_b = PRIM_BYTES[1]  # rects and boxes (32 B records)
-    alg_bytes_step = (cst.extension_rays * (32 + 16) + cst.node_fetches * 128 + cst.prim_fetches * prim_b +
+    ext, paths, exh = cst.extension_rays, cst.paths, cst.exhausted_paths
+    shade_bytes_step = (ext * (64 + 16) + (ext - paths + exh) * 64 + (paths - exh) * 20 +
+                        cst.shadow_node_fetches * 128 + cst.shadow_prim_fetches * prim_b +
+                        cst.shadow_instance_fetches * 64)
+    ext_bytes_step = (ext * (32 + 16) + cst.node_fetches * 128 + cst.prim_fetches * prim_b +
                       cst.instance_fetches * 64)
+    shade_ms = sum(s.shade_ms for s in steps) / len(steps)
     ext_ms = sum(s.extend_ms for s in steps) / len(steps)
-    launches = steps[0].extend_launches
-    achieved = alg_bytes_step / (ext_ms * 1e-3) / 1e9
-    traffic = None
-    tfile = REPO / "profiles" / "traffic_extend_cornell.json"
-    if tfile.exists():
-        try:
-            traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
-        except Exception:
-            traffic = None
-    roofline = {
-        "kernel": "k_extend (closest-hit BVH4 traversal)",
-        "bound": "hbm",
-        "achieved": round(achieved, 1),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": traffic,
-        "algorithmic_bytes_per_launch": round(alg_bytes_step / launches),
-        "avg_launch_ms": round(ext_ms / launches, 4),
-        "per_ray": {
-            "nodes": round(cst.node_fetches / cst.extension_rays, 3),
-            "prims": round(cst.prim_fetches / cst.extension_rays, 3),
-            "instances": round(cst.instance_fetches / cst.extension_rays, 3),
-        },
-    }
+    traffic = load_traffic()
+
+    def roof(kernel, key, bytes_step, ms, launches, extra):
+        achieved = bytes_step / (ms * 1e-3) / 1e9
+        t = traffic.get(key)
+        return dict({
+            "kernel": kernel,
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": t,
+            "algorithmic_bytes_per_launch": round(bytes_step / launches),
+            "avg_launch_ms": round(ms / launches, 4),
+        }, **extra)
+
+    roofline = roof("k_shade (shading + shadow-ray BVH4 traversal + compaction)", "k_shade", shade_bytes_step,
+                    shade_ms, steps[0].shade_launches, {
+                        "shadow_per_ray": {
+                            "nodes": round(cst.shadow_node_fetches / max(1, cst.shadow_rays), 3),
+                            "prims": round(cst.shadow_prim_fetches / max(1, cst.shadow_rays), 3),
+                            "instances": round(cst.shadow_instance_fetches / max(1, cst.shadow_rays), 3)}})
+    roofline_extend = roof("k_extend (closest-hit BVH4 traversal)", "k_extend", ext_bytes_step, ext_ms,
+                           steps[0].extend_launches, {
+                               "per_ray": {
+                                   "nodes": round(cst.node_fetches / ext, 3),
+                                   "prims": round(cst.prim_fetches / ext, 3),
+                                   "instances": round(cst.instance_fetches / ext, 3)}})
 
     cpu = None
     if not args.no_cpu:
@@ -190,10 +216,11 @@ def main():
         "rays_per_step": {"closest_hit": cst.extension_rays, "shadow": cst.shadow_rays},
         "kernel_ms_per_step": {
             "extend": round(ext_ms, 3),
-            "occlusion": round(sum(s.shadow_ms for s in steps) / len(steps), 3),
+            "shade": round(shade_ms, 3),
             "render_events": round(sum(s.gpu_ms for s in steps) / len(steps), 3),
         },
         "roofline": roofline,
+        "roofline_extend": roofline_extend,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define PT_API_VERSION 1
+#define PT_API_VERSION 2
 
 /* ------------------------------------------------------------------ status */
 typedef int32_t pt_status;
@@ -207,11 +207,16 @@ typedef struct pt_render_stats {
   uint64_t node_fetches;    /* only with PT_FLAG_COUNT_TRAVERSAL            */
   uint64_t prim_fetches;    /* only with PT_FLAG_COUNT_TRAVERSAL            */
   uint64_t instance_fetches;/* only with PT_FLAG_COUNT_TRAVERSAL            */
+  uint64_t shadow_node_fetches;     /* the same three counts for the     */
+  uint64_t shadow_prim_fetches;     /* occlusion queries (only with      */
+  uint64_t shadow_instance_fetches; /* PT_FLAG_COUNT_TRAVERSAL)          */
+  uint64_t exhausted_paths; /* paths still alive after max_depth bounces     */
   double gpu_ms;            /* render wall time on the stream (events)      */
   double extend_ms;         /* summed closest-hit kernel time               */
-  double shadow_ms;         /* summed occlusion kernel time                 */
+  double shade_ms;          /* summed shading kernel time (hit and miss
+                               programs, shadow rays, continuation)        */
   uint32_t extend_launches;
-  uint32_t shadow_launches;
+  uint32_t shade_launches;
 } pt_render_stats;
 
 typedef struct pt_accel_stats {

@@ -88,6 +88,7 @@ struct pt_context {
   uint32_t last_frame_slot = 0; // slot base of the last frame in its batch
   bool paths_valid = false;
   int shading_kinds = 0;  // distinct non-emitter material types of the scene
+  uint32_t mat_types = 0;  // bit t set when material type t occurs
 
   // ---- path queues
   uint64_t path_capacity = 0;
@@ -326,6 +327,8 @@ pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
   {
     bool seen[PT_MAT_COUNT] = {};
     c->shading_kinds = 0;
+    c->mat_types = 0;
+    for (auto& m : c->materials) c->mat_types |= 1u << m.type;
     for (auto& m : c->materials)
       if (m.type != PT_MAT_DIFFUSE_LIGHT && !seen[m.type]) {
         seen[m.type] = true;
@@ -703,13 +706,14 @@ static pt_status ensure_paths(pt_context* c, uint64_t cap) {
   hipError_t e;
   auto F4 = [&](float4** p) { return dalloc(B, (void**)p, cap * sizeof(float4)); };
   auto U32 = [&](uint32_t** p) { return dalloc(B, (void**)p, cap * 4); };
-  if ((e = F4(&P.ray_o)) || (e = F4(&P.ray_d)) || (e = F4(&P.thr)) || (e = F4(&P.rad)) || (e = U32(&P.seed)) ||
-      (e = U32(&P.sig)) || (e = F4(&P.hit)) || (e = F4(&P.nee_p)) || (e = F4(&P.nee_n)) || (e = F4(&P.nee_wo)) ||
-      (e = F4(&P.nee_wi)) || (e = F4(&P.nee_c)) || (e = F4(&P.result)) ||
-      (e = U32(&P.list_a)) || (e = U32(&P.list_b)) || (e = U32(&P.list_shadow)) || (e = U32(&P.list_sorted)) ||
-      (e = U32(&P.keys)) || (e = dalloc(B, (void**)&P.counters, CTR_WORDS * 4))) {
+  e = hipSuccess;
+  for (int k = 0; k < 3 && e == hipSuccess; k++)
+    if ((e = F4(&P.st[k].o)) == hipSuccess && (e = F4(&P.st[k].d)) == hipSuccess && (e = F4(&P.st[k].T)) == hipSuccess)
+      e = F4(&P.st[k].L);
+  if (e || (e = F4(&P.hit)) || (e = F4(&P.hit_sorted)) || (e = U32(&P.keys)) || (e = F4(&P.result)) ||
+      (e = U32(&P.sig_final)) || (e = dalloc(B, (void**)&P.counters, CTR_WORDS * 4))) {
     dfree(B);
-    return hip_fail(c, e, "hipMalloc (path queues)");
+    return hip_fail(c, e, "hipMalloc (path state)");
   }
   launch_fill_u32(P.counters, CTR_WORDS, 0u, c->stream);
   c->path_capacity = cap;
@@ -787,34 +791,29 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
   launch_fill_u32((uint32_t*)c->stats, STAT_WORDS * 2, 0u, c->stream);
   hipEvent_t t_begin = want_stats ? mark() : nullptr;
   DevPaths& P = c->paths;
+  const int matset = shade_material_set(c->mat_types);
   for (uint32_t f0 = 0; f0 < frame_count; f0 += fpb) {
     uint32_t nf = std::min(fpb, frame_count - f0);
     uint32_t nslots = nf * WH;
     rc.frame0 = frame_begin + f0;
     const int grid = grid_for(nslots);
     launch_raygen(P, rc, nslots, grid, c->stream);
-    uint32_t* lin = P.list_a;
-    uint32_t* lout = P.list_b;
-    int cin = CTR_ACTIVE_A, cout = CTR_ACTIVE_B;
+    int in = 0, out = 1;
     for (uint32_t depth = 0; depth < rp->max_depth; depth++) {
       hipEvent_t a = want_stats ? mark() : nullptr;
-      launch_extend(c->dsc, P, lin, P.counters + cin, c->spill, c->stats, count_trav, grid, c->stream);
+      launch_extend(c->dsc, P, in, depth == 0 ? 1e-6f : 1e-3f, c->spill, c->stats, count_trav, sort, grid, c->stream);
       hipEvent_t b = want_stats ? mark() : nullptr;
       if (want_stats) ext_ev.push_back({a, b});
-      const uint32_t* shade_list = lin;
-      if (sort) {
-        launch_sort(c->dsc, P, lin, P.counters + cin, grid, c->stream);
-        shade_list = P.list_sorted;
-      }
-      launch_shade(c->dsc, P, shade_list, P.counters + cin, lout, P.counters + cout, depth, rc, grid, c->stream);
+      if (sort) launch_sort(P, in, grid, c->stream);
       hipEvent_t a2 = want_stats ? mark() : nullptr;
-      launch_shadow_connect(c->dsc, P, lout, P.counters + cout, depth, rc, c->spill, c->stats, grid, c->stream);
+      launch_shade(c->dsc, P, in, sort, out, depth, rc, matset, count_trav, c->spill, c->stats, grid, c->stream);
       hipEvent_t b2 = want_stats ? mark() : nullptr;
       if (want_stats) sh_ev.push_back({a2, b2});
-      launch_bounce_reset(P, cin, c->stream);
-      std::swap(lin, lout);
-      std::swap(cin, cout);
+      launch_bounce_reset(P, in, c->stream);
+      std::swap(in, out);
     }
+    launch_exhausted(P, in, c->stats, grid, c->stream);
+    launch_bounce_reset(P, in, c->stream);
     launch_accumulate(P, rc, nf, c->accum, grid_for(WH), c->stream);
     c->last_fpb = nf;
   }
@@ -834,6 +833,10 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
     out->node_fetches = st[STAT_NODES];
     out->prim_fetches = st[STAT_PRIMS];
     out->instance_fetches = st[STAT_INSTS];
+    out->shadow_node_fetches = st[STAT_SH_NODES];
+    out->shadow_prim_fetches = st[STAT_SH_PRIMS];
+    out->shadow_instance_fetches = st[STAT_SH_INSTS];
+    out->exhausted_paths = st[STAT_EXHAUSTED];
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, t_begin, t_end);
     out->gpu_ms = ms;
@@ -845,10 +848,10 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
     for (auto& p : sh_ev) {
       float m = 0.f;
       (void)hipEventElapsedTime(&m, p.first, p.second);
-      out->shadow_ms += m;
+      out->shade_ms += m;
     }
     out->extend_launches = (uint32_t)ext_ev.size();
-    out->shadow_launches = (uint32_t)sh_ev.size();
+    out->shade_launches = (uint32_t)sh_ev.size();
     for (auto e : ev) (void)hipEventDestroy(e);
     if (ctr[CTR_STACK_OVERFLOW]) return fail(c, PT_ERR_DEVICE, "traversal stack overflow");
   }
@@ -900,7 +903,7 @@ pt_status pt_read_paths(pt_context* c, float* rgb, uint32_t* signature, uint32_t
     }
   if (bounces)
     for (uint64_t i = 0; i < WH; i++) bounces[i] = fb(res[i].w);
-  if (signature) HIPCHK(c, hipMemcpy(signature, c->paths.sig + base, WH * 4, hipMemcpyDeviceToHost));
+  if (signature) HIPCHK(c, hipMemcpy(signature, c->paths.sig_final + base, WH * 4, hipMemcpyDeviceToHost));
   return PT_OK;
 }
 

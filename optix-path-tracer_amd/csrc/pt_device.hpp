@@ -336,13 +336,26 @@ struct Surface {
 
 // Sample(): lambertian.cuh:7-19, oren_nayar.cuh:8-20, diffuse_light.cuh:7-13,
 // ashikhmin_shirley.cuh:11-82, torrance_sparrow.cuh:10-35
+// Material sets a kernel is compiled for (bit = 1 << PT_MAT_*).
+constexpr int MS_ALL = 0x7F;
+constexpr int MS_DIFFUSE = (1 << PT_MAT_LAMBERTIAN) | (1 << PT_MAT_OREN_NAYAR) | (1 << PT_MAT_DIFFUSE_LIGHT);
+constexpr int MS_SPHERES = (1 << PT_MAT_LAMBERTIAN) | (1 << PT_MAT_METAL) | (1 << PT_MAT_DIELECTRIC) |
+                           (1 << PT_MAT_DIFFUSE_LIGHT);
+template <int MS>
+PTD constexpr bool has_mat(int t) {
+  return (MS >> t) & 1;
+}
+
+template <int MS = MS_ALL>
 PTD v3 brdf_sample(const Surface& f, v3 Wo, v3 N, uint32_t& seed) {
-  if (f.type == PT_MAT_LAMBERTIAN || f.type == PT_MAT_OREN_NAYAR) {
+  if ((has_mat<MS>(PT_MAT_LAMBERTIAN) || has_mat<MS>(PT_MAT_OREN_NAYAR)) &&
+      (f.type == PT_MAT_LAMBERTIAN || f.type == PT_MAT_OREN_NAYAR)) {
     float u1 = rnd(seed);
     float u2 = rnd(seed);
     return onb_to_world(N, cosine_sample_hemisphere(u1, u2));
   }
-  if (f.type == PT_MAT_DIFFUSE_LIGHT) return random_on_unit_sphere(seed);
+  if (has_mat<MS>(PT_MAT_DIFFUSE_LIGHT) && f.type == PT_MAT_DIFFUSE_LIGHT) return random_on_unit_sphere(seed);
+  if (!has_mat<MS>(PT_MAT_ASHIKHMIN_SHIRLEY) && !has_mat<MS>(PT_MAT_TORRANCE_SPARROW)) return mk3(0.f);
   float nu = f.p0, nv = f.p1;
   v3 Nn = normalize(N);
   v3 T = normalize(cross(Nn, mk3(0.f, 1.f, 0.f)));
@@ -350,7 +363,7 @@ PTD v3 brdf_sample(const Surface& f, v3 Wo, v3 N, uint32_t& seed) {
   float rx = rnd(seed);
   float ry = rnd(seed);
   v3 H;
-  if (f.type == PT_MAT_ASHIKHMIN_SHIRLEY) {
+  if (has_mat<MS>(PT_MAT_ASHIKHMIN_SHIRLEY) && (!has_mat<MS>(PT_MAT_TORRANCE_SPARROW) || f.type == PT_MAT_ASHIKHMIN_SHIRLEY)) {
     if ((double)rx < 0.5) {
       rx = fminf(2 * rx, 1.f - 1e-6f);
       return onb_to_world(Nn, cosine_sample_hemisphere(rx, ry));
@@ -379,8 +392,12 @@ PTD v3 brdf_sample(const Surface& f, v3 Wo, v3 N, uint32_t& seed) {
 // Evaluate(): lambertian.cuh:21-42, oren_nayar.cuh:22-65,
 // diffuse_light.cuh:15-22, ashikhmin_shirley.cuh:84-151,
 // torrance_sparrow.cuh:37-64.  pdf keeps its incoming value on early returns.
+template <int MS = MS_ALL>
 PTD v3 brdf_eval(const Surface& f, v3 Wo, v3 Wi, v3 N, float& pdf) {
-  switch (f.type) {
+  int type = f.type;
+  // fold the dispatch for materials outside the compiled set
+  if (!has_mat<MS>(type)) return mk3(0.f);
+  switch (type) {
     case PT_MAT_LAMBERTIAN: {
       float cosine = dot(normalize(Wi), normalize(N));
       if (cosine < 0.f) return mk3(0.f);

@@ -26,17 +26,15 @@ void launch_trace_closest(const DevScene& sc, const pt_ray* rays, uint64_t n, pt
 void launch_trace_occluded(const DevScene& sc, const pt_ray* rays, uint64_t n, uint8_t* occ, uint2* spill,
                            uint32_t* overflow, int grid, hipStream_t st);
 void launch_raygen(const DevPaths& P, const RenderConsts& rc, uint32_t nslots, int grid, hipStream_t st);
-void launch_extend(const DevScene& sc, const DevPaths& P, const uint32_t* list, const uint32_t* count, uint2* spill,
-                   uint64_t* stats, bool count_traversal, int grid, hipStream_t st);
-void launch_sort(const DevScene& sc, const DevPaths& P, const uint32_t* list, const uint32_t* count, int grid,
-                 hipStream_t st);
-void launch_shade(const DevScene& sc, const DevPaths& P, const uint32_t* list, const uint32_t* count,
-                  uint32_t* next_list, uint32_t* next_count, uint32_t depth, const RenderConsts& rc, int grid,
+void launch_extend(const DevScene& sc, const DevPaths& P, int in, float tmin, uint2* spill, uint64_t* stats,
+                   bool count_traversal, bool sort, int grid, hipStream_t st);
+void launch_sort(const DevPaths& P, int in, int grid, hipStream_t st);
+void launch_shade(const DevScene& sc, const DevPaths& P, int in, bool sorted, int out, uint32_t depth,
+                  const RenderConsts& rc, int matset, bool count, uint2* spill, uint64_t* stats, int grid,
                   hipStream_t st);
-void launch_shadow_connect(const DevScene& sc, const DevPaths& P, uint32_t* next_list, uint32_t* next_count,
-                           uint32_t depth, const RenderConsts& rc, uint2* spill, uint64_t* stats, int grid,
-                           hipStream_t st);
-void launch_bounce_reset(const DevPaths& P, int zero_ctr, hipStream_t st);
+int shade_material_set(uint32_t types_mask);
+void launch_bounce_reset(const DevPaths& P, int in, hipStream_t st);
+void launch_exhausted(const DevPaths& P, int in, uint64_t* stats, int grid, hipStream_t st);
 void launch_accumulate(const DevPaths& P, const RenderConsts& rc, uint32_t nframes, double* accum, int grid,
                        hipStream_t st);
 void launch_display(const double* accum, uint32_t npix, uint32_t frames, uchar4* rgba, int grid, hipStream_t st);

@@ -113,41 +113,36 @@ struct DevCamera {
   float lens_radius, time0, time1;
 };
 
-// Path queues in HBM (structure of arrays, one slot per camera path of a
-// batch; queue lists hold slot indices).
-struct DevPaths {
-  float4* ray_o;   // (o.xyz, tmin)
-  float4* ray_d;   // (d.xyz, 0)
-  float4* thr;     // (T.xyz, flags: bit0 previous bounce specular)
-  float4* rad;     // (L.xyz, 0)
-  uint32_t* seed;
-  uint32_t* sig;
-  float4* hit;     // (t, inst bits, prim bits, 0)
-  float4* nee_p;   // (P.xyz, light index bits)
-  float4* nee_n;   // (N.xyz, material bits)
-  float4* nee_wo;  // (Wo.xyz, light pdf)
-  float4* nee_wi;  // (Wi.xyz, flags: bit0 emitter)
-  float4* nee_c;   // (surface c0.xyz, 0)
-  float4* result;  // (L.xyz, bounces bits)
-  uint32_t* list_a;
-  uint32_t* list_b;
-  uint32_t* list_shadow;
-  uint32_t* list_sorted;
-  uint32_t* keys;
-  uint32_t* counters;  // see CTR_*
+// Path state in HBM: 64 bytes per live path, structure of arrays, compacted
+// every bounce into the next ping-pong set so that every kernel streams it
+// contiguously.
+struct PathState {
+  float4* o;  // (origin.xyz, RNG state bits)
+  float4* d;  // (direction.xyz, slot bits: frame-in-batch * W*H + launch index)
+  float4* T;  // (throughput.xyz, flags bits: bit0 previous bounce specular)
+  float4* L;  // (radiance.xyz, decision-signature bits)
 };
 
-// device counters (uint32 unless noted)
-constexpr int CTR_ACTIVE_A = 0;
-constexpr int CTR_ACTIVE_B = 1;
-constexpr int CTR_SHADOW = 2;
-constexpr int CTR_BUCKETS = 4;        // 8 material buckets
-constexpr int CTR_BUCKET_CURSOR = 12; // 8 scatter cursors
+struct DevPaths {
+  PathState st[3];      // [0], [1] ping-pong, [2] material-sorted copy
+  float4* hit;          // per active path of the input set: (t, instance position, prim, instance id)
+  float4* hit_sorted;
+  uint32_t* keys;       // material bucket per active path (sorted shading)
+  float4* result;       // per slot: (color() value, bounces bits)
+  uint32_t* sig_final;  // per slot: decision signature
+  uint32_t* counters;   // see CTR_*
+};
+
+// device counters (uint32)
+constexpr int CTR_ACTIVE = 0;          // [0], [1]: live paths of state set 0 / 1
+constexpr int CTR_BUCKETS = 4;         // 8 material buckets
+constexpr int CTR_BUCKET_CURSOR = 12;  // 8 scatter cursors
 constexpr int CTR_STACK_OVERFLOW = 20;
 constexpr int CTR_WORDS = 32;
 // 64-bit statistics: [0] extension rays, [1] shadow rays, [2] nodes,
 // [3] prims, [4] instances
 constexpr int STAT_EXT = 0, STAT_SHADOW = 1, STAT_NODES = 2, STAT_PRIMS = 3, STAT_INSTS = 4;
-constexpr int STAT_WORDS = 8;
+constexpr int STAT_SH_NODES = 5, STAT_SH_PRIMS = 6, STAT_SH_INSTS = 7, STAT_EXHAUSTED = 8;
+constexpr int STAT_WORDS = 16;
 
 }  // namespace pt

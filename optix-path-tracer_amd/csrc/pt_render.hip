@@ -19,17 +19,6 @@ namespace pt {
 PTD uint32_t lane_rank(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
-// Wave-aggregated append of `value` to `list` where `pred` holds.
-PTD void wave_append(bool pred, uint32_t value, uint32_t* list, uint32_t* count) {
-  uint64_t mask = __ballot(pred);
-  if (mask == 0) return;
-  int leader = __ffsll((unsigned long long)mask) - 1;
-  uint32_t base = 0;
-  if ((int)__lane_id() == leader) base = atomicAdd(count, (uint32_t)__popcll(mask));
-  base = __shfl(base, leader);
-  if (pred) list[base + lane_rank(mask)] = value;
-}
-
 PTD Stack make_stack(uint32_t* s_ref, float* s_tn, uint2* spill, uint32_t* overflow) {
   Stack st;
   st.ref = s_ref + threadIdx.x;
@@ -88,6 +77,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_occluded(DevScene sc, const pt_
 // raygen.cu:148-168 renderPixel up to the first rtTrace, raygen.cu:50-64
 // generateRay, raygen.cu:66-74 color() set-up.
 __global__ __launch_bounds__(BLOCK) void k_raygen(DevPaths P, RenderConsts rc, uint32_t nslots) {
+  PathState S = P.st[0];
   for (uint32_t s = blockIdx.x * BLOCK + threadIdx.x; s < nslots; s += gridDim.x * BLOCK) {
     uint32_t f = s / rc.WH, pix = s - f * rc.WH;
     uint32_t x = pix % rc.W, y = pix / rc.W;
@@ -102,38 +92,63 @@ __global__ __launch_bounds__(BLOCK) void k_raygen(DevPaths P, RenderConsts rc, u
     v3 dir = mk3(c.llc[0], c.llc[1], c.llc[2]) + u * mk3(c.hor[0], c.hor[1], c.hor[2]) +
              v * mk3(c.ver[0], c.ver[1], c.ver[2]) - origin;
     (void)rnd(seed);  // prd.time (raygen.cu:69), consumed but unused
-    P.ray_o[s] = make_float4(origin.x, origin.y, origin.z, 1e-6f);
-    P.ray_d[s] = make_float4(dir.x, dir.y, dir.z, 0.f);
-    P.thr[s] = make_float4(1.f, 1.f, 1.f, 0.f);
-    P.rad[s] = make_float4(0.f, 0.f, 0.f, 0.f);
-    P.seed[s] = seed;
-    P.sig[s] = 0x811C9DC5u;
+    S.o[s] = make_float4(origin.x, origin.y, origin.z, __uint_as_float(seed));
+    S.d[s] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(s));
+    S.T[s] = make_float4(1.f, 1.f, 1.f, __uint_as_float(0u));
+    S.L[s] = make_float4(0.f, 0.f, 0.f, __uint_as_float(0x811C9DC5u));
     P.result[s] = make_float4(0.f, 0.f, 0.f, __uint_as_float(rc.max_depth));
-    P.list_a[s] = s;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    P.counters[CTR_ACTIVE_A] = nslots;
-    P.counters[CTR_ACTIVE_B] = 0;
-    P.counters[CTR_SHADOW] = 0;
-  }
+  if (blockIdx.x == 0 && threadIdx.x < CTR_STACK_OVERFLOW) P.counters[threadIdx.x] = threadIdx.x == 0 ? nslots : 0u;
 }
 
-// closest-hit query of every active path (raygen.cu:77)
-template <bool COUNT>
-__global__ __launch_bounds__(BLOCK) void k_extend(DevScene sc, DevPaths P, const uint32_t* __restrict__ list,
-                                                  const uint32_t* __restrict__ count_ptr, uint2* spill,
+// material bucket of a hit: 0 = miss, 1 + material type otherwise
+PTD uint32_t material_key(const DevScene& sc, float4 h) {
+  int pos = __float_as_int(h.y);
+  return pos < 0 ? 0u : 1u + (uint32_t)sc.mats[hit_material(sc, pos, __float_as_int(h.z))].type;
+}
+
+// Closest-hit query of every live path (raygen.cu:77).  With SORT the
+// material bucket of each hit is counted (per wave by ballot, per block in
+// LDS, one global atomic per bucket and block).
+template <bool COUNT, bool SORT>
+__global__ __launch_bounds__(BLOCK) void k_extend(DevScene sc, DevPaths P, int in, float tmin, uint2* spill,
                                                   uint64_t* stats) {
   __shared__ uint32_t s_ref[STK_LDS * BLOCK];
   __shared__ float s_tn[STK_LDS * BLOCK];
+  __shared__ uint32_t bcnt[8];
   Stack st = make_stack(s_ref, s_tn, spill, P.counters + CTR_STACK_OVERFLOW);
-  const uint32_t n = *count_ptr;
+  const uint32_t n = P.counters[CTR_ACTIVE + in];
+  const PathState S = P.st[in];
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd((unsigned long long*)&stats[STAT_EXT], (unsigned long long)n);
+  if (SORT) {
+    if (threadIdx.x < 8) bcnt[threadIdx.x] = 0;
+    __syncthreads();
+  }
   TraceCounts c{0, 0, 0};
-  for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
-    uint32_t s = list[i];
-    float4 o = P.ray_o[s], d = P.ray_d[s];
-    HitOut h = trace_closest<COUNT>(sc, mk3(o), mk3(d), o.w, FINF, st, c);
-    P.hit[s] = make_float4(h.t, __int_as_float(h.pos), __int_as_float(h.prim), __int_as_float(h.inst));
+  for (uint32_t i0 = blockIdx.x * BLOCK; i0 < n; i0 += gridDim.x * BLOCK) {
+    uint32_t i = i0 + threadIdx.x;
+    uint32_t key = 0xFFu;
+    if (i < n) {
+      float4 o = S.o[i], d = S.d[i];
+      HitOut h = trace_closest<COUNT>(sc, mk3(o), mk3(d), tmin, FINF, st, c);
+      float4 hv = make_float4(h.t, __int_as_float(h.pos), __int_as_float(h.prim), __int_as_float(h.inst));
+      P.hit[i] = hv;
+      if (SORT) {
+        key = material_key(sc, hv);
+        P.keys[i] = key;
+      }
+    }
+    if (SORT) {
+#pragma unroll
+      for (uint32_t b = 0; b < 8; b++) {
+        uint64_t m = __ballot(key == b);
+        if (m && key == b && lane_rank(m) == 0) atomicAdd(&bcnt[b], (uint32_t)__popcll(m));
+      }
+    }
+  }
+  if (SORT) {
+    __syncthreads();
+    if (threadIdx.x < 8 && bcnt[threadIdx.x]) atomicAdd(&P.counters[CTR_BUCKETS + threadIdx.x], bcnt[threadIdx.x]);
   }
   if (COUNT) {
     uint32_t a = c.nodes, b = c.prims, d = c.insts;
@@ -150,36 +165,6 @@ __global__ __launch_bounds__(BLOCK) void k_extend(DevScene sc, DevPaths P, const
   }
 }
 
-// Material sort: bucket 0 = miss, 1 + material type otherwise.  Counts and
-// scatter cursors are aggregated per wave (ballot) and per block (LDS), so
-// each block issues one global atomic per bucket.
-PTD uint32_t material_key(const DevScene& sc, const DevPaths& P, uint32_t s) {
-  float4 h = P.hit[s];
-  int pos = __float_as_int(h.y);
-  return pos < 0 ? 0u : 1u + (uint32_t)sc.mats[hit_material(sc, pos, __float_as_int(h.z))].type;
-}
-__global__ __launch_bounds__(BLOCK) void k_sort_count(DevScene sc, DevPaths P, const uint32_t* __restrict__ list,
-                                                      const uint32_t* __restrict__ count_ptr) {
-  __shared__ uint32_t cnt[8];
-  const uint32_t n = *count_ptr;
-  if (threadIdx.x < 8) cnt[threadIdx.x] = 0;
-  __syncthreads();
-  for (uint32_t i0 = blockIdx.x * BLOCK; i0 < n; i0 += gridDim.x * BLOCK) {
-    uint32_t i = i0 + threadIdx.x;
-    uint32_t key = 0xFFu;
-    if (i < n) {
-      key = material_key(sc, P, list[i]);
-      P.keys[i] = key;
-    }
-#pragma unroll
-    for (uint32_t b = 0; b < 8; b++) {
-      uint64_t m = __ballot(key == b);
-      if (m && key == b && lane_rank(m) == 0) atomicAdd(&cnt[b], (uint32_t)__popcll(m));
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x < 8 && cnt[threadIdx.x]) atomicAdd(&P.counters[CTR_BUCKETS + threadIdx.x], cnt[threadIdx.x]);
-}
 __global__ void k_sort_offsets(DevPaths P) {
   uint32_t acc = 0;
   for (int b = 0; b < 8; b++) {
@@ -187,14 +172,15 @@ __global__ void k_sort_offsets(DevPaths P) {
     acc += P.counters[CTR_BUCKETS + b];
   }
 }
-__global__ __launch_bounds__(BLOCK) void k_sort_scatter(DevPaths P, const uint32_t* __restrict__ list,
-                                                        const uint32_t* __restrict__ count_ptr) {
+// Move path state and hit into material-bucket order (state set 2); cursors
+// are claimed per wave (ballot) and per block (LDS).
+__global__ __launch_bounds__(BLOCK) void k_sort_scatter(DevPaths P, int in) {
   __shared__ uint32_t cnt[8], base[8];
-  const uint32_t n = *count_ptr;
+  const uint32_t n = P.counters[CTR_ACTIVE + in];
+  const PathState S = P.st[in], D = P.st[2];
   for (uint32_t i0 = blockIdx.x * BLOCK; i0 < n; i0 += gridDim.x * BLOCK) {
     uint32_t i = i0 + threadIdx.x;
     uint32_t key = i < n ? P.keys[i] : 0xFFu;
-    uint32_t s = i < n ? list[i] : 0;
     if (threadIdx.x < 8) cnt[threadIdx.x] = 0;
     __syncthreads();
     uint32_t wave_off = 0;
@@ -209,79 +195,79 @@ __global__ __launch_bounds__(BLOCK) void k_sort_scatter(DevPaths P, const uint32
       if (key == b) wave_off = o;
     }
     __syncthreads();
-    if (threadIdx.x < 8) base[threadIdx.x] = cnt[threadIdx.x] ? atomicAdd(&P.counters[CTR_BUCKET_CURSOR + threadIdx.x], cnt[threadIdx.x]) : 0u;
+    if (threadIdx.x < 8)
+      base[threadIdx.x] = cnt[threadIdx.x] ? atomicAdd(&P.counters[CTR_BUCKET_CURSOR + threadIdx.x], cnt[threadIdx.x]) : 0u;
     __syncthreads();
-    if (key < 8) P.list_sorted[base[key] + wave_off + lane_rank(mine)] = s;
+    if (key < 8) {
+      uint32_t j = base[key] + wave_off + lane_rank(mine);
+      D.o[j] = S.o[i];
+      D.d[j] = S.d[i];
+      D.T[j] = S.T[i];
+      D.L[j] = S.L[i];
+      P.hit_sorted[j] = P.hit[i];
+    }
     __syncthreads();
   }
 }
 
-// Scatter of the materials without light sampling plus the continuation
+// Scatter of the materials that do not sample lights plus the continuation
 // sample of Lambertian / Oren-Nayar, then Russian roulette (raygen.cu:96-118).
 // Returns false when the path ended (result written).
-PTD bool scatter_and_roulette(const Surface& f, v3 P, v3 Wo, v3 N, uint32_t& seed, uint32_t& sig, v3& T, v3 L,
-                              v3& dir, uint32_t& flags, uint32_t depth, const RenderConsts& rc, float4& result) {
+template <int MS>
+PTD bool scatter_and_roulette(const Surface& f, v3 Wo, v3 N, uint32_t& seed, uint32_t& sig, v3& T, v3 L, v3& dir,
+                              uint32_t& flags, uint32_t depth, const RenderConsts& rc, float4& result) {
   bool spec = false;
-  switch (f.type) {
-    case PT_MAT_LAMBERTIAN:
-    case PT_MAT_OREN_NAYAR: {  // lambertian.cu:62-72, oren_nayar.cu:59-69
-      v3 Wi = brdf_sample(f, Wo, N, seed);
-      float pdf = 1.f;
-      v3 att = brdf_eval(f, Wo, Wi, N, pdf);
-      dir = Wi;
-      T = T * clamp3(att / pdf, 0.f, 1.f);
-      break;
+  if ((has_mat<MS>(PT_MAT_LAMBERTIAN) || has_mat<MS>(PT_MAT_OREN_NAYAR)) &&
+      (f.type == PT_MAT_LAMBERTIAN || f.type == PT_MAT_OREN_NAYAR)) {
+    // lambertian.cu:62-72, oren_nayar.cu:59-69
+    v3 Wi = brdf_sample<MS>(f, Wo, N, seed);
+    float pdf = 1.f;
+    v3 att = brdf_eval<MS>(f, Wo, Wi, N, pdf);
+    dir = Wi;
+    T = T * clamp3(att / pdf, 0.f, 1.f);
+  } else if (has_mat<MS>(PT_MAT_METAL) && f.type == PT_MAT_METAL) {  // metal.cu:38-56
+    v3 refl = reflect(-Wo, N);
+    dir = refl + f.p0 * random_in_unit_sphere(seed);
+    T = T * f.c0;
+    spec = true;
+  } else if (has_mat<MS>(PT_MAT_DIELECTRIC) && f.type == PT_MAT_DIELECTRIC) {  // dielectric.cu:46-96
+    float ior = f.p0;
+    v3 W = -Wo;
+    v3 Nd = N;
+    float ni_over_nt;
+    float cosine = dot(W, Nd);
+    if (cosine > 0.f) {
+      Nd = -Nd;
+      ni_over_nt = ior;
+      cosine = ior * cosine / length(W);
+    } else {
+      ni_over_nt = 1.f / ior;
+      cosine = -cosine / length(W);
     }
-    case PT_MAT_METAL: {  // metal.cu:38-56
-      v3 refl = reflect(-Wo, N);
-      dir = refl + f.p0 * random_in_unit_sphere(seed);
-      T = T * f.c0;
-      spec = true;
-      break;
-    }
-    case PT_MAT_DIELECTRIC: {  // dielectric.cu:46-96
-      float ior = f.p0;
-      v3 W = -Wo;
-      v3 Nd = N;
-      float ni_over_nt;
-      float cosine = dot(W, Nd);
-      if (cosine > 0.f) {
-        Nd = -Nd;
-        ni_over_nt = ior;
-        cosine = ior * cosine / length(W);
-      } else {
-        ni_over_nt = 1.f / ior;
-        cosine = -cosine / length(W);
-      }
-      v3 refracted = mk3(0.f);
-      float reflect_prob;
-      if (Refract(W, Nd, ni_over_nt, refracted)) reflect_prob = schlick(cosine, ior);
-      else reflect_prob = 1.f;
-      bool refl = rnd(seed) < reflect_prob;
-      sig = sig_mix(sig, refl ? 0x11111111u : 0x22222222u);
-      dir = refl ? reflect(W, Nd) : normalize(refracted);
-      T = T * f.c0;
-      spec = true;
-      break;
-    }
-    case PT_MAT_ASHIKHMIN_SHIRLEY: {  // ashikhmin_shirley.cu:48-68
-      v3 Wi = brdf_sample(f, Wo, N, seed);
-      float pdf = 1.f;
-      v3 att = brdf_eval(f, Wo, Wi, N, pdf);
-      dir = Wi;
-      T = T * clamp3(att, 0.f, 1.f);
-      spec = true;
-      break;
-    }
-    default: {  // Torrance-Sparrow, torrance_sparrow.cu:39-59
-      v3 Wi = brdf_sample(f, Wo, N, seed);
-      float pdf = 1.f;
-      v3 att = brdf_eval(f, Wo, Wi, N, pdf);
-      dir = Wi;
-      T = T * clamp3(att / pdf, 0.f, 1.f);
-      spec = true;
-      break;
-    }
+    v3 refracted = mk3(0.f);
+    float reflect_prob;
+    if (Refract(W, Nd, ni_over_nt, refracted)) reflect_prob = schlick(cosine, ior);
+    else reflect_prob = 1.f;
+    bool refl = rnd(seed) < reflect_prob;
+    sig = sig_mix(sig, refl ? 0x11111111u : 0x22222222u);
+    dir = refl ? reflect(W, Nd) : normalize(refracted);
+    T = T * f.c0;
+    spec = true;
+  } else if (has_mat<MS>(PT_MAT_ASHIKHMIN_SHIRLEY) && f.type == PT_MAT_ASHIKHMIN_SHIRLEY) {
+    // ashikhmin_shirley.cu:48-68
+    v3 Wi = brdf_sample<MS>(f, Wo, N, seed);
+    float pdf = 1.f;
+    v3 att = brdf_eval<MS>(f, Wo, Wi, N, pdf);
+    dir = Wi;
+    T = T * clamp3(att, 0.f, 1.f);
+    spec = true;
+  } else if (has_mat<MS>(PT_MAT_TORRANCE_SPARROW)) {  // Torrance-Sparrow, torrance_sparrow.cu:39-59
+    v3 Wi = brdf_sample<MS>(f, Wo, N, seed);
+    float pdf = 1.f;
+    v3 att = brdf_eval<MS>(f, Wo, Wi, N, pdf);
+    dir = Wi;
+    T = T * clamp3(att / pdf, 0.f, 1.f);
+    spec = true;
   }
   flags = spec ? 1u : 0u;
   if (rc.russian) {
@@ -300,186 +286,167 @@ PTD bool scatter_and_roulette(const Surface& f, v3 P, v3 Wo, v3 N, uint32_t& see
   return true;
 }
 
-// Closest-hit / miss programs of every active path, first half of
-// Direct_Light (light_sample.cuh:35-59) for materials that sample lights.
-__global__ __launch_bounds__(BLOCK) void k_shade(DevScene sc, DevPaths P, const uint32_t* __restrict__ list,
-                                                 const uint32_t* __restrict__ count_ptr, uint32_t* next_list,
-                                                 uint32_t* next_count, uint32_t depth, RenderConsts rc) {
-  const uint32_t n = *count_ptr;
+// Closest-hit / miss programs of every live path for one bounce: hit record,
+// Direct_Light with its shadow ray traced here (light_sample.cuh:35-103;
+// lit iff the closest hit in (1e-3, inf) is a diffuse light with
+// dot(N, Wi) > 0, or nothing is hit), the emitter rules, the continuation
+// sample and Russian roulette.  Surviving paths are appended, wave by wave,
+// to state set `out`; finished paths write their slot's result.
+template <int MS, bool COUNT>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_shade(DevScene sc, DevPaths P, int in, int src, const float4* __restrict__ hits,
+                                                 int out, uint32_t depth, RenderConsts rc, uint2* spill,
+                                                 uint64_t* stats) {
+  __shared__ uint32_t s_ref[STK_LDS * BLOCK];
+  __shared__ float s_tn[STK_LDS * BLOCK];
+  Stack st = make_stack(s_ref, s_tn, spill, P.counters + CTR_STACK_OVERFLOW);
+  const uint32_t n = P.counters[CTR_ACTIVE + in];
+  const PathState S = P.st[src], D = P.st[out];
+  uint32_t* out_count = P.counters + CTR_ACTIVE + out;
+  TraceCounts tc{0, 0, 0};
+  uint32_t shadow_rays = 0;
   for (uint32_t i0 = blockIdx.x * BLOCK; i0 < n; i0 += gridDim.x * BLOCK) {
     uint32_t i = i0 + threadIdx.x;
-    bool alive = false, pending = false;
-    uint32_t s = 0;
+    bool alive = false;
+    float4 no = make_float4(0.f, 0.f, 0.f, 0.f), nd = no, nT = no, nL = no;
     if (i < n) {
-      s = list[i];
-      float4 hit = P.hit[s];
-      float4 d4 = P.ray_d[s];
-      float4 t4 = P.thr[s];
-      float4 l4 = P.rad[s];
+      float4 hit = hits[i];
+      float4 o4 = S.o[i], d4 = S.d[i], t4 = S.T[i], l4 = S.L[i];
       v3 d = mk3(d4), T = mk3(t4), L = mk3(l4);
-      uint32_t flags = __float_as_uint(t4.w);
-      uint32_t seed = P.seed[s];
+      uint32_t seed = __float_as_uint(o4.w), slot = __float_as_uint(d4.w);
+      uint32_t flags = __float_as_uint(t4.w), sig = __float_as_uint(l4.w);
       int pos = __float_as_int(hit.y), prim = __float_as_int(hit.z), inst = __float_as_int(hit.w);
-      uint32_t sig = sig_mix(P.sig[s], pos >= 0 ? (((uint32_t)inst << 20) ^ (uint32_t)prim) : 0xFFFFFFFFu);
+      sig = sig_mix(sig, pos >= 0 ? (((uint32_t)inst << 20) ^ (uint32_t)prim) : 0xFFFFFFFFu);
+      float4 res = make_float4(0.f, 0.f, 0.f, 0.f);
       if (pos < 0) {
         // miss programs (miss.cu), then color() returns L + clamp(T) (raygen.cu:81-82)
         T = T * miss_color(sc, d);
         v3 r = L + clamp3(T, 0.f, 1.f);
-        P.result[s] = make_float4(r.x, r.y, r.z, __uint_as_float(depth + 1));
+        res = make_float4(r.x, r.y, r.z, __uint_as_float(depth + 1));
       } else {
-        float4 o4 = P.ray_o[s];
         int mat = hit_material(sc, pos, prim);
         const DevMaterial& M = sc.mats[mat];
         bool need_uv = tex_needs_uv(sc, M.tex0) || (M.type == PT_MAT_ASHIKHMIN_SHIRLEY && tex_needs_uv(sc, M.tex1));
         HitRec r = hit_record(sc, pos, prim, hit.x, mk3(o4), d, need_uv);
         Surface f = make_surface(sc, r);
-        bool nee = (f.type == PT_MAT_LAMBERTIAN || f.type == PT_MAT_OREN_NAYAR || f.type == PT_MAT_DIFFUSE_LIGHT);
-        bool is_light = f.type == PT_MAT_DIFFUSE_LIGHT;
-        if (nee && sc.num_lights > 0) {
+        bool is_light = has_mat<MS>(PT_MAT_DIFFUSE_LIGHT) && f.type == PT_MAT_DIFFUSE_LIGHT;
+        bool nee = ((has_mat<MS>(PT_MAT_LAMBERTIAN) || has_mat<MS>(PT_MAT_OREN_NAYAR)) &&
+                    (f.type == PT_MAT_LAMBERTIAN || f.type == PT_MAT_OREN_NAYAR)) || is_light;
+        if (nee) {
+          v3 direct = mk3(0.f);
           int nl = sc.num_lights;
-          int index = ((int)(rnd(seed) * nl)) % nl;
-          if (!(is_light && nl == 1)) {
-            const DevLight& Lt = sc.lights[index];
-            v3 Wi = light_sample(Lt, r.P, seed);
-            float lpdf = light_pdf(Lt, r.P, Wi);
-            if (dot(Wi, r.N) < 0.f) {
-              sig = sig_mix(sig, 0x33333333u);
-            } else {
-              pending = true;
-              P.nee_p[s] = make_float4(r.P.x, r.P.y, r.P.z, __int_as_float(index));
-              P.nee_n[s] = make_float4(r.N.x, r.N.y, r.N.z, __int_as_float(mat));
-              P.nee_wo[s] = make_float4(r.Wo.x, r.Wo.y, r.Wo.z, lpdf);
-              P.nee_wi[s] = make_float4(Wi.x, Wi.y, Wi.z, __uint_as_float(is_light ? 1u : 0u));
-              P.nee_c[s] = make_float4(f.c0.x, f.c0.y, f.c0.z, 0.f);
+          if (nl > 0) {
+            int index = ((int)(rnd(seed) * nl)) % nl;
+            if (!(is_light && nl == 1)) {
+              const DevLight& Lt = sc.lights[index];
+              v3 emission = light_emission(Lt);
+              v3 Wi = light_sample(Lt, r.P, seed);
+              float lightPDF = light_pdf(Lt, r.P, Wi);
+              if (dot(Wi, r.N) < 0.f) {
+                sig = sig_mix(sig, 0x33333333u);
+              } else {
+                shadow_rays++;
+                HitOut sh = trace_closest<COUNT>(sc, r.P, Wi, 1e-3f, FINF, st, tc);
+                bool occluded = false;
+                if (sh.inst >= 0)
+                  occluded = !(sc.mats[hit_material(sc, sh.pos, sh.prim)].type == PT_MAT_DIFFUSE_LIGHT &&
+                               dot(r.N, Wi) > 0.f);
+                sig = sig_mix(sig, occluded ? 0xA5A5A5A5u : 0x5A5A5A5Au);
+                if (!occluded) {
+                  if (lightPDF != 0.f && !isNull(emission)) {
+                    float matPDF = 1.f;
+                    v3 mv = brdf_eval<MS>(f, r.Wo, Wi, r.N, matPDF);
+                    if (matPDF != 0.f && !isNull(mv)) {
+                      float w = power_heuristic(lightPDF, matPDF);
+                      direct = direct + mv * emission * w / lightPDF;
+                    }
+                  }
+                  v3 Wb = brdf_sample<MS>(f, r.Wo, r.N, seed);
+                  float matPDF = 1.f;
+                  v3 mv = brdf_eval<MS>(f, r.Wo, Wb, r.N, matPDF);
+                  if (matPDF != 0.f && !isNull(mv)) {
+                    float lp = light_pdf(Lt, r.P, Wb);
+                    if (lp != 0.f && !isNull(emission)) {
+                      float w = power_heuristic(matPDF, lp);
+                      direct = direct + mv * emission * w / matPDF;
+                    }
+                  }
+                }
+              }
             }
           }
+          L = L + T * direct;
         }
-        if (!pending) {
-          if (nee) L = L + T * mk3(0.f);  // L += T * direct with direct = 0
-          if (is_light) {
-            // diffuse_light.cu:43-47 and raygen.cu:85-90
-            if (dot(r.N, r.Wo) < 0.f) T = T * f.c0;
-            if (depth == 0 || (flags & 1u)) L = L + T;
-            P.result[s] = make_float4(L.x, L.y, L.z, __uint_as_float(depth + 1));
-          } else {
-            v3 ndir;
-            float4 res;
-            alive = scatter_and_roulette(f, r.P, r.Wo, r.N, seed, sig, T, L, ndir, flags, depth, rc, res);
-            if (alive) {
-              P.ray_o[s] = make_float4(r.P.x, r.P.y, r.P.z, 1e-3f);
-              P.ray_d[s] = make_float4(ndir.x, ndir.y, ndir.z, 0.f);
-            } else {
-              P.result[s] = res;
-            }
-          }
-        }
-        P.thr[s] = make_float4(T.x, T.y, T.z, __uint_as_float(flags));
-        P.rad[s] = make_float4(L.x, L.y, L.z, 0.f);
-        P.seed[s] = seed;
-      }
-      P.sig[s] = sig;
-    }
-    wave_append(alive, s, next_list, next_count);
-    wave_append(pending, s, P.list_shadow, P.counters + CTR_SHADOW);
-  }
-}
-
-// Shadow ray of Direct_Light (light_sample.cuh:61-72) under the rule: lit
-// iff the closest hit in (1e-3, inf) is a diffuse light with dot(N, Wi) > 0,
-// or nothing is hit; then the second half of Direct_Light
-// (light_sample.cuh:74-102), the radiance update of the closest-hit program
-// and the continuation bounce.
-__global__ __launch_bounds__(BLOCK) void k_shadow_connect(DevScene sc, DevPaths P, uint32_t* next_list,
-                                                          uint32_t* next_count, uint32_t depth, RenderConsts rc,
-                                                          uint2* spill, uint64_t* stats) {
-  __shared__ uint32_t s_ref[STK_LDS * BLOCK];
-  __shared__ float s_tn[STK_LDS * BLOCK];
-  Stack st = make_stack(s_ref, s_tn, spill, P.counters + CTR_STACK_OVERFLOW);
-  const uint32_t n = P.counters[CTR_SHADOW];
-  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd((unsigned long long*)&stats[STAT_SHADOW], (unsigned long long)n);
-  TraceCounts tc{0, 0, 0};
-  for (uint32_t i0 = blockIdx.x * BLOCK; i0 < n; i0 += gridDim.x * BLOCK) {
-    uint32_t i = i0 + threadIdx.x;
-    bool alive = false;
-    uint32_t s = 0;
-    if (i < n) {
-      s = P.list_shadow[i];
-      float4 p4 = P.nee_p[s], n4 = P.nee_n[s], wo4 = P.nee_wo[s], wi4 = P.nee_wi[s], c4 = P.nee_c[s];
-      v3 Pp = mk3(p4), N = mk3(n4), Wo = mk3(wo4), Wi = mk3(wi4);
-      HitOut sh = trace_closest<false>(sc, Pp, Wi, 1e-3f, FINF, st, tc);
-      bool occluded = false;
-      if (sh.inst >= 0)
-        occluded = !(sc.mats[hit_material(sc, sh.pos, sh.prim)].type == PT_MAT_DIFFUSE_LIGHT && dot(N, Wi) > 0.f);
-      float4 t4 = P.thr[s], l4 = P.rad[s];
-      v3 T = mk3(t4), L = mk3(l4);
-      uint32_t flags = __float_as_uint(t4.w);
-      uint32_t seed = P.seed[s];
-      uint32_t sig = P.sig[s];
-      int li = __float_as_int(p4.w), mat = __float_as_int(n4.w);
-      bool is_light = (__float_as_uint(wi4.w) & 1u) != 0;
-      float lightPDF = wo4.w;
-      const DevMaterial& M = sc.mats[mat];
-      Surface f;
-      f.type = M.type;
-      f.c0 = mk3(c4);
-      f.c1 = mk3(0.f);
-      f.p0 = M.p0;
-      f.p1 = M.p1;
-      sig = sig_mix(sig, occluded ? 0xA5A5A5A5u : 0x5A5A5A5Au);
-      v3 direct = mk3(0.f);
-      if (!occluded) {
-        v3 emission = light_emission(sc.lights[li]);
-        if (lightPDF != 0.f && !isNull(emission)) {
-          float matPDF = 1.f;
-          v3 mv = brdf_eval(f, Wo, Wi, N, matPDF);
-          if (matPDF != 0.f && !isNull(mv)) {
-            float w = power_heuristic(lightPDF, matPDF);
-            direct = direct + mv * emission * w / lightPDF;
-          }
-        }
-        v3 Wb = brdf_sample(f, Wo, N, seed);
-        float matPDF = 1.f;
-        v3 mv = brdf_eval(f, Wo, Wb, N, matPDF);
-        if (matPDF != 0.f && !isNull(mv)) {
-          float lp = light_pdf(sc.lights[li], Pp, Wb);
-          if (lp != 0.f && !isNull(emission)) {
-            float w = power_heuristic(matPDF, lp);
-            direct = direct + mv * emission * w / matPDF;
-          }
-        }
-      }
-      L = L + T * direct;
-      if (is_light) {
-        if (dot(N, Wo) < 0.f) T = T * f.c0;
-        if (depth == 0 || (flags & 1u)) L = L + T;
-        P.result[s] = make_float4(L.x, L.y, L.z, __uint_as_float(depth + 1));
-      } else {
-        v3 ndir;
-        float4 res;
-        alive = scatter_and_roulette(f, Pp, Wo, N, seed, sig, T, L, ndir, flags, depth, rc, res);
-        if (alive) {
-          P.ray_o[s] = make_float4(Pp.x, Pp.y, Pp.z, 1e-3f);
-          P.ray_d[s] = make_float4(ndir.x, ndir.y, ndir.z, 0.f);
+        if (is_light) {
+          // diffuse_light.cu:43-47 and raygen.cu:85-90
+          if (dot(r.N, r.Wo) < 0.f) T = T * f.c0;
+          if (depth == 0 || (flags & 1u)) L = L + T;
+          res = make_float4(L.x, L.y, L.z, __uint_as_float(depth + 1));
         } else {
-          P.result[s] = res;
+          v3 ndir;
+          alive = scatter_and_roulette<MS>(f, r.Wo, r.N, seed, sig, T, L, ndir, flags, depth, rc, res);
+          if (alive) {
+            no = make_float4(r.P.x, r.P.y, r.P.z, __uint_as_float(seed));
+            nd = make_float4(ndir.x, ndir.y, ndir.z, __uint_as_float(slot));
+            nT = make_float4(T.x, T.y, T.z, __uint_as_float(flags));
+            nL = make_float4(L.x, L.y, L.z, __uint_as_float(sig));
+          }
         }
       }
-      P.thr[s] = make_float4(T.x, T.y, T.z, __uint_as_float(flags));
-      P.rad[s] = make_float4(L.x, L.y, L.z, 0.f);
-      P.seed[s] = seed;
-      P.sig[s] = sig;
+      if (!alive) {
+        P.result[slot] = res;
+        P.sig_final[slot] = sig;
+      }
     }
-    wave_append(alive, s, next_list, next_count);
+    // wave-aggregated compaction of the survivors into set `out`
+    uint64_t mask = __ballot(alive);
+    if (mask) {
+      int leader = __ffsll((unsigned long long)mask) - 1;
+      uint32_t base = 0;
+      if ((int)__lane_id() == leader) base = atomicAdd(out_count, (uint32_t)__popcll(mask));
+      base = __shfl(base, leader);
+      if (alive) {
+        uint32_t j = base + lane_rank(mask);
+        D.o[j] = no;
+        D.d[j] = nd;
+        D.T[j] = nT;
+        D.L[j] = nL;
+      }
+    }
+  }
+  uint32_t a = tc.nodes, b = tc.prims, c = tc.insts;
+  for (int off = 32; off > 0; off >>= 1) {
+    shadow_rays += __shfl_down(shadow_rays, off);
+    if (COUNT) {
+      a += __shfl_down(a, off);
+      b += __shfl_down(b, off);
+      c += __shfl_down(c, off);
+    }
+  }
+  if (__lane_id() == 0) {
+    if (shadow_rays) atomicAdd((unsigned long long*)&stats[STAT_SHADOW], (unsigned long long)shadow_rays);
+    if (COUNT) {
+      atomicAdd((unsigned long long*)&stats[STAT_SH_NODES], (unsigned long long)a);
+      atomicAdd((unsigned long long*)&stats[STAT_SH_PRIMS], (unsigned long long)b);
+      atomicAdd((unsigned long long*)&stats[STAT_SH_INSTS], (unsigned long long)c);
+    }
   }
 }
 
-// Zero the counters the next bounce appends to.
-__global__ void k_bounce_reset(DevPaths P, int zero_ctr) {
-  if (threadIdx.x == 0) {
-    P.counters[zero_ctr] = 0;
-    P.counters[CTR_SHADOW] = 0;
-    for (int b = 0; b < 8; b++) P.counters[CTR_BUCKETS + b] = 0;
-  }
+// End of a bounce: the consumed set becomes the next output; buckets reset.
+__global__ void k_bounce_reset(DevPaths P, int in) {
+  if (threadIdx.x == 0) P.counters[CTR_ACTIVE + in] = 0;
+  if (threadIdx.x < 8) P.counters[CTR_BUCKETS + threadIdx.x] = 0;
+}
+
+// Paths alive after the last bounce end with color() = 0 (raygen.cu:121-122);
+// only their signature is recorded.
+__global__ __launch_bounds__(BLOCK) void k_exhausted(DevPaths P, int in, uint64_t* stats) {
+  const uint32_t n = P.counters[CTR_ACTIVE + in];
+  const PathState S = P.st[in];
+  if (blockIdx.x == 0 && threadIdx.x == 0 && n)
+    atomicAdd((unsigned long long*)&stats[STAT_EXHAUSTED], (unsigned long long)n);
+  for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK)
+    P.sig_final[__float_as_uint(S.d[i].w)] = __float_as_uint(S.L[i].w);
 }
 
 // raygen.cu:126-134 de_nan and raygen.cu:166 acc_buffer += (rgb, 1), in FP64
@@ -540,32 +507,54 @@ void launch_trace_occluded(const DevScene& sc, const pt_ray* rays, uint64_t n, u
 void launch_raygen(const DevPaths& P, const RenderConsts& rc, uint32_t nslots, int grid, hipStream_t st) {
   hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(BLOCK), 0, st, P, rc, nslots);
 }
-void launch_extend(const DevScene& sc, const DevPaths& P, const uint32_t* list, const uint32_t* count, uint2* spill,
-                   uint64_t* stats, bool count_traversal, int grid, hipStream_t st) {
-  if (count_traversal)
-    hipLaunchKernelGGL(k_extend<true>, dim3(grid), dim3(BLOCK), 0, st, sc, P, list, count, spill, stats);
+void launch_extend(const DevScene& sc, const DevPaths& P, int in, float tmin, uint2* spill, uint64_t* stats,
+                   bool count_traversal, bool sort, int grid, hipStream_t st) {
+  if (count_traversal && sort)
+    hipLaunchKernelGGL((k_extend<true, true>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, tmin, spill, stats);
+  else if (count_traversal)
+    hipLaunchKernelGGL((k_extend<true, false>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, tmin, spill, stats);
+  else if (sort)
+    hipLaunchKernelGGL((k_extend<false, true>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, tmin, spill, stats);
   else
-    hipLaunchKernelGGL(k_extend<false>, dim3(grid), dim3(BLOCK), 0, st, sc, P, list, count, spill, stats);
+    hipLaunchKernelGGL((k_extend<false, false>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, tmin, spill, stats);
 }
-void launch_sort(const DevScene& sc, const DevPaths& P, const uint32_t* list, const uint32_t* count, int grid,
-                 hipStream_t st) {
-  hipLaunchKernelGGL(k_sort_count, dim3(grid), dim3(BLOCK), 0, st, sc, P, list, count);
+void launch_sort(const DevPaths& P, int in, int grid, hipStream_t st) {
   hipLaunchKernelGGL(k_sort_offsets, dim3(1), dim3(1), 0, st, P);
-  hipLaunchKernelGGL(k_sort_scatter, dim3(grid), dim3(BLOCK), 0, st, P, list, count);
+  hipLaunchKernelGGL(k_sort_scatter, dim3(grid), dim3(BLOCK), 0, st, P, in);
 }
-void launch_shade(const DevScene& sc, const DevPaths& P, const uint32_t* list, const uint32_t* count,
-                  uint32_t* next_list, uint32_t* next_count, uint32_t depth, const RenderConsts& rc, int grid,
+template <int MS>
+static void launch_shade_ms(const DevScene& sc, const DevPaths& P, int in, int src, const float4* hits, int out,
+                            uint32_t depth, const RenderConsts& rc, bool count, uint2* spill, uint64_t* stats,
+                            int grid, hipStream_t st) {
+  if (count)
+    hipLaunchKernelGGL((k_shade<MS, true>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, src, hits, out, depth, rc,
+                       spill, stats);
+  else
+    hipLaunchKernelGGL((k_shade<MS, false>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, src, hits, out, depth, rc,
+                       spill, stats);
+}
+void launch_shade(const DevScene& sc, const DevPaths& P, int in, bool sorted, int out, uint32_t depth,
+                  const RenderConsts& rc, int matset, bool count, uint2* spill, uint64_t* stats, int grid,
                   hipStream_t st) {
-  hipLaunchKernelGGL(k_shade, dim3(grid), dim3(BLOCK), 0, st, sc, P, list, count, next_list, next_count, depth, rc);
+  int src = sorted ? 2 : in;
+  const float4* hits = sorted ? P.hit_sorted : P.hit;
+  if (matset == MS_DIFFUSE)
+    launch_shade_ms<MS_DIFFUSE>(sc, P, in, src, hits, out, depth, rc, count, spill, stats, grid, st);
+  else if (matset == MS_SPHERES)
+    launch_shade_ms<MS_SPHERES>(sc, P, in, src, hits, out, depth, rc, count, spill, stats, grid, st);
+  else
+    launch_shade_ms<MS_ALL>(sc, P, in, src, hits, out, depth, rc, count, spill, stats, grid, st);
 }
-void launch_shadow_connect(const DevScene& sc, const DevPaths& P, uint32_t* next_list, uint32_t* next_count,
-                           uint32_t depth, const RenderConsts& rc, uint2* spill, uint64_t* stats, int grid,
-                           hipStream_t st) {
-  hipLaunchKernelGGL(k_shadow_connect, dim3(grid), dim3(BLOCK), 0, st, sc, P, next_list, next_count, depth, rc, spill,
-                     stats);
+int shade_material_set(uint32_t types_mask) {
+  if ((types_mask & ~(uint32_t)MS_DIFFUSE) == 0) return MS_DIFFUSE;
+  if ((types_mask & ~(uint32_t)MS_SPHERES) == 0) return MS_SPHERES;
+  return MS_ALL;
 }
-void launch_bounce_reset(const DevPaths& P, int zero_ctr, hipStream_t st) {
-  hipLaunchKernelGGL(k_bounce_reset, dim3(1), dim3(64), 0, st, P, zero_ctr);
+void launch_bounce_reset(const DevPaths& P, int in, hipStream_t st) {
+  hipLaunchKernelGGL(k_bounce_reset, dim3(1), dim3(64), 0, st, P, in);
+}
+void launch_exhausted(const DevPaths& P, int in, uint64_t* stats, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_exhausted, dim3(grid), dim3(BLOCK), 0, st, P, in, stats);
 }
 void launch_accumulate(const DevPaths& P, const RenderConsts& rc, uint32_t nframes, double* accum, int grid,
                        hipStream_t st) {

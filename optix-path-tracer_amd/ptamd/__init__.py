@@ -106,9 +106,11 @@ class RenderParams(C.Structure):
 class RenderStats(C.Structure):
     _fields_ = [("paths", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("node_fetches", C.c_uint64), ("prim_fetches", C.c_uint64),
-                ("instance_fetches", C.c_uint64), ("gpu_ms", C.c_double), ("extend_ms", C.c_double),
-                ("shadow_ms", C.c_double), ("extend_launches", C.c_uint32),
-                ("shadow_launches", C.c_uint32)]
+                ("instance_fetches", C.c_uint64), ("shadow_node_fetches", C.c_uint64),
+                ("shadow_prim_fetches", C.c_uint64), ("shadow_instance_fetches", C.c_uint64),
+                ("exhausted_paths", C.c_uint64), ("gpu_ms", C.c_double), ("extend_ms", C.c_double),
+                ("shade_ms", C.c_double), ("extend_launches", C.c_uint32),
+                ("shade_launches", C.c_uint32)]
 
 
 class AccelStats(C.Structure):
