@@ -28,11 +28,20 @@ PTD float ord2f(uint32_t u) { return __uint_as_float((u & 0x80000000u) ? (u & 0x
 static inline int grid_for(uint64_t n) { return (int)std::max<uint64_t>(1, std::min<uint64_t>((n + BB - 1) / BB, 4096)); }
 
 // ------------------------------------------------------------- bounds
+// Each box is widened outward by x -+ (|x| 2^-23 + 2^-126), at least one ulp,
+// so that a ray lying exactly in a bounding plane with a zero direction
+// component never meets 0 * inf in the slab test and no primitive whose
+// (inclusive) intersection test accepts it is culled.
+PTD float widen_lo(float x) { return x - (fabsf(x) * 0x1p-23f + 0x1p-126f); }
+PTD float widen_hi(float x) { return x + (fabsf(x) * 0x1p-23f + 0x1p-126f); }
+PTD void put_box(float4* lo, float4* hi, uint32_t i, float lx, float ly, float lz, float hx, float hy, float hz) {
+  lo[i] = make_float4(widen_lo(lx), widen_lo(ly), widen_lo(lz), 0.f);
+  hi[i] = make_float4(widen_hi(hx), widen_hi(hy), widen_hi(hz), 0.f);
+}
 __global__ void k_bounds_spheres(const float4* sph, uint32_t n, float4* lo, float4* hi) {
   for (uint32_t i = blockIdx.x * BB + threadIdx.x; i < n; i += gridDim.x * BB) {
     float4 s = sph[i];
-    lo[i] = make_float4(s.x - s.w, s.y - s.w, s.z - s.w, 0.f);
-    hi[i] = make_float4(s.x + s.w, s.y + s.w, s.z + s.w, 0.f);
+    put_box(lo, hi, i, s.x - s.w, s.y - s.w, s.z - s.w, s.x + s.w, s.y + s.w, s.z + s.w);
   }
 }
 __global__ void k_bounds_rects(const float4* rect, uint32_t n, float4* lo, float4* hi) {
@@ -40,26 +49,24 @@ __global__ void k_bounds_rects(const float4* rect, uint32_t n, float4* lo, float
     float4 A = rect[2 * i], B = rect[2 * i + 1];
     int axis = (int)__float_as_uint(B.y);
     float k = B.x;
-    if (axis == 0) { lo[i] = make_float4(k - 0.0001f, A.x, A.z, 0.f); hi[i] = make_float4(k + 0.0001f, A.y, A.w, 0.f); }
-    else if (axis == 1) { lo[i] = make_float4(A.x, k - 0.0001f, A.z, 0.f); hi[i] = make_float4(A.y, k + 0.0001f, A.w, 0.f); }
-    else { lo[i] = make_float4(A.x, A.z, k - 0.0001f, 0.f); hi[i] = make_float4(A.y, A.w, k + 0.0001f, 0.f); }
+    if (axis == 0) put_box(lo, hi, i, k - 0.0001f, A.x, A.z, k + 0.0001f, A.y, A.w);
+    else if (axis == 1) put_box(lo, hi, i, A.x, k - 0.0001f, A.z, A.y, k + 0.0001f, A.w);
+    else put_box(lo, hi, i, A.x, A.z, k - 0.0001f, A.y, A.w, k + 0.0001f);
   }
 }
 __global__ void k_bounds_boxes(const float4* box, uint32_t n, float4* lo, float4* hi) {
   for (uint32_t i = blockIdx.x * BB + threadIdx.x; i < n; i += gridDim.x * BB) {
     float4 a = box[2 * i], b = box[2 * i + 1];
-    lo[i] = make_float4(a.x - 0.001f, a.y - 0.001f, a.z - 0.001f, 0.f);
-    hi[i] = make_float4(b.x + 0.001f, b.y + 0.001f, b.z + 0.001f, 0.f);
+    put_box(lo, hi, i, a.x - 0.001f, a.y - 0.001f, a.z - 0.001f, b.x + 0.001f, b.y + 0.001f, b.z + 0.001f);
   }
 }
 __global__ void k_bounds_tris(const float4* vtx, const uint4* idx, uint32_t n, float4* lo, float4* hi) {
   for (uint32_t i = blockIdx.x * BB + threadIdx.x; i < n; i += gridDim.x * BB) {
     uint4 t = idx[i];
     float4 a = vtx[t.x], b = vtx[t.y], c = vtx[t.z];
-    lo[i] = make_float4(fminf(fminf(a.x, b.x), c.x) - 0.0001f, fminf(fminf(a.y, b.y), c.y) - 0.0001f,
-                        fminf(fminf(a.z, b.z), c.z) - 0.0001f, 0.f);
-    hi[i] = make_float4(fmaxf(fmaxf(a.x, b.x), c.x) + 0.0001f, fmaxf(fmaxf(a.y, b.y), c.y) + 0.0001f,
-                        fmaxf(fmaxf(a.z, b.z), c.z) + 0.0001f, 0.f);
+    put_box(lo, hi, i, fminf(fminf(a.x, b.x), c.x) - 0.0001f, fminf(fminf(a.y, b.y), c.y) - 0.0001f,
+            fminf(fminf(a.z, b.z), c.z) - 0.0001f, fmaxf(fmaxf(a.x, b.x), c.x) + 0.0001f,
+            fmaxf(fmaxf(a.y, b.y), c.y) + 0.0001f, fmaxf(fmaxf(a.z, b.z), c.z) + 0.0001f);
   }
 }
 // world bounds of an instance: its structure's root box, 8 corners through
@@ -85,8 +92,7 @@ __global__ void k_bounds_instances(const uint4* insts, const float4* blas_root, 
       wl = mk3(fminf(wl.x, q.x), fminf(wl.y, q.y), fminf(wl.z, q.z));
       wh = mk3(fmaxf(wh.x, q.x), fmaxf(wh.y, q.y), fmaxf(wh.z, q.z));
     }
-    lo[i] = make_float4(wl.x, wl.y, wl.z, 0.f);
-    hi[i] = make_float4(wh.x, wh.y, wh.z, 0.f);
+    put_box(lo, hi, i, wl.x, wl.y, wl.z, wh.x, wh.y, wh.z);
   }
 }
 

@@ -19,6 +19,7 @@ LIB_PATH = PKG_DIR / "libpt_hip.so"
 
 PT_OK = 0
 
+PT_AXIS_X, PT_AXIS_Y, PT_AXIS_Z = 0, 1, 2
 PT_GEOM_SPHERES, PT_GEOM_RECTS, PT_GEOM_BOXES, PT_GEOM_MESH = 0, 1, 2, 3
 PT_MAT_LAMBERTIAN, PT_MAT_METAL, PT_MAT_DIELECTRIC, PT_MAT_DIFFUSE_LIGHT = 0, 1, 2, 3
 PT_MAT_ASHIKHMIN_SHIRLEY, PT_MAT_TORRANCE_SPARROW, PT_MAT_OREN_NAYAR = 4, 5, 6
@@ -181,7 +182,8 @@ def load():
 
 
 def ptr(a: np.ndarray):
-    return C.c_void_p(a.ctypes.data) if a is not None else None
+    # data_as keeps a reference to the array for the lifetime of the pointer
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
 
 
 class PtError(RuntimeError):

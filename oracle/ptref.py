@@ -55,6 +55,12 @@ def load():
             "ptref_miss": (None, [P, P, P]),
             "ptref_hit_record": (None, [P, C.c_int32, C.c_int32, C.c_float, P, P, P, P]),
             "ptref_libm": (None, [C.c_int32, P, P, P, C.c_uint64]),
+            "ptref_schlick": (C.c_float, [C.c_float, C.c_float]),
+            "ptref_refract": (C.c_int32, [P, P, C.c_float, P]),
+            "ptref_ggx_d": (C.c_float, [P, C.c_float, C.c_float]),
+            "ptref_onb": (None, [P, P]),
+            "ptref_power_heuristic": (C.c_float, [C.c_float, C.c_float]),
+            "ptref_random_points": (C.c_uint32, [C.c_int32, C.c_uint32, C.c_uint32, P]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(lib, name)
@@ -65,7 +71,8 @@ def load():
 
 
 def p(a):
-    return C.c_void_p(a.ctypes.data) if a is not None else None
+    # data_as keeps a reference to the array for the lifetime of the pointer
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
 
 
 def f3(v):
@@ -203,3 +210,34 @@ def cosine_sample(u1, u2, n):
     out = np.zeros(3, np.float32)
     load().ptref_cosine_sample(u1, u2, p(f3(n)), p(out))
     return out
+
+
+def schlick(cosine, ior):
+    return float(load().ptref_schlick(cosine, ior))
+
+
+def refract(v, n, eta):
+    out = np.zeros(3, np.float32)
+    ok = load().ptref_refract(p(f3(v)), p(f3(n)), eta, p(out))
+    return bool(ok), out
+
+
+def ggx_d(h, nu, nv):
+    return float(load().ptref_ggx_d(p(f3(h)), nu, nv))
+
+
+def onb(n):
+    out = np.zeros((3, 3), np.float32)
+    load().ptref_onb(p(f3(n)), p(out))
+    return out
+
+
+def power_heuristic(f, g):
+    return float(load().ptref_power_heuristic(f, g))
+
+
+def random_points(kind, seed, n):
+    """kind: 0 in unit disk, 1 in unit sphere, 2 on unit sphere."""
+    out = np.zeros((n, 3), np.float32)
+    st = load().ptref_random_points({"disk": 0, "ball": 1, "sphere": 2}[kind], seed, n, p(out))
+    return out, st

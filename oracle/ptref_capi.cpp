@@ -271,4 +271,42 @@ void ptref_libm(int32_t fn, const float* x, const float* y, float* out, uint64_t
   }
 }
 
+// material.cuh:30-39 schlick
+float ptref_schlick(float cosine, float ior) { return schlick(cosine, ior); }
+
+// material.cuh:50-63 Refract; returns 1 and writes the refracted direction
+// when the discriminant is positive, else 0
+int32_t ptref_refract(const float* v, const float* n, float ni_over_nt, float* out) {
+  f3 r = mk(0.f);
+  bool ok = Refract(mk(v[0], v[1], v[2]), mk(n[0], n[1], n[2]), ni_over_nt, r);
+  out[0] = r.x;
+  out[1] = r.y;
+  out[2] = r.z;
+  return ok ? 1 : 0;
+}
+
+// microfacets.cuh:85-99 GGX_D (half vector against world +Y)
+float ptref_ggx_d(const float* h, float nu, float nv) { return GGX_D(mk(h[0], h[1], h[2]), nu, nv); }
+
+// the tangent frame around a unit normal: rows t, b, n
+void ptref_onb(const float* n, float* out) {
+  Onb o(mk(n[0], n[1], n[2]));
+  float v[9] = {o.t.x, o.t.y, o.t.z, o.b.x, o.b.y, o.b.z, o.n.x, o.n.y, o.n.z};
+  std::memcpy(out, v, sizeof(v));
+}
+
+// light_sample.cuh:27-33 PowerHeuristic(1, f, 1, g)
+float ptref_power_heuristic(float f, float g) { return PowerHeuristic(1, f, 1, g); }
+
+// random.cuh-based samplers: 0 unit disk, 1 in unit sphere, 2 on unit sphere
+uint32_t ptref_random_points(int32_t kind, uint32_t seed, uint32_t n, float* out) {
+  for (uint32_t i = 0; i < n; i++) {
+    f3 p = kind == 0 ? random_in_unit_disk(seed) : (kind == 1 ? random_in_unit_sphere(seed) : random_on_unit_sphere(seed));
+    out[3 * i] = p.x;
+    out[3 * i + 1] = p.y;
+    out[3 * i + 2] = p.z;
+  }
+  return seed;
+}
+
 }  // extern "C"

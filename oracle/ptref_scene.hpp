@@ -107,6 +107,20 @@ inline Aabb bounds_tri(f3 a, f3 bb, f3 c) {
   return r;
 }
 
+// Every box the traversal tests is widened outward by at least one ulp:
+// x -+ (|x| 2^-23 + 2^-126).  A ray lying exactly in a bounding plane with a
+// zero direction component then never meets 0 * inf in the slab test, so
+// the structure never culls a primitive whose intersection program accepts
+// points on its (inclusive) bounds.
+inline float widen_lo(float x) { return x - (std::fabs(x) * 0x1p-23f + 0x1p-126f); }
+inline float widen_hi(float x) { return x + (std::fabs(x) * 0x1p-23f + 0x1p-126f); }
+inline Aabb widen(const Aabb& b) {
+  Aabb r;
+  r.lo = mk(widen_lo(b.lo.x), widen_lo(b.lo.y), widen_lo(b.lo.z));
+  r.hi = mk(widen_hi(b.hi.x), widen_hi(b.hi.y), widen_hi(b.hi.z));
+  return r;
+}
+
 // Candidate acceptance shared by every intersector: tmin < t and either
 // t < best or (t == best and the (instance, primitive) key is smaller).
 struct HitState {
@@ -505,12 +519,12 @@ struct Scene {
 
   Aabb prim_bounds(int kind, uint32_t first, uint32_t p) const {
     switch (kind) {
-      case PT_GEOM_SPHERES: return bounds(spheres.at(first + p));
-      case PT_GEOM_RECTS: return bounds(rects.at(first + p));
-      case PT_GEOM_BOXES: return bounds(boxes.at(first + p));
+      case PT_GEOM_SPHERES: return widen(bounds(spheres.at(first + p)));
+      case PT_GEOM_RECTS: return widen(bounds(rects.at(first + p)));
+      case PT_GEOM_BOXES: return widen(bounds(boxes.at(first + p)));
       default: {
         const Mesh& m = meshes.at(first);
-        return bounds_tri(m.v.at(m.idx[3 * p]), m.v.at(m.idx[3 * p + 1]), m.v.at(m.idx[3 * p + 2]));
+        return widen(bounds_tri(m.v.at(m.idx[3 * p]), m.v.at(m.idx[3 * p + 1]), m.v.at(m.idx[3 * p + 2])));
       }
     }
   }
@@ -525,7 +539,7 @@ struct Scene {
       Aabb pb; pb.lo = q; pb.hi = q;
       w = merge(w, pb);
     }
-    return w;
+    return widen(w);
   }
 
   // object-space ray of an instance (identity instances are not transformed)
