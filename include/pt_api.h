@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define PT_API_VERSION 2
+#define PT_API_VERSION 3
 
 /* ------------------------------------------------------------------ status */
 typedef int32_t pt_status;
@@ -318,6 +318,14 @@ pt_status pt_read_accel(struct pt_context* ctx, uint32_t* nodes,
                         uint64_t* num_nodes, uint32_t* prim_order,
                         uint32_t* morton, uint64_t* num_prims,
                         uint32_t* blas_info, uint32_t* num_blas);
+
+/* Copies of the environment importance-sampling tables built for an
+ * environment light (no reference counterpart): cond = height x width
+ * inclusive prefix sums of 64-bit fixed-point texel weights per row, marg =
+ * prefix sums of the row totals; width = height = total = 0 without an
+ * environment light.  Any pointer may be NULL. */
+pt_status pt_read_env_tables(struct pt_context* ctx, uint64_t* cond, uint64_t* marg,
+                             uint32_t* width, uint32_t* height, uint64_t* total);
 
 /* Version of this ABI (PT_API_VERSION). */
 int32_t pt_api_version(void);

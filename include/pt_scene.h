@@ -18,6 +18,7 @@ extern "C" {
 
 #define PT_SCENE_RANDOM_SPHERES 0 /* random-spheres scene  */
 #define PT_SCENE_CORNELL 1        /* Cornell box with MIS  */
+#define PT_SCENE_MESHES 2         /* displaced meshes under an importance-sampled HDRI */
 
 typedef struct pt_scene_handle pt_scene_handle;
 
@@ -27,11 +28,15 @@ typedef struct pt_scene_info {
   uint32_t max_depth;
   int32_t russian;
   uint32_t rr_min_depth;
-  uint32_t reserved[2];
+  uint32_t flags;          /* PT_FLAG_* the workload renders with */
+  uint32_t reserved;
 } pt_scene_info;
 
-/* Build scene `config` for a width x height image (0 = the workload's size);
- * `variant` seeds the scene RNG where the scene is random. */
+/* Build scene `config` for a width x height image (0 = the workload's size).
+ * `variant`: the scene RNG seed of PT_SCENE_RANDOM_SPHERES; for
+ * PT_SCENE_MESHES 0 builds the full workload and 1..6 the same layout with
+ * octaspheres subdivided `variant` times and a 256x128 sky (small copies for
+ * tests). */
 pt_status pt_scene_build(int32_t config, uint32_t width, uint32_t height, uint32_t variant,
                          pt_scene_handle** out);
 /* Views owned by the handle, valid until pt_scene_free. */

@@ -284,12 +284,17 @@ pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
     if (t.type == PT_TEX_IMAGE && (t.width == 0 || t.height == 0 || !t.texels)) return bad("empty image texture");
     if (t.type != PT_TEX_IMAGE && t.type != PT_TEX_CONSTANT) return bad("invalid texture type");
   }
-  for (uint32_t i = 0; i < d->num_lights; i++)
-    if (d->lights[i].type != PT_LIGHT_RECT && d->lights[i].type != PT_LIGHT_SPHERE)
-      return fail(c, PT_ERR_UNSUPPORTED, "light type not supported");
   if (d->miss.type < 0 || d->miss.type > 3) return bad("invalid miss program");
   if ((d->miss.type == PT_MISS_IMAGE || d->miss.type == PT_MISS_HDR) && !tex_ok(d->miss.texture))
     return bad("miss texture index out of range");
+  for (uint32_t i = 0; i < d->num_lights; i++) {
+    const int32_t t = d->lights[i].type;
+    if (t != PT_LIGHT_RECT && t != PT_LIGHT_SPHERE && t != PT_LIGHT_ENV)
+      return fail(c, PT_ERR_UNSUPPORTED, "light type not supported");
+    if (t == PT_LIGHT_ENV && (d->miss.type != PT_MISS_HDR || !d->miss.spherical ||
+                              d->textures[d->miss.texture].type != PT_TEX_IMAGE))
+      return bad("an environment light needs a spherical HDR miss texture");
+  }
 
   // ---- host copies
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -400,11 +405,12 @@ pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
     for (size_t k = 0; k + 3 < tx.size(); k += 4) texel_pool.push_back(make_float4(tx[k], tx[k + 1], tx[k + 2], tx[k + 3]));
     texs.push_back(dt);
   }
-  std::vector<DevLight> lts;
+  std::vector<DevLight> lts, lts_plain;
   for (auto& l : c->lights) {
     DevLight dl{l.type, l.axis, l.a0, l.a1, l.b0, l.b1, l.k, l.center[0], l.center[1], l.center[2], l.radius,
                 l.emission[0], l.emission[1], l.emission[2], {0.f, 0.f}};
     lts.push_back(dl);
+    if (l.type != PT_LIGHT_ENV) lts_plain.push_back(dl);
   }
   auto& L = c->scene_bufs;
   HIPCHK(c, upload(c, L, const_cast<float4**>(&s.sph_o), sph_o.data(), sph_o.size() * sizeof(float4)));
@@ -423,8 +429,33 @@ pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
   HIPCHK(c, upload(c, L, const_cast<DevTexture**>(&s.texs), texs.data(), texs.size() * sizeof(DevTexture)));
   HIPCHK(c, upload(c, L, const_cast<float4**>(&s.texels), texel_pool.data(), texel_pool.size() * sizeof(float4)));
   HIPCHK(c, upload(c, L, const_cast<DevLight**>(&s.lights), lts.data(), lts.size() * sizeof(DevLight)));
+  HIPCHK(c, upload(c, L, const_cast<DevLight**>(&s.lights_plain), lts_plain.data(),
+                   lts_plain.size() * sizeof(DevLight)));
   s.num_lights = (int32_t)lts.size();
+  s.num_lights_plain = (int32_t)lts_plain.size();
   s.miss = c->miss;
+  // environment importance-sampling tables, built on the GPU
+  s.env_cond = s.env_marg = nullptr;
+  s.env_total = 0;
+  s.env_w = s.env_h = 0;
+  if (lts.size() != lts_plain.size()) {
+    const DevTexture& t = texs[c->miss.texture];
+    unsigned long long *cond = nullptr, *marg = nullptr, *scratch = nullptr;
+    hipError_t e;
+    if ((e = dalloc(L, (void**)&cond, (size_t)t.w * t.h * 8)) != hipSuccess ||
+        (e = dalloc(L, (void**)&marg, (size_t)t.h * 8)) != hipSuccess ||
+        (e = dalloc(L, (void**)&scratch, (size_t)(t.h + 1) * 8)) != hipSuccess)
+      return hip_fail(c, e, "hipMalloc (environment tables)");
+    launch_env_tables(s.texels + t.offset, t.w, t.h, cond, marg, scratch, c->stream);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    unsigned long long total = 0;
+    HIPCHK(c, hipMemcpy(&total, marg + (t.h - 1), 8, hipMemcpyDeviceToHost));
+    s.env_cond = cond;
+    s.env_marg = marg;
+    s.env_total = total;
+    s.env_w = t.w;
+    s.env_h = t.h;
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->has_scene = true;
   return PT_OK;
@@ -751,6 +782,7 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
   if (frame_count == 0) return PT_OK;
   uint32_t fpb = rp->frames_per_batch ? rp->frames_per_batch : std::max<uint32_t>(1, PATHS_TARGET / WH);
   fpb = std::min(fpb, frame_count);
+  if ((uint64_t)fpb * WH > SLOT_MASK) return fail(c, PT_ERR_INVALID_ARG, "frames_per_batch x W x H exceeds 2^31 - 1");
   pt_status ps = ensure_paths(c, (uint64_t)fpb * WH);
   if (ps != PT_OK) return ps;
 
@@ -883,6 +915,21 @@ pt_status pt_read_display(pt_context* c, uint8_t* rgba) {
   launch_display(c->accum, n, c->frames, c->rgba, grid_for(n), c->stream);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipMemcpy(rgba, c->rgba, (size_t)n * 4, hipMemcpyDeviceToHost));
+  return PT_OK;
+}
+
+pt_status pt_read_env_tables(pt_context* c, uint64_t* cond, uint64_t* marg, uint32_t* width, uint32_t* height,
+                             uint64_t* total) {
+  if (!c) return PT_ERR_INVALID_ARG;
+  if (!c->has_scene) return fail(c, PT_ERR_STATE, "no scene");
+  (void)hipSetDevice(c->device);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const DevScene& s = c->dsc;
+  if (width) *width = s.env_w;
+  if (height) *height = s.env_h;
+  if (total) *total = s.env_total;
+  if (cond && s.env_cond) HIPCHK(c, hipMemcpy(cond, s.env_cond, (size_t)s.env_w * s.env_h * 8, hipMemcpyDeviceToHost));
+  if (marg && s.env_marg) HIPCHK(c, hipMemcpy(marg, s.env_marg, (size_t)s.env_h * 8, hipMemcpyDeviceToHost));
   return PT_OK;
 }
 

@@ -69,6 +69,11 @@ PTD uint32_t tea64(uint32_t val0, uint32_t val1) {
   }
   return v0;
 }
+// programs/random.cuh:50-55: the 24-bit LCG output itself
+PTD uint32_t lcg24(uint32_t& prev) {
+  prev = 1664525u * prev + 1013904223u;
+  return prev & 0x00FFFFFFu;
+}
 // programs/random.cuh:50-65: 24-bit LCG output / 2^24 (exact in fp32)
 PTD float rnd(uint32_t& prev) {
   prev = 1664525u * prev + 1013904223u;

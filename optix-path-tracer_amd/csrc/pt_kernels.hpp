@@ -34,6 +34,10 @@ void launch_shade(const DevScene& sc, const DevPaths& P, int in, bool sorted, in
                   hipStream_t st);
 int shade_material_set(uint32_t types_mask);
 void launch_bounce_reset(const DevPaths& P, int in, hipStream_t st);
+// environment importance-sampling tables of a W x H float4 map; scratch
+// holds 1 + H 64-bit words
+void launch_env_tables(const float4* tex, uint32_t W, uint32_t H, unsigned long long* cond,
+                       unsigned long long* marg, unsigned long long* scratch, hipStream_t st);
 void launch_exhausted(const DevPaths& P, int in, uint64_t* stats, int grid, hipStream_t st);
 void launch_accumulate(const DevPaths& P, const RenderConsts& rc, uint32_t nframes, double* accum, int grid,
                        hipStream_t st);

@@ -103,9 +103,18 @@ struct DevScene {
   const DevMaterial* mats;
   const DevTexture* texs;
   const float4* texels;
-  const DevLight* lights;
+  const DevLight* lights;        // every light, in scene order
   int32_t num_lights;
+  const DevLight* lights_plain;  // the same without environment lights
+  int32_t num_lights_plain;
   pt_miss_desc miss;
+  // environment importance sampling (env_total == 0: none): inclusive
+  // prefix sums of 64-bit fixed-point texel weights per row (env_cond,
+  // env_h x env_w) and over row totals (env_marg)
+  const unsigned long long* env_cond;
+  const unsigned long long* env_marg;
+  unsigned long long env_total;
+  uint32_t env_w, env_h;
 };
 
 struct DevCamera {
@@ -116,10 +125,12 @@ struct DevCamera {
 // Path state in HBM: 64 bytes per live path, structure of arrays, compacted
 // every bounce into the next ping-pong set so that every kernel streams it
 // contiguously.
+constexpr uint32_t SLOT_MASK = 0x7FFFFFFFu;
 struct PathState {
   float4* o;  // (origin.xyz, RNG state bits)
-  float4* d;  // (direction.xyz, slot bits: frame-in-batch * W*H + launch index)
-  float4* T;  // (throughput.xyz, flags bits: bit0 previous bounce specular)
+  float4* d;  // (direction.xyz, slot | last bounce specular << 31; slot =
+              //  frame-in-batch * W*H + launch index)
+  float4* T;  // (throughput.xyz, pdf of the last diffuse continuation sample)
   float4* L;  // (radiance.xyz, decision-signature bits)
 };
 

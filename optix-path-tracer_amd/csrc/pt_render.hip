@@ -94,7 +94,7 @@ __global__ __launch_bounds__(BLOCK) void k_raygen(DevPaths P, RenderConsts rc, u
     (void)rnd(seed);  // prd.time (raygen.cu:69), consumed but unused
     S.o[s] = make_float4(origin.x, origin.y, origin.z, __uint_as_float(seed));
     S.d[s] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(s));
-    S.T[s] = make_float4(1.f, 1.f, 1.f, __uint_as_float(0u));
+    S.T[s] = make_float4(1.f, 1.f, 1.f, 1.f);
     S.L[s] = make_float4(0.f, 0.f, 0.f, __uint_as_float(0x811C9DC5u));
     P.result[s] = make_float4(0.f, 0.f, 0.f, __uint_as_float(rc.max_depth));
   }
@@ -215,7 +215,8 @@ __global__ __launch_bounds__(BLOCK) void k_sort_scatter(DevPaths P, int in) {
 // Returns false when the path ended (result written).
 template <int MS>
 PTD bool scatter_and_roulette(const Surface& f, v3 Wo, v3 N, uint32_t& seed, uint32_t& sig, v3& T, v3 L, v3& dir,
-                              uint32_t& flags, uint32_t depth, const RenderConsts& rc, float4& result) {
+                              uint32_t& flags, float& prev_pdf, uint32_t depth, const RenderConsts& rc,
+                              float4& result) {
   bool spec = false;
   if ((has_mat<MS>(PT_MAT_LAMBERTIAN) || has_mat<MS>(PT_MAT_OREN_NAYAR)) &&
       (f.type == PT_MAT_LAMBERTIAN || f.type == PT_MAT_OREN_NAYAR)) {
@@ -225,6 +226,7 @@ PTD bool scatter_and_roulette(const Surface& f, v3 Wo, v3 N, uint32_t& seed, uin
     v3 att = brdf_eval<MS>(f, Wo, Wi, N, pdf);
     dir = Wi;
     T = T * clamp3(att / pdf, 0.f, 1.f);
+    prev_pdf = pdf;
   } else if (has_mat<MS>(PT_MAT_METAL) && f.type == PT_MAT_METAL) {  // metal.cu:38-56
     v3 refl = reflect(-Wo, N);
     dir = refl + f.p0 * random_in_unit_sphere(seed);
@@ -302,6 +304,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
   const uint32_t n = P.counters[CTR_ACTIVE + in];
   const PathState S = P.st[src], D = P.st[out];
   uint32_t* out_count = P.counters + CTR_ACTIVE + out;
+  // light table: environment lights take part only with PT_FLAG_ENV_MIS
+  const bool env_mis = (rc.flags & PT_FLAG_ENV_MIS) != 0;
+  const DevLight* lights = env_mis ? sc.lights : sc.lights_plain;
+  const int nl = env_mis ? sc.num_lights : sc.num_lights_plain;
+  const bool env_light = env_mis && sc.env_total != 0 && sc.num_lights != sc.num_lights_plain;
   TraceCounts tc{0, 0, 0};
   uint32_t shadow_rays = 0;
   for (uint32_t i0 = blockIdx.x * BLOCK; i0 < n; i0 += gridDim.x * BLOCK) {
@@ -312,16 +319,28 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
       float4 hit = hits[i];
       float4 o4 = S.o[i], d4 = S.d[i], t4 = S.T[i], l4 = S.L[i];
       v3 d = mk3(d4), T = mk3(t4), L = mk3(l4);
-      uint32_t seed = __float_as_uint(o4.w), slot = __float_as_uint(d4.w);
-      uint32_t flags = __float_as_uint(t4.w), sig = __float_as_uint(l4.w);
+      uint32_t seed = __float_as_uint(o4.w), dw = __float_as_uint(d4.w);
+      uint32_t slot = dw & SLOT_MASK, flags = dw >> 31, sig = __float_as_uint(l4.w);
+      float prev_pdf = t4.w;
       int pos = __float_as_int(hit.y), prim = __float_as_int(hit.z), inst = __float_as_int(hit.w);
       sig = sig_mix(sig, pos >= 0 ? (((uint32_t)inst << 20) ^ (uint32_t)prim) : 0xFFFFFFFFu);
       float4 res = make_float4(0.f, 0.f, 0.f, 0.f);
       if (pos < 0) {
-        // miss programs (miss.cu), then color() returns L + clamp(T) (raygen.cu:81-82)
-        T = T * miss_color(sc, d);
-        v3 r = L + clamp3(T, 0.f, 1.f);
-        res = make_float4(r.x, r.y, r.z, __uint_as_float(depth + 1));
+        if (env_mis && sc.miss.type == PT_MISS_HDR) {
+          // unclamped environment radiance, MIS-weighted against the light
+          // sample of the previous (diffuse) bounce
+          v3 Le = env_radiance(sc, d);
+          float w = 1.f;
+          if (env_light && depth > 0 && !(flags & 1u)) w = power_heuristic(prev_pdf, env_pdf(sc, d));
+          T = T * (Le * w);
+          v3 r = L + T;
+          res = make_float4(r.x, r.y, r.z, __uint_as_float(depth + 1));
+        } else {
+          // miss programs (miss.cu), then color() returns L + clamp(T) (raygen.cu:81-82)
+          T = T * miss_color(sc, d);
+          v3 r = L + clamp3(T, 0.f, 1.f);
+          res = make_float4(r.x, r.y, r.z, __uint_as_float(depth + 1));
+        }
       } else {
         int mat = hit_material(sc, pos, prim);
         const DevMaterial& M = sc.mats[mat];
@@ -333,14 +352,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
                     (f.type == PT_MAT_LAMBERTIAN || f.type == PT_MAT_OREN_NAYAR)) || is_light;
         if (nee) {
           v3 direct = mk3(0.f);
-          int nl = sc.num_lights;
           if (nl > 0) {
             int index = ((int)(rnd(seed) * nl)) % nl;
             if (!(is_light && nl == 1)) {
-              const DevLight& Lt = sc.lights[index];
-              v3 emission = light_emission(Lt);
-              v3 Wi = light_sample(Lt, r.P, seed);
-              float lightPDF = light_pdf(Lt, r.P, Wi);
+              const DevLight& Lt = lights[index];
+              const bool env = Lt.type == PT_LIGHT_ENV;
+              v3 Wi = env ? env_sample(sc, seed) : light_sample(Lt, r.P, seed);
+              float lightPDF = env ? env_pdf(sc, Wi) : light_pdf(Lt, r.P, Wi);
+              v3 emission = env ? env_radiance(sc, Wi) : light_emission(Lt);
               if (dot(Wi, r.N) < 0.f) {
                 sig = sig_mix(sig, 0x33333333u);
               } else {
@@ -348,8 +367,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
                 HitOut sh = trace_closest<COUNT>(sc, r.P, Wi, 1e-3f, FINF, st, tc);
                 bool occluded = false;
                 if (sh.inst >= 0)
-                  occluded = !(sc.mats[hit_material(sc, sh.pos, sh.prim)].type == PT_MAT_DIFFUSE_LIGHT &&
-                               dot(r.N, Wi) > 0.f);
+                  occluded = env || !(sc.mats[hit_material(sc, sh.pos, sh.prim)].type == PT_MAT_DIFFUSE_LIGHT &&
+                                      dot(r.N, Wi) > 0.f);
                 sig = sig_mix(sig, occluded ? 0xA5A5A5A5u : 0x5A5A5A5Au);
                 if (!occluded) {
                   if (lightPDF != 0.f && !isNull(emission)) {
@@ -360,14 +379,16 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
                       direct = direct + mv * emission * w / lightPDF;
                     }
                   }
-                  v3 Wb = brdf_sample<MS>(f, r.Wo, r.N, seed);
-                  float matPDF = 1.f;
-                  v3 mv = brdf_eval<MS>(f, r.Wo, Wb, r.N, matPDF);
-                  if (matPDF != 0.f && !isNull(mv)) {
-                    float lp = light_pdf(Lt, r.P, Wb);
-                    if (lp != 0.f && !isNull(emission)) {
-                      float w = power_heuristic(matPDF, lp);
-                      direct = direct + mv * emission * w / matPDF;
+                  if (!env) {
+                    v3 Wb = brdf_sample<MS>(f, r.Wo, r.N, seed);
+                    float matPDF = 1.f;
+                    v3 mv = brdf_eval<MS>(f, r.Wo, Wb, r.N, matPDF);
+                    if (matPDF != 0.f && !isNull(mv)) {
+                      float lp = light_pdf(Lt, r.P, Wb);
+                      if (lp != 0.f && !isNull(emission)) {
+                        float w = power_heuristic(matPDF, lp);
+                        direct = direct + mv * emission * w / matPDF;
+                      }
                     }
                   }
                 }
@@ -383,11 +404,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
           res = make_float4(L.x, L.y, L.z, __uint_as_float(depth + 1));
         } else {
           v3 ndir;
-          alive = scatter_and_roulette<MS>(f, r.Wo, r.N, seed, sig, T, L, ndir, flags, depth, rc, res);
+          alive = scatter_and_roulette<MS>(f, r.Wo, r.N, seed, sig, T, L, ndir, flags, prev_pdf, depth, rc, res);
           if (alive) {
             no = make_float4(r.P.x, r.P.y, r.P.z, __uint_as_float(seed));
-            nd = make_float4(ndir.x, ndir.y, ndir.z, __uint_as_float(slot));
-            nT = make_float4(T.x, T.y, T.z, __uint_as_float(flags));
+            nd = make_float4(ndir.x, ndir.y, ndir.z, __uint_as_float(slot | (flags << 31)));
+            nT = make_float4(T.x, T.y, T.z, prev_pdf);
             nL = make_float4(L.x, L.y, L.z, __uint_as_float(sig));
           }
         }
@@ -446,7 +467,7 @@ __global__ __launch_bounds__(BLOCK) void k_exhausted(DevPaths P, int in, uint64_
   if (blockIdx.x == 0 && threadIdx.x == 0 && n)
     atomicAdd((unsigned long long*)&stats[STAT_EXHAUSTED], (unsigned long long)n);
   for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK)
-    P.sig_final[__float_as_uint(S.d[i].w)] = __float_as_uint(S.L[i].w);
+    P.sig_final[__float_as_uint(S.d[i].w) & SLOT_MASK] = __float_as_uint(S.L[i].w);
 }
 
 // raygen.cu:126-134 de_nan and raygen.cu:166 acc_buffer += (rgb, 1), in FP64

@@ -128,6 +128,79 @@ PTD Surface make_surface(const DevScene& sc, const HitRec& r) {
   return f;
 }
 
+// ------------------------------------------------------- environment
+// Spherical HDR mapping of miss.cu:64-95: u = atan2(z, x) / 2pi,
+// v = acos(y / |d|) / pi (u wraps through the texture's repeat mode).
+PTD void env_uv(v3 d, float& u, float& v) {
+  float r = length(d);
+  float lon = patan2(d.z, d.x);
+  float lat = pacos(d.y / r);
+  u = lon * (1.f / (PI_F * 2.f));
+  v = lat * (1.f / PI_F);
+}
+// radiance the HDR miss program returns toward d (x2, miss.cu:93)
+PTD v3 env_radiance(const DevScene& sc, v3 d) {
+  float u, v;
+  env_uv(d, u, v);
+  return 2.f * tex_eval(sc, sc.miss.texture, u, v);
+}
+// floor(total * r / 2^24) for a 24-bit r, exactly
+PTD unsigned long long scale24(unsigned long long total, uint32_t r) {
+  return (total >> 24) * r + (((total & 0xFFFFFFull) * r) >> 24);
+}
+// first index in [0, n) whose inclusive prefix exceeds target
+PTD uint32_t upper_index(const unsigned long long* a, uint32_t n, unsigned long long target) {
+  uint32_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] > target) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+// Importance sampling of the environment (no reference counterpart):
+// density per unit solid angle p(u, v) W H / (2 pi^2 sin(theta)), p the
+// texel's share of the fixed-point weight total.
+PTD float env_pdf(const DevScene& sc, v3 d) {
+  if (sc.env_total == 0) return 0.f;
+  float u, v;
+  env_uv(d, u, v);
+  float uw = u - floorf(u);
+  long long i = (long long)(uw * (float)sc.env_w), j = (long long)(v * (float)sc.env_h);
+  if (i >= (long long)sc.env_w) i = sc.env_w - 1;
+  if (j >= (long long)sc.env_h) j = sc.env_h - 1;
+  if (i < 0) i = 0;
+  if (j < 0) j = 0;
+  const unsigned long long* row = sc.env_cond + (size_t)j * sc.env_w;
+  unsigned long long w = row[i] - (i ? row[i - 1] : 0ull);
+  float st = psin(v * PI_F);
+  if (w == 0 || !(st > 0.f)) return 0.f;
+  double p = (double)w / (double)sc.env_total * (double)sc.env_w * (double)sc.env_h;
+  return (float)(p / (2.0 * 3.14159265358979323846 * 3.14159265358979323846 * (double)st));
+}
+// two draws: row from the marginal table, column from the row's table,
+// continuous offsets inside the texel
+PTD v3 env_sample(const DevScene& sc, uint32_t& seed) {
+  uint32_t r1 = lcg24(seed);
+  uint32_t r2 = lcg24(seed);
+  if (sc.env_total == 0) return mk3(0.f, 1.f, 0.f);
+  unsigned long long tr = scale24(sc.env_total, r1);
+  uint32_t j = upper_index(sc.env_marg, sc.env_h, tr);
+  unsigned long long rprev = j ? sc.env_marg[j - 1] : 0ull;
+  unsigned long long rw = sc.env_marg[j] - rprev;
+  const unsigned long long* row = sc.env_cond + (size_t)j * sc.env_w;
+  unsigned long long tc = scale24(rw, r2);
+  uint32_t i = upper_index(row, sc.env_w, tc);
+  unsigned long long cprev = i ? row[i - 1] : 0ull;
+  float fu = (float)((double)(tc - cprev) / (double)(row[i] - cprev));
+  float fv = (float)((double)(tr - rprev) / (double)rw);
+  float u = ((float)i + fu) / (float)sc.env_w;
+  float v = ((float)j + fv) / (float)sc.env_h;
+  float lon = 2.f * PI_F * u, lat = PI_F * v;
+  float sl = psin(lat);
+  return mk3(sl * pcos(lon), pcos(lat), sl * psin(lon));
+}
+
 // miss.cu:30-95
 PTD v3 miss_color(const DevScene& sc, v3 d) {
   const pt_miss_desc& m = sc.miss;
@@ -149,11 +222,7 @@ PTD v3 miss_color(const DevScene& sc, v3 d) {
   }
   float u, v;
   if (m.spherical) {
-    float r = length(d);
-    float lon = patan2(d.z, d.x);
-    float lat = pacos(d.y / r);
-    u = lon * (1.f / (PI_F * 2.f));
-    v = lat * (1.f / PI_F);
+    env_uv(d, u, v);
   } else {
     float theta = patan2(d.x, d.z);
     theta = theta < 0.f ? theta + (2.f * PI_F) : theta;

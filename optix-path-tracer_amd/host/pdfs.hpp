@@ -51,3 +51,18 @@ struct Sphere_PDF : public PDF {
   float radius;
   float3 center;
 };
+
+// Importance sampling of the environment map set by setMissProgram(HDR, ...).
+// No counterpart in the reference, which only looks the map up on a miss
+// (miss.cu:64-95): with PT_FLAG_ENV_MIS the renderer samples the map in
+// proportion to luminance x sin(theta) and weights misses by MIS.  The
+// radiance is the miss program's own (x2); the emission entry is unused.
+struct Environment_PDF : public PDF {
+  pt_light_desc desc() const {
+    pt_light_desc l{};
+    l.type = PT_LIGHT_ENV;
+    return l;
+  }
+  Program createSample(Context&) const override { return light_program(ptfacade::ProgramObj::LIGHT_SAMPLE, desc()); }
+  Program createPDF(Context&) const override { return light_program(ptfacade::ProgramObj::LIGHT_PDF, desc()); }
+};

@@ -32,6 +32,14 @@ pt_status pt_scene_build(int32_t config, uint32_t width, uint32_t height, uint32
     switch (config) {
       case PT_SCENE_RANDOM_SPHERES: d = ConfigDefaults{400, 267, 16, 50, true, 11}; break;
       case PT_SCENE_CORNELL: d = ConfigDefaults{512, 512, 64, 16, true, 3}; break;
+      case PT_SCENE_MESHES:
+        d = ConfigDefaults{1920, 1080, 256, 12, true, 11};
+        if (variant > 6) {
+          delete h;
+          g_scene_err = "mesh scene variant must be 0..6";
+          return PT_ERR_INVALID_ARG;
+        }
+        break;
       default:
         delete h;
         g_scene_err = "unknown scene config";
@@ -39,8 +47,10 @@ pt_status pt_scene_build(int32_t config, uint32_t width, uint32_t height, uint32
     }
     h->app.W = width ? (int)width : d.W;
     h->app.H = height ? (int)height : d.H;
-    if (config == PT_SCENE_RANDOM_SPHERES) RandomSpheres(h->app, h->arena, variant);
-    else CornellBox(h->app, h->arena);
+    if (config == PT_SCENE_RANDOM_SPHERES) d = RandomSpheres(h->app, h->arena, variant);
+    else if (config == PT_SCENE_CORNELL) d = CornellBox(h->app, h->arena);
+    else if (variant == 0) d = MeshScene(h->app, h->arena);
+    else d = MeshScene(h->app, h->arena, (int)variant, 256, 128);
     h->app.context->rec.flatten(h->flat);
     h->camera = h->app.context->rec.camera();
     h->info.width = (uint32_t)h->app.W;
@@ -49,6 +59,7 @@ pt_status pt_scene_build(int32_t config, uint32_t width, uint32_t height, uint32
     h->info.max_depth = (uint32_t)d.depth;
     h->info.russian = d.russian ? 1 : 0;
     h->info.rr_min_depth = (uint32_t)d.rr_min_depth;
+    h->info.flags = d.flags;
   } catch (const char* e) {
     g_scene_err = e;
     delete h;

@@ -30,7 +30,7 @@ PT_FLAG_MATERIAL_SORT = 0x1
 PT_FLAG_KEEP_PATHS = 0x2
 PT_FLAG_ENV_MIS = 0x4
 PT_FLAG_COUNT_TRAVERSAL = 0x8
-PT_SCENE_RANDOM_SPHERES, PT_SCENE_CORNELL = 0, 1
+PT_SCENE_RANDOM_SPHERES, PT_SCENE_CORNELL, PT_SCENE_MESHES = 0, 1, 2
 
 
 class SphereDesc(C.Structure):
@@ -123,7 +123,7 @@ class AccelStats(C.Structure):
 class SceneInfo(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
                 ("max_depth", C.c_uint32), ("russian", C.c_int32), ("rr_min_depth", C.c_uint32),
-                ("reserved", C.c_uint32 * 2)]
+                ("flags", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 RAY_DTYPE = np.dtype([("org", np.float32, 3), ("tmin", np.float32), ("dir", np.float32, 3),
@@ -156,6 +156,8 @@ def _declare(lib):
         "pt_trace_occluded": (C.c_int32, [P, P, C.c_uint64, P]),
         "pt_read_accel": (C.c_int32, [P, P, C.POINTER(C.c_uint64), P, P, C.POINTER(C.c_uint64), P,
                                       C.POINTER(C.c_uint32)]),
+        "pt_read_env_tables": (C.c_int32, [P, P, P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                           C.POINTER(C.c_uint64)]),
         "pt_scene_build": (C.c_int32, [C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32,
                                        C.POINTER(C.c_void_p)]),
         "pt_scene_get_desc": (C.POINTER(SceneDesc), [P]),
@@ -216,10 +218,12 @@ class Scene:
         except Exception:
             pass
 
-    def render_params(self, flags=PT_FLAG_MATERIAL_SORT, frames_per_batch=0):
+    def render_params(self, flags=None, frames_per_batch=0):
+        """Render parameters of the workload; `flags` defaults to the
+        scene's own PT_FLAG_* set."""
         i = self.info
         return RenderParams(i.width, i.height, i.max_depth, i.russian, i.rr_min_depth,
-                            frames_per_batch, flags, 0)
+                            frames_per_batch, i.flags if flags is None else flags, 0)
 
 
 class Context:
@@ -304,6 +308,19 @@ class Context:
         self._check(self.lib.pt_trace_occluded(self.h, ptr(rays), len(rays), ptr(occ)),
                     "pt_trace_occluded")
         return occ
+
+    def read_env_tables(self):
+        """Environment importance-sampling tables: (cond (H, W) uint64,
+        marg (H,) uint64, total)."""
+        w, h, tot = C.c_uint32(), C.c_uint32(), C.c_uint64()
+        self._check(self.lib.pt_read_env_tables(self.h, None, None, C.byref(w), C.byref(h), C.byref(tot)),
+                    "pt_read_env_tables")
+        cond = np.zeros((h.value, w.value), np.uint64)
+        marg = np.zeros(h.value, np.uint64)
+        if w.value:
+            self._check(self.lib.pt_read_env_tables(self.h, ptr(cond), ptr(marg), C.byref(w), C.byref(h),
+                                                    C.byref(tot)), "pt_read_env_tables")
+        return cond, marg, tot.value
 
     def read_accel(self):
         nn, npm, nb = C.c_uint64(), C.c_uint64(), C.c_uint32()

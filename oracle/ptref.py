@@ -56,6 +56,8 @@ def load():
             "ptref_hit_record": (None, [P, C.c_int32, C.c_int32, C.c_float, P, P, P, P]),
             "ptref_libm": (None, [C.c_int32, P, P, P, C.c_uint64]),
             "ptref_schlick": (C.c_float, [C.c_float, C.c_float]),
+            "ptref_env_tables": (None, [P, P, P, P]),
+            "ptref_env_radiance": (None, [P, P, P]),
             "ptref_refract": (C.c_int32, [P, P, C.c_float, P]),
             "ptref_ggx_d": (C.c_float, [P, C.c_float, C.c_float]),
             "ptref_onb": (None, [P, P]),
@@ -157,6 +159,21 @@ class OracleScene:
 
     def light_pdf(self, li, P, wi):
         return float(self.lib.ptref_light_pdf(self.h, li, p(f3(P)), p(f3(wi))))
+
+    def env_tables(self):
+        sz = np.zeros(3, np.uint64)
+        self.lib.ptref_env_tables(self.h, p(sz), None, None)
+        W, H, total = (int(x) for x in sz)
+        cond = np.zeros((H, W), np.uint64)
+        marg = np.zeros(H, np.uint64)
+        if W and H:
+            self.lib.ptref_env_tables(self.h, p(sz), p(cond), p(marg))
+        return cond, marg, total
+
+    def env_radiance(self, d):
+        out = np.zeros(3, np.float32)
+        self.lib.ptref_env_radiance(self.h, p(f3(d)), p(out))
+        return out
 
     def miss(self, d):
         out = np.zeros(3, np.float32)

@@ -138,7 +138,8 @@ int32_t ptref_render(const void* sp, const pt_camera* cam, const pt_render_param
                      float* last_rgb, uint32_t* last_sig, uint32_t* last_bounces, uint64_t* stats) {
   try {
     const Scene& s = *static_cast<const Scene*>(sp);
-    Params p{rp->width, rp->height, rp->max_depth, rp->russian != 0, rp->rr_min_depth};
+    Params p{rp->width, rp->height, rp->max_depth, rp->russian != 0, rp->rr_min_depth,
+             (rp->flags & PT_FLAG_ENV_MIS) != 0};
     uint64_t npix = (uint64_t)p.W * p.H;
     if (pix_end > npix) pix_end = npix;
     if (nthreads < 1) nthreads = 1;
@@ -225,17 +226,37 @@ void ptref_cosine_sample(float u1, float u2, const float* n, float* out) {
   f3 w = Onb(mk(n[0], n[1], n[2])).to_world(cosine_sample_hemisphere(u1, u2));
   out[0] = w.x; out[1] = w.y; out[2] = w.z;
 }
-// light sample + pdf for light index `li` of the scene
+// light sample + pdf for light index `li` of the scene (environment lights
+// through the importance-sampling tables)
 void ptref_light_sample(const void* sp, int32_t li, const float* P, uint32_t* seed, float* wi, float* pdf) {
   const Scene& s = *static_cast<const Scene*>(sp);
   f3 p = mk(P[0], P[1], P[2]);
-  f3 w = light_sample(s.lights[li], p, *seed);
+  const Light& L = s.lights[li];
+  f3 w = L.type == PT_LIGHT_ENV ? env_sample(s, *seed) : light_sample(L, p, *seed);
   wi[0] = w.x; wi[1] = w.y; wi[2] = w.z;
-  *pdf = light_pdf(s.lights[li], p, w);
+  *pdf = L.type == PT_LIGHT_ENV ? env_pdf(s, w) : light_pdf(L, p, w);
 }
 float ptref_light_pdf(const void* sp, int32_t li, const float* P, const float* wi) {
   const Scene& s = *static_cast<const Scene*>(sp);
-  return light_pdf(s.lights[li], mk(P[0], P[1], P[2]), mk(wi[0], wi[1], wi[2]));
+  const Light& L = s.lights[li];
+  f3 w = mk(wi[0], wi[1], wi[2]);
+  return L.type == PT_LIGHT_ENV ? env_pdf(s, w) : light_pdf(L, mk(P[0], P[1], P[2]), w);
+}
+// environment tables: out[0] = W, out[1] = H, out[2] = total; cond (H*W)
+// and marg (H) copied when non-null
+void ptref_env_tables(const void* sp, uint64_t* out, uint64_t* cond, uint64_t* marg) {
+  const Scene& s = *static_cast<const Scene*>(sp);
+  out[0] = s.env.W;
+  out[1] = s.env.H;
+  out[2] = s.env.total;
+  if (cond && !s.env.cond.empty()) std::memcpy(cond, s.env.cond.data(), s.env.cond.size() * 8);
+  if (marg && !s.env.marg.empty()) std::memcpy(marg, s.env.marg.data(), s.env.marg.size() * 8);
+}
+// radiance of the environment (the HDR miss program) toward d
+void ptref_env_radiance(const void* sp, const float* d, float* out) {
+  const Scene& s = *static_cast<const Scene*>(sp);
+  f3 c = env_radiance(s, mk(d[0], d[1], d[2]));
+  out[0] = c.x; out[1] = c.y; out[2] = c.z;
 }
 void ptref_miss(const void* sp, const float* d, float* out) {
   const Scene& s = *static_cast<const Scene*>(sp);

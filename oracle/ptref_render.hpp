@@ -426,6 +426,79 @@ inline float light_pdf(const Light& l, f3 P, f3 Wi) {
   return 1.f / sa;
 }
 
+// ------------------------------------------------------- environment
+// Spherical HDR mapping of miss.cu:64-95: u = atan2(z, x) / 2pi,
+// v = acos(y / |d|) / pi (u wraps through the texture's repeat mode).
+inline void env_uv(f3 d, float& u, float& v) {
+  float r = length(d);
+  float lon = patan2(d.z, d.x);
+  float lat = pacos(d.y / r);
+  u = lon * (1.f / (PI_F * 2.f));
+  v = lat * (1.f / PI_F);
+}
+// radiance the miss program returns for direction d (x2, miss.cu:93)
+inline f3 env_radiance(const Scene& s, f3 d) {
+  float u, v;
+  env_uv(d, u, v);
+  return 2.f * tex_eval(s, s.miss.texture, u, v);
+}
+// floor(total * r / 2^24) for a 24-bit r, exactly
+inline uint64_t scale24(uint64_t total, uint32_t r) {
+  return (total >> 24) * r + (((total & 0xFFFFFFull) * r) >> 24);
+}
+// first index in [0, n) whose inclusive prefix exceeds target
+inline uint32_t upper_index(const uint64_t* a, uint32_t n, uint64_t target) {
+  uint32_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] > target) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+// density of env_sample per unit solid angle: p(u, v) W H / (2 pi^2 sin(theta))
+inline float env_pdf(const Scene& s, f3 d) {
+  const EnvCdf& e = s.env;
+  if (e.total == 0) return 0.f;
+  float u, v;
+  env_uv(d, u, v);
+  float uw = u - std::floor(u);
+  int64_t i = (int64_t)(uw * (float)e.W), j = (int64_t)(v * (float)e.H);
+  if (i >= (int64_t)e.W) i = e.W - 1;
+  if (j >= (int64_t)e.H) j = e.H - 1;
+  if (i < 0) i = 0;
+  if (j < 0) j = 0;
+  const uint64_t* row = &e.cond[(size_t)j * e.W];
+  uint64_t w = row[i] - (i ? row[i - 1] : 0);
+  float st = psin(v * PI_F);
+  if (w == 0 || !(st > 0.f)) return 0.f;
+  double p = (double)w / (double)e.total * (double)e.W * (double)e.H;
+  return (float)(p / (2.0 * 3.14159265358979323846 * 3.14159265358979323846 * (double)st));
+}
+// two draws: row from the marginal table, column from the row's table,
+// continuous offsets inside the texel
+inline f3 env_sample(const Scene& s, uint32_t& seed) {
+  const EnvCdf& e = s.env;
+  uint32_t r1 = lcg(seed);
+  uint32_t r2 = lcg(seed);
+  if (e.total == 0) return mk(0.f, 1.f, 0.f);
+  uint64_t tr = scale24(e.total, r1);
+  uint32_t j = upper_index(e.marg.data(), e.H, tr);
+  uint64_t rprev = j ? e.marg[j - 1] : 0;
+  uint64_t rw = e.marg[j] - rprev;
+  const uint64_t* row = &e.cond[(size_t)j * e.W];
+  uint64_t tc = scale24(rw, r2);
+  uint32_t i = upper_index(row, e.W, tc);
+  uint64_t cprev = i ? row[i - 1] : 0;
+  float fu = (float)((double)(tc - cprev) / (double)(row[i] - cprev));
+  float fv = (float)((double)(tr - rprev) / (double)rw);
+  float u = ((float)i + fu) / (float)e.W;
+  float v = ((float)j + fv) / (float)e.H;
+  float lon = 2.f * PI_F * u, lat = PI_F * v;
+  float sl = psin(lat);
+  return mk(sl * pcos(lon), pcos(lat), sl * psin(lon));
+}
+
 // light_sample.cuh:27-33
 inline float PowerHeuristic(unsigned int nf, float fPdf, unsigned int ng, float gPdf) {
   float f = nf * fPdf, g = ng * gPdf;
@@ -438,25 +511,30 @@ struct PathCounters {
   uint64_t ext = 0, shadow = 0;
 };
 
-// light_sample.cuh:35-103 with the shadow ray resolved by the shared rule
-inline f3 direct_light(const Scene& s, const Surface& f, f3 P, f3 Wo, f3 N, bool isLight, uint32_t& seed,
-                       uint32_t& sig, PathCounters& pc) {
+// light_sample.cuh:35-103 with the shadow ray resolved by the shared rule.
+// `lights` is the scene's table, without environment lights unless
+// environment MIS is on.  An environment light is lit iff the shadow ray
+// hits nothing; its BRDF-sampled half is left to the continuation ray, whose
+// miss is MIS-weighted against env_pdf.
+inline f3 direct_light(const Scene& s, const std::vector<Light>& lights, const Surface& f, f3 P, f3 Wo, f3 N,
+                       bool isLight, uint32_t& seed, uint32_t& sig, PathCounters& pc) {
   f3 direct = mk(0.f);
-  int nl = (int)s.lights.size();
+  int nl = (int)lights.size();
   if (nl == 0) return mk(0.f);
   int index = ((int)(rnd(seed) * nl)) % nl;
   if (isLight && nl == 1) return mk(0.f);
-  const Light& L = s.lights[index];
-  f3 emission = L.e;
-  f3 Wi = light_sample(L, P, seed);
-  float lightPDF = light_pdf(L, P, Wi);
+  const Light& L = lights[index];
+  const bool env = L.type == PT_LIGHT_ENV;
+  f3 Wi = env ? env_sample(s, seed) : light_sample(L, P, seed);
+  float lightPDF = env ? env_pdf(s, Wi) : light_pdf(L, P, Wi);
+  f3 emission = env ? env_radiance(s, Wi) : L.e;
   if (dot(Wi, N) < 0.f) { sig = mix(sig, 0x33333333u); return mk(0.f); }
   pc.shadow++;
   HitState h = s.closest(P, Wi, 1e-3f, INF);
   bool occluded = false;
   if (h.found()) {
     int m = s.material_of(h.inst, h.prim);
-    occluded = !(s.mats[m].type == PT_MAT_DIFFUSE_LIGHT && dot(N, Wi) > 0.f);
+    occluded = env || !(s.mats[m].type == PT_MAT_DIFFUSE_LIGHT && dot(N, Wi) > 0.f);
   }
   sig = mix(sig, occluded ? 0xA5A5A5A5u : 0x5A5A5A5Au);
   if (occluded) return mk(0.f);
@@ -468,6 +546,7 @@ inline f3 direct_light(const Scene& s, const Surface& f, f3 P, f3 Wo, f3 N, bool
       direct += mv * emission * w / lightPDF;
     }
   }
+  if (env) return direct;
   Wi = brdf_sample(f, Wo, N, seed);
   float matPDF = 1.f;
   f3 mv = brdf_eval(f, Wo, Wi, N, matPDF);
@@ -504,11 +583,7 @@ inline f3 miss_color(const Scene& s, f3 d) {
     default: {
       float u, v;
       if (m.spherical) {
-        float r = length(d);
-        float lon = patan2(d.z, d.x);
-        float lat = pacos(d.y / r);
-        u = lon * (1.f / (PI_F * 2.f));
-        v = lat * (1.f / PI_F);
+        env_uv(d, u, v);
       } else {
         float theta = patan2(d.x, d.z);
         theta = theta < 0.f ? theta + (2.f * PI_F) : theta;
@@ -525,6 +600,7 @@ struct Params {
   uint32_t W, H, max_depth;
   bool russian;
   uint32_t rr_min_depth;
+  bool env_mis;  // PT_FLAG_ENV_MIS: environment lights sampled, unclamped MIS-weighted misses
 };
 
 struct PathResult {
@@ -549,14 +625,27 @@ inline PathResult trace_path(const Scene& s, const pt_camera& cam, const Params&
            v * mk(cam.vertical[0], cam.vertical[1], cam.vertical[2]) - origin;
   float tmin = 1e-6f;
   (void)(cam.time0 + rnd(seed) * (cam.time1 - cam.time0));  // prd.time, unused
+  const std::vector<Light>& lights = p.env_mis ? s.lights : s.lights_plain;
+  const bool env_light = p.env_mis && s.env.total > 0 && lights.size() != s.lights_plain.size();
   f3 T = mk(1.f), L = mk(0.f);
   bool prevSpec = false;
+  float prevPdf = 1.f;  // pdf of the last continuation sample (Lambertian / Oren-Nayar)
   for (uint32_t depth = 0; depth < p.max_depth; depth++) {
     pc.ext++;
     res.bounces = depth + 1;
     HitState h = s.closest(origin, dir, tmin, INF);
     res.sig = mix(res.sig, h.found() ? (((uint32_t)h.inst << 20) ^ (uint32_t)h.prim) : 0xFFFFFFFFu);
     if (!h.found()) {
+      if (p.env_mis && s.miss.type == PT_MISS_HDR) {
+        // unclamped environment radiance, MIS-weighted against the light
+        // sample of the previous (diffuse) bounce
+        f3 Le = env_radiance(s, dir);
+        float w = 1.f;
+        if (env_light && depth > 0 && !prevSpec) w = PowerHeuristic(1, prevPdf, 1, env_pdf(s, dir));
+        T *= Le * w;
+        res.L = L + T;
+        return res;
+      }
       T *= miss_color(s, dir);
       res.L = L + clamp3(T, 0.f, 1.f);
       return res;
@@ -567,7 +656,7 @@ inline PathResult trace_path(const Scene& s, const pt_camera& cam, const Params&
     bool spec = false;
     switch (f.type) {
       case PT_MAT_DIFFUSE_LIGHT: {  // diffuse_light.cu:30-48
-        f3 dl = direct_light(s, f, P, Wo, N, true, seed, res.sig, pc);
+        f3 dl = direct_light(s, lights, f, P, Wo, N, true, seed, res.sig, pc);
         L += T * dl;
         if (dot(N, Wo) < 0.f) T *= f.c0;
         if (depth == 0 || prevSpec) L += T;
@@ -576,7 +665,7 @@ inline PathResult trace_path(const Scene& s, const pt_camera& cam, const Params&
       }
       case PT_MAT_LAMBERTIAN:
       case PT_MAT_OREN_NAYAR: {  // lambertian.cu:49-73, oren_nayar.cu:46-69
-        f3 dl = direct_light(s, f, P, Wo, N, false, seed, res.sig, pc);
+        f3 dl = direct_light(s, lights, f, P, Wo, N, false, seed, res.sig, pc);
         L += T * dl;
         f3 Wi = brdf_sample(f, Wo, N, seed);
         float pdf = 1.f;
@@ -584,6 +673,7 @@ inline PathResult trace_path(const Scene& s, const pt_camera& cam, const Params&
         origin = P;
         dir = Wi;
         T *= clamp3(att / pdf, 0.f, 1.f);
+        prevPdf = pdf;
         break;
       }
       case PT_MAT_METAL: {  // metal.cu:38-56

@@ -429,6 +429,55 @@ struct Counts {
   uint64_t nodes = 0, prims = 0, insts = 0;
 };
 
+// Importance-sampling tables of an environment map (no reference
+// counterpart; the reference only looks the map up on a miss, miss.cu:64-95).
+// Texel weight = luminance x sin(theta) in 64-bit fixed point; the tables are
+// inclusive prefix sums per row (cond) and over row totals (marg).  Integer
+// prefix sums are exact, so any summation order gives the same tables.
+struct EnvCdf {
+  uint32_t W = 0, H = 0;
+  std::vector<uint64_t> cond;  // H x W
+  std::vector<uint64_t> marg;  // H
+  uint64_t total = 0;
+  int tex = -1;
+};
+
+// Texel weight of row j (theta at the row centre), FP64 luminance (Rec. 709
+// weights) times sin(theta) from the shared float libm.
+inline double env_texel_weight(const float* t, uint32_t j, uint32_t H) {
+  double lum = 0.2126 * (double)t[0] + 0.7152 * (double)t[1] + 0.0722 * (double)t[2];
+  float st = psin(PI_F * (((float)j + 0.5f) / (float)H));
+  double w = lum * (double)st;
+  return (w > 0.0) ? w : 0.0;  // NaN and negative texels get no weight
+}
+// weights are scaled by 2^(40 - e), e the binary exponent of the largest
+// weight, so every texel fits in 40 bits and the total in 61
+inline void build_env_cdf(const Texture& t, int tex, EnvCdf& e) {
+  e.W = t.w;
+  e.H = t.h;
+  e.tex = tex;
+  double mx = 0.0;
+  for (uint32_t j = 0; j < e.H; j++)
+    for (uint32_t i = 0; i < e.W; i++) mx = std::fmax(mx, env_texel_weight(&t.texels[4 * ((size_t)j * e.W + i)], j, e.H));
+  int ex = 0;
+  std::frexp(mx, &ex);
+  double scale = std::ldexp(1.0, 40 - ex);
+  e.cond.assign((size_t)e.W * e.H, 0);
+  e.marg.assign(e.H, 0);
+  uint64_t run = 0;
+  for (uint32_t j = 0; j < e.H; j++) {
+    uint64_t acc = 0;
+    for (uint32_t i = 0; i < e.W; i++) {
+      double w = env_texel_weight(&t.texels[4 * ((size_t)j * e.W + i)], j, e.H);
+      acc += (mx > 0.0) ? (uint64_t)(w * scale) : 0;
+      e.cond[(size_t)j * e.W + i] = acc;
+    }
+    run += acc;
+    e.marg[j] = run;
+  }
+  e.total = run;
+}
+
 struct Scene {
   std::vector<Sphere> spheres;
   std::vector<Rect> rects;
@@ -439,7 +488,9 @@ struct Scene {
   std::vector<Material> mats;
   std::vector<Texture> texs;
   std::vector<Light> lights;
+  std::vector<Light> lights_plain;  // without environment lights
   pt_miss_desc miss{};
+  EnvCdf env;
   Bvh4 tlas;
 
   explicit Scene(const pt_scene_desc& d) {
@@ -489,6 +540,13 @@ struct Scene {
                         l.radius, mk(l.emission[0], l.emission[1], l.emission[2])});
     }
     miss = d.miss;
+    for (const Light& l : lights) {
+      if (l.type != PT_LIGHT_ENV) { lights_plain.push_back(l); continue; }
+      if (miss.type != PT_MISS_HDR || !miss.spherical || miss.texture < 0 ||
+          miss.texture >= (int)texs.size() || texs[miss.texture].type != PT_TEX_IMAGE)
+        throw std::runtime_error("an environment light needs a spherical HDR miss texture");
+    }
+    if (lights.size() != lights_plain.size()) build_env_cdf(texs[miss.texture], miss.texture, env);
     // bottom-level structures, shared between identical (kind, first, count)
     for (uint32_t i = 0; i < d.num_instances; i++) {
       auto& in = d.instances[i];

@@ -217,3 +217,57 @@ def test_render_continues_accumulation(ctx):
     a22, n22 = ctx.read_accum(W, H)
     assert n4 == n22 == 4
     assert np.array_equal(a4, a22)
+
+
+# ------------------------------------------- triangle meshes under an HDRI
+# PT_SCENE_MESHES variant 0 is the full workload (8 x 2^17 triangles, 2048 x
+# 1024 sky); variants 1..6 subdivide the octaspheres that many times and use
+# a 256 x 128 sky.
+@pytest.mark.parametrize("variant,size", [(3, (256, 128)), (0, (2048, 1024))])
+def test_env_tables_bit_exact(ctx, variant, size):
+    sc = ptamd.Scene(ptamd.PT_SCENE_MESHES, 64, 36, variant)
+    say(f"[gpu] environment tables variant={variant}")
+    ctx.set_scene(sc.desc)
+    cond, marg, total = ctx.read_env_tables()
+    ocond, omarg, ototal = ptref.OracleScene(sc.desc).env_tables()
+    assert cond.shape == (size[1], size[0])
+    assert total == ototal > 0
+    assert np.array_equal(marg, omarg)
+    assert np.array_equal(cond, ocond)
+
+
+def test_mesh_scene_accel_and_primary_hits(ctx):
+    W, H = 480, 270
+    sc = ptamd.Scene(ptamd.PT_SCENE_MESHES, W, H, 0)
+    orc = ptref.OracleScene(sc.desc)
+    say("[gpu] mesh scene (2^20 triangles): accel build + primary rays")
+    gpu_scene(ctx, sc.desc)
+    g_nodes, g_order, g_morton, g_info = ctx.read_accel()
+    o_nodes, o_order, o_morton, o_info = orc.accel()
+    assert np.array_equal(g_info, o_info)
+    assert np.array_equal(g_morton, o_morton) and np.array_equal(g_order, o_order)
+    assert np.array_equal(g_nodes, o_nodes)
+    rays = camera_rays(sc.camera, W, H)
+    g, gc = ctx.trace_closest(rays, counted=True)
+    o, oc = orc.trace(rays, counted=True, threads=16)
+    check_hits(g, o)
+    assert np.array_equal(gc, oc)
+    assert (o["inst"] >= 0).mean() > 0.2
+
+
+@pytest.mark.parametrize("variant,W,H,frames", [(4, 96, 54, 2), (0, 64, 36, 1)])
+@pytest.mark.parametrize("env_mis", [True, False])
+def test_mesh_paths_match_oracle(ctx, variant, W, H, frames, env_mis):
+    sc = ptamd.Scene(ptamd.PT_SCENE_MESHES, W, H, variant)
+    flags = ptamd.PT_FLAG_KEEP_PATHS | ptamd.PT_FLAG_MATERIAL_SORT | (ptamd.PT_FLAG_ENV_MIS if env_mis else 0)
+    params = sc.render_params(flags=flags, frames_per_batch=frames)
+    say(f"[gpu] render paths meshes variant={variant} {W}x{H} frames={frames} env_mis={env_mis}")
+    gpu_scene(ctx, sc.desc)
+    ctx.set_camera(sc.camera)
+    st = ctx.render(params, 0, frames)
+    rgb, sig, bounces = ctx.read_paths(W, H)
+    orc = ptref.OracleScene(sc.desc)
+    oacc, (orgb, osig, ob), ost = orc.render(sc.camera, params, 0, frames, keep_last=True, threads=16)
+    compare_paths(rgb, sig, orgb, osig)
+    assert st.paths == W * H * frames
+    assert st.shadow_rays > 0
