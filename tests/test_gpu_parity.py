@@ -270,4 +270,5 @@ def test_mesh_paths_match_oracle(ctx, variant, W, H, frames, env_mis):
     oacc, (orgb, osig, ob), ost = orc.render(sc.camera, params, 0, frames, keep_last=True, threads=16)
     compare_paths(rgb, sig, orgb, osig)
     assert st.paths == W * H * frames
-    assert st.shadow_rays > 0
+    # the only light is the environment, sampled only with PT_FLAG_ENV_MIS
+    assert (st.shadow_rays > 0) == env_mis
