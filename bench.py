@@ -73,7 +73,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--spp", type=int, default=0, help="override samples per pixel (0: workload's 64)")
+    ap.add_argument("--spp", type=int, default=0, help="override samples per pixel (0: the workload's own)")
+    ap.add_argument("--workload", choices=["cornell", "meshes"], default="cornell",
+                    help="cornell: Cornell box 512x512 64 spp (the reported metric); meshes: 8 displaced "
+                         "octaspheres (2^20 triangles) under an importance-sampled HDRI, 1920x1080")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-frames", type=int, default=64)
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -91,11 +94,12 @@ def main():
 
     import ptamd
 
-    scene = ptamd.Scene(ptamd.PT_SCENE_CORNELL)
+    meshes = args.workload == "meshes"
+    scene = ptamd.Scene(ptamd.PT_SCENE_MESHES if meshes else ptamd.PT_SCENE_CORNELL)
     info = scene.info
     W, H = info.width, info.height
     spp = args.spp or info.spp
-    params = scene.render_params(flags=ptamd.PT_FLAG_MATERIAL_SORT)
+    params = scene.render_params()
     ctx = ptamd.Context(0)
     ctx.set_scene(scene.desc)
     acc = ctx.build_accel()
@@ -103,7 +107,7 @@ def main():
     log(f"[bench] accel: {acc.num_nodes} nodes, {acc.num_instances} instances, build {acc.build_ms:.2f} ms")
 
     # traversal-counting pass (identical rays: the render is deterministic)
-    cparams = scene.render_params(flags=ptamd.PT_FLAG_MATERIAL_SORT | ptamd.PT_FLAG_COUNT_TRAVERSAL)
+    cparams = scene.render_params(flags=info.flags | ptamd.PT_FLAG_COUNT_TRAVERSAL)
     cst = ctx.render(cparams, 0, spp)
     log(f"[bench] counted pass: {cst.extension_rays} closest-hit, {cst.shadow_rays} shadow rays")
 
@@ -131,7 +135,9 @@ def main():
     # Roofline of the dominant kernel (k_shade: hit and miss programs, light
     # sampling, the shadow ray's traversal, continuation and compaction) and
     # of the closest-hit traversal kernel (k_extend).
-    prim_b = PRIM_BYTES[1]  # rects and boxes (32 B records)
+    # record bytes: rects and boxes (32 B) in the Cornell box, triangles
+    # (48 B; the one ground rect counted as a triangle) in the mesh scene
+    prim_b = PRIM_BYTES[3] if meshes else PRIM_BYTES[1]
     ext, paths, exh = cst.extension_rays, cst.paths, cst.exhausted_paths
     shade_bytes_step = (ext * (64 + 16) + (ext - paths + exh) * 64 + (paths - exh) * 20 +
                         cst.shadow_node_fetches * 128 + cst.shadow_prim_fetches * prim_b +
@@ -140,7 +146,7 @@ def main():
                       cst.instance_fetches * 64)
     shade_ms = sum(s.shade_ms for s in steps) / len(steps)
     ext_ms = sum(s.extend_ms for s in steps) / len(steps)
-    traffic = load_traffic()
+    traffic = {} if meshes else load_traffic()
 
     def roof(kernel, key, bytes_step, ms, launches, extra):
         achieved = bytes_step / (ms * 1e-3) / 1e9
@@ -171,7 +177,7 @@ def main():
                                    "instances": round(cst.instance_fetches / ext, 3)}})
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and not meshes:
         try:
             sys.path.insert(0, str(REPO / "oracle"))
             import ptref
@@ -208,9 +214,11 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"cornell_box_mis_{W}x{H}_{spp}spp_depth{info.max_depth}_rr{info.rr_min_depth}",
+            "workload": (f"meshes_2p20_tris_hdri_envmis_{W}x{H}_{spp}spp_depth{info.max_depth}" if meshes else
+                         f"cornell_box_mis_{W}x{H}_{spp}spp_depth{info.max_depth}_rr{info.rr_min_depth}"),
             "width": W, "height": H, "spp": spp, "max_depth": info.max_depth,
             "rr_min_depth": info.rr_min_depth, "material_sort": True,
+            "env_mis": bool(info.flags & ptamd.PT_FLAG_ENV_MIS),
         },
         "samples_per_s": round(samples_per_s),
         "rays_per_step": {"closest_hit": cst.extension_rays, "shadow": cst.shadow_rays},
