@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define PT_API_VERSION 3
+#define PT_API_VERSION 4
 
 /* ------------------------------------------------------------------ status */
 typedef int32_t pt_status;
@@ -326,6 +326,30 @@ pt_status pt_read_accel(struct pt_context* ctx, uint32_t* nodes,
  * environment light.  Any pointer may be NULL. */
 pt_status pt_read_env_tables(struct pt_context* ctx, uint64_t* cond, uint64_t* marg,
                              uint32_t* width, uint32_t* height, uint64_t* total);
+
+/* ------------------------------------------------- learned radiance cache
+ * No counterpart in the reference.  A 96 -> 64 -> 64 -> 64 -> 64 -> 3 MLP
+ * with ReLU hidden layers (PT_CACHE_PARAMS parameters in the order W1 (64 x
+ * 96, row-major), b1, W2, b2, W3, b3, W4, b4, W5 (3 x 64), b5), FP32 master
+ * weights, BF16 MFMA forward and backward with FP32 accumulation, Adam
+ * (beta1 0.9, beta2 0.999, eps 1e-8).  Features are BF16 bit patterns,
+ * PT_CACHE_INPUTS per sample; outputs and targets are PT_CACHE_OUTPUTS
+ * floats per sample (the renderer uses log(1 + radiance)).  Loss: mean over
+ * samples of the squared output error summed over the outputs. */
+#define PT_CACHE_INPUTS 96
+#define PT_CACHE_OUTPUTS 3
+#define PT_CACHE_PARAMS 18883
+/* (Re)initialise the network: He-uniform weights from mt19937(seed), zero
+ * biases, zero Adam state. */
+pt_status pt_cache_init(struct pt_context* ctx, uint32_t seed);
+/* Copy the FP32 parameters out (out != NULL) and/or replace them (in != NULL). */
+pt_status pt_cache_params(struct pt_context* ctx, float* out, const float* in);
+/* Network outputs for n samples (host arrays). */
+pt_status pt_cache_infer(struct pt_context* ctx, const uint16_t* features, uint64_t n, float* out);
+/* One Adam step on one batch (host arrays); loss and the summed gradient
+ * (PT_CACHE_PARAMS floats) are returned when the pointers are non-NULL. */
+pt_status pt_cache_train(struct pt_context* ctx, const uint16_t* features, const float* targets,
+                         uint64_t n, float lr, float* loss, float* grad);
 
 /* Version of this ABI (PT_API_VERSION). */
 int32_t pt_api_version(void);

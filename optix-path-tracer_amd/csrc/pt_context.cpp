@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <random>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -89,6 +90,27 @@ struct pt_context {
   bool paths_valid = false;
   int shading_kinds = 0;  // distinct non-emitter material types of the scene
   uint32_t mat_types = 0;  // bit t set when material type t occurs
+
+  // ---- radiance cache network
+  bool cache_ready = false;
+  std::vector<DevBuf> cache_bufs;   // parameters, Adam moments, packed operands
+  float* cp = nullptr;              // FP32 master parameters
+  float* cm1 = nullptr;             // Adam first moments
+  float* cm2 = nullptr;             // Adam second moments
+  float* cgrad = nullptr;           // gradient of the last training step
+  uint16_t* cfw = nullptr;          // packed forward operands (BF16)
+  uint16_t* cbw = nullptr;          // packed backward operands (BF16)
+  float* cbias = nullptr;           // biases (FP32, padded)
+  uint32_t cache_step = 0;
+  uint32_t cwork_cap = 0;           // samples the work buffers hold
+  std::vector<DevBuf> cwork_bufs;
+  uint16_t* cfeat = nullptr;        // features, BF16 [n][96]
+  float* ctgt = nullptr;            // targets / outputs [n][3]
+  uint16_t* cX = nullptr;           // unit-major activations
+  uint16_t* cdZ = nullptr;          // unit-major output gradients
+  float* cdb = nullptr;             // per-wave bias gradients
+  float* closs = nullptr;           // per-wave loss sums
+  float* cdw = nullptr;             // per-chunk weight gradients
 
   // ---- path queues
   uint64_t path_capacity = 0;
@@ -215,6 +237,8 @@ void pt_destroy(pt_context* c) {
   dfree(c->scene_bufs);
   dfree(c->accel_bufs);
   dfree(c->path_bufs);
+  dfree(c->cache_bufs);
+  dfree(c->cwork_bufs);
   if (c->accum) (void)hipFree(c->accum);
   if (c->rgba) (void)hipFree(c->rgba);
   if (c->spill) (void)hipFree(c->spill);
@@ -915,6 +939,132 @@ pt_status pt_read_display(pt_context* c, uint8_t* rgba) {
   launch_display(c->accum, n, c->frames, c->rgba, grid_for(n), c->stream);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipMemcpy(rgba, c->rgba, (size_t)n * 4, hipMemcpyDeviceToHost));
+  return PT_OK;
+}
+
+// ------------------------------------------------------- radiance cache
+static pt_status cache_work(pt_context* c, uint32_t n) {
+  if (n <= c->cwork_cap) return PT_OK;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  dfree(c->cwork_bufs);
+  c->cwork_cap = 0;
+  const uint32_t cap = std::max<uint32_t>(n, 1024);
+  const size_t ld = mlp_padded(cap), waves = mlp_fwd_bwd_waves(cap), chunks = mlp_chunks(cap);
+  auto& B = c->cwork_bufs;
+  hipError_t e;
+  if ((e = dalloc(B, (void**)&c->cfeat, (size_t)cap * MLP_IN * 2)) != hipSuccess ||
+      (e = dalloc(B, (void**)&c->ctgt, (size_t)cap * MLP_OUT * 4)) != hipSuccess ||
+      (e = dalloc(B, (void**)&c->cX, (size_t)MLP_X_ROWS * ld * 2)) != hipSuccess ||
+      (e = dalloc(B, (void**)&c->cdZ, (size_t)MLP_DZ_ROWS * ld * 2)) != hipSuccess ||
+      (e = dalloc(B, (void**)&c->cdb, waves * MLP_DB_WORDS * 4)) != hipSuccess ||
+      (e = dalloc(B, (void**)&c->closs, waves * 4)) != hipSuccess ||
+      (e = dalloc(B, (void**)&c->cdw, chunks * MLP_PARAMS * 4)) != hipSuccess) {
+    dfree(B);
+    return hip_fail(c, e, "hipMalloc (cache work buffers)");
+  }
+  c->cwork_cap = cap;
+  return PT_OK;
+}
+
+pt_status pt_cache_init(pt_context* c, uint32_t seed) {
+  if (!c) return PT_ERR_INVALID_ARG;
+  (void)hipSetDevice(c->device);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (!c->cache_ready) {
+    auto& B = c->cache_bufs;
+    hipError_t e;
+    if ((e = dalloc(B, (void**)&c->cp, MLP_PARAMS * 4)) != hipSuccess ||
+        (e = dalloc(B, (void**)&c->cm1, MLP_PARAMS * 4)) != hipSuccess ||
+        (e = dalloc(B, (void**)&c->cm2, MLP_PARAMS * 4)) != hipSuccess ||
+        (e = dalloc(B, (void**)&c->cgrad, MLP_PARAMS * 4)) != hipSuccess ||
+        (e = dalloc(B, (void**)&c->cfw, MLP_FW_UINT4 * 16)) != hipSuccess ||
+        (e = dalloc(B, (void**)&c->cbw, MLP_BW_UINT4 * 16)) != hipSuccess ||
+        (e = dalloc(B, (void**)&c->cbias, MLP_BIAS_WORDS * 4)) != hipSuccess) {
+      dfree(B);
+      return hip_fail(c, e, "hipMalloc (cache parameters)");
+    }
+  }
+  // He-uniform weights U(-sqrt(6 / fan_in), +sqrt(6 / fan_in)) from
+  // mt19937(seed) (24-bit uniforms), zero biases
+  std::vector<float> p(MLP_PARAMS, 0.f);
+  std::mt19937 g(seed);
+  const int in[5] = {MLP_IN, MLP_H, MLP_H, MLP_H, MLP_H}, out[5] = {MLP_H, MLP_H, MLP_H, MLP_H, MLP_OUT};
+  size_t off = 0;
+  for (int L = 0; L < 5; L++) {
+    const float a = std::sqrt(6.f / (float)in[L]);
+    for (int k = 0; k < in[L] * out[L]; k++) {
+      float u = (float)(g() >> 8) * (1.f / 16777216.f);
+      p[off + k] = (2.f * u - 1.f) * a;
+    }
+    off += (size_t)in[L] * out[L] + out[L];
+  }
+  HIPCHK(c, hipMemcpy(c->cp, p.data(), MLP_PARAMS * 4, hipMemcpyHostToDevice));
+  launch_fill_u32((uint32_t*)c->cm1, MLP_PARAMS, 0u, c->stream);
+  launch_fill_u32((uint32_t*)c->cm2, MLP_PARAMS, 0u, c->stream);
+  launch_fill_u32((uint32_t*)c->cgrad, MLP_PARAMS, 0u, c->stream);
+  launch_fill_u32((uint32_t*)c->cfw, MLP_FW_UINT4 * 4, 0u, c->stream);
+  launch_fill_u32((uint32_t*)c->cbw, MLP_BW_UINT4 * 4, 0u, c->stream);
+  launch_fill_u32((uint32_t*)c->cbias, MLP_BIAS_WORDS, 0u, c->stream);
+  launch_mlp_pack(c->cp, c->cfw, c->cbw, c->cbias, c->stream);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->cache_step = 0;
+  c->cache_ready = true;
+  return PT_OK;
+}
+
+pt_status pt_cache_params(pt_context* c, float* out, const float* in) {
+  if (!c) return PT_ERR_INVALID_ARG;
+  if (!c->cache_ready) return fail(c, PT_ERR_STATE, "pt_cache_init first");
+  (void)hipSetDevice(c->device);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (out) HIPCHK(c, hipMemcpy(out, c->cp, MLP_PARAMS * 4, hipMemcpyDeviceToHost));
+  if (in) {
+    HIPCHK(c, hipMemcpy(c->cp, in, MLP_PARAMS * 4, hipMemcpyHostToDevice));
+    launch_mlp_pack(c->cp, c->cfw, c->cbw, c->cbias, c->stream);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  return PT_OK;
+}
+
+pt_status pt_cache_infer(pt_context* c, const uint16_t* features, uint64_t n, float* out) {
+  if (!c || (n && (!features || !out))) return PT_ERR_INVALID_ARG;
+  if (!c->cache_ready) return fail(c, PT_ERR_STATE, "pt_cache_init first");
+  if (n == 0) return PT_OK;
+  if (n > (1u << 26)) return fail(c, PT_ERR_INVALID_ARG, "too many samples");
+  (void)hipSetDevice(c->device);
+  pt_status ps = cache_work(c, (uint32_t)n);
+  if (ps != PT_OK) return ps;
+  HIPCHK(c, hipMemcpy(c->cfeat, features, n * MLP_IN * 2, hipMemcpyHostToDevice));
+  const int grid = (int)std::min<uint64_t>(1024, (n + 63) / 64);
+  launch_mlp_infer(c->cfeat, (uint32_t)n, c->cfw, c->cbias, c->ctgt, grid, c->stream);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(out, c->ctgt, n * MLP_OUT * 4, hipMemcpyDeviceToHost));
+  return PT_OK;
+}
+
+pt_status pt_cache_train(pt_context* c, const uint16_t* features, const float* targets, uint64_t n, float lr,
+                         float* loss, float* grad) {
+  if (!c || !n || !features || !targets) return PT_ERR_INVALID_ARG;
+  if (!c->cache_ready) return fail(c, PT_ERR_STATE, "pt_cache_init first");
+  if (n > (1u << 22)) return fail(c, PT_ERR_INVALID_ARG, "training batch larger than 2^22 samples");
+  (void)hipSetDevice(c->device);
+  pt_status ps = cache_work(c, (uint32_t)n);
+  if (ps != PT_OK) return ps;
+  HIPCHK(c, hipMemcpy(c->cfeat, features, n * MLP_IN * 2, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->ctgt, targets, n * MLP_OUT * 4, hipMemcpyHostToDevice));
+  c->cache_step++;
+  launch_mlp_train(c->cfeat, c->ctgt, (uint32_t)n, c->cfw, c->cbw, c->cbias, c->cX, c->cdZ, c->cdb, c->closs, c->cdw,
+                   c->cp, c->cm1, c->cm2, c->cgrad, lr, c->cache_step, c->cfw, c->cbw, c->cbias, c->stream);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (loss) {
+    const uint32_t waves = mlp_fwd_bwd_waves((uint32_t)n);
+    std::vector<float> lp(waves);
+    HIPCHK(c, hipMemcpy(lp.data(), c->closs, waves * 4, hipMemcpyDeviceToHost));
+    double sum = 0.0;
+    for (float v : lp) sum += v;
+    *loss = (float)sum;
+  }
+  if (grad) HIPCHK(c, hipMemcpy(grad, c->cgrad, MLP_PARAMS * 4, hipMemcpyDeviceToHost));
   return PT_OK;
 }
 

@@ -45,6 +45,24 @@ void launch_display(const double* accum, uint32_t npix, uint32_t frames, uchar4*
 void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st);
 void launch_fill_f64(double* p, uint64_t n, double v, hipStream_t st);
 
+// radiance-cache network (pt_mlp.hip)
+void launch_mlp_pack(const float* params, uint16_t* fw, uint16_t* bw, float* bias, hipStream_t st);
+void launch_mlp_infer(const uint16_t* feats, uint32_t n, const uint16_t* fw, const float* bias, float* out, int grid,
+                      hipStream_t st);
+uint32_t mlp_padded(uint32_t n);
+uint32_t mlp_fwd_bwd_waves(uint32_t n);
+uint32_t mlp_chunks(uint32_t n);
+void launch_mlp_train(const uint16_t* feats, const float* targets, uint32_t n, const uint16_t* fw, const uint16_t* bw,
+                      const float* bias_in, uint16_t* X, uint16_t* dZ, float* db_partial, float* loss_partial,
+                      float* dw_partial, float* params, float* m1, float* m2, float* grad_out, float lr,
+                      uint32_t step, uint16_t* fw_out, uint16_t* bw_out, float* bias_out, hipStream_t st);
+constexpr int MLP_FW_UINT4 = 2432;   // packed forward operands (uint4 = 8 BF16)
+constexpr int MLP_BW_UINT4 = 1792;   // packed backward operands
+constexpr int MLP_BIAS_WORDS = 4 * 64 + 16;
+constexpr int MLP_X_ROWS = 352;      // unit-major activation rows
+constexpr int MLP_DZ_ROWS = 272;     // unit-major output-gradient rows
+constexpr int MLP_DB_WORDS = 4 * 64 + 3;
+
 // acceleration-structure build (pt_bvh.hip)
 struct BuildScratch {
   // sized for the largest structure of the scene

@@ -156,4 +156,13 @@ constexpr int STAT_EXT = 0, STAT_SHADOW = 1, STAT_NODES = 2, STAT_PRIMS = 3, STA
 constexpr int STAT_SH_NODES = 5, STAT_SH_PRIMS = 6, STAT_SH_INSTS = 7, STAT_EXHAUSTED = 8;
 constexpr int STAT_WORDS = 16;
 
+// ------------------------------------------------------- radiance cache
+// 96 -> 64 -> 64 -> 64 -> 64 -> 3 network (ReLU hidden layers); parameters
+// in the order W1 (64 x 96), b1, W2, b2, W3, b3, W4, b4, W5 (3 x 64), b5.
+constexpr int MLP_IN = 96;
+constexpr int MLP_H = 64;
+constexpr int MLP_OUT = 3;
+constexpr int MLP_PARAMS = 18883;
+constexpr int MLP_CHUNK = 1024;  // samples per split-K chunk of the weight-gradient GEMM
+
 }  // namespace pt

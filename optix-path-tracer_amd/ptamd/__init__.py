@@ -31,6 +31,7 @@ PT_FLAG_KEEP_PATHS = 0x2
 PT_FLAG_ENV_MIS = 0x4
 PT_FLAG_COUNT_TRAVERSAL = 0x8
 PT_SCENE_RANDOM_SPHERES, PT_SCENE_CORNELL, PT_SCENE_MESHES = 0, 1, 2
+PT_CACHE_INPUTS, PT_CACHE_OUTPUTS, PT_CACHE_PARAMS = 96, 3, 18883
 
 
 class SphereDesc(C.Structure):
@@ -158,6 +159,10 @@ def _declare(lib):
                                       C.POINTER(C.c_uint32)]),
         "pt_read_env_tables": (C.c_int32, [P, P, P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                            C.POINTER(C.c_uint64)]),
+        "pt_cache_init": (C.c_int32, [P, C.c_uint32]),
+        "pt_cache_params": (C.c_int32, [P, P, P]),
+        "pt_cache_infer": (C.c_int32, [P, P, C.c_uint64, P]),
+        "pt_cache_train": (C.c_int32, [P, P, P, C.c_uint64, C.c_float, C.POINTER(C.c_float), P]),
         "pt_scene_build": (C.c_int32, [C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32,
                                        C.POINTER(C.c_void_p)]),
         "pt_scene_get_desc": (C.POINTER(SceneDesc), [P]),
@@ -321,6 +326,31 @@ class Context:
             self._check(self.lib.pt_read_env_tables(self.h, ptr(cond), ptr(marg), C.byref(w), C.byref(h),
                                                     C.byref(tot)), "pt_read_env_tables")
         return cond, marg, tot.value
+
+    # ------------------------------------------------ radiance cache
+    def cache_init(self, seed=0):
+        self._check(self.lib.pt_cache_init(self.h, seed), "pt_cache_init")
+
+    def cache_params(self, new=None):
+        out = np.zeros(PT_CACHE_PARAMS, np.float32)
+        inp = None if new is None else np.ascontiguousarray(new, np.float32)
+        self._check(self.lib.pt_cache_params(self.h, ptr(out), ptr(inp)), "pt_cache_params")
+        return out
+
+    def cache_infer(self, feats_bf16):
+        f = np.ascontiguousarray(feats_bf16, np.uint16).reshape(-1, PT_CACHE_INPUTS)
+        out = np.zeros((len(f), PT_CACHE_OUTPUTS), np.float32)
+        self._check(self.lib.pt_cache_infer(self.h, ptr(f), len(f), ptr(out)), "pt_cache_infer")
+        return out
+
+    def cache_train(self, feats_bf16, targets, lr=1e-3):
+        f = np.ascontiguousarray(feats_bf16, np.uint16).reshape(-1, PT_CACHE_INPUTS)
+        t = np.ascontiguousarray(targets, np.float32).reshape(-1, PT_CACHE_OUTPUTS)
+        loss = C.c_float()
+        grad = np.zeros(PT_CACHE_PARAMS, np.float32)
+        self._check(self.lib.pt_cache_train(self.h, ptr(f), ptr(t), len(f), lr, C.byref(loss), ptr(grad)),
+                    "pt_cache_train")
+        return loss.value, grad
 
     def read_accel(self):
         nn, npm, nb = C.c_uint64(), C.c_uint64(), C.c_uint32()
