@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define PT_API_VERSION 4
+#define PT_API_VERSION 5
 
 /* ------------------------------------------------------------------ status */
 typedef int32_t pt_status;
@@ -83,6 +83,8 @@ typedef int32_t pt_status;
 #define PT_FLAG_KEEP_PATHS 0x2u      /* keep last frame's per-path radiance  */
 #define PT_FLAG_ENV_MIS 0x4u         /* env NEE + MIS, no miss clamp (new)   */
 #define PT_FLAG_COUNT_TRAVERSAL 0x8u /* count node/primitive fetches         */
+#define PT_FLAG_RADIANCE_CACHE 0x10u /* learned radiance cache (new): training
+                                        paths + queries, see pt_cache_*     */
 
 /* ------------------------------------------------------------ scene input */
 typedef struct pt_sphere_desc {
@@ -211,6 +213,10 @@ typedef struct pt_render_stats {
   uint64_t shadow_prim_fetches;     /* occlusion queries (only with      */
   uint64_t shadow_instance_fetches; /* PT_FLAG_COUNT_TRAVERSAL)          */
   uint64_t exhausted_paths; /* paths still alive after max_depth bounces     */
+  uint64_t cache_queries;   /* radiance cache, last batch: paths ended by a query */
+  uint64_t cache_train_vertices; /* training vertices recorded              */
+  double cache_loss;        /* loss of the last training step              */
+  uint32_t cache_steps;     /* training steps of the last batch            */
   double gpu_ms;            /* render wall time on the stream (events)      */
   double extend_ms;         /* summed closest-hit kernel time               */
   double shade_ms;          /* summed shading kernel time (hit and miss
@@ -350,6 +356,22 @@ pt_status pt_cache_infer(struct pt_context* ctx, const uint16_t* features, uint6
  * (PT_CACHE_PARAMS floats) are returned when the pointers are non-NULL. */
 pt_status pt_cache_train(struct pt_context* ctx, const uint16_t* features, const float* targets,
                          uint64_t n, float lr, float* loss, float* grad);
+
+/* Render-loop use of the cache (PT_FLAG_RADIANCE_CACHE): paths not chosen
+ * for training end at bounce `query_depth` (0 = the camera ray's hit) with
+ * L += T * (exp(network) - 1); a `train_fraction` of the paths (hashed on
+ * pixel and frame) are traced to the end and record every vertex; after
+ * each batch of frames the network takes `steps` Adam steps of `batch`
+ * vertices each with learning rate `lr`, then answers the batch's queries.
+ * Defaults: 2, 0.03, 16384, 4, 1e-3. */
+pt_status pt_cache_configure(struct pt_context* ctx, uint32_t query_depth, float train_fraction,
+                             uint32_t batch, uint32_t steps, float lr);
+/* Records of the last render batch, for checks: which = 0 training
+ * vertices (values: L before the vertex, T at it, target log(1 + radiance)),
+ * 1 queries (values: T, 0, 0, 0, network output).  9 floats and one slot
+ * per record, at most `max`; *count = records the batch produced. */
+pt_status pt_cache_read_records(struct pt_context* ctx, int32_t which, uint32_t max, uint16_t* features,
+                                float* values, uint32_t* slots, uint32_t* count);
 
 /* Version of this ABI (PT_API_VERSION). */
 int32_t pt_api_version(void);

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -102,6 +103,16 @@ struct pt_context {
   uint16_t* cbw = nullptr;          // packed backward operands (BF16)
   float* cbias = nullptr;           // biases (FP32, padded)
   uint32_t cache_step = 0;
+  // render-loop use (PT_FLAG_RADIANCE_CACHE)
+  uint32_t cache_query_depth = 2;
+  float cache_train_fraction = 0.03f;
+  uint32_t cache_batch = 16384;     // samples per Adam step
+  uint32_t cache_steps = 4;         // Adam steps per render batch
+  float cache_lr = 1e-3f;
+  float scene_lo[3] = {0, 0, 0}, scene_hi[3] = {1, 1, 1};  // top-level box, for the position encoding
+  uint64_t cache_rec_cap = 0;       // slots the record buffers were sized for
+  std::vector<DevBuf> crec_bufs;
+  uint32_t cache_last_train = 0, cache_last_query = 0, cache_last_steps = 0;
   uint32_t cwork_cap = 0;           // samples the work buffers hold
   std::vector<DevBuf> cwork_bufs;
   uint16_t* cfeat = nullptr;        // features, BF16 [n][96]
@@ -239,6 +250,7 @@ void pt_destroy(pt_context* c) {
   dfree(c->path_bufs);
   dfree(c->cache_bufs);
   dfree(c->cwork_bufs);
+  dfree(c->crec_bufs);
   if (c->accum) (void)hipFree(c->accum);
   if (c->rgba) (void)hipFree(c->rgba);
   if (c->spill) (void)hipFree(c->spill);
@@ -693,6 +705,26 @@ pt_status pt_build_accel(pt_context* c, pt_accel_stats* out) {
   c->total_prims_sorted = total_sorted;
   s.nodes = c->nodes_dev;
   s.tlas_root = (uint32_t)blas_nodes;
+  {
+    // scene box = union of the top-level root's children
+    uint32_t w[NODE_WORDS];
+    HIPCHK(c, hipMemcpy(w, c->nodes_dev + 8ull * blas_nodes, 128, hipMemcpyDeviceToHost));
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (int k = 0; k < 4; k++) {
+      if (w[24 + k] == EMPTY_CHILD) continue;
+      for (int a = 0; a < 3; a++) {
+        float l, h;
+        std::memcpy(&l, &w[8 * a + k], 4);
+        std::memcpy(&h, &w[8 * a + 4 + k], 4);
+        lo[a] = std::min(lo[a], l);
+        hi[a] = std::max(hi[a], h);
+      }
+    }
+    for (int a = 0; a < 3; a++) {
+      c->scene_lo[a] = lo[a];
+      c->scene_hi[a] = hi[a] > lo[a] ? hi[a] : lo[a] + 1.f;
+    }
+  }
   s.num_instances = ni;
   s.insts = insts_sorted;
   s.sph = sph;
@@ -775,6 +807,36 @@ static pt_status ensure_paths(pt_context* c, uint64_t cap) {
   return PT_OK;
 }
 
+static pt_status cache_work(pt_context* c, uint32_t n);
+
+// radiance-cache record buffers for `cap` path slots: one query per path,
+// training vertices for a quarter of the slots (3 % of the paths, a few
+// vertices each, fit with a wide margin; extra vertices are dropped)
+static pt_status ensure_cache_records(pt_context* c, uint64_t cap) {
+  if (cap <= c->cache_rec_cap) return PT_OK;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  dfree(c->crec_bufs);
+  c->cache_rec_cap = 0;
+  DevPaths& P = c->paths;
+  auto& B = c->crec_bufs;
+  const uint64_t tcap = std::max<uint64_t>(65536, cap / 4);
+  hipError_t e;
+  if ((e = dalloc(B, (void**)&P.q_feat, cap * 12 * 16)) != hipSuccess ||
+      (e = dalloc(B, (void**)&P.q_meta, cap * 16)) != hipSuccess ||
+      (e = dalloc(B, (void**)&P.q_out, cap * 12)) != hipSuccess ||
+      (e = dalloc(B, (void**)&P.t_feat, tcap * 12 * 16)) != hipSuccess ||
+      (e = dalloc(B, (void**)&P.t_L, tcap * 16)) != hipSuccess ||
+      (e = dalloc(B, (void**)&P.t_T, tcap * 16)) != hipSuccess ||
+      (e = dalloc(B, (void**)&P.t_tgt, tcap * 12)) != hipSuccess) {
+    dfree(B);
+    return hip_fail(c, e, "hipMalloc (radiance-cache records)");
+  }
+  P.q_cap = (uint32_t)cap;
+  P.t_cap = (uint32_t)tcap;
+  c->cache_rec_cap = cap;
+  return PT_OK;
+}
+
 pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_begin, uint32_t frame_count,
                     pt_render_stats* out) {
   if (!c || !rp) return PT_ERR_INVALID_ARG;
@@ -806,9 +868,15 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
   if (frame_count == 0) return PT_OK;
   uint32_t fpb = rp->frames_per_batch ? rp->frames_per_batch : std::max<uint32_t>(1, PATHS_TARGET / WH);
   fpb = std::min(fpb, frame_count);
-  if ((uint64_t)fpb * WH > SLOT_MASK) return fail(c, PT_ERR_INVALID_ARG, "frames_per_batch x W x H exceeds 2^31 - 1");
+  if ((uint64_t)fpb * WH > SLOT_MASK) return fail(c, PT_ERR_INVALID_ARG, "frames_per_batch x W x H exceeds 2^30 - 1");
+  const bool use_cache = (rp->flags & PT_FLAG_RADIANCE_CACHE) != 0;
+  if (use_cache && !c->cache_ready) return fail(c, PT_ERR_STATE, "PT_FLAG_RADIANCE_CACHE needs pt_cache_init");
   pt_status ps = ensure_paths(c, (uint64_t)fpb * WH);
   if (ps != PT_OK) return ps;
+  if (use_cache) {
+    if ((ps = ensure_cache_records(c, (uint64_t)fpb * WH)) != PT_OK) return ps;
+    if ((ps = cache_work(c, c->cache_batch)) != PT_OK) return ps;
+  }
 
   RenderConsts rc{};
   rc.W = rp->width;
@@ -827,6 +895,12 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
     rc.cam.u[k] = cm.u[k];
     rc.cam.v[k] = cm.v[k];
   }
+  for (int k = 0; k < 3; k++) {
+    rc.cache.lo[k] = c->scene_lo[k];
+    rc.cache.inv_ext[k] = 1.f / (c->scene_hi[k] - c->scene_lo[k]);
+  }
+  rc.cache.query_depth = c->cache_query_depth;
+  rc.cache.train_threshold = (uint32_t)std::min(16777216.0, std::max(0.0, (double)c->cache_train_fraction * 16777216.0));
   rc.cam.lens_radius = cm.lens_radius;
   rc.cam.time0 = cm.time0;
   rc.cam.time1 = cm.time1;
@@ -870,6 +944,32 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
     }
     launch_exhausted(P, in, c->stats, grid, c->stream);
     launch_bounce_reset(P, in, c->stream);
+    if (use_cache) {
+      // train on this batch's training vertices, then answer its queries
+      launch_cache_targets(P, c->stream);
+      uint32_t cnt[2] = {0, 0};  // CTR_QUERY, CTR_TRAIN
+      HIPCHK(c, hipMemcpyAsync(cnt, P.counters + CTR_QUERY, 8, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      const uint32_t nq = std::min(cnt[0], P.q_cap), nt = std::min(cnt[1], P.t_cap);
+      uint32_t steps = 0;
+      for (uint32_t k = 0; k < c->cache_steps && nt >= 32; k++) {
+        const uint32_t off = (uint32_t)(((uint64_t)k * c->cache_batch) % nt);
+        const uint32_t n = std::min(c->cache_batch, nt - off);
+        c->cache_step++;
+        launch_mlp_train((const uint16_t*)(P.t_feat + (size_t)off * 12), P.t_tgt + (size_t)off * 3, n, c->cfw, c->cbw,
+                         c->cbias, c->cX, c->cdZ, c->cdb, c->closs, c->cdw, c->cp, c->cm1, c->cm2, c->cgrad,
+                         c->cache_lr, c->cache_step, c->cfw, c->cbw, c->cbias, c->stream);
+        steps++;
+      }
+      if (nq) {
+        launch_mlp_infer((const uint16_t*)P.q_feat, nq, c->cfw, c->cbias, P.q_out,
+                         (int)std::min<uint32_t>(2048, (nq + 63) / 64), c->stream);
+        launch_cache_apply(P, c->stream);
+      }
+      c->cache_last_train = nt;
+      c->cache_last_query = nq;
+      c->cache_last_steps = steps;
+    }
     launch_accumulate(P, rc, nf, c->accum, grid_for(WH), c->stream);
     c->last_fpb = nf;
   }
@@ -893,6 +993,19 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
     out->shadow_prim_fetches = st[STAT_SH_PRIMS];
     out->shadow_instance_fetches = st[STAT_SH_INSTS];
     out->exhausted_paths = st[STAT_EXHAUSTED];
+    out->cache_queries = c->cache_last_query;
+    out->cache_train_vertices = c->cache_last_train;
+    out->cache_steps = c->cache_last_steps;
+    out->cache_loss = 0.0;
+    if (use_cache && c->cache_last_steps) {
+      const uint32_t n = std::min(c->cache_batch, c->cache_last_train);
+      const uint32_t waves = mlp_fwd_bwd_waves(n);
+      std::vector<float> lp(waves);
+      HIPCHK(c, hipMemcpy(lp.data(), c->closs, waves * 4, hipMemcpyDeviceToHost));
+      double sum = 0.0;
+      for (float v : lp) sum += v;
+      out->cache_loss = sum;
+    }
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, t_begin, t_end);
     out->gpu_ms = ms;
@@ -1065,6 +1178,59 @@ pt_status pt_cache_train(pt_context* c, const uint16_t* features, const float* t
     *loss = (float)sum;
   }
   if (grad) HIPCHK(c, hipMemcpy(grad, c->cgrad, MLP_PARAMS * 4, hipMemcpyDeviceToHost));
+  return PT_OK;
+}
+
+pt_status pt_cache_configure(pt_context* c, uint32_t query_depth, float train_fraction, uint32_t batch,
+                             uint32_t steps, float lr) {
+  if (!c) return PT_ERR_INVALID_ARG;
+  if (!(train_fraction >= 0.f && train_fraction <= 1.f)) return fail(c, PT_ERR_INVALID_ARG, "train_fraction not in [0, 1]");
+  if (batch < 32 || batch > (1u << 22)) return fail(c, PT_ERR_INVALID_ARG, "batch not in [32, 2^22]");
+  c->cache_query_depth = query_depth;
+  c->cache_train_fraction = train_fraction;
+  c->cache_batch = batch;
+  c->cache_steps = steps;
+  c->cache_lr = lr;
+  return PT_OK;
+}
+
+pt_status pt_cache_read_records(pt_context* c, int32_t which, uint32_t max, uint16_t* features, float* values,
+                                uint32_t* slots, uint32_t* count) {
+  if (!c || (which != 0 && which != 1)) return PT_ERR_INVALID_ARG;
+  if (!c->cache_rec_cap) return fail(c, PT_ERR_STATE, "no radiance-cache render yet");
+  (void)hipSetDevice(c->device);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const DevPaths& P = c->paths;
+  const uint32_t n = std::min(max, which == 0 ? c->cache_last_train : c->cache_last_query);
+  if (count) *count = which == 0 ? c->cache_last_train : c->cache_last_query;
+  if (n == 0) return PT_OK;
+  if (features)
+    HIPCHK(c, hipMemcpy(features, which == 0 ? P.t_feat : P.q_feat, (size_t)n * 192, hipMemcpyDeviceToHost));
+  std::vector<float4> a(n), b(n);
+  std::vector<float> t((size_t)n * 3);
+  if (which == 0) {
+    HIPCHK(c, hipMemcpy(a.data(), P.t_L, (size_t)n * 16, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(b.data(), P.t_T, (size_t)n * 16, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(t.data(), P.t_tgt, (size_t)n * 12, hipMemcpyDeviceToHost));
+  } else {
+    HIPCHK(c, hipMemcpy(a.data(), P.q_meta, (size_t)n * 16, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(t.data(), P.q_out, (size_t)n * 12, hipMemcpyDeviceToHost));
+  }
+  for (uint32_t i = 0; i < n; i++) {
+    uint32_t slot;
+    std::memcpy(&slot, &a[i].w, 4);
+    if (slots) slots[i] = slot & SLOT_MASK;
+    if (!values) continue;
+    float* v = values + 9ull * i;
+    if (which == 0) {
+      v[0] = a[i].x; v[1] = a[i].y; v[2] = a[i].z;
+      v[3] = b[i].x; v[4] = b[i].y; v[5] = b[i].z;
+    } else {
+      v[0] = a[i].x; v[1] = a[i].y; v[2] = a[i].z;
+      v[3] = v[4] = v[5] = 0.f;
+    }
+    v[6] = t[3 * i]; v[7] = t[3 * i + 1]; v[8] = t[3 * i + 2];
+  }
   return PT_OK;
 }
 

@@ -10,6 +10,14 @@
 
 namespace pt {
 
+// radiance cache: scene box for the position encoding, query depth and the
+// training-path threshold (24-bit hash < threshold)
+struct CacheConsts {
+  float lo[3], inv_ext[3];
+  uint32_t query_depth;
+  uint32_t train_threshold;
+};
+
 struct RenderConsts {
   uint32_t W, H, WH;
   uint32_t max_depth;
@@ -18,6 +26,7 @@ struct RenderConsts {
   uint32_t frame0;
   uint32_t flags;
   DevCamera cam;
+  CacheConsts cache;
 };
 
 // render path (pt_render.hip)
@@ -38,6 +47,9 @@ void launch_bounce_reset(const DevPaths& P, int in, hipStream_t st);
 // holds 1 + H 64-bit words
 void launch_env_tables(const float4* tex, uint32_t W, uint32_t H, unsigned long long* cond,
                        unsigned long long* marg, unsigned long long* scratch, hipStream_t st);
+// radiance cache in the render loop (pt_render.hip)
+void launch_cache_targets(const DevPaths& P, hipStream_t st);
+void launch_cache_apply(const DevPaths& P, hipStream_t st);
 void launch_exhausted(const DevPaths& P, int in, uint64_t* stats, int grid, hipStream_t st);
 void launch_accumulate(const DevPaths& P, const RenderConsts& rc, uint32_t nframes, double* accum, int grid,
                        hipStream_t st);

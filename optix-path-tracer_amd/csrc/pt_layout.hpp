@@ -125,10 +125,12 @@ struct DevCamera {
 // Path state in HBM: 64 bytes per live path, structure of arrays, compacted
 // every bounce into the next ping-pong set so that every kernel streams it
 // contiguously.
-constexpr uint32_t SLOT_MASK = 0x7FFFFFFFu;
+constexpr uint32_t SLOT_MASK = 0x3FFFFFFFu;
+constexpr uint32_t TRAIN_BIT = 1u << 30;  // radiance-cache training path
+constexpr uint32_t SPEC_BIT = 1u << 31;   // last bounce was specular
 struct PathState {
   float4* o;  // (origin.xyz, RNG state bits)
-  float4* d;  // (direction.xyz, slot | last bounce specular << 31; slot =
+  float4* d;  // (direction.xyz, slot | TRAIN_BIT | SPEC_BIT; slot =
               //  frame-in-batch * W*H + launch index)
   float4* T;  // (throughput.xyz, pdf of the last diffuse continuation sample)
   float4* L;  // (radiance.xyz, decision-signature bits)
@@ -142,6 +144,16 @@ struct DevPaths {
   float4* result;       // per slot: (color() value, bounces bits)
   uint32_t* sig_final;  // per slot: decision signature
   uint32_t* counters;   // see CTR_*
+  // radiance-cache records (PT_FLAG_RADIANCE_CACHE)
+  uint4* q_feat;        // per query: 96 BF16 features (12 uint4)
+  float4* q_meta;       // per query: (T.xyz, slot bits)
+  float* q_out;         // per query: network output (3 floats)
+  uint32_t q_cap;
+  uint4* t_feat;        // per training vertex: 96 BF16 features
+  float4* t_L;          // per training vertex: (L before the vertex, slot bits)
+  float4* t_T;          // per training vertex: (T at the vertex, 0)
+  float* t_tgt;         // per training vertex: log(1 + radiance from the vertex on)
+  uint32_t t_cap;
 };
 
 // device counters (uint32)
@@ -149,6 +161,8 @@ constexpr int CTR_ACTIVE = 0;          // [0], [1]: live paths of state set 0 / 
 constexpr int CTR_BUCKETS = 4;         // 8 material buckets
 constexpr int CTR_BUCKET_CURSOR = 12;  // 8 scatter cursors
 constexpr int CTR_STACK_OVERFLOW = 20;
+constexpr int CTR_QUERY = 21;          // radiance-cache queries of the batch
+constexpr int CTR_TRAIN = 22;          // radiance-cache training vertices of the batch
 constexpr int CTR_WORDS = 32;
 // 64-bit statistics: [0] extension rays, [1] shadow rays, [2] nodes,
 // [3] prims, [4] instances

@@ -19,6 +19,101 @@ namespace pt {
 PTD uint32_t lane_rank(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
+// Wave-aggregated append: the lanes with `pred` get consecutive indices from
+// one atomic on `counter` (usable in divergent code: only active lanes vote).
+PTD uint32_t append_index(bool pred, uint32_t* counter) {
+  uint64_t m = __ballot(pred);
+  if (!pred) return 0u;
+  int leader = __ffsll((unsigned long long)m) - 1;
+  uint32_t base = 0;
+  if ((int)__lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+  base = __shfl(base, leader);
+  return base + lane_rank(m);
+}
+
+// 24-bit hash of (launch index, frame) choosing radiance-cache training
+// paths; independent of the path's own random sequence
+PTD uint32_t train_hash(uint32_t pix, uint32_t frame) {
+  uint32_t h = pix * 0x9E3779B1u ^ (frame + 0x7F4A7C15u) * 0x85EBCA77u;
+  h ^= h >> 15;
+  h *= 0x2C1B3C6Du;
+  h ^= h >> 12;
+  h *= 0x297A2D39u;
+  h ^= h >> 15;
+  return h >> 8;
+}
+
+// Radiance-cache input features, 96 BF16 (83 used, the rest zero):
+//   [0, 72)   sin, cos (2^f pi p), f = 0..11, per axis of the position p
+//             normalised to the scene box (angle doubling from pi p)
+//   [72, 74)  view direction Wo as (theta / pi, phi / 2 pi + 1/2), theta from +Y
+//   [74, 76)  shading normal, same mapping
+//   76        roughness, [77, 80) diffuse albedo, [80, 83) specular albedo
+PTD uint32_t bf16_bits(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+struct FeatWriter {
+  uint4* dst;
+  float buf[8];
+  int n = 0, k = 0;
+  PTD void put(float v) {
+    buf[n++] = v;
+    if (n == 8) {
+      dst[k++] = make_uint4(bf16_bits(buf[0]) | (bf16_bits(buf[1]) << 16), bf16_bits(buf[2]) | (bf16_bits(buf[3]) << 16),
+                            bf16_bits(buf[4]) | (bf16_bits(buf[5]) << 16), bf16_bits(buf[6]) | (bf16_bits(buf[7]) << 16));
+      n = 0;
+    }
+  }
+};
+PTD void sph_dir(v3 w, float& a, float& b) {
+  a = pacos(fminf(fmaxf(w.y, -1.f), 1.f)) * (1.f / PI_F);
+  b = patan2(w.z, w.x) * (0.5f / PI_F) + 0.5f;
+}
+__device__ __attribute__((noinline)) void write_features(uint4* dst, const CacheConsts& cc, v3 P, v3 Wo, v3 N,
+                                                         const Surface& f) {
+  FeatWriter fw{dst};
+  const float pp[3] = {(P.x - cc.lo[0]) * cc.inv_ext[0], (P.y - cc.lo[1]) * cc.inv_ext[1],
+                       (P.z - cc.lo[2]) * cc.inv_ext[2]};
+  for (int a = 0; a < 3; a++) {
+    float sn = psin(PI_F * pp[a]), cs = pcos(PI_F * pp[a]);
+    for (int q = 0; q < 12; q++) {
+      fw.put(sn);
+      fw.put(cs);
+      float s2 = 2.f * sn * cs, c2 = cs * cs - sn * sn;
+      sn = s2;
+      cs = c2;
+    }
+  }
+  float a, b;
+  sph_dir(normalize(Wo), a, b);
+  fw.put(a);
+  fw.put(b);
+  sph_dir(normalize(N), a, b);
+  fw.put(a);
+  fw.put(b);
+  float rough = 1.f;
+  v3 rd = mk3(0.f), rs = mk3(0.f);
+  switch (f.type) {
+    case PT_MAT_LAMBERTIAN:
+    case PT_MAT_OREN_NAYAR: rd = f.c0; break;
+    case PT_MAT_METAL: rough = f.p0; rs = f.c0; break;
+    case PT_MAT_DIELECTRIC: rough = 0.f; rs = mk3(1.f); break;
+    case PT_MAT_ASHIKHMIN_SHIRLEY: rough = 1.f / sqrtf(1.f + 0.5f * (f.p0 + f.p1)); rd = f.c0; rs = f.c1; break;
+    case PT_MAT_TORRANCE_SPARROW: rough = sqrtf(f.p0); rs = f.c0; break;
+    default: break;
+  }
+  fw.put(rough);
+  fw.put(rd.x);
+  fw.put(rd.y);
+  fw.put(rd.z);
+  fw.put(rs.x);
+  fw.put(rs.y);
+  fw.put(rs.z);
+  while (fw.k < 12) fw.put(0.f);
+}
+
 PTD Stack make_stack(uint32_t* s_ref, float* s_tn, uint2* spill, uint32_t* overflow) {
   Stack st;
   st.ref = s_ref + threadIdx.x;
@@ -93,12 +188,15 @@ __global__ __launch_bounds__(BLOCK) void k_raygen(DevPaths P, RenderConsts rc, u
              v * mk3(c.ver[0], c.ver[1], c.ver[2]) - origin;
     (void)rnd(seed);  // prd.time (raygen.cu:69), consumed but unused
     S.o[s] = make_float4(origin.x, origin.y, origin.z, __uint_as_float(seed));
-    S.d[s] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(s));
+    uint32_t dw = s;
+    if ((rc.flags & PT_FLAG_RADIANCE_CACHE) && train_hash(pix, rc.frame0 + f) < rc.cache.train_threshold) dw |= TRAIN_BIT;
+    S.d[s] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(dw));
     S.T[s] = make_float4(1.f, 1.f, 1.f, 1.f);
     S.L[s] = make_float4(0.f, 0.f, 0.f, __uint_as_float(0x811C9DC5u));
     P.result[s] = make_float4(0.f, 0.f, 0.f, __uint_as_float(rc.max_depth));
   }
   if (blockIdx.x == 0 && threadIdx.x < CTR_STACK_OVERFLOW) P.counters[threadIdx.x] = threadIdx.x == 0 ? nslots : 0u;
+  if (blockIdx.x == 0 && (threadIdx.x == CTR_QUERY || threadIdx.x == CTR_TRAIN)) P.counters[threadIdx.x] = 0u;
 }
 
 // material bucket of a hit: 0 = miss, 1 + material type otherwise
@@ -294,7 +392,11 @@ PTD bool scatter_and_roulette(const Surface& f, v3 Wo, v3 N, uint32_t& seed, uin
 // dot(N, Wi) > 0, or nothing is hit), the emitter rules, the continuation
 // sample and Russian roulette.  Surviving paths are appended, wave by wave,
 // to state set `out`; finished paths write their slot's result.
-template <int MS, bool COUNT>
+// FEAT: SH_COUNT counts the shadow rays' traversal, SH_CACHE records
+// radiance-cache training vertices and queries, SH_ENV samples environment
+// lights; each is compiled only into the variants that use it.
+constexpr int SH_COUNT = 1, SH_CACHE = 2, SH_ENV = 4;
+template <int MS, int FEAT>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_shade(DevScene sc, DevPaths P, int in, int src, const float4* __restrict__ hits,
                                                  int out, uint32_t depth, RenderConsts rc, uint2* spill,
                                                  uint64_t* stats) {
@@ -305,10 +407,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
   const PathState S = P.st[src], D = P.st[out];
   uint32_t* out_count = P.counters + CTR_ACTIVE + out;
   // light table: environment lights take part only with PT_FLAG_ENV_MIS
-  const bool env_mis = (rc.flags & PT_FLAG_ENV_MIS) != 0;
+  constexpr bool COUNT = (FEAT & SH_COUNT) != 0;
+  const bool env_mis = (FEAT & SH_ENV) && (rc.flags & PT_FLAG_ENV_MIS) != 0;
   const DevLight* lights = env_mis ? sc.lights : sc.lights_plain;
   const int nl = env_mis ? sc.num_lights : sc.num_lights_plain;
   const bool env_light = env_mis && sc.env_total != 0 && sc.num_lights != sc.num_lights_plain;
+  const bool cache_on = (FEAT & SH_CACHE) && (rc.flags & PT_FLAG_RADIANCE_CACHE) != 0;
   TraceCounts tc{0, 0, 0};
   uint32_t shadow_rays = 0;
   for (uint32_t i0 = blockIdx.x * BLOCK; i0 < n; i0 += gridDim.x * BLOCK) {
@@ -320,7 +424,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
       float4 o4 = S.o[i], d4 = S.d[i], t4 = S.T[i], l4 = S.L[i];
       v3 d = mk3(d4), T = mk3(t4), L = mk3(l4);
       uint32_t seed = __float_as_uint(o4.w), dw = __float_as_uint(d4.w);
-      uint32_t slot = dw & SLOT_MASK, flags = dw >> 31, sig = __float_as_uint(l4.w);
+      // dw keeps slot | TRAIN_BIT | SPEC_BIT; fields are taken where used
+      uint32_t flags = dw >> 31, sig = __float_as_uint(l4.w);
       float prev_pdf = t4.w;
       int pos = __float_as_int(hit.y), prim = __float_as_int(hit.z), inst = __float_as_int(hit.w);
       sig = sig_mix(sig, pos >= 0 ? (((uint32_t)inst << 20) ^ (uint32_t)prim) : 0xFFFFFFFFu);
@@ -350,13 +455,36 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
         bool is_light = has_mat<MS>(PT_MAT_DIFFUSE_LIGHT) && f.type == PT_MAT_DIFFUSE_LIGHT;
         bool nee = ((has_mat<MS>(PT_MAT_LAMBERTIAN) || has_mat<MS>(PT_MAT_OREN_NAYAR)) &&
                     (f.type == PT_MAT_LAMBERTIAN || f.type == PT_MAT_OREN_NAYAR)) || is_light;
+        bool queried = false;
+        if (cache_on) {
+          // training paths record every vertex (radiance so far, throughput);
+          // other paths end at the query depth with a cache lookup
+          const bool rec = (dw & TRAIN_BIT) != 0;
+          const bool ask = !rec && depth == rc.cache.query_depth;
+          uint32_t ti = append_index(rec, P.counters + CTR_TRAIN);
+          uint32_t qi = append_index(ask, P.counters + CTR_QUERY);
+          if (rec && ti < P.t_cap) {
+            write_features(P.t_feat + (size_t)ti * 12, rc.cache, r.P, r.Wo, r.N, f);
+            P.t_L[ti] = make_float4(L.x, L.y, L.z, __uint_as_float(dw & SLOT_MASK));
+            P.t_T[ti] = make_float4(T.x, T.y, T.z, 0.f);
+          }
+          if (ask) {
+            queried = true;
+            if (qi < P.q_cap) {
+              write_features(P.q_feat + (size_t)qi * 12, rc.cache, r.P, r.Wo, r.N, f);
+              P.q_meta[qi] = make_float4(T.x, T.y, T.z, __uint_as_float(dw & SLOT_MASK));
+            }
+            res = make_float4(L.x, L.y, L.z, __uint_as_float(depth + 1));
+            nee = false;
+          }
+        }
         if (nee) {
           v3 direct = mk3(0.f);
           if (nl > 0) {
             int index = ((int)(rnd(seed) * nl)) % nl;
             if (!(is_light && nl == 1)) {
               const DevLight& Lt = lights[index];
-              const bool env = Lt.type == PT_LIGHT_ENV;
+              const bool env = (FEAT & SH_ENV) && Lt.type == PT_LIGHT_ENV;
               v3 Wi = env ? env_sample(sc, seed) : light_sample(Lt, r.P, seed);
               float lightPDF = env ? env_pdf(sc, Wi) : light_pdf(Lt, r.P, Wi);
               v3 emission = env ? env_radiance(sc, Wi) : light_emission(Lt);
@@ -397,7 +525,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
           }
           L = L + T * direct;
         }
-        if (is_light) {
+        if (queried) {
+          // result = L now; the cache adds T * radiance after inference
+        } else if (is_light) {
           // diffuse_light.cu:43-47 and raygen.cu:85-90
           if (dot(r.N, r.Wo) < 0.f) T = T * f.c0;
           if (depth == 0 || (flags & 1u)) L = L + T;
@@ -407,15 +537,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
           alive = scatter_and_roulette<MS>(f, r.Wo, r.N, seed, sig, T, L, ndir, flags, prev_pdf, depth, rc, res);
           if (alive) {
             no = make_float4(r.P.x, r.P.y, r.P.z, __uint_as_float(seed));
-            nd = make_float4(ndir.x, ndir.y, ndir.z, __uint_as_float(slot | (flags << 31)));
+            nd = make_float4(ndir.x, ndir.y, ndir.z, __uint_as_float((dw & ~SPEC_BIT) | (flags << 31)));
             nT = make_float4(T.x, T.y, T.z, prev_pdf);
             nL = make_float4(L.x, L.y, L.z, __uint_as_float(sig));
           }
         }
       }
       if (!alive) {
-        P.result[slot] = res;
-        P.sig_final[slot] = sig;
+        P.result[dw & SLOT_MASK] = res;
+        P.sig_final[dw & SLOT_MASK] = sig;
       }
     }
     // wave-aggregated compaction of the survivors into set `out`
@@ -468,6 +598,39 @@ __global__ __launch_bounds__(BLOCK) void k_exhausted(DevPaths P, int in, uint64_
     atomicAdd((unsigned long long*)&stats[STAT_EXHAUSTED], (unsigned long long)n);
   for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK)
     P.sig_final[__float_as_uint(S.d[i].w) & SLOT_MASK] = __float_as_uint(S.L[i].w);
+}
+
+// Radiance-cache training targets: log(1 + radiance leaving each recorded
+// vertex toward the camera side) = log(1 + (L_final - L_before) / T), per
+// channel, zero where T is zero or the value is not finite.
+__global__ __launch_bounds__(BLOCK) void k_cache_targets(DevPaths P) {
+  const uint32_t n = min(P.counters[CTR_TRAIN], P.t_cap);
+  for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+    float4 a = P.t_L[i], t = P.t_T[i];
+    float4 fin = P.result[__float_as_uint(a.w) & SLOT_MASK];
+    float L0[3] = {a.x, a.y, a.z}, T0[3] = {t.x, t.y, t.z}, Lf[3] = {fin.x, fin.y, fin.z};
+    for (int c = 0; c < 3; c++) {
+      float v = T0[c] > 0.f ? (Lf[c] - L0[c]) / T0[c] : 0.f;
+      v = (v > 0.f && v < 3.0e38f) ? v : 0.f;
+      P.t_tgt[3 * (size_t)i + c] = log1pf(v);
+    }
+  }
+}
+
+// result[slot] += T * (exp(network output) - 1) for every query of the batch
+__global__ __launch_bounds__(BLOCK) void k_cache_apply(DevPaths P) {
+  const uint32_t n = min(P.counters[CTR_QUERY], P.q_cap);
+  for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+    float4 m = P.q_meta[i];
+    const uint32_t slot = __float_as_uint(m.w) & SLOT_MASK;
+    float y0 = fmaxf(expm1f(P.q_out[3 * (size_t)i]), 0.f), y1 = fmaxf(expm1f(P.q_out[3 * (size_t)i + 1]), 0.f),
+          y2 = fmaxf(expm1f(P.q_out[3 * (size_t)i + 2]), 0.f);
+    float4 r = P.result[slot];
+    r.x += m.x * y0;
+    r.y += m.y * y1;
+    r.z += m.z * y2;
+    P.result[slot] = r;
+  }
 }
 
 // raygen.cu:126-134 de_nan and raygen.cu:166 acc_buffer += (rgb, 1), in FP64
@@ -543,28 +706,38 @@ void launch_sort(const DevPaths& P, int in, int grid, hipStream_t st) {
   hipLaunchKernelGGL(k_sort_offsets, dim3(1), dim3(1), 0, st, P);
   hipLaunchKernelGGL(k_sort_scatter, dim3(grid), dim3(BLOCK), 0, st, P, in);
 }
-template <int MS>
-static void launch_shade_ms(const DevScene& sc, const DevPaths& P, int in, int src, const float4* hits, int out,
-                            uint32_t depth, const RenderConsts& rc, bool count, uint2* spill, uint64_t* stats,
-                            int grid, hipStream_t st) {
-  if (count)
-    hipLaunchKernelGGL((k_shade<MS, true>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, src, hits, out, depth, rc,
-                       spill, stats);
-  else
-    hipLaunchKernelGGL((k_shade<MS, false>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, src, hits, out, depth, rc,
-                       spill, stats);
+template <int MS, int FEAT>
+static void launch_shade_v(const DevScene& sc, const DevPaths& P, int in, int src, const float4* hits, int out,
+                           uint32_t depth, const RenderConsts& rc, uint2* spill, uint64_t* stats, int grid,
+                           hipStream_t st) {
+  hipLaunchKernelGGL((k_shade<MS, FEAT>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, src, hits, out, depth, rc, spill,
+                     stats);
 }
 void launch_shade(const DevScene& sc, const DevPaths& P, int in, bool sorted, int out, uint32_t depth,
                   const RenderConsts& rc, int matset, bool count, uint2* spill, uint64_t* stats, int grid,
                   hipStream_t st) {
-  int src = sorted ? 2 : in;
+  const int src = sorted ? 2 : in;
   const float4* hits = sorted ? P.hit_sorted : P.hit;
-  if (matset == MS_DIFFUSE)
-    launch_shade_ms<MS_DIFFUSE>(sc, P, in, src, hits, out, depth, rc, count, spill, stats, grid, st);
-  else if (matset == MS_SPHERES)
-    launch_shade_ms<MS_SPHERES>(sc, P, in, src, hits, out, depth, rc, count, spill, stats, grid, st);
-  else
-    launch_shade_ms<MS_ALL>(sc, P, in, src, hits, out, depth, rc, count, spill, stats, grid, st);
+  const bool env = (rc.flags & PT_FLAG_ENV_MIS) && sc.num_lights != sc.num_lights_plain;
+  const bool cache = (rc.flags & PT_FLAG_RADIANCE_CACHE) != 0;
+#define PT_SHADE(MSV, F) launch_shade_v<MSV, F>(sc, P, in, src, hits, out, depth, rc, spill, stats, grid, st)
+  if (cache) {  // every material; no traversal counting
+    if (env) PT_SHADE(MS_ALL, SH_CACHE | SH_ENV);
+    else PT_SHADE(MS_ALL, SH_CACHE);
+  } else if (env) {
+    if (count) PT_SHADE(MS_ALL, SH_ENV | SH_COUNT);
+    else PT_SHADE(MS_ALL, SH_ENV);
+  } else if (matset == MS_DIFFUSE) {
+    if (count) PT_SHADE(MS_DIFFUSE, SH_COUNT);
+    else PT_SHADE(MS_DIFFUSE, 0);
+  } else if (matset == MS_SPHERES) {
+    if (count) PT_SHADE(MS_SPHERES, SH_COUNT);
+    else PT_SHADE(MS_SPHERES, 0);
+  } else {
+    if (count) PT_SHADE(MS_ALL, SH_COUNT);
+    else PT_SHADE(MS_ALL, 0);
+  }
+#undef PT_SHADE
 }
 int shade_material_set(uint32_t types_mask) {
   if ((types_mask & ~(uint32_t)MS_DIFFUSE) == 0) return MS_DIFFUSE;
@@ -573,6 +746,12 @@ int shade_material_set(uint32_t types_mask) {
 }
 void launch_bounce_reset(const DevPaths& P, int in, hipStream_t st) {
   hipLaunchKernelGGL(k_bounce_reset, dim3(1), dim3(64), 0, st, P, in);
+}
+void launch_cache_targets(const DevPaths& P, hipStream_t st) {
+  hipLaunchKernelGGL(k_cache_targets, dim3(1024), dim3(BLOCK), 0, st, P);
+}
+void launch_cache_apply(const DevPaths& P, hipStream_t st) {
+  hipLaunchKernelGGL(k_cache_apply, dim3(2048), dim3(BLOCK), 0, st, P);
 }
 void launch_exhausted(const DevPaths& P, int in, uint64_t* stats, int grid, hipStream_t st) {
   hipLaunchKernelGGL(k_exhausted, dim3(grid), dim3(BLOCK), 0, st, P, in, stats);

@@ -30,6 +30,7 @@ PT_FLAG_MATERIAL_SORT = 0x1
 PT_FLAG_KEEP_PATHS = 0x2
 PT_FLAG_ENV_MIS = 0x4
 PT_FLAG_COUNT_TRAVERSAL = 0x8
+PT_FLAG_RADIANCE_CACHE = 0x10
 PT_SCENE_RANDOM_SPHERES, PT_SCENE_CORNELL, PT_SCENE_MESHES = 0, 1, 2
 PT_CACHE_INPUTS, PT_CACHE_OUTPUTS, PT_CACHE_PARAMS = 96, 3, 18883
 
@@ -110,7 +111,8 @@ class RenderStats(C.Structure):
                 ("node_fetches", C.c_uint64), ("prim_fetches", C.c_uint64),
                 ("instance_fetches", C.c_uint64), ("shadow_node_fetches", C.c_uint64),
                 ("shadow_prim_fetches", C.c_uint64), ("shadow_instance_fetches", C.c_uint64),
-                ("exhausted_paths", C.c_uint64), ("gpu_ms", C.c_double), ("extend_ms", C.c_double),
+                ("exhausted_paths", C.c_uint64), ("cache_queries", C.c_uint64),
+                ("cache_train_vertices", C.c_uint64), ("cache_loss", C.c_double), ("cache_steps", C.c_uint32), ("gpu_ms", C.c_double), ("extend_ms", C.c_double),
                 ("shade_ms", C.c_double), ("extend_launches", C.c_uint32),
                 ("shade_launches", C.c_uint32)]
 
@@ -160,6 +162,8 @@ def _declare(lib):
         "pt_read_env_tables": (C.c_int32, [P, P, P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                            C.POINTER(C.c_uint64)]),
         "pt_cache_init": (C.c_int32, [P, C.c_uint32]),
+        "pt_cache_configure": (C.c_int32, [P, C.c_uint32, C.c_float, C.c_uint32, C.c_uint32, C.c_float]),
+        "pt_cache_read_records": (C.c_int32, [P, C.c_int32, C.c_uint32, P, P, P, C.POINTER(C.c_uint32)]),
         "pt_cache_params": (C.c_int32, [P, P, P]),
         "pt_cache_infer": (C.c_int32, [P, P, C.c_uint64, P]),
         "pt_cache_train": (C.c_int32, [P, P, P, C.c_uint64, C.c_float, C.POINTER(C.c_float), P]),
@@ -351,6 +355,24 @@ class Context:
         self._check(self.lib.pt_cache_train(self.h, ptr(f), ptr(t), len(f), lr, C.byref(loss), ptr(grad)),
                     "pt_cache_train")
         return loss.value, grad
+
+    def cache_configure(self, query_depth=2, train_fraction=0.03, batch=16384, steps=4, lr=1e-3):
+        self._check(self.lib.pt_cache_configure(self.h, query_depth, train_fraction, batch, steps, lr),
+                    "pt_cache_configure")
+
+    def cache_records(self, which, max_records=1 << 20):
+        """Last batch's training vertices (which=0) or queries (1):
+        (features (n, 96) uint16, values (n, 9) float32, slots (n,), total)."""
+        cnt = C.c_uint32()
+        self._check(self.lib.pt_cache_read_records(self.h, which, 0, None, None, None, C.byref(cnt)),
+                    "pt_cache_read_records")
+        n = min(cnt.value, max_records)
+        feats = np.zeros((n, PT_CACHE_INPUTS), np.uint16)
+        vals = np.zeros((n, 9), np.float32)
+        slots = np.zeros(n, np.uint32)
+        self._check(self.lib.pt_cache_read_records(self.h, which, n, ptr(feats), ptr(vals), ptr(slots),
+                                                   C.byref(cnt)), "pt_cache_read_records")
+        return feats, vals, slots, cnt.value
 
     def read_accel(self):
         nn, npm, nb = C.c_uint64(), C.c_uint64(), C.c_uint32()

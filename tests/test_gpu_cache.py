@@ -110,3 +110,105 @@ def test_training_fits_a_smooth_function(ctx):
     assert last < 0.1 * first, (first, last)
     y = ctx.cache_infer(f)
     assert np.mean((y - t) ** 2) * 3 < 0.1 * first
+
+
+# ------------------------------------------------ the cache in the renderer
+def cornell(ctx, W, H):
+    sc = ptamd.Scene(ptamd.PT_SCENE_CORNELL, W, H)
+    ctx.set_scene(sc.desc)
+    ctx.build_accel()
+    ctx.set_camera(sc.camera)
+    return sc
+
+
+def render_image(ctx, params, f0, n):
+    st = ctx.render(params, f0, n)
+    acc, frames = ctx.read_accum(params.width, params.height)
+    return acc / frames, st
+
+
+def relmse(img, ref):
+    return float(np.mean((img - ref) ** 2 / (ref ** 2 + 0.01)))
+
+
+def test_training_records_leave_paths_unchanged(ctx):
+    """With no queries (query depth beyond max depth) the cache only records
+    training vertices: the image is bit-identical to the plain render, and
+    each training path's first vertex (L = 0, T = 1) has the target
+    log(1 + the path's own result)."""
+    say("[gpu] cache records without queries")
+    W = H = 64
+    sc = cornell(ctx, W, H)
+    ctx.cache_init(5)
+    ctx.cache_configure(query_depth=1000, train_fraction=0.5, batch=4096, steps=2)
+    plain = sc.render_params(flags=ptamd.PT_FLAG_KEEP_PATHS, frames_per_batch=1)
+    cached = sc.render_params(flags=ptamd.PT_FLAG_KEEP_PATHS | ptamd.PT_FLAG_RADIANCE_CACHE, frames_per_batch=1)
+    a, _ = render_image(ctx, plain, 0, 3)
+    b, st = render_image(ctx, cached, 0, 3)
+    assert np.array_equal(a, b)
+    assert st.cache_queries == 0 and st.cache_train_vertices > 0 and st.cache_steps == 2
+    rgb, sig, bounces = ctx.read_paths(W, H)
+    feats, vals, slots, total = ctx.cache_records(0)
+    assert total == len(slots) > 0.4 * W * H
+    first = np.all(vals[:, 0:3] == 0, 1) & np.all(vals[:, 3:6] == 1, 1)
+    assert first.sum() > 0.4 * W * H
+    want = np.log1p(np.maximum(rgb[slots[first]], 0))
+    np.testing.assert_allclose(vals[first, 6:9], want, rtol=1e-5, atol=1e-6)
+    f = M.from_bf16_bits(feats)
+    np.testing.assert_array_equal(f[:, 83:], 0)
+    s2c2 = f[:, 0] ** 2 + f[:, 1] ** 2  # first frequency of x: sin^2 + cos^2
+    assert np.all(np.abs(s2c2 - 1) < 2e-2)
+    albedos = {(0.65, 0.05, 0.05), (0.73, 0.73, 0.73), (0.12, 0.45, 0.15), (0.0, 0.0, 0.0)}
+    rd = {tuple(float(v) for v in np.round(x, 2)) for x in f[:, 77:80].astype(np.float64)}
+    assert rd <= albedos, rd
+
+
+def test_queries_end_paths_and_training_converges(ctx):
+    say("[gpu] cache queries + online training")
+    W = H = 128
+    sc = cornell(ctx, W, H)
+    ctx.cache_init(6)
+    ctx.cache_configure(query_depth=2, train_fraction=0.03, batch=4096, steps=4)
+    params = sc.render_params(flags=ptamd.PT_FLAG_KEEP_PATHS | ptamd.PT_FLAG_RADIANCE_CACHE, frames_per_batch=1)
+    losses = []
+    for k in range(60):
+        st = ctx.render(params, k, 1)
+        losses.append(st.cache_loss)
+    assert st.cache_queries > 0.8 * W * H
+    _, _, bounces = ctx.read_paths(W, H)
+    _, _, tslots, _ = ctx.cache_records(0)
+    _, qv, qslots, nq = ctx.cache_records(1)
+    assert nq == st.cache_queries
+    # paths that were not training paths end by the third vertex
+    others = np.setdiff1d(np.arange(W * H), tslots)
+    assert np.all(bounces[others] <= 3)
+    assert np.all(bounces[qslots] == 3)
+    assert np.mean(losses[-10:]) < 0.5 * np.mean(losses[:5]), losses
+
+
+def test_equal_time_gain_over_plain_tracer(ctx):
+    """relMSE against a converged image of the plain tracer, at equal GPU
+    time: the cache form (warmed up on the same view) against the plain
+    form.  Reports both; the cache must not be worse."""
+    say("[gpu] cache vs plain at equal time")
+    W = H = 128
+    sc = cornell(ctx, W, H)
+    plain = sc.render_params(flags=0)
+    ref, _ = render_image(ctx, plain, 1000, 4096)
+    img_p, st_p = render_image(ctx, plain, 0, 32)
+    t_plain = st_p.gpu_ms
+    ctx.cache_init(7)
+    ctx.cache_configure(query_depth=2, train_fraction=0.03, batch=4096, steps=4)
+    cached = sc.render_params(flags=ptamd.PT_FLAG_RADIANCE_CACHE, frames_per_batch=1)
+    for k in range(64):  # warm-up on the same view (not timed, not accumulated)
+        ctx.render(cached, 5000 + k, 1)
+    t, n = 0.0, 0
+    while t < t_plain and n < 4096:
+        st = ctx.render(cached, 0 if n == 0 else n, 1)
+        t += st.gpu_ms
+        n += 1
+    img_c, _ = ctx.read_accum(W, H)
+    img_c = img_c / n
+    e_p, e_c = relmse(img_p, ref), relmse(img_c, ref)
+    say(f"[gpu] equal time {t_plain:.2f} ms: plain 32 spp relMSE {e_p:.5f}; cache {n} spp relMSE {e_c:.5f}")
+    assert e_c <= e_p
