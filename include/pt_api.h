@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define PT_API_VERSION 5
+#define PT_API_VERSION 6
 
 /* ------------------------------------------------------------------ status */
 typedef int32_t pt_status;
@@ -85,6 +85,11 @@ typedef int32_t pt_status;
 #define PT_FLAG_COUNT_TRAVERSAL 0x8u /* count node/primitive fetches         */
 #define PT_FLAG_RADIANCE_CACHE 0x10u /* learned radiance cache (new): training
                                         paths + queries, see pt_cache_*     */
+#define PT_FLAG_GRAPH 0x20u          /* replay each full batch from one
+                                        captured hipGraph (no per-kernel
+                                        timing in the stats)                */
+#define PT_FLAG_TUNED 0x40u          /* use pt_autotune's batch size, sort
+                                        and graph choice when it matches     */
 
 /* ------------------------------------------------------------ scene input */
 typedef struct pt_sphere_desc {
@@ -332,6 +337,22 @@ pt_status pt_read_accel(struct pt_context* ctx, uint32_t* nodes,
  * environment light.  Any pointer may be NULL. */
 pt_status pt_read_env_tables(struct pt_context* ctx, uint64_t* cond, uint64_t* marg,
                              uint32_t* width, uint32_t* height, uint64_t* total);
+
+/* ------------------------------------------------------------ autotuning
+ * No counterpart in the reference.  pt_autotune renders `frames` frames of
+ * the current scene under candidate launch shapes (paths in flight 1..16 Mi,
+ * material sort on/off, batch graphs on/off), keeps the fastest, and
+ * pt_render uses it for renders with PT_FLAG_TUNED and the same resolution,
+ * depth and flags.  Results are identical under every shape. */
+typedef struct pt_tune_result {
+  uint32_t frames_per_batch;
+  int32_t material_sort;
+  int32_t graph;
+  uint32_t candidates;
+  double ms_per_frame;
+} pt_tune_result;
+pt_status pt_autotune(struct pt_context* ctx, const pt_render_params* params, uint32_t frames,
+                      pt_tune_result* out_or_null);
 
 /* ------------------------------------------------- learned radiance cache
  * No counterpart in the reference.  A 96 -> 64 -> 64 -> 64 -> 64 -> 3 MLP

@@ -123,6 +123,19 @@ struct pt_context {
   float* closs = nullptr;           // per-wave loss sums
   float* cdw = nullptr;             // per-chunk weight gradients
 
+  // ---- batch graph (PT_FLAG_GRAPH) and autotuned launch shape
+  uint32_t* frame_dev = nullptr;    // first frame of the next replayed batch
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<uint8_t> graph_key;   // launch state the graph was captured for
+  struct Tuned {
+    bool valid = false;
+    uint32_t W = 0, H = 0, max_depth = 0, flags = 0;
+    uint32_t frames_per_batch = 0;
+    bool sort = false, graph = false;
+    double ms_per_frame = 0.0;
+  } tuned;
+
   // ---- path queues
   uint64_t path_capacity = 0;
   std::vector<DevBuf> path_bufs;
@@ -251,6 +264,9 @@ void pt_destroy(pt_context* c) {
   dfree(c->cache_bufs);
   dfree(c->cwork_bufs);
   dfree(c->crec_bufs);
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  if (c->frame_dev) (void)hipFree(c->frame_dev);
   if (c->accum) (void)hipFree(c->accum);
   if (c->rgba) (void)hipFree(c->rgba);
   if (c->spill) (void)hipFree(c->spill);
@@ -866,7 +882,14 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
     c->frames = 0;
   }
   if (frame_count == 0) return PT_OK;
-  uint32_t fpb = rp->frames_per_batch ? rp->frames_per_batch : std::max<uint32_t>(1, PATHS_TARGET / WH);
+  // launch shape: the autotuned one for this scene and resolution when asked
+  // for and available, else frames_per_batch or ~4 Mi paths in flight
+  const uint32_t tune_bits = PT_FLAG_TUNED | PT_FLAG_GRAPH | PT_FLAG_MATERIAL_SORT;
+  const pt_context::Tuned& tu = c->tuned;
+  const bool tuned = (rp->flags & PT_FLAG_TUNED) && tu.valid && tu.W == rp->width && tu.H == rp->height &&
+                     tu.max_depth == rp->max_depth && (tu.flags & ~tune_bits) == (rp->flags & ~tune_bits);
+  uint32_t fpb = tuned ? tu.frames_per_batch
+                       : (rp->frames_per_batch ? rp->frames_per_batch : std::max<uint32_t>(1, PATHS_TARGET / WH));
   fpb = std::min(fpb, frame_count);
   if ((uint64_t)fpb * WH > SLOT_MASK) return fail(c, PT_ERR_INVALID_ARG, "frames_per_batch x W x H exceeds 2^30 - 1");
   const bool use_cache = (rp->flags & PT_FLAG_RADIANCE_CACHE) != 0;
@@ -908,7 +931,10 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
   const bool want_stats = out != nullptr;
   const bool count_trav = (rp->flags & PT_FLAG_COUNT_TRAVERSAL) != 0;
   // sorting pays only when more than one shading program can meet in a wave
-  const bool sort = (rp->flags & PT_FLAG_MATERIAL_SORT) != 0 && c->shading_kinds > 1;
+  const bool sort = (tuned ? tu.sort : (rp->flags & PT_FLAG_MATERIAL_SORT) != 0) && c->shading_kinds > 1;
+  // whole batches replayed from one captured graph (no per-kernel timing)
+  const bool use_graph = (tuned ? tu.graph : (rp->flags & PT_FLAG_GRAPH) != 0) && !use_cache;
+  const bool kernel_events = want_stats && !use_graph;
   std::vector<hipEvent_t> ev;
   auto mark = [&]() -> hipEvent_t {
     hipEvent_t e;
@@ -922,28 +948,75 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
   hipEvent_t t_begin = want_stats ? mark() : nullptr;
   DevPaths& P = c->paths;
   const int matset = shade_material_set(c->mat_types);
-  for (uint32_t f0 = 0; f0 < frame_count; f0 += fpb) {
-    uint32_t nf = std::min(fpb, frame_count - f0);
-    uint32_t nslots = nf * WH;
-    rc.frame0 = frame_begin + f0;
-    const int grid = grid_for(nslots);
+  // one batch: camera rays, the bounce loop, the paths that outlived it
+  auto enqueue_batch = [&](uint32_t nslots, int grid, bool events) {
     launch_raygen(P, rc, nslots, grid, c->stream);
     int in = 0, out = 1;
     for (uint32_t depth = 0; depth < rp->max_depth; depth++) {
-      hipEvent_t a = want_stats ? mark() : nullptr;
+      hipEvent_t a = events ? mark() : nullptr;
       launch_extend(c->dsc, P, in, depth == 0 ? 1e-6f : 1e-3f, c->spill, c->stats, count_trav, sort, grid, c->stream);
-      hipEvent_t b = want_stats ? mark() : nullptr;
-      if (want_stats) ext_ev.push_back({a, b});
+      hipEvent_t b = events ? mark() : nullptr;
+      if (events) ext_ev.push_back({a, b});
       if (sort) launch_sort(P, in, grid, c->stream);
-      hipEvent_t a2 = want_stats ? mark() : nullptr;
+      hipEvent_t a2 = events ? mark() : nullptr;
       launch_shade(c->dsc, P, in, sort, out, depth, rc, matset, count_trav, c->spill, c->stats, grid, c->stream);
-      hipEvent_t b2 = want_stats ? mark() : nullptr;
-      if (want_stats) sh_ev.push_back({a2, b2});
+      hipEvent_t b2 = events ? mark() : nullptr;
+      if (events) sh_ev.push_back({a2, b2});
       launch_bounce_reset(P, in, c->stream);
       std::swap(in, out);
     }
     launch_exhausted(P, in, c->stats, grid, c->stream);
     launch_bounce_reset(P, in, c->stream);
+  };
+  if (use_graph) {
+    if (!c->frame_dev) HIPCHK(c, hipMalloc(&c->frame_dev, 64));
+    launch_set_frame(c->frame_dev, frame_begin, c->stream);
+  }
+  for (uint32_t f0 = 0; f0 < frame_count; f0 += fpb) {
+    uint32_t nf = std::min(fpb, frame_count - f0);
+    uint32_t nslots = nf * WH;
+    rc.frame0 = frame_begin + f0;
+    const int grid = grid_for(nslots);
+    if (use_graph && nf == fpb) {
+      // full batches replay a graph captured once for this launch state;
+      // the batch's first frame comes from frame_dev, advanced in-graph
+      RenderConsts grc = rc;
+      grc.frame0 = 0;
+      grc.frame_dev = c->frame_dev;
+      std::vector<uint8_t> key;
+      auto add = [&](const void* p, size_t n) { key.insert(key.end(), (const uint8_t*)p, (const uint8_t*)p + n); };
+      add(&grc, sizeof(grc));
+      add(&c->dsc, sizeof(c->dsc));
+      add(&P, sizeof(P));
+      uint32_t shape[6] = {nslots, (uint32_t)grid, rp->max_depth, (uint32_t)sort, (uint32_t)matset, (uint32_t)count_trav};
+      add(shape, sizeof(shape));
+      add(&c->accum, sizeof(c->accum));
+      add(&c->stats, sizeof(c->stats));
+      if (!c->graph_exec || key != c->graph_key) {
+        if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+        if (c->graph) (void)hipGraphDestroy(c->graph);
+        c->graph_exec = nullptr;
+        c->graph = nullptr;
+        c->graph_key.clear();
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        RenderConsts saved = rc;
+        rc = grc;
+        enqueue_batch(nslots, grid, false);
+        launch_accumulate(P, rc, nf, c->accum, grid_for(WH), c->stream);
+        launch_next_frames(c->frame_dev, nf, c->stream);
+        rc = saved;
+        hipGraph_t g = nullptr;
+        HIPCHK(c, hipStreamEndCapture(c->stream, &g));
+        c->graph = g;
+        HIPCHK(c, hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
+        c->graph_key = key;
+      }
+      HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
+      c->last_fpb = nf;
+      continue;
+    }
+    if (use_graph) launch_set_frame(c->frame_dev, frame_begin + f0 + nf, c->stream);
+    enqueue_batch(nslots, grid, kernel_events);
     if (use_cache) {
       // train on this batch's training vertices, then answer its queries
       launch_cache_targets(P, c->stream);
@@ -1178,6 +1251,66 @@ pt_status pt_cache_train(pt_context* c, const uint16_t* features, const float* t
     *loss = (float)sum;
   }
   if (grad) HIPCHK(c, hipMemcpy(grad, c->cgrad, MLP_PARAMS * 4, hipMemcpyDeviceToHost));
+  return PT_OK;
+}
+
+// Launch-shape autotuner: renders `frames` frames (frames 0.. of the current
+// scene, camera and parameters) under every candidate shape, twice each,
+// and keeps the fastest second run.  Candidates: paths in flight 1, 2, 4, 8
+// and 16 Mi (frames per batch = paths / (W H), at least 1), material sort on
+// and off (when the scene has more than one shading program), batch graphs
+// on and off.
+pt_status pt_autotune(pt_context* c, const pt_render_params* rp, uint32_t frames, pt_tune_result* out) {
+  if (!c || !rp || frames == 0) return PT_ERR_INVALID_ARG;
+  if (rp->flags & PT_FLAG_RADIANCE_CACHE) return fail(c, PT_ERR_UNSUPPORTED, "autotune the plain path only");
+  const uint64_t WH = (uint64_t)rp->width * rp->height;
+  if (WH == 0) return fail(c, PT_ERR_INVALID_ARG, "empty image");
+  std::vector<uint32_t> fpbs;
+  for (uint64_t paths : {1ull << 20, 1ull << 21, 1ull << 22, 1ull << 23, 1ull << 24}) {
+    uint32_t f = (uint32_t)std::max<uint64_t>(1, paths / WH);
+    f = std::min(f, frames);
+    if (f * WH > SLOT_MASK) continue;
+    if (std::find(fpbs.begin(), fpbs.end(), f) == fpbs.end()) fpbs.push_back(f);
+  }
+  const bool can_sort = c->shading_kinds > 1;
+  pt_context::Tuned best;
+  best.ms_per_frame = 1e300;
+  c->tuned.valid = false;
+  for (uint32_t f : fpbs)
+    for (int so = 0; so < (can_sort ? 2 : 1); so++)
+      for (int gr = 0; gr < 2; gr++) {
+        pt_render_params p = *rp;
+        p.frames_per_batch = f;
+        p.flags = (rp->flags & ~(PT_FLAG_TUNED | PT_FLAG_GRAPH | PT_FLAG_MATERIAL_SORT)) |
+                  (so ? PT_FLAG_MATERIAL_SORT : 0u) | (gr ? PT_FLAG_GRAPH : 0u);
+        pt_render_stats st{};
+        double ms = 0.0;
+        for (int rep = 0; rep < 2; rep++) {
+          pt_status ps = pt_render(c, &p, 0, frames, &st);
+          if (ps != PT_OK) return ps;
+          ms = st.gpu_ms;
+        }
+        const double per = ms / frames;
+        if (per < best.ms_per_frame) {
+          best.ms_per_frame = per;
+          best.frames_per_batch = f;
+          best.sort = so != 0;
+          best.graph = gr != 0;
+        }
+      }
+  best.valid = true;
+  best.W = rp->width;
+  best.H = rp->height;
+  best.max_depth = rp->max_depth;
+  best.flags = rp->flags;
+  c->tuned = best;
+  if (out) {
+    out->frames_per_batch = best.frames_per_batch;
+    out->material_sort = best.sort ? 1 : 0;
+    out->graph = best.graph ? 1 : 0;
+    out->candidates = (uint32_t)(fpbs.size() * (can_sort ? 2 : 1) * 2);
+    out->ms_per_frame = best.ms_per_frame;
+  }
   return PT_OK;
 }
 

@@ -27,6 +27,7 @@ struct RenderConsts {
   uint32_t flags;
   DevCamera cam;
   CacheConsts cache;
+  const uint32_t* frame_dev;  // non-null: first frame of the batch read from device memory
 };
 
 // render path (pt_render.hip)
@@ -35,6 +36,9 @@ void launch_trace_closest(const DevScene& sc, const pt_ray* rays, uint64_t n, pt
 void launch_trace_occluded(const DevScene& sc, const pt_ray* rays, uint64_t n, uint8_t* occ, uint2* spill,
                            uint32_t* overflow, int grid, hipStream_t st);
 void launch_raygen(const DevPaths& P, const RenderConsts& rc, uint32_t nslots, int grid, hipStream_t st);
+// frame_dev[0] = value (k_set_frame) / frame_dev[0] += step (k_next_frames)
+void launch_set_frame(uint32_t* frame_dev, uint32_t value, hipStream_t st);
+void launch_next_frames(uint32_t* frame_dev, uint32_t step, hipStream_t st);
 void launch_extend(const DevScene& sc, const DevPaths& P, int in, float tmin, uint2* spill, uint64_t* stats,
                    bool count_traversal, bool sort, int grid, hipStream_t st);
 void launch_sort(const DevPaths& P, int in, int grid, hipStream_t st);

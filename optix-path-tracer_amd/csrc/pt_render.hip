@@ -173,6 +173,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_occluded(DevScene sc, const pt_
 // generateRay, raygen.cu:66-74 color() set-up.
 __global__ __launch_bounds__(BLOCK) void k_raygen(DevPaths P, RenderConsts rc, uint32_t nslots) {
   PathState S = P.st[0];
+  if (rc.frame_dev) rc.frame0 = *rc.frame_dev;  // replayed graphs take the batch's frame from memory
   for (uint32_t s = blockIdx.x * BLOCK + threadIdx.x; s < nslots; s += gridDim.x * BLOCK) {
     uint32_t f = s / rc.WH, pix = s - f * rc.WH;
     uint32_t x = pix % rc.W, y = pix / rc.W;
@@ -687,6 +688,14 @@ void launch_trace_closest(const DevScene& sc, const pt_ray* rays, uint64_t n, pt
 void launch_trace_occluded(const DevScene& sc, const pt_ray* rays, uint64_t n, uint8_t* occ, uint2* spill,
                            uint32_t* overflow, int grid, hipStream_t st) {
   hipLaunchKernelGGL(k_trace_occluded, dim3(grid), dim3(BLOCK), 0, st, sc, rays, n, occ, spill, overflow);
+}
+__global__ void k_set_frame(uint32_t* f, uint32_t v) { *f = v; }
+__global__ void k_next_frames(uint32_t* f, uint32_t step) { *f += step; }
+void launch_set_frame(uint32_t* frame_dev, uint32_t value, hipStream_t st) {
+  hipLaunchKernelGGL(k_set_frame, dim3(1), dim3(1), 0, st, frame_dev, value);
+}
+void launch_next_frames(uint32_t* frame_dev, uint32_t step, hipStream_t st) {
+  hipLaunchKernelGGL(k_next_frames, dim3(1), dim3(1), 0, st, frame_dev, step);
 }
 void launch_raygen(const DevPaths& P, const RenderConsts& rc, uint32_t nslots, int grid, hipStream_t st) {
   hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(BLOCK), 0, st, P, rc, nslots);

@@ -31,6 +31,8 @@ PT_FLAG_KEEP_PATHS = 0x2
 PT_FLAG_ENV_MIS = 0x4
 PT_FLAG_COUNT_TRAVERSAL = 0x8
 PT_FLAG_RADIANCE_CACHE = 0x10
+PT_FLAG_GRAPH = 0x20
+PT_FLAG_TUNED = 0x40
 PT_SCENE_RANDOM_SPHERES, PT_SCENE_CORNELL, PT_SCENE_MESHES = 0, 1, 2
 PT_CACHE_INPUTS, PT_CACHE_OUTPUTS, PT_CACHE_PARAMS = 96, 3, 18883
 
@@ -123,6 +125,11 @@ class AccelStats(C.Structure):
                 ("build_ms", C.c_double)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("frames_per_batch", C.c_uint32), ("material_sort", C.c_int32), ("graph", C.c_int32),
+                ("candidates", C.c_uint32), ("ms_per_frame", C.c_double)]
+
+
 class SceneInfo(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
                 ("max_depth", C.c_uint32), ("russian", C.c_int32), ("rr_min_depth", C.c_uint32),
@@ -161,6 +168,7 @@ def _declare(lib):
                                       C.POINTER(C.c_uint32)]),
         "pt_read_env_tables": (C.c_int32, [P, P, P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                            C.POINTER(C.c_uint64)]),
+        "pt_autotune": (C.c_int32, [P, C.POINTER(RenderParams), C.c_uint32, C.POINTER(TuneResult)]),
         "pt_cache_init": (C.c_int32, [P, C.c_uint32]),
         "pt_cache_configure": (C.c_int32, [P, C.c_uint32, C.c_float, C.c_uint32, C.c_uint32, C.c_float]),
         "pt_cache_read_records": (C.c_int32, [P, C.c_int32, C.c_uint32, P, P, P, C.POINTER(C.c_uint32)]),
@@ -330,6 +338,11 @@ class Context:
             self._check(self.lib.pt_read_env_tables(self.h, ptr(cond), ptr(marg), C.byref(w), C.byref(h),
                                                     C.byref(tot)), "pt_read_env_tables")
         return cond, marg, tot.value
+
+    def autotune(self, params, frames):
+        r = TuneResult()
+        self._check(self.lib.pt_autotune(self.h, C.byref(params), frames, C.byref(r)), "pt_autotune")
+        return r
 
     # ------------------------------------------------ radiance cache
     def cache_init(self, seed=0):

@@ -39,7 +39,7 @@ STRUCTS = {
     "pt_material_desc": ptamd.MaterialDesc, "pt_texture_desc": ptamd.TextureDesc,
     "pt_light_desc": ptamd.LightDesc, "pt_miss_desc": ptamd.MissDesc, "pt_scene_desc": ptamd.SceneDesc,
     "pt_camera": ptamd.Camera, "pt_render_params": ptamd.RenderParams, "pt_render_stats": ptamd.RenderStats,
-    "pt_accel_stats": ptamd.AccelStats, "pt_scene_info": ptamd.SceneInfo,
+    "pt_accel_stats": ptamd.AccelStats, "pt_scene_info": ptamd.SceneInfo, "pt_tune_result": ptamd.TuneResult,
 }
 
 
