@@ -68,15 +68,120 @@ def load_traffic():
         return {}
 
 
+def orbit_camera(scene, frame, frames_per_turn=300, r=20.0, height=4.0, vfov=45.0):
+    """Camera of the radiance-cache workload: orbit of radius r around the
+    origin, one turn per `frames_per_turn` frames (frame 0 is the scene's own
+    pose at (20, 4, 0))."""
+    import math
+
+    import ptamd
+
+    a = 2 * math.pi * frame / frames_per_turn
+    o = (r * math.cos(a), height, r * math.sin(a))
+    w = [o[k] for k in range(3)]
+    n = math.sqrt(sum(x * x for x in w))
+    w = [x / n for x in w]
+    up = (0.0, 1.0, 0.0)
+    u = [up[1] * w[2] - up[2] * w[1], up[2] * w[0] - up[0] * w[2], up[0] * w[1] - up[1] * w[0]]
+    n = math.sqrt(sum(x * x for x in u))
+    u = [x / n for x in u]
+    v = [w[1] * u[2] - w[2] * u[1], w[2] * u[0] - w[0] * u[2], w[0] * u[1] - w[1] * u[0]]
+    aspect = scene.info.width / scene.info.height
+    hh = math.tan(math.radians(vfov) / 2)
+    hw = aspect * hh
+    f = 10.0  # focus distance, as the scene camera
+    c = ptamd.Camera()
+    c.origin[:] = o
+    c.lower_left[:] = [o[k] - hw * f * u[k] - hh * f * v[k] - f * w[k] for k in range(3)]
+    c.horizontal[:] = [2 * hw * f * u[k] for k in range(3)]
+    c.vertical[:] = [2 * hh * f * v[k] for k in range(3)]
+    c.u[:] = u
+    c.v[:] = v
+    c.w[:] = w
+    c.lens_radius, c.time0, c.time1 = 0.0, 0.0, 1.0
+    return c
+
+
+def cache_workload(args):
+    """Mesh scene, 1 spp per frame on an orbiting camera with the radiance
+    cache training online; then, on the frozen view of the last frame, the
+    plain tracer and the cache tracer accumulate for the same GPU time and
+    their relMSE against a high-spp plain render (the unbiased estimator the
+    oracle pins) is compared."""
+    import numpy as np
+
+    import ptamd
+
+    W, H = 960, 540
+    scene = ptamd.Scene(ptamd.PT_SCENE_MESHES, W, H)
+    ctx = ptamd.Context(0)
+    ctx.set_scene(scene.desc)
+    ctx.build_accel()
+    ctx.cache_init(0)
+    ctx.cache_configure()
+    base = scene.info.flags
+    plain = scene.render_params(flags=base)
+    cached = scene.render_params(flags=base | ptamd.PT_FLAG_RADIANCE_CACHE, frames_per_batch=1)
+    t_plain = t_cache = 0.0
+    for f in range(args.frames):
+        ctx.set_camera(orbit_camera(scene, f))
+        t_plain += ctx.render(plain, f, 1).gpu_ms
+        st = ctx.render(cached, f, 1)
+        t_cache += st.gpu_ms
+        if f % 10 == 0:
+            log(f"[bench] frame {f}: cache loss {st.cache_loss:.4f}, {st.cache_queries} queries, "
+                f"{st.cache_train_vertices} training vertices")
+    ctx.set_camera(orbit_camera(scene, args.frames))
+    ref_spp = 1024
+    ref_st = ctx.render(plain, 100000, ref_spp)
+    ref, _ = ctx.read_accum(W, H)
+    ref /= ref_spp
+    log(f"[bench] reference {ref_spp} spp in {ref_st.gpu_ms:.1f} ms")
+    budget_spp = 16
+    st_p = ctx.render(plain, 0, budget_spp)
+    img_p, _ = ctx.read_accum(W, H)
+    img_p /= budget_spp
+    budget = st_p.gpu_ms
+    t, n = 0.0, 0
+    while t < budget and n < 4096:
+        t += ctx.render(cached, n, 1).gpu_ms
+        n += 1
+    img_c, _ = ctx.read_accum(W, H)
+    img_c /= n
+    ctx.close()
+
+    def relmse(a):
+        return float(np.mean((a - ref) ** 2 / (ref ** 2 + 0.01)))
+
+    e_p, e_c = relmse(img_p), relmse(img_c)
+    out = {
+        "metric": "equal-time relMSE, radiance cache / plain",
+        "value": round(e_c / e_p, 4),
+        "unit": "ratio",
+        "n_gpus": 1,
+        "higher_is_better": False,
+        "data": "synthetic",
+        "dtype": "f32 tracing, bf16 network",
+        "config": {"workload": f"meshes_hdri_orbit_{W}x{H}_1spp_per_frame_cache", "frames": args.frames},
+        "plain": {"spp": budget_spp, "gpu_ms": round(budget, 2), "relmse": e_p},
+        "cache": {"spp": n, "gpu_ms": round(t, 2), "relmse": e_c},
+        "animation_ms_per_frame": {"plain": round(t_plain / args.frames, 3), "cache": round(t_cache / args.frames, 3)},
+        "reference_spp": ref_spp,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--spp", type=int, default=0, help="override samples per pixel (0: the workload's own)")
-    ap.add_argument("--workload", choices=["cornell", "meshes"], default="cornell",
+    ap.add_argument("--workload", choices=["cornell", "meshes", "cache"], default="cornell",
                     help="cornell: Cornell box 512x512 64 spp (the reported metric); meshes: 8 displaced "
-                         "octaspheres (2^20 triangles) under an importance-sampled HDRI, 1920x1080")
+                         "octaspheres (2^20 triangles) under an importance-sampled HDRI, 1920x1080; cache: the "
+                         "mesh scene on an orbiting camera with the learned radiance cache, equal-time relMSE")
+    ap.add_argument("--frames", type=int, default=60, help="cache workload: animation frames before the evaluation")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-frames", type=int, default=64)
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -94,6 +199,9 @@ def main():
 
     import ptamd
 
+    if args.workload == "cache":
+        cache_workload(args)
+        return
     meshes = args.workload == "meshes"
     scene = ptamd.Scene(ptamd.PT_SCENE_MESHES if meshes else ptamd.PT_SCENE_CORNELL)
     info = scene.info
