@@ -80,21 +80,26 @@ PTD HitOut trace_closest(const DevScene& sc, v3 o, v3 d, float tmin, float tmax,
   int base_sp = 0;  // stack depth when the instance was entered
   uint32_t node_base = sc.tlas_root, prim_base = 0, kind = 0, inst_id = 0;
   st.sp = 0;
-  st.put(0, 0u, tmin);
-  st.sp = 1;
+  // the entry in hand: the top-level root first; the nearest child of each
+  // node and an instance's root are taken directly instead of through the
+  // stack, which visits the same nodes in the same order
+  uint32_t ref = 0u;
+  float etn = tmin;
+  bool have = true;
   // bound on loop trips: a well-formed structure never comes close
   uint32_t trips = 0;
   while (++trips < (1u << 22)) {
-    if (cur >= 0 && st.sp == base_sp) {
-      cur = -1;
-      node_base = sc.tlas_root;
-      ro = o; rd = d; rinv = winv;
+    if (!have) {
+      if (cur >= 0 && st.sp == base_sp) {
+        cur = -1;
+        node_base = sc.tlas_root;
+        ro = o; rd = d; rinv = winv;
+      }
+      if (st.sp == 0) break;
+      st.pop(ref, etn);
+      if (ref == EMPTY_CHILD || etn > h.t) continue;
     }
-    if (st.sp == 0) break;
-    uint32_t ref;
-    float etn;
-    st.pop(ref, etn);
-    if (ref == EMPTY_CHILD || etn > h.t) continue;
+    have = false;
     if (ref & LEAF_BIT) {
       uint32_t first = ref & 0x0FFFFFFFu;
       uint32_t count = ((ref >> 28) & 7u) + 1u;
@@ -118,8 +123,8 @@ PTD HitOut trace_closest(const DevScene& sc, v3 o, v3 d, float tmin, float tmax,
         if (kind == PT_GEOM_MESH) tr = tri_setup(rd);
         cur = (int)first;
         base_sp = st.sp;
-        st.put(st.sp, 0u, etn);
-        st.sp++;
+        ref = 0u;  // the bottom-level root, next
+        have = true;
         continue;
       }
       for (uint32_t k = 0; k < count; k++) {
@@ -178,7 +183,8 @@ PTD HitOut trace_closest(const DevScene& sc, v3 o, v3 d, float tmin, float tmax,
                           h.t);
       cref[c] = cnc[c] ? (LEAF_BIT | ((cnc[c] - 1u) << 28) | chd[c]) : chd[c];
     }
-    // rank by (tn, slot); nearest ends on top of the stack
+    // rank by (tn, slot): the nearest is taken next, the others go on the
+    // stack with the second nearest on top
     int nh = 0;
 #pragma unroll
     for (int c = 0; c < 4; c++) nh += (tn[c] != FINF);
@@ -189,9 +195,18 @@ PTD HitOut trace_closest(const DevScene& sc, v3 o, v3 d, float tmin, float tmax,
 #pragma unroll
       for (int e = 0; e < 4; e++)
         if (e != c && tn[e] != FINF && (tn[e] < tn[c] || (tn[e] == tn[c] && e < c))) rank++;
-      st.put(st.sp + nh - 1 - rank, cref[c], tn[c]);
+      if (rank == 0) {
+        // the cull a pop would apply (the slab test's far bound is 2^-22 loose)
+        if (!(tn[c] > h.t)) {
+          ref = cref[c];
+          etn = tn[c];
+          have = true;
+        }
+      } else {
+        st.put(st.sp + nh - 1 - rank, cref[c], tn[c]);
+      }
     }
-    st.sp += nh;
+    if (nh > 0) st.sp += nh - 1;
   }
   return h;
 }
