@@ -61,11 +61,11 @@ def test_autotune_picks_a_shape_with_identical_results(ctx):
     sc = setup(ctx, ptamd.PT_SCENE_CORNELL, W, H)
     say("[gpu] autotune cornell 128x128")
     p = sc.render_params()
-    ref, _ = image(ctx, p, 0, 16)
-    r = ctx.autotune(p, 16)
+    ref, _ = image(ctx, p, 0, 128)
+    r = ctx.autotune(p, 128)
     say(f"[gpu] tuned: frames_per_batch={r.frames_per_batch} sort={r.material_sort} graph={r.graph} "
         f"{r.ms_per_frame:.3f} ms/frame over {r.candidates} candidates")
     assert r.candidates >= 4 and r.frames_per_batch >= 1 and r.ms_per_frame > 0
     t = sc.render_params(flags=p.flags | ptamd.PT_FLAG_TUNED)
-    a, _ = image(ctx, t, 0, 16)
+    a, _ = image(ctx, t, 0, 128)
     assert np.array_equal(a, ref)
