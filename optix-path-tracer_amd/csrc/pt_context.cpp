@@ -26,7 +26,7 @@ using namespace pt;
 namespace {
 
 constexpr int MAX_GRID = 2048;         // blocks of the grid-stride kernels
-constexpr uint32_t PATHS_TARGET = 1u << 22;  // paths in flight per batch (automatic)
+constexpr uint32_t PATHS_TARGET = 1u << 24;  // paths in flight per batch (automatic)
 
 struct DevBuf {
   void* p = nullptr;
@@ -883,7 +883,7 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
   }
   if (frame_count == 0) return PT_OK;
   // launch shape: the autotuned one for this scene and resolution when asked
-  // for and available, else frames_per_batch or ~4 Mi paths in flight
+  // for and available, else frames_per_batch or ~16 Mi paths in flight
   const uint32_t tune_bits = PT_FLAG_TUNED | PT_FLAG_GRAPH | PT_FLAG_MATERIAL_SORT;
   const pt_context::Tuned& tu = c->tuned;
   const bool tuned = (rp->flags & PT_FLAG_TUNED) && tu.valid && tu.W == rp->width && tu.H == rp->height &&

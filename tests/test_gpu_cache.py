@@ -178,39 +178,46 @@ def test_queries_end_paths_and_training_converges(ctx):
     _, _, tslots, _ = ctx.cache_records(0)
     _, qv, qslots, nq = ctx.cache_records(1)
     assert nq == st.cache_queries > 0.3 * W * H
-    # paths that were not training paths end by the third vertex, and every
-    # one that got there ended with a query
+    # paths that were not training paths end by the third vertex; every query
+    # ends such a path there, and the ones that end there without a query
+    # are the rays that left the box through its open front
     others = np.setdiff1d(np.arange(W * H), tslots)
     assert np.all(bounces[others] <= 3)
     assert np.all(bounces[qslots] == 3)
-    assert set(qslots) == set(others[bounces[others] == 3])
+    third = others[bounces[others] == 3]
+    assert np.setdiff1d(qslots, third).size == 0
+    escaped = np.setdiff1d(third, qslots).size
+    say(f"[gpu] {nq} queries, {escaped} third-segment escapes of {third.size}")
+    assert escaped < 0.3 * third.size, (escaped, third.size)
     assert np.mean(losses[-10:]) < 0.5 * np.mean(losses[:5]), losses
 
 
 def test_equal_time_gain_over_plain_tracer(ctx):
-    """relMSE against a converged image of the plain tracer (4096 spp), at
-    equal GPU time: the cache form, trained online on the same view in
-    batches of 16 frames, against the plain form.  Both are reported; at
-    equal samples the cache paths are shorter, and at equal time the cache
-    must not be worse."""
+    """relMSE against a converged image of the plain tracer (4096 spp): the
+    cache form, trained online on the same view, against the plain form,
+    both rendering 128 frames in one 128-frame batch (2 Mi paths).
+
+    The plain tracer is unbiased, so its variance falls as 1 / time; its
+    error is scaled by t_plain / t_cache to the cache's GPU time (training
+    and the host round trip for the record counts included).  The scaling
+    also shrinks the reference's own noise, which stays in the plain
+    error at any time, so it errs in the plain tracer's favour.  The cache
+    must be better per sample and at equal time."""
     say("[gpu] cache vs plain at equal time")
     W = H = 128
+    fpb = 128
     sc = cornell(ctx, W, H)
-    plain = sc.render_params(flags=0)
+    plain = sc.render_params(flags=0, frames_per_batch=fpb)
     ref, _ = render_image(ctx, plain, 1000, 4096)
-    img_p, st_p = render_image(ctx, plain, 0, 64)
-    t_plain = st_p.gpu_ms
+    img_p, st_p = render_image(ctx, plain, 0, fpb)
     ctx.cache_init(7)
     ctx.cache_configure(query_depth=2, train_fraction=0.03, batch=4096, steps=4)
-    cached = sc.render_params(flags=ptamd.PT_FLAG_RADIANCE_CACHE, frames_per_batch=16)
-    ctx.render(cached, 5000, 256)  # warm-up training on the same view (not accumulated)
-    t, n = 0.0, 0
-    while t < t_plain and n < 4096:
-        t += ctx.render(cached, n, 16).gpu_ms
-        n += 16
-    img_c, frames = ctx.read_accum(W, H)
-    img_c = img_c / frames
+    cached = sc.render_params(flags=ptamd.PT_FLAG_RADIANCE_CACHE, frames_per_batch=fpb)
+    ctx.render(cached, 5000, 2 * fpb)  # warm-up training on the same view (not accumulated)
+    img_c, st_c = render_image(ctx, cached, 0, fpb)
     e_p, e_c = relmse(img_p, ref), relmse(img_c, ref)
-    say(f"[gpu] equal time {t_plain:.2f} ms: plain 64 spp relMSE {e_p:.5f}; cache {n} spp ({t:.2f} ms) "
-        f"relMSE {e_c:.5f}")
-    assert e_c <= e_p
+    e_eq = e_p * st_p.gpu_ms / st_c.gpu_ms
+    say(f"[gpu] {fpb} spp: plain {st_p.gpu_ms:.2f} ms relMSE {e_p:.5f}; cache {st_c.gpu_ms:.2f} ms "
+        f"relMSE {e_c:.5f}; plain at the cache's time {e_eq:.5f}")
+    assert e_c < e_p
+    assert e_c <= e_eq
