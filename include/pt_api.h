@@ -383,7 +383,9 @@ pt_status pt_cache_train(struct pt_context* ctx, const uint16_t* features, const
  * L += T * (exp(network) - 1); a `train_fraction` of the paths (hashed on
  * pixel and frame) are traced to the end and record every vertex; after
  * each batch of frames the network takes `steps` Adam steps of `batch`
- * vertices each with learning rate `lr`, then answers the batch's queries.
+ * vertices each (steps x batch vertices spread evenly over the batch's
+ * records, each used once) with learning rate `lr`, then answers the
+ * batch's queries.
  * Defaults: 2, 0.03, 16384, 4, 1e-3. */
 pt_status pt_cache_configure(struct pt_context* ctx, uint32_t query_depth, float train_fraction,
                              uint32_t batch, uint32_t steps, float lr);

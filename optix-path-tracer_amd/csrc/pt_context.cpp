@@ -25,7 +25,6 @@ using namespace pt;
 
 namespace {
 
-constexpr int MAX_GRID = 2048;         // blocks of the grid-stride kernels
 constexpr uint32_t PATHS_TARGET = 1u << 24;  // paths in flight per batch (automatic)
 
 struct DevBuf {
@@ -113,6 +112,8 @@ struct pt_context {
   uint64_t cache_rec_cap = 0;       // slots the record buffers were sized for
   std::vector<DevBuf> crec_bufs;
   uint32_t cache_last_train = 0, cache_last_query = 0, cache_last_steps = 0;
+  uint32_t cache_last_batch = 0;    // samples of the last Adam step
+  int cache_last_grid = 0;          // grid of the last cache batch (its waves own the records)
   uint32_t cwork_cap = 0;           // samples the work buffers hold
   std::vector<DevBuf> cwork_bufs;
   uint16_t* cfeat = nullptr;        // features, BF16 [n][96]
@@ -141,7 +142,8 @@ struct pt_context {
   std::vector<DevBuf> path_bufs;
   DevPaths paths{};
   uint2* spill = nullptr;
-  uint64_t* stats = nullptr;
+  uint64_t* stats = nullptr;        // per-wave rows [MAX_WAVES][STAT_WORDS]
+  uint64_t* stats_total = nullptr;  // [STAT_WORDS]
   uint32_t* qcounters = nullptr;  // counters of the query kernels
 };
 
@@ -243,7 +245,8 @@ pt_status pt_create(int32_t device, pt_context** out) {
   }
   size_t spill_bytes = (size_t)MAX_GRID * BLOCK * STK_SPILL * sizeof(uint2);
   if ((e = hipMalloc(&c->spill, spill_bytes)) != hipSuccess ||
-      (e = hipMalloc(&c->stats, STAT_WORDS * sizeof(uint64_t))) != hipSuccess ||
+      (e = hipMalloc(&c->stats, (size_t)MAX_WAVES * STAT_WORDS * sizeof(uint64_t))) != hipSuccess ||
+      (e = hipMalloc(&c->stats_total, STAT_WORDS * sizeof(uint64_t))) != hipSuccess ||
       (e = hipMalloc(&c->qcounters, CTR_WORDS * sizeof(uint32_t))) != hipSuccess) {
     g_create_err = std::string("hipMalloc: ") + hipGetErrorString(e);
     pt_destroy(c);
@@ -271,6 +274,7 @@ void pt_destroy(pt_context* c) {
   if (c->rgba) (void)hipFree(c->rgba);
   if (c->spill) (void)hipFree(c->spill);
   if (c->stats) (void)hipFree(c->stats);
+  if (c->stats_total) (void)hipFree(c->stats_total);
   if (c->qcounters) (void)hipFree(c->qcounters);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -814,7 +818,12 @@ static pt_status ensure_paths(pt_context* c, uint64_t cap) {
     if ((e = F4(&P.st[k].o)) == hipSuccess && (e = F4(&P.st[k].d)) == hipSuccess && (e = F4(&P.st[k].T)) == hipSuccess)
       e = F4(&P.st[k].L);
   if (e || (e = F4(&P.hit)) || (e = F4(&P.hit_sorted)) || (e = U32(&P.keys)) || (e = F4(&P.result)) ||
-      (e = U32(&P.sig_final)) || (e = dalloc(B, (void**)&P.counters, CTR_WORDS * 4))) {
+      (e = U32(&P.sig_final)) || (e = dalloc(B, (void**)&P.counters, CTR_WORDS * 4)) ||
+      (e = dalloc(B, (void**)&P.seg, (size_t)3 * MAX_WAVES * 4)) ||
+      (e = dalloc(B, (void**)&P.bcount, (size_t)8 * MAX_WAVES * 4)) ||
+      (e = dalloc(B, (void**)&P.bbase, (size_t)8 * MAX_WAVES * 4)) ||
+      (e = dalloc(B, (void**)&P.tseg, (size_t)MAX_WAVES * 4)) || (e = dalloc(B, (void**)&P.qseg, (size_t)MAX_WAVES * 4)) ||
+      (e = dalloc(B, (void**)&P.tpre, (size_t)MAX_WAVES * 4)) || (e = dalloc(B, (void**)&P.qpre, (size_t)MAX_WAVES * 4))) {
     dfree(B);
     return hip_fail(c, e, "hipMalloc (path state)");
   }
@@ -825,9 +834,10 @@ static pt_status ensure_paths(pt_context* c, uint64_t cap) {
 
 static pt_status cache_work(pt_context* c, uint32_t n);
 
-// radiance-cache record buffers for `cap` path slots: one query per path,
-// training vertices for a quarter of the slots (3 % of the paths, a few
-// vertices each, fit with a wide margin; extra vertices are dropped)
+// radiance-cache record buffers for `cap` path slots: one query per path;
+// training vertices for a quarter of the slots, split among the waves of a
+// batch like the path state (3 % of the paths, a few vertices each, fit with
+// a wide margin; a wave's vertices beyond its share are dropped)
 static pt_status ensure_cache_records(pt_context* c, uint64_t cap) {
   if (cap <= c->cache_rec_cap) return PT_OK;
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -842,8 +852,7 @@ static pt_status ensure_cache_records(pt_context* c, uint64_t cap) {
       (e = dalloc(B, (void**)&P.q_out, cap * 12)) != hipSuccess ||
       (e = dalloc(B, (void**)&P.t_feat, tcap * 12 * 16)) != hipSuccess ||
       (e = dalloc(B, (void**)&P.t_L, tcap * 16)) != hipSuccess ||
-      (e = dalloc(B, (void**)&P.t_T, tcap * 16)) != hipSuccess ||
-      (e = dalloc(B, (void**)&P.t_tgt, tcap * 12)) != hipSuccess) {
+      (e = dalloc(B, (void**)&P.t_T, tcap * 16)) != hipSuccess) {
     dfree(B);
     return hip_fail(c, e, "hipMalloc (radiance-cache records)");
   }
@@ -898,7 +907,9 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
   if (ps != PT_OK) return ps;
   if (use_cache) {
     if ((ps = ensure_cache_records(c, (uint64_t)fpb * WH)) != PT_OK) return ps;
-    if ((ps = cache_work(c, c->cache_batch)) != PT_OK) return ps;
+    // dense training rows of a batch: steps x batch, at most every record
+    const uint64_t rows = std::min<uint64_t>((uint64_t)c->cache_steps * c->cache_batch, c->paths.t_cap);
+    if ((ps = cache_work(c, (uint32_t)std::max<uint64_t>(rows, c->cache_batch))) != PT_OK) return ps;
   }
 
   RenderConsts rc{};
@@ -944,7 +955,7 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
     return e;
   };
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ext_ev, sh_ev;
-  launch_fill_u32((uint32_t*)c->stats, STAT_WORDS * 2, 0u, c->stream);
+  launch_fill_u32((uint32_t*)c->stats, (uint64_t)MAX_WAVES * STAT_WORDS * 2, 0u, c->stream);
   hipEvent_t t_begin = want_stats ? mark() : nullptr;
   DevPaths& P = c->paths;
   const int matset = shade_material_set(c->mat_types);
@@ -962,11 +973,9 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
       launch_shade(c->dsc, P, in, sort, out, depth, rc, matset, count_trav, c->spill, c->stats, grid, c->stream);
       hipEvent_t b2 = events ? mark() : nullptr;
       if (events) sh_ev.push_back({a2, b2});
-      launch_bounce_reset(P, in, c->stream);
       std::swap(in, out);
     }
     launch_exhausted(P, in, c->stats, grid, c->stream);
-    launch_bounce_reset(P, in, c->stream);
   };
   if (use_graph) {
     if (!c->frame_dev) HIPCHK(c, hipMalloc(&c->frame_dev, 64));
@@ -1018,42 +1027,51 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
     if (use_graph) launch_set_frame(c->frame_dev, frame_begin + f0 + nf, c->stream);
     enqueue_batch(nslots, grid, kernel_events);
     if (use_cache) {
-      // train on this batch's training vertices, then answer its queries
-      launch_cache_targets(P, c->stream);
+      // train on this batch's training vertices (an evenly spread dense
+      // copy of steps x batch of them, with targets), then answer its queries
+      launch_cache_scan(P, grid, c->stream);
       uint32_t cnt[2] = {0, 0};  // CTR_QUERY, CTR_TRAIN
       HIPCHK(c, hipMemcpyAsync(cnt, P.counters + CTR_QUERY, 8, hipMemcpyDeviceToHost, c->stream));
       HIPCHK(c, hipStreamSynchronize(c->stream));
-      const uint32_t nq = std::min(cnt[0], P.q_cap), nt = std::min(cnt[1], P.t_cap);
+      const uint32_t nq = cnt[0], nt = cnt[1];
+      const uint32_t rows = (uint32_t)std::min<uint64_t>((uint64_t)c->cache_steps * c->cache_batch, nt);
       uint32_t steps = 0;
-      for (uint32_t k = 0; k < c->cache_steps && nt >= 32; k++) {
-        const uint32_t off = (uint32_t)(((uint64_t)k * c->cache_batch) % nt);
-        const uint32_t n = std::min(c->cache_batch, nt - off);
-        c->cache_step++;
-        launch_mlp_train((const uint16_t*)(P.t_feat + (size_t)off * 12), P.t_tgt + (size_t)off * 3, n, c->cfw, c->cbw,
-                         c->cbias, c->cX, c->cdZ, c->cdb, c->closs, c->cdw, c->cp, c->cm1, c->cm2, c->cgrad,
-                         c->cache_lr, c->cache_step, c->cfw, c->cbw, c->cbias, c->stream);
-        steps++;
+      if (rows >= 32) {
+        launch_cache_gather(P, 0, rows, grid, (uint4*)c->cfeat, c->ctgt, nullptr, nullptr, c->stream);
+        for (uint32_t k = 0; k < c->cache_steps; k++) {
+          const uint32_t off = k * c->cache_batch;
+          if (off >= rows || rows - off < 32) break;
+          const uint32_t n = std::min(c->cache_batch, rows - off);
+          c->cache_step++;
+          launch_mlp_train(c->cfeat + (size_t)off * MLP_IN, c->ctgt + (size_t)off * 3, n, c->cfw, c->cbw, c->cbias,
+                           c->cX, c->cdZ, c->cdb, c->closs, c->cdw, c->cp, c->cm1, c->cm2, c->cgrad, c->cache_lr,
+                           c->cache_step, c->cfw, c->cbw, c->cbias, c->stream);
+          steps++;
+          c->cache_last_batch = n;
+        }
       }
       if (nq) {
-        launch_mlp_infer((const uint16_t*)P.q_feat, nq, c->cfw, c->cbias, P.q_out,
-                         (int)std::min<uint32_t>(2048, (nq + 63) / 64), c->stream);
-        launch_cache_apply(P, c->stream);
+        launch_mlp_infer((const uint16_t*)P.q_feat, nslots, c->cfw, c->cbias, P.q_out, 2048, c->stream, P.qseg,
+                         (uint32_t)grid * BLOCK);
+        launch_cache_apply(P, grid, c->stream);
       }
       c->cache_last_train = nt;
       c->cache_last_query = nq;
       c->cache_last_steps = steps;
+      c->cache_last_grid = grid;
     }
     launch_accumulate(P, rc, nf, c->accum, grid_for(WH), c->stream);
     c->last_fpb = nf;
   }
   hipEvent_t t_end = want_stats ? mark() : nullptr;
+  if (want_stats) launch_stats_reduce(c->stats, c->stats_total, c->stream);
   c->frames = frame_begin + frame_count;
   c->paths_valid = true;
   if (want_stats) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     uint64_t st[STAT_WORDS];
     uint32_t ctr[CTR_WORDS];
-    HIPCHK(c, hipMemcpy(st, c->stats, sizeof(st), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(st, c->stats_total, sizeof(st), hipMemcpyDeviceToHost));
     HIPCHK(c, hipMemcpy(ctr, P.counters, sizeof(ctr), hipMemcpyDeviceToHost));
     std::memset(out, 0, sizeof(*out));
     out->paths = (uint64_t)frame_count * WH;
@@ -1071,7 +1089,7 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
     out->cache_steps = c->cache_last_steps;
     out->cache_loss = 0.0;
     if (use_cache && c->cache_last_steps) {
-      const uint32_t n = std::min(c->cache_batch, c->cache_last_train);
+      const uint32_t n = c->cache_last_batch;
       const uint32_t waves = mlp_fwd_bwd_waves(n);
       std::vector<float> lp(waves);
       HIPCHK(c, hipMemcpy(lp.data(), c->closs, waves * 4, hipMemcpyDeviceToHost));
@@ -1330,40 +1348,33 @@ pt_status pt_cache_configure(pt_context* c, uint32_t query_depth, float train_fr
 pt_status pt_cache_read_records(pt_context* c, int32_t which, uint32_t max, uint16_t* features, float* values,
                                 uint32_t* slots, uint32_t* count) {
   if (!c || (which != 0 && which != 1)) return PT_ERR_INVALID_ARG;
-  if (!c->cache_rec_cap) return fail(c, PT_ERR_STATE, "no radiance-cache render yet");
+  if (!c->cache_rec_cap || !c->cache_last_grid) return fail(c, PT_ERR_STATE, "no radiance-cache render yet");
   (void)hipSetDevice(c->device);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   const DevPaths& P = c->paths;
-  const uint32_t n = std::min(max, which == 0 ? c->cache_last_train : c->cache_last_query);
-  if (count) *count = which == 0 ? c->cache_last_train : c->cache_last_query;
+  const uint32_t total = which == 0 ? c->cache_last_train : c->cache_last_query;
+  const uint32_t n = std::min(max, total);
+  if (count) *count = total;
   if (n == 0) return PT_OK;
-  if (features)
-    HIPCHK(c, hipMemcpy(features, which == 0 ? P.t_feat : P.q_feat, (size_t)n * 192, hipMemcpyDeviceToHost));
-  std::vector<float4> a(n), b(n);
-  std::vector<float> t((size_t)n * 3);
-  if (which == 0) {
-    HIPCHK(c, hipMemcpy(a.data(), P.t_L, (size_t)n * 16, hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(b.data(), P.t_T, (size_t)n * 16, hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(t.data(), P.t_tgt, (size_t)n * 12, hipMemcpyDeviceToHost));
-  } else {
-    HIPCHK(c, hipMemcpy(a.data(), P.q_meta, (size_t)n * 16, hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(t.data(), P.q_out, (size_t)n * 12, hipMemcpyDeviceToHost));
+  // every record in order (rows == total), copied densely, then the first n
+  std::vector<DevBuf> tmp;
+  uint4* dfeat = nullptr;
+  float* dvals = nullptr;
+  uint32_t* dslots = nullptr;
+  hipError_t e;
+  if ((e = dalloc(tmp, (void**)&dfeat, (size_t)total * 192)) != hipSuccess ||
+      (e = dalloc(tmp, (void**)&dvals, (size_t)total * 36)) != hipSuccess ||
+      (e = dalloc(tmp, (void**)&dslots, (size_t)total * 4)) != hipSuccess) {
+    dfree(tmp);
+    return hip_fail(c, e, "hipMalloc (record copies)");
   }
-  for (uint32_t i = 0; i < n; i++) {
-    uint32_t slot;
-    std::memcpy(&slot, &a[i].w, 4);
-    if (slots) slots[i] = slot & SLOT_MASK;
-    if (!values) continue;
-    float* v = values + 9ull * i;
-    if (which == 0) {
-      v[0] = a[i].x; v[1] = a[i].y; v[2] = a[i].z;
-      v[3] = b[i].x; v[4] = b[i].y; v[5] = b[i].z;
-    } else {
-      v[0] = a[i].x; v[1] = a[i].y; v[2] = a[i].z;
-      v[3] = v[4] = v[5] = 0.f;
-    }
-    v[6] = t[3 * i]; v[7] = t[3 * i + 1]; v[8] = t[3 * i + 2];
-  }
+  launch_cache_gather(P, which, total, c->cache_last_grid, dfeat, nullptr, dvals, dslots, c->stream);
+  e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess && features) e = hipMemcpy(features, dfeat, (size_t)n * 192, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && values) e = hipMemcpy(values, dvals, (size_t)n * 36, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && slots) e = hipMemcpy(slots, dslots, (size_t)n * 4, hipMemcpyDeviceToHost);
+  dfree(tmp);
+  if (e != hipSuccess) return hip_fail(c, e, "reading radiance-cache records");
   return PT_OK;
 }
 

@@ -46,15 +46,21 @@ void launch_shade(const DevScene& sc, const DevPaths& P, int in, bool sorted, in
                   const RenderConsts& rc, int matset, bool count, uint2* spill, uint64_t* stats, int grid,
                   hipStream_t st);
 int shade_material_set(uint32_t types_mask);
-void launch_bounce_reset(const DevPaths& P, int in, hipStream_t st);
 // environment importance-sampling tables of a W x H float4 map; scratch
 // holds 1 + H 64-bit words
 void launch_env_tables(const float4* tex, uint32_t W, uint32_t H, unsigned long long* cond,
                        unsigned long long* marg, unsigned long long* scratch, hipStream_t st);
-// radiance cache in the render loop (pt_render.hip)
-void launch_cache_targets(const DevPaths& P, hipStream_t st);
-void launch_cache_apply(const DevPaths& P, hipStream_t st);
+// radiance cache in the render loop (pt_render.hip); `grid` is the batch's
+// grid (its waves own the record segments)
+void launch_cache_scan(const DevPaths& P, int grid, hipStream_t st);
+// dense copies of `rows` records spread over the batch's records (which 0:
+// training vertices with targets, 1: queries); tgt, vals, slots optional
+void launch_cache_gather(const DevPaths& P, int which, uint32_t rows, int grid, uint4* feat, float* tgt, float* vals,
+                         uint32_t* slots, hipStream_t st);
+void launch_cache_apply(const DevPaths& P, int grid, hipStream_t st);
 void launch_exhausted(const DevPaths& P, int in, uint64_t* stats, int grid, hipStream_t st);
+// per-wave statistics rows [MAX_WAVES][STAT_WORDS] -> totals [STAT_WORDS]
+void launch_stats_reduce(const uint64_t* part, uint64_t* total, hipStream_t st);
 void launch_accumulate(const DevPaths& P, const RenderConsts& rc, uint32_t nframes, double* accum, int grid,
                        hipStream_t st);
 void launch_display(const double* accum, uint32_t npix, uint32_t frames, uchar4* rgba, int grid, hipStream_t st);
@@ -63,8 +69,10 @@ void launch_fill_f64(double* p, uint64_t n, double v, hipStream_t st);
 
 // radiance-cache network (pt_mlp.hip)
 void launch_mlp_pack(const float* params, uint16_t* fw, uint16_t* bw, float* bias, hipStream_t st);
+// rows 0..n-1 dense, or (seg non-null) positions of a wave-segmented set
+// with the given stride and per-wave counts
 void launch_mlp_infer(const uint16_t* feats, uint32_t n, const uint16_t* fw, const float* bias, float* out, int grid,
-                      hipStream_t st);
+                      hipStream_t st, const uint32_t* seg = nullptr, uint32_t stride = 0);
 uint32_t mlp_padded(uint32_t n);
 uint32_t mlp_fwd_bwd_waves(uint32_t n);
 uint32_t mlp_chunks(uint32_t n);

@@ -41,6 +41,10 @@ constexpr int INST_WORDS = 32;
 constexpr int BLOCK = 256;
 constexpr int STK_LDS = 16;
 constexpr int STK_SPILL = 48;
+// Grid of the path kernels: at most MAX_GRID blocks of BLOCK / 64 waves.
+constexpr int MAX_GRID = 2048;
+constexpr int WAVES = BLOCK / 64;
+constexpr int MAX_WAVES = MAX_GRID * WAVES;
 
 struct DevMaterial {
   int32_t type;
@@ -125,6 +129,15 @@ struct DevCamera {
 // Path state in HBM: 64 bytes per live path, structure of arrays, compacted
 // every bounce into the next ping-pong set so that every kernel streams it
 // contiguously.
+//
+// Wave segments: a state set is split among the waves of the batch's grid
+// (G blocks).  Wave w = (block b, wave v of the block) owns the positions
+//   pos(j) = (j / 64) * G * BLOCK + b * BLOCK + v * 64 + j % 64,  j = 0 .. n_w - 1
+// and n_w is seg[set * MAX_WAVES + w].  Every kernel of a batch runs on the
+// same grid; a wave reads its own segment of the input set and compacts its
+// survivors (never more than its inputs) into its own segment of the output
+// set, so the hot loop takes no global atomic.  A dense set of n positions
+// is the case n_w = (positions of the segment below n).
 constexpr uint32_t SLOT_MASK = 0x3FFFFFFFu;
 constexpr uint32_t TRAIN_BIT = 1u << 30;  // radiance-cache training path
 constexpr uint32_t SPEC_BIT = 1u << 31;   // last bounce was specular
@@ -144,7 +157,14 @@ struct DevPaths {
   float4* result;       // per slot: (color() value, bounces bits)
   uint32_t* sig_final;  // per slot: decision signature
   uint32_t* counters;   // see CTR_*
-  // radiance-cache records (PT_FLAG_RADIANCE_CACHE)
+  uint32_t* seg;        // [3][MAX_WAVES] live paths per wave segment of each state set
+  uint32_t* bcount;     // [8][MAX_WAVES] paths per material bucket and wave (sorted shading)
+  uint32_t* bbase;      // [8][MAX_WAVES] first sorted position of each (bucket, wave)
+  // radiance-cache records (PT_FLAG_RADIANCE_CACHE), kept in wave segments
+  // like the path state: queries of the query-depth bounce in each wave's
+  // segment of the q_* arrays (counts qseg), training vertices of the whole
+  // batch in each wave's segment of the t_* arrays (running counts tseg,
+  // capped by t_cap); tpre / qpre are the exclusive prefix sums over waves
   uint4* q_feat;        // per query: 96 BF16 features (12 uint4)
   float4* q_meta;       // per query: (T.xyz, slot bits)
   float* q_out;         // per query: network output (3 floats)
@@ -152,20 +172,22 @@ struct DevPaths {
   uint4* t_feat;        // per training vertex: 96 BF16 features
   float4* t_L;          // per training vertex: (L before the vertex, slot bits)
   float4* t_T;          // per training vertex: (T at the vertex, 0)
-  float* t_tgt;         // per training vertex: log(1 + radiance from the vertex on)
   uint32_t t_cap;
+  uint32_t* tseg;       // [MAX_WAVES]
+  uint32_t* qseg;       // [MAX_WAVES]
+  uint32_t* tpre;       // [MAX_WAVES]
+  uint32_t* qpre;       // [MAX_WAVES]
 };
 
 // device counters (uint32)
-constexpr int CTR_ACTIVE = 0;          // [0], [1]: live paths of state set 0 / 1
-constexpr int CTR_BUCKETS = 4;         // 8 material buckets
-constexpr int CTR_BUCKET_CURSOR = 12;  // 8 scatter cursors
+constexpr int CTR_SORTED = 0;          // paths in the material-sorted set
 constexpr int CTR_STACK_OVERFLOW = 20;
 constexpr int CTR_QUERY = 21;          // radiance-cache queries of the batch
 constexpr int CTR_TRAIN = 22;          // radiance-cache training vertices of the batch
 constexpr int CTR_WORDS = 32;
 // 64-bit statistics: [0] extension rays, [1] shadow rays, [2] nodes,
-// [3] prims, [4] instances
+// [3] prims, [4] instances, ... kept per wave ([MAX_WAVES][STAT_WORDS],
+// each row updated by its own wave only) and summed once per render.
 constexpr int STAT_EXT = 0, STAT_SHADOW = 1, STAT_NODES = 2, STAT_PRIMS = 3, STAT_INSTS = 4;
 constexpr int STAT_SH_NODES = 5, STAT_SH_PRIMS = 6, STAT_SH_INSTS = 7, STAT_EXHAUSTED = 8;
 constexpr int STAT_WORDS = 16;

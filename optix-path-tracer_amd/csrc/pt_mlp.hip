@@ -163,9 +163,13 @@ PTD void stage(uint4* dst, const uint4* src, int n) {
   for (int i = threadIdx.x; i < n; i += MB) dst[i] = src[i];
 }
 
-// Inference on n samples (BF16 rows of 96 features = 12 uint4 each).
+// Inference on n rows (BF16 rows of 96 features = 12 uint4 each).  With
+// `seg`, the rows are positions of a wave-segmented record set (segments of
+// a grid whose stride is `stride` positions, counts seg[w]): a 16-row tile
+// lies in one 64-position run of one segment and only rows below the
+// segment's count are evaluated.
 __global__ __launch_bounds__(MB) void k_mlp_infer(const uint4* feats, uint32_t n, const uint4* fw_g, const float* bias_g,
-                                                  float* out) {
+                                                  float* out, const uint32_t* seg, uint32_t stride) {
   __shared__ uint4 fw[FW_TOTAL];
   __shared__ float bias[4 * 64 + 16];
   stage(fw, fw_g, FW_TOTAL);
@@ -176,6 +180,13 @@ __global__ __launch_bounds__(MB) void k_mlp_infer(const uint4* feats, uint32_t n
   for (uint32_t base = (uint32_t)wave * 16; base < n; base += gridDim.x * (MB / 64) * 16) {
     uint32_t row = base + q;
     bool valid = row < n;
+    if (seg) {
+      const uint32_t run = base / stride, rem = base - run * stride;
+      const uint32_t live = __builtin_amdgcn_readfirstlane(seg[rem >> 6]);
+      const uint32_t j0 = run * 64u + (rem & 63u);
+      if (j0 >= live) continue;  // the whole tile is past the segment's count
+      valid = valid && j0 + (uint32_t)q < live;
+    }
     Fwd f;
     for (int s = 0; s < 3; s++) f.x[s] = valid ? feats[(size_t)row * 12 + 4 * s + g] : make_uint4(0, 0, 0, 0);
     forward(fw, bias, g, f);
@@ -373,8 +384,9 @@ void launch_mlp_pack(const float* params, uint16_t* fw, uint16_t* bw, float* bia
   hipLaunchKernelGGL(k_mlp_pack, dim3((MLP_PARAMS + MB - 1) / MB), dim3(MB), 0, st, params, fw, bw, bias);
 }
 void launch_mlp_infer(const uint16_t* feats, uint32_t n, const uint16_t* fw, const float* bias, float* out, int grid,
-                      hipStream_t st) {
-  hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(MB), 0, st, (const uint4*)feats, n, (const uint4*)fw, bias, out);
+                      hipStream_t st, const uint32_t* seg, uint32_t stride) {
+  hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(MB), 0, st, (const uint4*)feats, n, (const uint4*)fw, bias, out, seg,
+                     stride);
 }
 uint32_t mlp_padded(uint32_t n) { return (n + 31) & ~31u; }
 uint32_t mlp_fwd_bwd_waves(uint32_t n) { return mlp_padded(n) / 16; }

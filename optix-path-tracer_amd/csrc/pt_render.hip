@@ -7,9 +7,11 @@
 // Wavefront, one bounce at a time over SoA path state in HBM:
 //   raygen -> [extend -> (material sort) -> shade -> shadow+connect]*
 //   -> accumulate
-// Queues are slot lists compacted with wave64 ballots: one atomic per wave.
-// All kernels are grid-stride loops that read their item count from device
-// memory, so a whole batch is enqueued without host synchronisation.
+// Path sets are split into wave segments (pt_layout.hpp): each wave reads
+// its own segment and compacts its survivors into its own segment of the
+// next set with a ballot, so the bounce loop takes no global atomic.  The
+// counts live in device memory, so a whole batch is enqueued without host
+// synchronisation.
 #include "pt_kernels.hpp"
 #include "pt_shade.hpp"
 
@@ -19,16 +21,30 @@ namespace pt {
 PTD uint32_t lane_rank(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
-// Wave-aggregated append: the lanes with `pred` get consecutive indices from
-// one atomic on `counter` (usable in divergent code: only active lanes vote).
-PTD uint32_t append_index(bool pred, uint32_t* counter) {
-  uint64_t m = __ballot(pred);
-  if (!pred) return 0u;
-  int leader = __ffsll((unsigned long long)m) - 1;
-  uint32_t base = 0;
-  if ((int)__lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-  base = __shfl(base, leader);
-  return base + lane_rank(m);
+// This wave's segment of a path set (layout in pt_layout.hpp).
+struct Seg {
+  uint32_t stride;  // G * BLOCK
+  uint32_t base;    // b * BLOCK + v * 64
+  uint32_t w;       // wave index in the grid
+  PTD uint32_t pos(uint32_t j) const { return (j >> 6) * stride + base + (j & 63u); }
+};
+PTD Seg my_seg() {
+  const uint32_t v = threadIdx.x >> 6;
+  return Seg{gridDim.x * BLOCK, blockIdx.x * BLOCK + v * 64u, blockIdx.x * WAVES + v};
+}
+// positions of the segment below n: the segment count of a dense set
+PTD uint32_t dense_count(uint32_t n, const Seg& g) {
+  const uint32_t full = n / g.stride, r = n - full * g.stride;
+  return full * 64u + (r > g.base ? min(r - g.base, 64u) : 0u);
+}
+PTD uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+// live paths of this wave's segment of state set `set`
+PTD uint32_t seg_count(const DevPaths& P, int set, const Seg& g) {
+  return uniform(P.seg[(size_t)set * MAX_WAVES + g.w]);
+}
+// per-wave statistics row (one writer: lane 0 of the wave)
+PTD void stat_add(uint64_t* stats, const Seg& g, int k, uint64_t v) {
+  if (v) stats[(size_t)g.w * STAT_WORDS + k] += v;
 }
 
 // 24-bit hash of (launch index, frame) choosing radiance-cache training
@@ -171,9 +187,16 @@ __global__ __launch_bounds__(BLOCK) void k_trace_occluded(DevScene sc, const pt_
 // ------------------------------------------------------ wavefront kernels
 // raygen.cu:148-168 renderPixel up to the first rtTrace, raygen.cu:50-64
 // generateRay, raygen.cu:66-74 color() set-up.
+// Slot s is generated at position s of state set 0 (a dense set).
 __global__ __launch_bounds__(BLOCK) void k_raygen(DevPaths P, RenderConsts rc, uint32_t nslots) {
   PathState S = P.st[0];
   if (rc.frame_dev) rc.frame0 = *rc.frame_dev;  // replayed graphs take the batch's frame from memory
+  const Seg g = my_seg();
+  if (__lane_id() == 0) {
+    P.seg[g.w] = dense_count(nslots, g);
+    P.tseg[g.w] = 0u;
+    P.qseg[g.w] = 0u;
+  }
   for (uint32_t s = blockIdx.x * BLOCK + threadIdx.x; s < nslots; s += gridDim.x * BLOCK) {
     uint32_t f = s / rc.WH, pix = s - f * rc.WH;
     uint32_t x = pix % rc.W, y = pix / rc.W;
@@ -196,8 +219,6 @@ __global__ __launch_bounds__(BLOCK) void k_raygen(DevPaths P, RenderConsts rc, u
     S.L[s] = make_float4(0.f, 0.f, 0.f, __uint_as_float(0x811C9DC5u));
     P.result[s] = make_float4(0.f, 0.f, 0.f, __uint_as_float(rc.max_depth));
   }
-  if (blockIdx.x == 0 && threadIdx.x < CTR_STACK_OVERFLOW) P.counters[threadIdx.x] = threadIdx.x == 0 ? nslots : 0u;
-  if (blockIdx.x == 0 && (threadIdx.x == CTR_QUERY || threadIdx.x == CTR_TRAIN)) P.counters[threadIdx.x] = 0u;
 }
 
 // material bucket of a hit: 0 = miss, 1 + material type otherwise
@@ -207,27 +228,22 @@ PTD uint32_t material_key(const DevScene& sc, float4 h) {
 }
 
 // Closest-hit query of every live path (raygen.cu:77).  With SORT the
-// material bucket of each hit is counted (per wave by ballot, per block in
-// LDS, one global atomic per bucket and block).
+// material bucket of each hit is recorded and counted per wave and bucket.
 template <bool COUNT, bool SORT>
 __global__ __launch_bounds__(BLOCK) void k_extend(DevScene sc, DevPaths P, int in, float tmin, uint2* spill,
                                                   uint64_t* stats) {
   __shared__ uint32_t s_ref[STK_LDS * BLOCK];
   __shared__ float s_tn[STK_LDS * BLOCK];
-  __shared__ uint32_t bcnt[8];
   Stack st = make_stack(s_ref, s_tn, spill, P.counters + CTR_STACK_OVERFLOW);
-  const uint32_t n = P.counters[CTR_ACTIVE + in];
+  const Seg g = my_seg();
+  const uint32_t n = seg_count(P, in, g);
   const PathState S = P.st[in];
-  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd((unsigned long long*)&stats[STAT_EXT], (unsigned long long)n);
-  if (SORT) {
-    if (threadIdx.x < 8) bcnt[threadIdx.x] = 0;
-    __syncthreads();
-  }
+  uint32_t bc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   TraceCounts c{0, 0, 0};
-  for (uint32_t i0 = blockIdx.x * BLOCK; i0 < n; i0 += gridDim.x * BLOCK) {
-    uint32_t i = i0 + threadIdx.x;
+  for (uint32_t j0 = 0; j0 < n; j0 += 64) {
+    const uint32_t j = j0 + __lane_id(), i = g.pos(j);
     uint32_t key = 0xFFu;
-    if (i < n) {
+    if (j < n) {
       float4 o = S.o[i], d = S.d[i];
       HitOut h = trace_closest<COUNT>(sc, mk3(o), mk3(d), tmin, FINF, st, c);
       float4 hv = make_float4(h.t, __int_as_float(h.pos), __int_as_float(h.prim), __int_as_float(h.inst));
@@ -239,74 +255,89 @@ __global__ __launch_bounds__(BLOCK) void k_extend(DevScene sc, DevPaths P, int i
     }
     if (SORT) {
 #pragma unroll
-      for (uint32_t b = 0; b < 8; b++) {
-        uint64_t m = __ballot(key == b);
-        if (m && key == b && lane_rank(m) == 0) atomicAdd(&bcnt[b], (uint32_t)__popcll(m));
-      }
+      for (uint32_t b = 0; b < 8; b++) bc[b] += (uint32_t)__popcll(__ballot(key == b));
     }
   }
-  if (SORT) {
-    __syncthreads();
-    if (threadIdx.x < 8 && bcnt[threadIdx.x]) atomicAdd(&P.counters[CTR_BUCKETS + threadIdx.x], bcnt[threadIdx.x]);
-  }
+  uint32_t a = c.nodes, b = c.prims, d = c.insts;
   if (COUNT) {
-    uint32_t a = c.nodes, b = c.prims, d = c.insts;
     for (int off = 32; off > 0; off >>= 1) {
       a += __shfl_down(a, off);
       b += __shfl_down(b, off);
       d += __shfl_down(d, off);
     }
-    if (__lane_id() == 0) {
-      atomicAdd((unsigned long long*)&stats[STAT_NODES], (unsigned long long)a);
-      atomicAdd((unsigned long long*)&stats[STAT_PRIMS], (unsigned long long)b);
-      atomicAdd((unsigned long long*)&stats[STAT_INSTS], (unsigned long long)d);
+  }
+  if (__lane_id() == 0) {
+    if (SORT) {
+#pragma unroll
+      for (int k = 0; k < 8; k++) P.bcount[(size_t)k * MAX_WAVES + g.w] = bc[k];
+    }
+    stat_add(stats, g, STAT_EXT, n);
+    if (COUNT) {
+      stat_add(stats, g, STAT_NODES, a);
+      stat_add(stats, g, STAT_PRIMS, b);
+      stat_add(stats, g, STAT_INSTS, d);
     }
   }
 }
 
-__global__ void k_sort_offsets(DevPaths P) {
-  uint32_t acc = 0;
-  for (int b = 0; b < 8; b++) {
-    P.counters[CTR_BUCKET_CURSOR + b] = acc;
-    acc += P.counters[CTR_BUCKETS + b];
-  }
-}
-// Move path state and hit into material-bucket order (state set 2); cursors
-// are claimed per wave (ballot) and per block (LDS).
-__global__ __launch_bounds__(BLOCK) void k_sort_scatter(DevPaths P, int in) {
-  __shared__ uint32_t cnt[8], base[8];
-  const uint32_t n = P.counters[CTR_ACTIVE + in];
-  const PathState S = P.st[in], D = P.st[2];
-  for (uint32_t i0 = blockIdx.x * BLOCK; i0 < n; i0 += gridDim.x * BLOCK) {
-    uint32_t i = i0 + threadIdx.x;
-    uint32_t key = i < n ? P.keys[i] : 0xFFu;
-    if (threadIdx.x < 8) cnt[threadIdx.x] = 0;
+// Exclusive scan of the (bucket, wave) counts in bucket-major order: the
+// first sorted position of each wave's paths of each bucket, and the size
+// of the sorted set.  One block of SCAN_THREADS threads, SCAN_PER entries
+// per thread.
+constexpr int SCAN_THREADS = 1024;
+__global__ __launch_bounds__(SCAN_THREADS) void k_sort_scan(DevPaths P, uint32_t waves) {
+  __shared__ uint32_t part[SCAN_THREADS];
+  const uint32_t total = 8u * waves;
+  const uint32_t per = (total + SCAN_THREADS - 1) / SCAN_THREADS;
+  const uint32_t e0 = threadIdx.x * per, e1 = min(e0 + per, total);
+  auto at = [&](uint32_t e) -> size_t { return (size_t)(e / waves) * MAX_WAVES + e % waves; };
+  uint32_t sum = 0;
+  for (uint32_t e = e0; e < e1; e++) sum += P.bcount[at(e)];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  // Hillis-Steele inclusive scan of the per-thread sums
+  for (int off = 1; off < SCAN_THREADS; off <<= 1) {
+    uint32_t v = threadIdx.x >= (uint32_t)off ? part[threadIdx.x - off] : 0u;
     __syncthreads();
-    uint32_t wave_off = 0;
-    uint64_t mine = 0;
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
+  for (uint32_t e = e0; e < e1; e++) {
+    const size_t k = at(e);
+    P.bbase[k] = run;
+    run += P.bcount[k];
+  }
+  if (threadIdx.x == SCAN_THREADS - 1) P.counters[CTR_SORTED] = part[SCAN_THREADS - 1];
+}
+// Move path state and hit into material-bucket order (state set 2, a dense
+// set): each wave writes its bucket-b paths from its scanned base on.
+__global__ __launch_bounds__(BLOCK) void k_sort_scatter(DevPaths P, int in) {
+  const Seg g = my_seg();
+  const uint32_t n = seg_count(P, in, g);
+  const PathState S = P.st[in], D = P.st[2];
+  uint32_t cur[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) cur[k] = uniform(P.bbase[(size_t)k * MAX_WAVES + g.w]);
+  for (uint32_t j0 = 0; j0 < n; j0 += 64) {
+    const uint32_t j = j0 + __lane_id(), i = g.pos(j);
+    const uint32_t key = j < n ? P.keys[i] : 0xFFu;
+    uint32_t dst = 0;
 #pragma unroll
     for (uint32_t b = 0; b < 8; b++) {
-      uint64_t m = __ballot(key == b);
-      uint32_t o = 0;
-      if (m && key == b && lane_rank(m) == 0) o = atomicAdd(&cnt[b], (uint32_t)__popcll(m));
-      if (key == b) mine = m;
-      o = __shfl(o, m ? (int)(__ffsll((unsigned long long)m) - 1) : 0);
-      if (key == b) wave_off = o;
+      const uint64_t m = __ballot(key == b);
+      if (key == b) dst = cur[b] + lane_rank(m);
+      cur[b] += (uint32_t)__popcll(m);
     }
-    __syncthreads();
-    if (threadIdx.x < 8)
-      base[threadIdx.x] = cnt[threadIdx.x] ? atomicAdd(&P.counters[CTR_BUCKET_CURSOR + threadIdx.x], cnt[threadIdx.x]) : 0u;
-    __syncthreads();
     if (key < 8) {
-      uint32_t j = base[key] + wave_off + lane_rank(mine);
-      D.o[j] = S.o[i];
-      D.d[j] = S.d[i];
-      D.T[j] = S.T[i];
-      D.L[j] = S.L[i];
-      P.hit_sorted[j] = P.hit[i];
+      D.o[dst] = S.o[i];
+      D.d[dst] = S.d[i];
+      D.T[dst] = S.T[i];
+      D.L[dst] = S.L[i];
+      P.hit_sorted[dst] = P.hit[i];
     }
-    __syncthreads();
   }
+  if (__lane_id() == 0) P.seg[2 * MAX_WAVES + g.w] = dense_count(P.counters[CTR_SORTED], g);
 }
 
 // Scatter of the materials that do not sample lights plus the continuation
@@ -404,9 +435,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
   __shared__ uint32_t s_ref[STK_LDS * BLOCK];
   __shared__ float s_tn[STK_LDS * BLOCK];
   Stack st = make_stack(s_ref, s_tn, spill, P.counters + CTR_STACK_OVERFLOW);
-  const uint32_t n = P.counters[CTR_ACTIVE + in];
+  const Seg g = my_seg();
+  const uint32_t n = seg_count(P, src, g);
   const PathState S = P.st[src], D = P.st[out];
-  uint32_t* out_count = P.counters + CTR_ACTIVE + out;
+  uint32_t survivors = 0;  // wave-uniform: paths written to this wave's segment of `out`
   // light table: environment lights take part only with PT_FLAG_ENV_MIS
   constexpr bool COUNT = (FEAT & SH_COUNT) != 0;
   const bool env_mis = (FEAT & SH_ENV) && (rc.flags & PT_FLAG_ENV_MIS) != 0;
@@ -416,11 +448,18 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
   const bool cache_on = (FEAT & SH_CACHE) && (rc.flags & PT_FLAG_RADIANCE_CACHE) != 0;
   TraceCounts tc{0, 0, 0};
   uint32_t shadow_rays = 0;
-  for (uint32_t i0 = blockIdx.x * BLOCK; i0 < n; i0 += gridDim.x * BLOCK) {
-    uint32_t i = i0 + threadIdx.x;
-    bool alive = false;
+  // radiance-cache records of this wave (wave-uniform running counts):
+  // training vertices continue the batch's count, queries start at zero
+  uint32_t tcount = 0, qcount = 0, tcap = 0;
+  if (cache_on) {
+    tcount = uniform(P.tseg[g.w]);
+    tcap = dense_count(P.t_cap, g);
+  }
+  for (uint32_t j0 = 0; j0 < n; j0 += 64) {
+    const uint32_t i = g.pos(j0 + __lane_id());
+    bool alive = false, did_rec = false, did_ask = false;
     float4 no = make_float4(0.f, 0.f, 0.f, 0.f), nd = no, nT = no, nL = no;
-    if (i < n) {
+    if (j0 + __lane_id() < n) {
       float4 hit = hits[i];
       float4 o4 = S.o[i], d4 = S.d[i], t4 = S.T[i], l4 = S.L[i];
       v3 d = mk3(d4), T = mk3(t4), L = mk3(l4);
@@ -460,21 +499,26 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
         if (cache_on) {
           // training paths record every vertex (radiance so far, throughput);
           // other paths end at the query depth with a cache lookup
+          // (every lane with rec or ask is active here, so the ballots rank
+          // exactly the lanes the converged counts below add up)
           const bool rec = (dw & TRAIN_BIT) != 0;
           const bool ask = !rec && depth == rc.cache.query_depth;
-          uint32_t ti = append_index(rec, P.counters + CTR_TRAIN);
-          uint32_t qi = append_index(ask, P.counters + CTR_QUERY);
-          if (rec && ti < P.t_cap) {
-            write_features(P.t_feat + (size_t)ti * 12, rc.cache, r.P, r.Wo, r.N, f);
-            P.t_L[ti] = make_float4(L.x, L.y, L.z, __uint_as_float(dw & SLOT_MASK));
-            P.t_T[ti] = make_float4(T.x, T.y, T.z, 0.f);
+          const uint32_t tj = tcount + lane_rank(__ballot(rec));
+          const uint32_t qj = qcount + lane_rank(__ballot(ask));
+          did_rec = rec;
+          did_ask = ask;
+          if (rec && tj < tcap) {
+            const uint32_t k = g.pos(tj);
+            write_features(P.t_feat + (size_t)k * 12, rc.cache, r.P, r.Wo, r.N, f);
+            P.t_L[k] = make_float4(L.x, L.y, L.z, __uint_as_float(dw & SLOT_MASK));
+            P.t_T[k] = make_float4(T.x, T.y, T.z, 0.f);
           }
           if (ask) {
+            // queries never outnumber the wave's inputs: the position exists
             queried = true;
-            if (qi < P.q_cap) {
-              write_features(P.q_feat + (size_t)qi * 12, rc.cache, r.P, r.Wo, r.N, f);
-              P.q_meta[qi] = make_float4(T.x, T.y, T.z, __uint_as_float(dw & SLOT_MASK));
-            }
+            const uint32_t k = g.pos(qj);
+            write_features(P.q_feat + (size_t)k * 12, rc.cache, r.P, r.Wo, r.N, f);
+            P.q_meta[k] = make_float4(T.x, T.y, T.z, __uint_as_float(dw & SLOT_MASK));
             res = make_float4(L.x, L.y, L.z, __uint_as_float(depth + 1));
             nee = false;
           }
@@ -549,20 +593,20 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
         P.sig_final[dw & SLOT_MASK] = sig;
       }
     }
-    // wave-aggregated compaction of the survivors into set `out`
-    uint64_t mask = __ballot(alive);
-    if (mask) {
-      int leader = __ffsll((unsigned long long)mask) - 1;
-      uint32_t base = 0;
-      if ((int)__lane_id() == leader) base = atomicAdd(out_count, (uint32_t)__popcll(mask));
-      base = __shfl(base, leader);
-      if (alive) {
-        uint32_t j = base + lane_rank(mask);
-        D.o[j] = no;
-        D.d[j] = nd;
-        D.T[j] = nT;
-        D.L[j] = nL;
-      }
+    // compaction of the survivors into this wave's segment of set `out`
+    // (never more than the wave's inputs, so every position exists)
+    const uint64_t mask = __ballot(alive);
+    if (alive) {
+      const uint32_t k = g.pos(survivors + lane_rank(mask));
+      D.o[k] = no;
+      D.d[k] = nd;
+      D.T[k] = nT;
+      D.L[k] = nL;
+    }
+    survivors += (uint32_t)__popcll(mask);
+    if (cache_on) {
+      tcount = min(tcount + (uint32_t)__popcll(__ballot(did_rec)), tcap);
+      qcount += (uint32_t)__popcll(__ballot(did_ask));
     }
   }
   uint32_t a = tc.nodes, b = tc.prims, c = tc.insts;
@@ -575,53 +619,141 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
     }
   }
   if (__lane_id() == 0) {
-    if (shadow_rays) atomicAdd((unsigned long long*)&stats[STAT_SHADOW], (unsigned long long)shadow_rays);
+    P.seg[(size_t)out * MAX_WAVES + g.w] = survivors;
+    if (cache_on) {
+      P.tseg[g.w] = tcount;
+      if (depth == rc.cache.query_depth) P.qseg[g.w] = qcount;
+    }
+    stat_add(stats, g, STAT_SHADOW, shadow_rays);
     if (COUNT) {
-      atomicAdd((unsigned long long*)&stats[STAT_SH_NODES], (unsigned long long)a);
-      atomicAdd((unsigned long long*)&stats[STAT_SH_PRIMS], (unsigned long long)b);
-      atomicAdd((unsigned long long*)&stats[STAT_SH_INSTS], (unsigned long long)c);
+      stat_add(stats, g, STAT_SH_NODES, a);
+      stat_add(stats, g, STAT_SH_PRIMS, b);
+      stat_add(stats, g, STAT_SH_INSTS, c);
     }
   }
-}
-
-// End of a bounce: the consumed set becomes the next output; buckets reset.
-__global__ void k_bounce_reset(DevPaths P, int in) {
-  if (threadIdx.x == 0) P.counters[CTR_ACTIVE + in] = 0;
-  if (threadIdx.x < 8) P.counters[CTR_BUCKETS + threadIdx.x] = 0;
 }
 
 // Paths alive after the last bounce end with color() = 0 (raygen.cu:121-122);
 // only their signature is recorded.
 __global__ __launch_bounds__(BLOCK) void k_exhausted(DevPaths P, int in, uint64_t* stats) {
-  const uint32_t n = P.counters[CTR_ACTIVE + in];
+  const Seg g = my_seg();
+  const uint32_t n = seg_count(P, in, g);
   const PathState S = P.st[in];
-  if (blockIdx.x == 0 && threadIdx.x == 0 && n)
-    atomicAdd((unsigned long long*)&stats[STAT_EXHAUSTED], (unsigned long long)n);
-  for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK)
+  if (__lane_id() == 0) stat_add(stats, g, STAT_EXHAUSTED, n);
+  for (uint32_t j = __lane_id(); j < n; j += 64) {
+    const uint32_t i = g.pos(j);
     P.sig_final[__float_as_uint(S.d[i].w) & SLOT_MASK] = __float_as_uint(S.L[i].w);
-}
-
-// Radiance-cache training targets: log(1 + radiance leaving each recorded
-// vertex toward the camera side) = log(1 + (L_final - L_before) / T), per
-// channel, zero where T is zero or the value is not finite.
-__global__ __launch_bounds__(BLOCK) void k_cache_targets(DevPaths P) {
-  const uint32_t n = min(P.counters[CTR_TRAIN], P.t_cap);
-  for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
-    float4 a = P.t_L[i], t = P.t_T[i];
-    float4 fin = P.result[__float_as_uint(a.w) & SLOT_MASK];
-    float L0[3] = {a.x, a.y, a.z}, T0[3] = {t.x, t.y, t.z}, Lf[3] = {fin.x, fin.y, fin.z};
-    for (int c = 0; c < 3; c++) {
-      float v = T0[c] > 0.f ? (Lf[c] - L0[c]) / T0[c] : 0.f;
-      v = (v > 0.f && v < 3.0e38f) ? v : 0.f;
-      P.t_tgt[3 * (size_t)i + c] = log1pf(v);
-    }
   }
 }
 
-// result[slot] += T * (exp(network output) - 1) for every query of the batch
+// Sum of the per-wave statistics rows of a render (one block per word).
+__global__ __launch_bounds__(BLOCK) void k_stats_reduce(const uint64_t* __restrict__ part, uint64_t* __restrict__ total) {
+  __shared__ unsigned long long red[BLOCK];
+  unsigned long long v = 0;
+  for (uint32_t w = threadIdx.x; w < (uint32_t)MAX_WAVES; w += BLOCK) v += part[(size_t)w * STAT_WORDS + blockIdx.x];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = BLOCK / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) total[blockIdx.x] = red[0];
+}
+
+// Radiance-cache records after a batch: exclusive prefix sums of the
+// per-wave training and query counts (waves of the batch's grid) and their
+// totals.  One block of SCAN_THREADS threads.
+__global__ __launch_bounds__(SCAN_THREADS) void k_cache_scan(DevPaths P, uint32_t waves) {
+  __shared__ uint32_t part[SCAN_THREADS];
+  const uint32_t per = (waves + SCAN_THREADS - 1) / SCAN_THREADS;
+  const uint32_t e0 = min(threadIdx.x * per, waves), e1 = min(e0 + per, waves);
+  for (int which = 0; which < 2; which++) {
+    const uint32_t* cnt = which == 0 ? P.tseg : P.qseg;
+    uint32_t* pre = which == 0 ? P.tpre : P.qpre;
+    uint32_t sum = 0;
+    for (uint32_t e = e0; e < e1; e++) sum += cnt[e];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < SCAN_THREADS; off <<= 1) {
+      uint32_t v = threadIdx.x >= (uint32_t)off ? part[threadIdx.x - off] : 0u;
+      __syncthreads();
+      part[threadIdx.x] += v;
+      __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
+    for (uint32_t e = e0; e < e1; e++) {
+      pre[e] = run;
+      run += cnt[e];
+    }
+    if (threadIdx.x == SCAN_THREADS - 1) P.counters[which == 0 ? CTR_TRAIN : CTR_QUERY] = part[SCAN_THREADS - 1];
+    __syncthreads();
+  }
+}
+
+// Dense copies of `rows` records (which 0: training vertices, 1: queries)
+// spread evenly over the batch's `total` records (all of them when rows ==
+// total): features, and for training vertices the target
+//   log(1 + radiance leaving the vertex toward the camera side)
+//     = log(1 + (L_final - L_before) / T)
+// per channel, zero where T is zero or the value is not finite.  vals (9
+// per row: L, T, target for training vertices; T, 0, network output for
+// queries) and slots are optional.
+__global__ __launch_bounds__(BLOCK) void k_cache_gather(DevPaths P, int which, uint32_t rows, uint32_t waves,
+                                                        uint32_t stride, uint4* __restrict__ feat,
+                                                        float* __restrict__ tgt, float* __restrict__ vals,
+                                                        uint32_t* __restrict__ slots) {
+  const uint32_t total = P.counters[which == 0 ? CTR_TRAIN : CTR_QUERY];
+  const uint32_t* pre = which == 0 ? P.tpre : P.qpre;
+  for (uint32_t r = blockIdx.x * BLOCK + threadIdx.x; r < rows && r < total; r += gridDim.x * BLOCK) {
+    const uint32_t idx = rows >= total ? r : (uint32_t)((uint64_t)r * total / rows);
+    uint32_t lo = 0, hi = waves - 1;  // last wave whose prefix is <= idx
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) >> 1;
+      if (pre[mid] <= idx) lo = mid;
+      else hi = mid - 1;
+    }
+    const uint32_t j = idx - pre[lo];
+    const size_t k = (size_t)(j >> 6) * stride + (size_t)lo * 64 + (j & 63u);
+    const uint4* src = (which == 0 ? P.t_feat : P.q_feat) + k * 12;
+    for (int q = 0; q < 12; q++) feat[(size_t)r * 12 + q] = src[q];
+    float out[9];
+    uint32_t slot;
+    if (which == 0) {
+      const float4 a = P.t_L[k], t = P.t_T[k];
+      slot = __float_as_uint(a.w) & SLOT_MASK;
+      const float4 fin = P.result[slot];
+      const float L0[3] = {a.x, a.y, a.z}, T0[3] = {t.x, t.y, t.z}, Lf[3] = {fin.x, fin.y, fin.z};
+      for (int c = 0; c < 3; c++) {
+        float v = T0[c] > 0.f ? (Lf[c] - L0[c]) / T0[c] : 0.f;
+        v = (v > 0.f && v < 3.0e38f) ? v : 0.f;
+        out[c] = L0[c];
+        out[3 + c] = T0[c];
+        out[6 + c] = log1pf(v);
+      }
+      if (tgt)
+        for (int c = 0; c < 3; c++) tgt[(size_t)r * 3 + c] = out[6 + c];
+    } else {
+      const float4 m = P.q_meta[k];
+      slot = __float_as_uint(m.w) & SLOT_MASK;
+      out[0] = m.x;
+      out[1] = m.y;
+      out[2] = m.z;
+      out[3] = out[4] = out[5] = 0.f;
+      for (int c = 0; c < 3; c++) out[6 + c] = P.q_out[k * 3 + c];
+    }
+    if (vals)
+      for (int c = 0; c < 9; c++) vals[(size_t)r * 9 + c] = out[c];
+    if (slots) slots[r] = slot;
+  }
+}
+
+// The queries' answers: result += T * max(expm1(y), 0) per channel, over
+// each wave's query segment (same grid as the batch).
 __global__ __launch_bounds__(BLOCK) void k_cache_apply(DevPaths P) {
-  const uint32_t n = min(P.counters[CTR_QUERY], P.q_cap);
-  for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+  const Seg g = my_seg();
+  const uint32_t n = uniform(P.qseg[g.w]);
+  for (uint32_t j = __lane_id(); j < n; j += 64) {
+    const uint32_t i = g.pos(j);
     float4 m = P.q_meta[i];
     const uint32_t slot = __float_as_uint(m.w) & SLOT_MASK;
     float y0 = fmaxf(expm1f(P.q_out[3 * (size_t)i]), 0.f), y1 = fmaxf(expm1f(P.q_out[3 * (size_t)i + 1]), 0.f),
@@ -712,7 +844,7 @@ void launch_extend(const DevScene& sc, const DevPaths& P, int in, float tmin, ui
     hipLaunchKernelGGL((k_extend<false, false>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, tmin, spill, stats);
 }
 void launch_sort(const DevPaths& P, int in, int grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_sort_offsets, dim3(1), dim3(1), 0, st, P);
+  hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(SCAN_THREADS), 0, st, P, (uint32_t)(grid * WAVES));
   hipLaunchKernelGGL(k_sort_scatter, dim3(grid), dim3(BLOCK), 0, st, P, in);
 }
 template <int MS, int FEAT>
@@ -753,17 +885,23 @@ int shade_material_set(uint32_t types_mask) {
   if ((types_mask & ~(uint32_t)MS_SPHERES) == 0) return MS_SPHERES;
   return MS_ALL;
 }
-void launch_bounce_reset(const DevPaths& P, int in, hipStream_t st) {
-  hipLaunchKernelGGL(k_bounce_reset, dim3(1), dim3(64), 0, st, P, in);
+void launch_cache_scan(const DevPaths& P, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_cache_scan, dim3(1), dim3(SCAN_THREADS), 0, st, P, (uint32_t)(grid * WAVES));
 }
-void launch_cache_targets(const DevPaths& P, hipStream_t st) {
-  hipLaunchKernelGGL(k_cache_targets, dim3(1024), dim3(BLOCK), 0, st, P);
+void launch_cache_gather(const DevPaths& P, int which, uint32_t rows, int grid, uint4* feat, float* tgt, float* vals,
+                         uint32_t* slots, hipStream_t st) {
+  const int g = (int)std::max<uint32_t>(1, std::min<uint32_t>((rows + BLOCK - 1) / BLOCK, 2048));
+  hipLaunchKernelGGL(k_cache_gather, dim3(g), dim3(BLOCK), 0, st, P, which, rows, (uint32_t)(grid * WAVES),
+                     (uint32_t)(grid * BLOCK), feat, tgt, vals, slots);
 }
-void launch_cache_apply(const DevPaths& P, hipStream_t st) {
-  hipLaunchKernelGGL(k_cache_apply, dim3(2048), dim3(BLOCK), 0, st, P);
+void launch_cache_apply(const DevPaths& P, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_cache_apply, dim3(grid), dim3(BLOCK), 0, st, P);
 }
 void launch_exhausted(const DevPaths& P, int in, uint64_t* stats, int grid, hipStream_t st) {
   hipLaunchKernelGGL(k_exhausted, dim3(grid), dim3(BLOCK), 0, st, P, in, stats);
+}
+void launch_stats_reduce(const uint64_t* part, uint64_t* total, hipStream_t st) {
+  hipLaunchKernelGGL(k_stats_reduce, dim3(STAT_WORDS), dim3(BLOCK), 0, st, part, total);
 }
 void launch_accumulate(const DevPaths& P, const RenderConsts& rc, uint32_t nframes, double* accum, int grid,
                        hipStream_t st) {
