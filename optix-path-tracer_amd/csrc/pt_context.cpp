@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <numeric>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -120,7 +121,7 @@ struct pt_context {
   float* ctgt = nullptr;            // targets / outputs [n][3]
   uint16_t* cX = nullptr;           // unit-major activations
   uint16_t* cdZ = nullptr;          // unit-major output gradients
-  float* cdb = nullptr;             // per-wave bias gradients
+  float* cdb = nullptr;             // per-wave bias gradients, then their sums
   float* closs = nullptr;           // per-wave loss sums
   float* cdw = nullptr;             // per-chunk weight gradients
 
@@ -200,6 +201,15 @@ inline float bf(uint32_t u) {
 }
 
 int grid_for(uint64_t n) { return (int)std::max<uint64_t>(1, std::min<uint64_t>((n + BLOCK - 1) / BLOCK, MAX_GRID)); }
+
+// an odd stride near 0.618 n that is coprime to n (a full-period
+// multiplicative order of 0..n-1)
+uint32_t golden_stride(uint32_t n) {
+  if (n < 3) return 1u;
+  uint32_t a = (uint32_t)((double)n * 0.6180339887498949) | 1u;
+  while (std::gcd(a, n) != 1u) a += 2u;
+  return a % n ? a % n : 1u;
+}
 }  // namespace
 
 // gather kernels of pt_bvh.hip: BVH-ordered primitive and instance records
@@ -1037,7 +1047,8 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
       const uint32_t rows = (uint32_t)std::min<uint64_t>((uint64_t)c->cache_steps * c->cache_batch, nt);
       uint32_t steps = 0;
       if (rows >= 32) {
-        launch_cache_gather(P, 0, rows, grid, (uint4*)c->cfeat, c->ctgt, nullptr, nullptr, c->stream);
+        launch_cache_gather(P, 0, rows, golden_stride(rows), grid, (uint4*)c->cfeat, c->ctgt, nullptr, nullptr,
+                            c->stream);
         for (uint32_t k = 0; k < c->cache_steps; k++) {
           const uint32_t off = k * c->cache_batch;
           if (off >= rows || rows - off < 32) break;
@@ -1160,7 +1171,7 @@ static pt_status cache_work(pt_context* c, uint32_t n) {
       (e = dalloc(B, (void**)&c->ctgt, (size_t)cap * MLP_OUT * 4)) != hipSuccess ||
       (e = dalloc(B, (void**)&c->cX, (size_t)MLP_X_ROWS * ld * 2)) != hipSuccess ||
       (e = dalloc(B, (void**)&c->cdZ, (size_t)MLP_DZ_ROWS * ld * 2)) != hipSuccess ||
-      (e = dalloc(B, (void**)&c->cdb, waves * MLP_DB_WORDS * 4)) != hipSuccess ||
+      (e = dalloc(B, (void**)&c->cdb, (waves + 1) * MLP_DB_WORDS * 4)) != hipSuccess ||
       (e = dalloc(B, (void**)&c->closs, waves * 4)) != hipSuccess ||
       (e = dalloc(B, (void**)&c->cdw, chunks * MLP_PARAMS * 4)) != hipSuccess) {
     dfree(B);
@@ -1368,7 +1379,7 @@ pt_status pt_cache_read_records(pt_context* c, int32_t which, uint32_t max, uint
     dfree(tmp);
     return hip_fail(c, e, "hipMalloc (record copies)");
   }
-  launch_cache_gather(P, which, total, c->cache_last_grid, dfeat, nullptr, dvals, dslots, c->stream);
+  launch_cache_gather(P, which, total, 1u, c->cache_last_grid, dfeat, nullptr, dvals, dslots, c->stream);
   e = hipStreamSynchronize(c->stream);
   if (e == hipSuccess && features) e = hipMemcpy(features, dfeat, (size_t)n * 192, hipMemcpyDeviceToHost);
   if (e == hipSuccess && values) e = hipMemcpy(values, dvals, (size_t)n * 36, hipMemcpyDeviceToHost);

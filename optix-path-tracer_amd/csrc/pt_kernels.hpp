@@ -54,9 +54,10 @@ void launch_env_tables(const float4* tex, uint32_t W, uint32_t H, unsigned long 
 // grid (its waves own the record segments)
 void launch_cache_scan(const DevPaths& P, int grid, hipStream_t st);
 // dense copies of `rows` records spread over the batch's records (which 0:
-// training vertices with targets, 1: queries); tgt, vals, slots optional
-void launch_cache_gather(const DevPaths& P, int which, uint32_t rows, int grid, uint4* feat, float* tgt, float* vals,
-                         uint32_t* slots, hipStream_t st);
+// training vertices with targets, 1: queries), row r taking rank
+// (r * mul) mod rows (mul coprime to rows); tgt, vals, slots optional
+void launch_cache_gather(const DevPaths& P, int which, uint32_t rows, uint32_t mul, int grid, uint4* feat, float* tgt,
+                         float* vals, uint32_t* slots, hipStream_t st);
 void launch_cache_apply(const DevPaths& P, int grid, hipStream_t st);
 void launch_exhausted(const DevPaths& P, int in, uint64_t* stats, int grid, hipStream_t st);
 // per-wave statistics rows [MAX_WAVES][STAT_WORDS] -> totals [STAT_WORDS]

@@ -351,19 +351,28 @@ __global__ __launch_bounds__(MB) void k_mlp_wgrad(const uint16_t* X, const uint1
 }
 
 // ------------------------------------------------------------ Adam
-// Sums the partial gradients in a fixed order, then one Adam step
-// (Kingma & Ba 2015) on the FP32 master weights, and repacks the operands.
-__global__ __launch_bounds__(MB) void k_mlp_adam(const float* dw_partial, uint32_t nchunks, const float* db_partial,
-                                                 uint32_t nwaves, float* params, float* m1, float* m2, float* grad_out,
-                                                 float lr, float beta1, float beta2, float eps, float bc1, float bc2,
+// Bias gradients: one wave per bias unit sums the per-wave partials (lane l
+// takes partials l, l + 64, ...; then a fixed shuffle tree), into db_sum.
+__global__ __launch_bounds__(MB) void k_mlp_bias_reduce(const float* db_partial, uint32_t nwaves, float* db_sum) {
+  const int unit = blockIdx.x * (MB / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (unit >= DB_WORDS) return;
+  float v = 0.f;
+  for (uint32_t w = lane; w < nwaves; w += 64) v += db_partial[(size_t)w * DB_WORDS + unit];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if (lane == 0) db_sum[unit] = v;
+}
+// Sums the weight partials in a fixed order, then one Adam step (Kingma &
+// Ba 2015) on the FP32 master weights, and repacks the operands.
+__global__ __launch_bounds__(MB) void k_mlp_adam(const float* dw_partial, uint32_t nchunks, const float* db_sum,
+                                                 float* params, float* m1, float* m2, float* grad_out, float lr,
+                                                 float beta1, float beta2, float eps, float bc1, float bc2,
                                                  uint16_t* fw, uint16_t* bw, float* bias) {
   for (int p = blockIdx.x * MB + threadIdx.x; p < MLP_PARAMS; p += gridDim.x * MB) {
     int L = 0;
     while (L < 4 && p >= W_OFF[L + 1]) L++;
     float gsum = 0.f;
     if (p >= B_OFF[L]) {
-      int unit = L * 64 + (p - B_OFF[L]);
-      for (uint32_t w = 0; w < nwaves; w++) gsum += db_partial[(size_t)w * DB_WORDS + unit];
+      gsum = db_sum[L * 64 + (p - B_OFF[L])];
     } else {
       for (uint32_t c = 0; c < nchunks; c++) gsum += dw_partial[(size_t)c * MLP_PARAMS + p];
     }
@@ -403,11 +412,16 @@ void launch_mlp_train(const uint16_t* feats, const float* targets, uint32_t n, c
   const uint32_t wg = N_TILES * nch;
   hipLaunchKernelGGL(k_mlp_wgrad, dim3((wg + 3) / 4), dim3(MB), 0, st, (const uint16_t*)X, (const uint16_t*)dZ, ld,
                      (uint32_t)MLP_CHUNK, dw_partial);
+  // the bias sums go after the per-wave partials (db_partial has a spare
+  // row of DB_WORDS floats past the last wave)
+  float* db_sum = db_partial + (size_t)waves * DB_WORDS;
+  hipLaunchKernelGGL(k_mlp_bias_reduce, dim3((DB_WORDS + MB / 64 - 1) / (MB / 64)), dim3(MB), 0, st,
+                     (const float*)db_partial, waves, db_sum);
   const float b1 = 0.9f, b2 = 0.999f;
   const float bc1 = 1.f - powf(b1, (float)step), bc2 = 1.f - powf(b2, (float)step);
   hipLaunchKernelGGL(k_mlp_adam, dim3((MLP_PARAMS + MB - 1) / MB), dim3(MB), 0, st, (const float*)dw_partial, nch,
-                     (const float*)db_partial, waves, params, m1, m2, grad_out, lr, b1, b2, 1e-8f, bc1, bc2, fw_out,
-                     bw_out, bias_out);
+                     (const float*)db_sum, params, m1, m2, grad_out, lr, b1, b2, 1e-8f, bc1, bc2, fw_out, bw_out,
+                     bias_out);
 }
 
 }  // namespace pt

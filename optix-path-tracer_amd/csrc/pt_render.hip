@@ -185,9 +185,47 @@ __global__ __launch_bounds__(BLOCK) void k_trace_occluded(DevScene sc, const pt_
 }
 
 // ------------------------------------------------------ wavefront kernels
+// Slot of position p in a batch of nslots.  Slots come in 64-slot chunks
+// (64 consecutive pixels of one row, coherent camera rays); the chunk index
+// goes through a keyed 4-round Feistel permutation, cycle-walked into the Q
+// full chunks, so that the chunks a wave owns come from all over the
+// batch's frames and every wave keeps about the image-average share of
+// surviving paths (with slot = position a wave would keep the same 64
+// pixels in every frame, and the waves on the light would idle).  A partial
+// last chunk keeps its place.  mix32 is the "lowbias32" integer hash
+// (C. Wellons).
+PTD uint32_t mix32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7FEB352Du;
+  x ^= x >> 15;
+  x *= 0x846CA68Bu;
+  x ^= x >> 16;
+  return x;
+}
+PTD uint32_t slot_of(uint32_t p, uint32_t nslots) {
+  const uint32_t Q = nslots >> 6, q = p >> 6;
+  if (Q < 2 || q >= Q) return p;
+  int bits = 32 - __clz(Q - 1);  // 2^bits >= Q, made even
+  bits += bits & 1;
+  const int half = bits >> 1;
+  const uint32_t mask = (1u << half) - 1u;
+  uint32_t x = q;
+  do {  // a permutation of [0, 2^bits): the walk from q < Q returns below Q
+    uint32_t L = x >> half, R = x & mask;
+    for (uint32_t r = 1; r <= 4; r++) {
+      const uint32_t t = L ^ (mix32(R ^ (0x9E3779B9u * r)) & mask);
+      L = R;
+      R = t;
+    }
+    x = (L << half) | R;
+  } while (x >= Q);
+  return (x << 6) | (p & 63u);
+}
+
 // raygen.cu:148-168 renderPixel up to the first rtTrace, raygen.cu:50-64
 // generateRay, raygen.cu:66-74 color() set-up.
-// Slot s is generated at position s of state set 0 (a dense set).
+// Position p of state set 0 (a dense set) starts the path of slot
+// slot_of(p, nslots).
 __global__ __launch_bounds__(BLOCK) void k_raygen(DevPaths P, RenderConsts rc, uint32_t nslots) {
   PathState S = P.st[0];
   if (rc.frame_dev) rc.frame0 = *rc.frame_dev;  // replayed graphs take the batch's frame from memory
@@ -197,7 +235,8 @@ __global__ __launch_bounds__(BLOCK) void k_raygen(DevPaths P, RenderConsts rc, u
     P.tseg[g.w] = 0u;
     P.qseg[g.w] = 0u;
   }
-  for (uint32_t s = blockIdx.x * BLOCK + threadIdx.x; s < nslots; s += gridDim.x * BLOCK) {
+  for (uint32_t p = blockIdx.x * BLOCK + threadIdx.x; p < nslots; p += gridDim.x * BLOCK) {
+    const uint32_t s = slot_of(p, nslots);
     uint32_t f = s / rc.WH, pix = s - f * rc.WH;
     uint32_t x = pix % rc.W, y = pix / rc.W;
     uint32_t seed = tea64(rc.W * y + x, rc.frame0 + f);
@@ -211,12 +250,12 @@ __global__ __launch_bounds__(BLOCK) void k_raygen(DevPaths P, RenderConsts rc, u
     v3 dir = mk3(c.llc[0], c.llc[1], c.llc[2]) + u * mk3(c.hor[0], c.hor[1], c.hor[2]) +
              v * mk3(c.ver[0], c.ver[1], c.ver[2]) - origin;
     (void)rnd(seed);  // prd.time (raygen.cu:69), consumed but unused
-    S.o[s] = make_float4(origin.x, origin.y, origin.z, __uint_as_float(seed));
+    S.o[p] = make_float4(origin.x, origin.y, origin.z, __uint_as_float(seed));
     uint32_t dw = s;
     if ((rc.flags & PT_FLAG_RADIANCE_CACHE) && train_hash(pix, rc.frame0 + f) < rc.cache.train_threshold) dw |= TRAIN_BIT;
-    S.d[s] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(dw));
-    S.T[s] = make_float4(1.f, 1.f, 1.f, 1.f);
-    S.L[s] = make_float4(0.f, 0.f, 0.f, __uint_as_float(0x811C9DC5u));
+    S.d[p] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(dw));
+    S.T[p] = make_float4(1.f, 1.f, 1.f, 1.f);
+    S.L[p] = make_float4(0.f, 0.f, 0.f, __uint_as_float(0x811C9DC5u));
     P.result[s] = make_float4(0.f, 0.f, 0.f, __uint_as_float(rc.max_depth));
   }
 }
@@ -692,20 +731,24 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_cache_scan(DevPaths P, uint32_
 
 // Dense copies of `rows` records (which 0: training vertices, 1: queries)
 // spread evenly over the batch's `total` records (all of them when rows ==
-// total): features, and for training vertices the target
+// total), row r taking the record of rank (r * mul) mod rows (mul coprime to
+// rows: with mul ~ 0.618 rows every run of rows, such as one Adam batch,
+// samples the whole image instead of a few waves' pixels): features, and
+// for training vertices the target
 //   log(1 + radiance leaving the vertex toward the camera side)
 //     = log(1 + (L_final - L_before) / T)
 // per channel, zero where T is zero or the value is not finite.  vals (9
 // per row: L, T, target for training vertices; T, 0, network output for
 // queries) and slots are optional.
-__global__ __launch_bounds__(BLOCK) void k_cache_gather(DevPaths P, int which, uint32_t rows, uint32_t waves,
-                                                        uint32_t stride, uint4* __restrict__ feat,
+__global__ __launch_bounds__(BLOCK) void k_cache_gather(DevPaths P, int which, uint32_t rows, uint32_t mul,
+                                                        uint32_t waves, uint32_t stride, uint4* __restrict__ feat,
                                                         float* __restrict__ tgt, float* __restrict__ vals,
                                                         uint32_t* __restrict__ slots) {
   const uint32_t total = P.counters[which == 0 ? CTR_TRAIN : CTR_QUERY];
   const uint32_t* pre = which == 0 ? P.tpre : P.qpre;
   for (uint32_t r = blockIdx.x * BLOCK + threadIdx.x; r < rows && r < total; r += gridDim.x * BLOCK) {
-    const uint32_t idx = rows >= total ? r : (uint32_t)((uint64_t)r * total / rows);
+    const uint32_t rank = (uint32_t)((uint64_t)r * mul % rows);
+    const uint32_t idx = rows >= total ? rank : (uint32_t)((uint64_t)rank * total / rows);
     uint32_t lo = 0, hi = waves - 1;  // last wave whose prefix is <= idx
     while (lo < hi) {
       const uint32_t mid = (lo + hi + 1) >> 1;
@@ -888,10 +931,10 @@ int shade_material_set(uint32_t types_mask) {
 void launch_cache_scan(const DevPaths& P, int grid, hipStream_t st) {
   hipLaunchKernelGGL(k_cache_scan, dim3(1), dim3(SCAN_THREADS), 0, st, P, (uint32_t)(grid * WAVES));
 }
-void launch_cache_gather(const DevPaths& P, int which, uint32_t rows, int grid, uint4* feat, float* tgt, float* vals,
-                         uint32_t* slots, hipStream_t st) {
+void launch_cache_gather(const DevPaths& P, int which, uint32_t rows, uint32_t mul, int grid, uint4* feat, float* tgt,
+                         float* vals, uint32_t* slots, hipStream_t st) {
   const int g = (int)std::max<uint32_t>(1, std::min<uint32_t>((rows + BLOCK - 1) / BLOCK, 2048));
-  hipLaunchKernelGGL(k_cache_gather, dim3(g), dim3(BLOCK), 0, st, P, which, rows, (uint32_t)(grid * WAVES),
+  hipLaunchKernelGGL(k_cache_gather, dim3(g), dim3(BLOCK), 0, st, P, which, rows, mul, (uint32_t)(grid * WAVES),
                      (uint32_t)(grid * BLOCK), feat, tgt, vals, slots);
 }
 void launch_cache_apply(const DevPaths& P, int grid, hipStream_t st) {

@@ -176,8 +176,12 @@ def test_queries_end_paths_and_training_converges(ctx):
         losses.append(st.cache_loss)
     _, _, bounces = ctx.read_paths(W, H)
     _, _, tslots, _ = ctx.cache_records(0)
-    _, qv, qslots, nq = ctx.cache_records(1)
+    qf, qv, qslots, nq = ctx.cache_records(1)
     assert nq == st.cache_queries > 0.3 * W * H
+    # the batch's answers (inference over the queries where the shading
+    # kernel left them) equal a dense inference of the same features
+    np.testing.assert_array_equal(qv[:, 6:9], ctx.cache_infer(qf))
+    assert np.unique(qslots).size == nq
     # paths that were not training paths end by the third vertex; every query
     # ends such a path there, and the ones that end there without a query
     # are the rays that left the box through its open front
@@ -211,7 +215,9 @@ def test_equal_time_gain_over_plain_tracer(ctx):
     ref, _ = render_image(ctx, plain, 1000, 4096)
     img_p, st_p = render_image(ctx, plain, 0, fpb)
     ctx.cache_init(7)
-    ctx.cache_configure(query_depth=2, train_fraction=0.03, batch=4096, steps=4)
+    # 16 Adam steps of 4096 vertices per 128-frame batch (a third of the
+    # batch's training vertices); 4 steps leave the network under-trained
+    ctx.cache_configure(query_depth=2, train_fraction=0.03, batch=4096, steps=16)
     cached = sc.render_params(flags=ptamd.PT_FLAG_RADIANCE_CACHE, frames_per_batch=fpb)
     ctx.render(cached, 5000, 2 * fpb)  # warm-up training on the same view (not accumulated)
     img_c, st_c = render_image(ctx, cached, 0, fpb)
