@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define PT_API_VERSION 6
+#define PT_API_VERSION 7
 
 /* ------------------------------------------------------------------ status */
 typedef int32_t pt_status;
@@ -285,6 +285,21 @@ pt_status pt_set_camera(struct pt_context* ctx, const pt_camera* cam);
 pt_status pt_render(struct pt_context* ctx, const pt_render_params* params,
                     uint32_t frame_begin, uint32_t frame_count,
                     pt_render_stats* out_or_null);
+
+/* No counterpart in the reference (it renders one scene per OptiX context):
+ * frames [0, frame_counts[i]) of every ctxs[i], i < n <= 256, in one
+ * wavefront.  The (scene, frame) work items are cut into batches of about
+ * 16 Mi paths; within a batch all paths share one path pool and one launch
+ * per kernel and bounce, so many small or uneven scenes fill the GPU
+ * together.  Every context needs a built scene and a camera, all on one
+ * device; ctxs[0] runs the batch.  Each context's accumulation restarts at
+ * frame 0 and ends equal, bit for bit, to pt_render(ctxs[i], &params[i],
+ * 0, frame_counts[i]).  The radiance cache is not available in a batch;
+ * material sort and traversal counting follow params[0].flags.  stats (if
+ * given) cover the whole batch; this call synchronises when they are asked
+ * for, else it is asynchronous on ctxs[0]'s stream. */
+pt_status pt_render_batch(struct pt_context* const* ctxs, const pt_render_params* params,
+                          const uint32_t* frame_counts, uint32_t n, pt_render_stats* stats_or_null);
 
 pt_status pt_sync(struct pt_context* ctx);
 

@@ -112,6 +112,13 @@ struct pt_context {
   float scene_lo[3] = {0, 0, 0}, scene_hi[3] = {1, 1, 1};  // top-level box, for the position encoding
   uint64_t cache_rec_cap = 0;       // slots the record buffers were sized for
   std::vector<DevBuf> crec_bufs;
+  // batch of scenes (pt_render_batch run by this context): scene and
+  // constant tables, work-item tables
+  std::vector<DevBuf> batch_bufs;
+  size_t batch_scene_cap = 0, batch_item_cap = 0;
+  DevScene* d_scenes = nullptr;
+  RenderConsts* d_rcs = nullptr;
+  uint32_t *d_item_off = nullptr, *d_item_scene = nullptr, *d_item_frame = nullptr;
   uint32_t cache_last_train = 0, cache_last_query = 0, cache_last_steps = 0;
   uint32_t cache_last_batch = 0;    // samples of the last Adam step
   int cache_last_grid = 0;          // grid of the last cache batch (its waves own the records)
@@ -277,6 +284,7 @@ void pt_destroy(pt_context* c) {
   dfree(c->cache_bufs);
   dfree(c->cwork_bufs);
   dfree(c->crec_bufs);
+  dfree(c->batch_bufs);
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
   if (c->frame_dev) (void)hipFree(c->frame_dev);
@@ -872,6 +880,37 @@ static pt_status ensure_cache_records(pt_context* c, uint64_t cap) {
   return PT_OK;
 }
 
+// per-launch constants of a render of context c with parameters rp
+static RenderConsts render_consts(const pt_context* c, const pt_render_params* rp) {
+  RenderConsts rc{};
+  rc.W = rp->width;
+  rc.H = rp->height;
+  rc.WH = rp->width * rp->height;
+  rc.max_depth = rp->max_depth;
+  rc.russian = rp->russian;
+  rc.rr_min_depth = rp->rr_min_depth;
+  rc.flags = rp->flags;
+  const pt_camera& cm = c->cam;
+  for (int k = 0; k < 3; k++) {
+    rc.cam.llc[k] = cm.lower_left[k];
+    rc.cam.hor[k] = cm.horizontal[k];
+    rc.cam.ver[k] = cm.vertical[k];
+    rc.cam.org[k] = cm.origin[k];
+    rc.cam.u[k] = cm.u[k];
+    rc.cam.v[k] = cm.v[k];
+  }
+  for (int k = 0; k < 3; k++) {
+    rc.cache.lo[k] = c->scene_lo[k];
+    rc.cache.inv_ext[k] = 1.f / (c->scene_hi[k] - c->scene_lo[k]);
+  }
+  rc.cache.query_depth = c->cache_query_depth;
+  rc.cache.train_threshold = (uint32_t)std::min(16777216.0, std::max(0.0, (double)c->cache_train_fraction * 16777216.0));
+  rc.cam.lens_radius = cm.lens_radius;
+  rc.cam.time0 = cm.time0;
+  rc.cam.time1 = cm.time1;
+  return rc;
+}
+
 pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_begin, uint32_t frame_count,
                     pt_render_stats* out) {
   if (!c || !rp) return PT_ERR_INVALID_ARG;
@@ -922,32 +961,7 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
     if ((ps = cache_work(c, (uint32_t)std::max<uint64_t>(rows, c->cache_batch))) != PT_OK) return ps;
   }
 
-  RenderConsts rc{};
-  rc.W = rp->width;
-  rc.H = rp->height;
-  rc.WH = WH;
-  rc.max_depth = rp->max_depth;
-  rc.russian = rp->russian;
-  rc.rr_min_depth = rp->rr_min_depth;
-  rc.flags = rp->flags;
-  const pt_camera& cm = c->cam;
-  for (int k = 0; k < 3; k++) {
-    rc.cam.llc[k] = cm.lower_left[k];
-    rc.cam.hor[k] = cm.horizontal[k];
-    rc.cam.ver[k] = cm.vertical[k];
-    rc.cam.org[k] = cm.origin[k];
-    rc.cam.u[k] = cm.u[k];
-    rc.cam.v[k] = cm.v[k];
-  }
-  for (int k = 0; k < 3; k++) {
-    rc.cache.lo[k] = c->scene_lo[k];
-    rc.cache.inv_ext[k] = 1.f / (c->scene_hi[k] - c->scene_lo[k]);
-  }
-  rc.cache.query_depth = c->cache_query_depth;
-  rc.cache.train_threshold = (uint32_t)std::min(16777216.0, std::max(0.0, (double)c->cache_train_fraction * 16777216.0));
-  rc.cam.lens_radius = cm.lens_radius;
-  rc.cam.time0 = cm.time0;
-  rc.cam.time1 = cm.time1;
+  RenderConsts rc = render_consts(c, rp);
 
   const bool want_stats = out != nullptr;
   const bool count_trav = (rp->flags & PT_FLAG_COUNT_TRAVERSAL) != 0;
@@ -1280,6 +1294,193 @@ pt_status pt_cache_train(pt_context* c, const uint16_t* features, const float* t
     *loss = (float)sum;
   }
   if (grad) HIPCHK(c, hipMemcpy(grad, c->cgrad, MLP_PARAMS * 4, hipMemcpyDeviceToHost));
+  return PT_OK;
+}
+
+// A batch of scenes in one wavefront: the frames of every context form one
+// work list of (scene, frame) items, cut into batches of about PATHS_TARGET
+// slots; each batch's paths share one path pool and one launch per kernel
+// and bounce, each path finding its scene through the index kept in its RNG
+// state.  The results of every context are those of its own pt_render.
+pt_status pt_render_batch(pt_context* const* ctxs, const pt_render_params* params, const uint32_t* frame_counts,
+                          uint32_t n, pt_render_stats* out) {
+  if (!ctxs || !params || !frame_counts || n == 0 || !ctxs[0]) return PT_ERR_INVALID_ARG;
+  pt_context* B = ctxs[0];
+  if (n > 256) return fail(B, PT_ERR_INVALID_ARG, "at most 256 scenes per batch");
+  uint32_t union_types = 0, max_depth = 0;
+  bool env = false;
+  for (uint32_t i = 0; i < n; i++) {
+    pt_context* c = ctxs[i];
+    const pt_render_params* rp = params + i;
+    if (!c) return fail(B, PT_ERR_INVALID_ARG, "null context in the batch");
+    for (uint32_t j = 0; j < i; j++)
+      if (ctxs[j] == c) return fail(B, PT_ERR_INVALID_ARG, "a context appears twice in the batch");
+    if (c->device != B->device) return fail(B, PT_ERR_INVALID_ARG, "the batch's contexts are on different devices");
+    if (!c->accel_ready || !c->has_camera) return fail(B, PT_ERR_STATE, "a batch context has no built scene or camera");
+    if (rp->width == 0 || rp->height == 0 || (uint64_t)rp->width * rp->height > (1ull << 28) || rp->max_depth == 0)
+      return fail(B, PT_ERR_INVALID_ARG, "invalid render parameters in the batch");
+    if ((uint64_t)rp->width * rp->height > SLOT_MASK)
+      return fail(B, PT_ERR_INVALID_ARG, "image too large for one batch");
+    if (rp->flags & PT_FLAG_RADIANCE_CACHE) return fail(B, PT_ERR_UNSUPPORTED, "no radiance cache in a batch");
+    union_types |= c->mat_types;
+    max_depth = std::max(max_depth, rp->max_depth);
+    env = env || ((rp->flags & PT_FLAG_ENV_MIS) && c->dsc.num_lights != c->dsc.num_lights_plain);
+  }
+  (void)hipSetDevice(B->device);
+  for (uint32_t i = 0; i < n; i++) HIPCHK(B, hipStreamSynchronize(ctxs[i]->stream));
+  // accumulation buffers, cleared on the batch's stream
+  for (uint32_t i = 0; i < n; i++) {
+    pt_context* c = ctxs[i];
+    const pt_render_params* rp = params + i;
+    const uint32_t WH = rp->width * rp->height;
+    if (rp->width != c->W || rp->height != c->H || !c->accum) {
+      if (c->accum) (void)hipFree(c->accum);
+      if (c->rgba) (void)hipFree(c->rgba);
+      c->accum = nullptr;
+      c->rgba = nullptr;
+      hipError_t e;
+      if ((e = hipMalloc(&c->accum, (size_t)WH * 3 * sizeof(double))) != hipSuccess ||
+          (e = hipMalloc(&c->rgba, (size_t)WH * sizeof(uchar4))) != hipSuccess)
+        return hip_fail(B, e, "hipMalloc (accumulation)");
+      c->W = rp->width;
+      c->H = rp->height;
+    }
+    launch_fill_f64(c->accum, (uint64_t)WH * 3, 0.0, B->stream);
+    c->frames = 0;
+    c->paths_valid = false;
+  }
+  // work items (scene, frame) in scene-then-frame order, cut into batches
+  std::vector<uint32_t> off, scene, frame;
+  std::vector<size_t> first;      // first item of each batch
+  std::vector<uint32_t> bslots;   // slots of each batch
+  uint64_t cur = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    const uint32_t WH = params[i].width * params[i].height;
+    for (uint32_t f = 0; f < frame_counts[i]; f++) {
+      if (first.empty() || (cur > 0 && cur + WH > std::max<uint64_t>(PATHS_TARGET, WH))) {
+        if (!first.empty()) bslots.push_back((uint32_t)cur);
+        first.push_back(off.size());
+        cur = 0;
+      }
+      off.push_back((uint32_t)cur);
+      scene.push_back(i);
+      frame.push_back(f);
+      cur += WH;
+    }
+  }
+  if (!first.empty()) bslots.push_back((uint32_t)cur);
+  first.push_back(off.size());
+  const size_t nitems = off.size();
+  uint32_t max_slots = 0;
+  for (uint32_t b : bslots) max_slots = std::max(max_slots, b);
+  // tables
+  if (n > B->batch_scene_cap || nitems > B->batch_item_cap) {
+    dfree(B->batch_bufs);
+    B->batch_scene_cap = B->batch_item_cap = 0;
+    const size_t sc = std::max<size_t>(n, 64), ic = std::max<size_t>(nitems, 1024);
+    hipError_t e;
+    if ((e = dalloc(B->batch_bufs, (void**)&B->d_scenes, sc * sizeof(DevScene))) != hipSuccess ||
+        (e = dalloc(B->batch_bufs, (void**)&B->d_rcs, sc * sizeof(RenderConsts))) != hipSuccess ||
+        (e = dalloc(B->batch_bufs, (void**)&B->d_item_off, ic * 4)) != hipSuccess ||
+        (e = dalloc(B->batch_bufs, (void**)&B->d_item_scene, ic * 4)) != hipSuccess ||
+        (e = dalloc(B->batch_bufs, (void**)&B->d_item_frame, ic * 4)) != hipSuccess) {
+      dfree(B->batch_bufs);
+      return hip_fail(B, e, "hipMalloc (batch tables)");
+    }
+    B->batch_scene_cap = sc;
+    B->batch_item_cap = ic;
+  }
+  std::vector<DevScene> scenes(n);
+  std::vector<RenderConsts> rcs(n);
+  for (uint32_t i = 0; i < n; i++) {
+    scenes[i] = ctxs[i]->dsc;
+    rcs[i] = render_consts(ctxs[i], params + i);
+  }
+  HIPCHK(B, hipMemcpy(B->d_scenes, scenes.data(), n * sizeof(DevScene), hipMemcpyHostToDevice));
+  HIPCHK(B, hipMemcpy(B->d_rcs, rcs.data(), n * sizeof(RenderConsts), hipMemcpyHostToDevice));
+  if (nitems) {
+    HIPCHK(B, hipMemcpy(B->d_item_off, off.data(), nitems * 4, hipMemcpyHostToDevice));
+    HIPCHK(B, hipMemcpy(B->d_item_scene, scene.data(), nitems * 4, hipMemcpyHostToDevice));
+    HIPCHK(B, hipMemcpy(B->d_item_frame, frame.data(), nitems * 4, hipMemcpyHostToDevice));
+  }
+  if (max_slots) {
+    pt_status ps = ensure_paths(B, max_slots);
+    if (ps != PT_OK) return ps;
+  }
+  const bool want_stats = out != nullptr;
+  const bool count_trav = (params[0].flags & PT_FLAG_COUNT_TRAVERSAL) != 0;
+  const bool sort = (params[0].flags & PT_FLAG_MATERIAL_SORT) && __builtin_popcount(union_types & ~(1u << PT_MAT_DIFFUSE_LIGHT)) > 1;
+  DevPaths& P = B->paths;
+  launch_fill_u32((uint32_t*)B->stats, (uint64_t)MAX_WAVES * STAT_WORDS * 2, 0u, B->stream);
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  if (want_stats) {
+    HIPCHK(B, hipEventCreate(&t0));
+    HIPCHK(B, hipEventCreate(&t1));
+    HIPCHK(B, hipEventRecord(t0, B->stream));
+  }
+  RenderConsts rc0 = rcs[0];
+  rc0.flags &= ~(uint32_t)PT_FLAG_RADIANCE_CACHE;
+  for (size_t b = 0; b + 1 < first.size(); b++) {
+    const uint32_t nslots = bslots[b];
+    const size_t k0 = first[b], k1 = first[b + 1];
+    if (nslots == 0) continue;
+    const int grid = grid_for(nslots);
+    launch_raygen_batch(P, B->d_rcs, B->d_item_off + k0, B->d_item_scene + k0, B->d_item_frame + k0,
+                        (uint32_t)(k1 - k0), nslots, grid, B->stream);
+    int in = 0, o = 1;
+    for (uint32_t depth = 0; depth < max_depth; depth++) {
+      launch_extend(B->dsc, P, in, depth == 0 ? 1e-6f : 1e-3f, B->spill, B->stats, count_trav, sort, grid, B->stream,
+                    B->d_scenes);
+      if (sort) launch_sort(P, in, grid, B->stream);
+      launch_shade(B->dsc, P, in, sort, o, depth, rc0, shade_material_set(union_types), count_trav, B->spill, B->stats, grid, B->stream,
+                   B->d_scenes, B->d_rcs, env);
+      std::swap(in, o);
+    }
+    launch_exhausted(P, in, B->stats, grid, B->stream);
+    // each scene's frames of this batch, in frame order
+    for (size_t k = k0; k < k1;) {
+      size_t e = k;
+      while (e < k1 && scene[e] == scene[k]) e++;
+      DevPaths Pk = P;
+      Pk.result = P.result + off[k];
+      const uint32_t i = scene[k];
+      launch_accumulate(Pk, rcs[i], (uint32_t)(e - k), ctxs[i]->accum, grid_for(rcs[i].WH), B->stream);
+      k = e;
+    }
+  }
+  for (uint32_t i = 0; i < n; i++) ctxs[i]->frames = frame_counts[i];
+  // later work on any batch context (reads, renders) comes after the batch
+  hipEvent_t done = nullptr;
+  HIPCHK(B, hipEventCreateWithFlags(&done, hipEventDisableTiming));
+  HIPCHK(B, hipEventRecord(done, B->stream));
+  for (uint32_t i = 1; i < n; i++) HIPCHK(B, hipStreamWaitEvent(ctxs[i]->stream, done, 0));
+  (void)hipEventDestroy(done);
+  if (want_stats) {
+    HIPCHK(B, hipEventRecord(t1, B->stream));
+    launch_stats_reduce(B->stats, B->stats_total, B->stream);
+    HIPCHK(B, hipStreamSynchronize(B->stream));
+    uint64_t st[STAT_WORDS];
+    HIPCHK(B, hipMemcpy(st, B->stats_total, sizeof(st), hipMemcpyDeviceToHost));
+    std::memset(out, 0, sizeof(*out));
+    for (uint32_t i = 0; i < n; i++) out->paths += (uint64_t)frame_counts[i] * params[i].width * params[i].height;
+    out->extension_rays = st[STAT_EXT];
+    out->shadow_rays = st[STAT_SHADOW];
+    out->node_fetches = st[STAT_NODES];
+    out->prim_fetches = st[STAT_PRIMS];
+    out->instance_fetches = st[STAT_INSTS];
+    out->shadow_node_fetches = st[STAT_SH_NODES];
+    out->shadow_prim_fetches = st[STAT_SH_PRIMS];
+    out->shadow_instance_fetches = st[STAT_SH_INSTS];
+    out->exhausted_paths = st[STAT_EXHAUSTED];
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, t0, t1);
+    out->gpu_ms = ms;
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    uint32_t ovf = 0;
+    HIPCHK(B, hipMemcpy(&ovf, P.counters + CTR_STACK_OVERFLOW, 4, hipMemcpyDeviceToHost));
+    if (ovf) return fail(B, PT_ERR_DEVICE, "traversal stack overflow");
+  }
   return PT_OK;
 }
 

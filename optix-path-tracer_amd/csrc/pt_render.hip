@@ -224,39 +224,73 @@ PTD uint32_t slot_of(uint32_t p, uint32_t nslots) {
 
 // raygen.cu:148-168 renderPixel up to the first rtTrace, raygen.cu:50-64
 // generateRay, raygen.cu:66-74 color() set-up.
-// Position p of state set 0 (a dense set) starts the path of slot
-// slot_of(p, nslots).
-__global__ __launch_bounds__(BLOCK) void k_raygen(DevPaths P, RenderConsts rc, uint32_t nslots) {
-  PathState S = P.st[0];
-  if (rc.frame_dev) rc.frame0 = *rc.frame_dev;  // replayed graphs take the batch's frame from memory
-  const Seg g = my_seg();
+// One camera path: slot s (frame `frame` of the scene, pixel `pix`) at
+// position p of state set 0; in a batch of scenes the RNG state's top byte
+// is replaced by the scene index.
+PTD void start_path(const DevPaths& P, const RenderConsts& rc, uint32_t p, uint32_t s, uint32_t frame, uint32_t pix,
+                    bool batch, uint32_t sid) {
+  const PathState S = P.st[0];
+  uint32_t x = pix % rc.W, y = pix / rc.W;
+  uint32_t seed = tea64(rc.W * y + x, frame);
+  float u = ((float)x + rnd(seed)) / (float)rc.W;
+  float v = ((float)y + rnd(seed)) / (float)rc.H;
+  const DevCamera& c = rc.cam;
+  v3 cu = mk3(c.u[0], c.u[1], c.u[2]), cv = mk3(c.v[0], c.v[1], c.v[2]);
+  v3 rd = c.lens_radius * random_in_unit_disk(seed);
+  v3 lens = cu * rd.x + cv * rd.y;
+  v3 origin = mk3(c.org[0], c.org[1], c.org[2]) + lens;
+  v3 dir = mk3(c.llc[0], c.llc[1], c.llc[2]) + u * mk3(c.hor[0], c.hor[1], c.hor[2]) +
+           v * mk3(c.ver[0], c.ver[1], c.ver[2]) - origin;
+  (void)rnd(seed);  // prd.time (raygen.cu:69), consumed but unused
+  S.o[p] = make_float4(origin.x, origin.y, origin.z, __uint_as_float(batch ? ((seed & 0xFFFFFFu) | (sid << 24)) : seed));
+  uint32_t dw = s;
+  if ((rc.flags & PT_FLAG_RADIANCE_CACHE) && train_hash(pix, frame) < rc.cache.train_threshold) dw |= TRAIN_BIT;
+  S.d[p] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(dw));
+  S.T[p] = make_float4(1.f, 1.f, 1.f, 1.f);
+  S.L[p] = make_float4(0.f, 0.f, 0.f, __uint_as_float(0x811C9DC5u));
+  P.result[s] = make_float4(0.f, 0.f, 0.f, __uint_as_float(rc.max_depth));
+}
+PTD void reset_segments(const DevPaths& P, uint32_t nslots, const Seg& g) {
   if (__lane_id() == 0) {
     P.seg[g.w] = dense_count(nslots, g);
     P.tseg[g.w] = 0u;
     P.qseg[g.w] = 0u;
   }
+}
+
+// raygen.cu:148-168 renderPixel up to the first rtTrace, raygen.cu:50-64
+// generateRay, raygen.cu:66-74 color() set-up.  Position p of state set 0
+// (a dense set) starts the path of slot slot_of(p, nslots).
+__global__ __launch_bounds__(BLOCK) void k_raygen(DevPaths P, RenderConsts rc, uint32_t nslots) {
+  if (rc.frame_dev) rc.frame0 = *rc.frame_dev;  // replayed graphs take the batch's frame from memory
+  reset_segments(P, nslots, my_seg());
   for (uint32_t p = blockIdx.x * BLOCK + threadIdx.x; p < nslots; p += gridDim.x * BLOCK) {
     const uint32_t s = slot_of(p, nslots);
-    uint32_t f = s / rc.WH, pix = s - f * rc.WH;
-    uint32_t x = pix % rc.W, y = pix / rc.W;
-    uint32_t seed = tea64(rc.W * y + x, rc.frame0 + f);
-    float u = ((float)x + rnd(seed)) / (float)rc.W;
-    float v = ((float)y + rnd(seed)) / (float)rc.H;
-    const DevCamera& c = rc.cam;
-    v3 cu = mk3(c.u[0], c.u[1], c.u[2]), cv = mk3(c.v[0], c.v[1], c.v[2]);
-    v3 rd = c.lens_radius * random_in_unit_disk(seed);
-    v3 lens = cu * rd.x + cv * rd.y;
-    v3 origin = mk3(c.org[0], c.org[1], c.org[2]) + lens;
-    v3 dir = mk3(c.llc[0], c.llc[1], c.llc[2]) + u * mk3(c.hor[0], c.hor[1], c.hor[2]) +
-             v * mk3(c.ver[0], c.ver[1], c.ver[2]) - origin;
-    (void)rnd(seed);  // prd.time (raygen.cu:69), consumed but unused
-    S.o[p] = make_float4(origin.x, origin.y, origin.z, __uint_as_float(seed));
-    uint32_t dw = s;
-    if ((rc.flags & PT_FLAG_RADIANCE_CACHE) && train_hash(pix, rc.frame0 + f) < rc.cache.train_threshold) dw |= TRAIN_BIT;
-    S.d[p] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(dw));
-    S.T[p] = make_float4(1.f, 1.f, 1.f, 1.f);
-    S.L[p] = make_float4(0.f, 0.f, 0.f, __uint_as_float(0x811C9DC5u));
-    P.result[s] = make_float4(0.f, 0.f, 0.f, __uint_as_float(rc.max_depth));
+    const uint32_t f = s / rc.WH;
+    start_path(P, rc, p, s, rc.frame0 + f, s - f * rc.WH, false, 0u);
+  }
+}
+
+// The same for a batch of scenes: slots [item_off[k], item_off[k + 1]) are
+// frame item_frame[k] of scene item_scene[k] (k < nitems, item_off[0] = 0,
+// item_off[nitems] = nslots); the scene index goes into the RNG state's top
+// byte.
+__global__ __launch_bounds__(BLOCK) void k_raygen_batch(DevPaths P, const RenderConsts* __restrict__ rcs,
+                                                        const uint32_t* __restrict__ item_off,
+                                                        const uint32_t* __restrict__ item_scene,
+                                                        const uint32_t* __restrict__ item_frame, uint32_t nitems,
+                                                        uint32_t nslots) {
+  reset_segments(P, nslots, my_seg());
+  for (uint32_t p = blockIdx.x * BLOCK + threadIdx.x; p < nslots; p += gridDim.x * BLOCK) {
+    const uint32_t s = slot_of(p, nslots);
+    uint32_t lo = 0, hi = nitems - 1;  // last item starting at or before s
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) >> 1;
+      if (item_off[mid] <= s) lo = mid;
+      else hi = mid - 1;
+    }
+    const uint32_t sid = item_scene[lo];
+    start_path(P, rcs[sid], p, s, item_frame[lo], s - item_off[lo], true, sid);
   }
 }
 
@@ -268,9 +302,9 @@ PTD uint32_t material_key(const DevScene& sc, float4 h) {
 
 // Closest-hit query of every live path (raygen.cu:77).  With SORT the
 // material bucket of each hit is recorded and counted per wave and bucket.
-template <bool COUNT, bool SORT>
-__global__ __launch_bounds__(BLOCK) void k_extend(DevScene sc, DevPaths P, int in, float tmin, uint2* spill,
-                                                  uint64_t* stats) {
+template <bool COUNT, bool SORT, bool MULTI>
+__global__ __launch_bounds__(BLOCK) void k_extend(DevScene sc0, DevPaths P, int in, float tmin, uint2* spill,
+                                                  uint64_t* stats, const DevScene* __restrict__ scenes) {
   __shared__ uint32_t s_ref[STK_LDS * BLOCK];
   __shared__ float s_tn[STK_LDS * BLOCK];
   Stack st = make_stack(s_ref, s_tn, spill, P.counters + CTR_STACK_OVERFLOW);
@@ -284,6 +318,7 @@ __global__ __launch_bounds__(BLOCK) void k_extend(DevScene sc, DevPaths P, int i
     uint32_t key = 0xFFu;
     if (j < n) {
       float4 o = S.o[i], d = S.d[i];
+      const DevScene& sc = MULTI ? scenes[__float_as_uint(o.w) >> 24] : sc0;
       HitOut h = trace_closest<COUNT>(sc, mk3(o), mk3(d), tmin, FINF, st, c);
       float4 hv = make_float4(h.t, __int_as_float(h.pos), __int_as_float(h.prim), __int_as_float(h.inst));
       P.hit[i] = hv;
@@ -466,11 +501,13 @@ PTD bool scatter_and_roulette(const Surface& f, v3 Wo, v3 N, uint32_t& seed, uin
 // FEAT: SH_COUNT counts the shadow rays' traversal, SH_CACHE records
 // radiance-cache training vertices and queries, SH_ENV samples environment
 // lights; each is compiled only into the variants that use it.
-constexpr int SH_COUNT = 1, SH_CACHE = 2, SH_ENV = 4;
+constexpr int SH_COUNT = 1, SH_CACHE = 2, SH_ENV = 4, SH_MULTI = 8;
 template <int MS, int FEAT>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_shade(DevScene sc, DevPaths P, int in, int src, const float4* __restrict__ hits,
-                                                 int out, uint32_t depth, RenderConsts rc, uint2* spill,
-                                                 uint64_t* stats) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_shade(
+    DevScene sc0, DevPaths P, int in, int src, const float4* __restrict__ hits, int out, uint32_t depth,
+    RenderConsts rc0, uint2* spill, uint64_t* stats, const DevScene* __restrict__ scenes,
+    const RenderConsts* __restrict__ rcs) {
+  constexpr bool MULTI = (FEAT & SH_MULTI) != 0;
   __shared__ uint32_t s_ref[STK_LDS * BLOCK];
   __shared__ float s_tn[STK_LDS * BLOCK];
   Stack st = make_stack(s_ref, s_tn, spill, P.counters + CTR_STACK_OVERFLOW);
@@ -478,15 +515,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
   const uint32_t n = seg_count(P, src, g);
   const PathState S = P.st[src], D = P.st[out];
   uint32_t survivors = 0;  // wave-uniform: paths written to this wave's segment of `out`
-  // light table: environment lights take part only with PT_FLAG_ENV_MIS
   constexpr bool COUNT = (FEAT & SH_COUNT) != 0;
-  const bool env_mis = (FEAT & SH_ENV) && (rc.flags & PT_FLAG_ENV_MIS) != 0;
-  const DevLight* lights = env_mis ? sc.lights : sc.lights_plain;
-  const int nl = env_mis ? sc.num_lights : sc.num_lights_plain;
-  const bool env_light = env_mis && sc.env_total != 0 && sc.num_lights != sc.num_lights_plain;
-  const bool cache_on = (FEAT & SH_CACHE) && (rc.flags & PT_FLAG_RADIANCE_CACHE) != 0;
+  const bool cache_on = !MULTI && (FEAT & SH_CACHE) && (rc0.flags & PT_FLAG_RADIANCE_CACHE) != 0;
   TraceCounts tc{0, 0, 0};
-  uint32_t shadow_rays = 0;
+  uint32_t shadow_rays = 0, exhausted = 0;
   // radiance-cache records of this wave (wave-uniform running counts):
   // training vertices continue the batch's count, queries start at zero
   uint32_t tcount = 0, qcount = 0, tcap = 0;
@@ -503,6 +535,16 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
       float4 o4 = S.o[i], d4 = S.d[i], t4 = S.T[i], l4 = S.L[i];
       v3 d = mk3(d4), T = mk3(t4), L = mk3(l4);
       uint32_t seed = __float_as_uint(o4.w), dw = __float_as_uint(d4.w);
+      // a batch of scenes keeps the path's scene in the RNG state's top
+      // byte (the 24-bit outputs never depend on it)
+      const uint32_t sid = MULTI ? (seed >> 24) : 0u;
+      const DevScene& sc = MULTI ? scenes[sid] : sc0;
+      const RenderConsts& rc = MULTI ? rcs[sid] : rc0;
+      // light table: environment lights take part only with PT_FLAG_ENV_MIS
+      const bool env_mis = (FEAT & SH_ENV) && (rc.flags & PT_FLAG_ENV_MIS) != 0;
+      const DevLight* lights = env_mis ? sc.lights : sc.lights_plain;
+      const int nl = env_mis ? sc.num_lights : sc.num_lights_plain;
+      const bool env_light = env_mis && sc.env_total != 0 && sc.num_lights != sc.num_lights_plain;
       // dw keeps slot | TRAIN_BIT | SPEC_BIT; fields are taken where used
       uint32_t flags = dw >> 31, sig = __float_as_uint(l4.w);
       float prev_pdf = t4.w;
@@ -620,14 +662,21 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
           v3 ndir;
           alive = scatter_and_roulette<MS>(f, r.Wo, r.N, seed, sig, T, L, ndir, flags, prev_pdf, depth, rc, res);
           if (alive) {
-            no = make_float4(r.P.x, r.P.y, r.P.z, __uint_as_float(seed));
+            no = make_float4(r.P.x, r.P.y, r.P.z, __uint_as_float(MULTI ? ((seed & 0xFFFFFFu) | (sid << 24)) : seed));
             nd = make_float4(ndir.x, ndir.y, ndir.z, __uint_as_float((dw & ~SPEC_BIT) | (flags << 31)));
             nT = make_float4(T.x, T.y, T.z, prev_pdf);
             nL = make_float4(L.x, L.y, L.z, __uint_as_float(sig));
           }
         }
       }
-      if (!alive) {
+      if (MULTI && alive && depth + 1 >= rc.max_depth) {
+        // the last bounce of this path's scene: it ends like the paths that
+        // outlive a single scene's loop (k_exhausted), result left as raygen
+        // set it
+        alive = false;
+        exhausted++;
+        P.sig_final[dw & SLOT_MASK] = sig;
+      } else if (!alive) {
         P.result[dw & SLOT_MASK] = res;
         P.sig_final[dw & SLOT_MASK] = sig;
       }
@@ -651,6 +700,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
   uint32_t a = tc.nodes, b = tc.prims, c = tc.insts;
   for (int off = 32; off > 0; off >>= 1) {
     shadow_rays += __shfl_down(shadow_rays, off);
+    if (MULTI) exhausted += __shfl_down(exhausted, off);
     if (COUNT) {
       a += __shfl_down(a, off);
       b += __shfl_down(b, off);
@@ -661,9 +711,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
     P.seg[(size_t)out * MAX_WAVES + g.w] = survivors;
     if (cache_on) {
       P.tseg[g.w] = tcount;
-      if (depth == rc.cache.query_depth) P.qseg[g.w] = qcount;
+      if (depth == rc0.cache.query_depth) P.qseg[g.w] = qcount;
     }
     stat_add(stats, g, STAT_SHADOW, shadow_rays);
+    if (MULTI) stat_add(stats, g, STAT_EXHAUSTED, exhausted);
     if (COUNT) {
       stat_add(stats, g, STAT_SH_NODES, a);
       stat_add(stats, g, STAT_SH_PRIMS, b);
@@ -875,16 +926,28 @@ void launch_next_frames(uint32_t* frame_dev, uint32_t step, hipStream_t st) {
 void launch_raygen(const DevPaths& P, const RenderConsts& rc, uint32_t nslots, int grid, hipStream_t st) {
   hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(BLOCK), 0, st, P, rc, nslots);
 }
+void launch_raygen_batch(const DevPaths& P, const RenderConsts* rcs, const uint32_t* item_off,
+                         const uint32_t* item_scene, const uint32_t* item_frame, uint32_t nitems, uint32_t nslots,
+                         int grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_raygen_batch, dim3(grid), dim3(BLOCK), 0, st, P, rcs, item_off, item_scene, item_frame, nitems,
+                     nslots);
+}
 void launch_extend(const DevScene& sc, const DevPaths& P, int in, float tmin, uint2* spill, uint64_t* stats,
-                   bool count_traversal, bool sort, int grid, hipStream_t st) {
-  if (count_traversal && sort)
-    hipLaunchKernelGGL((k_extend<true, true>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, tmin, spill, stats);
-  else if (count_traversal)
-    hipLaunchKernelGGL((k_extend<true, false>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, tmin, spill, stats);
-  else if (sort)
-    hipLaunchKernelGGL((k_extend<false, true>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, tmin, spill, stats);
-  else
-    hipLaunchKernelGGL((k_extend<false, false>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, tmin, spill, stats);
+                   bool count_traversal, bool sort, int grid, hipStream_t st, const DevScene* scenes) {
+#define PT_EXTEND(C, S, M) \
+  hipLaunchKernelGGL((k_extend<C, S, M>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, tmin, spill, stats, scenes)
+  if (scenes) {
+    if (count_traversal && sort) PT_EXTEND(true, true, true);
+    else if (count_traversal) PT_EXTEND(true, false, true);
+    else if (sort) PT_EXTEND(false, true, true);
+    else PT_EXTEND(false, false, true);
+  } else {
+    if (count_traversal && sort) PT_EXTEND(true, true, false);
+    else if (count_traversal) PT_EXTEND(true, false, false);
+    else if (sort) PT_EXTEND(false, true, false);
+    else PT_EXTEND(false, false, false);
+  }
+#undef PT_EXTEND
 }
 void launch_sort(const DevPaths& P, int in, int grid, hipStream_t st) {
   hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(SCAN_THREADS), 0, st, P, (uint32_t)(grid * WAVES));
@@ -893,19 +956,28 @@ void launch_sort(const DevPaths& P, int in, int grid, hipStream_t st) {
 template <int MS, int FEAT>
 static void launch_shade_v(const DevScene& sc, const DevPaths& P, int in, int src, const float4* hits, int out,
                            uint32_t depth, const RenderConsts& rc, uint2* spill, uint64_t* stats, int grid,
-                           hipStream_t st) {
+                           hipStream_t st, const DevScene* scenes, const RenderConsts* rcs) {
   hipLaunchKernelGGL((k_shade<MS, FEAT>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, src, hits, out, depth, rc, spill,
-                     stats);
+                     stats, scenes, rcs);
 }
 void launch_shade(const DevScene& sc, const DevPaths& P, int in, bool sorted, int out, uint32_t depth,
                   const RenderConsts& rc, int matset, bool count, uint2* spill, uint64_t* stats, int grid,
-                  hipStream_t st) {
+                  hipStream_t st, const DevScene* scenes, const RenderConsts* rcs, bool batch_env) {
   const int src = sorted ? 2 : in;
   const float4* hits = sorted ? P.hit_sorted : P.hit;
   const bool env = (rc.flags & PT_FLAG_ENV_MIS) && sc.num_lights != sc.num_lights_plain;
   const bool cache = (rc.flags & PT_FLAG_RADIANCE_CACHE) != 0;
-#define PT_SHADE(MSV, F) launch_shade_v<MSV, F>(sc, P, in, src, hits, out, depth, rc, spill, stats, grid, st)
-  if (cache) {  // every material; no traversal counting
+#define PT_SHADE(MSV, F) \
+  launch_shade_v<MSV, F>(sc, P, in, src, hits, out, depth, rc, spill, stats, grid, st, scenes, rcs)
+  if (scenes) {  // a batch of scenes: every material, per-path scene
+    if (batch_env) {
+      if (count) PT_SHADE(MS_ALL, SH_MULTI | SH_ENV | SH_COUNT);
+      else PT_SHADE(MS_ALL, SH_MULTI | SH_ENV);
+    } else {
+      if (count) PT_SHADE(MS_ALL, SH_MULTI | SH_COUNT);
+      else PT_SHADE(MS_ALL, SH_MULTI);
+    }
+  } else if (cache) {  // every material; no traversal counting
     if (env) PT_SHADE(MS_ALL, SH_CACHE | SH_ENV);
     else PT_SHADE(MS_ALL, SH_CACHE);
   } else if (env) {

@@ -156,6 +156,8 @@ def _declare(lib):
         "pt_set_camera": (C.c_int32, [P, C.POINTER(Camera)]),
         "pt_render": (C.c_int32, [P, C.POINTER(RenderParams), C.c_uint32, C.c_uint32,
                                   C.POINTER(RenderStats)]),
+        "pt_render_batch": (C.c_int32, [C.POINTER(C.c_void_p), C.POINTER(RenderParams), C.POINTER(C.c_uint32),
+                                        C.c_uint32, C.POINTER(RenderStats)]),
         "pt_sync": (C.c_int32, [P]),
         "pt_read_accum": (C.c_int32, [P, P, C.POINTER(C.c_uint32)]),
         "pt_read_display": (C.c_int32, [P, P]),
@@ -398,6 +400,20 @@ class Context:
         self._check(self.lib.pt_read_accel(self.h, ptr(nodes), C.byref(nn), ptr(order), ptr(morton),
                                            C.byref(npm), ptr(info), C.byref(nb)), "pt_read_accel")
         return nodes, order, morton, info
+
+
+def render_batch(contexts, params, frames, stats=True):
+    """pt_render_batch: frames [0, frames[i]) of every contexts[i] in one
+    wavefront (contexts[0] runs it); returns the batch's RenderStats."""
+    n = len(contexts)
+    lib = contexts[0].lib
+    hs = (C.c_void_p * n)(*[c.h for c in contexts])
+    ps = (RenderParams * n)(*params)
+    fs = (C.c_uint32 * n)(*frames)
+    s = RenderStats() if stats else None
+    contexts[0]._check(lib.pt_render_batch(hs, ps, fs, n, C.byref(s) if s is not None else None),
+                       "pt_render_batch")
+    return s
 
 
 def header_symbols():
