@@ -172,16 +172,81 @@ def cache_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def batch_workload(args):
+    """The 64-scene batch: 32 sphere scenes (50..5000 spheres) and 32 mesh
+    scenes (10k..2M triangles, sky light), square resolutions 256..1024, 4,
+    16 or 64 spp, every scene's frames rendered by one pt_render_batch per
+    step.  Beside it, the same scenes rendered one pt_render each, back to
+    back (the GPU time the batch saves on small and uneven scenes)."""
+    import ptamd
+
+    scenes = [ptamd.Scene(ptamd.PT_SCENE_BATCH, 0, 0, i) for i in range(64)]
+    ctxs = [ptamd.Context(0) for _ in scenes]
+    t0 = time.perf_counter()
+    prims = 0
+    for sc, ctx in zip(scenes, ctxs):
+        ctx.set_scene(sc.desc)
+        a = ctx.build_accel()
+        ctx.set_camera(sc.camera)
+        prims += a.num_primitives
+    log(f"[bench] 64 scenes built ({prims} primitives) in {time.perf_counter() - t0:.1f} s")
+    params = [sc.render_params() for sc in scenes]
+    frames = [sc.info.spp for sc in scenes]
+    paths = sum(sc.info.width * sc.info.height * sc.info.spp for sc in scenes)
+    st = ptamd.render_batch(ctxs, params, frames)  # rays of one step (and warm-up)
+    rays_step = st.extension_rays + st.shadow_rays
+    log(f"[bench] batch: {paths} paths, {st.extension_rays} closest-hit + {st.shadow_rays} shadow rays, "
+        f"{st.gpu_ms:.1f} ms")
+    for _ in range(args.warmup):
+        ptamd.render_batch(ctxs, params, frames, stats=False)
+    ctxs[0].sync()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ptamd.render_batch(ctxs, params, frames, stats=False)
+        ctxs[0].sync()
+        log(f"[bench] step {k} done")
+    wall = time.perf_counter() - t0
+    # the same work as 64 separate renders (one scene at a time)
+    seq_ms = 0.0
+    for sc, ctx, p, f in zip(scenes, ctxs, params, frames):
+        seq_ms += ctx.render(p, 0, f).gpu_ms
+    for ctx in ctxs:
+        ctx.close()
+    ms = wall * 1e3 / args.steps
+    out = {
+        "metric": "Mrays/s",
+        "value": round(rays_step * args.steps / wall / 1e6, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "batch_64_scenes_32_spheres_50-5000_32_meshes_10k-2M_256-1024px_4-64spp",
+                   "paths": paths, "primitives": prims},
+        "samples_per_s": round(paths * args.steps / wall),
+        "rays_per_step": {"closest_hit": st.extension_rays, "shadow": st.shadow_rays},
+        "one_render_per_scene_ms": round(seq_ms, 2),
+        "batch_speedup": round(seq_ms / ms, 3),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--spp", type=int, default=0, help="override samples per pixel (0: the workload's own)")
-    ap.add_argument("--workload", choices=["cornell", "meshes", "cache"], default="cornell",
+    ap.add_argument("--workload", choices=["cornell", "meshes", "cache", "batch"], default="cornell",
                     help="cornell: Cornell box 512x512 64 spp (the reported metric); meshes: 8 displaced "
                          "octaspheres (2^20 triangles) under an importance-sampled HDRI, 1920x1080; cache: the "
-                         "mesh scene on an orbiting camera with the learned radiance cache, equal-time relMSE")
+                         "mesh scene on an orbiting camera with the learned radiance cache, equal-time relMSE; "
+                         "batch: 64 sphere and mesh scenes of uneven size in one pt_render_batch")
     ap.add_argument("--frames", type=int, default=60, help="cache workload: animation frames before the evaluation")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-frames", type=int, default=64)
@@ -202,6 +267,9 @@ def main():
 
     if args.workload == "cache":
         cache_workload(args)
+        return
+    if args.workload == "batch":
+        batch_workload(args)
         return
     meshes = args.workload == "meshes"
     scene = ptamd.Scene(ptamd.PT_SCENE_MESHES if meshes else ptamd.PT_SCENE_CORNELL)

@@ -19,6 +19,7 @@ extern "C" {
 #define PT_SCENE_RANDOM_SPHERES 0 /* random-spheres scene  */
 #define PT_SCENE_CORNELL 1        /* Cornell box with MIS  */
 #define PT_SCENE_MESHES 2         /* displaced meshes under an importance-sampled HDRI */
+#define PT_SCENE_BATCH 3          /* scene `variant` (0..63) of the 64-scene batch workload */
 
 typedef struct pt_scene_handle pt_scene_handle;
 
@@ -36,7 +37,10 @@ typedef struct pt_scene_info {
  * `variant`: the scene RNG seed of PT_SCENE_RANDOM_SPHERES; for
  * PT_SCENE_MESHES 0 builds the full workload and 1..6 the same layout with
  * octaspheres subdivided `variant` times and a 256x128 sky (small copies for
- * tests). */
+ * tests); for PT_SCENE_BATCH the scene index: 0..31 sphere scenes of
+ * 50..5000 spheres, 32..63 mesh scenes of 10k..2M triangles under a sky
+ * light, each with its own square resolution (side 256..1024) and 4, 16 or
+ * 64 spp in its info. */
 pt_status pt_scene_build(int32_t config, uint32_t width, uint32_t height, uint32_t variant,
                          pt_scene_handle** out);
 /* Views owned by the handle, valid until pt_scene_free. */

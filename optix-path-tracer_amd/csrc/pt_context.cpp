@@ -260,9 +260,7 @@ pt_status pt_create(int32_t device, pt_context** out) {
     delete c;
     return PT_ERR_DEVICE;
   }
-  size_t spill_bytes = (size_t)MAX_GRID * BLOCK * STK_SPILL * sizeof(uint2);
-  if ((e = hipMalloc(&c->spill, spill_bytes)) != hipSuccess ||
-      (e = hipMalloc(&c->stats, (size_t)MAX_WAVES * STAT_WORDS * sizeof(uint64_t))) != hipSuccess ||
+  if ((e = hipMalloc(&c->stats, (size_t)MAX_WAVES * STAT_WORDS * sizeof(uint64_t))) != hipSuccess ||
       (e = hipMalloc(&c->stats_total, STAT_WORDS * sizeof(uint64_t))) != hipSuccess ||
       (e = hipMalloc(&c->qcounters, CTR_WORDS * sizeof(uint32_t))) != hipSuccess) {
     g_create_err = std::string("hipMalloc: ") + hipGetErrorString(e);
@@ -821,6 +819,15 @@ pt_status pt_set_camera(pt_context* c, const pt_camera* cam) {
   return PT_OK;
 }
 
+// traversal-stack spill area of the grid-stride kernels, allocated by the
+// first call that traces (a context that only takes part in batches run by
+// another context never needs one)
+static pt_status ensure_spill(pt_context* c) {
+  if (c->spill) return PT_OK;
+  HIPCHK(c, hipMalloc(&c->spill, (size_t)MAX_GRID * BLOCK * STK_SPILL * sizeof(uint2)));
+  return PT_OK;
+}
+
 static pt_status ensure_paths(pt_context* c, uint64_t cap) {
   if (cap <= c->path_capacity) return PT_OK;
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -953,6 +960,7 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
   const bool use_cache = (rp->flags & PT_FLAG_RADIANCE_CACHE) != 0;
   if (use_cache && !c->cache_ready) return fail(c, PT_ERR_STATE, "PT_FLAG_RADIANCE_CACHE needs pt_cache_init");
   pt_status ps = ensure_paths(c, (uint64_t)fpb * WH);
+  if (ps == PT_OK) ps = ensure_spill(c);
   if (ps != PT_OK) return ps;
   if (use_cache) {
     if ((ps = ensure_cache_records(c, (uint64_t)fpb * WH)) != PT_OK) return ps;
@@ -1405,6 +1413,7 @@ pt_status pt_render_batch(pt_context* const* ctxs, const pt_render_params* param
   }
   if (max_slots) {
     pt_status ps = ensure_paths(B, max_slots);
+    if (ps == PT_OK) ps = ensure_spill(B);
     if (ps != PT_OK) return ps;
   }
   const bool want_stats = out != nullptr;
@@ -1631,6 +1640,7 @@ static pt_status trace_common(pt_context* c, const pt_ray* rays, uint64_t n, voi
   if (!c->accel_ready) return fail(c, PT_ERR_STATE, "acceleration structure not built");
   if (n == 0) return PT_OK;
   (void)hipSetDevice(c->device);
+  if (pt_status ps = ensure_spill(c); ps != PT_OK) return ps;
   std::vector<DevBuf> T;
   pt_ray* dr = nullptr;
   void* dout = nullptr;
@@ -1679,6 +1689,7 @@ pt_status pt_trace_closest_device(pt_context* c, const pt_ray* d_rays, uint64_t 
   if (!c->accel_ready) return fail(c, PT_ERR_STATE, "acceleration structure not built");
   if (n == 0) return PT_OK;
   (void)hipSetDevice(c->device);
+  if (pt_status ps = ensure_spill(c); ps != PT_OK) return ps;
   launch_trace_closest(c->dsc, d_rays, n, d_hits, nullptr, c->spill, c->qcounters + CTR_STACK_OVERFLOW, grid_for(n),
                        c->stream);
   hipError_t e = hipGetLastError();

@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <array>
+#include <map>
 #include <cmath>
 #include <memory>
 #include <numeric>
@@ -375,5 +376,190 @@ inline ConfigDefaults MeshScene(App_State& app, SceneArena& A, int levels = 7, u
   camera.set(app.context);
   ConfigDefaults d{1920, 1080, 256, 12, true, 11};
   d.flags = PT_FLAG_MATERIAL_SORT | PT_FLAG_ENV_MIS;
+  return d;
+}
+
+// ---------------------------------------------------------------- C3
+// The 64-scene batch workload: scenes 0..31 are sphere scenes of 50..5000
+// spheres, 32..63 mesh scenes of 10k..2M triangles (both log-uniform), each
+// at a square resolution of side log-uniform in [256, 1024] and 4, 16 or 64
+// samples per pixel.  The parameters come from one mt19937(3) stream (three
+// 24-bit draws per scene, in scene order); scene i's content from the scene
+// RNG reseeded with 3000 + i.
+struct BatchSceneSpec {
+  bool spheres;
+  uint32_t count;  // spheres or triangles asked for
+  uint32_t side;   // image width = height
+  uint32_t spp;
+};
+constexpr uint32_t BATCH_SCENES = 64;
+inline BatchSceneSpec batch_scene_spec(uint32_t i) {
+  std::mt19937 g(3);
+  auto u = [&g]() { return (double)(g() >> 8) * (1.0 / 16777216.0); };
+  BatchSceneSpec s{};
+  for (uint32_t k = 0; k <= i; k++) {
+    const double a = u(), b = u(), c = u();
+    s.spheres = k < BATCH_SCENES / 2;
+    s.count = s.spheres ? (uint32_t)std::lround(50.0 * std::pow(100.0, a))
+                        : (uint32_t)std::lround(1e4 * std::pow(200.0, a));
+    s.side = (uint32_t)std::lround(256.0 * std::pow(4.0, b));
+    const uint32_t spps[3] = {4, 16, 64};
+    s.spp = spps[std::min(2, (int)(c * 3.0))];
+  }
+  return s;
+}
+
+// Sphere scene of `count` spheres: the random-spheres recipe (ground sphere,
+// 80 % Lambertian, 15 % metal, 5 % glass, gradient sky) on a jittered
+// ceil(sqrt(count))^2 grid, camera pulled back with the grid.
+inline ConfigDefaults BatchSpheres(App_State& app, SceneArena& A, uint32_t seed, uint32_t count) {
+  rnd_reseed(seed);
+  Light_Sampler lights;
+  setRayGenerationProgram(app.context, lights);
+  setMissProgram(app.context, GRADIENT, make_float3(1.f), make_float3(0.5f, 0.7f, 1.f));
+  setExceptionProgram(app.context);
+  Group group = app.context->createGroup();
+  group->setAcceleration(app.context->createAcceleration("Trbvh"));
+  Hitable_List list;
+  list.push(A.hit<Sphere>(make_float3(0.f, -1000.f, 0.f), 1000.f,
+                          A.mat<Lambertian>(A.tex<Constant_Texture>(0.5f))));
+  const int K = (int)std::ceil(std::sqrt((double)count));
+  uint32_t made = 0;
+  for (int a = -K / 2; a < K - K / 2 && made < count; a++) {
+    for (int b = -K / 2; b < K - K / 2 && made < count; b++, made++) {
+      float choose_mat = rnd();
+      float3 center = make_float3((float)a + 0.9f * rnd(), 0.2f, (float)b + 0.9f * rnd());
+      BRDF* m;
+      if (choose_mat < 0.8f) {
+        float r = rnd() * rnd(), g = rnd() * rnd(), bl = rnd() * rnd();
+        m = A.mat<Lambertian>(A.tex<Constant_Texture>(r, g, bl));
+      } else if (choose_mat < 0.95f) {
+        float r = 0.5f * (1.f + rnd()), g = 0.5f * (1.f + rnd()), bl = 0.5f * (1.f + rnd());
+        m = A.mat<Metal>(A.tex<Constant_Texture>(r, g, bl), 0.5f * rnd());
+      } else {
+        Texture* one = A.tex<Constant_Texture>(1.f);
+        m = A.mat<Dielectric>(one, one, 1.5f, 0.f);
+      }
+      list.push(A.hit<Sphere>(center, 0.2f, m));
+    }
+  }
+  list.addElementsTo(group, app.context);
+  app.context["world"]->set(group);
+  const float pull = std::max(1.f, (float)K / 22.f);
+  Camera camera(make_float3(13.f * pull, 2.f * pull, 3.f * pull), make_float3(0.f), make_float3(0.f, 1.f, 0.f),
+                20.f, float(app.W) / float(app.H), 0.f, 10.f, 0.f, 1.f);
+  camera.set(app.context);
+  ConfigDefaults d{app.W, app.H, 16, 50, true, 11};
+  d.flags = PT_FLAG_MATERIAL_SORT;
+  return d;
+}
+
+// Displaced octasphere of subdivision `levels` and noise seed `seed`,
+// generated once per (levels, seed) and shared by every scene that uses it.
+inline void cached_displaced_sphere(int levels, uint32_t seed, std::vector<float3>& v, std::vector<uint3>& f,
+                                    std::vector<float3>& n) {
+  struct Entry {
+    std::vector<float3> v, n;
+    std::vector<uint3> f;
+  };
+  static std::map<std::pair<int, uint32_t>, Entry> cache;
+  auto key = std::make_pair(levels, seed);
+  auto it = cache.find(key);
+  if (it == cache.end()) {
+    Entry e;
+    displaced_sphere(levels, seed, 0.05, e.v, e.f, e.n);
+    it = cache.emplace(key, std::move(e)).first;
+  }
+  v = it->second.v;
+  f = it->second.f;
+  n = it->second.n;
+}
+
+// Mesh scene of about `count` triangles: M displaced octaspheres of
+// subdivision L (8 * 4^L triangles each, L in [2, 7], M in [1, 32]) with
+// the mesh-scene materials and random similarity transforms, a ground quad,
+// and a 256x128 synthetic sky importance-sampled as an environment light.
+inline ConfigDefaults BatchMeshes(App_State& app, SceneArena& A, uint32_t seed, uint32_t count) {
+  Light_Sampler lights;
+  Environment_PDF env;
+  lights.pdf.push_back(env.createPDF(app.context));
+  lights.sample.push_back(env.createSample(app.context));
+  lights.emissions.push_back(make_float3(1.f));
+  setRayGenerationProgram(app.context, lights);
+  HDR_Texture* sky = A.tex<HDR_Texture>(256u, 128u, synthetic_sky(256, 128, 35.0, 40.0, 5e4, 0.5));
+  setMissProgram(app.context, HDR, sky, true);
+  setExceptionProgram(app.context);
+  Group group = app.context->createGroup();
+  group->setAcceleration(app.context->createAcceleration("Trbvh"));
+  int L = (int)std::floor(std::log((double)count / 64.0) / std::log(4.0));
+  L = std::max(2, std::min(7, L));
+  const double per = 8.0 * std::pow(4.0, L);
+  const uint32_t M = (uint32_t)std::max(1.0, std::min(32.0, std::round((double)count / per)));
+  rnd_reseed(seed);
+  Hitable_List list;
+  Mesh_List meshes;
+  double ymin = 1e30;
+  const double PI = 3.14159265358979323846;
+  const float spread = 10.f * std::sqrt((float)M / 8.f);
+  for (uint32_t i = 0; i < M; i++) {
+    float s = 0.5f + 1.5f * rnd();
+    float ang = 360.f * rnd();
+    float tx = spread * (2.f * rnd() - 1.f), ty = spread * (2.f * rnd() - 1.f), tz = spread * (2.f * rnd() - 1.f);
+    BRDF* m;
+    switch (i % 4) {
+      case 0: {
+        float r = 0.05f + 0.55f * rnd();
+        float c0 = 0.4f + 0.6f * rnd(), c1 = 0.4f + 0.6f * rnd(), c2 = 0.4f + 0.6f * rnd();
+        m = A.mat<Torrance_Sparrow>(A.tex<Constant_Texture>(c0, c1, c2), r, r);
+        break;
+      }
+      case 1: {
+        float nu = 10.f + 990.f * rnd(), nv = 10.f + 990.f * rnd();
+        float d0 = 0.1f + 0.8f * rnd(), d1 = 0.1f + 0.8f * rnd(), d2 = 0.1f + 0.8f * rnd();
+        float sp = 0.04f + 0.26f * rnd();
+        m = A.mat<Ashikhmin_Shirley>(A.tex<Constant_Texture>(d0, d1, d2), A.tex<Constant_Texture>(sp), nu, nv);
+        break;
+      }
+      case 2: {
+        float sigma = (float)(30.0 * PI / 180.0) * rnd();
+        float c0 = 0.2f + 0.7f * rnd(), c1 = 0.2f + 0.7f * rnd(), c2 = 0.2f + 0.7f * rnd();
+        m = A.mat<Oren_Nayar>(A.tex<Constant_Texture>(c0, c1, c2), sigma);
+        break;
+      }
+      default: {
+        Texture* one = A.tex<Constant_Texture>(1.f);
+        m = A.mat<Dielectric>(one, one, 1.5f, 0.f);
+        break;
+      }
+    }
+    std::vector<float3> v, n;
+    std::vector<uint3> f;
+    cached_displaced_sphere(L, i, v, f, n);
+    for (const float3& p : v) ymin = std::fmin(ymin, (double)ty + (double)s * (double)p.y);
+    A.meshes.emplace_back(new Mesh(std::move(v), std::move(f), m, std::move(n)));
+    Mesh* mesh = A.meshes.back().get();
+    mesh->scale(make_float3(s));
+    mesh->rotate(ang, Y_AXIS);
+    mesh->translate(make_float3(tx, ty, tz));
+    meshes.push(mesh);
+  }
+  meshes.addElementsTo(group, app.context);
+  const float g = 2.f * spread;
+  list.push(A.hit<AARect>(-g, g, -g, g, (float)ymin, false, Y_AXIS, A.mat<Lambertian>(A.tex<Constant_Texture>(0.5f))));
+  list.addElementsTo(group, app.context);
+  app.context["world"]->set(group);
+  Camera camera(make_float3(2.f * spread, 0.4f * spread, 0.f), make_float3(0.f), make_float3(0.f, 1.f, 0.f), 45.f,
+                float(app.W) / float(app.H), 0.f, 10.f, 0.f, 1.f);
+  camera.set(app.context);
+  ConfigDefaults d{app.W, app.H, 16, 12, true, 11};
+  d.flags = PT_FLAG_MATERIAL_SORT | PT_FLAG_ENV_MIS;
+  return d;
+}
+
+// Scene i of the batch workload at its own resolution (or W x H if given).
+inline ConfigDefaults BatchScene(App_State& app, SceneArena& A, uint32_t i) {
+  const BatchSceneSpec s = batch_scene_spec(i);
+  ConfigDefaults d = s.spheres ? BatchSpheres(app, A, 3000 + i, s.count) : BatchMeshes(app, A, 3000 + i, s.count);
+  d.spp = (int)s.spp;
   return d;
 }

@@ -40,6 +40,16 @@ pt_status pt_scene_build(int32_t config, uint32_t width, uint32_t height, uint32
           return PT_ERR_INVALID_ARG;
         }
         break;
+      case PT_SCENE_BATCH: {
+        if (variant >= BATCH_SCENES) {
+          delete h;
+          g_scene_err = "batch scene index must be 0..63";
+          return PT_ERR_INVALID_ARG;
+        }
+        const BatchSceneSpec bs = batch_scene_spec(variant);
+        d = ConfigDefaults{(int)bs.side, (int)bs.side, (int)bs.spp, 16, true, 11};
+        break;
+      }
       default:
         delete h;
         g_scene_err = "unknown scene config";
@@ -49,6 +59,7 @@ pt_status pt_scene_build(int32_t config, uint32_t width, uint32_t height, uint32
     h->app.H = height ? (int)height : d.H;
     if (config == PT_SCENE_RANDOM_SPHERES) d = RandomSpheres(h->app, h->arena, variant);
     else if (config == PT_SCENE_CORNELL) d = CornellBox(h->app, h->arena);
+    else if (config == PT_SCENE_BATCH) d = BatchScene(h->app, h->arena, variant);
     else if (variant == 0) d = MeshScene(h->app, h->arena);
     else d = MeshScene(h->app, h->arena, (int)variant, 256, 128);
     h->app.context->rec.flatten(h->flat);
