@@ -203,6 +203,32 @@ def test_paths_match_oracle(ctx, name, W, H, frames, sort):
     assert np.mean(disp[..., :3] == odisp[..., :3]) > 0.9
 
 
+def test_paths_match_oracle_light_geometry_moved(ctx):
+    """The light table and the emissive geometry are independent (as in the
+    reference, where the light PDFs are set up apart from the hitables):
+    with the emissive rect moved half a unit below the rect the light
+    samples, shadow rays meet the emitter short of the sample point or pass
+    it, and the paths still match the oracle's closest-hit shadow rule."""
+    W, H, frames = 64, 64, 2
+    sc = ptamd.Scene(ptamd.PT_SCENE_CORNELL, W, H)
+    d = sc.desc
+    moved = 0
+    for i in range(d.num_rects):
+        if d.materials[d.rects[i].material].type == ptamd.PT_MAT_DIFFUSE_LIGHT:
+            d.rects[i].k -= 0.5
+            moved += 1
+    assert moved == 1
+    params = sc.render_params(flags=ptamd.PT_FLAG_KEEP_PATHS, frames_per_batch=2)
+    say("[gpu] render paths cornell with the light geometry moved")
+    gpu_scene(ctx, d)
+    ctx.set_camera(sc.camera)
+    ctx.render(params, 0, frames)
+    rgb, sig, _ = ctx.read_paths(W, H)
+    orc = ptref.OracleScene(d)
+    _, (orgb, osig, _), _ = orc.render(sc.camera, params, 0, frames, keep_last=True)
+    compare_paths(rgb, sig, orgb, osig)
+
+
 def test_render_continues_accumulation(ctx):
     W, H = 64, 64
     sc = ptamd.Scene(ptamd.PT_SCENE_CORNELL, W, H)
