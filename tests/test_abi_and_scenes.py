@@ -158,3 +158,33 @@ def test_random_spheres_scene():
     s2 = ptamd.Scene(ptamd.PT_SCENE_RANDOM_SPHERES)
     assert all(s2.desc.spheres[k].center[:] == d.spheres[k].center[:] for k in range(n))
     assert len(big) == 3
+
+
+def test_batch_workload_scenes():
+    """The 64-scene batch: scenes 0..31 hold 50..5000 spheres (plus the
+    ground), 32..63 hold 10k..2M triangles under an environment light;
+    square resolutions of side 256..1024; 4, 16 or 64 spp; deterministic."""
+    sides, spps = [], set()
+    for i in (0, 1, 17, 31, 32, 45, 63):
+        s = ptamd.Scene(ptamd.PT_SCENE_BATCH, 0, 0, i)
+        d, info = s.desc, s.info
+        assert info.width == info.height and 256 <= info.width <= 1024
+        assert info.spp in (4, 16, 64)
+        sides.append(info.width)
+        spps.add(info.spp)
+        if i < 32:
+            assert d.num_meshes == 0 and 51 <= d.num_spheres <= 5001
+            assert d.miss.type == ptamd.PT_MISS_GRADIENT and d.num_lights == 0
+        else:
+            tris = sum(d.meshes[k].num_triangles for k in range(d.num_meshes))
+            assert d.num_spheres == 0 and 5000 <= tris <= 2_500_000
+            assert d.num_lights == 1 and d.lights[0].type == ptamd.PT_LIGHT_ENV
+            assert info.flags & ptamd.PT_FLAG_ENV_MIS
+        again = ptamd.Scene(ptamd.PT_SCENE_BATCH, 0, 0, i)
+        assert again.info.width == info.width and again.desc.num_spheres == d.num_spheres
+        if d.num_spheres:
+            k = d.num_spheres - 1
+            assert again.desc.spheres[k].center[:] == d.spheres[k].center[:]
+    assert len(set(sides)) > 3 and len(spps) > 1
+    with pytest.raises(ptamd.PtError):
+        ptamd.Scene(ptamd.PT_SCENE_BATCH, 0, 0, 64)
