@@ -43,6 +43,16 @@ typedef struct pt_scene_info {
  * 64 spp in its info. */
 pt_status pt_scene_build(int32_t config, uint32_t width, uint32_t height, uint32_t variant,
                          pt_scene_handle** out);
+/* A scene around one Wavefront OBJ model (mesh.hpp:49-204: the file's
+ * faces, normals, uvs and per-face MTL materials, each a Lambertian with
+ * its map_Kd image, PNG or binary PPM/PGM, or its Ka colour), lit by the
+ * Radiance .hdr environment `hdr_file` (importance-sampled; info.flags asks
+ * for PT_FLAG_ENV_MIS) or, when hdr_file is NULL, by the gradient sky.
+ * Paths inside the files are relative to assets_folder (may be NULL); the
+ * camera frames the model's bounding box from +Z; 0 for width / height
+ * means 512. */
+pt_status pt_scene_build_obj(const char* obj_file, const char* assets_folder, const char* hdr_file,
+                             uint32_t width, uint32_t height, pt_scene_handle** out);
 /* Views owned by the handle, valid until pt_scene_free. */
 const pt_scene_desc* pt_scene_get_desc(const pt_scene_handle* h);
 const pt_camera* pt_scene_get_camera(const pt_scene_handle* h);

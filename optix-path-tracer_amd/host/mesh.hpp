@@ -45,14 +45,25 @@ class Mesh {
       if (givenMaterial) {
         mat = givenMaterial->assignTo(g_context)->index;
       } else {
-        // one Lambertian per MTL material from its ambient colour
-        // (mesh.hpp:77-102); diffuse texture maps are not supported yet
+        // one Lambertian per MTL material, textured by its diffuse map or
+        // coloured by its ambient colour (mesh.hpp:77-102: one Lambertian
+        // whose Vector_Texture picks the face's texture, the same image)
         std::vector<int> mats;
         for (auto& m : om.materials) {
           pt_texture_desc t{};
-          t.type = PT_TEX_CONSTANT;
-          t.rgb[0] = m.ambient[0]; t.rgb[1] = m.ambient[1]; t.rgb[2] = m.ambient[2];
-          int ti = g_context->rec.add_texture(nullptr, t, nullptr);
+          int ti;
+          if (!m.diffuse_texname.empty()) {
+            uint32_t w = 0, h = 0;
+            auto texels = image_texels(assetsFolder + m.diffuse_texname, w, h);
+            t.type = PT_TEX_IMAGE;
+            t.width = w;
+            t.height = h;
+            ti = g_context->rec.add_texture(nullptr, t, texels);
+          } else {
+            t.type = PT_TEX_CONSTANT;
+            t.rgb[0] = m.ambient[0]; t.rgb[1] = m.ambient[1]; t.rgb[2] = m.ambient[2];
+            ti = g_context->rec.add_texture(nullptr, t, nullptr);
+          }
           pt_material_desc d{PT_MAT_LAMBERTIAN, {ti, -1}, {0, 0, 0, 0}};
           mats.push_back(g_context->rec.add_material(nullptr, d));
         }

@@ -32,16 +32,17 @@ inline void setRayGenerationProgram(Context& g_context, Light_Sampler& lights) {
 typedef enum { GRADIENT, CONSTANT, IMG, HDR } Miss_Programs;
 
 // programs.hpp:35-63: image backgrounds from a file
+// (the texture is recorded without an identity key: the loaded image is
+// not shared with any texture object)
 inline void setMissProgram(Context& g_context, Miss_Programs id, std::string fileName, bool isSpherical = true) {
-  if (id == HDR) {
-    HDR_Texture img(fileName);
-    int t = img.assignTo(g_context)->texture;
-    g_context->rec.miss = pt_miss_desc{PT_MISS_HDR, {0, 0, 0}, {0, 0, 0}, t, isSpherical ? 1 : 0};
-  } else if (id == IMG) {
-    throw "Parameters invalid, LDR image backgrounds are not supported by the GPU path yet";
-  } else {
-    throw "Parameters invalid, miss program unknown or not yet implemented";
-  }
+  if (id != HDR && id != IMG) throw "Parameters invalid, miss program unknown or not yet implemented";
+  pt_texture_desc t{};
+  t.type = PT_TEX_IMAGE;
+  auto texels = id == HDR ? hdr_texels(fileName, t.width, t.height) : image_texels(fileName, t.width, t.height);
+  int ti = g_context->rec.add_texture(nullptr, t, texels);
+  // miss.cu:64-99 environmental_mapping (HDR) / :51-59 image_background (IMG)
+  g_context->rec.miss = pt_miss_desc{id == HDR ? PT_MISS_HDR : PT_MISS_IMAGE, {0, 0, 0}, {0, 0, 0}, ti,
+                                     isSpherical ? 1 : 0};
   g_context->invalidate();
 }
 

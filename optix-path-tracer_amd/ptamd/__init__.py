@@ -179,6 +179,8 @@ def _declare(lib):
         "pt_cache_train": (C.c_int32, [P, P, P, C.c_uint64, C.c_float, C.POINTER(C.c_float), P]),
         "pt_scene_build": (C.c_int32, [C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32,
                                        C.POINTER(C.c_void_p)]),
+        "pt_scene_build_obj": (C.c_int32, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_uint32, C.c_uint32,
+                                           C.POINTER(C.c_void_p)]),
         "pt_scene_get_desc": (C.POINTER(SceneDesc), [P]),
         "pt_scene_get_camera": (C.POINTER(Camera), [P]),
         "pt_scene_get_info": (C.POINTER(SceneInfo), [P]),
@@ -214,17 +216,31 @@ class PtError(RuntimeError):
 class Scene:
     """A synthetic scene built by the host scene API inside the library."""
 
-    def __init__(self, config: int, width: int = 0, height: int = 0, variant: int = 0):
+    def __init__(self, config: int, width: int = 0, height: int = 0, variant: int = 0, _handle=None):
         lib = load()
-        h = C.c_void_p()
-        st = lib.pt_scene_build(config, width, height, variant, C.byref(h))
-        if st != PT_OK:
-            raise PtError(f"pt_scene_build: {lib.pt_scene_last_error().decode()}")
+        h = _handle
+        if h is None:
+            h = C.c_void_p()
+            st = lib.pt_scene_build(config, width, height, variant, C.byref(h))
+            if st != PT_OK:
+                raise PtError(f"pt_scene_build: {lib.pt_scene_last_error().decode()}")
         self._lib = lib
         self.handle = h
         self.desc = lib.pt_scene_get_desc(h).contents
         self.camera = lib.pt_scene_get_camera(h).contents
         self.info = lib.pt_scene_get_info(h).contents
+
+    @classmethod
+    def from_obj(cls, obj_file, assets_folder=None, hdr_file=None, width=0, height=0):
+        """pt_scene_build_obj: an OBJ/MTL model under an .hdr environment or
+        the gradient sky."""
+        lib = load()
+        h = C.c_void_p()
+        enc = lambda x: None if x is None else str(x).encode()
+        st = lib.pt_scene_build_obj(enc(obj_file), enc(assets_folder), enc(hdr_file), width, height, C.byref(h))
+        if st != PT_OK:
+            raise PtError(f"pt_scene_build_obj: {lib.pt_scene_last_error().decode()}")
+        return cls(-1, _handle=h)
 
     def close(self):
         if self.handle:
