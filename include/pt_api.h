@@ -307,6 +307,13 @@ pt_status pt_sync(struct pt_context* ctx);
  * image; *frames = number of accumulated frames. */
 pt_status pt_read_accum(struct pt_context* ctx, double* rgb, uint32_t* frames);
 
+/* Save_PNG / Save_HDR (image_save.hpp:7-73) of the accumulation average:
+ * PT_IMAGE_PNG writes 8-bit RGB, 255.99 * clamp(sqrt(c), 0, 1) truncated;
+ * PT_IMAGE_HDR writes Radiance RGBE of c / (1 + c); rows top first. */
+#define PT_IMAGE_PNG 0
+#define PT_IMAGE_HDR 1
+pt_status pt_save_image(struct pt_context* ctx, const char* path, int32_t format);
+
 /* displayBuffer->map() (main.cpp:269): W*H*4 bytes, make_Color of the
  * accumulation (raygen.cu:136-146). */
 pt_status pt_read_display(struct pt_context* ctx, uint8_t* rgba);

@@ -21,6 +21,8 @@
 #include "../../include/pt_api.h"
 #include "pt_kernels.hpp"
 #include "pt_layout.hpp"
+#include "../host/png_write.hpp"
+#include "../host/rgbe.hpp"
 
 using namespace pt;
 
@@ -1165,6 +1167,38 @@ pt_status pt_read_accum(pt_context* c, double* rgb, uint32_t* frames) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (rgb) HIPCHK(c, hipMemcpy(rgb, c->accum, (size_t)c->W * c->H * 3 * sizeof(double), hipMemcpyDeviceToHost));
   if (frames) *frames = c->frames;
+  return PT_OK;
+}
+
+// image_save.hpp:7-73 (Save_PNG / Save_HDR) on the accumulation average:
+// PNG: 255.99 * clamp(sqrt(c), 0, 1) truncated to 8 bits; HDR: c / (1 + c)
+// as flat RGBE; rows top first.
+pt_status pt_save_image(pt_context* c, const char* path, int32_t format) {
+  if (!c || !path || (format != PT_IMAGE_PNG && format != PT_IMAGE_HDR)) return PT_ERR_INVALID_ARG;
+  if (!c->accum || c->frames == 0) return fail(c, PT_ERR_STATE, "nothing rendered");
+  const size_t n = (size_t)c->W * c->H;
+  std::vector<double> acc(n * 3);
+  uint32_t frames = 0;
+  pt_status ps = pt_read_accum(c, acc.data(), &frames);
+  if (ps != PT_OK) return ps;
+  const float fr = (float)frames;
+  bool ok;
+  if (format == PT_IMAGE_PNG) {
+    std::vector<unsigned char> rgb(n * 3);
+    for (size_t i = 0; i < n * 3; i++) {
+      const float v = std::sqrt((float)acc[i] / fr);
+      rgb[i] = (unsigned char)(int)(255.99f * std::fmin(std::fmax(v, 0.f), 1.f));
+    }
+    ok = write_png_rgb(path, (int)c->W, (int)c->H, rgb.data()) != 0;
+  } else {
+    std::vector<float> rgb(n * 3);
+    for (size_t i = 0; i < n * 3; i++) {
+      const float v = (float)acc[i] / fr;
+      rgb[i] = v / (1.f + v);
+    }
+    ok = rgbe_save(path, c->W, c->H, rgb.data());
+  }
+  if (!ok) return fail(c, PT_ERR_INVALID_ARG, std::string("cannot write ") + path);
   return PT_OK;
 }
 

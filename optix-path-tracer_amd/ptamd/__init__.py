@@ -24,6 +24,7 @@ PT_GEOM_SPHERES, PT_GEOM_RECTS, PT_GEOM_BOXES, PT_GEOM_MESH = 0, 1, 2, 3
 PT_MAT_LAMBERTIAN, PT_MAT_METAL, PT_MAT_DIELECTRIC, PT_MAT_DIFFUSE_LIGHT = 0, 1, 2, 3
 PT_MAT_ASHIKHMIN_SHIRLEY, PT_MAT_TORRANCE_SPARROW, PT_MAT_OREN_NAYAR = 4, 5, 6
 PT_TEX_CONSTANT, PT_TEX_IMAGE = 0, 1
+PT_IMAGE_PNG, PT_IMAGE_HDR = 0, 1
 PT_LIGHT_RECT, PT_LIGHT_SPHERE, PT_LIGHT_ENV = 0, 1, 2
 PT_MISS_GRADIENT, PT_MISS_CONSTANT, PT_MISS_IMAGE, PT_MISS_HDR = 0, 1, 2, 3
 PT_FLAG_MATERIAL_SORT = 0x1
@@ -159,6 +160,7 @@ def _declare(lib):
         "pt_render_batch": (C.c_int32, [C.POINTER(C.c_void_p), C.POINTER(RenderParams), C.POINTER(C.c_uint32),
                                         C.c_uint32, C.POINTER(RenderStats)]),
         "pt_sync": (C.c_int32, [P]),
+        "pt_save_image": (C.c_int32, [P, C.c_char_p, C.c_int32]),
         "pt_read_accum": (C.c_int32, [P, P, C.POINTER(C.c_uint32)]),
         "pt_read_display": (C.c_int32, [P, P]),
         "pt_read_paths": (C.c_int32, [P, P, P, P]),
@@ -312,6 +314,13 @@ class Context:
         fr = C.c_uint32()
         self._check(self.lib.pt_read_accum(self.h, ptr(a), C.byref(fr)), "pt_read_accum")
         return a, fr.value
+
+    def save_image(self, path, fmt=None):
+        """pt_save_image: PNG (sqrt gamma) or Radiance HDR (c / (1 + c)) of
+        the accumulation average; the format follows the suffix unless given."""
+        if fmt is None:
+            fmt = PT_IMAGE_HDR if str(path).lower().endswith(".hdr") else PT_IMAGE_PNG
+        self._check(self.lib.pt_save_image(self.h, str(path).encode(), fmt), "pt_save_image")
 
     def read_display(self, w: int, h: int):
         a = np.zeros((h, w, 4), np.uint8)
