@@ -66,6 +66,16 @@ typedef int32_t pt_status;
 /* Textures (host_includes/textures.hpp). */
 #define PT_TEX_CONSTANT 0  /* rgb                                           */
 #define PT_TEX_IMAGE 1     /* width x height float4 texels, bilinear, repeat */
+/* checkered_texture.cu: the `width` texture where sin(10x) sin(10y) sin(10z)
+ * of the world hit point is negative, else the `height` texture (neither
+ * may be a checker) */
+#define PT_TEX_CHECKER 2
+/* noise_texture.cu: rgb[0] = scale; texels = 1792 floats: 256 gradient
+ * vectors (x, y, z, 0), then perm_x, perm_y, perm_z (256 each, as floats) */
+#define PT_TEX_NOISE 3
+/* gradient_texture.cu: texels = 12 floats, colours A, B, C as (r, g, b, 0) */
+#define PT_TEX_GRADIENT 4
+#define PT_NOISE_FLOATS 1792
 
 /* Lights (host_includes/pdfs.hpp + Light_Sampler, host_common.hpp:65). */
 #define PT_LIGHT_RECT 0    /* axis, a0, a1, b0, b1, k                   */
@@ -146,7 +156,8 @@ typedef struct pt_texture_desc {
   int32_t type; /* PT_TEX_* */
   float rgb[3];
   uint32_t width, height;
-  const float* texels; /* width * height * 4 floats, row 0 = v in [0, 1/h) */
+  const float* texels; /* image: width * height * 4 floats, row 0 = v in
+                        * [0, 1/h); noise and gradient: see PT_TEX_* */
 } pt_texture_desc;
 
 typedef struct pt_light_desc {

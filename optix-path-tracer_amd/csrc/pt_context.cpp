@@ -356,7 +356,19 @@ pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
   for (uint32_t i = 0; i < d->num_textures; i++) {
     const pt_texture_desc& t = d->textures[i];
     if (t.type == PT_TEX_IMAGE && (t.width == 0 || t.height == 0 || !t.texels)) return bad("empty image texture");
-    if (t.type != PT_TEX_IMAGE && t.type != PT_TEX_CONSTANT) return bad("invalid texture type");
+    if (t.type < PT_TEX_CONSTANT || t.type > PT_TEX_GRADIENT) return bad("invalid texture type");
+    if ((t.type == PT_TEX_NOISE || t.type == PT_TEX_GRADIENT) && !t.texels) return bad("procedural texture without tables");
+    if (t.type == PT_TEX_NOISE) {
+      for (int k = 0; k < 3 * 256; k++) {
+        const float q = t.texels[1024 + k];
+        if (!(q >= 0.f && q <= 255.f) || q != (float)(int)q) return bad("noise permutation entry not in 0..255");
+      }
+    }
+    if (t.type == PT_TEX_CHECKER) {
+      if (t.width >= d->num_textures || t.height >= d->num_textures) return bad("checker texture index out of range");
+      if (d->textures[t.width].type == PT_TEX_CHECKER || d->textures[t.height].type == PT_TEX_CHECKER)
+        return bad("nested checker textures are not supported");
+    }
   }
   if (d->miss.type < 0 || d->miss.type > 3) return bad("invalid miss program");
   if ((d->miss.type == PT_MISS_IMAGE || d->miss.type == PT_MISS_HDR) && !tex_ok(d->miss.texture))
@@ -398,6 +410,8 @@ pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
   for (auto& t : c->textures) {
     std::vector<float> tx;
     if (t.type == PT_TEX_IMAGE) tx.assign(t.texels, t.texels + 4ull * t.width * t.height);
+    if (t.type == PT_TEX_NOISE) tx.assign(t.texels, t.texels + PT_NOISE_FLOATS);
+    if (t.type == PT_TEX_GRADIENT) tx.assign(t.texels, t.texels + 12);
     c->texels.push_back(std::move(tx));
     t.texels = nullptr;
   }

@@ -110,19 +110,76 @@ PTD v3 tex_image(const DevScene& sc, const DevTexture& t, float u, float v) {
   v3 bot = c01 * (1.f - fx) + c11 * fx;
   return top * (1.f - fy) + bot * fy;
 }
-PTD v3 tex_eval(const DevScene& sc, int ti, float u, float v) {
-  const DevTexture& t = sc.texs[ti];
+// noise_texture.cu:13-54 perlin_interp / noise over the texel-pool tables
+// (256 gradient vectors, then perm_x, perm_y, perm_z as floats)
+PTD float perlin_noise(const float4* tab, v3 p) {
+  float u = p.x - floorf(p.x), v = p.y - floorf(p.y), w = p.z - floorf(p.z);
+  int i = (int)floorf(p.x), j = (int)floorf(p.y), k = (int)floorf(p.z);
+  const float* perm = reinterpret_cast<const float*>(tab + 256);
+  float uu = u * u * (3 - 2 * u), vv = v * v * (3 - 2 * v), ww = w * w * (3 - 2 * w);
+  float accum = 0;
+  for (int di = 0; di < 2; di++)
+    for (int dj = 0; dj < 2; dj++)
+      for (int dk = 0; dk < 2; dk++) {
+        int h = (int)perm[(i + di) & 255] ^ (int)perm[256 + ((j + dj) & 255)] ^ (int)perm[512 + ((k + dk) & 255)];
+        float4 c = tab[h];
+        v3 wv = mk3(u - (float)di, v - (float)dj, w - (float)dk);
+        accum += ((float)di * uu + (float)(1 - di) * (1 - uu)) * ((float)dj * vv + (float)(1 - dj) * (1 - vv)) *
+                 ((float)dk * ww + (float)(1 - dk) * (1 - ww)) * dot(mk3(c), wv);
+      }
+  return accum;
+}
+// noise_texture.cu:56-68 turb (7 octaves)
+PTD float perlin_turb(const float4* tab, v3 p) {
+  float accum = 0, weight = 1.0f;
+  v3 tp = p;
+  for (int i = 0; i < 7; i++) {
+    accum += weight * perlin_noise(tab, tp);
+    weight *= 0.5f;
+    tp = tp * 2.f;
+  }
+  return fabsf(accum);
+}
+// every texture kind but the checker, at (u, v) and world point P
+PTD v3 tex_leaf(const DevScene& sc, const DevTexture& t, float u, float v, v3 P) {
   if (t.type == PT_TEX_CONSTANT) return mk3(t.r, t.g, t.b);
+  if (t.type == PT_TEX_NOISE) {
+    // noise_texture.cu:70-84: the axis switch falls through to the Z case
+    const float4* tab = sc.texels + t.offset;
+    float sv = psin(t.r * P.z + 5 * perlin_turb(tab, t.r * P));
+    return mk3(1.f) * 0.5f * (1 + sv);
+  }
+  if (t.type == PT_TEX_GRADIENT) {
+    // gradient_texture.cu:7-14
+    const float4* c = sc.texels + t.offset;
+    v3 ud = normalize(P);
+    float x = fabsf(ud.x), y = fabsf(ud.y), z = fabsf(ud.z);
+    return x * mk3(c[0]) + y * mk3(c[1]) + z * mk3(c[2]);
+  }
   return tex_image(sc, t, u, v);
 }
-PTD bool tex_needs_uv(const DevScene& sc, int ti) { return ti >= 0 && sc.texs[ti].type != PT_TEX_CONSTANT; }
+PTD v3 tex_eval(const DevScene& sc, int ti, float u, float v, v3 P = mk3(0.f)) {
+  const DevTexture& t = sc.texs[ti];
+  if (t.type == PT_TEX_CHECKER) {
+    // checkered_texture.cu:8-15 (odd in w, even in h)
+    float sines = psin(10 * P.x) * psin(10 * P.y) * psin(10 * P.z);
+    return tex_leaf(sc, sc.texs[sines < 0 ? t.w : t.h], u, v, P);
+  }
+  return tex_leaf(sc, t, u, v, P);
+}
+PTD bool tex_needs_uv(const DevScene& sc, int ti) {
+  if (ti < 0) return false;
+  const DevTexture& t = sc.texs[ti];
+  if (t.type == PT_TEX_CHECKER) return sc.texs[t.w].type == PT_TEX_IMAGE || sc.texs[t.h].type == PT_TEX_IMAGE;
+  return t.type == PT_TEX_IMAGE;
+}
 
 PTD Surface make_surface(const DevScene& sc, const HitRec& r) {
   const DevMaterial& m = sc.mats[r.mat];
   Surface f;
   f.type = m.type;
-  f.c0 = tex_eval(sc, m.tex0, r.u, r.v);
-  f.c1 = (m.type == PT_MAT_ASHIKHMIN_SHIRLEY) ? tex_eval(sc, m.tex1, r.u, r.v) : mk3(0.f);
+  f.c0 = tex_eval(sc, m.tex0, r.u, r.v, r.P);
+  f.c1 = (m.type == PT_MAT_ASHIKHMIN_SHIRLEY) ? tex_eval(sc, m.tex1, r.u, r.v, r.P) : mk3(0.f);
   f.p0 = m.p0;
   f.p1 = m.p1;
   return f;

@@ -155,6 +155,72 @@ inline ConfigDefaults CornellBox(App_State& app, SceneArena& A) {
   return ConfigDefaults{512, 512, 64, 16, true, 3};
 }
 
+// scenes.hpp:95-186: the random-spheres layout on a checkered ground, lit by
+// a rect light under a dark sky, with a Perlin-noise metal sphere.  The
+// scene RNG is seeded with `seed`; colours are drawn r, g, b in that order.
+// The reference's Earth sphere samples assets/other_textures/map.jpg, which
+// the reference tree does not ship: a Gradient_Texture stands in for it.
+inline ConfigDefaults MovingSpheres(App_State& app, SceneArena& A, uint32_t seed = 0) {
+  rnd_reseed(seed);
+  Light_Sampler lights;
+  Rectangle_PDF rect_pdf(3.f, 5.f, 1.f, 3.f, -0.5f, Z_AXIS);
+  lights.pdf.push_back(rect_pdf.createPDF(app.context));
+  lights.sample.push_back(rect_pdf.createSample(app.context));
+  lights.emissions.push_back(make_float3(4.f));
+  setRayGenerationProgram(app.context, lights);
+  setMissProgram(app.context, CONSTANT);
+  setExceptionProgram(app.context);
+  Group group = app.context->createGroup();
+  group->setAcceleration(app.context->createAcceleration("Trbvh"));
+  Hitable_List list;
+  Texture* groundTx = A.tex<Checker_Texture>(A.tex<Constant_Texture>(0.2f, 0.3f, 0.1f),
+                                             A.tex<Constant_Texture>(0.9f, 0.9f, 0.9f));
+  list.push(A.hit<Sphere>(make_float3(0.f, -1000.f, -1.f), 1000.f, A.mat<Lambertian>(groundTx)));
+  auto tint = [&]() {
+    const float r = 0.5f * (1.f + rnd());
+    const float g = 0.5f * (1.f + rnd());
+    const float b = 0.5f * (1.f + rnd());
+    return A.tex<Constant_Texture>(r, g, b);
+  };
+  for (int a = -11; a < 11; a++) {
+    for (int b = -11; b < 11; b++) {
+      const float choose_mat = rnd();
+      const float cx = (float)a + rnd();
+      const float cz = (float)b + rnd();
+      const float3 center = make_float3(cx, 0.2f, cz);
+      rnd();  // center2's offset: the reference draws it and renders static spheres
+      BRDF* m;
+      if (choose_mat < (1.f / 3)) {
+        m = A.mat<Lambertian>(tint());
+      } else if (choose_mat < (2.f / 3)) {
+        Texture* t = tint();
+        m = A.mat<Metal>(t, 0.5f * rnd());
+      } else {
+        Texture* t = tint();
+        m = A.mat<Dielectric>(t, t, 1.5f, 0.f);
+      }
+      list.push(A.hit<Sphere>(center, 0.2f, m));
+    }
+  }
+  Texture* earth = A.tex<Gradient_Texture>(make_float3(0.1f, 0.3f, 0.8f), make_float3(0.2f, 0.7f, 0.2f),
+                                           make_float3(0.9f, 0.9f, 0.95f));
+  list.push(A.hit<Sphere>(make_float3(-4.f, 1.f, 2.f), 1.f, A.mat<Lambertian>(earth)));
+  Texture* gtx1 = A.tex<Constant_Texture>(1.f);
+  const float gr = rnd();
+  const float gg = rnd();
+  const float gb = rnd();
+  list.push(A.hit<Sphere>(make_float3(4.f, 1.f, 1.f), 1.f,
+                          A.mat<Dielectric>(gtx1, A.tex<Constant_Texture>(gr, gg, gb), 1.5f, 0.f)));
+  list.push(A.hit<Sphere>(make_float3(0.f, 1.f, 1.5f), 1.f, A.mat<Metal>(A.tex<Noise_Texture>(4.f), 0.f)));
+  list.push(A.hit<AARect>(3.f, 5.f, 1.f, 3.f, -0.5f, false, Z_AXIS, A.mat<Diffuse_Light>(A.tex<Constant_Texture>(4.f))));
+  list.addElementsTo(group, app.context);
+  app.context["world"]->set(group);
+  Camera camera(make_float3(13.f, 2.f, 3.f), make_float3(0.f), make_float3(0.f, 1.f, 0.f), 20.f,
+                float(app.W) / float(app.H), 0.1f, 10.f, 0.f, 1.f);
+  camera.set(app.context);
+  return ConfigDefaults{400, 267, 16, 50, true, 11};
+}
+
 // ------------------------------------------------------------ meshes
 // Perlin's improved gradient noise (Perlin, "Improving Noise", SIGGRAPH
 // 2002) over a permutation shuffled by mt19937(seed), and its fBm sum.

@@ -1,10 +1,11 @@
 // textures.hpp -- host texture classes (host_includes/textures.hpp).
-// Constant_Texture and HDR_Texture (file or in-memory float data) are
-// supported by the GPU path; the procedural textures of the reference
-// (Checker, Noise, Gradient, Vector) throw until their device programs exist.
+// Constant, Checker, Noise, Image, HDR (file or in-memory float data) and
+// Gradient textures are supported by the GPU path; Vector_Texture (a
+// texture per mesh face) throws until its device program exists.
 #pragma once
 
 #include <fstream>
+#include <utility>
 
 #include "buffers.hpp"
 #include "host_common.hpp"
@@ -112,20 +113,81 @@ struct Image_Texture : public Texture {
   mutable std::shared_ptr<std::vector<float>> data;
 };
 
-// Texture kinds of the reference without a device program yet.
+// textures.hpp:39-56: the odd texture where sin(10x) sin(10y) sin(10z) of
+// the hit point is negative, else the even one.  The two sub-texture
+// indices travel in width (odd) and height (even); neither may itself be a
+// checker.
+struct Checker_Texture : public Texture {
+  Checker_Texture(const Texture* o, const Texture* e) : odd(o), even(e) {}
+  Program assignTo(Context& g_context) const override {
+    if (!odd || !even) throw "Checker_Texture needs two textures";
+    pt_texture_desc t{};
+    t.type = PT_TEX_CHECKER;
+    t.width = (uint32_t)odd->assignTo(g_context)->texture;
+    t.height = (uint32_t)even->assignTo(g_context)->texture;
+    return texture_program(g_context->rec.add_texture(this, t, nullptr));
+  }
+  const Texture* odd;
+  const Texture* even;
+};
+
+// textures.hpp:56-112: Perlin noise from 256 random unit vectors and three
+// permutations of 0..255, drawn from rnd() at every assignTo as the
+// reference does (a texture shared by two materials gets two tables).  The
+// three components of a vector are drawn x, y, z (the reference passes the
+// draws as function arguments, whose evaluation order C++ leaves open).
+// The axis is kept for the interface only: the sampled value always uses z
+// (noise_texture.cu).
+struct Noise_Texture : public Texture {
+  Noise_Texture(const float s, const AXIS ax = X_AXIS) : scale(s), ax(ax) {}
+  static void permute(float* p) {
+    for (int i = 0; i < 256; i++) p[i] = (float)i;
+    for (int i = 256 - 1; i > 0; i--) {
+      const int target = int(rnd() * (float)(i + 1));
+      std::swap(p[i], p[target]);
+    }
+  }
+  Program assignTo(Context& g_context) const override {
+    auto data = std::make_shared<std::vector<float>>(PT_NOISE_FLOATS, 0.f);
+    float* v = data->data();
+    for (int i = 0; i < 256; i++) {
+      const float x = -1.f + 2.f * rnd();
+      const float y = -1.f + 2.f * rnd();
+      const float z = -1.f + 2.f * rnd();
+      const float l = std::sqrt(x * x + y * y + z * z);
+      v[4 * i] = x / l;
+      v[4 * i + 1] = y / l;
+      v[4 * i + 2] = z / l;
+    }
+    for (int a = 0; a < 3; a++) permute(v + 1024 + 256 * a);
+    pt_texture_desc t{};
+    t.type = PT_TEX_NOISE;
+    t.rgb[0] = scale;
+    return texture_program(g_context->rec.add_texture(nullptr, t, data));
+  }
+  const float scale;
+  const AXIS ax;
+};
+
+// textures.hpp:233-251: colours A, B, C weighted by |n| of the normalised
+// hit point.
+struct Gradient_Texture : public Texture {
+  Gradient_Texture(const float3& a, const float3& b, const float3& c) : colorA(a), colorB(b), colorC(c) {}
+  Program assignTo(Context& g_context) const override {
+    auto data = std::make_shared<std::vector<float>>(std::vector<float>{
+        colorA.x, colorA.y, colorA.z, 0.f, colorB.x, colorB.y, colorB.z, 0.f, colorC.x, colorC.y, colorC.z, 0.f});
+    pt_texture_desc t{};
+    t.type = PT_TEX_GRADIENT;
+    return texture_program(g_context->rec.add_texture(this, t, data));
+  }
+  const float3 colorA, colorB, colorC;
+};
+
+// Per-face texture lists of the reference without a device program yet.
 struct Unsupported_Texture : public Texture {
   explicit Unsupported_Texture(const char* what) : what(what) {}
   Program assignTo(Context&) const override { throw std::string(what) + " is not supported by the GPU path yet"; }
   const char* what;
-};
-struct Checker_Texture : Unsupported_Texture {
-  Checker_Texture(const Texture*, const Texture*) : Unsupported_Texture("Checker_Texture") {}
-};
-struct Noise_Texture : Unsupported_Texture {
-  Noise_Texture(const float, const AXIS = X_AXIS) : Unsupported_Texture("Noise_Texture") {}
-};
-struct Gradient_Texture : Unsupported_Texture {
-  Gradient_Texture(const float3&, const float3&, const float3&) : Unsupported_Texture("Gradient_Texture") {}
 };
 struct Vector_Texture : Unsupported_Texture {
   Vector_Texture(const std::vector<Texture*>&) : Unsupported_Texture("Vector_Texture") {}

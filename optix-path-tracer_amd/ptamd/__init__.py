@@ -23,7 +23,8 @@ PT_AXIS_X, PT_AXIS_Y, PT_AXIS_Z = 0, 1, 2
 PT_GEOM_SPHERES, PT_GEOM_RECTS, PT_GEOM_BOXES, PT_GEOM_MESH = 0, 1, 2, 3
 PT_MAT_LAMBERTIAN, PT_MAT_METAL, PT_MAT_DIELECTRIC, PT_MAT_DIFFUSE_LIGHT = 0, 1, 2, 3
 PT_MAT_ASHIKHMIN_SHIRLEY, PT_MAT_TORRANCE_SPARROW, PT_MAT_OREN_NAYAR = 4, 5, 6
-PT_TEX_CONSTANT, PT_TEX_IMAGE = 0, 1
+PT_TEX_CONSTANT, PT_TEX_IMAGE, PT_TEX_CHECKER, PT_TEX_NOISE, PT_TEX_GRADIENT = 0, 1, 2, 3, 4
+PT_NOISE_FLOATS = 1792
 PT_IMAGE_PNG, PT_IMAGE_HDR = 0, 1
 PT_LIGHT_RECT, PT_LIGHT_SPHERE, PT_LIGHT_ENV = 0, 1, 2
 PT_MISS_GRADIENT, PT_MISS_CONSTANT, PT_MISS_IMAGE, PT_MISS_HDR = 0, 1, 2, 3
@@ -34,7 +35,7 @@ PT_FLAG_COUNT_TRAVERSAL = 0x8
 PT_FLAG_RADIANCE_CACHE = 0x10
 PT_FLAG_GRAPH = 0x20
 PT_FLAG_TUNED = 0x40
-PT_SCENE_RANDOM_SPHERES, PT_SCENE_CORNELL, PT_SCENE_MESHES, PT_SCENE_BATCH = 0, 1, 2, 3
+PT_SCENE_RANDOM_SPHERES, PT_SCENE_CORNELL, PT_SCENE_MESHES, PT_SCENE_BATCH, PT_SCENE_MOVING_SPHERES = 0, 1, 2, 3, 4
 PT_CACHE_INPUTS, PT_CACHE_OUTPUTS, PT_CACHE_PARAMS = 96, 3, 18883
 
 

@@ -115,10 +115,62 @@ inline f3 tex_image(const Texture& t, float u, float v) {
   f3 bot = px(i0, j1) * (1.f - fx) + px(i1, j1) * fx;
   return top * (1.f - fy) + bot * fy;
 }
-inline f3 tex_eval(const Scene& s, int tex, float u, float v) {
+// noise_texture.cu:13-54 perlin_interp / noise: tables as pt_api.h gives
+// them (256 gradient vectors (x, y, z, 0), then perm_x, perm_y, perm_z)
+inline float perlin_noise(const std::vector<float>& tab, f3 p) {
+  float u = p.x - std::floor(p.x), v = p.y - std::floor(p.y), w = p.z - std::floor(p.z);
+  int i = (int)std::floor(p.x), j = (int)std::floor(p.y), k = (int)std::floor(p.z);
+  auto perm = [&](int axis, int idx) { return (int)tab[1024 + 256 * axis + idx]; };
+  float uu = u * u * (3 - 2 * u), vv = v * v * (3 - 2 * v), ww = w * w * (3 - 2 * w);
+  float accum = 0;
+  for (int di = 0; di < 2; di++)
+    for (int dj = 0; dj < 2; dj++)
+      for (int dk = 0; dk < 2; dk++) {
+        int h = perm(0, (i + di) & 255) ^ perm(1, (j + dj) & 255) ^ perm(2, (k + dk) & 255);
+        f3 c = mk(tab[4 * h], tab[4 * h + 1], tab[4 * h + 2]);
+        f3 wv = mk(u - (float)di, v - (float)dj, w - (float)dk);
+        accum += ((float)di * uu + (float)(1 - di) * (1 - uu)) * ((float)dj * vv + (float)(1 - dj) * (1 - vv)) *
+                 ((float)dk * ww + (float)(1 - dk) * (1 - ww)) * dot(c, wv);
+      }
+  return accum;
+}
+// noise_texture.cu:56-68 turb
+inline float perlin_turb(const std::vector<float>& tab, f3 p) {
+  float accum = 0, weight = 1.0f;
+  f3 tp = p;
+  for (int i = 0; i < 7; i++) {
+    accum += weight * perlin_noise(tab, tp);
+    weight *= 0.5f;
+    tp = tp * 2.f;
+  }
+  return std::fabs(accum);
+}
+inline f3 tex_leaf(const Texture& t, float u, float v, f3 P) {
+  switch (t.type) {
+    case PT_TEX_CONSTANT: return t.rgb;
+    case PT_TEX_NOISE: {
+      // noise_texture.cu:70-84: the axis switch falls through to the Z case
+      float sv = psin(t.rgb.x * P.z + 5 * perlin_turb(t.texels, t.rgb.x * P));
+      return mk(1.f) * 0.5f * (1 + sv);
+    }
+    case PT_TEX_GRADIENT: {  // gradient_texture.cu:7-14
+      const std::vector<float>& c = t.texels;
+      f3 ud = normalize(P);
+      float x = std::fabs(ud.x), y = std::fabs(ud.y), z = std::fabs(ud.z);
+      return x * mk(c[0], c[1], c[2]) + y * mk(c[4], c[5], c[6]) + z * mk(c[8], c[9], c[10]);
+    }
+    default: return tex_image(t, u, v);
+  }
+}
+// the texture at (u, v) and the world hit point P
+inline f3 tex_eval(const Scene& s, int tex, float u, float v, f3 P = mk(0.f)) {
   const Texture& t = s.texs[tex];
-  if (t.type == PT_TEX_CONSTANT) return t.rgb;
-  return tex_image(t, u, v);
+  if (t.type == PT_TEX_CHECKER) {
+    // checkered_texture.cu:8-15: odd texture in w, even in h
+    float sines = psin(10 * P.x) * psin(10 * P.y) * psin(10 * P.z);
+    return tex_leaf(s.texs[sines < 0 ? t.w : t.h], u, v, P);
+  }
+  return tex_leaf(t, u, v, P);
 }
 
 // ----------------------------------------------------------------- BRDFs
@@ -132,8 +184,8 @@ inline Surface make_surface(const Scene& s, const HitRec& r) {
   const Material& m = s.mats[r.mat];
   Surface f;
   f.type = m.type;
-  f.c0 = tex_eval(s, m.tex[0], r.u, r.v);
-  f.c1 = (m.type == PT_MAT_ASHIKHMIN_SHIRLEY) ? tex_eval(s, m.tex[1], r.u, r.v) : mk(0.f);
+  f.c0 = tex_eval(s, m.tex[0], r.u, r.v, r.P);
+  f.c1 = (m.type == PT_MAT_ASHIKHMIN_SHIRLEY) ? tex_eval(s, m.tex[1], r.u, r.v, r.P) : mk(0.f);
   f.p0 = m.p[0];
   f.p1 = m.p[1];
   return f;

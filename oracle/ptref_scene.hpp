@@ -531,6 +531,12 @@ struct Scene {
       if (t.type == PT_TEX_IMAGE) {
         tx.w = t.width; tx.h = t.height;
         tx.texels.assign(t.texels, t.texels + 4 * (size_t)t.width * t.height);
+      } else if (t.type == PT_TEX_CHECKER) {
+        tx.w = t.width; tx.h = t.height;  // odd, even
+      } else if (t.type == PT_TEX_NOISE) {
+        tx.texels.assign(t.texels, t.texels + PT_NOISE_FLOATS);
+      } else if (t.type == PT_TEX_GRADIENT) {
+        tx.texels.assign(t.texels, t.texels + 12);
       }
       texs.push_back(std::move(tx));
     }

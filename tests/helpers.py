@@ -24,7 +24,26 @@ class PyScene:
 
     def texture(self, rgb=None, image=None):
         self.textures.append((ptamd.PT_TEX_IMAGE if image is not None else ptamd.PT_TEX_CONSTANT,
-                              rgb or (0, 0, 0), image))
+                              rgb or (0, 0, 0), image, (0, 0), None))
+        return len(self.textures) - 1
+
+    def checker(self, odd, even):
+        """odd where sin(10x) sin(10y) sin(10z) < 0, else even (texture indices)"""
+        self.textures.append((ptamd.PT_TEX_CHECKER, (0, 0, 0), None, (odd, even), None))
+        return len(self.textures) - 1
+
+    def noise(self, scale, ranvec, perms):
+        """ranvec (256, 3) gradient vectors, perms (3, 256) permutations"""
+        data = np.zeros(ptamd.PT_NOISE_FLOATS, np.float32)
+        data[:1024].reshape(256, 4)[:, :3] = ranvec
+        data[1024:] = np.asarray(perms, np.float32).ravel()
+        self.textures.append((ptamd.PT_TEX_NOISE, (scale, 0, 0), None, (0, 0), data))
+        return len(self.textures) - 1
+
+    def gradient(self, a, b, c):
+        data = np.zeros(12, np.float32)
+        data.reshape(3, 4)[:, :3] = [a, b, c]
+        self.textures.append((ptamd.PT_TEX_GRADIENT, (0, 0, 0), None, (0, 0), data))
         return len(self.textures) - 1
 
     def material(self, mtype, tex0, tex1=-1, params=(0, 0, 0, 0)):
@@ -106,15 +125,18 @@ class PyScene:
         d.num_materials = len(mats)
         d.materials = arr(ptamd.MaterialDesc, mats)
         texs = []
-        for t, rgb, img in self.textures:
+        for t, rgb, img, wh, data in self.textures:
             x = ptamd.TextureDesc()
             x.type = t
             x.rgb = (C.c_float * 3)(*rgb)
+            x.width, x.height = wh
             if img is not None:
-                img = np.ascontiguousarray(img, np.float32)
-                keep.append(img)
+                data = img
                 x.height, x.width = img.shape[:2]
-                x.texels = img.ctypes.data_as(C.POINTER(C.c_float))
+            if data is not None:
+                data = np.ascontiguousarray(data, np.float32)
+                keep.append(data)
+                x.texels = data.ctypes.data_as(C.POINTER(C.c_float))
             texs.append(x)
         d.num_textures = len(texs)
         d.textures = arr(ptamd.TextureDesc, texs)
