@@ -356,33 +356,63 @@ __global__ __launch_bounds__(BLOCK) void k_extend(DevScene sc0, DevPaths P, int 
 
 // Exclusive scan of the (bucket, wave) counts in bucket-major order: the
 // first sorted position of each wave's paths of each bucket, and the size
-// of the sorted set.  One block of SCAN_THREADS threads, SCAN_PER entries
-// per thread.
+// of the sorted set.  The [8][MAX_WAVES] table is scanned whole, entries of
+// waves >= `waves` read as 0.  Each of the block's 16 waves owns a
+// contiguous chunk of 4096 entries, read coalesced 16 loads at a time: a
+// first pass sums the chunk, and after the chunk totals are exchanged a
+// second pass (re-reading from L2) scans 64 entries at a time across the
+// wave's lanes and writes the positions.
 constexpr int SCAN_THREADS = 1024;
+constexpr int SCAN_ITERS = 8 * MAX_WAVES / SCAN_THREADS;  // 64-entry rows per wave
+constexpr int SCAN_BATCH = 16;
+static_assert(SCAN_ITERS * SCAN_THREADS == 8 * MAX_WAVES, "scan covers the table");
+static_assert(SCAN_ITERS % SCAN_BATCH == 0, "whole batches");
+static_assert((MAX_WAVES & (MAX_WAVES - 1)) == 0, "wave index is the low bits of an entry");
 __global__ __launch_bounds__(SCAN_THREADS) void k_sort_scan(DevPaths P, uint32_t waves) {
-  __shared__ uint32_t part[SCAN_THREADS];
-  const uint32_t total = 8u * waves;
-  const uint32_t per = (total + SCAN_THREADS - 1) / SCAN_THREADS;
-  const uint32_t e0 = threadIdx.x * per, e1 = min(e0 + per, total);
-  auto at = [&](uint32_t e) -> size_t { return (size_t)(e / waves) * MAX_WAVES + e % waves; };
+  constexpr int NW = SCAN_THREADS / 64;
+  __shared__ uint32_t wsum[NW];
+  const uint32_t lane = __lane_id(), wv = threadIdx.x >> 6;
+  const uint32_t e0 = wv * (SCAN_ITERS * 64) + lane;
+  auto count = [&](uint32_t e) -> uint32_t { return (e & (MAX_WAVES - 1)) < waves ? P.bcount[e] : 0u; };
   uint32_t sum = 0;
-  for (uint32_t e = e0; e < e1; e++) sum += P.bcount[at(e)];
-  part[threadIdx.x] = sum;
+#pragma unroll 1
+  for (int b0 = 0; b0 < SCAN_ITERS; b0 += SCAN_BATCH) {
+    uint32_t v[SCAN_BATCH];
+#pragma unroll
+    for (int i = 0; i < SCAN_BATCH; i++) v[i] = count(e0 + 64u * (b0 + i));
+#pragma unroll
+    for (int i = 0; i < SCAN_BATCH; i++) sum += v[i];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+  if (lane == 0) wsum[wv] = sum;
   __syncthreads();
-  // Hillis-Steele inclusive scan of the per-thread sums
-  for (int off = 1; off < SCAN_THREADS; off <<= 1) {
-    uint32_t v = threadIdx.x >= (uint32_t)off ? part[threadIdx.x - off] : 0u;
-    __syncthreads();
-    part[threadIdx.x] += v;
-    __syncthreads();
+  uint32_t run = 0, all = 0;
+#pragma unroll
+  for (int k = 0; k < NW; k++) {
+    const uint32_t t = wsum[k];
+    run += (uint32_t)k < wv ? t : 0u;
+    all += t;
   }
-  uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
-  for (uint32_t e = e0; e < e1; e++) {
-    const size_t k = at(e);
-    P.bbase[k] = run;
-    run += P.bcount[k];
+#pragma unroll 1
+  for (int b0 = 0; b0 < SCAN_ITERS; b0 += SCAN_BATCH) {
+    uint32_t v[SCAN_BATCH];
+#pragma unroll
+    for (int i = 0; i < SCAN_BATCH; i++) v[i] = count(e0 + 64u * (b0 + i));
+#pragma unroll
+    for (int i = 0; i < SCAN_BATCH; i++) {
+      uint32_t s = v[i];
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(s, off);
+        if (lane >= (uint32_t)off) s += t;
+      }
+      const uint32_t e = e0 + 64u * (b0 + i);
+      if ((e & (MAX_WAVES - 1)) < waves) P.bbase[e] = run + s - v[i];
+      run += __shfl(s, 63);
+    }
   }
-  if (threadIdx.x == SCAN_THREADS - 1) P.counters[CTR_SORTED] = part[SCAN_THREADS - 1];
+  if (threadIdx.x == 0) P.counters[CTR_SORTED] = all;
 }
 // Move path state and hit into material-bucket order (state set 2, a dense
 // set): each wave writes its bucket-b paths from its scanned base on.
