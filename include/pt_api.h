@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define PT_API_VERSION 7
+#define PT_API_VERSION 8
 
 /* ------------------------------------------------------------------ status */
 typedef int32_t pt_status;
@@ -49,6 +49,7 @@ typedef int32_t pt_status;
 #define PT_GEOM_RECTS 1     /* programs/hitables/aarect.cu              */
 #define PT_GEOM_BOXES 2     /* programs/hitables/box.cu                 */
 #define PT_GEOM_MESH 3      /* programs/hitables/triangle.cu, mesh.hpp  */
+#define PT_GEOM_VOLUMES 4   /* programs/hitables/volume_{sphere,box}.cu */
 
 /* Materials (host_includes/materials.hpp). */
 #define PT_MAT_LAMBERTIAN 0        /* tex[0]=albedo                            */
@@ -61,7 +62,10 @@ typedef int32_t pt_status;
 #define PT_MAT_TORRANCE_SPARROW 5  /* tex[0]=Rs, param[0]=alpha_u,
                                       param[1]=alpha_v (already roughness^2)   */
 #define PT_MAT_OREN_NAYAR 6        /* tex[0]=albedo, param[0]=A, param[1]=B    */
-#define PT_MAT_COUNT 7
+#define PT_MAT_ISOTROPIC 7         /* tex[0]=albedo; phase function 1/(4 pi)   */
+#define PT_MAT_NORMAL 8            /* param[0] != 0: shading normal, else the
+                                      geometric one, as 0.5 N + 0.5           */
+#define PT_MAT_COUNT 9
 
 /* Textures (host_includes/textures.hpp). */
 #define PT_TEX_CONSTANT 0  /* rgb                                           */
@@ -120,6 +124,19 @@ typedef struct pt_box_desc {
   float pmax[3];
   int32_t material;
 } pt_box_desc;
+
+/* Constant-density medium (hitables.hpp:175-209 Volumetric_Sphere,
+ * :316-349 Volumetric_Box): a path entering the boundary scatters after a
+ * free flight of -ln(xi) / density, if that ends inside. */
+#define PT_VOLUME_SPHERE 0 /* a = center, b[0] = radius */
+#define PT_VOLUME_BOX 1    /* a = pmin, b = pmax         */
+typedef struct pt_volume_desc {
+  int32_t shape; /* PT_VOLUME_* */
+  float a[3];
+  float b[3];
+  float density;
+  int32_t material;
+} pt_volume_desc;
 
 typedef struct pt_mesh_desc {
   uint32_t num_vertices;
@@ -195,6 +212,8 @@ typedef struct pt_scene_desc {
   uint32_t num_lights;
   const pt_light_desc* lights;
   pt_miss_desc miss;
+  uint32_t num_volumes;
+  const pt_volume_desc* volumes;
 } pt_scene_desc;
 
 /* Camera variables exactly as Camera::set uploads them (camera.hpp:40-51). */

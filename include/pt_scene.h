@@ -21,6 +21,7 @@ extern "C" {
 #define PT_SCENE_MESHES 2         /* displaced meshes under an importance-sampled HDRI */
 #define PT_SCENE_BATCH 3          /* scene `variant` (0..63) of the 64-scene batch workload */
 #define PT_SCENE_MOVING_SPHERES 4 /* checkered ground, noise and gradient spheres, rect light */
+#define PT_SCENE_NEXT_WEEK 5      /* ground boxes, media, Perlin sphere, 1000-sphere group */
 
 typedef struct pt_scene_handle pt_scene_handle;
 
@@ -35,8 +36,8 @@ typedef struct pt_scene_info {
 } pt_scene_info;
 
 /* Build scene `config` for a width x height image (0 = the workload's size).
- * `variant`: the scene RNG seed of PT_SCENE_RANDOM_SPHERES and
- * PT_SCENE_MOVING_SPHERES; for
+ * `variant`: the scene RNG seed of PT_SCENE_RANDOM_SPHERES,
+ * PT_SCENE_MOVING_SPHERES and PT_SCENE_NEXT_WEEK; for
  * PT_SCENE_MESHES 0 builds the full workload and 1..6 the same layout with
  * octaspheres subdivided `variant` times and a 256x128 sky (small copies for
  * tests); for PT_SCENE_BATCH the scene index: 0..31 sphere scenes of

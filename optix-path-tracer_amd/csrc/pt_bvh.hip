@@ -60,6 +60,16 @@ __global__ void k_bounds_boxes(const float4* box, uint32_t n, float4* lo, float4
     put_box(lo, hi, i, a.x - 0.001f, a.y - 0.001f, a.z - 0.001f, b.x + 0.001f, b.y + 0.001f, b.z + 0.001f);
   }
 }
+// volume_sphere.cu:111-115 (center -+ radius), volume_box.cu:90-94 (-+1e-4)
+__global__ void k_bounds_volumes(const float4* vol, uint32_t n, float4* lo, float4* hi) {
+  for (uint32_t i = blockIdx.x * BB + threadIdx.x; i < n; i += gridDim.x * BB) {
+    float4 a = vol[2 * i], b = vol[2 * i + 1];
+    if (__float_as_uint(b.w) >> 31)
+      put_box(lo, hi, i, a.x - 0.0001f, a.y - 0.0001f, a.z - 0.0001f, b.x + 0.0001f, b.y + 0.0001f, b.z + 0.0001f);
+    else
+      put_box(lo, hi, i, a.x - b.x, a.y - b.x, a.z - b.x, a.x + b.x, a.y + b.x, a.z + b.x);
+  }
+}
 __global__ void k_bounds_tris(const float4* vtx, const uint4* idx, uint32_t n, float4* lo, float4* hi) {
   for (uint32_t i = blockIdx.x * BB + threadIdx.x; i < n; i += gridDim.x * BB) {
     uint4 t = idx[i];
@@ -575,6 +585,9 @@ void launch_bounds_rects(const float4* rect, uint32_t n, float4* lo, float4* hi,
 }
 void launch_bounds_boxes(const float4* box, uint32_t n, float4* lo, float4* hi, hipStream_t st) {
   hipLaunchKernelGGL(k_bounds_boxes, dim3(grid_for(n)), dim3(BB), 0, st, box, n, lo, hi);
+}
+void launch_bounds_volumes(const float4* vol, uint32_t n, float4* lo, float4* hi, hipStream_t st) {
+  hipLaunchKernelGGL(k_bounds_volumes, dim3(grid_for(n)), dim3(BB), 0, st, vol, n, lo, hi);
 }
 void launch_bounds_tris(const float4* vtx, const uint4* idx, uint32_t n, float4* lo, float4* hi, hipStream_t st) {
   hipLaunchKernelGGL(k_bounds_tris, dim3(grid_for(n)), dim3(BB), 0, st, vtx, idx, n, lo, hi);

@@ -54,6 +54,7 @@ struct pt_context {
   std::vector<pt_sphere_desc> spheres;
   std::vector<pt_rect_desc> rects;
   std::vector<pt_box_desc> boxes;
+  std::vector<pt_volume_desc> volumes;
   struct MeshHost {
     std::vector<float> v, n, uv;
     std::vector<uint32_t> idx;
@@ -307,7 +308,7 @@ pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
   auto bad = [&](const std::string& m) { return fail(c, PT_ERR_INVALID_ARG, m); };
   if ((d->num_spheres && !d->spheres) || (d->num_rects && !d->rects) || (d->num_boxes && !d->boxes) ||
       (d->num_meshes && !d->meshes) || (d->num_instances && !d->instances) || (d->num_materials && !d->materials) ||
-      (d->num_textures && !d->textures) || (d->num_lights && !d->lights))
+      (d->num_textures && !d->textures) || (d->num_lights && !d->lights) || (d->num_volumes && !d->volumes))
     return bad("scene array pointer is NULL with a non-zero count");
   if (d->num_instances == 0) return bad("scene has no instances");
   auto mat_ok = [&](int32_t m) { return m >= 0 && (uint32_t)m < d->num_materials; };
@@ -320,6 +321,12 @@ pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
   }
   for (uint32_t i = 0; i < d->num_boxes; i++)
     if (!mat_ok(d->boxes[i].material)) return bad("box material index out of range");
+  for (uint32_t i = 0; i < d->num_volumes; i++) {
+    const pt_volume_desc& v = d->volumes[i];
+    if (!mat_ok(v.material)) return bad("volume material index out of range");
+    if (v.shape != PT_VOLUME_SPHERE && v.shape != PT_VOLUME_BOX) return bad("invalid volume shape");
+    if (!(v.density > 0.f) || !std::isfinite(v.density)) return bad("volume density must be positive and finite");
+  }
   for (uint32_t i = 0; i < d->num_meshes; i++) {
     const pt_mesh_desc& m = d->meshes[i];
     if (m.num_triangles == 0 || !m.vertices || !m.indices) return bad("mesh without vertices or triangles");
@@ -338,8 +345,9 @@ pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
                    : in.kind == PT_GEOM_RECTS ? d->num_rects
                    : in.kind == PT_GEOM_BOXES ? d->num_boxes
                    : in.kind == PT_GEOM_MESH ? d->num_meshes
-                                             : 0;
-    if (in.kind < 0 || in.kind > 3) return bad("invalid instance kind");
+                   : in.kind == PT_GEOM_VOLUMES ? d->num_volumes
+                                                : 0;
+    if (in.kind < 0 || in.kind > PT_GEOM_VOLUMES) return bad("invalid instance kind");
     if (in.kind == PT_GEOM_MESH) {
       if (in.first >= lim) return bad("instance mesh index out of range");
     } else if (in.count == 0 || (uint64_t)in.first + in.count > lim) {
@@ -349,7 +357,8 @@ pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
   for (uint32_t i = 0; i < d->num_materials; i++) {
     const pt_material_desc& m = d->materials[i];
     if (m.type < 0 || m.type >= PT_MAT_COUNT) return bad("invalid material type");
-    if (!tex_ok(m.tex[0])) return bad("material texture index out of range");
+    if (!tex_ok(m.tex[0]) && !(m.type == PT_MAT_NORMAL && m.tex[0] == -1))
+      return bad("material texture index out of range");
     if ((m.type == PT_MAT_ASHIKHMIN_SHIRLEY || m.type == PT_MAT_DIELECTRIC) && !tex_ok(m.tex[1]))
       return bad("material second texture index out of range");
   }
@@ -391,6 +400,7 @@ pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
   c->spheres.assign(d->spheres, d->spheres + d->num_spheres);
   c->rects.assign(d->rects, d->rects + d->num_rects);
   c->boxes.assign(d->boxes, d->boxes + d->num_boxes);
+  c->volumes.assign(d->volumes, d->volumes + d->num_volumes);
   c->meshes.clear();
   for (uint32_t i = 0; i < d->num_meshes; i++) {
     const pt_mesh_desc& m = d->meshes[i];
@@ -455,6 +465,15 @@ pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
     box_o[2 * i + 1] = make_float4(q.pmax[0], q.pmax[1], q.pmax[2], 0.f);
     box_mat[i] = q.material;
   }
+  std::vector<float4> vol_o(2 * c->volumes.size());
+  std::vector<int32_t> vol_mat(c->volumes.size());
+  for (size_t i = 0; i < c->volumes.size(); i++) {
+    auto& q = c->volumes[i];
+    const uint32_t box = q.shape == PT_VOLUME_BOX ? 1u : 0u;
+    vol_o[2 * i] = make_float4(q.a[0], q.a[1], q.a[2], q.density);
+    vol_o[2 * i + 1] = make_float4(q.b[0], q.b[1], q.b[2], bf((uint32_t)i | (box << 31)));
+    vol_mat[i] = q.material;
+  }
   std::vector<DevMesh> dmesh;
   std::vector<float4> vtx, nrm;
   std::vector<float2> uv;
@@ -507,6 +526,10 @@ pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
   HIPCHK(c, upload(c, L, const_cast<int32_t**>(&s.rect_mat), rect_mat.data(), rect_mat.size() * 4));
   HIPCHK(c, upload(c, L, const_cast<float4**>(&s.box_o), box_o.data(), box_o.size() * sizeof(float4)));
   HIPCHK(c, upload(c, L, const_cast<int32_t**>(&s.box_mat), box_mat.data(), box_mat.size() * 4));
+  HIPCHK(c, upload(c, L, const_cast<float4**>(&s.vol_o), vol_o.data(), vol_o.size() * sizeof(float4)));
+  HIPCHK(c, upload(c, L, const_cast<int32_t**>(&s.vol_mat), vol_mat.data(), vol_mat.size() * 4));
+  s.vol_instances = 0;
+  for (auto& in : c->instances) s.vol_instances += in.kind == PT_GEOM_VOLUMES ? 1u : 0u;
   HIPCHK(c, upload(c, L, const_cast<DevMesh**>(&s.meshes), dmesh.data(), dmesh.size() * sizeof(DevMesh)));
   HIPCHK(c, upload(c, L, const_cast<float4**>(&s.vtx), vtx.data(), vtx.size() * sizeof(float4)));
   HIPCHK(c, upload(c, L, const_cast<float4**>(&s.nrm), nrm.data(), nrm.size() * sizeof(float4)));
@@ -568,7 +591,7 @@ pt_status pt_build_accel(pt_context* c, pt_accel_stats* out) {
   std::vector<int> inst_blas(c->instances.size());
   c->blas.clear();
   uint64_t maxn = c->instances.size();
-  uint32_t pool_n[4] = {0, 0, 0, 0};
+  uint32_t pool_n[5] = {0, 0, 0, 0, 0};
   for (size_t i = 0; i < c->instances.size(); i++) {
     const auto& in = c->instances[i];
     uint32_t count = in.kind == PT_GEOM_MESH ? (uint32_t)(c->meshes[in.first].idx.size() / 3) : in.count;
@@ -621,13 +644,14 @@ pt_status pt_build_accel(pt_context* c, pt_accel_stats* out) {
     return hip_fail(c, e, "hipMalloc");
   }
   // sorted primitive pools
-  float4 *sph = nullptr, *rect = nullptr, *box = nullptr, *tri = nullptr;
+  float4 *sph = nullptr, *rect = nullptr, *box = nullptr, *tri = nullptr, *vol = nullptr;
   uint32_t* sph_idx = nullptr;
   if ((e = dalloc(A, (void**)&sph, pool_n[0] * sizeof(float4))) ||
       (e = dalloc(A, (void**)&sph_idx, pool_n[0] * 4)) ||
       (e = dalloc(A, (void**)&rect, 2ull * pool_n[1] * sizeof(float4))) ||
       (e = dalloc(A, (void**)&box, 2ull * pool_n[2] * sizeof(float4))) ||
-      (e = dalloc(A, (void**)&tri, 3ull * pool_n[3] * sizeof(float4)))) {
+      (e = dalloc(A, (void**)&tri, 3ull * pool_n[3] * sizeof(float4))) ||
+      (e = dalloc(A, (void**)&vol, 2ull * pool_n[4] * sizeof(float4)))) {
     dfree(S);
     return hip_fail(c, e, "hipMalloc (primitive pools)");
   }
@@ -640,6 +664,7 @@ pt_status pt_build_accel(pt_context* c, pt_accel_stats* out) {
       case PT_GEOM_SPHERES: launch_bounds_spheres(s.sph_o + bi.first, n, sb.lo, sb.hi, c->stream); break;
       case PT_GEOM_RECTS: launch_bounds_rects(s.rect_o + 2ull * bi.first, n, sb.lo, sb.hi, c->stream); break;
       case PT_GEOM_BOXES: launch_bounds_boxes(s.box_o + 2ull * bi.first, n, sb.lo, sb.hi, c->stream); break;
+      case PT_GEOM_VOLUMES: launch_bounds_volumes(s.vol_o + 2ull * bi.first, n, sb.lo, sb.hi, c->stream); break;
       default: {
         // device mesh table offsets are computed in set_scene in mesh order
         uint32_t vbase = 0, ibase = 0;
@@ -678,6 +703,10 @@ pt_status pt_build_accel(pt_context* c, pt_accel_stats* out) {
         break;
       case PT_GEOM_BOXES:
         launch_gather_records(s.box_o + 2ull * bi.first, ord, n, 2, box + 2ull * bi.prim_base, bi.first, 0,
+                              c->stream);
+        break;
+      case PT_GEOM_VOLUMES:
+        launch_gather_records(s.vol_o + 2ull * bi.first, ord, n, 2, vol + 2ull * bi.prim_base, bi.first, 0,
                               c->stream);
         break;
       default: {
@@ -784,8 +813,9 @@ pt_status pt_build_accel(pt_context* c, pt_accel_stats* out) {
   s.rect = rect;
   s.box = box;
   s.tri = tri;
+  s.vol = vol;
   c->accel_bytes = c->total_nodes * 128 + pool_n[0] * 20ull + pool_n[1] * 32ull + pool_n[2] * 32ull +
-                   pool_n[3] * 48ull + ni * 128ull;
+                   pool_n[3] * 48ull + pool_n[4] * 32ull + ni * 128ull;
   c->accel_ready = true;
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, e0, e1);
@@ -975,6 +1005,8 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
   if ((uint64_t)fpb * WH > SLOT_MASK) return fail(c, PT_ERR_INVALID_ARG, "frames_per_batch x W x H exceeds 2^30 - 1");
   const bool use_cache = (rp->flags & PT_FLAG_RADIANCE_CACHE) != 0;
   if (use_cache && !c->cache_ready) return fail(c, PT_ERR_STATE, "PT_FLAG_RADIANCE_CACHE needs pt_cache_init");
+  if (use_cache && (c->dsc.vol_instances != 0 || (c->mat_types >> PT_MAT_ISOTROPIC) != 0))
+    return fail(c, PT_ERR_UNSUPPORTED, "the radiance cache does not take volumes, isotropic or normal-shader materials");
   pt_status ps = ensure_paths(c, (uint64_t)fpb * WH);
   if (ps == PT_OK) ps = ensure_spill(c);
   if (ps != PT_OK) return ps;
@@ -1378,6 +1410,8 @@ pt_status pt_render_batch(pt_context* const* ctxs, const pt_render_params* param
     if ((uint64_t)rp->width * rp->height > SLOT_MASK)
       return fail(B, PT_ERR_INVALID_ARG, "image too large for one batch");
     if (rp->flags & PT_FLAG_RADIANCE_CACHE) return fail(B, PT_ERR_UNSUPPORTED, "no radiance cache in a batch");
+    if (c->dsc.vol_instances != 0 || (c->mat_types >> PT_MAT_ISOTROPIC) != 0)
+      return fail(B, PT_ERR_UNSUPPORTED, "volumes, isotropic and normal-shader materials render with pt_render only");
     union_types |= c->mat_types;
     max_depth = std::max(max_depth, rp->max_depth);
     env = env || ((rp->flags & PT_FLAG_ENV_MIS) && c->dsc.num_lights != c->dsc.num_lights_plain);

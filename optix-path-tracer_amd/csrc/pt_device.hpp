@@ -189,6 +189,71 @@ PTD float isect_box(v3 lo, v3 hi, v3 o, v3 d, float tmin) {
   return tn > tmin ? tn : tf;
 }
 
+// "lowbias32" integer hash (C. Wellons)
+PTD uint32_t mix32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7FEB352Du;
+  x ^= x >> 15;
+  x *= 0x846CA68Bu;
+  x ^= x >> 16;
+  return x;
+}
+// Free-flight sample of a volume.  The reference draws rnd(prd.seed) inside
+// the intersection program (volume_sphere.cu:84, volume_box.cu:63), so the
+// draw would depend on the traversal's visit order; here xi is a hash of
+// the ray's RNG state, the volume and its instance, and the path's own
+// stream is left as it is.
+PTD float vol_xi(uint32_t seed, uint32_t vol, uint32_t inst) {
+  const uint32_t h = mix32(seed ^ mix32(vol * 0x9E3779B9u ^ mix32(inst + 0x632BE5ABu)));
+  return (float)(h >> 8) * (1.0f / 16777216.0f);
+}
+constexpr float FLT_BIG = 3.40282347e+38f;
+// hit_boundary of volume_sphere.cu:21-55 and volume_box.cu:20-40: the first
+// boundary root in (lo, hi); A = (a, density), B = (b, index | box << 31)
+PTD bool vol_boundary(float4 A, float4 B, bool box, v3 o, v3 d, float lo, float hi, float& rec) {
+  if (box) {
+    v3 t0 = (mk3(A) - o) / d;
+    v3 t1 = (mk3(B) - o) / d;
+    float ta = max_component(min_vec(t0, t1));
+    float tb = min_component(max_vec(t0, t1));
+    if (ta > tb) return false;
+    if (ta < hi && ta > lo) { rec = ta; return true; }
+    if (tb < hi && tb > lo) { rec = tb; return true; }
+    return false;
+  }
+  v3 oc = o - mk3(A);
+  float a = dot(d, d);
+  float b = dot(oc, d);
+  float c = dot(oc, oc) - B.x * B.x;
+  float disc = b * b - a * c;
+  if (disc < 0.f) return false;
+  float temp = (-b - sqrtf(disc)) / a;
+  if (temp < hi && temp > lo) { rec = temp; return true; }
+  temp = (-b + sqrtf(disc)) / a;
+  if (temp < hi && temp > lo) { rec = temp; return true; }
+  return false;
+}
+// volume_sphere.cu:63-110, volume_box.cu:42-85: entry and exit of the
+// boundary, then a free flight of -ln(xi) / density; the scatter point, or
+// +inf when the flight leaves the medium first (the reference reports the
+// point even then; the boundary test of Shirley's constant medium is kept
+// here).  The exit is the boundary's own, where the reference clips it to
+// the current closest hit: the closest hit is the same either way.
+PTD float isect_volume(float4 A, float4 B, v3 o, v3 d, float tmin, float xi) {
+  const bool box = (__float_as_uint(B.w) >> 31) != 0;
+  float r1 = 0.f, r2 = 0.f;
+  if (!vol_boundary(A, B, box, o, d, -FLT_BIG, FLT_BIG, r1)) return FINF;
+  if (!vol_boundary(A, B, box, o, d, (float)((double)r1 + 0.0001), FLT_BIG, r2)) return FINF;
+  if (r1 < tmin) r1 = tmin;
+  if (r1 >= r2) return FINF;
+  if (r1 < 0.f) r1 = 0.f;
+  const float len = length(d);
+  const float inside = (r2 - r1) * len;
+  const float hd = -(1.f / A.w) * plogf(xi);
+  if (!(hd < inside)) return FINF;
+  return r1 + hd / len;
+}
+
 // Watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2(1), 2013).
 struct TriRay {
   int kx, ky, kz;
@@ -342,7 +407,8 @@ struct Surface {
 // Sample(): lambertian.cuh:7-19, oren_nayar.cuh:8-20, diffuse_light.cuh:7-13,
 // ashikhmin_shirley.cuh:11-82, torrance_sparrow.cuh:10-35
 // Material sets a kernel is compiled for (bit = 1 << PT_MAT_*).
-constexpr int MS_ALL = 0x7F;
+constexpr int MS_ALL = 0x7F;   // the surface materials
+constexpr int MS_FULL = 0x1FF; // and the isotropic medium and the normal shader
 constexpr int MS_DIFFUSE = (1 << PT_MAT_LAMBERTIAN) | (1 << PT_MAT_OREN_NAYAR) | (1 << PT_MAT_DIFFUSE_LIGHT);
 constexpr int MS_SPHERES = (1 << PT_MAT_LAMBERTIAN) | (1 << PT_MAT_METAL) | (1 << PT_MAT_DIELECTRIC) |
                            (1 << PT_MAT_DIFFUSE_LIGHT);
@@ -360,6 +426,7 @@ PTD v3 brdf_sample(const Surface& f, v3 Wo, v3 N, uint32_t& seed) {
     return onb_to_world(N, cosine_sample_hemisphere(u1, u2));
   }
   if (has_mat<MS>(PT_MAT_DIFFUSE_LIGHT) && f.type == PT_MAT_DIFFUSE_LIGHT) return random_on_unit_sphere(seed);
+  if (has_mat<MS>(PT_MAT_ISOTROPIC) && f.type == PT_MAT_ISOTROPIC) return random_on_unit_sphere(seed);  // isotropic.cuh:7-13
   if (!has_mat<MS>(PT_MAT_ASHIKHMIN_SHIRLEY) && !has_mat<MS>(PT_MAT_TORRANCE_SPARROW)) return mk3(0.f);
   float nu = f.p0, nv = f.p1;
   v3 Nn = normalize(N);
@@ -428,6 +495,13 @@ PTD v3 brdf_eval(const Surface& f, v3 Wo, v3 Wi, v3 N, float& pdf) {
     case PT_MAT_DIFFUSE_LIGHT:
       pdf = 1.f;
       return f.c0;
+    case PT_MAT_ISOTROPIC: {
+      // isotropic.cuh:15-32 returns pi/4 for both; the phase function and
+      // its pdf are 1/(4 pi)
+      const float k = 0.25f / PI_F;
+      pdf = k;
+      return f.c0 * k;
+    }
     case PT_MAT_ASHIKHMIN_SHIRLEY: {
       v3 Rd = f.c0, Rs = f.c1;
       float nu = f.p0, nv = f.p1;

@@ -134,6 +134,7 @@ uint32_t build_bvh4(BuildScratch& sb, uint32_t n, int leaf_max, uint4* nodes_out
 void launch_bounds_spheres(const float4* sph, uint32_t n, float4* lo, float4* hi, hipStream_t st);
 void launch_bounds_rects(const float4* rect, uint32_t n, float4* lo, float4* hi, hipStream_t st);
 void launch_bounds_boxes(const float4* box, uint32_t n, float4* lo, float4* hi, hipStream_t st);
+void launch_bounds_volumes(const float4* vol, uint32_t n, float4* lo, float4* hi, hipStream_t st);
 void launch_bounds_tris(const float4* vtx, const uint4* idx, uint32_t n, float4* lo, float4* hi, hipStream_t st);
 void launch_bounds_instances(const uint4* insts_unsorted, const float4* blas_root, uint32_t n, float4* lo,
                              float4* hi, hipStream_t st);

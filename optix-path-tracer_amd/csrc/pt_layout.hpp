@@ -98,6 +98,12 @@ struct DevScene {
   const int32_t* sph_mat;
   const int32_t* rect_mat;
   const int32_t* box_mat;
+  // constant-density media: 2 float4 per volume, (a, density) and
+  // (b, original index | box << 31); sorted pool and original order
+  const float4* vol;
+  const float4* vol_o;
+  const int32_t* vol_mat;
+  uint32_t vol_instances;  // instances of kind PT_GEOM_VOLUMES (0: none)
   const DevMesh* meshes;
   const float4* vtx;     // vertex pool (xyz)
   const float4* nrm;     // normal pool (xyz)

@@ -112,5 +112,7 @@ PTL float ppow(float x, float y) {
   return pexp_d((double)y * pln(x));
 }
 PTL float ppow5(float x) { return (x * x) * (x * x) * x; }
+// natural log in float through pln; -inf at 0
+PTL float plogf(float x) { return x > 0.0f ? (float)pln(x) : -__builtin_huge_valf(); }
 
 }  // namespace pt

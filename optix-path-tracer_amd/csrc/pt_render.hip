@@ -141,7 +141,8 @@ PTD Stack make_stack(uint32_t* s_ref, float* s_tn, uint2* spill, uint32_t* overf
 }
 
 // ------------------------------------------------------- query kernels
-template <bool COUNT>
+// VOL: volumes draw their free flight from the ray's index
+template <bool COUNT, bool VOL>
 __global__ __launch_bounds__(BLOCK) void k_trace_closest(DevScene sc, const pt_ray* __restrict__ rays, uint64_t n,
                                                          pt_hit* __restrict__ hits, uint32_t* __restrict__ counts,
                                                          uint2* spill, uint32_t* overflow) {
@@ -152,7 +153,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_closest(DevScene sc, const pt_r
     const float4* r4 = reinterpret_cast<const float4*>(rays + i);
     float4 a = r4[0], b = r4[1];
     TraceCounts c{0, 0, 0};
-    HitOut h = trace_closest<COUNT>(sc, mk3(a), mk3(b), a.w, b.w, st, c);
+    HitOut h = trace_closest<COUNT, VOL>(sc, mk3(a), mk3(b), a.w, b.w, st, c, (uint32_t)i);
     pt_hit out;
     out.t = h.inst >= 0 ? h.t : FINF;
     out.inst = h.inst;
@@ -167,6 +168,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_closest(DevScene sc, const pt_r
   }
 }
 
+template <bool VOL>
 __global__ __launch_bounds__(BLOCK) void k_trace_occluded(DevScene sc, const pt_ray* __restrict__ rays, uint64_t n,
                                                           uint8_t* __restrict__ occ, uint2* spill,
                                                           uint32_t* overflow) {
@@ -177,7 +179,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_occluded(DevScene sc, const pt_
     const float4* r4 = reinterpret_cast<const float4*>(rays + i);
     float4 a = r4[0], b = r4[1];
     TraceCounts c{0, 0, 0};
-    HitOut h = trace_closest<false>(sc, mk3(a), mk3(b), a.w, b.w, st, c);
+    HitOut h = trace_closest<false, VOL>(sc, mk3(a), mk3(b), a.w, b.w, st, c, (uint32_t)i);
     uint8_t o = 0;
     if (h.inst >= 0) o = sc.mats[hit_material(sc, h.pos, h.prim)].type == PT_MAT_DIFFUSE_LIGHT ? 0 : 1;
     occ[i] = o;
@@ -192,16 +194,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_occluded(DevScene sc, const pt_
 // batch's frames and every wave keeps about the image-average share of
 // surviving paths (with slot = position a wave would keep the same 64
 // pixels in every frame, and the waves on the light would idle).  A partial
-// last chunk keeps its place.  mix32 is the "lowbias32" integer hash
-// (C. Wellons).
-PTD uint32_t mix32(uint32_t x) {
-  x ^= x >> 16;
-  x *= 0x7FEB352Du;
-  x ^= x >> 15;
-  x *= 0x846CA68Bu;
-  x ^= x >> 16;
-  return x;
-}
+// last chunk keeps its place.
 PTD uint32_t slot_of(uint32_t p, uint32_t nslots) {
   const uint32_t Q = nslots >> 6, q = p >> 6;
   if (Q < 2 || q >= Q) return p;
@@ -297,12 +290,13 @@ __global__ __launch_bounds__(BLOCK) void k_raygen_batch(DevPaths P, const Render
 // material bucket of a hit: 0 = miss, 1 + material type otherwise
 PTD uint32_t material_key(const DevScene& sc, float4 h) {
   int pos = __float_as_int(h.y);
-  return pos < 0 ? 0u : 1u + (uint32_t)sc.mats[hit_material(sc, pos, __float_as_int(h.z))].type;
+  // the isotropic medium and the normal shader share the last bucket
+  return pos < 0 ? 0u : 1u + (uint32_t)min(sc.mats[hit_material(sc, pos, __float_as_int(h.z))].type, 6);
 }
 
 // Closest-hit query of every live path (raygen.cu:77).  With SORT the
 // material bucket of each hit is recorded and counted per wave and bucket.
-template <bool COUNT, bool SORT, bool MULTI>
+template <bool COUNT, bool SORT, bool MULTI, bool VOL>
 __global__ __launch_bounds__(BLOCK) void k_extend(DevScene sc0, DevPaths P, int in, float tmin, uint2* spill,
                                                   uint64_t* stats, const DevScene* __restrict__ scenes) {
   __shared__ uint32_t s_ref[STK_LDS * BLOCK];
@@ -319,7 +313,7 @@ __global__ __launch_bounds__(BLOCK) void k_extend(DevScene sc0, DevPaths P, int 
     if (j < n) {
       float4 o = S.o[i], d = S.d[i];
       const DevScene& sc = MULTI ? scenes[__float_as_uint(o.w) >> 24] : sc0;
-      HitOut h = trace_closest<COUNT>(sc, mk3(o), mk3(d), tmin, FINF, st, c);
+      HitOut h = trace_closest<COUNT, VOL>(sc, mk3(o), mk3(d), tmin, FINF, st, c, __float_as_uint(o.w));
       float4 hv = make_float4(h.t, __int_as_float(h.pos), __int_as_float(h.prim), __int_as_float(h.inst));
       P.hit[i] = hv;
       if (SORT) {
@@ -497,6 +491,13 @@ PTD bool scatter_and_roulette(const Surface& f, v3 Wo, v3 N, uint32_t& seed, uin
     dir = Wi;
     T = T * clamp3(att, 0.f, 1.f);
     spec = true;
+  } else if (has_mat<MS>(PT_MAT_ISOTROPIC) && f.type == PT_MAT_ISOTROPIC) {  // isotropic.cu:42-50
+    v3 Wi = brdf_sample<MS>(f, Wo, N, seed);
+    float pdf = 1.f;
+    v3 att = brdf_eval<MS>(f, Wo, Wi, N, pdf);
+    dir = Wi;
+    T = T * (att / pdf);
+    prev_pdf = pdf;
   } else if (has_mat<MS>(PT_MAT_TORRANCE_SPARROW)) {  // Torrance-Sparrow, torrance_sparrow.cu:39-59
     v3 Wi = brdf_sample<MS>(f, Wo, N, seed);
     float pdf = 1.f;
@@ -530,14 +531,16 @@ PTD bool scatter_and_roulette(const Surface& f, v3 Wo, v3 N, uint32_t& seed, uin
 // to state set `out`; finished paths write their slot's result.
 // FEAT: SH_COUNT counts the shadow rays' traversal, SH_CACHE records
 // radiance-cache training vertices and queries, SH_ENV samples environment
-// lights; each is compiled only into the variants that use it.
-constexpr int SH_COUNT = 1, SH_CACHE = 2, SH_ENV = 4, SH_MULTI = 8;
+// lights, SH_VOL traces shadow rays through volumes; each is compiled only
+// into the variants that use it.
+constexpr int SH_COUNT = 1, SH_CACHE = 2, SH_ENV = 4, SH_MULTI = 8, SH_VOL = 16;
 template <int MS, int FEAT>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_shade(
     DevScene sc0, DevPaths P, int in, int src, const float4* __restrict__ hits, int out, uint32_t depth,
     RenderConsts rc0, uint2* spill, uint64_t* stats, const DevScene* __restrict__ scenes,
     const RenderConsts* __restrict__ rcs) {
   constexpr bool MULTI = (FEAT & SH_MULTI) != 0;
+  constexpr bool VOL = (FEAT & SH_VOL) != 0;
   __shared__ uint32_t s_ref[STK_LDS * BLOCK];
   __shared__ float s_tn[STK_LDS * BLOCK];
   Stack st = make_stack(s_ref, s_tn, spill, P.counters + CTR_STACK_OVERFLOW);
@@ -604,8 +607,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
         HitRec r = hit_record(sc, pos, prim, hit.x, mk3(o4), d, need_uv);
         Surface f = make_surface(sc, r);
         bool is_light = has_mat<MS>(PT_MAT_DIFFUSE_LIGHT) && f.type == PT_MAT_DIFFUSE_LIGHT;
+        // the isotropic medium samples lights without the hemisphere test of
+        // light_sample.cuh:60 and :71 (its +X normal is not a surface's)
+        const bool iso = has_mat<MS>(PT_MAT_ISOTROPIC) && f.type == PT_MAT_ISOTROPIC;
         bool nee = ((has_mat<MS>(PT_MAT_LAMBERTIAN) || has_mat<MS>(PT_MAT_OREN_NAYAR)) &&
-                    (f.type == PT_MAT_LAMBERTIAN || f.type == PT_MAT_OREN_NAYAR)) || is_light;
+                    (f.type == PT_MAT_LAMBERTIAN || f.type == PT_MAT_OREN_NAYAR)) || is_light || iso;
         bool queried = false;
         if (cache_on) {
           // training paths record every vertex (radiance so far, throughput);
@@ -644,15 +650,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
               v3 Wi = env ? env_sample(sc, seed) : light_sample(Lt, r.P, seed);
               float lightPDF = env ? env_pdf(sc, Wi) : light_pdf(Lt, r.P, Wi);
               v3 emission = env ? env_radiance(sc, Wi) : light_emission(Lt);
-              if (dot(Wi, r.N) < 0.f) {
+              if (!iso && dot(Wi, r.N) < 0.f) {
                 sig = sig_mix(sig, 0x33333333u);
               } else {
                 shadow_rays++;
-                HitOut sh = trace_closest<COUNT>(sc, r.P, Wi, 1e-3f, FINF, st, tc);
+                HitOut sh = trace_closest<COUNT, VOL>(sc, r.P, Wi, 1e-3f, FINF, st, tc, seed);
                 bool occluded = false;
                 if (sh.inst >= 0)
                   occluded = env || !(sc.mats[hit_material(sc, sh.pos, sh.prim)].type == PT_MAT_DIFFUSE_LIGHT &&
-                                      dot(r.N, Wi) > 0.f);
+                                      (iso || dot(r.N, Wi) > 0.f));
                 sig = sig_mix(sig, occluded ? 0xA5A5A5A5u : 0x5A5A5A5Au);
                 if (!occluded) {
                   if (lightPDF != 0.f && !isNull(emission)) {
@@ -683,6 +689,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
         }
         if (queried) {
           // result = L now; the cache adds T * radiance after inference
+        } else if (has_mat<MS>(PT_MAT_NORMAL) && f.type == PT_MAT_NORMAL) {
+          // normal_shader.cu:22-34: the radiance becomes 0.5 N + 0.5 and the
+          // path is cancelled (color() returns it)
+          v3 nn = f.p0 != 0.f ? r.N : geometric_normal(sc, pos, prim, r.N);
+          v3 c = nn * 0.5f + mk3(0.5f);
+          res = make_float4(c.x, c.y, c.z, __uint_as_float(depth + 1));
         } else if (is_light) {
           // diffuse_light.cu:43-47 and raygen.cu:85-90
           if (dot(r.N, r.Wo) < 0.f) T = T * f.c0;
@@ -934,16 +946,24 @@ __global__ void k_fill_f64(double* p, uint64_t n, double v) {
 // --------------------------------------------------------- host launchers
 void launch_trace_closest(const DevScene& sc, const pt_ray* rays, uint64_t n, pt_hit* hits, uint32_t* counts,
                           uint2* spill, uint32_t* overflow, int grid, hipStream_t st) {
-  if (counts)
-    hipLaunchKernelGGL(k_trace_closest<true>, dim3(grid), dim3(BLOCK), 0, st, sc, rays, n, hits, counts, spill,
-                       overflow);
-  else
-    hipLaunchKernelGGL(k_trace_closest<false>, dim3(grid), dim3(BLOCK), 0, st, sc, rays, n, hits, counts, spill,
-                       overflow);
+#define PT_TRACE(C, V) \
+  hipLaunchKernelGGL((k_trace_closest<C, V>), dim3(grid), dim3(BLOCK), 0, st, sc, rays, n, hits, counts, spill, overflow)
+  const bool vol = sc.vol_instances != 0;
+  if (counts) {
+    if (vol) PT_TRACE(true, true);
+    else PT_TRACE(true, false);
+  } else {
+    if (vol) PT_TRACE(false, true);
+    else PT_TRACE(false, false);
+  }
+#undef PT_TRACE
 }
 void launch_trace_occluded(const DevScene& sc, const pt_ray* rays, uint64_t n, uint8_t* occ, uint2* spill,
                            uint32_t* overflow, int grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_trace_occluded, dim3(grid), dim3(BLOCK), 0, st, sc, rays, n, occ, spill, overflow);
+  if (sc.vol_instances != 0)
+    hipLaunchKernelGGL(k_trace_occluded<true>, dim3(grid), dim3(BLOCK), 0, st, sc, rays, n, occ, spill, overflow);
+  else
+    hipLaunchKernelGGL(k_trace_occluded<false>, dim3(grid), dim3(BLOCK), 0, st, sc, rays, n, occ, spill, overflow);
 }
 __global__ void k_set_frame(uint32_t* f, uint32_t v) { *f = v; }
 __global__ void k_next_frames(uint32_t* f, uint32_t step) { *f += step; }
@@ -965,8 +985,15 @@ void launch_raygen_batch(const DevPaths& P, const RenderConsts* rcs, const uint3
 void launch_extend(const DevScene& sc, const DevPaths& P, int in, float tmin, uint2* spill, uint64_t* stats,
                    bool count_traversal, bool sort, int grid, hipStream_t st, const DevScene* scenes) {
 #define PT_EXTEND(C, S, M) \
-  hipLaunchKernelGGL((k_extend<C, S, M>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, tmin, spill, stats, scenes)
-  if (scenes) {
+  hipLaunchKernelGGL((k_extend<C, S, M, false>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, tmin, spill, stats, scenes)
+#define PT_EXTEND_VOL(C, S) \
+  hipLaunchKernelGGL((k_extend<C, S, false, true>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, tmin, spill, stats, scenes)
+  if (!scenes && sc.vol_instances != 0) {  // volumes: single scenes only
+    if (count_traversal && sort) PT_EXTEND_VOL(true, true);
+    else if (count_traversal) PT_EXTEND_VOL(true, false);
+    else if (sort) PT_EXTEND_VOL(false, true);
+    else PT_EXTEND_VOL(false, false);
+  } else if (scenes) {
     if (count_traversal && sort) PT_EXTEND(true, true, true);
     else if (count_traversal) PT_EXTEND(true, false, true);
     else if (sort) PT_EXTEND(false, true, true);
@@ -978,6 +1005,7 @@ void launch_extend(const DevScene& sc, const DevPaths& P, int in, float tmin, ui
     else PT_EXTEND(false, false, false);
   }
 #undef PT_EXTEND
+#undef PT_EXTEND_VOL
 }
 void launch_sort(const DevPaths& P, int in, int grid, hipStream_t st) {
   hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(SCAN_THREADS), 0, st, P, (uint32_t)(grid * WAVES));
@@ -1007,6 +1035,15 @@ void launch_shade(const DevScene& sc, const DevPaths& P, int in, bool sorted, in
       if (count) PT_SHADE(MS_ALL, SH_MULTI | SH_COUNT);
       else PT_SHADE(MS_ALL, SH_MULTI);
     }
+  } else if (matset == MS_FULL || sc.vol_instances != 0) {
+    // media or the normal shader (single scenes, no radiance cache)
+    if (env) {
+      if (count) PT_SHADE(MS_FULL, SH_VOL | SH_ENV | SH_COUNT);
+      else PT_SHADE(MS_FULL, SH_VOL | SH_ENV);
+    } else {
+      if (count) PT_SHADE(MS_FULL, SH_VOL | SH_COUNT);
+      else PT_SHADE(MS_FULL, SH_VOL);
+    }
   } else if (cache) {  // every material; no traversal counting
     if (env) PT_SHADE(MS_ALL, SH_CACHE | SH_ENV);
     else PT_SHADE(MS_ALL, SH_CACHE);
@@ -1028,7 +1065,8 @@ void launch_shade(const DevScene& sc, const DevPaths& P, int in, bool sorted, in
 int shade_material_set(uint32_t types_mask) {
   if ((types_mask & ~(uint32_t)MS_DIFFUSE) == 0) return MS_DIFFUSE;
   if ((types_mask & ~(uint32_t)MS_SPHERES) == 0) return MS_SPHERES;
-  return MS_ALL;
+  if ((types_mask & ~(uint32_t)MS_ALL) == 0) return MS_ALL;
+  return MS_FULL;
 }
 void launch_cache_scan(const DevPaths& P, int grid, hipStream_t st) {
   hipLaunchKernelGGL(k_cache_scan, dim3(1), dim3(SCAN_THREADS), 0, st, P, (uint32_t)(grid * WAVES));

@@ -60,6 +60,11 @@ PTD HitRec hit_record(const DevScene& sc, int inst_pos, int prim, float t, v3 o,
     v3 neg = mk3(t == t0.x ? 1.f : 0.f, t == t0.y ? 1.f : 0.f, t == t0.z ? 1.f : 0.f);
     v3 pos = mk3(t == t1.x ? 1.f : 0.f, t == t1.y ? 1.f : 0.f, t == t1.z ? 1.f : 0.f);
     n = pos - neg;
+  } else if (kind == PT_GEOM_VOLUMES) {
+    // volume_sphere.cu:88-100: the scatter point, normal +X, no uv
+    r.mat = sc.vol_mat[prim];
+    hp = ro + t * rd;
+    n = mk3(1.f, 0.f, 0.f);
   } else {
     const DevMesh& m = sc.meshes[ir[7].z];
     uint4 ix = sc.idx[m.ibase + prim];
@@ -167,6 +172,20 @@ PTD v3 tex_eval(const DevScene& sc, int ti, float u, float v, v3 P = mk3(0.f)) {
   }
   return tex_leaf(sc, t, u, v, P);
 }
+// Geometric normal of a hit for normal_shader.cu:27-31: differs from the
+// hit record's (shading) normal only on meshes with vertex normals.
+PTD v3 geometric_normal(const DevScene& sc, int inst_pos, int prim, v3 n_shading) {
+  const uint4* ir = sc.insts + (size_t)inst_pos * 8;
+  if ((ir[3].z & 0xFFu) != PT_GEOM_MESH) return n_shading;
+  const DevMesh& m = sc.meshes[ir[7].z];
+  if (!m.has_n) return n_shading;
+  uint4 ix = sc.idx[m.ibase + prim];
+  v3 a = mk3(sc.vtx[m.vbase + ix.x]), b = mk3(sc.vtx[m.vbase + ix.y]), c = mk3(sc.vtx[m.vbase + ix.z]);
+  v3 n = cross(b - a, c - a);
+  if ((ir[3].z >> 8) != 0) n = xf_normal(as_f4(ir[0]), as_f4(ir[1]), as_f4(ir[2]), n);
+  return normalize(n);
+}
+
 PTD bool tex_needs_uv(const DevScene& sc, int ti) {
   if (ti < 0) return false;
   const DevTexture& t = sc.texs[ti];
@@ -178,7 +197,7 @@ PTD Surface make_surface(const DevScene& sc, const HitRec& r) {
   const DevMaterial& m = sc.mats[r.mat];
   Surface f;
   f.type = m.type;
-  f.c0 = tex_eval(sc, m.tex0, r.u, r.v, r.P);
+  f.c0 = m.tex0 >= 0 ? tex_eval(sc, m.tex0, r.u, r.v, r.P) : mk3(0.f);  // the normal shader has none
   f.c1 = (m.type == PT_MAT_ASHIKHMIN_SHIRLEY) ? tex_eval(sc, m.tex1, r.u, r.v, r.P) : mk3(0.f);
   f.p0 = m.p0;
   f.p1 = m.p1;
@@ -354,6 +373,7 @@ PTD int hit_material(const DevScene& sc, int inst_pos, int prim) {
   if (kind == PT_GEOM_SPHERES) return sc.sph_mat[prim];
   if (kind == PT_GEOM_RECTS) return sc.rect_mat[prim];
   if (kind == PT_GEOM_BOXES) return sc.box_mat[prim];
+  if (kind == PT_GEOM_VOLUMES) return sc.vol_mat[prim];
   const DevMesh& m = sc.meshes[ir[7].z];
   return m.fmat_base >= 0 ? sc.fmat[m.fmat_base + prim] : m.mat;
 }

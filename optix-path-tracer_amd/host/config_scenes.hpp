@@ -221,6 +221,70 @@ inline ConfigDefaults MovingSpheres(App_State& app, SceneArena& A, uint32_t seed
   return ConfigDefaults{400, 267, 16, 50, true, 11};
 }
 
+// scenes.hpp:263-375: 400 random-height ground boxes, a rect light, glass,
+// metal and brown spheres, a blue medium inside a glass sphere, a thin white
+// medium around everything, a Perlin sphere and 1000 small spheres in one
+// rotated, translated group (Hitable_List::addListTo).  The scene RNG is
+// seeded with `seed`; the Earth's map.jpg is not in the reference tree, so
+// a Gradient_Texture stands in for it.
+inline ConfigDefaults NextWeek(App_State& app, SceneArena& A, uint32_t seed = 0) {
+  rnd_reseed(seed);
+  Light_Sampler lights;
+  Rectangle_PDF rect_pdf(113.f, 443.f, 127.f, 432.f, 554.f, Y_AXIS);
+  lights.pdf.push_back(rect_pdf.createPDF(app.context));
+  lights.sample.push_back(rect_pdf.createSample(app.context));
+  lights.emissions.push_back(make_float3(7.f));
+  setRayGenerationProgram(app.context, lights);
+  setMissProgram(app.context, CONSTANT);
+  setExceptionProgram(app.context);
+  Group group = app.context->createGroup();
+  group->setAcceleration(app.context->createAcceleration("Trbvh"));
+  Hitable_List list;
+  BRDF* ground = A.mat<Lambertian>(A.tex<Constant_Texture>(0.48f, 0.83f, 0.53f));
+  for (int i = 0; i < 20; i++) {
+    for (int j = 0; j < 20; j++) {
+      const float w = 100.f;
+      const float x0 = -1000 + i * w, z0 = -1000 + j * w, y0 = 0.f;
+      const float x1 = x0 + w, y1 = 100 * (rnd() + 0.01f), z1 = z0 + w;
+      list.push(A.hit<Box>(make_float3(x0, y0, z0), make_float3(x1, y1, z1), ground));
+    }
+  }
+  list.push(A.hit<AARect>(113.f, 443.f, 127.f, 432.f, 554.f, true, Y_AXIS,
+                          A.mat<Diffuse_Light>(A.tex<Constant_Texture>(7.f))));
+  list.push(A.hit<Sphere>(make_float3(400.f, 400.f, 200.f), 50.f,
+                          A.mat<Lambertian>(A.tex<Constant_Texture>(0.7f, 0.3f, 0.1f))));
+  Texture* glassTx1 = A.tex<Constant_Texture>(1.f);
+  BRDF* glass = A.mat<Dielectric>(glassTx1, glassTx1, 1.5f, 0.f);
+  list.push(A.hit<Sphere>(make_float3(260.f, 150.f, 45.f), 50.f, glass));
+  list.push(A.hit<Sphere>(make_float3(0.f, 150.f, 145.f), 50.f,
+                          A.mat<Metal>(A.tex<Constant_Texture>(0.8f, 0.8f, 0.9f), 10.f)));
+  list.push(A.hit<Sphere>(make_float3(360.f, 150.f, 45.f), 70.f, glass));
+  list.push(A.hit<Volumetric_Sphere>(make_float3(360.f, 150.f, 45.f), 70.f, 0.2f,
+                                     A.mat<Isotropic>(A.tex<Constant_Texture>(0.2f, 0.4f, 0.9f))));
+  list.push(A.hit<Volumetric_Sphere>(make_float3(0.f), 5000.f, 0.0001f, A.mat<Isotropic>(glassTx1)));
+  Texture* earth = A.tex<Gradient_Texture>(make_float3(0.1f, 0.3f, 0.8f), make_float3(0.2f, 0.7f, 0.2f),
+                                           make_float3(0.9f, 0.9f, 0.95f));
+  list.push(A.hit<Sphere>(make_float3(400.f, 200.f, 400.f), 100.f, A.mat<Lambertian>(earth)));
+  list.push(A.hit<Sphere>(make_float3(220.f, 280.f, 300.f), 80.f, A.mat<Lambertian>(A.tex<Noise_Texture>(0.1f))));
+  Hitable_List spheres;
+  BRDF* whiteMt = A.mat<Lambertian>(A.tex<Constant_Texture>(0.73f));
+  for (int j = 0; j < 1000; j++) {
+    const float x = 165 * rnd();
+    const float y = 165 * rnd();
+    const float z = 165 * rnd();
+    spheres.push(A.hit<Sphere>(make_float3(x, y, z), 10.f, whiteMt));
+  }
+  spheres.translate(make_float3(-100.f, 270.f, 395.f));
+  spheres.rotate(15.f, Y_AXIS);
+  spheres.addListTo(group, app.context);
+  list.addElementsTo(group, app.context);
+  app.context["world"]->set(group);
+  Camera camera(make_float3(478.f, 278.f, -600.f), make_float3(278.f, 278.f, 0.f), make_float3(0.f, 1.f, 0.f), 40.f,
+                float(app.W) / float(app.H), 0.f, 10.f, 0.f, 1.f);
+  camera.set(app.context);
+  return ConfigDefaults{500, 500, 64, 50, true, 11};
+}
+
 // ------------------------------------------------------------ meshes
 // Perlin's improved gradient noise (Perlin, "Improving Noise", SIGGRAPH
 // 2002) over a permutation shuffled by mt19937(seed), and its fBm sum.

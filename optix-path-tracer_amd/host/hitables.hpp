@@ -1,6 +1,6 @@
 // hitables.hpp -- geometry builders of the host scene API
-// (host_includes/hitables.hpp:20-528): Sphere, AARect, Box and
-// Hitable_List with the reference's transform-order semantics
+// (host_includes/hitables.hpp:20-528): Sphere, AARect, Box, the
+// volumetric sphere and box, and Hitable_List with the reference's transform-order semantics
 // (Hitable::addTo reverses its list, Hitable_List::addElementsTo and
 // addListTo do not).
 #pragma once
@@ -97,22 +97,46 @@ class Box : public Hitable {
   const float3 p0, p1;
 };
 
-// Hitables of the reference without a device program yet
-// (moving_sphere.cu, volume_sphere.cu, volume_box.cu, cylinder.cu).
+// hitables.hpp:175-209: constant-density medium inside a sphere
+class Volumetric_Sphere : public Hitable {
+ public:
+  Volumetric_Sphere(const float3& c, const float r, const float d, BRDF* material)
+      : Hitable(material), center(c), radius(r), density(d) {}
+  GeometryInstance getGeometryInstance(Context& g_context) override {
+    ptfacade::Prim p;
+    p.kind = PT_GEOM_VOLUMES;
+    p.v = pt_volume_desc{PT_VOLUME_SPHERE, {center.x, center.y, center.z}, {radius, 0.f, 0.f}, density, -1};
+    return make_instance(g_context, p);
+  }
+
+ protected:
+  const float3 center;
+  const float radius;
+  const float density;
+};
+// hitables.hpp:316-349: constant-density medium inside a box
+class Volumetric_Box : public Hitable {
+ public:
+  Volumetric_Box(const float3& p0, const float3& p1, const float d, BRDF* material)
+      : Hitable(material), p0(p0), p1(p1), density(d) {}
+  GeometryInstance getGeometryInstance(Context& g_context) override {
+    ptfacade::Prim p;
+    p.kind = PT_GEOM_VOLUMES;
+    p.v = pt_volume_desc{PT_VOLUME_BOX, {p0.x, p0.y, p0.z}, {p1.x, p1.y, p1.z}, density, -1};
+    return make_instance(g_context, p);
+  }
+
+ protected:
+  const float3 p0, p1;
+  const float density;
+};
+
+// Hitables of the reference without a device program yet (moving_sphere.cu
+// is marked broken at hitables.hpp:130; cylinder.cu has coefficient bugs).
 class Moving_Sphere : public Hitable {
  public:
   Moving_Sphere(const float3&, const float3&, const float, const float, const float, BRDF* m) : Hitable(m) {}
   GeometryInstance getGeometryInstance(Context&) override { throw "Moving_Sphere is not supported by the GPU path yet"; }
-};
-class Volumetric_Sphere : public Hitable {
- public:
-  Volumetric_Sphere(const float3&, const float, const float, BRDF* m) : Hitable(m) {}
-  GeometryInstance getGeometryInstance(Context&) override { throw "Volumetric_Sphere is not supported by the GPU path yet"; }
-};
-class Volumetric_Box : public Hitable {
- public:
-  Volumetric_Box(const float3&, const float3&, const float, BRDF* m) : Hitable(m) {}
-  GeometryInstance getGeometryInstance(Context&) override { throw "Volumetric_Box is not supported by the GPU path yet"; }
 };
 class Cylinder : public Hitable {
  public:

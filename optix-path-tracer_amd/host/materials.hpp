@@ -117,13 +117,21 @@ struct Torrance_Sparrow : public BRDF {
   const float nu, nv;
 };
 
-// Materials of the reference without a device program yet (isotropic.cu,
-// normal_shader.cu).
+// materials.hpp:125-140: the phase function of a constant-density medium
 struct Isotropic : public BRDF {
-  Isotropic(const Texture*) {}
-  Material assignTo(Context&) const override { throw "Isotropic is not supported by the GPU path yet"; }
+  Isotropic(const Texture* t) : texture(t) {}
+  Material assignTo(Context& g_context) const override {
+    pt_material_desc d{PT_MAT_ISOTROPIC, {tex_index(texture, g_context), -1}, {0, 0, 0, 0}};
+    return record(g_context, d);
+  }
+  const Texture* texture;
 };
+// materials.hpp:143-159: 0.5 N + 0.5 of the shading or geometric normal
 struct Normal_Shader : public BRDF {
-  Normal_Shader(const bool = false) {}
-  Material assignTo(Context&) const override { throw "Normal_Shader is not supported by the GPU path yet"; }
+  Normal_Shader(const bool useShadingNormal = false) : useShadingNormal(useShadingNormal) {}
+  Material assignTo(Context& g_context) const override {
+    pt_material_desc d{PT_MAT_NORMAL, {-1, -1}, {useShadingNormal ? 1.f : 0.f, 0, 0, 0}};
+    return record(g_context, d);
+  }
+  const bool useShadingNormal;
 };

@@ -218,6 +218,7 @@ struct Prim {
   pt_sphere_desc s{};
   pt_rect_desc r{};
   pt_box_desc b{};
+  pt_volume_desc v{};
   int mesh = -1;
 };
 
@@ -321,6 +322,7 @@ struct FlatScene {
   std::vector<pt_sphere_desc> spheres;
   std::vector<pt_rect_desc> rects;
   std::vector<pt_box_desc> boxes;
+  std::vector<pt_volume_desc> volumes;
   std::vector<pt_mesh_desc> meshes;
   std::vector<pt_instance_desc> instances;
   std::vector<pt_material_desc> materials;
@@ -360,7 +362,7 @@ struct Recorder {
     return idx;
   }
 
-  // Flatten the world group.  Untransformed spheres, rects and boxes share
+  // Flatten the world group.  Untransformed spheres, rects, boxes and volumes share
   // one identity instance per kind; every transformed node and every mesh
   // gets its own instance.
   void flatten(FlatScene& out) const {
@@ -407,7 +409,8 @@ struct Recorder {
       out.instances.push_back(in);
     };
     // identity instances of the analytic kinds, then transformed groups
-    for (int kind = PT_GEOM_SPHERES; kind <= PT_GEOM_BOXES; kind++) {
+    static const int analytic[4] = {PT_GEOM_SPHERES, PT_GEOM_RECTS, PT_GEOM_BOXES, PT_GEOM_VOLUMES};
+    for (int kind : analytic) {
       uint32_t first = 0, count = 0;
       for (auto& it : items) {
         if (it.p.kind != kind || it.xf) continue;
@@ -418,7 +421,7 @@ struct Recorder {
       if (count) add_instance(kind, first, count, false, Mat4::identity());
     }
     for (int g = 0; g < next_group; g++) {
-      for (int kind = PT_GEOM_SPHERES; kind <= PT_GEOM_BOXES; kind++) {
+      for (int kind : analytic) {
         uint32_t first = 0, count = 0;
         Mat4 m = Mat4::identity();
         for (auto& it : items) {
@@ -459,6 +462,8 @@ struct Recorder {
     d.rects = out.rects.data();
     d.num_boxes = (uint32_t)out.boxes.size();
     d.boxes = out.boxes.data();
+    d.num_volumes = (uint32_t)out.volumes.size();
+    d.volumes = out.volumes.data();
     d.num_meshes = (uint32_t)out.meshes.size();
     d.meshes = out.meshes.data();
     d.num_instances = (uint32_t)out.instances.size();
@@ -485,6 +490,12 @@ struct Recorder {
       r.material = mat;
       out.rects.push_back(r);
       return (uint32_t)out.rects.size() - 1;
+    }
+    if (p.kind == PT_GEOM_VOLUMES) {
+      pt_volume_desc v = p.v;
+      v.material = mat;
+      out.volumes.push_back(v);
+      return (uint32_t)out.volumes.size() - 1;
     }
     pt_box_desc b = p.b;
     b.material = mat;

@@ -33,6 +33,7 @@ pt_status pt_scene_build(int32_t config, uint32_t width, uint32_t height, uint32
       case PT_SCENE_RANDOM_SPHERES: d = ConfigDefaults{400, 267, 16, 50, true, 11}; break;
       case PT_SCENE_CORNELL: d = ConfigDefaults{512, 512, 64, 16, true, 3}; break;
       case PT_SCENE_MOVING_SPHERES: d = ConfigDefaults{400, 267, 16, 50, true, 11}; break;
+      case PT_SCENE_NEXT_WEEK: d = ConfigDefaults{500, 500, 64, 50, true, 11}; break;
       case PT_SCENE_MESHES:
         d = ConfigDefaults{1920, 1080, 256, 12, true, 11};
         if (variant > 6) {
@@ -62,6 +63,7 @@ pt_status pt_scene_build(int32_t config, uint32_t width, uint32_t height, uint32
     else if (config == PT_SCENE_CORNELL) d = CornellBox(h->app, h->arena);
     else if (config == PT_SCENE_BATCH) d = BatchScene(h->app, h->arena, variant);
     else if (config == PT_SCENE_MOVING_SPHERES) d = MovingSpheres(h->app, h->arena, variant);
+    else if (config == PT_SCENE_NEXT_WEEK) d = NextWeek(h->app, h->arena, variant);
     else if (variant == 0) d = MeshScene(h->app, h->arena);
     else d = MeshScene(h->app, h->arena, (int)variant, 256, 128);
     h->app.context->rec.flatten(h->flat);

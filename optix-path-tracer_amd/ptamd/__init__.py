@@ -20,9 +20,11 @@ LIB_PATH = PKG_DIR / "libpt_hip.so"
 PT_OK = 0
 
 PT_AXIS_X, PT_AXIS_Y, PT_AXIS_Z = 0, 1, 2
-PT_GEOM_SPHERES, PT_GEOM_RECTS, PT_GEOM_BOXES, PT_GEOM_MESH = 0, 1, 2, 3
+PT_GEOM_SPHERES, PT_GEOM_RECTS, PT_GEOM_BOXES, PT_GEOM_MESH, PT_GEOM_VOLUMES = 0, 1, 2, 3, 4
+PT_VOLUME_SPHERE, PT_VOLUME_BOX = 0, 1
 PT_MAT_LAMBERTIAN, PT_MAT_METAL, PT_MAT_DIELECTRIC, PT_MAT_DIFFUSE_LIGHT = 0, 1, 2, 3
 PT_MAT_ASHIKHMIN_SHIRLEY, PT_MAT_TORRANCE_SPARROW, PT_MAT_OREN_NAYAR = 4, 5, 6
+PT_MAT_ISOTROPIC, PT_MAT_NORMAL = 7, 8
 PT_TEX_CONSTANT, PT_TEX_IMAGE, PT_TEX_CHECKER, PT_TEX_NOISE, PT_TEX_GRADIENT = 0, 1, 2, 3, 4
 PT_NOISE_FLOATS = 1792
 PT_IMAGE_PNG, PT_IMAGE_HDR = 0, 1
@@ -36,6 +38,7 @@ PT_FLAG_RADIANCE_CACHE = 0x10
 PT_FLAG_GRAPH = 0x20
 PT_FLAG_TUNED = 0x40
 PT_SCENE_RANDOM_SPHERES, PT_SCENE_CORNELL, PT_SCENE_MESHES, PT_SCENE_BATCH, PT_SCENE_MOVING_SPHERES = 0, 1, 2, 3, 4
+PT_SCENE_NEXT_WEEK = 5
 PT_CACHE_INPUTS, PT_CACHE_OUTPUTS, PT_CACHE_PARAMS = 96, 3, 18883
 
 
@@ -50,6 +53,11 @@ class RectDesc(C.Structure):
 
 class BoxDesc(C.Structure):
     _fields_ = [("pmin", C.c_float * 3), ("pmax", C.c_float * 3), ("material", C.c_int32)]
+
+
+class VolumeDesc(C.Structure):
+    _fields_ = [("shape", C.c_int32), ("a", C.c_float * 3), ("b", C.c_float * 3), ("density", C.c_float),
+                ("material", C.c_int32)]
 
 
 class MeshDesc(C.Structure):
@@ -94,7 +102,8 @@ class SceneDesc(C.Structure):
                 ("num_materials", C.c_uint32), ("materials", C.POINTER(MaterialDesc)),
                 ("num_textures", C.c_uint32), ("textures", C.POINTER(TextureDesc)),
                 ("num_lights", C.c_uint32), ("lights", C.POINTER(LightDesc)),
-                ("miss", MissDesc)]
+                ("miss", MissDesc),
+                ("num_volumes", C.c_uint32), ("volumes", C.POINTER(VolumeDesc))]
 
 
 class Camera(C.Structure):

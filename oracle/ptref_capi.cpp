@@ -99,7 +99,9 @@ void ptref_trace(const void* sp, const pt_ray* rays, uint64_t n, pt_hit* hits, u
         const pt_ray& r = rays[i];
         f3 o = mk(r.org[0], r.org[1], r.org[2]), d = mk(r.dir[0], r.dir[1], r.dir[2]);
         Counts c;
-        HitState h = mode == 1 ? s.brute(o, d, r.tmin, r.tmax) : s.closest(o, d, r.tmin, r.tmax, &c);
+        // volumes draw their free flight from the ray's index
+        HitState h = mode == 1 ? s.brute(o, d, r.tmin, r.tmax, (uint32_t)i)
+                               : s.closest(o, d, r.tmin, r.tmax, &c, (uint32_t)i);
         hits[i].t = h.found() ? h.t : INF;
         hits[i].inst = h.inst;
         hits[i].prim = h.prim;
@@ -124,7 +126,8 @@ void ptref_occluded(const void* sp, const pt_ray* rays, uint64_t n, uint8_t* occ
   const Scene& s = *static_cast<const Scene*>(sp);
   for (uint64_t i = 0; i < n; i++) {
     const pt_ray& r = rays[i];
-    HitState h = s.closest(mk(r.org[0], r.org[1], r.org[2]), mk(r.dir[0], r.dir[1], r.dir[2]), r.tmin, r.tmax);
+    HitState h = s.closest(mk(r.org[0], r.org[1], r.org[2]), mk(r.dir[0], r.dir[1], r.dir[2]), r.tmin, r.tmax,
+                           nullptr, (uint32_t)i);
     occ[i] = h.found() ? (s.mats[s.material_of(h.inst, h.prim)].type == PT_MAT_DIFFUSE_LIGHT ? 0 : 1) : 0;
   }
 }

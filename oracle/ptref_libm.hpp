@@ -121,5 +121,7 @@ inline float ppow(float x, float y) {
   return pexp_d((double)y * pln(x));
 }
 inline float ppow5(float x) { return (x * x) * (x * x) * x; }
+// natural log in float through pln; -inf at 0
+inline float plogf(float x) { return x > 0.0f ? (float)pln(x) : -std::numeric_limits<float>::infinity(); }
 
 }  // namespace ptref

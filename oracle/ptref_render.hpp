@@ -67,6 +67,11 @@ inline HitRec hit_record(const Scene& s, int inst, int prim, float t, f3 o, f3 d
       n = pos - neg;
       break;
     }
+    case PT_GEOM_VOLUMES: {  // volume_sphere.cu:88-100: normal +X, no uv
+      hp = ro + t * rd;
+      n = mk(1.f, 0.f, 0.f);
+      break;
+    }
     default: {
       const Mesh& m = s.meshes[I.first];
       uint32_t i0 = m.idx[3 * prim], i1 = m.idx[3 * prim + 1], i2 = m.idx[3 * prim + 2];
@@ -96,6 +101,19 @@ inline HitRec hit_record(const Scene& s, int inst, int prim, float t, f3 o, f3 d
     r.N = normalize(n);
   }
   return r;
+}
+
+// normal_shader.cu:27-31: the geometric normal differs from the hit
+// record's only on meshes with vertex normals
+inline f3 geometric_normal(const Scene& s, int inst, int prim, f3 n_shading) {
+  const Instance& I = s.insts[inst];
+  if (I.kind != PT_GEOM_MESH) return n_shading;
+  const Mesh& m = s.meshes[I.first];
+  if (m.n.empty()) return n_shading;
+  f3 a = m.v[m.idx[3 * prim]], b = m.v[m.idx[3 * prim + 1]], c = m.v[m.idx[3 * prim + 2]];
+  f3 n = cross(b - a, c - a);
+  if (I.xf) n = xform_normal(I.w2o, n);
+  return normalize(n);
 }
 
 // Bilinear lookup with repeat wrap, texel centres at (i + 0.5) / w
@@ -184,7 +202,7 @@ inline Surface make_surface(const Scene& s, const HitRec& r) {
   const Material& m = s.mats[r.mat];
   Surface f;
   f.type = m.type;
-  f.c0 = tex_eval(s, m.tex[0], r.u, r.v, r.P);
+  f.c0 = m.tex[0] >= 0 ? tex_eval(s, m.tex[0], r.u, r.v, r.P) : mk(0.f);  // the normal shader has none
   f.c1 = (m.type == PT_MAT_ASHIKHMIN_SHIRLEY) ? tex_eval(s, m.tex[1], r.u, r.v, r.P) : mk(0.f);
   f.p0 = m.p[0];
   f.p1 = m.p[1];
@@ -295,6 +313,7 @@ inline f3 brdf_sample(const Surface& f, f3 Wo, f3 N, uint32_t& seed) {
       return Onb(N).to_world(cosine_sample_hemisphere(u1, u2));
     }
     case PT_MAT_DIFFUSE_LIGHT:
+    case PT_MAT_ISOTROPIC:  // isotropic.cuh:7-13
       return random_on_unit_sphere(seed);
     case PT_MAT_ASHIKHMIN_SHIRLEY: {
       float nu = f.p0, nv = f.p1;
@@ -369,6 +388,13 @@ inline f3 brdf_eval(const Surface& f, f3 Wo, f3 Wi, f3 N, float& pdf) {
     case PT_MAT_DIFFUSE_LIGHT:
       pdf = 1.f;
       return f.c0;
+    case PT_MAT_ISOTROPIC: {
+      // isotropic.cuh:15-32 returns pi/4 for both; phase function and pdf
+      // are 1/(4 pi)
+      const float k = 0.25f / PI_F;
+      pdf = k;
+      return f.c0 * k;
+    }
     case PT_MAT_ASHIKHMIN_SHIRLEY: {
       f3 Rd = f.c0, Rs = f.c1;
       float nu = f.p0, nv = f.p1;
@@ -580,13 +606,16 @@ inline f3 direct_light(const Scene& s, const std::vector<Light>& lights, const S
   f3 Wi = env ? env_sample(s, seed) : light_sample(L, P, seed);
   float lightPDF = env ? env_pdf(s, Wi) : light_pdf(L, P, Wi);
   f3 emission = env ? env_radiance(s, Wi) : L.e;
-  if (dot(Wi, N) < 0.f) { sig = mix(sig, 0x33333333u); return mk(0.f); }
+  // the isotropic medium's +X normal is not a surface's: no hemisphere tests
+  // (light_sample.cuh:60 and the shadow any-hit's :71)
+  const bool iso = f.type == PT_MAT_ISOTROPIC;
+  if (!iso && dot(Wi, N) < 0.f) { sig = mix(sig, 0x33333333u); return mk(0.f); }
   pc.shadow++;
-  HitState h = s.closest(P, Wi, 1e-3f, INF);
+  HitState h = s.closest(P, Wi, 1e-3f, INF, nullptr, seed);
   bool occluded = false;
   if (h.found()) {
     int m = s.material_of(h.inst, h.prim);
-    occluded = env || !(s.mats[m].type == PT_MAT_DIFFUSE_LIGHT && dot(N, Wi) > 0.f);
+    occluded = env || !(s.mats[m].type == PT_MAT_DIFFUSE_LIGHT && (iso || dot(N, Wi) > 0.f));
   }
   sig = mix(sig, occluded ? 0xA5A5A5A5u : 0x5A5A5A5Au);
   if (occluded) return mk(0.f);
@@ -685,7 +714,7 @@ inline PathResult trace_path(const Scene& s, const pt_camera& cam, const Params&
   for (uint32_t depth = 0; depth < p.max_depth; depth++) {
     pc.ext++;
     res.bounces = depth + 1;
-    HitState h = s.closest(origin, dir, tmin, INF);
+    HitState h = s.closest(origin, dir, tmin, INF, nullptr, seed);
     res.sig = mix(res.sig, h.found() ? (((uint32_t)h.inst << 20) ^ (uint32_t)h.prim) : 0xFFFFFFFFu);
     if (!h.found()) {
       if (p.env_mis && s.miss.type == PT_MISS_HDR) {
@@ -727,6 +756,23 @@ inline PathResult trace_path(const Scene& s, const pt_camera& cam, const Params&
         T *= clamp3(att / pdf, 0.f, 1.f);
         prevPdf = pdf;
         break;
+      }
+      case PT_MAT_ISOTROPIC: {  // isotropic.cu:27-51
+        f3 dl = direct_light(s, lights, f, P, Wo, N, false, seed, res.sig, pc);
+        L += T * dl;
+        f3 Wi = brdf_sample(f, Wo, N, seed);
+        float pdf = 1.f;
+        f3 att = brdf_eval(f, Wo, Wi, N, pdf);
+        origin = P;
+        dir = Wi;
+        T *= att / pdf;
+        prevPdf = pdf;
+        break;
+      }
+      case PT_MAT_NORMAL: {  // normal_shader.cu:22-34: radiance = 0.5 N + 0.5, cancelled
+        f3 n = f.p0 != 0.f ? N : geometric_normal(s, h.inst, h.prim, N);
+        res.L = n * 0.5f + mk(0.5f);
+        return res;
       }
       case PT_MAT_METAL: {  // metal.cu:38-56
         f3 refl = reflect(-Wo, N);

@@ -68,6 +68,9 @@ inline f3 xform_normal(const float* w2o, f3 n) {
 struct Sphere { f3 c; float r; int mat; };
 struct Rect { int axis, flip; float a0, a1, b0, b1, k; int mat; };
 struct Box { f3 lo, hi; int mat; };
+// hitables.hpp:175-209 / :316-349: shape PT_VOLUME_SPHERE (a = center,
+// b.x = radius) or PT_VOLUME_BOX (a = lo, b = hi)
+struct Volume { int shape; f3 a, b; float density; int mat; };
 struct Mesh {
   std::vector<f3> v, n;
   std::vector<f2> uv;
@@ -97,6 +100,18 @@ inline Aabb bounds(const Box& x) {
   b.lo = x.lo - mk(0.001f);
   b.hi = x.hi + mk(0.001f);
   return b;
+}
+// volume_sphere.cu:111-115 (no padding), volume_box.cu:90-94 (1e-4)
+inline Aabb bounds(const Volume& v) {
+  Aabb r;
+  if (v.shape == PT_VOLUME_BOX) {
+    r.lo = mk(v.a.x - 0.0001f, v.a.y - 0.0001f, v.a.z - 0.0001f);
+    r.hi = mk(v.b.x + 0.0001f, v.b.y + 0.0001f, v.b.z + 0.0001f);
+  } else {
+    r.lo = mk(v.a.x - v.b.x, v.a.y - v.b.x, v.a.z - v.b.x);
+    r.hi = mk(v.a.x + v.b.x, v.a.y + v.b.x, v.a.z + v.b.x);
+  }
+  return r;
 }
 inline Aabb bounds_tri(f3 a, f3 bb, f3 c) {
   Aabb r;
@@ -168,6 +183,64 @@ inline void isect_box(const Box& x, f3 o, f3 d, float tmin, HitState& h, int ins
   if (tn <= tf) {
     if (!h.offer(tn, tmin, inst, prim)) h.offer(tf, tmin, inst, prim);
   }
+}
+
+// "lowbias32" integer hash (C. Wellons)
+inline uint32_t mix32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7FEB352Du;
+  x ^= x >> 15;
+  x *= 0x846CA68Bu;
+  x ^= x >> 16;
+  return x;
+}
+// Free-flight sample of volume `vol` in instance `inst` for a ray whose path
+// RNG state is `seed`.  The reference draws rnd(prd.seed) inside the
+// intersection program (volume_sphere.cu:84), which depends on the visit
+// order; the hash leaves the path's stream untouched.
+inline float vol_xi(uint32_t seed, uint32_t vol, uint32_t inst) {
+  const uint32_t h = mix32(seed ^ mix32(vol * 0x9E3779B9u ^ mix32(inst + 0x632BE5ABu)));
+  return (float)(h >> 8) * (1.0f / 16777216.0f);
+}
+// hit_boundary of volume_sphere.cu:21-55 / volume_box.cu:20-40
+inline bool vol_boundary(const Volume& v, f3 o, f3 d, float lo, float hi, float& rec) {
+  if (v.shape == PT_VOLUME_BOX) {
+    f3 t0 = (v.a - o) / d;
+    f3 t1 = (v.b - o) / d;
+    float ta = max_component(min_vec(t0, t1));
+    float tb = min_component(max_vec(t0, t1));
+    if (ta > tb) return false;
+    if (ta < hi && ta > lo) { rec = ta; return true; }
+    if (tb < hi && tb > lo) { rec = tb; return true; }
+    return false;
+  }
+  f3 oc = o - v.a;
+  float a = dot(d, d);
+  float b = dot(oc, d);
+  float c = dot(oc, oc) - v.b.x * v.b.x;
+  float disc = b * b - a * c;
+  if (disc < 0.f) return false;
+  float temp = (-b - std::sqrt(disc)) / a;
+  if (temp < hi && temp > lo) { rec = temp; return true; }
+  temp = (-b + std::sqrt(disc)) / a;
+  if (temp < hi && temp > lo) { rec = temp; return true; }
+  return false;
+}
+// volume_sphere.cu:63-110 / volume_box.cu:42-85 with the boundary test of
+// Shirley's constant medium: the free flight -ln(xi) / density from the
+// entry (clamped to tmin) must end before the exit.
+inline void isect_volume(const Volume& v, f3 o, f3 d, float tmin, float xi, HitState& h, int inst, int prim) {
+  float r1 = 0.f, r2 = 0.f;
+  if (!vol_boundary(v, o, d, -FLT_MAX, FLT_MAX, r1)) return;
+  if (!vol_boundary(v, o, d, (float)((double)r1 + 0.0001), FLT_MAX, r2)) return;
+  if (r1 < tmin) r1 = tmin;
+  if (r1 >= r2) return;
+  if (r1 < 0.f) r1 = 0.f;
+  const float len = length(d);
+  const float inside = (r2 - r1) * len;
+  const float hd = -(1.f / v.density) * plogf(xi);
+  if (!(hd < inside)) return;
+  h.offer(r1 + hd / len, tmin, inst, prim);
 }
 
 // Watertight ray/triangle test (Woop, Benthin, Wald, "Watertight
@@ -482,6 +555,7 @@ struct Scene {
   std::vector<Sphere> spheres;
   std::vector<Rect> rects;
   std::vector<Box> boxes;
+  std::vector<Volume> volumes;
   std::vector<Mesh> meshes;
   std::vector<Instance> insts;
   std::vector<Blas> blas;
@@ -505,6 +579,10 @@ struct Scene {
     for (uint32_t i = 0; i < d.num_boxes; i++) {
       auto& b = d.boxes[i];
       boxes.push_back({mk(b.pmin[0], b.pmin[1], b.pmin[2]), mk(b.pmax[0], b.pmax[1], b.pmax[2]), b.material});
+    }
+    for (uint32_t i = 0; i < d.num_volumes; i++) {
+      auto& v = d.volumes[i];
+      volumes.push_back({v.shape, mk(v.a[0], v.a[1], v.a[2]), mk(v.b[0], v.b[1], v.b[2]), v.density, v.material});
     }
     for (uint32_t i = 0; i < d.num_meshes; i++) {
       auto& m = d.meshes[i];
@@ -586,6 +664,7 @@ struct Scene {
       case PT_GEOM_SPHERES: return widen(bounds(spheres.at(first + p)));
       case PT_GEOM_RECTS: return widen(bounds(rects.at(first + p)));
       case PT_GEOM_BOXES: return widen(bounds(boxes.at(first + p)));
+      case PT_GEOM_VOLUMES: return widen(bounds(volumes.at(first + p)));
       default: {
         const Mesh& m = meshes.at(first);
         return widen(bounds_tri(m.v.at(m.idx[3 * p]), m.v.at(m.idx[3 * p + 1]), m.v.at(m.idx[3 * p + 2])));
@@ -613,13 +692,17 @@ struct Scene {
     od = xform_vector(I.w2o, d);
   }
 
+  // `vseed`: the ray's RNG state, for volume free flights
   void isect_prim(const Instance& I, int inst, uint32_t local, f3 o, f3 d, const TriRay& tr, float tmin,
-                  HitState& h) const {
+                  HitState& h, uint32_t vseed) const {
     uint32_t g = I.first + local;
     switch (I.kind) {
       case PT_GEOM_SPHERES: isect_sphere(spheres[g], o, d, tmin, h, inst, (int)g); break;
       case PT_GEOM_RECTS: isect_rect(rects[g], o, d, tmin, h, inst, (int)g); break;
       case PT_GEOM_BOXES: isect_box(boxes[g], o, d, tmin, h, inst, (int)g); break;
+      case PT_GEOM_VOLUMES:
+        isect_volume(volumes[g], o, d, tmin, vol_xi(vseed, g, (uint32_t)inst), h, inst, (int)g);
+        break;
       default: {
         const Mesh& m = meshes[I.first];
         float t, b1, b2;
@@ -642,7 +725,7 @@ struct Scene {
   // Closest hit through the two-level BVH4.  Children are visited nearest
   // first (entry distance, then slot); popped entries farther than the
   // current hit are skipped.
-  HitState closest(f3 o, f3 d, float tmin, float tmax, Counts* cnt = nullptr) const {
+  HitState closest(f3 o, f3 d, float tmin, float tmax, Counts* cnt = nullptr, uint32_t vseed = 0) const {
     HitState h;
     h.t = tmax;
     struct Entry { uint32_t ref; float tn; };
@@ -680,7 +763,7 @@ struct Scene {
           const Instance& I = insts[cur];
           for (uint32_t k = 0; k < count; k++) {
             if (cnt) cnt->prims++;
-            isect_prim(I, cur, bvh.order[first + k], ro, rd, tr, tmin, h);
+            isect_prim(I, cur, bvh.order[first + k], ro, rd, tr, tmin, h, vseed);
           }
         }
         continue;
@@ -707,7 +790,7 @@ struct Scene {
   }
 
   // Brute force over every instance and primitive.
-  HitState brute(f3 o, f3 d, float tmin, float tmax) const {
+  HitState brute(f3 o, f3 d, float tmin, float tmax, uint32_t vseed = 0) const {
     HitState h;
     h.t = tmax;
     for (size_t i = 0; i < insts.size(); i++) {
@@ -716,7 +799,7 @@ struct Scene {
       to_object(I, o, d, ro, rd);
       TriRay tr{};
       if (I.kind == PT_GEOM_MESH) tr = tri_ray_setup(rd);
-      for (uint32_t p = 0; p < I.count; p++) isect_prim(I, (int)i, p, ro, rd, tr, tmin, h);
+      for (uint32_t p = 0; p < I.count; p++) isect_prim(I, (int)i, p, ro, rd, tr, tmin, h, vseed);
     }
     return h;
   }
@@ -727,6 +810,7 @@ struct Scene {
       case PT_GEOM_SPHERES: return spheres[prim].mat;
       case PT_GEOM_RECTS: return rects[prim].mat;
       case PT_GEOM_BOXES: return boxes[prim].mat;
+      case PT_GEOM_VOLUMES: return volumes[prim].mat;
       default: {
         const Mesh& m = meshes[I.first];
         return m.face_mat.empty() ? m.mat : m.face_mat[prim];

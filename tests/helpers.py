@@ -14,6 +14,7 @@ class PyScene:
 
     def __init__(self):
         self.spheres, self.rects, self.boxes = [], [], []
+        self.volumes = []      # (shape, a (3), b (3), density, mat)
         self.meshes = []       # (verts (n,3) f32, idx (m,3) u32, normals|None, uvs|None, mat)
         self.instances = []    # (kind, first, count, o2w 3x4 or None)
         self.materials = []    # (type, tex0, tex1, params)
@@ -73,6 +74,9 @@ class PyScene:
                                            for c, r, m in self.spheres])
         d.num_rects = len(self.rects)
         d.rects = arr(ptamd.RectDesc, [ptamd.RectDesc(*r) for r in self.rects])
+        d.num_volumes = len(self.volumes)
+        d.volumes = arr(ptamd.VolumeDesc, [ptamd.VolumeDesc(sh, (C.c_float * 3)(*a), (C.c_float * 3)(*b), den, m)
+                                           for sh, a, b, den, m in self.volumes])
         d.num_boxes = len(self.boxes)
         d.boxes = arr(ptamd.BoxDesc, [ptamd.BoxDesc((C.c_float * 3)(*lo), (C.c_float * 3)(*hi), m)
                                       for lo, hi, m in self.boxes])

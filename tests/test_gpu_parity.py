@@ -461,3 +461,91 @@ def test_moving_spheres_paths_match_oracle(ctx):
     _, (orgb, osig, _), _ = orc.render(sc.camera, params, 0, frames, keep_last=True)
     compare_paths(rgb, sig, orgb, osig)
     assert rgb.max() > 0
+
+
+def media_scene():
+    """Constant-density media: a sphere medium around a Lambertian ball, a
+    box medium under a rotation and a scale, a thin medium around the whole
+    scene; a normal-shaded mesh with vertex normals; a rect light."""
+    s = PyScene()
+    blue = s.material(ptamd.PT_MAT_ISOTROPIC, s.texture((0.2, 0.4, 0.9)))
+    white = s.material(ptamd.PT_MAT_ISOTROPIC, s.texture((1, 1, 1)))
+    s.volumes += [(ptamd.PT_VOLUME_SPHERE, (0, 1, 0), (1.2, 0, 0), 0.8, blue),
+                  (ptamd.PT_VOLUME_SPHERE, (0, 0, 0), (50, 0, 0), 0.01, white),
+                  (ptamd.PT_VOLUME_BOX, (-0.5, -0.5, -0.5), (0.5, 0.5, 0.5), 1.5, blue)]
+    s.instance(ptamd.PT_GEOM_VOLUMES, 0, 2)
+    c, sn = np.cos(0.5), np.sin(0.5)
+    M = np.array([[c * 1.5, 0, sn, 2.2], [0, 1.0, 0, 0.6], [-sn * 1.5, 0, c, 0.3]])
+    s.instance(ptamd.PT_GEOM_VOLUMES, 2, 1, M)
+    s.spheres += [((0, 1, 0), 0.5, s.lambert((0.8, 0.3, 0.2))), ((0, -100, 0), 100.0, s.lambert((0.5, 0.5, 0.5)))]
+    s.instance(ptamd.PT_GEOM_SPHERES, 0, 2)
+    v, f = icosphere(2, 0.6, (-2.0, 0.7, 0.5))
+    nrm = (v - np.float32([-2.0, 0.7, 0.5])) * np.float32(-1)  # inward vertex normals
+    s.meshes.append((v, f, nrm, None, s.material(ptamd.PT_MAT_NORMAL, -1, params=(0.0,))))
+    s.instance(ptamd.PT_GEOM_MESH, 0, 0)
+    light = s.material(ptamd.PT_MAT_DIFFUSE_LIGHT, s.texture((6, 6, 6)))
+    s.rects.append((ptamd.PT_AXIS_Y, 1, -1.5, 1.5, -1.5, 1.5, 4.0, light))
+    s.instance(ptamd.PT_GEOM_RECTS, 0, 1)
+    s.lights.append({"type": ptamd.PT_LIGHT_RECT, "axis": ptamd.PT_AXIS_Y, "rect": (-1.5, 1.5, -1.5, 1.5, 4.0),
+                     "emission": (6, 6, 6)})
+    s.miss = (ptamd.PT_MISS_CONSTANT, (0.05, 0.05, 0.08), (0, 0, 0), -1, 1)
+    return s
+
+
+def test_volume_queries_match_oracle(ctx):
+    """Closest-hit and occlusion queries through media (free flights drawn
+    from the ray index): the same hits and distances as the oracle's BVH
+    and brute force."""
+    s = media_scene()
+    desc = s.desc()
+    orc = ptref.OracleScene(desc)
+    rays = random_rays(20000, -3, 3, seed=12)
+    say("[gpu] closest-hit queries through media")
+    gpu_scene(ctx, desc)
+    g, gc = ctx.trace_closest(rays, counted=True)
+    o, oc = orc.trace(rays, counted=True)
+    check_hits(g, o)
+    assert np.array_equal(gc, oc)
+    check_hits(g, orc.trace(rays, brute=True))
+    vol_hits = (g["inst"] <= 1) & (g["inst"] >= 0)
+    assert vol_hits.sum() > 1000
+    assert np.array_equal(ctx.trace_occluded(rays), orc.occluded(rays))
+
+
+def test_media_paths_match_oracle(ctx):
+    """Isotropic scattering in sphere and box media (light sampling without
+    the hemisphere test, shadow rays through media), the normal shader on a
+    mesh with vertex normals: per-path results match the oracle."""
+    s = media_scene()
+    desc = s.desc()
+    W, H, frames = 64, 48, 2
+    cam = look_camera((0.5, 2.0, 6.0), W, H, vfov=50.0)
+    for flags in (ptamd.PT_FLAG_KEEP_PATHS, ptamd.PT_FLAG_KEEP_PATHS | ptamd.PT_FLAG_MATERIAL_SORT):
+        params = ptamd.RenderParams(W, H, 12, 1, 3, 2, flags, 0)
+        say(f"[gpu] render paths media flags={flags}")
+        gpu_scene(ctx, desc)
+        ctx.set_camera(cam)
+        st = ctx.render(params, 0, frames)
+        rgb, sig, _ = ctx.read_paths(W, H)
+        orc = ptref.OracleScene(desc)
+        _, (orgb, osig, _), ost = orc.render(cam, params, 0, frames, keep_last=True)
+        compare_paths(rgb, sig, orgb, osig)
+        assert st.shadow_rays == ost[2] and st.extension_rays == ost[1]
+        assert rgb.max() > 0
+
+
+def test_next_week_paths_match_oracle(ctx):
+    """scenes.hpp:263-375 (ground boxes, media, Perlin sphere, 1000-sphere
+    group under a rotation and translation): per-path results match."""
+    sc = ptamd.Scene(ptamd.PT_SCENE_NEXT_WEEK, 64, 64)
+    W, H, frames = 64, 64, 2
+    params = sc.render_params(flags=sc.info.flags | ptamd.PT_FLAG_KEEP_PATHS, frames_per_batch=2)
+    say("[gpu] render paths next week")
+    gpu_scene(ctx, sc.desc)
+    ctx.set_camera(sc.camera)
+    ctx.render(params, 0, frames)
+    rgb, sig, _ = ctx.read_paths(W, H)
+    orc = ptref.OracleScene(sc.desc)
+    _, (orgb, osig, _), _ = orc.render(sc.camera, params, 0, frames, keep_last=True)
+    compare_paths(rgb, sig, orgb, osig)
+    assert rgb.max() > 0
