@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define PT_API_VERSION 8
+#define PT_API_VERSION 9
 
 /* ------------------------------------------------------------------ status */
 typedef int32_t pt_status;
@@ -50,6 +50,8 @@ typedef int32_t pt_status;
 #define PT_GEOM_BOXES 2     /* programs/hitables/box.cu                 */
 #define PT_GEOM_MESH 3      /* programs/hitables/triangle.cu, mesh.hpp  */
 #define PT_GEOM_VOLUMES 4   /* programs/hitables/volume_{sphere,box}.cu */
+#define PT_GEOM_CYLINDERS 5 /* programs/hitables/cylinder.cu            */
+#define PT_GEOM_MOVING_SPHERES 6 /* programs/hitables/moving_sphere.cu  */
 
 /* Materials (host_includes/materials.hpp). */
 #define PT_MAT_LAMBERTIAN 0        /* tex[0]=albedo                            */
@@ -138,6 +140,27 @@ typedef struct pt_volume_desc {
   int32_t material;
 } pt_volume_desc;
 
+/* Open cylinder along +Y (hitables.hpp:411-454, cylinder.cu fixed to its
+ * intent): the lateral surface of radius `radius` from origin.y to
+ * origin.y + length, outward radial normal, no uv. */
+typedef struct pt_cylinder_desc {
+  float origin[3];
+  float length;
+  float radius;
+  int32_t material;
+} pt_cylinder_desc;
+
+/* Sphere whose center moves linearly from center0 at time0 to center1 at
+ * time1 (hitables.hpp:131-172, moving_sphere.cu); paths sample their time
+ * in [camera time0, time1]; ray queries see it at time 0. */
+typedef struct pt_moving_sphere_desc {
+  float center0[3];
+  float center1[3];
+  float radius;
+  float time0, time1;
+  int32_t material;
+} pt_moving_sphere_desc;
+
 typedef struct pt_mesh_desc {
   uint32_t num_vertices;
   const float* vertices; /* 3 * num_vertices                                */
@@ -214,6 +237,10 @@ typedef struct pt_scene_desc {
   pt_miss_desc miss;
   uint32_t num_volumes;
   const pt_volume_desc* volumes;
+  uint32_t num_cylinders;
+  const pt_cylinder_desc* cylinders;
+  uint32_t num_moving_spheres;
+  const pt_moving_sphere_desc* moving_spheres;
 } pt_scene_desc;
 
 /* Camera variables exactly as Camera::set uploads them (camera.hpp:40-51). */

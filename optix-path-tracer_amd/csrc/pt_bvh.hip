@@ -70,6 +70,25 @@ __global__ void k_bounds_volumes(const float4* vol, uint32_t n, float4* lo, floa
       put_box(lo, hi, i, a.x - b.x, a.y - b.x, a.z - b.x, a.x + b.x, a.y + b.x, a.z + b.x);
   }
 }
+// cylinder.cu:100-107: origin and origin + length (0, 1, 0), -+ radius
+__global__ void k_bounds_cylinders(const float4* cyl, uint32_t n, float4* lo, float4* hi) {
+  for (uint32_t i = blockIdx.x * BB + threadIdx.x; i < n; i += gridDim.x * BB) {
+    float4 a = cyl[2 * i], b = cyl[2 * i + 1];
+    float ox = a.x + a.w * 0.f, oy = a.y + a.w * 1.f, oz = a.z + a.w * 0.f;
+    put_box(lo, hi, i, fminf(a.x, ox) - b.x, fminf(a.y, oy) - b.x, fminf(a.z, oz) - b.x, fmaxf(a.x, ox) + b.x,
+            fmaxf(a.y, oy) + b.x, fmaxf(a.z, oz) + b.x);
+  }
+}
+// moving_sphere.cu:137-147: the union of the boxes at time0 and time1
+__global__ void k_bounds_moving(const float4* mov, uint32_t n, float4* lo, float4* hi) {
+  for (uint32_t i = blockIdx.x * BB + threadIdx.x; i < n; i += gridDim.x * BB) {
+    float4 a = mov[3 * i], b = mov[3 * i + 1], c = mov[3 * i + 2];
+    v3 p = moving_center(a, b, c, c.x), q = moving_center(a, b, c, c.y);
+    const float r = a.w;
+    put_box(lo, hi, i, fminf(p.x - r, q.x - r), fminf(p.y - r, q.y - r), fminf(p.z - r, q.z - r),
+            fmaxf(p.x + r, q.x + r), fmaxf(p.y + r, q.y + r), fmaxf(p.z + r, q.z + r));
+  }
+}
 __global__ void k_bounds_tris(const float4* vtx, const uint4* idx, uint32_t n, float4* lo, float4* hi) {
   for (uint32_t i = blockIdx.x * BB + threadIdx.x; i < n; i += gridDim.x * BB) {
     uint4 t = idx[i];
@@ -588,6 +607,12 @@ void launch_bounds_boxes(const float4* box, uint32_t n, float4* lo, float4* hi, 
 }
 void launch_bounds_volumes(const float4* vol, uint32_t n, float4* lo, float4* hi, hipStream_t st) {
   hipLaunchKernelGGL(k_bounds_volumes, dim3(grid_for(n)), dim3(BB), 0, st, vol, n, lo, hi);
+}
+void launch_bounds_cylinders(const float4* cyl, uint32_t n, float4* lo, float4* hi, hipStream_t st) {
+  hipLaunchKernelGGL(k_bounds_cylinders, dim3(grid_for(n)), dim3(BB), 0, st, cyl, n, lo, hi);
+}
+void launch_bounds_moving(const float4* mov, uint32_t n, float4* lo, float4* hi, hipStream_t st) {
+  hipLaunchKernelGGL(k_bounds_moving, dim3(grid_for(n)), dim3(BB), 0, st, mov, n, lo, hi);
 }
 void launch_bounds_tris(const float4* vtx, const uint4* idx, uint32_t n, float4* lo, float4* hi, hipStream_t st) {
   hipLaunchKernelGGL(k_bounds_tris, dim3(grid_for(n)), dim3(BB), 0, st, vtx, idx, n, lo, hi);

@@ -55,6 +55,8 @@ struct pt_context {
   std::vector<pt_rect_desc> rects;
   std::vector<pt_box_desc> boxes;
   std::vector<pt_volume_desc> volumes;
+  std::vector<pt_cylinder_desc> cylinders;
+  std::vector<pt_moving_sphere_desc> moving;
   struct MeshHost {
     std::vector<float> v, n, uv;
     std::vector<uint32_t> idx;
@@ -308,7 +310,8 @@ pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
   auto bad = [&](const std::string& m) { return fail(c, PT_ERR_INVALID_ARG, m); };
   if ((d->num_spheres && !d->spheres) || (d->num_rects && !d->rects) || (d->num_boxes && !d->boxes) ||
       (d->num_meshes && !d->meshes) || (d->num_instances && !d->instances) || (d->num_materials && !d->materials) ||
-      (d->num_textures && !d->textures) || (d->num_lights && !d->lights) || (d->num_volumes && !d->volumes))
+      (d->num_textures && !d->textures) || (d->num_lights && !d->lights) || (d->num_volumes && !d->volumes) ||
+      (d->num_cylinders && !d->cylinders) || (d->num_moving_spheres && !d->moving_spheres))
     return bad("scene array pointer is NULL with a non-zero count");
   if (d->num_instances == 0) return bad("scene has no instances");
   auto mat_ok = [&](int32_t m) { return m >= 0 && (uint32_t)m < d->num_materials; };
@@ -326,6 +329,12 @@ pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
     if (!mat_ok(v.material)) return bad("volume material index out of range");
     if (v.shape != PT_VOLUME_SPHERE && v.shape != PT_VOLUME_BOX) return bad("invalid volume shape");
     if (!(v.density > 0.f) || !std::isfinite(v.density)) return bad("volume density must be positive and finite");
+  }
+  for (uint32_t i = 0; i < d->num_cylinders; i++)
+    if (!mat_ok(d->cylinders[i].material)) return bad("cylinder material index out of range");
+  for (uint32_t i = 0; i < d->num_moving_spheres; i++) {
+    if (!mat_ok(d->moving_spheres[i].material)) return bad("moving sphere material index out of range");
+    if (!(d->moving_spheres[i].time1 != d->moving_spheres[i].time0)) return bad("moving sphere needs time1 != time0");
   }
   for (uint32_t i = 0; i < d->num_meshes; i++) {
     const pt_mesh_desc& m = d->meshes[i];
@@ -346,8 +355,10 @@ pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
                    : in.kind == PT_GEOM_BOXES ? d->num_boxes
                    : in.kind == PT_GEOM_MESH ? d->num_meshes
                    : in.kind == PT_GEOM_VOLUMES ? d->num_volumes
-                                                : 0;
-    if (in.kind < 0 || in.kind > PT_GEOM_VOLUMES) return bad("invalid instance kind");
+                   : in.kind == PT_GEOM_CYLINDERS ? d->num_cylinders
+                   : in.kind == PT_GEOM_MOVING_SPHERES ? d->num_moving_spheres
+                                                       : 0;
+    if (in.kind < 0 || in.kind > PT_GEOM_MOVING_SPHERES) return bad("invalid instance kind");
     if (in.kind == PT_GEOM_MESH) {
       if (in.first >= lim) return bad("instance mesh index out of range");
     } else if (in.count == 0 || (uint64_t)in.first + in.count > lim) {
@@ -401,6 +412,8 @@ pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
   c->rects.assign(d->rects, d->rects + d->num_rects);
   c->boxes.assign(d->boxes, d->boxes + d->num_boxes);
   c->volumes.assign(d->volumes, d->volumes + d->num_volumes);
+  c->cylinders.assign(d->cylinders, d->cylinders + d->num_cylinders);
+  c->moving.assign(d->moving_spheres, d->moving_spheres + d->num_moving_spheres);
   c->meshes.clear();
   for (uint32_t i = 0; i < d->num_meshes; i++) {
     const pt_mesh_desc& m = d->meshes[i];
@@ -474,6 +487,23 @@ pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
     vol_o[2 * i + 1] = make_float4(q.b[0], q.b[1], q.b[2], bf((uint32_t)i | (box << 31)));
     vol_mat[i] = q.material;
   }
+  std::vector<float4> cyl_o(2 * c->cylinders.size());
+  std::vector<int32_t> cyl_mat(c->cylinders.size());
+  for (size_t i = 0; i < c->cylinders.size(); i++) {
+    auto& q = c->cylinders[i];
+    cyl_o[2 * i] = make_float4(q.origin[0], q.origin[1], q.origin[2], q.length);
+    cyl_o[2 * i + 1] = make_float4(q.radius, 0.f, 0.f, bf((uint32_t)i));
+    cyl_mat[i] = q.material;
+  }
+  std::vector<float4> mov_o(3 * c->moving.size());
+  std::vector<int32_t> mov_mat(c->moving.size());
+  for (size_t i = 0; i < c->moving.size(); i++) {
+    auto& q = c->moving[i];
+    mov_o[3 * i] = make_float4(q.center0[0], q.center0[1], q.center0[2], q.radius);
+    mov_o[3 * i + 1] = make_float4(q.center1[0], q.center1[1], q.center1[2], bf((uint32_t)i));
+    mov_o[3 * i + 2] = make_float4(q.time0, q.time1, 0.f, 0.f);
+    mov_mat[i] = q.material;
+  }
   std::vector<DevMesh> dmesh;
   std::vector<float4> vtx, nrm;
   std::vector<float2> uv;
@@ -528,8 +558,16 @@ pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
   HIPCHK(c, upload(c, L, const_cast<int32_t**>(&s.box_mat), box_mat.data(), box_mat.size() * 4));
   HIPCHK(c, upload(c, L, const_cast<float4**>(&s.vol_o), vol_o.data(), vol_o.size() * sizeof(float4)));
   HIPCHK(c, upload(c, L, const_cast<int32_t**>(&s.vol_mat), vol_mat.data(), vol_mat.size() * 4));
-  s.vol_instances = 0;
-  for (auto& in : c->instances) s.vol_instances += in.kind == PT_GEOM_VOLUMES ? 1u : 0u;
+  HIPCHK(c, upload(c, L, const_cast<float4**>(&s.cyl_o), cyl_o.data(), cyl_o.size() * sizeof(float4)));
+  HIPCHK(c, upload(c, L, const_cast<int32_t**>(&s.cyl_mat), cyl_mat.data(), cyl_mat.size() * 4));
+  HIPCHK(c, upload(c, L, const_cast<float4**>(&s.mov_o), mov_o.data(), mov_o.size() * sizeof(float4)));
+  HIPCHK(c, upload(c, L, const_cast<int32_t**>(&s.mov_mat), mov_mat.data(), mov_mat.size() * 4));
+  s.ext_instances = 0;
+  s.mov_instances = 0;
+  for (auto& in : c->instances) {
+    s.ext_instances += in.kind >= PT_GEOM_VOLUMES ? 1u : 0u;
+    s.mov_instances += in.kind == PT_GEOM_MOVING_SPHERES ? 1u : 0u;
+  }
   HIPCHK(c, upload(c, L, const_cast<DevMesh**>(&s.meshes), dmesh.data(), dmesh.size() * sizeof(DevMesh)));
   HIPCHK(c, upload(c, L, const_cast<float4**>(&s.vtx), vtx.data(), vtx.size() * sizeof(float4)));
   HIPCHK(c, upload(c, L, const_cast<float4**>(&s.nrm), nrm.data(), nrm.size() * sizeof(float4)));
@@ -591,7 +629,7 @@ pt_status pt_build_accel(pt_context* c, pt_accel_stats* out) {
   std::vector<int> inst_blas(c->instances.size());
   c->blas.clear();
   uint64_t maxn = c->instances.size();
-  uint32_t pool_n[5] = {0, 0, 0, 0, 0};
+  uint32_t pool_n[7] = {0, 0, 0, 0, 0, 0, 0};
   for (size_t i = 0; i < c->instances.size(); i++) {
     const auto& in = c->instances[i];
     uint32_t count = in.kind == PT_GEOM_MESH ? (uint32_t)(c->meshes[in.first].idx.size() / 3) : in.count;
@@ -644,14 +682,17 @@ pt_status pt_build_accel(pt_context* c, pt_accel_stats* out) {
     return hip_fail(c, e, "hipMalloc");
   }
   // sorted primitive pools
-  float4 *sph = nullptr, *rect = nullptr, *box = nullptr, *tri = nullptr, *vol = nullptr;
+  float4 *sph = nullptr, *rect = nullptr, *box = nullptr, *tri = nullptr, *vol = nullptr, *cyl = nullptr,
+         *mov = nullptr;
   uint32_t* sph_idx = nullptr;
   if ((e = dalloc(A, (void**)&sph, pool_n[0] * sizeof(float4))) ||
       (e = dalloc(A, (void**)&sph_idx, pool_n[0] * 4)) ||
       (e = dalloc(A, (void**)&rect, 2ull * pool_n[1] * sizeof(float4))) ||
       (e = dalloc(A, (void**)&box, 2ull * pool_n[2] * sizeof(float4))) ||
       (e = dalloc(A, (void**)&tri, 3ull * pool_n[3] * sizeof(float4))) ||
-      (e = dalloc(A, (void**)&vol, 2ull * pool_n[4] * sizeof(float4)))) {
+      (e = dalloc(A, (void**)&vol, 2ull * pool_n[4] * sizeof(float4))) ||
+      (e = dalloc(A, (void**)&cyl, 2ull * pool_n[5] * sizeof(float4))) ||
+      (e = dalloc(A, (void**)&mov, 3ull * pool_n[6] * sizeof(float4)))) {
     dfree(S);
     return hip_fail(c, e, "hipMalloc (primitive pools)");
   }
@@ -665,6 +706,10 @@ pt_status pt_build_accel(pt_context* c, pt_accel_stats* out) {
       case PT_GEOM_RECTS: launch_bounds_rects(s.rect_o + 2ull * bi.first, n, sb.lo, sb.hi, c->stream); break;
       case PT_GEOM_BOXES: launch_bounds_boxes(s.box_o + 2ull * bi.first, n, sb.lo, sb.hi, c->stream); break;
       case PT_GEOM_VOLUMES: launch_bounds_volumes(s.vol_o + 2ull * bi.first, n, sb.lo, sb.hi, c->stream); break;
+      case PT_GEOM_CYLINDERS: launch_bounds_cylinders(s.cyl_o + 2ull * bi.first, n, sb.lo, sb.hi, c->stream); break;
+      case PT_GEOM_MOVING_SPHERES:
+        launch_bounds_moving(s.mov_o + 3ull * bi.first, n, sb.lo, sb.hi, c->stream);
+        break;
       default: {
         // device mesh table offsets are computed in set_scene in mesh order
         uint32_t vbase = 0, ibase = 0;
@@ -707,6 +752,14 @@ pt_status pt_build_accel(pt_context* c, pt_accel_stats* out) {
         break;
       case PT_GEOM_VOLUMES:
         launch_gather_records(s.vol_o + 2ull * bi.first, ord, n, 2, vol + 2ull * bi.prim_base, bi.first, 0,
+                              c->stream);
+        break;
+      case PT_GEOM_CYLINDERS:
+        launch_gather_records(s.cyl_o + 2ull * bi.first, ord, n, 2, cyl + 2ull * bi.prim_base, bi.first, 0,
+                              c->stream);
+        break;
+      case PT_GEOM_MOVING_SPHERES:
+        launch_gather_records(s.mov_o + 3ull * bi.first, ord, n, 3, mov + 3ull * bi.prim_base, bi.first, 0,
                               c->stream);
         break;
       default: {
@@ -814,8 +867,10 @@ pt_status pt_build_accel(pt_context* c, pt_accel_stats* out) {
   s.box = box;
   s.tri = tri;
   s.vol = vol;
+  s.cyl = cyl;
+  s.mov = mov;
   c->accel_bytes = c->total_nodes * 128 + pool_n[0] * 20ull + pool_n[1] * 32ull + pool_n[2] * 32ull +
-                   pool_n[3] * 48ull + pool_n[4] * 32ull + ni * 128ull;
+                   pool_n[3] * 48ull + pool_n[4] * 32ull + pool_n[5] * 32ull + pool_n[6] * 48ull + ni * 128ull;
   c->accel_ready = true;
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, e0, e1);
@@ -889,7 +944,7 @@ static pt_status ensure_paths(pt_context* c, uint64_t cap) {
     if ((e = F4(&P.st[k].o)) == hipSuccess && (e = F4(&P.st[k].d)) == hipSuccess && (e = F4(&P.st[k].T)) == hipSuccess)
       e = F4(&P.st[k].L);
   if (e || (e = F4(&P.hit)) || (e = F4(&P.hit_sorted)) || (e = U32(&P.keys)) || (e = F4(&P.result)) ||
-      (e = U32(&P.sig_final)) || (e = dalloc(B, (void**)&P.counters, CTR_WORDS * 4)) ||
+      (e = U32(&P.sig_final)) || (e = U32((uint32_t**)&P.ptime)) || (e = dalloc(B, (void**)&P.counters, CTR_WORDS * 4)) ||
       (e = dalloc(B, (void**)&P.seg, (size_t)3 * MAX_WAVES * 4)) ||
       (e = dalloc(B, (void**)&P.bcount, (size_t)8 * MAX_WAVES * 4)) ||
       (e = dalloc(B, (void**)&P.bbase, (size_t)8 * MAX_WAVES * 4)) ||
@@ -959,6 +1014,7 @@ static RenderConsts render_consts(const pt_context* c, const pt_render_params* r
   rc.cache.query_depth = c->cache_query_depth;
   rc.cache.train_threshold = (uint32_t)std::min(16777216.0, std::max(0.0, (double)c->cache_train_fraction * 16777216.0));
   rc.cam.lens_radius = cm.lens_radius;
+  rc.store_time = c->dsc.mov_instances != 0 ? 1u : 0u;
   rc.cam.time0 = cm.time0;
   rc.cam.time1 = cm.time1;
   return rc;
@@ -1005,7 +1061,7 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
   if ((uint64_t)fpb * WH > SLOT_MASK) return fail(c, PT_ERR_INVALID_ARG, "frames_per_batch x W x H exceeds 2^30 - 1");
   const bool use_cache = (rp->flags & PT_FLAG_RADIANCE_CACHE) != 0;
   if (use_cache && !c->cache_ready) return fail(c, PT_ERR_STATE, "PT_FLAG_RADIANCE_CACHE needs pt_cache_init");
-  if (use_cache && (c->dsc.vol_instances != 0 || (c->mat_types >> PT_MAT_ISOTROPIC) != 0))
+  if (use_cache && (c->dsc.ext_instances != 0 || (c->mat_types >> PT_MAT_ISOTROPIC) != 0))
     return fail(c, PT_ERR_UNSUPPORTED, "the radiance cache does not take volumes, isotropic or normal-shader materials");
   pt_status ps = ensure_paths(c, (uint64_t)fpb * WH);
   if (ps == PT_OK) ps = ensure_spill(c);
@@ -1410,7 +1466,7 @@ pt_status pt_render_batch(pt_context* const* ctxs, const pt_render_params* param
     if ((uint64_t)rp->width * rp->height > SLOT_MASK)
       return fail(B, PT_ERR_INVALID_ARG, "image too large for one batch");
     if (rp->flags & PT_FLAG_RADIANCE_CACHE) return fail(B, PT_ERR_UNSUPPORTED, "no radiance cache in a batch");
-    if (c->dsc.vol_instances != 0 || (c->mat_types >> PT_MAT_ISOTROPIC) != 0)
+    if (c->dsc.ext_instances != 0 || (c->mat_types >> PT_MAT_ISOTROPIC) != 0)
       return fail(B, PT_ERR_UNSUPPORTED, "volumes, isotropic and normal-shader materials render with pt_render only");
     union_types |= c->mat_types;
     max_depth = std::max(max_depth, rp->max_depth);

@@ -254,6 +254,33 @@ PTD float isect_volume(float4 A, float4 B, v3 o, v3 d, float tmin, float xi) {
   return r1 + hd / len;
 }
 
+// cylinder.cu:47-66 fixed to its intent (the reference's coefficients
+// read b = d.x P.x + d.z + P.z and c = P.x^2 + P.z^2 + R^2, and it rejects
+// t < delta): the lateral surface x^2 + z^2 = R^2 around the origin's
+// vertical, for 0 <= y - origin.y <= length; the nearer valid root.  Rays
+// parallel to the axis (and grazing ones, delta < 1e-8, as the reference)
+// miss.  A = (origin, length), B.x = radius.
+PTD float isect_cylinder(float4 A, float4 B, v3 o, v3 d, float tmin) {
+  v3 P0 = o - mk3(A);
+  float a = d.x * d.x + d.z * d.z;
+  float b = d.x * P0.x + d.z * P0.z;
+  float c = P0.x * P0.x + P0.z * P0.z - B.x * B.x;
+  float delta = b * b - a * c;
+  if (delta < 1e-8f) return FINF;
+  float sq = sqrtf(delta);
+  float t = (-b - sq) / a;
+  float y = P0.y + t * d.y;
+  if (t > tmin && y >= 0.f && y <= A.w) return t;
+  t = (-b + sq) / a;
+  y = P0.y + t * d.y;
+  if (t > tmin && y >= 0.f && y <= A.w) return t;
+  return FINF;
+}
+// moving_sphere.cu center(): center0 + ((time - time0) / (time1 - time0)) (center1 - center0)
+PTD v3 moving_center(float4 A, float4 B, float4 C, float time) {
+  return mk3(A) + ((time - C.x) / (C.y - C.x)) * (mk3(B) - mk3(A));
+}
+
 // Watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2(1), 2013).
 struct TriRay {
   int kx, ky, kz;

@@ -28,6 +28,7 @@ struct RenderConsts {
   DevCamera cam;
   CacheConsts cache;
   const uint32_t* frame_dev;  // non-null: first frame of the batch read from device memory
+  uint32_t store_time;        // keep each path's ray time (the scene has moving spheres)
 };
 
 // render path (pt_render.hip)
@@ -135,6 +136,8 @@ void launch_bounds_spheres(const float4* sph, uint32_t n, float4* lo, float4* hi
 void launch_bounds_rects(const float4* rect, uint32_t n, float4* lo, float4* hi, hipStream_t st);
 void launch_bounds_boxes(const float4* box, uint32_t n, float4* lo, float4* hi, hipStream_t st);
 void launch_bounds_volumes(const float4* vol, uint32_t n, float4* lo, float4* hi, hipStream_t st);
+void launch_bounds_cylinders(const float4* cyl, uint32_t n, float4* lo, float4* hi, hipStream_t st);
+void launch_bounds_moving(const float4* mov, uint32_t n, float4* lo, float4* hi, hipStream_t st);
 void launch_bounds_tris(const float4* vtx, const uint4* idx, uint32_t n, float4* lo, float4* hi, hipStream_t st);
 void launch_bounds_instances(const uint4* insts_unsorted, const float4* blas_root, uint32_t n, float4* lo,
                              float4* hi, hipStream_t st);

@@ -103,7 +103,16 @@ struct DevScene {
   const float4* vol;
   const float4* vol_o;
   const int32_t* vol_mat;
-  uint32_t vol_instances;  // instances of kind PT_GEOM_VOLUMES (0: none)
+  // open cylinders: (origin, length), (radius, 0, 0, original index)
+  const float4* cyl;
+  const float4* cyl_o;
+  const int32_t* cyl_mat;
+  // moving spheres: (center0, radius), (center1, original index), (time0, time1, 0, 0)
+  const float4* mov;
+  const float4* mov_o;
+  const int32_t* mov_mat;
+  uint32_t ext_instances;  // instances of the kinds volumes, cylinders, moving spheres (0: none)
+  uint32_t mov_instances;  // instances of moving spheres (0: paths keep no time)
   const DevMesh* meshes;
   const float4* vtx;     // vertex pool (xyz)
   const float4* nrm;     // normal pool (xyz)
@@ -161,6 +170,7 @@ struct DevPaths {
   float4* hit_sorted;
   uint32_t* keys;       // material bucket per active path (sorted shading)
   float4* result;       // per slot: (color() value, bounces bits)
+  float* ptime;         // per slot: the path's ray time (scenes with moving spheres)
   uint32_t* sig_final;  // per slot: decision signature
   uint32_t* counters;   // see CTR_*
   uint32_t* seg;        // [3][MAX_WAVES] live paths per wave segment of each state set

@@ -141,8 +141,8 @@ PTD Stack make_stack(uint32_t* s_ref, float* s_tn, uint2* spill, uint32_t* overf
 }
 
 // ------------------------------------------------------- query kernels
-// VOL: volumes draw their free flight from the ray's index
-template <bool COUNT, bool VOL>
+// EXT: volumes draw their free flight from the ray's index
+template <bool COUNT, bool EXT>
 __global__ __launch_bounds__(BLOCK) void k_trace_closest(DevScene sc, const pt_ray* __restrict__ rays, uint64_t n,
                                                          pt_hit* __restrict__ hits, uint32_t* __restrict__ counts,
                                                          uint2* spill, uint32_t* overflow) {
@@ -153,7 +153,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_closest(DevScene sc, const pt_r
     const float4* r4 = reinterpret_cast<const float4*>(rays + i);
     float4 a = r4[0], b = r4[1];
     TraceCounts c{0, 0, 0};
-    HitOut h = trace_closest<COUNT, VOL>(sc, mk3(a), mk3(b), a.w, b.w, st, c, (uint32_t)i);
+    HitOut h = trace_closest<COUNT, EXT>(sc, mk3(a), mk3(b), a.w, b.w, st, c, (uint32_t)i);
     pt_hit out;
     out.t = h.inst >= 0 ? h.t : FINF;
     out.inst = h.inst;
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_closest(DevScene sc, const pt_r
   }
 }
 
-template <bool VOL>
+template <bool EXT>
 __global__ __launch_bounds__(BLOCK) void k_trace_occluded(DevScene sc, const pt_ray* __restrict__ rays, uint64_t n,
                                                           uint8_t* __restrict__ occ, uint2* spill,
                                                           uint32_t* overflow) {
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_occluded(DevScene sc, const pt_
     const float4* r4 = reinterpret_cast<const float4*>(rays + i);
     float4 a = r4[0], b = r4[1];
     TraceCounts c{0, 0, 0};
-    HitOut h = trace_closest<false, VOL>(sc, mk3(a), mk3(b), a.w, b.w, st, c, (uint32_t)i);
+    HitOut h = trace_closest<false, EXT>(sc, mk3(a), mk3(b), a.w, b.w, st, c, (uint32_t)i);
     uint8_t o = 0;
     if (h.inst >= 0) o = sc.mats[hit_material(sc, h.pos, h.prim)].type == PT_MAT_DIFFUSE_LIGHT ? 0 : 1;
     occ[i] = o;
@@ -234,7 +234,9 @@ PTD void start_path(const DevPaths& P, const RenderConsts& rc, uint32_t p, uint3
   v3 origin = mk3(c.org[0], c.org[1], c.org[2]) + lens;
   v3 dir = mk3(c.llc[0], c.llc[1], c.llc[2]) + u * mk3(c.hor[0], c.hor[1], c.hor[2]) +
            v * mk3(c.ver[0], c.ver[1], c.ver[2]) - origin;
-  (void)rnd(seed);  // prd.time (raygen.cu:69), consumed but unused
+  // prd.time (raygen.cu:69); kept per slot when the scene has moving spheres
+  const float tm = c.time0 + rnd(seed) * (c.time1 - c.time0);
+  if (rc.store_time) P.ptime[s] = tm;
   S.o[p] = make_float4(origin.x, origin.y, origin.z, __uint_as_float(batch ? ((seed & 0xFFFFFFu) | (sid << 24)) : seed));
   uint32_t dw = s;
   if ((rc.flags & PT_FLAG_RADIANCE_CACHE) && train_hash(pix, frame) < rc.cache.train_threshold) dw |= TRAIN_BIT;
@@ -296,7 +298,7 @@ PTD uint32_t material_key(const DevScene& sc, float4 h) {
 
 // Closest-hit query of every live path (raygen.cu:77).  With SORT the
 // material bucket of each hit is recorded and counted per wave and bucket.
-template <bool COUNT, bool SORT, bool MULTI, bool VOL>
+template <bool COUNT, bool SORT, bool MULTI, bool EXT>
 __global__ __launch_bounds__(BLOCK) void k_extend(DevScene sc0, DevPaths P, int in, float tmin, uint2* spill,
                                                   uint64_t* stats, const DevScene* __restrict__ scenes) {
   __shared__ uint32_t s_ref[STK_LDS * BLOCK];
@@ -313,7 +315,8 @@ __global__ __launch_bounds__(BLOCK) void k_extend(DevScene sc0, DevPaths P, int 
     if (j < n) {
       float4 o = S.o[i], d = S.d[i];
       const DevScene& sc = MULTI ? scenes[__float_as_uint(o.w) >> 24] : sc0;
-      HitOut h = trace_closest<COUNT, VOL>(sc, mk3(o), mk3(d), tmin, FINF, st, c, __float_as_uint(o.w));
+      const float tm = (EXT && sc.mov_instances) ? P.ptime[__float_as_uint(d.w) & SLOT_MASK] : 0.f;
+      HitOut h = trace_closest<COUNT, EXT>(sc, mk3(o), mk3(d), tmin, FINF, st, c, __float_as_uint(o.w), tm);
       float4 hv = make_float4(h.t, __int_as_float(h.pos), __int_as_float(h.prim), __int_as_float(h.inst));
       P.hit[i] = hv;
       if (SORT) {
@@ -531,16 +534,16 @@ PTD bool scatter_and_roulette(const Surface& f, v3 Wo, v3 N, uint32_t& seed, uin
 // to state set `out`; finished paths write their slot's result.
 // FEAT: SH_COUNT counts the shadow rays' traversal, SH_CACHE records
 // radiance-cache training vertices and queries, SH_ENV samples environment
-// lights, SH_VOL traces shadow rays through volumes; each is compiled only
+// lights, SH_EXT traces shadow rays through volumes; each is compiled only
 // into the variants that use it.
-constexpr int SH_COUNT = 1, SH_CACHE = 2, SH_ENV = 4, SH_MULTI = 8, SH_VOL = 16;
+constexpr int SH_COUNT = 1, SH_CACHE = 2, SH_ENV = 4, SH_MULTI = 8, SH_EXT = 16;
 template <int MS, int FEAT>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_shade(
     DevScene sc0, DevPaths P, int in, int src, const float4* __restrict__ hits, int out, uint32_t depth,
     RenderConsts rc0, uint2* spill, uint64_t* stats, const DevScene* __restrict__ scenes,
     const RenderConsts* __restrict__ rcs) {
   constexpr bool MULTI = (FEAT & SH_MULTI) != 0;
-  constexpr bool VOL = (FEAT & SH_VOL) != 0;
+  constexpr bool EXT = (FEAT & SH_EXT) != 0;
   __shared__ uint32_t s_ref[STK_LDS * BLOCK];
   __shared__ float s_tn[STK_LDS * BLOCK];
   Stack st = make_stack(s_ref, s_tn, spill, P.counters + CTR_STACK_OVERFLOW);
@@ -604,7 +607,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
         int mat = hit_material(sc, pos, prim);
         const DevMaterial& M = sc.mats[mat];
         bool need_uv = tex_needs_uv(sc, M.tex0) || (M.type == PT_MAT_ASHIKHMIN_SHIRLEY && tex_needs_uv(sc, M.tex1));
-        HitRec r = hit_record(sc, pos, prim, hit.x, mk3(o4), d, need_uv);
+        const float tm = (EXT && sc.mov_instances) ? P.ptime[dw & SLOT_MASK] : 0.f;
+        HitRec r = hit_record(sc, pos, prim, hit.x, mk3(o4), d, need_uv, tm);
         Surface f = make_surface(sc, r);
         bool is_light = has_mat<MS>(PT_MAT_DIFFUSE_LIGHT) && f.type == PT_MAT_DIFFUSE_LIGHT;
         // the isotropic medium samples lights without the hemisphere test of
@@ -654,7 +658,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
                 sig = sig_mix(sig, 0x33333333u);
               } else {
                 shadow_rays++;
-                HitOut sh = trace_closest<COUNT, VOL>(sc, r.P, Wi, 1e-3f, FINF, st, tc, seed);
+                HitOut sh = trace_closest<COUNT, EXT>(sc, r.P, Wi, 1e-3f, FINF, st, tc, seed, tm);
                 bool occluded = false;
                 if (sh.inst >= 0)
                   occluded = env || !(sc.mats[hit_material(sc, sh.pos, sh.prim)].type == PT_MAT_DIFFUSE_LIGHT &&
@@ -948,7 +952,7 @@ void launch_trace_closest(const DevScene& sc, const pt_ray* rays, uint64_t n, pt
                           uint2* spill, uint32_t* overflow, int grid, hipStream_t st) {
 #define PT_TRACE(C, V) \
   hipLaunchKernelGGL((k_trace_closest<C, V>), dim3(grid), dim3(BLOCK), 0, st, sc, rays, n, hits, counts, spill, overflow)
-  const bool vol = sc.vol_instances != 0;
+  const bool vol = sc.ext_instances != 0;
   if (counts) {
     if (vol) PT_TRACE(true, true);
     else PT_TRACE(true, false);
@@ -960,7 +964,7 @@ void launch_trace_closest(const DevScene& sc, const pt_ray* rays, uint64_t n, pt
 }
 void launch_trace_occluded(const DevScene& sc, const pt_ray* rays, uint64_t n, uint8_t* occ, uint2* spill,
                            uint32_t* overflow, int grid, hipStream_t st) {
-  if (sc.vol_instances != 0)
+  if (sc.ext_instances != 0)
     hipLaunchKernelGGL(k_trace_occluded<true>, dim3(grid), dim3(BLOCK), 0, st, sc, rays, n, occ, spill, overflow);
   else
     hipLaunchKernelGGL(k_trace_occluded<false>, dim3(grid), dim3(BLOCK), 0, st, sc, rays, n, occ, spill, overflow);
@@ -986,13 +990,13 @@ void launch_extend(const DevScene& sc, const DevPaths& P, int in, float tmin, ui
                    bool count_traversal, bool sort, int grid, hipStream_t st, const DevScene* scenes) {
 #define PT_EXTEND(C, S, M) \
   hipLaunchKernelGGL((k_extend<C, S, M, false>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, tmin, spill, stats, scenes)
-#define PT_EXTEND_VOL(C, S) \
+#define PT_EXTEND_EXT(C, S) \
   hipLaunchKernelGGL((k_extend<C, S, false, true>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, tmin, spill, stats, scenes)
-  if (!scenes && sc.vol_instances != 0) {  // volumes: single scenes only
-    if (count_traversal && sort) PT_EXTEND_VOL(true, true);
-    else if (count_traversal) PT_EXTEND_VOL(true, false);
-    else if (sort) PT_EXTEND_VOL(false, true);
-    else PT_EXTEND_VOL(false, false);
+  if (!scenes && sc.ext_instances != 0) {  // volumes: single scenes only
+    if (count_traversal && sort) PT_EXTEND_EXT(true, true);
+    else if (count_traversal) PT_EXTEND_EXT(true, false);
+    else if (sort) PT_EXTEND_EXT(false, true);
+    else PT_EXTEND_EXT(false, false);
   } else if (scenes) {
     if (count_traversal && sort) PT_EXTEND(true, true, true);
     else if (count_traversal) PT_EXTEND(true, false, true);
@@ -1005,7 +1009,7 @@ void launch_extend(const DevScene& sc, const DevPaths& P, int in, float tmin, ui
     else PT_EXTEND(false, false, false);
   }
 #undef PT_EXTEND
-#undef PT_EXTEND_VOL
+#undef PT_EXTEND_EXT
 }
 void launch_sort(const DevPaths& P, int in, int grid, hipStream_t st) {
   hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(SCAN_THREADS), 0, st, P, (uint32_t)(grid * WAVES));
@@ -1035,14 +1039,14 @@ void launch_shade(const DevScene& sc, const DevPaths& P, int in, bool sorted, in
       if (count) PT_SHADE(MS_ALL, SH_MULTI | SH_COUNT);
       else PT_SHADE(MS_ALL, SH_MULTI);
     }
-  } else if (matset == MS_FULL || sc.vol_instances != 0) {
+  } else if (matset == MS_FULL || sc.ext_instances != 0) {
     // media or the normal shader (single scenes, no radiance cache)
     if (env) {
-      if (count) PT_SHADE(MS_FULL, SH_VOL | SH_ENV | SH_COUNT);
-      else PT_SHADE(MS_FULL, SH_VOL | SH_ENV);
+      if (count) PT_SHADE(MS_FULL, SH_EXT | SH_ENV | SH_COUNT);
+      else PT_SHADE(MS_FULL, SH_EXT | SH_ENV);
     } else {
-      if (count) PT_SHADE(MS_FULL, SH_VOL | SH_COUNT);
-      else PT_SHADE(MS_FULL, SH_VOL);
+      if (count) PT_SHADE(MS_FULL, SH_EXT | SH_COUNT);
+      else PT_SHADE(MS_FULL, SH_EXT);
     }
   } else if (cache) {  // every material; no traversal counting
     if (env) PT_SHADE(MS_ALL, SH_CACHE | SH_ENV);

@@ -15,7 +15,8 @@ struct HitRec {
 // Get_HitRecord of sphere.cu:65-93, aarect.cu:71-115, box.cu:24-33 + 56-81,
 // triangle.cu:85-149, rebuilt in object space from the object-space ray and
 // then transformed to world space.  u, v are only produced when `need_uv`.
-PTD HitRec hit_record(const DevScene& sc, int inst_pos, int prim, float t, v3 o, v3 d, bool need_uv_hint) {
+PTD HitRec hit_record(const DevScene& sc, int inst_pos, int prim, float t, v3 o, v3 d, bool need_uv_hint,
+                      float time = 0.f) {
   const uint4* ir = sc.insts + (size_t)inst_pos * 8;
   float4 w0 = as_f4(ir[0]), w1 = as_f4(ir[1]), w2 = as_f4(ir[2]);
   uint4 info = ir[3];
@@ -60,6 +61,25 @@ PTD HitRec hit_record(const DevScene& sc, int inst_pos, int prim, float t, v3 o,
     v3 neg = mk3(t == t0.x ? 1.f : 0.f, t == t0.y ? 1.f : 0.f, t == t0.z ? 1.f : 0.f);
     v3 pos = mk3(t == t1.x ? 1.f : 0.f, t == t1.y ? 1.f : 0.f, t == t1.z ? 1.f : 0.f);
     n = pos - neg;
+  } else if (kind == PT_GEOM_CYLINDERS) {
+    // cylinder.cu:70-97: radial normal, no uv
+    float4 A = sc.cyl_o[2 * prim];
+    r.mat = sc.cyl_mat[prim];
+    hp = ro + t * rd;
+    n = normalize(hp - mk3(A.x, hp.y, A.z));
+  } else if (kind == PT_GEOM_MOVING_SPHERES) {
+    // moving_sphere.cu:78-91 with the center at the path's time, in object space
+    float4 A = sc.mov_o[3 * prim], B = sc.mov_o[3 * prim + 1], C = sc.mov_o[3 * prim + 2];
+    r.mat = sc.mov_mat[prim];
+    v3 c = moving_center(A, B, C, time);
+    hp = ro + t * rd;
+    n = (hp - c) / A.w;
+    if (need_uv_hint) {
+      float phi = patan2(n.z, n.x);
+      float theta = pasin(n.y);
+      r.u = 1.f - (phi + PI_F) / (2.f * PI_F);
+      r.v = (theta + PI_F / 2.f) / PI_F;
+    }
   } else if (kind == PT_GEOM_VOLUMES) {
     // volume_sphere.cu:88-100: the scatter point, normal +X, no uv
     r.mat = sc.vol_mat[prim];
@@ -374,6 +394,8 @@ PTD int hit_material(const DevScene& sc, int inst_pos, int prim) {
   if (kind == PT_GEOM_RECTS) return sc.rect_mat[prim];
   if (kind == PT_GEOM_BOXES) return sc.box_mat[prim];
   if (kind == PT_GEOM_VOLUMES) return sc.vol_mat[prim];
+  if (kind == PT_GEOM_CYLINDERS) return sc.cyl_mat[prim];
+  if (kind == PT_GEOM_MOVING_SPHERES) return sc.mov_mat[prim];
   const DevMesh& m = sc.meshes[ir[7].z];
   return m.fmat_base >= 0 ? sc.fmat[m.fmat_base + prim] : m.mat;
 }

@@ -70,11 +70,13 @@ PTD float slab_test(float lox, float hix, float loy, float hiy, float loz, float
   return (tn <= tf * SLAB_FAR_SCALE) ? tn : FINF;
 }
 
-// VOL: the scene holds volume instances; their free flights are drawn from
-// `vseed` (the ray's RNG state) and the volume and instance ids.
-template <bool COUNT, bool VOL = false>
+// EXT: the scene holds instances of the extended kinds (volumes,
+// cylinders, moving spheres); volume free flights are drawn from `vseed`
+// (the ray's RNG state) and the volume and instance ids, moving spheres are
+// placed at `time`.
+template <bool COUNT, bool EXT = false>
 PTD HitOut trace_closest(const DevScene& sc, v3 o, v3 d, float tmin, float tmax, Stack& st, TraceCounts& cnt,
-                         uint32_t vseed = 0u) {
+                         uint32_t vseed = 0u, float time = 0.f) {
   HitOut h{tmax, -1, -1, -1};
   const v3 winv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
   v3 ro = o, rd = d, rinv = winv;
@@ -145,10 +147,19 @@ PTD HitOut trace_closest(const DevScene& sc, v3 o, v3 d, float tmin, float tmax,
           float4 lo = sc.box[2 * p], hi = sc.box[2 * p + 1];
           t = isect_box(mk3(lo), mk3(hi), ro, rd, tmin);
           orig = __float_as_uint(lo.w);
-        } else if (VOL && kind == PT_GEOM_VOLUMES) {
+        } else if (EXT && kind == PT_GEOM_VOLUMES) {
           float4 A = sc.vol[2 * p], B = sc.vol[2 * p + 1];
           orig = __float_as_uint(B.w) & 0x7FFFFFFFu;
           t = isect_volume(A, B, ro, rd, tmin, vol_xi(vseed, orig, inst_id));
+        } else if (EXT && kind == PT_GEOM_CYLINDERS) {
+          float4 A = sc.cyl[2 * p], B = sc.cyl[2 * p + 1];
+          orig = __float_as_uint(B.w);
+          t = isect_cylinder(A, B, ro, rd, tmin);
+        } else if (EXT && kind == PT_GEOM_MOVING_SPHERES) {
+          float4 A = sc.mov[3 * p], B = sc.mov[3 * p + 1], C = sc.mov[3 * p + 2];
+          v3 c = moving_center(A, B, C, time);
+          orig = __float_as_uint(B.w);
+          t = isect_sphere(make_float4(c.x, c.y, c.z, A.w), ro, rd, tmin);
         } else {
           float4 a = sc.tri[3 * p], b = sc.tri[3 * p + 1], c = sc.tri[3 * p + 2];
           float b1, b2;

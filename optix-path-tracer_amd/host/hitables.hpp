@@ -1,6 +1,6 @@
 // hitables.hpp -- geometry builders of the host scene API
-// (host_includes/hitables.hpp:20-528): Sphere, AARect, Box, the
-// volumetric sphere and box, and Hitable_List with the reference's transform-order semantics
+// (host_includes/hitables.hpp:20-528): Sphere, Moving_Sphere, AARect, Box,
+// the volumetric sphere and box, Cylinder and Hitable_List with the reference's transform-order semantics
 // (Hitable::addTo reverses its list, Hitable_List::addElementsTo and
 // addListTo do not).
 #pragma once
@@ -131,17 +131,43 @@ class Volumetric_Box : public Hitable {
   const float density;
 };
 
-// Hitables of the reference without a device program yet (moving_sphere.cu
-// is marked broken at hitables.hpp:130; cylinder.cu has coefficient bugs).
+// hitables.hpp:131-172: a sphere moving from c0 at t0 to c1 at t1 (marked
+// broken in the reference at :130; built here as Shirley's moving sphere,
+// with each path's time drawn in the camera's shutter interval)
 class Moving_Sphere : public Hitable {
  public:
-  Moving_Sphere(const float3&, const float3&, const float, const float, const float, BRDF* m) : Hitable(m) {}
-  GeometryInstance getGeometryInstance(Context&) override { throw "Moving_Sphere is not supported by the GPU path yet"; }
+  Moving_Sphere(const float3& c0, const float3& c1, const float r, const float t0, const float t1, BRDF* material)
+      : Hitable(material), center0(c0), center1(c1), radius(r), time0(t0), time1(t1) {}
+  GeometryInstance getGeometryInstance(Context& g_context) override {
+    ptfacade::Prim p;
+    p.kind = PT_GEOM_MOVING_SPHERES;
+    p.m = pt_moving_sphere_desc{{center0.x, center0.y, center0.z}, {center1.x, center1.y, center1.z}, radius, time0,
+                                time1, -1};
+    return make_instance(g_context, p);
+  }
+
+ protected:
+  const float3 center0, center1;
+  const float radius;
+  const float time0, time1;
 };
+// hitables.hpp:411-454: the open lateral surface of a +Y cylinder
+// (cylinder.cu's intersection fixed to its intent)
 class Cylinder : public Hitable {
  public:
-  Cylinder(const float3&, const float&, const float, BRDF* m) : Hitable(m) {}
-  GeometryInstance getGeometryInstance(Context&) override { throw "Cylinder is not supported by the GPU path yet"; }
+  Cylinder(const float3& oo, const float& ll, const float r, BRDF* material)
+      : Hitable(material), O(oo), length(ll), radius(r) {}
+  GeometryInstance getGeometryInstance(Context& g_context) override {
+    ptfacade::Prim p;
+    p.kind = PT_GEOM_CYLINDERS;
+    p.c = pt_cylinder_desc{{O.x, O.y, O.z}, length, radius, -1};
+    return make_instance(g_context, p);
+  }
+
+ protected:
+  const float3 O;
+  const float length;
+  const float radius;
 };
 
 // hitables.hpp:457-528

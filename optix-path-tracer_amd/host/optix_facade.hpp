@@ -219,6 +219,8 @@ struct Prim {
   pt_rect_desc r{};
   pt_box_desc b{};
   pt_volume_desc v{};
+  pt_cylinder_desc c{};
+  pt_moving_sphere_desc m{};
   int mesh = -1;
 };
 
@@ -323,6 +325,8 @@ struct FlatScene {
   std::vector<pt_rect_desc> rects;
   std::vector<pt_box_desc> boxes;
   std::vector<pt_volume_desc> volumes;
+  std::vector<pt_cylinder_desc> cylinders;
+  std::vector<pt_moving_sphere_desc> moving;
   std::vector<pt_mesh_desc> meshes;
   std::vector<pt_instance_desc> instances;
   std::vector<pt_material_desc> materials;
@@ -409,7 +413,8 @@ struct Recorder {
       out.instances.push_back(in);
     };
     // identity instances of the analytic kinds, then transformed groups
-    static const int analytic[4] = {PT_GEOM_SPHERES, PT_GEOM_RECTS, PT_GEOM_BOXES, PT_GEOM_VOLUMES};
+    static const int analytic[6] = {PT_GEOM_SPHERES, PT_GEOM_RECTS,     PT_GEOM_BOXES,
+                                    PT_GEOM_VOLUMES, PT_GEOM_CYLINDERS, PT_GEOM_MOVING_SPHERES};
     for (int kind : analytic) {
       uint32_t first = 0, count = 0;
       for (auto& it : items) {
@@ -464,6 +469,10 @@ struct Recorder {
     d.boxes = out.boxes.data();
     d.num_volumes = (uint32_t)out.volumes.size();
     d.volumes = out.volumes.data();
+    d.num_cylinders = (uint32_t)out.cylinders.size();
+    d.cylinders = out.cylinders.data();
+    d.num_moving_spheres = (uint32_t)out.moving.size();
+    d.moving_spheres = out.moving.data();
     d.num_meshes = (uint32_t)out.meshes.size();
     d.meshes = out.meshes.data();
     d.num_instances = (uint32_t)out.instances.size();
@@ -490,6 +499,18 @@ struct Recorder {
       r.material = mat;
       out.rects.push_back(r);
       return (uint32_t)out.rects.size() - 1;
+    }
+    if (p.kind == PT_GEOM_CYLINDERS) {
+      pt_cylinder_desc c = p.c;
+      c.material = mat;
+      out.cylinders.push_back(c);
+      return (uint32_t)out.cylinders.size() - 1;
+    }
+    if (p.kind == PT_GEOM_MOVING_SPHERES) {
+      pt_moving_sphere_desc m = p.m;
+      m.material = mat;
+      out.moving.push_back(m);
+      return (uint32_t)out.moving.size() - 1;
     }
     if (p.kind == PT_GEOM_VOLUMES) {
       pt_volume_desc v = p.v;

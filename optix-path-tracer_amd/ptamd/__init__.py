@@ -22,6 +22,7 @@ PT_OK = 0
 PT_AXIS_X, PT_AXIS_Y, PT_AXIS_Z = 0, 1, 2
 PT_GEOM_SPHERES, PT_GEOM_RECTS, PT_GEOM_BOXES, PT_GEOM_MESH, PT_GEOM_VOLUMES = 0, 1, 2, 3, 4
 PT_VOLUME_SPHERE, PT_VOLUME_BOX = 0, 1
+PT_GEOM_CYLINDERS, PT_GEOM_MOVING_SPHERES = 5, 6
 PT_MAT_LAMBERTIAN, PT_MAT_METAL, PT_MAT_DIELECTRIC, PT_MAT_DIFFUSE_LIGHT = 0, 1, 2, 3
 PT_MAT_ASHIKHMIN_SHIRLEY, PT_MAT_TORRANCE_SPARROW, PT_MAT_OREN_NAYAR = 4, 5, 6
 PT_MAT_ISOTROPIC, PT_MAT_NORMAL = 7, 8
@@ -58,6 +59,15 @@ class BoxDesc(C.Structure):
 class VolumeDesc(C.Structure):
     _fields_ = [("shape", C.c_int32), ("a", C.c_float * 3), ("b", C.c_float * 3), ("density", C.c_float),
                 ("material", C.c_int32)]
+
+
+class CylinderDesc(C.Structure):
+    _fields_ = [("origin", C.c_float * 3), ("length", C.c_float), ("radius", C.c_float), ("material", C.c_int32)]
+
+
+class MovingSphereDesc(C.Structure):
+    _fields_ = [("center0", C.c_float * 3), ("center1", C.c_float * 3), ("radius", C.c_float),
+                ("time0", C.c_float), ("time1", C.c_float), ("material", C.c_int32)]
 
 
 class MeshDesc(C.Structure):
@@ -103,7 +113,9 @@ class SceneDesc(C.Structure):
                 ("num_textures", C.c_uint32), ("textures", C.POINTER(TextureDesc)),
                 ("num_lights", C.c_uint32), ("lights", C.POINTER(LightDesc)),
                 ("miss", MissDesc),
-                ("num_volumes", C.c_uint32), ("volumes", C.POINTER(VolumeDesc))]
+                ("num_volumes", C.c_uint32), ("volumes", C.POINTER(VolumeDesc)),
+                ("num_cylinders", C.c_uint32), ("cylinders", C.POINTER(CylinderDesc)),
+                ("num_moving_spheres", C.c_uint32), ("moving_spheres", C.POINTER(MovingSphereDesc))]
 
 
 class Camera(C.Structure):

@@ -29,7 +29,7 @@ struct HitRec {
 };
 
 // sphere.cu:65-93, aarect.cu:71-115, box.cu:24-33 + 56-81, triangle.cu:85-149
-inline HitRec hit_record(const Scene& s, int inst, int prim, float t, f3 o, f3 d) {
+inline HitRec hit_record(const Scene& s, int inst, int prim, float t, f3 o, f3 d, float time = 0.f) {
   const Instance& I = s.insts[inst];
   f3 ro, rd;
   s.to_object(I, o, d, ro, rd);
@@ -65,6 +65,23 @@ inline HitRec hit_record(const Scene& s, int inst, int prim, float t, f3 o, f3 d
       f3 neg = mk(t == t0.x ? 1.f : 0.f, t == t0.y ? 1.f : 0.f, t == t0.z ? 1.f : 0.f);
       f3 pos = mk(t == t1.x ? 1.f : 0.f, t == t1.y ? 1.f : 0.f, t == t1.z ? 1.f : 0.f);
       n = pos - neg;
+      break;
+    }
+    case PT_GEOM_CYLINDERS: {  // cylinder.cu:70-97: radial normal, no uv
+      const Cylinder& cy = s.cylinders[prim];
+      hp = ro + t * rd;
+      n = normalize(hp - mk(cy.o.x, hp.y, cy.o.z));
+      break;
+    }
+    case PT_GEOM_MOVING_SPHERES: {  // moving_sphere.cu:78-91, in object space at the ray's time
+      const MovingSphere& m = s.moving[prim];
+      f3 c = moving_center(m, time);
+      hp = ro + t * rd;
+      n = (hp - c) / m.r;
+      float phi = patan2(n.z, n.x);
+      float theta = pasin(n.y);
+      r.u = 1.f - (phi + PI_F) / (2.f * PI_F);
+      r.v = (theta + PI_F / 2.f) / PI_F;
       break;
     }
     case PT_GEOM_VOLUMES: {  // volume_sphere.cu:88-100: normal +X, no uv
@@ -595,7 +612,7 @@ struct PathCounters {
 // hits nothing; its BRDF-sampled half is left to the continuation ray, whose
 // miss is MIS-weighted against env_pdf.
 inline f3 direct_light(const Scene& s, const std::vector<Light>& lights, const Surface& f, f3 P, f3 Wo, f3 N,
-                       bool isLight, uint32_t& seed, uint32_t& sig, PathCounters& pc) {
+                       bool isLight, uint32_t& seed, uint32_t& sig, PathCounters& pc, float time) {
   f3 direct = mk(0.f);
   int nl = (int)lights.size();
   if (nl == 0) return mk(0.f);
@@ -611,7 +628,7 @@ inline f3 direct_light(const Scene& s, const std::vector<Light>& lights, const S
   const bool iso = f.type == PT_MAT_ISOTROPIC;
   if (!iso && dot(Wi, N) < 0.f) { sig = mix(sig, 0x33333333u); return mk(0.f); }
   pc.shadow++;
-  HitState h = s.closest(P, Wi, 1e-3f, INF, nullptr, seed);
+  HitState h = s.closest(P, Wi, 1e-3f, INF, nullptr, seed, time);
   bool occluded = false;
   if (h.found()) {
     int m = s.material_of(h.inst, h.prim);
@@ -705,7 +722,7 @@ inline PathResult trace_path(const Scene& s, const pt_camera& cam, const Params&
            u * mk(cam.horizontal[0], cam.horizontal[1], cam.horizontal[2]) +
            v * mk(cam.vertical[0], cam.vertical[1], cam.vertical[2]) - origin;
   float tmin = 1e-6f;
-  (void)(cam.time0 + rnd(seed) * (cam.time1 - cam.time0));  // prd.time, unused
+  const float time = cam.time0 + rnd(seed) * (cam.time1 - cam.time0);  // prd.time (raygen.cu:69)
   const std::vector<Light>& lights = p.env_mis ? s.lights : s.lights_plain;
   const bool env_light = p.env_mis && s.env.total > 0 && lights.size() != s.lights_plain.size();
   f3 T = mk(1.f), L = mk(0.f);
@@ -714,7 +731,7 @@ inline PathResult trace_path(const Scene& s, const pt_camera& cam, const Params&
   for (uint32_t depth = 0; depth < p.max_depth; depth++) {
     pc.ext++;
     res.bounces = depth + 1;
-    HitState h = s.closest(origin, dir, tmin, INF, nullptr, seed);
+    HitState h = s.closest(origin, dir, tmin, INF, nullptr, seed, time);
     res.sig = mix(res.sig, h.found() ? (((uint32_t)h.inst << 20) ^ (uint32_t)h.prim) : 0xFFFFFFFFu);
     if (!h.found()) {
       if (p.env_mis && s.miss.type == PT_MISS_HDR) {
@@ -731,13 +748,13 @@ inline PathResult trace_path(const Scene& s, const pt_camera& cam, const Params&
       res.L = L + clamp3(T, 0.f, 1.f);
       return res;
     }
-    HitRec r = hit_record(s, h.inst, h.prim, h.t, origin, dir);
+    HitRec r = hit_record(s, h.inst, h.prim, h.t, origin, dir, time);
     Surface f = make_surface(s, r);
     f3 P = r.P, Wo = r.Wo, N = r.N;
     bool spec = false;
     switch (f.type) {
       case PT_MAT_DIFFUSE_LIGHT: {  // diffuse_light.cu:30-48
-        f3 dl = direct_light(s, lights, f, P, Wo, N, true, seed, res.sig, pc);
+        f3 dl = direct_light(s, lights, f, P, Wo, N, true, seed, res.sig, pc, time);
         L += T * dl;
         if (dot(N, Wo) < 0.f) T *= f.c0;
         if (depth == 0 || prevSpec) L += T;
@@ -746,7 +763,7 @@ inline PathResult trace_path(const Scene& s, const pt_camera& cam, const Params&
       }
       case PT_MAT_LAMBERTIAN:
       case PT_MAT_OREN_NAYAR: {  // lambertian.cu:49-73, oren_nayar.cu:46-69
-        f3 dl = direct_light(s, lights, f, P, Wo, N, false, seed, res.sig, pc);
+        f3 dl = direct_light(s, lights, f, P, Wo, N, false, seed, res.sig, pc, time);
         L += T * dl;
         f3 Wi = brdf_sample(f, Wo, N, seed);
         float pdf = 1.f;
@@ -758,7 +775,7 @@ inline PathResult trace_path(const Scene& s, const pt_camera& cam, const Params&
         break;
       }
       case PT_MAT_ISOTROPIC: {  // isotropic.cu:27-51
-        f3 dl = direct_light(s, lights, f, P, Wo, N, false, seed, res.sig, pc);
+        f3 dl = direct_light(s, lights, f, P, Wo, N, false, seed, res.sig, pc, time);
         L += T * dl;
         f3 Wi = brdf_sample(f, Wo, N, seed);
         float pdf = 1.f;

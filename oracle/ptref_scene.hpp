@@ -71,6 +71,14 @@ struct Box { f3 lo, hi; int mat; };
 // hitables.hpp:175-209 / :316-349: shape PT_VOLUME_SPHERE (a = center,
 // b.x = radius) or PT_VOLUME_BOX (a = lo, b = hi)
 struct Volume { int shape; f3 a, b; float density; int mat; };
+// hitables.hpp:411-454: lateral surface around the vertical through o
+struct Cylinder { f3 o; float L, R; int mat; };
+// hitables.hpp:131-172
+struct MovingSphere { f3 c0, c1; float r, t0, t1; int mat; };
+// moving_sphere.cu center()
+inline f3 moving_center(const MovingSphere& m, float time) {
+  return m.c0 + ((time - m.t0) / (m.t1 - m.t0)) * (m.c1 - m.c0);
+}
 struct Mesh {
   std::vector<f3> v, n;
   std::vector<f2> uv;
@@ -111,6 +119,22 @@ inline Aabb bounds(const Volume& v) {
     r.lo = mk(v.a.x - v.b.x, v.a.y - v.b.x, v.a.z - v.b.x);
     r.hi = mk(v.a.x + v.b.x, v.a.y + v.b.x, v.a.z + v.b.x);
   }
+  return r;
+}
+// cylinder.cu:100-107
+inline Aabb bounds(const Cylinder& c) {
+  f3 o2 = mk(c.o.x + c.L * 0.f, c.o.y + c.L * 1.f, c.o.z + c.L * 0.f);
+  Aabb r;
+  r.lo = mk(std::fmin(c.o.x, o2.x) - c.R, std::fmin(c.o.y, o2.y) - c.R, std::fmin(c.o.z, o2.z) - c.R);
+  r.hi = mk(std::fmax(c.o.x, o2.x) + c.R, std::fmax(c.o.y, o2.y) + c.R, std::fmax(c.o.z, o2.z) + c.R);
+  return r;
+}
+// moving_sphere.cu:137-147: boxes at time0 and time1
+inline Aabb bounds(const MovingSphere& m) {
+  f3 p = moving_center(m, m.t0), q = moving_center(m, m.t1);
+  Aabb r;
+  r.lo = mk(std::fmin(p.x - m.r, q.x - m.r), std::fmin(p.y - m.r, q.y - m.r), std::fmin(p.z - m.r, q.z - m.r));
+  r.hi = mk(std::fmax(p.x + m.r, q.x + m.r), std::fmax(p.y + m.r, q.y + m.r), std::fmax(p.z + m.r, q.z + m.r));
   return r;
 }
 inline Aabb bounds_tri(f3 a, f3 bb, f3 c) {
@@ -241,6 +265,26 @@ inline void isect_volume(const Volume& v, f3 o, f3 d, float tmin, float xi, HitS
   const float hd = -(1.f / v.density) * plogf(xi);
   if (!(hd < inside)) return;
   h.offer(r1 + hd / len, tmin, inst, prim);
+}
+
+// cylinder.cu:47-66 fixed to intent (its b and c terms read
+// d.x P.x + d.z + P.z and P.x^2 + P.z^2 + R^2, and it rejects t < delta):
+// x^2 + z^2 = R^2 around the origin's vertical with 0 <= y - o.y <= L; the
+// nearer valid root; delta < 1e-8 misses as in the reference.
+inline void isect_cylinder(const Cylinder& cy, f3 o, f3 d, float tmin, HitState& h, int inst, int prim) {
+  f3 P0 = o - cy.o;
+  float a = d.x * d.x + d.z * d.z;
+  float b = d.x * P0.x + d.z * P0.z;
+  float c = P0.x * P0.x + P0.z * P0.z - cy.R * cy.R;
+  float delta = b * b - a * c;
+  if (delta < 1e-8f) return;
+  float sq = std::sqrt(delta);
+  float t = (-b - sq) / a;
+  float y = P0.y + t * d.y;
+  if (t > tmin && y >= 0.f && y <= cy.L) { h.offer(t, tmin, inst, prim); return; }
+  t = (-b + sq) / a;
+  y = P0.y + t * d.y;
+  if (t > tmin && y >= 0.f && y <= cy.L) h.offer(t, tmin, inst, prim);
 }
 
 // Watertight ray/triangle test (Woop, Benthin, Wald, "Watertight
@@ -556,6 +600,8 @@ struct Scene {
   std::vector<Rect> rects;
   std::vector<Box> boxes;
   std::vector<Volume> volumes;
+  std::vector<Cylinder> cylinders;
+  std::vector<MovingSphere> moving;
   std::vector<Mesh> meshes;
   std::vector<Instance> insts;
   std::vector<Blas> blas;
@@ -583,6 +629,15 @@ struct Scene {
     for (uint32_t i = 0; i < d.num_volumes; i++) {
       auto& v = d.volumes[i];
       volumes.push_back({v.shape, mk(v.a[0], v.a[1], v.a[2]), mk(v.b[0], v.b[1], v.b[2]), v.density, v.material});
+    }
+    for (uint32_t i = 0; i < d.num_cylinders; i++) {
+      auto& c = d.cylinders[i];
+      cylinders.push_back({mk(c.origin[0], c.origin[1], c.origin[2]), c.length, c.radius, c.material});
+    }
+    for (uint32_t i = 0; i < d.num_moving_spheres; i++) {
+      auto& m = d.moving_spheres[i];
+      moving.push_back({mk(m.center0[0], m.center0[1], m.center0[2]), mk(m.center1[0], m.center1[1], m.center1[2]),
+                        m.radius, m.time0, m.time1, m.material});
     }
     for (uint32_t i = 0; i < d.num_meshes; i++) {
       auto& m = d.meshes[i];
@@ -665,6 +720,8 @@ struct Scene {
       case PT_GEOM_RECTS: return widen(bounds(rects.at(first + p)));
       case PT_GEOM_BOXES: return widen(bounds(boxes.at(first + p)));
       case PT_GEOM_VOLUMES: return widen(bounds(volumes.at(first + p)));
+      case PT_GEOM_CYLINDERS: return widen(bounds(cylinders.at(first + p)));
+      case PT_GEOM_MOVING_SPHERES: return widen(bounds(moving.at(first + p)));
       default: {
         const Mesh& m = meshes.at(first);
         return widen(bounds_tri(m.v.at(m.idx[3 * p]), m.v.at(m.idx[3 * p + 1]), m.v.at(m.idx[3 * p + 2])));
@@ -692,9 +749,10 @@ struct Scene {
     od = xform_vector(I.w2o, d);
   }
 
-  // `vseed`: the ray's RNG state, for volume free flights
+  // `vseed`: the ray's RNG state, for volume free flights; `time`: the
+  // ray's time, for moving spheres
   void isect_prim(const Instance& I, int inst, uint32_t local, f3 o, f3 d, const TriRay& tr, float tmin,
-                  HitState& h, uint32_t vseed) const {
+                  HitState& h, uint32_t vseed, float time) const {
     uint32_t g = I.first + local;
     switch (I.kind) {
       case PT_GEOM_SPHERES: isect_sphere(spheres[g], o, d, tmin, h, inst, (int)g); break;
@@ -703,6 +761,12 @@ struct Scene {
       case PT_GEOM_VOLUMES:
         isect_volume(volumes[g], o, d, tmin, vol_xi(vseed, g, (uint32_t)inst), h, inst, (int)g);
         break;
+      case PT_GEOM_CYLINDERS: isect_cylinder(cylinders[g], o, d, tmin, h, inst, (int)g); break;
+      case PT_GEOM_MOVING_SPHERES: {
+        const MovingSphere& m = moving[g];
+        isect_sphere(Sphere{moving_center(m, time), m.r, m.mat}, o, d, tmin, h, inst, (int)g);
+        break;
+      }
       default: {
         const Mesh& m = meshes[I.first];
         float t, b1, b2;
@@ -725,7 +789,8 @@ struct Scene {
   // Closest hit through the two-level BVH4.  Children are visited nearest
   // first (entry distance, then slot); popped entries farther than the
   // current hit are skipped.
-  HitState closest(f3 o, f3 d, float tmin, float tmax, Counts* cnt = nullptr, uint32_t vseed = 0) const {
+  HitState closest(f3 o, f3 d, float tmin, float tmax, Counts* cnt = nullptr, uint32_t vseed = 0,
+                   float time = 0.f) const {
     HitState h;
     h.t = tmax;
     struct Entry { uint32_t ref; float tn; };
@@ -763,7 +828,7 @@ struct Scene {
           const Instance& I = insts[cur];
           for (uint32_t k = 0; k < count; k++) {
             if (cnt) cnt->prims++;
-            isect_prim(I, cur, bvh.order[first + k], ro, rd, tr, tmin, h, vseed);
+            isect_prim(I, cur, bvh.order[first + k], ro, rd, tr, tmin, h, vseed, time);
           }
         }
         continue;
@@ -790,7 +855,7 @@ struct Scene {
   }
 
   // Brute force over every instance and primitive.
-  HitState brute(f3 o, f3 d, float tmin, float tmax, uint32_t vseed = 0) const {
+  HitState brute(f3 o, f3 d, float tmin, float tmax, uint32_t vseed = 0, float time = 0.f) const {
     HitState h;
     h.t = tmax;
     for (size_t i = 0; i < insts.size(); i++) {
@@ -799,7 +864,7 @@ struct Scene {
       to_object(I, o, d, ro, rd);
       TriRay tr{};
       if (I.kind == PT_GEOM_MESH) tr = tri_ray_setup(rd);
-      for (uint32_t p = 0; p < I.count; p++) isect_prim(I, (int)i, p, ro, rd, tr, tmin, h, vseed);
+      for (uint32_t p = 0; p < I.count; p++) isect_prim(I, (int)i, p, ro, rd, tr, tmin, h, vseed, time);
     }
     return h;
   }
@@ -811,6 +876,8 @@ struct Scene {
       case PT_GEOM_RECTS: return rects[prim].mat;
       case PT_GEOM_BOXES: return boxes[prim].mat;
       case PT_GEOM_VOLUMES: return volumes[prim].mat;
+      case PT_GEOM_CYLINDERS: return cylinders[prim].mat;
+      case PT_GEOM_MOVING_SPHERES: return moving[prim].mat;
       default: {
         const Mesh& m = meshes[I.first];
         return m.face_mat.empty() ? m.mat : m.face_mat[prim];

@@ -15,6 +15,8 @@ class PyScene:
     def __init__(self):
         self.spheres, self.rects, self.boxes = [], [], []
         self.volumes = []      # (shape, a (3), b (3), density, mat)
+        self.cylinders = []    # (origin (3), length, radius, mat)
+        self.moving = []       # (center0 (3), center1 (3), radius, time0, time1, mat)
         self.meshes = []       # (verts (n,3) f32, idx (m,3) u32, normals|None, uvs|None, mat)
         self.instances = []    # (kind, first, count, o2w 3x4 or None)
         self.materials = []    # (type, tex0, tex1, params)
@@ -77,6 +79,13 @@ class PyScene:
         d.num_volumes = len(self.volumes)
         d.volumes = arr(ptamd.VolumeDesc, [ptamd.VolumeDesc(sh, (C.c_float * 3)(*a), (C.c_float * 3)(*b), den, m)
                                            for sh, a, b, den, m in self.volumes])
+        d.num_cylinders = len(self.cylinders)
+        d.cylinders = arr(ptamd.CylinderDesc, [ptamd.CylinderDesc((C.c_float * 3)(*o), L, R, m)
+                                               for o, L, R, m in self.cylinders])
+        d.num_moving_spheres = len(self.moving)
+        d.moving_spheres = arr(ptamd.MovingSphereDesc,
+                               [ptamd.MovingSphereDesc((C.c_float * 3)(*c0), (C.c_float * 3)(*c1), r, t0, t1, m)
+                                for c0, c1, r, t0, t1, m in self.moving])
         d.num_boxes = len(self.boxes)
         d.boxes = arr(ptamd.BoxDesc, [ptamd.BoxDesc((C.c_float * 3)(*lo), (C.c_float * 3)(*hi), m)
                                       for lo, hi, m in self.boxes])

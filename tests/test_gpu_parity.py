@@ -549,3 +549,51 @@ def test_next_week_paths_match_oracle(ctx):
     _, (orgb, osig, _), _ = orc.render(sc.camera, params, 0, frames, keep_last=True)
     compare_paths(rgb, sig, orgb, osig)
     assert rgb.max() > 0
+
+
+def shapes_scene():
+    """Open cylinders (one rotated and scaled by an instance), moving
+    spheres swept over the shutter, a ground sphere and a sphere light."""
+    s = PyScene()
+    red, white = s.lambert((0.8, 0.2, 0.1)), s.lambert((0.7, 0.7, 0.7))
+    metal = s.material(ptamd.PT_MAT_METAL, s.texture((0.8, 0.8, 0.9)), params=(0.05,))
+    s.cylinders += [((0.0, -0.5, 0.0), 1.5, 0.6, red), ((0.0, 0.0, 0.0), 1.0, 0.5, metal)]
+    s.instance(ptamd.PT_GEOM_CYLINDERS, 0, 1)
+    c, sn = np.cos(1.2), np.sin(1.2)
+    s.instance(ptamd.PT_GEOM_CYLINDERS, 1, 1, np.array([[c, -sn, 0, 1.8], [sn, c, 0, 0.2], [0, 0, 1, -0.5]]))
+    s.moving += [((-2.2, 0.0, 0.8), (-1.2, 0.6, 0.8), 0.4, 0.0, 1.0, white),
+                 ((1.5, 1.2, 1.0), (1.5, 0.2, 1.0), 0.3, 0.0, 1.0, metal)]
+    s.instance(ptamd.PT_GEOM_MOVING_SPHERES, 0, 2)
+    s.spheres += [((0, -100.5, 0), 100.0, white), ((0, 5, 2), 0.8, s.material(ptamd.PT_MAT_DIFFUSE_LIGHT,
+                                                                                 s.texture((8, 8, 8))))]
+    s.instance(ptamd.PT_GEOM_SPHERES, 0, 2)
+    s.lights.append({"type": ptamd.PT_LIGHT_SPHERE, "center": (0, 5, 2), "radius": 0.8, "emission": (8, 8, 8)})
+    return s
+
+
+def test_shapes_queries_and_paths_match_oracle(ctx):
+    """Closest-hit queries (moving spheres at time 0) and per-path results
+    with each path's time drawn in the shutter [0, 1]: GPU == oracle."""
+    s = shapes_scene()
+    desc = s.desc()
+    orc = ptref.OracleScene(desc)
+    rays = random_rays(20000, -3, 3, seed=13)
+    say("[gpu] closest-hit queries cylinders and moving spheres")
+    gpu_scene(ctx, desc)
+    g, gc = ctx.trace_closest(rays, counted=True)
+    o, oc = orc.trace(rays, counted=True)
+    check_hits(g, o)
+    assert np.array_equal(gc, oc)
+    assert ((g["inst"] >= 0) & (g["inst"] <= 2)).sum() > 500
+    W, H, frames = 64, 48, 2
+    cam = look_camera((0.3, 1.5, 6.0), W, H, vfov=45.0)
+    cam.time0, cam.time1 = 0.0, 1.0
+    for flags in (ptamd.PT_FLAG_KEEP_PATHS, ptamd.PT_FLAG_KEEP_PATHS | ptamd.PT_FLAG_MATERIAL_SORT):
+        params = ptamd.RenderParams(W, H, 10, 1, 3, 2, flags, 0)
+        say(f"[gpu] render paths cylinders and moving spheres flags={flags}")
+        ctx.set_camera(cam)
+        ctx.render(params, 0, frames)
+        rgb, sig, _ = ctx.read_paths(W, H)
+        _, (orgb, osig, _), _ = orc.render(cam, params, 0, frames, keep_last=True)
+        compare_paths(rgb, sig, orgb, osig)
+        assert rgb.max() > 0
