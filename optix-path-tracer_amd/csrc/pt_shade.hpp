@@ -56,8 +56,10 @@ PTD HitRec hit_record(const DevScene& sc, int inst_pos, int prim, float t, v3 o,
     float4 lo = sc.box_o[2 * prim], hi = sc.box_o[2 * prim + 1];
     r.mat = sc.box_mat[prim];
     hp = ro + t * rd;
-    v3 t0 = (mk3(lo) - ro) / rd;
-    v3 t1 = (mk3(hi) - ro) / rd;
+    // the slab distances exactly as the intersection formed them
+    v3 rinv = mk3(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+    v3 t0 = mulv(mk3(lo) - ro, rinv);
+    v3 t1 = mulv(mk3(hi) - ro, rinv);
     v3 neg = mk3(t == t0.x ? 1.f : 0.f, t == t0.y ? 1.f : 0.f, t == t0.z ? 1.f : 0.f);
     v3 pos = mk3(t == t1.x ? 1.f : 0.f, t == t1.y ? 1.f : 0.f, t == t1.z ? 1.f : 0.f);
     n = pos - neg;

@@ -141,11 +141,11 @@ PTD HitOut trace_closest(const DevScene& sc, v3 o, v3 d, float tmin, float tmax,
           t = isect_sphere(sc.sph[p], ro, rd, tmin);
         } else if (kind == PT_GEOM_RECTS) {
           float4 A = sc.rect[2 * p], B = sc.rect[2 * p + 1];
-          t = isect_rect(A, B.x, (int)__float_as_uint(B.y), ro, rd);
+          t = isect_rect(A, B.x, (int)__float_as_uint(B.y), ro, rd, rinv);
           orig = __float_as_uint(B.w);
         } else if (kind == PT_GEOM_BOXES) {
           float4 lo = sc.box[2 * p], hi = sc.box[2 * p + 1];
-          t = isect_box(mk3(lo), mk3(hi), ro, rd, tmin);
+          t = isect_box(mk3(lo), mk3(hi), ro, rinv, tmin);
           orig = __float_as_uint(lo.w);
         } else if (EXT && kind == PT_GEOM_VOLUMES) {
           float4 A = sc.vol[2 * p], B = sc.vol[2 * p + 1];

@@ -60,8 +60,9 @@ inline HitRec hit_record(const Scene& s, int inst, int prim, float t, f3 o, f3 d
     case PT_GEOM_BOXES: {
       const Box& b = s.boxes[prim];
       hp = ro + t * rd;
-      f3 t0 = (b.lo - ro) / rd;
-      f3 t1 = (b.hi - ro) / rd;
+      f3 rinv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+      f3 t0 = mulv(b.lo - ro, rinv);
+      f3 t1 = mulv(b.hi - ro, rinv);
       f3 neg = mk(t == t0.x ? 1.f : 0.f, t == t0.y ? 1.f : 0.f, t == t0.z ? 1.f : 0.f);
       f3 pos = mk(t == t1.x ? 1.f : 0.f, t == t1.y ? 1.f : 0.f, t == t1.z ? 1.f : 0.f);
       n = pos - neg;

@@ -190,18 +190,23 @@ inline void isect_sphere(const Sphere& s, f3 o, f3 d, float tmin, HitState& h, i
   h.offer(t, tmin, inst, prim);
 }
 // aarect.cu:20-69 -- inclusive bounds
-inline void isect_rect(const Rect& r, f3 o, f3 d, float tmin, HitState& h, int inst, int prim) {
+// The plane and slab distances of rects and boxes multiply by the ray's
+// reciprocal direction `inv` (1 / d per component, once per instance)
+// where the reference divides (it was compiled with --use_fast_math, whose
+// division is approximate anyway).
+inline void isect_rect(const Rect& r, f3 o, f3 d, f3 inv, float tmin, HitState& h, int inst, int prim) {
   float t, a, b;
-  if (r.axis == PT_AXIS_X) { t = (r.k - o.x) / d.x; a = o.y + t * d.y; b = o.z + t * d.z; }
-  else if (r.axis == PT_AXIS_Y) { t = (r.k - o.y) / d.y; a = o.x + t * d.x; b = o.z + t * d.z; }
-  else { t = (r.k - o.z) / d.z; a = o.x + t * d.x; b = o.y + t * d.y; }
+  if (r.axis == PT_AXIS_X) { t = (r.k - o.x) * inv.x; a = o.y + t * d.y; b = o.z + t * d.z; }
+  else if (r.axis == PT_AXIS_Y) { t = (r.k - o.y) * inv.y; a = o.x + t * d.x; b = o.z + t * d.z; }
+  else { t = (r.k - o.z) * inv.z; a = o.x + t * d.x; b = o.y + t * d.y; }
   if (a < r.a0 || a > r.a1 || b < r.b0 || b > r.b1) return;
   h.offer(t, tmin, inst, prim);
 }
 // box.cu:36-53 -- slab test, tmin offered first, tmax only if tmin rejected
-inline void isect_box(const Box& x, f3 o, f3 d, float tmin, HitState& h, int inst, int prim) {
-  f3 t0 = (x.lo - o) / d;
-  f3 t1 = (x.hi - o) / d;
+inline f3 mulv(f3 a, f3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+inline void isect_box(const Box& x, f3 o, f3 inv, float tmin, HitState& h, int inst, int prim) {
+  f3 t0 = mulv(x.lo - o, inv);
+  f3 t1 = mulv(x.hi - o, inv);
   float tn = max_component(min_vec(t0, t1));
   float tf = min_component(max_vec(t0, t1));
   if (tn <= tf) {
@@ -756,8 +761,8 @@ struct Scene {
     uint32_t g = I.first + local;
     switch (I.kind) {
       case PT_GEOM_SPHERES: isect_sphere(spheres[g], o, d, tmin, h, inst, (int)g); break;
-      case PT_GEOM_RECTS: isect_rect(rects[g], o, d, tmin, h, inst, (int)g); break;
-      case PT_GEOM_BOXES: isect_box(boxes[g], o, d, tmin, h, inst, (int)g); break;
+      case PT_GEOM_RECTS: isect_rect(rects[g], o, d, mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z), tmin, h, inst, (int)g); break;
+      case PT_GEOM_BOXES: isect_box(boxes[g], o, mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z), tmin, h, inst, (int)g); break;
       case PT_GEOM_VOLUMES:
         isect_volume(volumes[g], o, d, tmin, vol_xi(vseed, g, (uint32_t)inst), h, inst, (int)g);
         break;
