@@ -34,7 +34,11 @@ inline f3 operator*(f3 a, f3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
 inline f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
 inline f3 operator*(float s, f3 a) { return {s * a.x, s * a.y, s * a.z}; }
 inline f3 operator/(f3 a, f3 b) { return {a.x / b.x, a.y / b.y, a.z / b.z}; }
-inline f3 operator/(f3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+// one IEEE reciprocal and three multiplies (as the GPU code)
+inline f3 operator/(f3 a, float s) {
+  const float r = 1.0f / s;
+  return {a.x * r, a.y * r, a.z * r};
+}
 inline f3& operator+=(f3& a, f3 b) { a = a + b; return a; }
 inline f3& operator*=(f3& a, f3 b) { a = a * b; return a; }
 inline f3& operator*=(f3& a, float s) { a = a * s; return a; }
