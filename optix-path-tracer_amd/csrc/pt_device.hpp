@@ -509,7 +509,7 @@ PTD v3 brdf_eval(const Surface& f, v3 Wo, v3 Wi, v3 N, float& pdf) {
     case PT_MAT_LAMBERTIAN: {
       float cosine = dot(normalize(Wi), normalize(N));
       if (cosine < 0.f) return mk3(0.f);
-      pdf = dot(normalize(Wi), normalize(N)) / PI_F;
+      pdf = dot(normalize(Wi), normalize(N)) * (1.0f / PI_F);  // lambertian.cuh:33-38, 1/pi folded
       return (cosine * f.c0) / PI_F;
     }
     case PT_MAT_OREN_NAYAR: {

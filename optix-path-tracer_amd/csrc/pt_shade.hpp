@@ -48,9 +48,17 @@ PTD HitRec hit_record(const DevScene& sc, int inst_pos, int prim, float t, v3 o,
     r.mat = sc.rect_mat[prim];
     int axis = (int)__float_as_uint(B.y);
     hp = ro + t * rd;
-    if (axis == 0) { n = mk3(1.f, 0.f, 0.f); r.u = (hp.y - A.x) / (A.y - A.x); r.v = (hp.z - A.z) / (A.w - A.z); }
-    else if (axis == 1) { n = mk3(0.f, 1.f, 0.f); r.u = (hp.x - A.x) / (A.y - A.x); r.v = (hp.z - A.z) / (A.w - A.z); }
-    else { n = mk3(0.f, 0.f, 1.f); r.u = (hp.x - A.x) / (A.y - A.x); r.v = (hp.y - A.z) / (A.w - A.z); }
+    // aarect.cu:88-100; u, v only where a texture reads them
+    if (axis == 0) {
+      n = mk3(1.f, 0.f, 0.f);
+      if (need_uv_hint) { r.u = (hp.y - A.x) / (A.y - A.x); r.v = (hp.z - A.z) / (A.w - A.z); }
+    } else if (axis == 1) {
+      n = mk3(0.f, 1.f, 0.f);
+      if (need_uv_hint) { r.u = (hp.x - A.x) / (A.y - A.x); r.v = (hp.z - A.z) / (A.w - A.z); }
+    } else {
+      n = mk3(0.f, 0.f, 1.f);
+      if (need_uv_hint) { r.u = (hp.x - A.x) / (A.y - A.x); r.v = (hp.y - A.z) / (A.w - A.z); }
+    }
     if (__float_as_uint(B.z)) n = -n;
   } else if (kind == PT_GEOM_BOXES) {
     float4 lo = sc.box_o[2 * prim], hi = sc.box_o[2 * prim + 1];

@@ -384,7 +384,7 @@ inline f3 brdf_eval(const Surface& f, f3 Wo, f3 Wi, f3 N, float& pdf) {
     case PT_MAT_LAMBERTIAN: {
       float cosine = dot(normalize(Wi), normalize(N));
       if (cosine < 0.f) return mk(0.f);
-      pdf = dot(normalize(Wi), normalize(N)) / PI_F;
+      pdf = dot(normalize(Wi), normalize(N)) * (1.0f / PI_F);  // lambertian.cuh:33-38, 1/pi folded
       return (cosine * f.c0) / PI_F;
     }
     case PT_MAT_OREN_NAYAR: {
