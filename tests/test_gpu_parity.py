@@ -633,3 +633,50 @@ def test_many_instances_match_oracle(ctx):
     rgb, sig, _ = ctx.read_paths(W, H)
     _, (orgb, osig, _), _ = orc.render(cam, params, 0, frames, keep_last=True)
     compare_paths(rgb, sig, orgb, osig)
+
+
+def test_extended_kinds_rejected(ctx):
+    """Malformed media, cylinders and moving spheres are refused by
+    pt_set_scene with an error, and the batch and radiance-cache paths
+    refuse scenes holding them."""
+    def scene(**kw):
+        s = PyScene()
+        m = s.lambert((0.5, 0.5, 0.5))
+        if "volume" in kw:
+            s.volumes.append(kw["volume"] + (m,))
+            s.instance(ptamd.PT_GEOM_VOLUMES, 0, 1)
+        if "moving" in kw:
+            s.moving.append(kw["moving"] + (m,))
+            s.instance(ptamd.PT_GEOM_MOVING_SPHERES, 0, 1)
+        if "cyl_mat" in kw:
+            s.cylinders.append(((0, 0, 0), 1.0, 1.0, kw["cyl_mat"]))
+            s.instance(ptamd.PT_GEOM_CYLINDERS, 0, 1)
+        if "mat" in kw:
+            s.materials[m] = kw["mat"]
+            s.spheres.append(((0, 0, 0), 1.0, m))
+            s.instance(ptamd.PT_GEOM_SPHERES, 0, 1)
+        return s
+    bad = [scene(volume=(ptamd.PT_VOLUME_SPHERE, (0, 0, 0), (1, 0, 0), 0.0)),
+           scene(volume=(ptamd.PT_VOLUME_SPHERE, (0, 0, 0), (1, 0, 0), float("inf"))),
+           scene(volume=(7, (0, 0, 0), (1, 0, 0), 1.0)),
+           scene(moving=((0, 0, 0), (1, 0, 0), 0.5, 1.0, 1.0)),
+           scene(cyl_mat=5),
+           scene(mat=(ptamd.PT_MAT_LAMBERTIAN, -1, -1, (0, 0, 0, 0)))]
+    say("[gpu] malformed extended-kind scenes")
+    for s in bad:
+        with pytest.raises(ptamd.PtError):
+            ctx.set_scene(s.desc())
+    good = scene(volume=(ptamd.PT_VOLUME_BOX, (0, 0, 0), (1, 1, 1), 0.5))
+    sc = ptamd.Scene(ptamd.PT_SCENE_CORNELL, 16, 16)
+    other = ptamd.Context(0)
+    try:
+        gpu_scene(ctx, good.desc())
+        ctx.set_camera(sc.camera)
+        p = ptamd.RenderParams(16, 16, 4, 1, 3, 1, 0, 0)
+        ctx.render(p, 0, 1)
+        gpu_scene(other, sc.desc)
+        other.set_camera(sc.camera)
+        with pytest.raises(ptamd.PtError):
+            ptamd.render_batch([ctx, other], [p, sc.render_params()], [1, 1])
+    finally:
+        other.close()
