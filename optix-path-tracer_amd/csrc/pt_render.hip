@@ -546,6 +546,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
   constexpr bool EXT = (FEAT & SH_EXT) != 0;
   __shared__ uint32_t s_ref[STK_LDS * BLOCK];
   __shared__ float s_tn[STK_LDS * BLOCK];
+  // each lane's T and L wait here while its shadow ray is traced: fewer
+  // registers live across the traversal (fewer scratch spills; with the
+  // stack, 38 KB per block keeps 4 blocks of 4 waves per CU)
+  __shared__ float s_keep[6 * BLOCK];
   Stack st = make_stack(s_ref, s_tn, spill, P.counters + CTR_STACK_OVERFLOW);
   const Seg g = my_seg();
   const uint32_t n = seg_count(P, src, g);
@@ -653,12 +657,21 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
               const bool env = (FEAT & SH_ENV) && Lt.type == PT_LIGHT_ENV;
               v3 Wi = env ? env_sample(sc, seed) : light_sample(Lt, r.P, seed);
               float lightPDF = env ? env_pdf(sc, Wi) : light_pdf(Lt, r.P, Wi);
-              v3 emission = env ? env_radiance(sc, Wi) : light_emission(Lt);
               if (!iso && dot(Wi, r.N) < 0.f) {
                 sig = sig_mix(sig, 0x33333333u);
               } else {
                 shadow_rays++;
+                const uint32_t kt = threadIdx.x;
+                s_keep[kt] = T.x;
+                s_keep[BLOCK + kt] = T.y;
+                s_keep[2 * BLOCK + kt] = T.z;
+                s_keep[3 * BLOCK + kt] = L.x;
+                s_keep[4 * BLOCK + kt] = L.y;
+                s_keep[5 * BLOCK + kt] = L.z;
                 HitOut sh = trace_closest<COUNT, EXT>(sc, r.P, Wi, 1e-3f, FINF, st, tc, seed, tm);
+                T = mk3(s_keep[kt], s_keep[BLOCK + kt], s_keep[2 * BLOCK + kt]);
+                L = mk3(s_keep[3 * BLOCK + kt], s_keep[4 * BLOCK + kt], s_keep[5 * BLOCK + kt]);
+                const v3 emission = env ? env_radiance(sc, Wi) : light_emission(Lt);
                 bool occluded = false;
                 if (sh.inst >= 0)
                   occluded = env || !(sc.mats[hit_material(sc, sh.pos, sh.prim)].type == PT_MAT_DIFFUSE_LIGHT &&
