@@ -6,7 +6,8 @@ import numpy as np
 import pytest
 
 import ptamd
-from asset_files import png_bytes, ppm_bytes, rgbe_bytes, rgbe_to_float
+from asset_files import (bmp_bytes, jpeg_bytes, jpeg_reference_decode, png_bytes, ppm_bytes, rgbe_bytes,
+                         rgbe_to_float, tga_bytes)
 
 OBJ = """# two quads in the z = 0 plane, the second with negative indices
 mtllib scene.mtl
@@ -136,7 +137,7 @@ def test_ppm_texture_and_bad_images(tmp_path):
     with pytest.raises(ptamd.PtError, match="truncated"):
         ptamd.Scene.from_obj("scene.obj", str(tmp_path) + "/")
     (tmp_path / "tex.ppm").write_bytes(b"GIF89a....")
-    with pytest.raises(ptamd.PtError, match="only PNG"):
+    with pytest.raises(ptamd.PtError, match="images are read"):
         ptamd.Scene.from_obj("scene.obj", str(tmp_path) + "/")
     (tmp_path / "tex.ppm").unlink()
     with pytest.raises(ptamd.PtError, match="cannot read"):
@@ -161,3 +162,71 @@ def test_radiance_hdr_environment(tmp_path):
     got = texels(d, d.miss.texture)
     # rows in file order (the environment mapping's v = 0 is the file's top)
     np.testing.assert_array_equal(got[..., :3], rgbe_to_float(rgbe.astype(np.uint8)))
+
+
+def texture_of(tmp_path, name, data):
+    """load `data` as the map_Kd image `name` of the two-quad model"""
+    (tmp_path / "scene.obj").write_text(OBJ)
+    (tmp_path / "scene.mtl").write_text(MTL.replace("tex.png", name))
+    (tmp_path / name).write_bytes(data)
+    sc = ptamd.Scene.from_obj("scene.obj", str(tmp_path) + "/")
+    d = sc.desc
+    fm = np.ctypeslib.as_array(d.meshes[0].face_materials, shape=(4,))
+    return texels(d, d.materials[fm[0]].tex[0])[::-1], sc  # back to file row order
+
+
+@pytest.mark.parametrize("case", ["grey", "444", "420", "422", "420_restart", "separate_scans", "420_restart1"])
+def test_jpeg_textures(tmp_path, case):
+    """Baseline JPEG (ITU-T T.81) written by the test encoder: the decode
+    equals a float64 reference decode of the same quantized coefficients
+    to one level (rounding of different operation orders)."""
+    rng = np.random.default_rng(sum(case.encode()))
+    H, W = 21, 35
+    yy, xx = np.mgrid[0:H, 0:W]
+    smooth = np.stack([120 + 100 * np.sin(xx / 5.0 + k) * np.cos(yy / 7.0) for k in range(3)], 2)
+    img = np.clip(smooth + rng.normal(0, 12, smooth.shape), 0, 255).astype(np.uint8)
+    kw = dict(grey=dict(), **{"444": dict(sampling=(1, 1)), "420": dict(sampling=(2, 2)),
+                             "422": dict(sampling=(2, 1)), "420_restart": dict(sampling=(2, 2), restart=2),
+                             "420_restart1": dict(sampling=(2, 2), restart=1),
+                             "separate_scans": dict(sampling=(2, 2), separate_scans=True, restart=3)})[case]
+    src = img[..., 0] if case == "grey" else img
+    data, comps, q, hv = jpeg_bytes(src, seed=7, **kw)
+    want = jpeg_reference_decode(H, W, comps, q, hv)
+    got, _ = texture_of(tmp_path, "tex.jpg", data)
+    assert got.shape == (H, W, 4)
+    diff = np.abs(np.round(got[..., :3] * 255).astype(int) - want.astype(int))
+    assert diff.max() <= 1, diff.max()
+    assert (diff == 0).mean() > 0.99
+    # the decode is near the source (a lossy but faithful code)
+    ref = np.repeat(src[..., None], 3, 2) if case == "grey" else src
+    assert np.abs(want.astype(int) - ref.astype(int)).mean() < 12
+
+
+def test_jpeg_refusals(tmp_path):
+    img = np.full((8, 8, 3), 100, np.uint8)
+    data, _, _, _ = jpeg_bytes(img)
+    prog = data.replace(b"\xff\xc0", b"\xff\xc2", 1)
+    with pytest.raises(ptamd.PtError, match="progressive"):
+        texture_of(tmp_path, "tex.jpg", prog)
+    with pytest.raises(ptamd.PtError, match="JPEG"):
+        texture_of(tmp_path, "tex.jpg", data[: len(data) // 2])
+
+
+@pytest.mark.parametrize("fmt", ["tga_rgb", "tga_rgba_rle_top", "tga_grey_rle", "bmp24", "bmp32_top"])
+def test_tga_bmp_textures(tmp_path, fmt):
+    rng = np.random.default_rng(len(fmt))
+    H, W = 6, 5
+    ch = 4 if "rgba" in fmt or "32" in fmt else 3
+    img = rng.integers(0, 256, (H, W, ch)).astype(np.uint8)
+    img[2, 1:4] = img[2, 1]  # a run for the RLE coder
+    if fmt == "tga_grey_rle":
+        img = img[..., 0]
+        data, name, want = tga_bytes(img, rle=True), "tex.tga", np.repeat(img[..., None], 3, 2)
+    elif fmt.startswith("tga"):
+        data, name, want = tga_bytes(img, rle="rle" in fmt, top_down="top" in fmt), "tex.tga", img
+    else:
+        data, name, want = bmp_bytes(img, top_down="top" in fmt), "tex.bmp", img
+    got, _ = texture_of(tmp_path, name, data)
+    np.testing.assert_array_equal(got[..., : want.shape[2]], want.astype(np.float32) * np.float32(1.0 / 255.0))
+    if want.shape[2] == 3:
+        assert (got[..., 3] == 1.0).all()
