@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define PT_API_VERSION 9
+#define PT_API_VERSION 10
 
 /* ------------------------------------------------------------------ status */
 typedef int32_t pt_status;
@@ -82,6 +82,10 @@ typedef int32_t pt_status;
 /* gradient_texture.cu: texels = 12 floats, colours A, B, C as (r, g, b, 0) */
 #define PT_TEX_GRADIENT 4
 #define PT_NOISE_FLOATS 1792
+/* vector.cu: child texture i of `width` (texels: the child indices as
+ * floats), i = the hit's face index (pt_mesh_desc.face_indices; 0 on other
+ * primitives); black when i is out of range.  Children are not vectors. */
+#define PT_TEX_VECTOR 5
 
 /* Lights (host_includes/pdfs.hpp + Light_Sampler, host_common.hpp:65). */
 #define PT_LIGHT_RECT 0    /* axis, a0, a1, b0, b1, k                   */
@@ -171,6 +175,9 @@ typedef struct pt_mesh_desc {
   const int32_t* face_materials; /* optional per-face material (NULL: use
                                     `material` for every face)               */
   int32_t material;
+  const int32_t* face_indices;   /* optional per-face index that a
+                                    PT_TEX_VECTOR texture selects by (the
+                                    reference's material buffer; NULL: 0)    */
 } pt_mesh_desc;
 
 /* One instance = one group of primitives of a single kind (a bottom-level

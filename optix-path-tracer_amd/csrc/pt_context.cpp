@@ -61,6 +61,7 @@ struct pt_context {
     std::vector<float> v, n, uv;
     std::vector<uint32_t> idx;
     std::vector<int32_t> fmat;
+    std::vector<int32_t> fidx;
     int32_t mat;
   };
   std::vector<MeshHost> meshes;
@@ -376,7 +377,20 @@ pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
   for (uint32_t i = 0; i < d->num_textures; i++) {
     const pt_texture_desc& t = d->textures[i];
     if (t.type == PT_TEX_IMAGE && (t.width == 0 || t.height == 0 || !t.texels)) return bad("empty image texture");
-    if (t.type < PT_TEX_CONSTANT || t.type > PT_TEX_GRADIENT) return bad("invalid texture type");
+    if (t.type < PT_TEX_CONSTANT || t.type > PT_TEX_VECTOR) return bad("invalid texture type");
+    if (t.type == PT_TEX_VECTOR) {
+      if (t.width == 0 || !t.texels) return bad("empty vector texture");
+      for (uint32_t k = 0; k < t.width; k++) {
+        const float q = t.texels[k];
+        if (!(q >= 0.f && q < (float)d->num_textures) || q != (float)(int)q) return bad("vector texture child out of range");
+        const int32_t ty = d->textures[(int)q].type;
+        if (ty == PT_TEX_VECTOR) return bad("nested vector textures are not supported");
+        if (ty == PT_TEX_CHECKER) {
+          const pt_texture_desc& ck = d->textures[(int)q];
+          if (ck.width >= d->num_textures || ck.height >= d->num_textures) return bad("checker texture index out of range");
+        }
+      }
+    }
     if ((t.type == PT_TEX_NOISE || t.type == PT_TEX_GRADIENT) && !t.texels) return bad("procedural texture without tables");
     if (t.type == PT_TEX_NOISE) {
       for (int k = 0; k < 3 * 256; k++) {
@@ -386,8 +400,12 @@ pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
     }
     if (t.type == PT_TEX_CHECKER) {
       if (t.width >= d->num_textures || t.height >= d->num_textures) return bad("checker texture index out of range");
-      if (d->textures[t.width].type == PT_TEX_CHECKER || d->textures[t.height].type == PT_TEX_CHECKER)
-        return bad("nested checker textures are not supported");
+      if (d->textures[t.width].type >= PT_TEX_CHECKER && d->textures[t.width].type != PT_TEX_NOISE &&
+          d->textures[t.width].type != PT_TEX_GRADIENT)
+        return bad("a checker's children are constant, image, noise or gradient textures");
+      if (d->textures[t.height].type >= PT_TEX_CHECKER && d->textures[t.height].type != PT_TEX_NOISE &&
+          d->textures[t.height].type != PT_TEX_GRADIENT)
+        return bad("a checker's children are constant, image, noise or gradient textures");
     }
   }
   if (d->miss.type < 0 || d->miss.type > 3) return bad("invalid miss program");
@@ -423,6 +441,7 @@ pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
     if (m.uvs) h.uv.assign(m.uvs, m.uvs + 2ull * m.num_vertices);
     h.idx.assign(m.indices, m.indices + 3ull * m.num_triangles);
     if (m.face_materials) h.fmat.assign(m.face_materials, m.face_materials + m.num_triangles);
+    if (m.face_indices) h.fidx.assign(m.face_indices, m.face_indices + m.num_triangles);
     h.mat = m.material;
     c->meshes.push_back(std::move(h));
   }
@@ -435,6 +454,10 @@ pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
     if (t.type == PT_TEX_IMAGE) tx.assign(t.texels, t.texels + 4ull * t.width * t.height);
     if (t.type == PT_TEX_NOISE) tx.assign(t.texels, t.texels + PT_NOISE_FLOATS);
     if (t.type == PT_TEX_GRADIENT) tx.assign(t.texels, t.texels + 12);
+    if (t.type == PT_TEX_VECTOR) {
+      tx.assign(t.texels, t.texels + t.width);
+      tx.resize((tx.size() + 3) / 4 * 4, 0.f);  // whole float4 texels
+    }
     c->texels.push_back(std::move(tx));
     t.texels = nullptr;
   }
@@ -508,7 +531,7 @@ pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
   std::vector<float4> vtx, nrm;
   std::vector<float2> uv;
   std::vector<uint4> idx;
-  std::vector<int32_t> fmat;
+  std::vector<int32_t> fmat, fidx;
   for (auto& m : c->meshes) {
     DevMesh dm{};
     uint32_t nv = (uint32_t)(m.v.size() / 3);
@@ -519,6 +542,8 @@ pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
     dm.has_uv = !m.uv.empty();
     dm.mat = m.mat;
     dm.fmat_base = m.fmat.empty() ? -1 : (int32_t)fmat.size();
+    dm.fidx_base = m.fidx.empty() ? -1 : (int32_t)fidx.size();
+    fidx.insert(fidx.end(), m.fidx.begin(), m.fidx.end());
     for (uint32_t k = 0; k < nv; k++) {
       vtx.push_back(make_float4(m.v[3 * k], m.v[3 * k + 1], m.v[3 * k + 2], 0.f));
       nrm.push_back(dm.has_n ? make_float4(m.n[3 * k], m.n[3 * k + 1], m.n[3 * k + 2], 0.f) : make_float4(0, 0, 0, 0));
@@ -574,6 +599,7 @@ pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
   HIPCHK(c, upload(c, L, const_cast<float2**>(&s.uv), uv.data(), uv.size() * sizeof(float2)));
   HIPCHK(c, upload(c, L, const_cast<uint4**>(&s.idx), idx.data(), idx.size() * sizeof(uint4)));
   HIPCHK(c, upload(c, L, const_cast<int32_t**>(&s.fmat), fmat.data(), fmat.size() * 4));
+  HIPCHK(c, upload(c, L, const_cast<int32_t**>(&s.fidx), fidx.data(), fidx.size() * 4));
   HIPCHK(c, upload(c, L, const_cast<DevMaterial**>(&s.mats), mats.data(), mats.size() * sizeof(DevMaterial)));
   HIPCHK(c, upload(c, L, const_cast<DevTexture**>(&s.texs), texs.data(), texs.size() * sizeof(DevTexture)));
   HIPCHK(c, upload(c, L, const_cast<float4**>(&s.texels), texel_pool.data(), texel_pool.size() * sizeof(float4)));

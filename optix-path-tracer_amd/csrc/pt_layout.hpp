@@ -76,7 +76,7 @@ struct DevMesh {
   int32_t has_n, has_uv;
   int32_t mat;      // mesh material when fmat_base < 0
   int32_t fmat_base;  // first per-face material, or -1
-  uint32_t pad;
+  int32_t fidx_base;  // first per-face vector-texture index, or -1
 };
 
 // Everything a kernel needs to see of the scene, passed by value.
@@ -119,6 +119,7 @@ struct DevScene {
   const float2* uv;      // uv pool
   const uint4* idx;      // index pool (x, y, z: mesh-local vertex ids)
   const int32_t* fmat;   // per-face materials
+  const int32_t* fidx;   // per-face vector-texture indices
   const DevMaterial* mats;
   const DevTexture* texs;
   const float4* texels;

@@ -10,6 +10,7 @@ struct HitRec {
   v3 P, N, Wo;
   float u, v;
   int mat;
+  int fidx;  // the face index a vector texture selects by (0 off meshes)
 };
 
 // Get_HitRecord of sphere.cu:65-93, aarect.cu:71-115, box.cu:24-33 + 56-81,
@@ -31,6 +32,7 @@ PTD HitRec hit_record(const DevScene& sc, int inst_pos, int prim, float t, v3 o,
   r.Wo = normalize(-d);
   r.u = 0.f;
   r.v = 0.f;
+  r.fidx = 0;
   v3 hp, n;
   if (kind == PT_GEOM_SPHERES) {
     float4 s = sc.sph_o[prim];
@@ -114,6 +116,7 @@ PTD HitRec hit_record(const DevScene& sc, int inst_pos, int prim, float t, v3 o,
       r.v = ua.y * b0 + ub.y * b1 + uc.y * b2;
     }
     r.mat = m.fmat_base >= 0 ? sc.fmat[m.fmat_base + prim] : m.mat;
+    if (m.fidx_base >= 0) r.fidx = sc.fidx[m.fidx_base + prim];
   }
   if (xf) {
     float4 m0 = as_f4(ir[4]), m1 = as_f4(ir[5]), m2 = as_f4(ir[6]);
@@ -193,7 +196,14 @@ PTD v3 tex_leaf(const DevScene& sc, const DevTexture& t, float u, float v, v3 P)
   }
   return tex_image(sc, t, u, v);
 }
-PTD v3 tex_eval(const DevScene& sc, int ti, float u, float v, v3 P = mk3(0.f)) {
+// vector.cu:7-12: child `fidx` of a vector texture (children see index 0),
+// black when it is out of range
+PTD v3 tex_eval(const DevScene& sc, int ti, float u, float v, v3 P = mk3(0.f), int fidx = 0) {
+  if (sc.texs[ti].type == PT_TEX_VECTOR) {
+    const DevTexture& tv = sc.texs[ti];
+    if (fidx < 0 || (uint32_t)fidx >= tv.w) return mk3(0.f);
+    ti = (int)reinterpret_cast<const float*>(sc.texels + tv.offset)[fidx];
+  }
   const DevTexture& t = sc.texs[ti];
   if (t.type == PT_TEX_CHECKER) {
     // checkered_texture.cu:8-15 (odd in w, even in h)
@@ -220,15 +230,15 @@ PTD bool tex_needs_uv(const DevScene& sc, int ti) {
   if (ti < 0) return false;
   const DevTexture& t = sc.texs[ti];
   if (t.type == PT_TEX_CHECKER) return sc.texs[t.w].type == PT_TEX_IMAGE || sc.texs[t.h].type == PT_TEX_IMAGE;
-  return t.type == PT_TEX_IMAGE;
+  return t.type == PT_TEX_IMAGE || t.type == PT_TEX_VECTOR;  // vectors: conservatively
 }
 
 PTD Surface make_surface(const DevScene& sc, const HitRec& r) {
   const DevMaterial& m = sc.mats[r.mat];
   Surface f;
   f.type = m.type;
-  f.c0 = m.tex0 >= 0 ? tex_eval(sc, m.tex0, r.u, r.v, r.P) : mk3(0.f);  // the normal shader has none
-  f.c1 = (m.type == PT_MAT_ASHIKHMIN_SHIRLEY) ? tex_eval(sc, m.tex1, r.u, r.v, r.P) : mk3(0.f);
+  f.c0 = m.tex0 >= 0 ? tex_eval(sc, m.tex0, r.u, r.v, r.P, r.fidx) : mk3(0.f);  // the normal shader has none
+  f.c1 = (m.type == PT_MAT_ASHIKHMIN_SHIRLEY) ? tex_eval(sc, m.tex1, r.u, r.v, r.P, r.fidx) : mk3(0.f);
   f.p0 = m.p0;
   f.p1 = m.p1;
   return f;

@@ -43,6 +43,8 @@ class Mesh {
       std::string err;
       if (!obj_load(assetsFolder + fileName, assetsFolder, om, err)) throw std::string("Failed to load/parse .obj: ") + err;
       if (givenMaterial) {
+        // mesh.hpp:142-145: with a given material every face's
+        // material-buffer entry is 0 (a Vector_Texture picks its first child)
         mat = givenMaterial->assignTo(g_context)->index;
       } else {
         // one Lambertian per MTL material, textured by its diffuse map or

@@ -317,6 +317,7 @@ struct MeshData {
   std::vector<float> v, n, uv;
   std::vector<uint32_t> idx;
   std::vector<int32_t> fmat;
+  std::vector<int32_t> fidx;  // per-face vector-texture index (the material buffer)
   int32_t material = 0;
 };
 
@@ -452,6 +453,7 @@ struct Recorder {
       d.num_triangles = (uint32_t)(m.idx.size() / 3);
       d.indices = m.idx.data();
       d.face_materials = m.fmat.empty() ? nullptr : m.fmat.data();
+      d.face_indices = m.fidx.empty() ? nullptr : m.fidx.data();
       d.material = m.material;
       out.meshes.push_back(d);
     }

@@ -1,7 +1,6 @@
 // textures.hpp -- host texture classes (host_includes/textures.hpp).
-// Constant, Checker, Noise, Image, HDR (file or in-memory float data) and
-// Gradient textures are supported by the GPU path; Vector_Texture (a
-// texture per mesh face) throws until its device program exists.
+// Constant, Checker, Noise, Image, HDR (file or in-memory float data),
+// Gradient and Vector textures, all evaluated by the GPU path.
 #pragma once
 
 #include <fstream>
@@ -183,14 +182,24 @@ struct Gradient_Texture : public Texture {
   const float3 colorA, colorB, colorC;
 };
 
-// Per-face texture lists of the reference without a device program yet.
-struct Unsupported_Texture : public Texture {
-  explicit Unsupported_Texture(const char* what) : what(what) {}
-  Program assignTo(Context&) const override { throw std::string(what) + " is not supported by the GPU path yet"; }
-  const char* what;
-};
-struct Vector_Texture : Unsupported_Texture {
-  Vector_Texture(const std::vector<Texture*>&) : Unsupported_Texture("Vector_Texture") {}
+// textures.hpp:253-272: the list a mesh's faces pick from by their
+// material-buffer index (vector.cu); the children are assigned in order
+// and must not be vectors themselves.
+struct Vector_Texture : public Texture {
+  Vector_Texture(const std::vector<Texture*>& tv) : texture_vector(tv) {}
+  Program assignTo(Context& g_context) const override {
+    if (texture_vector.empty()) throw "Vector_Texture needs at least one texture";
+    auto data = std::make_shared<std::vector<float>>();
+    for (const Texture* t : texture_vector) {
+      if (!t) throw "Vector_Texture holds a NULL texture";
+      data->push_back((float)t->assignTo(g_context)->texture);
+    }
+    pt_texture_desc d{};
+    d.type = PT_TEX_VECTOR;
+    d.width = (uint32_t)texture_vector.size();
+    return texture_program(g_context->rec.add_texture(this, d, data));
+  }
+  const std::vector<Texture*> texture_vector;
 };
 
 // textures.hpp:274-290

@@ -28,6 +28,7 @@ PT_MAT_ASHIKHMIN_SHIRLEY, PT_MAT_TORRANCE_SPARROW, PT_MAT_OREN_NAYAR = 4, 5, 6
 PT_MAT_ISOTROPIC, PT_MAT_NORMAL = 7, 8
 PT_TEX_CONSTANT, PT_TEX_IMAGE, PT_TEX_CHECKER, PT_TEX_NOISE, PT_TEX_GRADIENT = 0, 1, 2, 3, 4
 PT_NOISE_FLOATS = 1792
+PT_TEX_VECTOR = 5
 PT_IMAGE_PNG, PT_IMAGE_HDR = 0, 1
 PT_LIGHT_RECT, PT_LIGHT_SPHERE, PT_LIGHT_ENV = 0, 1, 2
 PT_MISS_GRADIENT, PT_MISS_CONSTANT, PT_MISS_IMAGE, PT_MISS_HDR = 0, 1, 2, 3
@@ -74,7 +75,8 @@ class MeshDesc(C.Structure):
     _fields_ = [("num_vertices", C.c_uint32), ("vertices", C.POINTER(C.c_float)),
                 ("normals", C.POINTER(C.c_float)), ("uvs", C.POINTER(C.c_float)),
                 ("num_triangles", C.c_uint32), ("indices", C.POINTER(C.c_uint32)),
-                ("face_materials", C.POINTER(C.c_int32)), ("material", C.c_int32)]
+                ("face_materials", C.POINTER(C.c_int32)), ("material", C.c_int32),
+                ("face_indices", C.POINTER(C.c_int32))]
 
 
 class InstanceDesc(C.Structure):

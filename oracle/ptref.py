@@ -53,7 +53,7 @@ def load():
             "ptref_light_sample": (None, [P, C.c_int32, P, P, P, P]),
             "ptref_light_pdf": (C.c_float, [P, C.c_int32, P, P]),
             "ptref_miss": (None, [P, P, P]),
-            "ptref_texture": (None, [P, C.c_int32, C.c_float, C.c_float, P, P]),
+            "ptref_texture": (None, [P, C.c_int32, C.c_float, C.c_float, P, C.c_int32, P]),
             "ptref_hit_record": (None, [P, C.c_int32, C.c_int32, C.c_float, P, P, P, P]),
             "ptref_libm": (None, [C.c_int32, P, P, P, C.c_uint64]),
             "ptref_schlick": (C.c_float, [C.c_float, C.c_float]),
@@ -181,9 +181,9 @@ class OracleScene:
         self.lib.ptref_miss(self.h, p(f3(d)), p(out))
         return out
 
-    def texture(self, tex, u, v, P):
+    def texture(self, tex, u, v, P, index=0):
         out = np.zeros(3, np.float32)
-        self.lib.ptref_texture(self.h, tex, u, v, p(f3(P)), p(out))
+        self.lib.ptref_texture(self.h, tex, u, v, p(f3(P)), index, p(out))
         return out
 
     def hit_record(self, inst, prim, t, o, d):

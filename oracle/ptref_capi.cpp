@@ -267,9 +267,9 @@ void ptref_miss(const void* sp, const float* d, float* out) {
   out[0] = c.x; out[1] = c.y; out[2] = c.z;
 }
 // texture `tex` at (u, v) and world point P
-void ptref_texture(const void* sp, int32_t tex, float u, float v, const float* P, float* out) {
+void ptref_texture(const void* sp, int32_t tex, float u, float v, const float* P, int32_t fidx, float* out) {
   const Scene& s = *static_cast<const Scene*>(sp);
-  f3 c = tex_eval(s, tex, u, v, mk(P[0], P[1], P[2]));
+  f3 c = tex_eval(s, tex, u, v, mk(P[0], P[1], P[2]), fidx);
   out[0] = c.x; out[1] = c.y; out[2] = c.z;
 }
 // hit record of a given hit: out = P[3], N[3], Wo[3], u, v, material

@@ -26,6 +26,7 @@ struct HitRec {
   f3 P, N, Wo;
   float u = 0.f, v = 0.f;
   int mat;
+  int fidx = 0;  // the face index a vector texture selects by (0 off meshes)
 };
 
 // sphere.cu:65-93, aarect.cu:71-115, box.cu:24-33 + 56-81, triangle.cu:85-149
@@ -108,6 +109,7 @@ inline HitRec hit_record(const Scene& s, int inst, int prim, float t, f3 o, f3 d
         r.u = m.uv[i0].x * b0 + m.uv[i1].x * b1 + m.uv[i2].x * b2;
         r.v = m.uv[i0].y * b0 + m.uv[i1].y * b1 + m.uv[i2].y * b2;
       }
+      if (!m.face_idx.empty()) r.fidx = m.face_idx[prim];
       break;
     }
   }
@@ -199,7 +201,14 @@ inline f3 tex_leaf(const Texture& t, float u, float v, f3 P) {
   }
 }
 // the texture at (u, v) and the world hit point P
-inline f3 tex_eval(const Scene& s, int tex, float u, float v, f3 P = mk(0.f)) {
+// vector.cu:7-12: child `fidx` of a vector texture (children see index 0),
+// black when it is out of range
+inline f3 tex_eval(const Scene& s, int tex, float u, float v, f3 P = mk(0.f), int fidx = 0) {
+  if (s.texs[tex].type == PT_TEX_VECTOR) {
+    const Texture& tv = s.texs[tex];
+    if (fidx < 0 || (uint32_t)fidx >= tv.w) return mk(0.f);
+    tex = (int)tv.texels[fidx];
+  }
   const Texture& t = s.texs[tex];
   if (t.type == PT_TEX_CHECKER) {
     // checkered_texture.cu:8-15: odd texture in w, even in h
@@ -220,8 +229,8 @@ inline Surface make_surface(const Scene& s, const HitRec& r) {
   const Material& m = s.mats[r.mat];
   Surface f;
   f.type = m.type;
-  f.c0 = m.tex[0] >= 0 ? tex_eval(s, m.tex[0], r.u, r.v, r.P) : mk(0.f);  // the normal shader has none
-  f.c1 = (m.type == PT_MAT_ASHIKHMIN_SHIRLEY) ? tex_eval(s, m.tex[1], r.u, r.v, r.P) : mk(0.f);
+  f.c0 = m.tex[0] >= 0 ? tex_eval(s, m.tex[0], r.u, r.v, r.P, r.fidx) : mk(0.f);  // the normal shader has none
+  f.c1 = (m.type == PT_MAT_ASHIKHMIN_SHIRLEY) ? tex_eval(s, m.tex[1], r.u, r.v, r.P, r.fidx) : mk(0.f);
   f.p0 = m.p[0];
   f.p1 = m.p[1];
   return f;

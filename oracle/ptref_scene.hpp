@@ -84,6 +84,7 @@ struct Mesh {
   std::vector<f2> uv;
   std::vector<uint32_t> idx;
   std::vector<int> face_mat;
+  std::vector<int> face_idx;  // per-face vector-texture index (empty: 0)
   int mat = 0;
   uint32_t ntri() const { return (uint32_t)(idx.size() / 3); }
 };
@@ -654,6 +655,7 @@ struct Scene {
       }
       me.idx.assign(m.indices, m.indices + 3 * (size_t)m.num_triangles);
       if (m.face_materials) me.face_mat.assign(m.face_materials, m.face_materials + m.num_triangles);
+      if (m.face_indices) me.face_idx.assign(m.face_indices, m.face_indices + m.num_triangles);
       me.mat = m.material;
       meshes.push_back(std::move(me));
     }
@@ -675,6 +677,9 @@ struct Scene {
         tx.texels.assign(t.texels, t.texels + PT_NOISE_FLOATS);
       } else if (t.type == PT_TEX_GRADIENT) {
         tx.texels.assign(t.texels, t.texels + 12);
+      } else if (t.type == PT_TEX_VECTOR) {
+        tx.w = t.width;  // child indices
+        tx.texels.assign(t.texels, t.texels + t.width);
       }
       texs.push_back(std::move(tx));
     }

@@ -90,7 +90,9 @@ class PyScene:
         d.boxes = arr(ptamd.BoxDesc, [ptamd.BoxDesc((C.c_float * 3)(*lo), (C.c_float * 3)(*hi), m)
                                       for lo, hi, m in self.boxes])
         ms = []
-        for v, idx, n, uv, mat in self.meshes:
+        for mesh in self.meshes:
+            v, idx, n, uv, mat = mesh[:5]
+            fi = mesh[5] if len(mesh) > 5 else None  # per-face vector-texture indices
             v = np.ascontiguousarray(v, np.float32)
             idx = np.ascontiguousarray(idx, np.uint32)
             keep += [v, idx]
@@ -108,6 +110,10 @@ class PyScene:
             md.num_triangles = len(idx)
             md.indices = idx.ctypes.data_as(C.POINTER(C.c_uint32))
             md.material = mat
+            if fi is not None:
+                fi = np.ascontiguousarray(fi, np.int32)
+                keep.append(fi)
+                md.face_indices = fi.ctypes.data_as(C.POINTER(C.c_int32))
             ms.append(md)
         d.num_meshes = len(ms)
         d.meshes = arr(ptamd.MeshDesc, ms)

@@ -680,3 +680,44 @@ def test_extended_kinds_rejected(ctx):
             ptamd.render_batch([ctx, other], [p, sc.render_params()], [1, 1])
     finally:
         other.close()
+
+
+def test_vector_texture_paths_match_oracle(ctx):
+    """vector.cu: a mesh whose faces pick a constant, an image or a checker
+    (or black, out of range) through per-face indices, and a sphere (index
+    0) under the same Lambertian: per-path results match the oracle."""
+    rng = np.random.default_rng(31)
+    s = PyScene()
+    red = s.texture((0.8, 0.1, 0.1))
+    img = np.concatenate([rng.uniform(0, 1, (8, 8, 3)), np.ones((8, 8, 1))], 2).astype(np.float32)
+    im = s.texture(image=img)
+    chk = s.checker(s.texture((0.1, 0.1, 0.8)), s.texture((0.9, 0.9, 0.9)))
+    s.textures.append((ptamd.PT_TEX_VECTOR, (0, 0, 0), None, (3, 0), np.float32([red, im, chk])))
+    vec = len(s.textures) - 1
+    lam = s.material(ptamd.PT_MAT_LAMBERTIAN, vec)
+    v, f = icosphere(2, 1.0, (0.0, 0.0, 0.0))
+    uv = np.stack([v[:, 0] * 0.5 + 0.5, v[:, 1] * 0.5 + 0.5], 1).astype(np.float32)
+    s.meshes.append((v, f, None, uv, lam, np.arange(len(f), dtype=np.int32) % 4))
+    s.instance(ptamd.PT_GEOM_MESH, 0, 0)
+    s.spheres += [((2.2, 0, 0), 0.7, lam), ((0, -100.8, 0), 100.0, s.lambert((0.5, 0.5, 0.5)))]
+    s.instance(ptamd.PT_GEOM_SPHERES, 0, 2)
+    desc = s.desc()
+    W, H, frames = 64, 40, 2
+    cam = look_camera((0.8, 1.0, 5.0), W, H)
+    params = ptamd.RenderParams(W, H, 6, 1, 3, 2, ptamd.PT_FLAG_KEEP_PATHS, 0)
+    say("[gpu] render paths vector texture")
+    gpu_scene(ctx, desc)
+    ctx.set_camera(cam)
+    ctx.render(params, 0, frames)
+    rgb, sig, _ = ctx.read_paths(W, H)
+    orc = ptref.OracleScene(desc)
+    _, (orgb, osig, _), _ = orc.render(cam, params, 0, frames, keep_last=True)
+    compare_paths(rgb, sig, orgb, osig)
+    assert rgb.max() > 0
+    bad = PyScene()
+    a = bad.texture((1, 1, 1))
+    bad.textures.append((ptamd.PT_TEX_VECTOR, (0, 0, 0), None, (2, 0), np.float32([a, 7])))
+    bad.spheres.append(((0, 0, 0), 1.0, bad.material(ptamd.PT_MAT_LAMBERTIAN, len(bad.textures) - 1)))
+    bad.instance(ptamd.PT_GEOM_SPHERES, 0, 1)
+    with pytest.raises(ptamd.PtError):
+        ctx.set_scene(bad.desc())

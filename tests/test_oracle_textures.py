@@ -233,3 +233,25 @@ def test_moving_spheres_oracle_render():
     acc = np.asarray(acc)
     assert np.isfinite(acc).all() and acc.max() > 0
     orc.close()
+
+
+def test_vector_texture_selects_by_face_index(tex_scene):
+    """vector.cu:7-12: child i of the list, black out of range"""
+    s = PyScene()
+    a = s.texture((0.1, 0.2, 0.3))
+    b = s.texture((0.7, 0.8, 0.9))
+    img = np.random.default_rng(9).uniform(0, 1, (4, 8, 4)).astype(np.float32)
+    im = s.texture(image=img)
+    chk = s.checker(a, b)
+    data = np.float32([b, im, chk, a])
+    s.textures.append((ptamd.PT_TEX_VECTOR, (0, 0, 0), None, (4, 0), data))
+    vec = len(s.textures) - 1
+    s.spheres.append(((0, 0, 0), 1.0, s.material(ptamd.PT_MAT_LAMBERTIAN, vec)))
+    s.instance(ptamd.PT_GEOM_SPHERES, 0, 1)
+    orc = ptref.OracleScene(s.desc())
+    P = np.float32([0.31, -0.72, 1.17])
+    for i, child in enumerate([b, im, chk, a]):
+        np.testing.assert_array_equal(orc.texture(vec, 0.3, 0.6, P, i), orc.texture(child, 0.3, 0.6, P))
+    for i in (-1, 4, 100):
+        np.testing.assert_array_equal(orc.texture(vec, 0.3, 0.6, P, i), np.zeros(3, np.float32))
+    orc.close()
