@@ -298,8 +298,10 @@ PTD uint32_t material_key(const DevScene& sc, float4 h) {
 
 // Closest-hit query of every live path (raygen.cu:77).  With SORT the
 // material bucket of each hit is recorded and counted per wave and bucket.
+// 5 waves per SIMD (at most 96 VGPRs, no spills): latency-bound traversal
+// of large meshes gains 3.5 % over 4 waves; the Cornell box is unchanged.
 template <bool COUNT, bool SORT, bool MULTI, bool EXT>
-__global__ __launch_bounds__(BLOCK) void k_extend(DevScene sc0, DevPaths P, int in, float tmin, uint2* spill,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_extend(DevScene sc0, DevPaths P, int in, float tmin, uint2* spill,
                                                   uint64_t* stats, const DevScene* __restrict__ scenes) {
   __shared__ uint32_t s_ref[STK_LDS * BLOCK];
   __shared__ float s_tn[STK_LDS * BLOCK];
