@@ -342,8 +342,10 @@ inline bool load_jpeg(const std::vector<uint8_t>& f, LdrImage& img, std::string&
         q += 17 + total;
       }
     } else if (m == 0xDD) {  // DRI
+      if (len < 4) return err = "bad JPEG restart interval", false;
       restart = be16(q0);
     } else if (m == 0xC0 || m == 0xC1) {  // SOF0 / SOF1
+      if (len < 8 || q0 + 6 + 3 * (size_t)f[q0 + 5] > q1) return err = "bad JPEG frame header", false;
       if (f[q0] != 8) return err = "only 8-bit JPEG images are read", false;
       H = be16(q0 + 1);
       W = be16(q0 + 3);
@@ -376,6 +378,8 @@ inline bool load_jpeg(const std::vector<uint8_t>& f, LdrImage& img, std::string&
       return err = "lossless, hierarchical and arithmetic-coded JPEG images are not supported", false;
     } else if (m == 0xDA) {  // SOS
       if (!frame) return err = "JPEG scan before the frame header", false;
+      if (len < 3 || q0 + 1 + 2 * (size_t)f[q0] + 3 > q1 || f[q0] == 0 || f[q0] > 4)
+        return err = "bad JPEG scan header", false;
       const int ns = f[q0];
       std::vector<int> sc(ns);
       for (int k = 0; k < ns; k++) {

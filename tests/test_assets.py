@@ -230,3 +230,28 @@ def test_tga_bmp_textures(tmp_path, fmt):
     np.testing.assert_array_equal(got[..., : want.shape[2]], want.astype(np.float32) * np.float32(1.0 / 255.0))
     if want.shape[2] == 3:
         assert (got[..., 3] == 1.0).all()
+
+
+def test_malformed_images_are_refused_or_read(tmp_path):
+    """Random byte mutations and truncations of valid JPEG, TGA and BMP files
+    either decode or raise PtError; the reader never reads out of bounds
+    (a fault would end the test process)."""
+    rng = np.random.default_rng(40)
+    img = rng.integers(0, 256, (13, 17, 3)).astype(np.uint8)
+    files = {"tex.jpg": jpeg_bytes(img, sampling=(2, 2), restart=2)[0], "tex.tga": tga_bytes(img, rle=True),
+             "tex.bmp": bmp_bytes(img)}
+    outcomes = {"ok": 0, "refused": 0}
+    for name, good in files.items():
+        for trial in range(60):
+            data = bytearray(good)
+            if trial % 3 == 0:
+                data = data[: int(rng.integers(1, len(data)))]
+            else:
+                for _ in range(int(rng.integers(1, 6))):
+                    data[int(rng.integers(0, len(data)))] = int(rng.integers(0, 256))
+            try:
+                texture_of(tmp_path, name, bytes(data))
+                outcomes["ok"] += 1
+            except ptamd.PtError:
+                outcomes["refused"] += 1
+    assert outcomes["refused"] > 20 and sum(outcomes.values()) == 180
