@@ -258,7 +258,7 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
               "(samples would shard by frame with one reduction of the FP64 accumulation)",
               file=sys.stderr)
         sys.exit(2)

@@ -485,7 +485,7 @@ __global__ void k_gather_u32(const uint32_t* src, uint32_t n, uint32_t* dst) {
   for (uint32_t i = blockIdx.x * BB + threadIdx.x; i < n; i += gridDim.x * BB) dst[i] = src[i];
 }
 
-// ------------------------------------------------------------- driver
+// --------------------------------------------------------- build sequence
 uint32_t build_bvh4(BuildScratch& sb, uint32_t n, int leaf_max, uint4* nodes_out, uint32_t* order_out,
                     uint32_t* morton_out, float4* root_lo_hi, hipStream_t st) {
   if (n == 0 || n > sb.capacity) return 0;
