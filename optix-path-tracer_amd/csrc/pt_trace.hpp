@@ -74,7 +74,9 @@ PTD float slab_test(float lox, float hix, float loy, float hiy, float loz, float
 // cylinders, moving spheres); volume free flights are drawn from `vseed`
 // (the ray's RNG state) and the volume and instance ids, moving spheres are
 // placed at `time`.
-template <bool COUNT, bool EXT = false>
+// MESH: the scene holds mesh instances (without them the triangle test and
+// its per-instance ray setup are compiled out).
+template <bool COUNT, bool EXT = false, bool MESH = true>
 PTD HitOut trace_closest(const DevScene& sc, v3 o, v3 d, float tmin, float tmax, Stack& st, TraceCounts& cnt,
                          uint32_t vseed = 0u, float time = 0.f) {
   HitOut h{tmax, -1, -1, -1};
@@ -125,7 +127,7 @@ PTD HitOut trace_closest(const DevScene& sc, v3 o, v3 d, float tmin, float tmax,
           rd = xf_vector(w0, w1, w2, d);
           rinv = mk3(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
         }
-        if (kind == PT_GEOM_MESH) tr = tri_setup(rd);
+        if (MESH && kind == PT_GEOM_MESH) tr = tri_setup(rd);
         cur = (int)first;
         base_sp = st.sp;
         ref = 0u;  // the bottom-level root, next
@@ -135,7 +137,7 @@ PTD HitOut trace_closest(const DevScene& sc, v3 o, v3 d, float tmin, float tmax,
       for (uint32_t k = 0; k < count; k++) {
         uint32_t p = prim_base + first + k;
         if (COUNT) cnt.prims++;
-        float t;
+        float t = FINF;
         uint32_t orig = 0xFFFFFFFFu;
         if (kind == PT_GEOM_SPHERES) {
           t = isect_sphere(sc.sph[p], ro, rd, tmin);
@@ -160,7 +162,7 @@ PTD HitOut trace_closest(const DevScene& sc, v3 o, v3 d, float tmin, float tmax,
           v3 c = moving_center(A, B, C, time);
           orig = __float_as_uint(B.w);
           t = isect_sphere(make_float4(c.x, c.y, c.z, A.w), ro, rd, tmin);
-        } else {
+        } else if (MESH) {
           float4 a = sc.tri[3 * p], b = sc.tri[3 * p + 1], c = sc.tri[3 * p + 2];
           float b1, b2;
           if (!tri_test(tr, ro, mk3(a), mk3(b), mk3(c), t, b1, b2)) t = FINF;
