@@ -590,8 +590,10 @@ pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
   s.ext_instances = 0;
   s.mov_instances = 0;
   s.mesh_instances = 0;
+  s.sph_instances = 0;
   for (auto& in : c->instances) {
     s.mesh_instances += in.kind == PT_GEOM_MESH ? 1u : 0u;
+    s.sph_instances += in.kind == PT_GEOM_SPHERES ? 1u : 0u;
     s.ext_instances += in.kind >= PT_GEOM_VOLUMES ? 1u : 0u;
     s.mov_instances += in.kind == PT_GEOM_MOVING_SPHERES ? 1u : 0u;
   }

@@ -300,7 +300,7 @@ PTD uint32_t material_key(const DevScene& sc, float4 h) {
 // material bucket of each hit is recorded and counted per wave and bucket.
 // 5 waves per SIMD (at most 96 VGPRs, no spills): latency-bound traversal
 // of large meshes gains 3.5 % over 4 waves; the Cornell box is unchanged.
-template <bool COUNT, bool SORT, bool MULTI, bool EXT, bool MESH = true>
+template <bool COUNT, bool SORT, bool MULTI, bool EXT, bool MESH = true, bool SPH = true>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_extend(DevScene sc0, DevPaths P, int in, float tmin, uint2* spill,
                                                   uint64_t* stats, const DevScene* __restrict__ scenes) {
   __shared__ uint32_t s_ref[STK_LDS * BLOCK];
@@ -318,7 +318,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8))) v
       float4 o = S.o[i], d = S.d[i];
       const DevScene& sc = MULTI ? scenes[__float_as_uint(o.w) >> 24] : sc0;
       const float tm = (EXT && sc.mov_instances) ? P.ptime[__float_as_uint(d.w) & SLOT_MASK] : 0.f;
-      HitOut h = trace_closest<COUNT, EXT, MESH>(sc, mk3(o), mk3(d), tmin, FINF, st, c, __float_as_uint(o.w), tm);
+      HitOut h = trace_closest<COUNT, EXT, MESH, SPH>(sc, mk3(o), mk3(d), tmin, FINF, st, c, __float_as_uint(o.w), tm);
       float4 hv = make_float4(h.t, __int_as_float(h.pos), __int_as_float(h.prim), __int_as_float(h.inst));
       P.hit[i] = hv;
       if (SORT) {
@@ -538,7 +538,7 @@ PTD bool scatter_and_roulette(const Surface& f, v3 Wo, v3 N, uint32_t& seed, uin
 // radiance-cache training vertices and queries, SH_ENV samples environment
 // lights, SH_EXT traces shadow rays through volumes; each is compiled only
 // into the variants that use it.
-constexpr int SH_COUNT = 1, SH_CACHE = 2, SH_ENV = 4, SH_MULTI = 8, SH_EXT = 16, SH_NOMESH = 32;
+constexpr int SH_COUNT = 1, SH_CACHE = 2, SH_ENV = 4, SH_MULTI = 8, SH_EXT = 16, SH_NOMESH = 32, SH_NOSPH = 64;
 template <int MS, int FEAT>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_shade(
     DevScene sc0, DevPaths P, int in, int src, const float4* __restrict__ hits, int out, uint32_t depth,
@@ -547,6 +547,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
   constexpr bool MULTI = (FEAT & SH_MULTI) != 0;
   constexpr bool EXT = (FEAT & SH_EXT) != 0;
   constexpr bool MESH = (FEAT & SH_NOMESH) == 0;
+  constexpr bool SPH = (FEAT & SH_NOSPH) == 0;
   __shared__ uint32_t s_ref[STK_LDS * BLOCK];
   __shared__ float s_tn[STK_LDS * BLOCK];
   // each lane's T and L wait here while its shadow ray is traced: fewer
@@ -671,7 +672,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
                 s_keep[3 * BLOCK + kt] = L.x;
                 s_keep[4 * BLOCK + kt] = L.y;
                 s_keep[5 * BLOCK + kt] = L.z;
-                HitOut sh = trace_closest<COUNT, EXT, MESH>(sc, r.P, Wi, 1e-3f, FINF, st, tc, seed, tm);
+                HitOut sh = trace_closest<COUNT, EXT, MESH, SPH>(sc, r.P, Wi, 1e-3f, FINF, st, tc, seed, tm);
                 T = mk3(s_keep[kt], s_keep[BLOCK + kt], s_keep[2 * BLOCK + kt]);
                 L = mk3(s_keep[3 * BLOCK + kt], s_keep[4 * BLOCK + kt], s_keep[5 * BLOCK + kt]);
                 const v3 emission = env ? env_radiance(sc, Wi) : light_emission(Lt);
@@ -1018,6 +1019,15 @@ void launch_extend(const DevScene& sc, const DevPaths& P, int in, float tmin, ui
     else if (count_traversal) PT_EXTEND(true, false, true);
     else if (sort) PT_EXTEND(false, true, true);
     else PT_EXTEND(false, false, true);
+  } else if (sc.mesh_instances == 0 && sc.sph_instances == 0) {  // rects and boxes only
+#define PT_EXTEND_RB(C, S) \
+  hipLaunchKernelGGL((k_extend<C, S, false, false, false, false>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, tmin, \
+                     spill, stats, scenes)
+    if (count_traversal && sort) PT_EXTEND_RB(true, true);
+    else if (count_traversal) PT_EXTEND_RB(true, false);
+    else if (sort) PT_EXTEND_RB(false, true);
+    else PT_EXTEND_RB(false, false);
+#undef PT_EXTEND_RB
   } else if (sc.mesh_instances == 0) {  // analytic primitives only
 #define PT_EXTEND_NM(C, S) \
   hipLaunchKernelGGL((k_extend<C, S, false, false, false>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, tmin, spill, stats, \
@@ -1080,6 +1090,7 @@ void launch_shade(const DevScene& sc, const DevPaths& P, int in, bool sorted, in
     if (count) PT_SHADE(MS_ALL, SH_ENV | SH_COUNT);
     else PT_SHADE(MS_ALL, SH_ENV);
   } else if (matset == MS_DIFFUSE && sc.mesh_instances == 0) {
+    // (a sphere-free k_shade allocates its registers worse: 8 spills for 5)
     if (count) PT_SHADE(MS_DIFFUSE, SH_COUNT | SH_NOMESH);
     else PT_SHADE(MS_DIFFUSE, SH_NOMESH);
   } else if (matset == MS_SPHERES && sc.mesh_instances == 0) {

@@ -74,9 +74,10 @@ PTD float slab_test(float lox, float hix, float loy, float hiy, float loz, float
 // cylinders, moving spheres); volume free flights are drawn from `vseed`
 // (the ray's RNG state) and the volume and instance ids, moving spheres are
 // placed at `time`.
-// MESH: the scene holds mesh instances (without them the triangle test and
-// its per-instance ray setup are compiled out).
-template <bool COUNT, bool EXT = false, bool MESH = true>
+// MESH / SPH: the scene holds mesh / sphere instances (without them the
+// triangle test and its per-instance ray setup, or the sphere test, are
+// compiled out).
+template <bool COUNT, bool EXT = false, bool MESH = true, bool SPH = true>
 PTD HitOut trace_closest(const DevScene& sc, v3 o, v3 d, float tmin, float tmax, Stack& st, TraceCounts& cnt,
                          uint32_t vseed = 0u, float time = 0.f) {
   HitOut h{tmax, -1, -1, -1};
@@ -139,7 +140,7 @@ PTD HitOut trace_closest(const DevScene& sc, v3 o, v3 d, float tmin, float tmax,
         if (COUNT) cnt.prims++;
         float t = FINF;
         uint32_t orig = 0xFFFFFFFFu;
-        if (kind == PT_GEOM_SPHERES) {
+        if (SPH && kind == PT_GEOM_SPHERES) {
           t = isect_sphere(sc.sph[p], ro, rd, tmin);
         } else if (kind == PT_GEOM_RECTS) {
           float4 A = sc.rect[2 * p], B = sc.rect[2 * p + 1];
@@ -171,11 +172,11 @@ PTD HitOut trace_closest(const DevScene& sc, v3 o, v3 d, float tmin, float tmax,
         if (!(t > tmin)) continue;
         bool take = t < h.t;
         if (!take && t == h.t && h.inst >= 0) {
-          if (kind == PT_GEOM_SPHERES) orig = sc.sph_idx[p];
+          if (SPH && kind == PT_GEOM_SPHERES) orig = sc.sph_idx[p];
           take = ((int)inst_id < h.inst) || ((int)inst_id == h.inst && (int)orig < h.prim);
         }
         if (take) {
-          if (kind == PT_GEOM_SPHERES && orig == 0xFFFFFFFFu) orig = sc.sph_idx[p];
+          if (SPH && kind == PT_GEOM_SPHERES && orig == 0xFFFFFFFFu) orig = sc.sph_idx[p];
           h.t = t;
           h.inst = (int)inst_id;
           h.prim = (int)orig;
