@@ -589,11 +589,9 @@ pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
   HIPCHK(c, upload(c, L, const_cast<int32_t**>(&s.mov_mat), mov_mat.data(), mov_mat.size() * 4));
   s.ext_instances = 0;
   s.mov_instances = 0;
-  s.mesh_instances = 0;
-  s.sph_instances = 0;
+  s.kinds = 0;
   for (auto& in : c->instances) {
-    s.mesh_instances += in.kind == PT_GEOM_MESH ? 1u : 0u;
-    s.sph_instances += in.kind == PT_GEOM_SPHERES ? 1u : 0u;
+    if (in.kind <= PT_GEOM_MESH) s.kinds |= 1u << in.kind;
     s.ext_instances += in.kind >= PT_GEOM_VOLUMES ? 1u : 0u;
     s.mov_instances += in.kind == PT_GEOM_MOVING_SPHERES ? 1u : 0u;
   }

@@ -113,8 +113,7 @@ struct DevScene {
   const int32_t* mov_mat;
   uint32_t ext_instances;  // instances of the kinds volumes, cylinders, moving spheres (0: none)
   uint32_t mov_instances;  // instances of moving spheres (0: paths keep no time)
-  uint32_t mesh_instances; // instances of meshes (0: traversal without the triangle test)
-  uint32_t sph_instances;  // instances of spheres (0: traversal without the sphere test)
+  uint32_t kinds;          // bit k: instances of PT_GEOM kind k < 4 occur
   const DevMesh* meshes;
   const float4* vtx;     // vertex pool (xyz)
   const float4* nrm;     // normal pool (xyz)

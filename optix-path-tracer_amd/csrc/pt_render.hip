@@ -300,7 +300,7 @@ PTD uint32_t material_key(const DevScene& sc, float4 h) {
 // material bucket of each hit is recorded and counted per wave and bucket.
 // 5 waves per SIMD (at most 96 VGPRs, no spills): latency-bound traversal
 // of large meshes gains 3.5 % over 4 waves; the Cornell box is unchanged.
-template <bool COUNT, bool SORT, bool MULTI, bool EXT, bool MESH = true, bool SPH = true>
+template <bool COUNT, bool SORT, bool MULTI, bool EXT, int KINDS = KIND_ALL>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_extend(DevScene sc0, DevPaths P, int in, float tmin, uint2* spill,
                                                   uint64_t* stats, const DevScene* __restrict__ scenes) {
   __shared__ uint32_t s_ref[STK_LDS * BLOCK];
@@ -318,7 +318,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8))) v
       float4 o = S.o[i], d = S.d[i];
       const DevScene& sc = MULTI ? scenes[__float_as_uint(o.w) >> 24] : sc0;
       const float tm = (EXT && sc.mov_instances) ? P.ptime[__float_as_uint(d.w) & SLOT_MASK] : 0.f;
-      HitOut h = trace_closest<COUNT, EXT, MESH, SPH>(sc, mk3(o), mk3(d), tmin, FINF, st, c, __float_as_uint(o.w), tm);
+      HitOut h = trace_closest<COUNT, EXT, KINDS>(sc, mk3(o), mk3(d), tmin, FINF, st, c, __float_as_uint(o.w), tm);
       float4 hv = make_float4(h.t, __int_as_float(h.pos), __int_as_float(h.prim), __int_as_float(h.inst));
       P.hit[i] = hv;
       if (SORT) {
@@ -538,16 +538,14 @@ PTD bool scatter_and_roulette(const Surface& f, v3 Wo, v3 N, uint32_t& seed, uin
 // radiance-cache training vertices and queries, SH_ENV samples environment
 // lights, SH_EXT traces shadow rays through volumes; each is compiled only
 // into the variants that use it.
-constexpr int SH_COUNT = 1, SH_CACHE = 2, SH_ENV = 4, SH_MULTI = 8, SH_EXT = 16, SH_NOMESH = 32, SH_NOSPH = 64;
-template <int MS, int FEAT>
+constexpr int SH_COUNT = 1, SH_CACHE = 2, SH_ENV = 4, SH_MULTI = 8, SH_EXT = 16;
+template <int MS, int FEAT, int KINDS = KIND_ALL>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_shade(
     DevScene sc0, DevPaths P, int in, int src, const float4* __restrict__ hits, int out, uint32_t depth,
     RenderConsts rc0, uint2* spill, uint64_t* stats, const DevScene* __restrict__ scenes,
     const RenderConsts* __restrict__ rcs) {
   constexpr bool MULTI = (FEAT & SH_MULTI) != 0;
   constexpr bool EXT = (FEAT & SH_EXT) != 0;
-  constexpr bool MESH = (FEAT & SH_NOMESH) == 0;
-  constexpr bool SPH = (FEAT & SH_NOSPH) == 0;
   __shared__ uint32_t s_ref[STK_LDS * BLOCK];
   __shared__ float s_tn[STK_LDS * BLOCK];
   // each lane's T and L wait here while its shadow ray is traced: fewer
@@ -672,7 +670,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
                 s_keep[3 * BLOCK + kt] = L.x;
                 s_keep[4 * BLOCK + kt] = L.y;
                 s_keep[5 * BLOCK + kt] = L.z;
-                HitOut sh = trace_closest<COUNT, EXT, MESH, SPH>(sc, r.P, Wi, 1e-3f, FINF, st, tc, seed, tm);
+                HitOut sh = trace_closest<COUNT, EXT, KINDS>(sc, r.P, Wi, 1e-3f, FINF, st, tc, seed, tm);
                 T = mk3(s_keep[kt], s_keep[BLOCK + kt], s_keep[2 * BLOCK + kt]);
                 L = mk3(s_keep[3 * BLOCK + kt], s_keep[4 * BLOCK + kt], s_keep[5 * BLOCK + kt]);
                 const v3 emission = env ? env_radiance(sc, Wi) : light_emission(Lt);
@@ -1003,58 +1001,49 @@ void launch_raygen_batch(const DevPaths& P, const RenderConsts* rcs, const uint3
   hipLaunchKernelGGL(k_raygen_batch, dim3(grid), dim3(BLOCK), 0, st, P, rcs, item_off, item_scene, item_frame, nitems,
                      nslots);
 }
+// instantiation for the scene's kinds: rects + boxes (the Cornell box),
+// rects + meshes (meshes on a ground rect), spheres, analytic, all
+static int extend_kinds(uint32_t k) {
+  if (k == 6u || k == 10u || k == 1u) return (int)k;
+  if ((k & 8u) == 0) return 7;
+  return KIND_ALL;
+}
 void launch_extend(const DevScene& sc, const DevPaths& P, int in, float tmin, uint2* spill, uint64_t* stats,
                    bool count_traversal, bool sort, int grid, hipStream_t st, const DevScene* scenes) {
-#define PT_EXTEND(C, S, M) \
-  hipLaunchKernelGGL((k_extend<C, S, M, false>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, tmin, spill, stats, scenes)
-#define PT_EXTEND_EXT(C, S) \
-  hipLaunchKernelGGL((k_extend<C, S, false, true>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, tmin, spill, stats, scenes)
-  if (!scenes && sc.ext_instances != 0) {  // volumes: single scenes only
-    if (count_traversal && sort) PT_EXTEND_EXT(true, true);
-    else if (count_traversal) PT_EXTEND_EXT(true, false);
-    else if (sort) PT_EXTEND_EXT(false, true);
-    else PT_EXTEND_EXT(false, false);
-  } else if (scenes) {
-    if (count_traversal && sort) PT_EXTEND(true, true, true);
-    else if (count_traversal) PT_EXTEND(true, false, true);
-    else if (sort) PT_EXTEND(false, true, true);
-    else PT_EXTEND(false, false, true);
-  } else if (sc.mesh_instances == 0 && sc.sph_instances == 0) {  // rects and boxes only
-#define PT_EXTEND_RB(C, S) \
-  hipLaunchKernelGGL((k_extend<C, S, false, false, false, false>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, tmin, \
-                     spill, stats, scenes)
-    if (count_traversal && sort) PT_EXTEND_RB(true, true);
-    else if (count_traversal) PT_EXTEND_RB(true, false);
-    else if (sort) PT_EXTEND_RB(false, true);
-    else PT_EXTEND_RB(false, false);
-#undef PT_EXTEND_RB
-  } else if (sc.mesh_instances == 0) {  // analytic primitives only
-#define PT_EXTEND_NM(C, S) \
-  hipLaunchKernelGGL((k_extend<C, S, false, false, false>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, tmin, spill, stats, \
-                     scenes)
-    if (count_traversal && sort) PT_EXTEND_NM(true, true);
-    else if (count_traversal) PT_EXTEND_NM(true, false);
-    else if (sort) PT_EXTEND_NM(false, true);
-    else PT_EXTEND_NM(false, false);
-#undef PT_EXTEND_NM
+#define PT_EXTEND(C, S, M, E, K) \
+  hipLaunchKernelGGL((k_extend<C, S, M, E, K>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, tmin, spill, stats, scenes)
+#define PT_EXTEND4(M, E, K)                                   \
+  do {                                                        \
+    if (count_traversal && sort) PT_EXTEND(true, true, M, E, K); \
+    else if (count_traversal) PT_EXTEND(true, false, M, E, K);   \
+    else if (sort) PT_EXTEND(false, true, M, E, K);              \
+    else PT_EXTEND(false, false, M, E, K);                       \
+  } while (0)
+  if (scenes) {
+    PT_EXTEND4(true, false, KIND_ALL);
+  } else if (sc.ext_instances != 0) {  // volumes, cylinders, moving spheres: single scenes only
+    PT_EXTEND4(false, true, KIND_ALL);
   } else {
-    if (count_traversal && sort) PT_EXTEND(true, true, false);
-    else if (count_traversal) PT_EXTEND(true, false, false);
-    else if (sort) PT_EXTEND(false, true, false);
-    else PT_EXTEND(false, false, false);
+    switch (extend_kinds(sc.kinds)) {
+      case 6: PT_EXTEND4(false, false, 6); break;
+      case 10: PT_EXTEND4(false, false, 10); break;
+      case 1: PT_EXTEND4(false, false, 1); break;
+      case 7: PT_EXTEND4(false, false, 7); break;
+      default: PT_EXTEND4(false, false, KIND_ALL);
+    }
   }
+#undef PT_EXTEND4
 #undef PT_EXTEND
-#undef PT_EXTEND_EXT
 }
 void launch_sort(const DevPaths& P, int in, int grid, hipStream_t st) {
   hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(SCAN_THREADS), 0, st, P, (uint32_t)(grid * WAVES));
   hipLaunchKernelGGL(k_sort_scatter, dim3(grid), dim3(BLOCK), 0, st, P, in);
 }
-template <int MS, int FEAT>
+template <int MS, int FEAT, int KINDS = KIND_ALL>
 static void launch_shade_v(const DevScene& sc, const DevPaths& P, int in, int src, const float4* hits, int out,
                            uint32_t depth, const RenderConsts& rc, uint2* spill, uint64_t* stats, int grid,
                            hipStream_t st, const DevScene* scenes, const RenderConsts* rcs) {
-  hipLaunchKernelGGL((k_shade<MS, FEAT>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, src, hits, out, depth, rc, spill,
+  hipLaunchKernelGGL((k_shade<MS, FEAT, KINDS>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, src, hits, out, depth, rc, spill,
                      stats, scenes, rcs);
 }
 void launch_shade(const DevScene& sc, const DevPaths& P, int in, bool sorted, int out, uint32_t depth,
@@ -1066,6 +1055,8 @@ void launch_shade(const DevScene& sc, const DevPaths& P, int in, bool sorted, in
   const bool cache = (rc.flags & PT_FLAG_RADIANCE_CACHE) != 0;
 #define PT_SHADE(MSV, F) \
   launch_shade_v<MSV, F>(sc, P, in, src, hits, out, depth, rc, spill, stats, grid, st, scenes, rcs)
+#define PT_SHADE_K(MSV, F, K) \
+  launch_shade_v<MSV, F, K>(sc, P, in, src, hits, out, depth, rc, spill, stats, grid, st, scenes, rcs)
   if (scenes) {  // a batch of scenes: every material, per-path scene
     if (batch_env) {
       if (count) PT_SHADE(MS_ALL, SH_MULTI | SH_ENV | SH_COUNT);
@@ -1086,16 +1077,19 @@ void launch_shade(const DevScene& sc, const DevPaths& P, int in, bool sorted, in
   } else if (cache) {  // every material; no traversal counting
     if (env) PT_SHADE(MS_ALL, SH_CACHE | SH_ENV);
     else PT_SHADE(MS_ALL, SH_CACHE);
+  } else if (env && sc.kinds == 10u) {  // meshes on a ground rect
+    if (count) PT_SHADE_K(MS_ALL, SH_ENV | SH_COUNT, 10);
+    else PT_SHADE_K(MS_ALL, SH_ENV, 10);
   } else if (env) {
     if (count) PT_SHADE(MS_ALL, SH_ENV | SH_COUNT);
     else PT_SHADE(MS_ALL, SH_ENV);
-  } else if (matset == MS_DIFFUSE && sc.mesh_instances == 0) {
-    // (a sphere-free k_shade allocates its registers worse: 8 spills for 5)
-    if (count) PT_SHADE(MS_DIFFUSE, SH_COUNT | SH_NOMESH);
-    else PT_SHADE(MS_DIFFUSE, SH_NOMESH);
-  } else if (matset == MS_SPHERES && sc.mesh_instances == 0) {
-    if (count) PT_SHADE(MS_SPHERES, SH_COUNT | SH_NOMESH);
-    else PT_SHADE(MS_SPHERES, SH_NOMESH);
+  } else if (matset == MS_DIFFUSE && (sc.kinds & 8u) == 0) {
+    // analytic only (a sphere-free k_shade allocates its registers worse)
+    if (count) PT_SHADE_K(MS_DIFFUSE, SH_COUNT, 7);
+    else PT_SHADE_K(MS_DIFFUSE, 0, 7);
+  } else if (matset == MS_SPHERES && (sc.kinds & 8u) == 0) {
+    if (count) PT_SHADE_K(MS_SPHERES, SH_COUNT, 7);
+    else PT_SHADE_K(MS_SPHERES, 0, 7);
   } else if (matset == MS_DIFFUSE) {
     if (count) PT_SHADE(MS_DIFFUSE, SH_COUNT);
     else PT_SHADE(MS_DIFFUSE, 0);
@@ -1106,6 +1100,7 @@ void launch_shade(const DevScene& sc, const DevPaths& P, int in, bool sorted, in
     if (count) PT_SHADE(MS_ALL, SH_COUNT);
     else PT_SHADE(MS_ALL, 0);
   }
+#undef PT_SHADE_K
 #undef PT_SHADE
 }
 int shade_material_set(uint32_t types_mask) {

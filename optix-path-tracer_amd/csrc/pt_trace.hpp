@@ -74,12 +74,14 @@ PTD float slab_test(float lox, float hix, float loy, float hiy, float loz, float
 // cylinders, moving spheres); volume free flights are drawn from `vseed`
 // (the ray's RNG state) and the volume and instance ids, moving spheres are
 // placed at `time`.
-// MESH / SPH: the scene holds mesh / sphere instances (without them the
-// triangle test and its per-instance ray setup, or the sphere test, are
-// compiled out).
-template <bool COUNT, bool EXT = false, bool MESH = true, bool SPH = true>
+// KINDS: bit k set when instances of PT_GEOM kind k (spheres, rects,
+// boxes, meshes) may occur; the tests of the other kinds (and the
+// triangle test's per-instance ray setup) are compiled out.
+constexpr int KIND_ALL = 15;
+template <bool COUNT, bool EXT = false, int KINDS = KIND_ALL>
 PTD HitOut trace_closest(const DevScene& sc, v3 o, v3 d, float tmin, float tmax, Stack& st, TraceCounts& cnt,
                          uint32_t vseed = 0u, float time = 0.f) {
+  constexpr bool SPH = (KINDS & 1) != 0, RECT = (KINDS & 2) != 0, BOX = (KINDS & 4) != 0, MESH = (KINDS & 8) != 0;
   HitOut h{tmax, -1, -1, -1};
   const v3 winv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
   v3 ro = o, rd = d, rinv = winv;
@@ -142,11 +144,11 @@ PTD HitOut trace_closest(const DevScene& sc, v3 o, v3 d, float tmin, float tmax,
         uint32_t orig = 0xFFFFFFFFu;
         if (SPH && kind == PT_GEOM_SPHERES) {
           t = isect_sphere(sc.sph[p], ro, rd, tmin);
-        } else if (kind == PT_GEOM_RECTS) {
+        } else if (RECT && kind == PT_GEOM_RECTS) {
           float4 A = sc.rect[2 * p], B = sc.rect[2 * p + 1];
           t = isect_rect(A, B.x, (int)__float_as_uint(B.y), ro, rd, rinv);
           orig = __float_as_uint(B.w);
-        } else if (kind == PT_GEOM_BOXES) {
+        } else if (BOX && kind == PT_GEOM_BOXES) {
           float4 lo = sc.box[2 * p], hi = sc.box[2 * p + 1];
           t = isect_box(mk3(lo), mk3(hi), ro, rinv, tmin);
           orig = __float_as_uint(lo.w);
