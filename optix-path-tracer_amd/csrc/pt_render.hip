@@ -614,7 +614,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
         const DevMaterial& M = sc.mats[mat];
         bool need_uv = tex_needs_uv(sc, M.tex0) || (M.type == PT_MAT_ASHIKHMIN_SHIRLEY && tex_needs_uv(sc, M.tex1));
         const float tm = (EXT && sc.mov_instances) ? P.ptime[dw & SLOT_MASK] : 0.f;
-        HitRec r = hit_record(sc, pos, prim, hit.x, mk3(o4), d, need_uv, tm);
+        HitRec r = hit_record<KINDS, EXT>(sc, pos, prim, hit.x, mk3(o4), d, need_uv, tm);
         Surface f = make_surface(sc, r);
         bool is_light = has_mat<MS>(PT_MAT_DIFFUSE_LIGHT) && f.type == PT_MAT_DIFFUSE_LIGHT;
         // the isotropic medium samples lights without the hemisphere test of

@@ -16,6 +16,9 @@ struct HitRec {
 // Get_HitRecord of sphere.cu:65-93, aarect.cu:71-115, box.cu:24-33 + 56-81,
 // triangle.cu:85-149, rebuilt in object space from the object-space ray and
 // then transformed to world space.  u, v are only produced when `need_uv`.
+// KINDS / EXT as in trace_closest: only the kinds that can occur are
+// compiled in.
+template <int KINDS = 15, bool EXT = true>
 PTD HitRec hit_record(const DevScene& sc, int inst_pos, int prim, float t, v3 o, v3 d, bool need_uv_hint,
                       float time = 0.f) {
   const uint4* ir = sc.insts + (size_t)inst_pos * 8;
@@ -33,8 +36,9 @@ PTD HitRec hit_record(const DevScene& sc, int inst_pos, int prim, float t, v3 o,
   r.u = 0.f;
   r.v = 0.f;
   r.fidx = 0;
-  v3 hp, n;
-  if (kind == PT_GEOM_SPHERES) {
+  v3 hp = ro, n = mk3(0.f, 1.f, 0.f);  // every kind that occurs sets both
+  r.mat = 0;
+  if ((KINDS & 1) && kind == PT_GEOM_SPHERES) {
     float4 s = sc.sph_o[prim];
     r.mat = sc.sph_mat[prim];
     hp = ro + t * rd;
@@ -45,7 +49,7 @@ PTD HitRec hit_record(const DevScene& sc, int inst_pos, int prim, float t, v3 o,
       r.u = 1.f - (phi + PI_F) / (2.f * PI_F);
       r.v = (theta + PI_F / 2.f) / PI_F;
     }
-  } else if (kind == PT_GEOM_RECTS) {
+  } else if ((KINDS & 2) && kind == PT_GEOM_RECTS) {
     float4 A = sc.rect_o[2 * prim], B = sc.rect_o[2 * prim + 1];
     r.mat = sc.rect_mat[prim];
     int axis = (int)__float_as_uint(B.y);
@@ -62,7 +66,7 @@ PTD HitRec hit_record(const DevScene& sc, int inst_pos, int prim, float t, v3 o,
       if (need_uv_hint) { r.u = (hp.x - A.x) / (A.y - A.x); r.v = (hp.y - A.z) / (A.w - A.z); }
     }
     if (__float_as_uint(B.z)) n = -n;
-  } else if (kind == PT_GEOM_BOXES) {
+  } else if ((KINDS & 4) && kind == PT_GEOM_BOXES) {
     float4 lo = sc.box_o[2 * prim], hi = sc.box_o[2 * prim + 1];
     r.mat = sc.box_mat[prim];
     hp = ro + t * rd;
@@ -73,13 +77,13 @@ PTD HitRec hit_record(const DevScene& sc, int inst_pos, int prim, float t, v3 o,
     v3 neg = mk3(t == t0.x ? 1.f : 0.f, t == t0.y ? 1.f : 0.f, t == t0.z ? 1.f : 0.f);
     v3 pos = mk3(t == t1.x ? 1.f : 0.f, t == t1.y ? 1.f : 0.f, t == t1.z ? 1.f : 0.f);
     n = pos - neg;
-  } else if (kind == PT_GEOM_CYLINDERS) {
+  } else if (EXT && kind == PT_GEOM_CYLINDERS) {
     // cylinder.cu:70-97: radial normal, no uv
     float4 A = sc.cyl_o[2 * prim];
     r.mat = sc.cyl_mat[prim];
     hp = ro + t * rd;
     n = normalize(hp - mk3(A.x, hp.y, A.z));
-  } else if (kind == PT_GEOM_MOVING_SPHERES) {
+  } else if (EXT && kind == PT_GEOM_MOVING_SPHERES) {
     // moving_sphere.cu:78-91 with the center at the path's time, in object space
     float4 A = sc.mov_o[3 * prim], B = sc.mov_o[3 * prim + 1], C = sc.mov_o[3 * prim + 2];
     r.mat = sc.mov_mat[prim];
@@ -92,12 +96,12 @@ PTD HitRec hit_record(const DevScene& sc, int inst_pos, int prim, float t, v3 o,
       r.u = 1.f - (phi + PI_F) / (2.f * PI_F);
       r.v = (theta + PI_F / 2.f) / PI_F;
     }
-  } else if (kind == PT_GEOM_VOLUMES) {
+  } else if (EXT && kind == PT_GEOM_VOLUMES) {
     // volume_sphere.cu:88-100: the scatter point, normal +X, no uv
     r.mat = sc.vol_mat[prim];
     hp = ro + t * rd;
     n = mk3(1.f, 0.f, 0.f);
-  } else {
+  } else if (KINDS & 8) {
     const DevMesh& m = sc.meshes[ir[7].z];
     uint4 ix = sc.idx[m.ibase + prim];
     v3 a = mk3(sc.vtx[m.vbase + ix.x]), b = mk3(sc.vtx[m.vbase + ix.y]), c = mk3(sc.vtx[m.vbase + ix.z]);
