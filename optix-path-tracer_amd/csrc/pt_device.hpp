@@ -62,10 +62,12 @@ PTD v3 reflect(v3 i, v3 n) { return i - n * (2.0f * dot(n, i)); }
 PTD float comp(v3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
 
 // ------------------------------------------------------------------ RNG
-// programs/random.cuh:33-47
+// programs/random.cuh:33-47.  Fully unrolled so that every v + s0 is one add
+// of a literal: 6 VALU ops per half round, 768 per seed (the path start is
+// VALU-bound on this hash).
 PTD uint32_t tea64(uint32_t val0, uint32_t val1) {
   uint32_t v0 = val0, v1 = val1, s0 = 0;
-#pragma unroll 16
+#pragma unroll
   for (int n = 0; n < 64; n++) {
     s0 += 0x9e3779b9u;
     v0 += ((v1 << 4) + 0xa341316cu) ^ (v1 + s0) ^ ((v1 >> 5) + 0xc8013ea4u);
