@@ -215,6 +215,12 @@ PTD uint32_t slot_of(uint32_t p, uint32_t nslots) {
   return (x << 6) | (p & 63u);
 }
 
+// Throughput (w: pdf of the previous bounce) and radiance (w: path signature,
+// the FNV-1a offset basis) of every path before its first bounce; the first
+// shading pass takes them from here instead of reading them back.
+PTD float4 path_T0() { return make_float4(1.f, 1.f, 1.f, 1.f); }
+PTD float4 path_L0() { return make_float4(0.f, 0.f, 0.f, __uint_as_float(0x811C9DC5u)); }
+
 // raygen.cu:148-168 renderPixel up to the first rtTrace, raygen.cu:50-64
 // generateRay, raygen.cu:66-74 color() set-up.
 // One camera path: slot s (frame `frame` of the scene, pixel `pix`) at
@@ -241,8 +247,8 @@ PTD void start_path(const DevPaths& P, const RenderConsts& rc, uint32_t p, uint3
   uint32_t dw = s;
   if ((rc.flags & PT_FLAG_RADIANCE_CACHE) && train_hash(pix, frame) < rc.cache.train_threshold) dw |= TRAIN_BIT;
   S.d[p] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(dw));
-  S.T[p] = make_float4(1.f, 1.f, 1.f, 1.f);
-  S.L[p] = make_float4(0.f, 0.f, 0.f, __uint_as_float(0x811C9DC5u));
+  S.T[p] = path_T0();
+  S.L[p] = path_L0();
   P.result[s] = make_float4(0.f, 0.f, 0.f, __uint_as_float(rc.max_depth));
 }
 PTD void reset_segments(const DevPaths& P, uint32_t nslots, const Seg& g) {
@@ -574,7 +580,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
     float4 no = make_float4(0.f, 0.f, 0.f, 0.f), nd = no, nT = no, nL = no;
     if (j0 + __lane_id() < n) {
       float4 hit = hits[i];
-      float4 o4 = S.o[i], d4 = S.d[i], t4 = S.T[i], l4 = S.L[i];
+      float4 o4 = S.o[i], d4 = S.d[i];
+      const float4 t4 = depth == 0 ? path_T0() : S.T[i], l4 = depth == 0 ? path_L0() : S.L[i];
       v3 d = mk3(d4), T = mk3(t4), L = mk3(l4);
       uint32_t seed = __float_as_uint(o4.w), dw = __float_as_uint(d4.w);
       // a batch of scenes keeps the path's scene in the RNG state's top
