@@ -216,8 +216,11 @@ PTD uint32_t slot_of(uint32_t p, uint32_t nslots) {
 }
 
 // Throughput (w: pdf of the previous bounce) and radiance (w: path signature,
-// the FNV-1a offset basis) of every path before its first bounce; the first
-// shading pass takes them from here instead of reading them back.
+// the FNV-1a offset basis) of every path before its first bounce.  The first
+// shading pass (depth 0) takes them from here, so the path start does not
+// store them: state set 0 holds only o and d until depth 1 rewrites it
+// (max_depth >= 1; the material sort copies the unset T and L of depth 0,
+// which nothing reads).
 PTD float4 path_T0() { return make_float4(1.f, 1.f, 1.f, 1.f); }
 PTD float4 path_L0() { return make_float4(0.f, 0.f, 0.f, __uint_as_float(0x811C9DC5u)); }
 
@@ -247,8 +250,6 @@ PTD void start_path(const DevPaths& P, const RenderConsts& rc, uint32_t p, uint3
   uint32_t dw = s;
   if ((rc.flags & PT_FLAG_RADIANCE_CACHE) && train_hash(pix, frame) < rc.cache.train_threshold) dw |= TRAIN_BIT;
   S.d[p] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(dw));
-  S.T[p] = path_T0();
-  S.L[p] = path_L0();
   P.result[s] = make_float4(0.f, 0.f, 0.f, __uint_as_float(rc.max_depth));
 }
 PTD void reset_segments(const DevPaths& P, uint32_t nslots, const Seg& g) {
