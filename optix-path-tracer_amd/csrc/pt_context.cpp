@@ -1125,11 +1125,12 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
   const int matset = shade_material_set(c->mat_types);
   // one batch: camera rays, the bounce loop, the paths that outlived it
   auto enqueue_batch = [&](uint32_t nslots, int grid, bool events) {
-    launch_raygen(P, rc, nslots, grid, c->stream);
     int in = 0, out = 1;
     for (uint32_t depth = 0; depth < rp->max_depth; depth++) {
       hipEvent_t a = events ? mark() : nullptr;
-      launch_extend(c->dsc, P, in, depth == 0 ? 1e-6f : 1e-3f, c->spill, c->stats, count_trav, sort, grid, c->stream);
+      // depth 0 starts the camera paths in the same launch
+      launch_extend(c->dsc, P, in, depth == 0 ? 1e-6f : 1e-3f, c->spill, c->stats, count_trav, sort, grid, c->stream,
+                    nullptr, depth == 0 ? &rc : nullptr, depth == 0 ? nslots : 0u);
       hipEvent_t b = events ? mark() : nullptr;
       if (events) ext_ev.push_back({a, b});
       if (sort) launch_sort(P, in, grid, c->stream);

@@ -36,7 +36,6 @@ void launch_trace_closest(const DevScene& sc, const pt_ray* rays, uint64_t n, pt
                           uint2* spill, uint32_t* overflow, int grid, hipStream_t st);
 void launch_trace_occluded(const DevScene& sc, const pt_ray* rays, uint64_t n, uint8_t* occ, uint2* spill,
                            uint32_t* overflow, int grid, hipStream_t st);
-void launch_raygen(const DevPaths& P, const RenderConsts& rc, uint32_t nslots, int grid, hipStream_t st);
 // a batch of scenes: slots [item_off[k], item_off[k + 1]) are frame
 // item_frame[k] of scene item_scene[k] (tables in device memory)
 void launch_raygen_batch(const DevPaths& P, const RenderConsts* rcs, const uint32_t* item_off,
@@ -46,9 +45,13 @@ void launch_raygen_batch(const DevPaths& P, const RenderConsts* rcs, const uint3
 void launch_set_frame(uint32_t* frame_dev, uint32_t value, hipStream_t st);
 void launch_next_frames(uint32_t* frame_dev, uint32_t step, hipStream_t st);
 // `scenes` (device table, non-null for a batch of scenes): each path's
-// scene comes from its RNG state's top byte instead of `sc`
+// scene comes from its RNG state's top byte instead of `sc`.
+// `start` (single scenes, depth 0, in == 0): the launch first starts the
+// nstart camera paths of the dense state set 0 with these constants
+// (k_raygen_batch does it for a batch of scenes).
 void launch_extend(const DevScene& sc, const DevPaths& P, int in, float tmin, uint2* spill, uint64_t* stats,
-                   bool count_traversal, bool sort, int grid, hipStream_t st, const DevScene* scenes = nullptr);
+                   bool count_traversal, bool sort, int grid, hipStream_t st, const DevScene* scenes = nullptr,
+                   const RenderConsts* start = nullptr, uint32_t nstart = 0);
 void launch_sort(const DevPaths& P, int in, int grid, hipStream_t st);
 void launch_shade(const DevScene& sc, const DevPaths& P, int in, bool sorted, int out, uint32_t depth,
                   const RenderConsts& rc, int matset, bool count, uint2* spill, uint64_t* stats, int grid,

@@ -228,9 +228,13 @@ PTD float4 path_L0() { return make_float4(0.f, 0.f, 0.f, __uint_as_float(0x811C9
 // generateRay, raygen.cu:66-74 color() set-up.
 // One camera path: slot s (frame `frame` of the scene, pixel `pix`) at
 // position p of state set 0; in a batch of scenes the RNG state's top byte
-// is replaced by the scene index.
-PTD void start_path(const DevPaths& P, const RenderConsts& rc, uint32_t p, uint32_t s, uint32_t frame, uint32_t pix,
-                    bool batch, uint32_t sid) {
+// is replaced by the scene index.  Returns the stored ray and its time.
+struct PathStart {
+  float4 o, d;
+  float tm;
+};
+PTD PathStart start_path(const DevPaths& P, const RenderConsts& rc, uint32_t p, uint32_t s, uint32_t frame,
+                         uint32_t pix, bool batch, uint32_t sid) {
   const PathState S = P.st[0];
   uint32_t x = pix % rc.W, y = pix / rc.W;
   uint32_t seed = tea64(rc.W * y + x, frame);
@@ -246,30 +250,21 @@ PTD void start_path(const DevPaths& P, const RenderConsts& rc, uint32_t p, uint3
   // prd.time (raygen.cu:69); kept per slot when the scene has moving spheres
   const float tm = c.time0 + rnd(seed) * (c.time1 - c.time0);
   if (rc.store_time) P.ptime[s] = tm;
-  S.o[p] = make_float4(origin.x, origin.y, origin.z, __uint_as_float(batch ? ((seed & 0xFFFFFFu) | (sid << 24)) : seed));
+  const float4 o4 =
+      make_float4(origin.x, origin.y, origin.z, __uint_as_float(batch ? ((seed & 0xFFFFFFu) | (sid << 24)) : seed));
   uint32_t dw = s;
   if ((rc.flags & PT_FLAG_RADIANCE_CACHE) && train_hash(pix, frame) < rc.cache.train_threshold) dw |= TRAIN_BIT;
-  S.d[p] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(dw));
+  const float4 d4 = make_float4(dir.x, dir.y, dir.z, __uint_as_float(dw));
+  S.o[p] = o4;
+  S.d[p] = d4;
   P.result[s] = make_float4(0.f, 0.f, 0.f, __uint_as_float(rc.max_depth));
+  return PathStart{o4, d4, tm};
 }
 PTD void reset_segments(const DevPaths& P, uint32_t nslots, const Seg& g) {
   if (__lane_id() == 0) {
     P.seg[g.w] = dense_count(nslots, g);
     P.tseg[g.w] = 0u;
     P.qseg[g.w] = 0u;
-  }
-}
-
-// raygen.cu:148-168 renderPixel up to the first rtTrace, raygen.cu:50-64
-// generateRay, raygen.cu:66-74 color() set-up.  Position p of state set 0
-// (a dense set) starts the path of slot slot_of(p, nslots).
-__global__ __launch_bounds__(BLOCK) void k_raygen(DevPaths P, RenderConsts rc, uint32_t nslots) {
-  if (rc.frame_dev) rc.frame0 = *rc.frame_dev;  // replayed graphs take the batch's frame from memory
-  reset_segments(P, nslots, my_seg());
-  for (uint32_t p = blockIdx.x * BLOCK + threadIdx.x; p < nslots; p += gridDim.x * BLOCK) {
-    const uint32_t s = slot_of(p, nslots);
-    const uint32_t f = s / rc.WH;
-    start_path(P, rc, p, s, rc.frame0 + f, s - f * rc.WH, false, 0u);
   }
 }
 
@@ -307,14 +302,31 @@ PTD uint32_t material_key(const DevScene& sc, float4 h) {
 // material bucket of each hit is recorded and counted per wave and bucket.
 // 5 waves per SIMD (at most 96 VGPRs, no spills): latency-bound traversal
 // of large meshes gains 3.5 % over 4 waves; the Cornell box is unchanged.
-template <bool COUNT, bool SORT, bool MULTI, bool EXT, int KINDS = KIND_ALL>
+//
+// START (depth 0 of a single scene, in == 0): the query starts the nstart
+// camera paths itself (raygen.cu:148-168, 50-74): position i of the
+// dense state set 0 is slot slot_of(i, nstart), its ray is stored for the
+// shading pass and traced at once.  The path start is VALU-bound (the 64
+// round seed hash) and the traversal latency-bound, so one launch hides
+// most of the former under the latter.  A separate instantiation: the
+// path-start code costs the traversal of the other depths a few spills.
+template <bool COUNT, bool SORT, bool MULTI, bool EXT, int KINDS = KIND_ALL, bool START = false>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_extend(DevScene sc0, DevPaths P, int in, float tmin, uint2* spill,
-                                                  uint64_t* stats, const DevScene* __restrict__ scenes) {
+                                                  uint64_t* stats, const DevScene* __restrict__ scenes,
+                                                  RenderConsts rc, uint32_t nstart) {
   __shared__ uint32_t s_ref[STK_LDS * BLOCK];
   __shared__ float s_tn[STK_LDS * BLOCK];
   Stack st = make_stack(s_ref, s_tn, spill, P.counters + CTR_STACK_OVERFLOW);
   const Seg g = my_seg();
-  const uint32_t n = seg_count(P, in, g);
+  constexpr bool starts = START && !MULTI;
+  uint32_t n;
+  if (starts) {
+    if (rc.frame_dev) rc.frame0 = *rc.frame_dev;  // replayed graphs take the batch's frame from memory
+    n = dense_count(nstart, g);
+    reset_segments(P, nstart, g);
+  } else {
+    n = seg_count(P, in, g);
+  }
   const PathState S = P.st[in];
   uint32_t bc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   TraceCounts c{0, 0, 0};
@@ -322,9 +334,20 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8))) v
     const uint32_t j = j0 + __lane_id(), i = g.pos(j);
     uint32_t key = 0xFFu;
     if (j < n) {
-      float4 o = S.o[i], d = S.d[i];
+      float4 o, d;
+      float tm = 0.f;
+      if (starts) {
+        const uint32_t s = slot_of(i, nstart), f = s / rc.WH;
+        const PathStart ps = start_path(P, rc, i, s, rc.frame0 + f, s - f * rc.WH, false, 0u);
+        o = ps.o;
+        d = ps.d;
+        if (EXT && sc0.mov_instances) tm = ps.tm;
+      } else {
+        o = S.o[i];
+        d = S.d[i];
+      }
       const DevScene& sc = MULTI ? scenes[__float_as_uint(o.w) >> 24] : sc0;
-      const float tm = (EXT && sc.mov_instances) ? P.ptime[__float_as_uint(d.w) & SLOT_MASK] : 0.f;
+      if (EXT && sc.mov_instances && !starts) tm = P.ptime[__float_as_uint(d.w) & SLOT_MASK];
       HitOut h = trace_closest<COUNT, EXT, KINDS>(sc, mk3(o), mk3(d), tmin, FINF, st, c, __float_as_uint(o.w), tm);
       float4 hv = make_float4(h.t, __int_as_float(h.pos), __int_as_float(h.prim), __int_as_float(h.inst));
       P.hit[i] = hv;
@@ -1000,9 +1023,6 @@ void launch_set_frame(uint32_t* frame_dev, uint32_t value, hipStream_t st) {
 void launch_next_frames(uint32_t* frame_dev, uint32_t step, hipStream_t st) {
   hipLaunchKernelGGL(k_next_frames, dim3(1), dim3(1), 0, st, frame_dev, step);
 }
-void launch_raygen(const DevPaths& P, const RenderConsts& rc, uint32_t nslots, int grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(BLOCK), 0, st, P, rc, nslots);
-}
 void launch_raygen_batch(const DevPaths& P, const RenderConsts* rcs, const uint32_t* item_off,
                          const uint32_t* item_scene, const uint32_t* item_frame, uint32_t nitems, uint32_t nslots,
                          int grid, hipStream_t st) {
@@ -1017,29 +1037,42 @@ static int extend_kinds(uint32_t k) {
   return KIND_ALL;
 }
 void launch_extend(const DevScene& sc, const DevPaths& P, int in, float tmin, uint2* spill, uint64_t* stats,
-                   bool count_traversal, bool sort, int grid, hipStream_t st, const DevScene* scenes) {
-#define PT_EXTEND(C, S, M, E, K) \
-  hipLaunchKernelGGL((k_extend<C, S, M, E, K>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, tmin, spill, stats, scenes)
-#define PT_EXTEND4(M, E, K)                                   \
-  do {                                                        \
-    if (count_traversal && sort) PT_EXTEND(true, true, M, E, K); \
-    else if (count_traversal) PT_EXTEND(true, false, M, E, K);   \
-    else if (sort) PT_EXTEND(false, true, M, E, K);              \
-    else PT_EXTEND(false, false, M, E, K);                       \
+                   bool count_traversal, bool sort, int grid, hipStream_t st, const DevScene* scenes,
+                   const RenderConsts* start, uint32_t nstart) {
+  const RenderConsts rc = start ? *start : RenderConsts{};
+  const bool starts = start != nullptr && scenes == nullptr && nstart != 0;
+#define PT_EXTEND(C, S, M, E, K, ST)                                                                                \
+  hipLaunchKernelGGL((k_extend<C, S, M, E, K, ST>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, tmin, spill, stats, \
+                     scenes, rc, nstart)
+#define PT_EXTEND4(M, E, K, ST)                                          \
+  do {                                                                   \
+    if (count_traversal && sort) PT_EXTEND(true, true, M, E, K, ST);     \
+    else if (count_traversal) PT_EXTEND(true, false, M, E, K, ST);       \
+    else if (sort) PT_EXTEND(false, true, M, E, K, ST);                  \
+    else PT_EXTEND(false, false, M, E, K, ST);                           \
+  } while (0)
+#define PT_EXTEND_KINDS(ST)                                                                              \
+  do {                                                                                                   \
+    if (sc.ext_instances != 0) { /* volumes, cylinders, moving spheres: single scenes only */           \
+      PT_EXTEND4(false, true, KIND_ALL, ST);                                                             \
+    } else {                                                                                             \
+      switch (extend_kinds(sc.kinds)) {                                                                  \
+        case 6: PT_EXTEND4(false, false, 6, ST); break;                                                  \
+        case 10: PT_EXTEND4(false, false, 10, ST); break;                                                \
+        case 1: PT_EXTEND4(false, false, 1, ST); break;                                                  \
+        case 7: PT_EXTEND4(false, false, 7, ST); break;                                                  \
+        default: PT_EXTEND4(false, false, KIND_ALL, ST);                                                 \
+      }                                                                                                  \
+    }                                                                                                    \
   } while (0)
   if (scenes) {
-    PT_EXTEND4(true, false, KIND_ALL);
-  } else if (sc.ext_instances != 0) {  // volumes, cylinders, moving spheres: single scenes only
-    PT_EXTEND4(false, true, KIND_ALL);
+    PT_EXTEND4(true, false, KIND_ALL, false);
+  } else if (starts) {
+    PT_EXTEND_KINDS(true);
   } else {
-    switch (extend_kinds(sc.kinds)) {
-      case 6: PT_EXTEND4(false, false, 6); break;
-      case 10: PT_EXTEND4(false, false, 10); break;
-      case 1: PT_EXTEND4(false, false, 1); break;
-      case 7: PT_EXTEND4(false, false, 7); break;
-      default: PT_EXTEND4(false, false, KIND_ALL);
-    }
+    PT_EXTEND_KINDS(false);
   }
+#undef PT_EXTEND_KINDS
 #undef PT_EXTEND4
 #undef PT_EXTEND
 }
