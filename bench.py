@@ -8,14 +8,17 @@ structure and path queues resident in HBM before the timed region.
   value    = (closest-hit + shadow queries) / wall time of the timed steps
   roofline = the dominant kernel, k_shade (hit/miss programs, light sample,
              shadow-ray traversal, continuation, compaction of survivors):
-             algorithmic bytes = per shaded path 64 B state + 16 B hit read,
-             64 B survivor state or 20 B result written; per shadow ray
+             algorithmic bytes = per shaded path 64 B state (32 B at depth
+             0, whose throughput and radiance are constants) + 16 B hit
+             read, 64 B survivor state or 20 B result written; per shadow ray
              128 B per BVH4 node + record bytes per primitive + 64 B per
              instance entered (counted by a traversal-counting pass over the
              identical deterministic rays).  Material, light and texture
              tables are not counted.  Divided by the kernel's summed duration
              (HIP events on the render stream).
-  roofline_extend = the same for k_extend (32 B ray + 16 B hit + traversal).
+  roofline_extend = the same for k_extend (32 B ray + 16 B hit + traversal;
+             the depth-0 launch starts the camera paths: it writes the ray
+             instead of reading it, and 16 B of result per path).
   cpu_baseline = the scalar CPU restatement (oracle) on the host cores, on
              all 64 frames on 16 threads, in Mrays/s.
 """
@@ -316,10 +319,10 @@ def main():
     # (48 B; the one ground rect counted as a triangle) in the mesh scene
     prim_b = PRIM_BYTES[3] if meshes else PRIM_BYTES[1]
     ext, paths, exh = cst.extension_rays, cst.paths, cst.exhausted_paths
-    shade_bytes_step = (ext * (64 + 16) + (ext - paths + exh) * 64 + (paths - exh) * 20 +
+    shade_bytes_step = (ext * (64 + 16) - paths * 32 + (ext - paths + exh) * 64 + (paths - exh) * 20 +
                         cst.shadow_node_fetches * 128 + cst.shadow_prim_fetches * prim_b +
                         cst.shadow_instance_fetches * 64)
-    ext_bytes_step = (ext * (32 + 16) + cst.node_fetches * 128 + cst.prim_fetches * prim_b +
+    ext_bytes_step = (ext * (32 + 16) + paths * 16 + cst.node_fetches * 128 + cst.prim_fetches * prim_b +
                       cst.instance_fetches * 64)
     shade_ms = sum(s.shade_ms for s in steps) / len(steps)
     ext_ms = sum(s.extend_ms for s in steps) / len(steps)
@@ -346,7 +349,8 @@ def main():
                             "nodes": round(cst.shadow_node_fetches / max(1, cst.shadow_rays), 3),
                             "prims": round(cst.shadow_prim_fetches / max(1, cst.shadow_rays), 3),
                             "instances": round(cst.shadow_instance_fetches / max(1, cst.shadow_rays), 3)}})
-    roofline_extend = roof("k_extend (closest-hit BVH4 traversal)", "k_extend", ext_bytes_step, ext_ms,
+    roofline_extend = roof("k_extend (closest-hit BVH4 traversal; camera paths started at depth 0)", "k_extend",
+                           ext_bytes_step, ext_ms,
                            steps[0].extend_launches, {
                                "per_ray": {
                                    "nodes": round(cst.node_fetches / ext, 3),
