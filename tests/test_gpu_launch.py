@@ -37,11 +37,12 @@ def image(ctx, params, f0, n):
     return acc, st
 
 
-@pytest.mark.parametrize("cfg", [ptamd.PT_SCENE_CORNELL, ptamd.PT_SCENE_MESHES])
-def test_graph_replay_is_bit_identical(ctx, cfg):
-    W, H = 96, 64
+# 45x31: a ragged path count (2790 slots per two-frame batch)
+@pytest.mark.parametrize("cfg,W,H", [(ptamd.PT_SCENE_CORNELL, 96, 64), (ptamd.PT_SCENE_MESHES, 96, 64),
+                                     (ptamd.PT_SCENE_CORNELL, 45, 31)])
+def test_graph_replay_is_bit_identical(ctx, cfg, W, H):
     sc = setup(ctx, cfg, W, H, 3 if cfg == ptamd.PT_SCENE_MESHES else 0)
-    say(f"[gpu] graph replay scene {cfg}")
+    say(f"[gpu] graph replay scene {cfg} {W}x{H}")
     base = sc.render_params(frames_per_batch=2)
     ref, _ = image(ctx, base, 0, 7)
     g = sc.render_params(flags=sc.info.flags | ptamd.PT_FLAG_GRAPH, frames_per_batch=2)
