@@ -1481,7 +1481,7 @@ pt_status pt_render_batch(pt_context* const* ctxs, const pt_render_params* param
   pt_context* B = ctxs[0];
   if (n > 256) return fail(B, PT_ERR_INVALID_ARG, "at most 256 scenes per batch");
   uint32_t union_types = 0, max_depth = 0;
-  bool env = false;
+  bool env = false, ext = false;  // environment MIS; media, cylinders, moving spheres or the normal shader
   for (uint32_t i = 0; i < n; i++) {
     pt_context* c = ctxs[i];
     const pt_render_params* rp = params + i;
@@ -1495,8 +1495,7 @@ pt_status pt_render_batch(pt_context* const* ctxs, const pt_render_params* param
     if ((uint64_t)rp->width * rp->height > SLOT_MASK)
       return fail(B, PT_ERR_INVALID_ARG, "image too large for one batch");
     if (rp->flags & PT_FLAG_RADIANCE_CACHE) return fail(B, PT_ERR_UNSUPPORTED, "no radiance cache in a batch");
-    if (c->dsc.ext_instances != 0 || (c->mat_types >> PT_MAT_ISOTROPIC) != 0)
-      return fail(B, PT_ERR_UNSUPPORTED, "volumes, isotropic and normal-shader materials render with pt_render only");
+    ext = ext || c->dsc.ext_instances != 0 || (c->mat_types >> PT_MAT_ISOTROPIC) != 0;
     union_types |= c->mat_types;
     max_depth = std::max(max_depth, rp->max_depth);
     env = env || ((rp->flags & PT_FLAG_ENV_MIS) && c->dsc.num_lights != c->dsc.num_lights_plain);
@@ -1606,10 +1605,10 @@ pt_status pt_render_batch(pt_context* const* ctxs, const pt_render_params* param
     int in = 0, o = 1;
     for (uint32_t depth = 0; depth < max_depth; depth++) {
       launch_extend(B->dsc, P, in, depth == 0 ? 1e-6f : 1e-3f, B->spill, B->stats, count_trav, sort, grid, B->stream,
-                    B->d_scenes);
+                    B->d_scenes, nullptr, 0u, ext);
       if (sort) launch_sort(P, in, grid, B->stream);
       launch_shade(B->dsc, P, in, sort, o, depth, rc0, shade_material_set(union_types), count_trav, B->spill, B->stats, grid, B->stream,
-                   B->d_scenes, B->d_rcs, env);
+                   B->d_scenes, B->d_rcs, env, ext);
       std::swap(in, o);
     }
     launch_exhausted(P, in, B->stats, grid, B->stream);

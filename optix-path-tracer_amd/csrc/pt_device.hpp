@@ -213,9 +213,12 @@ PTD uint32_t mix32(uint32_t x) {
 // the intersection program (volume_sphere.cu:84, volume_box.cu:63), so the
 // draw would depend on the traversal's visit order; here xi is a hash of
 // the ray's RNG state, the volume and its instance, and the path's own
-// stream is left as it is.
+// stream is left as it is.  Only the low 24 bits of the state enter (the
+// bits every rnd() output depends on): in a batch of scenes the top byte
+// holds the scene index, and the path draws the same free flights as in a
+// render of its scene alone.
 PTD float vol_xi(uint32_t seed, uint32_t vol, uint32_t inst) {
-  const uint32_t h = mix32(seed ^ mix32(vol * 0x9E3779B9u ^ mix32(inst + 0x632BE5ABu)));
+  const uint32_t h = mix32((seed & 0xFFFFFFu) ^ mix32(vol * 0x9E3779B9u ^ mix32(inst + 0x632BE5ABu)));
   return (float)(h >> 8) * (1.0f / 16777216.0f);
 }
 constexpr float FLT_BIG = 3.40282347e+38f;

@@ -51,12 +51,12 @@ void launch_next_frames(uint32_t* frame_dev, uint32_t step, hipStream_t st);
 // (k_raygen_batch does it for a batch of scenes).
 void launch_extend(const DevScene& sc, const DevPaths& P, int in, float tmin, uint2* spill, uint64_t* stats,
                    bool count_traversal, bool sort, int grid, hipStream_t st, const DevScene* scenes = nullptr,
-                   const RenderConsts* start = nullptr, uint32_t nstart = 0);
+                   const RenderConsts* start = nullptr, uint32_t nstart = 0, bool multi_ext = false);
 void launch_sort(const DevPaths& P, int in, int grid, hipStream_t st);
 void launch_shade(const DevScene& sc, const DevPaths& P, int in, bool sorted, int out, uint32_t depth,
                   const RenderConsts& rc, int matset, bool count, uint2* spill, uint64_t* stats, int grid,
                   hipStream_t st, const DevScene* scenes = nullptr, const RenderConsts* rcs = nullptr,
-                  bool batch_env = false);
+                  bool batch_env = false, bool batch_ext = false);
 int shade_material_set(uint32_t types_mask);
 // environment importance-sampling tables of a W x H float4 map; scratch
 // holds 1 + H 64-bit words

@@ -1038,7 +1038,7 @@ static int extend_kinds(uint32_t k) {
 }
 void launch_extend(const DevScene& sc, const DevPaths& P, int in, float tmin, uint2* spill, uint64_t* stats,
                    bool count_traversal, bool sort, int grid, hipStream_t st, const DevScene* scenes,
-                   const RenderConsts* start, uint32_t nstart) {
+                   const RenderConsts* start, uint32_t nstart, bool multi_ext) {
   const RenderConsts rc = start ? *start : RenderConsts{};
   const bool starts = start != nullptr && scenes == nullptr && nstart != 0;
 #define PT_EXTEND(C, S, M, E, K, ST)                                                                                \
@@ -1053,7 +1053,7 @@ void launch_extend(const DevScene& sc, const DevPaths& P, int in, float tmin, ui
   } while (0)
 #define PT_EXTEND_KINDS(ST)                                                                              \
   do {                                                                                                   \
-    if (sc.ext_instances != 0) { /* volumes, cylinders, moving spheres: single scenes only */           \
+    if (sc.ext_instances != 0) { /* volumes, cylinders, moving spheres */                              \
       PT_EXTEND4(false, true, KIND_ALL, ST);                                                             \
     } else {                                                                                             \
       switch (extend_kinds(sc.kinds)) {                                                                  \
@@ -1065,7 +1065,9 @@ void launch_extend(const DevScene& sc, const DevPaths& P, int in, float tmin, ui
       }                                                                                                  \
     }                                                                                                    \
   } while (0)
-  if (scenes) {
+  if (scenes && multi_ext) {  // a batch holding volumes, cylinders or moving spheres
+    PT_EXTEND4(true, true, KIND_ALL, false);
+  } else if (scenes) {
     PT_EXTEND4(true, false, KIND_ALL, false);
   } else if (starts) {
     PT_EXTEND_KINDS(true);
@@ -1089,7 +1091,7 @@ static void launch_shade_v(const DevScene& sc, const DevPaths& P, int in, int sr
 }
 void launch_shade(const DevScene& sc, const DevPaths& P, int in, bool sorted, int out, uint32_t depth,
                   const RenderConsts& rc, int matset, bool count, uint2* spill, uint64_t* stats, int grid,
-                  hipStream_t st, const DevScene* scenes, const RenderConsts* rcs, bool batch_env) {
+                  hipStream_t st, const DevScene* scenes, const RenderConsts* rcs, bool batch_env, bool batch_ext) {
   const int src = sorted ? 2 : in;
   const float4* hits = sorted ? P.hit_sorted : P.hit;
   const bool env = (rc.flags & PT_FLAG_ENV_MIS) && sc.num_lights != sc.num_lights_plain;
@@ -1098,7 +1100,15 @@ void launch_shade(const DevScene& sc, const DevPaths& P, int in, bool sorted, in
   launch_shade_v<MSV, F>(sc, P, in, src, hits, out, depth, rc, spill, stats, grid, st, scenes, rcs)
 #define PT_SHADE_K(MSV, F, K) \
   launch_shade_v<MSV, F, K>(sc, P, in, src, hits, out, depth, rc, spill, stats, grid, st, scenes, rcs)
-  if (scenes) {  // a batch of scenes: every material, per-path scene
+  if (scenes && batch_ext) {  // a batch with media, cylinders, moving spheres or the normal shader
+    if (batch_env) {
+      if (count) PT_SHADE(MS_FULL, SH_MULTI | SH_EXT | SH_ENV | SH_COUNT);
+      else PT_SHADE(MS_FULL, SH_MULTI | SH_EXT | SH_ENV);
+    } else {
+      if (count) PT_SHADE(MS_FULL, SH_MULTI | SH_EXT | SH_COUNT);
+      else PT_SHADE(MS_FULL, SH_MULTI | SH_EXT);
+    }
+  } else if (scenes) {  // a batch of scenes: every material, per-path scene
     if (batch_env) {
       if (count) PT_SHADE(MS_ALL, SH_MULTI | SH_ENV | SH_COUNT);
       else PT_SHADE(MS_ALL, SH_MULTI | SH_ENV);

@@ -227,9 +227,11 @@ inline uint32_t mix32(uint32_t x) {
 // Free-flight sample of volume `vol` in instance `inst` for a ray whose path
 // RNG state is `seed`.  The reference draws rnd(prd.seed) inside the
 // intersection program (volume_sphere.cu:84), which depends on the visit
-// order; the hash leaves the path's stream untouched.
+// order; the hash leaves the path's stream untouched.  Only the low 24 bits
+// of the state enter (the bits every rnd() output depends on), so a path
+// draws the same free flights whatever the state's top byte holds.
 inline float vol_xi(uint32_t seed, uint32_t vol, uint32_t inst) {
-  const uint32_t h = mix32(seed ^ mix32(vol * 0x9E3779B9u ^ mix32(inst + 0x632BE5ABu)));
+  const uint32_t h = mix32((seed & 0xFFFFFFu) ^ mix32(vol * 0x9E3779B9u ^ mix32(inst + 0x632BE5ABu)));
   return (float)(h >> 8) * (1.0f / 16777216.0f);
 }
 // hit_boundary of volume_sphere.cu:21-55 / volume_box.cu:20-40

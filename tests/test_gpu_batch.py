@@ -64,6 +64,33 @@ def test_batch_equals_single_renders(scenes):
     assert np.array_equal(a2, alone[1][0])
 
 
+def test_batch_with_media_and_moving_spheres_equals_single_renders():
+    """Scenes with constant media, isotropic phase functions, moving spheres
+    and noise textures (The Next Week, the moving-spheres scene) batched with
+    the Cornell box: each accumulation equals its scene's own render bit for
+    bit (the media's free flights hash only the RNG state's low 24 bits, so
+    the scene index in the top byte leaves them unchanged)."""
+    made = [make(ptamd.PT_SCENE_NEXT_WEEK, 48, 48), make(ptamd.PT_SCENE_MOVING_SPHERES, 40, 27),
+            make(ptamd.PT_SCENE_CORNELL, 32, 32)]
+    try:
+        frames = [3, 2, 2]
+        alone = []
+        for (sc, ctx), f in zip(made, frames):
+            say(f"[gpu] single render {sc.info.width}x{sc.info.height} x {f}")
+            alone.append(single(sc, ctx, f))
+        say("[gpu] batch: next week, moving spheres, cornell")
+        st = ptamd.render_batch([c for _, c in made], [s.render_params() for s, _ in made], frames)
+        for (sc, ctx), (acc, n, _), f in zip(made, alone, frames):
+            got, m = ctx.read_accum(sc.info.width, sc.info.height)
+            assert m == n == f
+            assert np.array_equal(got, acc)
+        assert st.extension_rays == sum(a[2].extension_rays for a in alone)
+        assert st.shadow_rays == sum(a[2].shadow_rays for a in alone)
+    finally:
+        for _, ctx in made:
+            ctx.close()
+
+
 def test_batch_split_across_path_pools(scenes):
     """512 x 512 x 70 frames exceed one 16 Mi-path batch: the work list is cut
     inside the large scene's frames and the small scenes follow."""
