@@ -641,7 +641,8 @@ def test_many_instances_match_oracle(ctx):
 
 def test_extended_kinds_rejected(ctx):
     """Malformed media, cylinders and moving spheres are refused by
-    pt_set_scene with an error; a well-formed media scene renders."""
+    pt_set_scene with an error; a well-formed media scene renders, and the
+    radiance cache refuses it."""
     def scene(**kw):
         s = PyScene()
         m = s.lambert((0.5, 0.5, 0.5))
@@ -674,6 +675,15 @@ def test_extended_kinds_rejected(ctx):
     gpu_scene(ctx, good.desc())
     ctx.set_camera(sc.camera)
     ctx.render(ptamd.RenderParams(16, 16, 4, 1, 3, 1, 0, 0), 0, 1)
+    own = ptamd.Context(0)
+    try:
+        gpu_scene(own, good.desc())
+        own.set_camera(sc.camera)
+        own.cache_init(9)
+        with pytest.raises(ptamd.PtError):
+            own.render(ptamd.RenderParams(16, 16, 4, 1, 3, 1, ptamd.PT_FLAG_RADIANCE_CACHE, 0), 0, 1)
+    finally:
+        own.close()
 
 
 def test_vector_texture_paths_match_oracle(ctx):
