@@ -257,7 +257,7 @@ PTD float isect_volume(float4 A, float4 B, v3 o, v3 d, float tmin, float xi) {
   const bool box = (__float_as_uint(B.w) >> 31) != 0;
   float r1 = 0.f, r2 = 0.f;
   if (!vol_boundary(A, B, box, o, d, -FLT_BIG, FLT_BIG, r1)) return FINF;
-  if (!vol_boundary(A, B, box, o, d, (float)((double)r1 + 0.0001), FLT_BIG, r2)) return FINF;
+  if (!vol_boundary(A, B, box, o, d, r1 + 0.0001f, FLT_BIG, r2)) return FINF;
   if (r1 < tmin) r1 = tmin;
   if (r1 >= r2) return FINF;
   if (r1 < 0.f) r1 = 0.f;
@@ -300,6 +300,26 @@ struct TriRay {
   int kx, ky, kz;
   float Sx, Sy, Sz;
 };
+// a * b = hi + lo exactly (Dekker's product over Veltkamp's 12-bit split; no
+// fused multiply-add)
+PTD void two_prod(float a, float b, float& hi, float& lo) {
+  const float sa = a * 4097.0f, sb = b * 4097.0f;
+  const float ah = sa - (sa - a), al = a - ah;
+  const float bh = sb - (sb - b), bl = b - bh;
+  hi = a * b;
+  lo = (((ah * bh - hi) + ah * bl) + al * bh) + al * bl;
+}
+// a * b - c * d with the exact sign and a nearly correctly rounded value: the
+// edge functions of the watertight test when one of them rounds to zero
+// (Woop et al. 2013 recompute them at higher precision there)
+PTD float diff_of_products(float a, float b, float c, float d) {
+  float h1, l1, h2, l2;
+  two_prod(a, b, h1, l1);
+  two_prod(c, d, h2, l2);
+  const float s = h1 - h2, z = s - h1;
+  const float t = (h1 - (s - z)) + (-h2 - z);
+  return s + (t + (l1 - l2));
+}
 PTD TriRay tri_setup(v3 d) {
   TriRay r;
   float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
@@ -324,12 +344,9 @@ PTD bool tri_test(const TriRay& r, v3 o, v3 v0, v3 v1, v3 v2, float& t, float& b
   float V = Ax * Cy - Ay * Cx;
   float W = Bx * Ay - By * Ax;
   if (U == 0.0f || V == 0.0f || W == 0.0f) {
-    double CxBy = (double)Cx * (double)By, CyBx = (double)Cy * (double)Bx;
-    U = (float)(CxBy - CyBx);
-    double AxCy = (double)Ax * (double)Cy, AyCx = (double)Ay * (double)Cx;
-    V = (float)(AxCy - AyCx);
-    double BxAy = (double)Bx * (double)Ay, ByAx = (double)By * (double)Ax;
-    W = (float)(BxAy - ByAx);
+    U = diff_of_products(Cx, By, Cy, Bx);
+    V = diff_of_products(Ax, Cy, Ay, Cx);
+    W = diff_of_products(Bx, Ay, By, Ax);
   }
   if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
   float det = U + V + W;
@@ -394,7 +411,7 @@ PTD v3 GGX_Sample(v3 origin, float rx, float ry, float nu, float nv) {
   float cosTheta = so.y;
   if (cosTheta > 0.9999f) {
     float r = sqrtf(rx / (1 - rx));
-    float phi = (float)(6.28318530718 * (double)ry);
+    float phi = 6.28318530718f * ry;
     slopeX = r * pcos(phi);
     slopeY = r * psin(phi);
   } else {
@@ -477,7 +494,7 @@ PTD v3 brdf_sample(const Surface& f, v3 Wo, v3 N, uint32_t& seed) {
   float ry = rnd(seed);
   v3 H;
   if (has_mat<MS>(PT_MAT_ASHIKHMIN_SHIRLEY) && (!has_mat<MS>(PT_MAT_TORRANCE_SPARROW) || f.type == PT_MAT_ASHIKHMIN_SHIRLEY)) {
-    if ((double)rx < 0.5) {
+    if (rx < 0.5f) {
       rx = fminf(2 * rx, 1.f - 1e-6f);
       return onb_to_world(Nn, cosine_sample_hemisphere(rx, ry));
     }
@@ -521,7 +538,7 @@ PTD v3 brdf_eval(const Surface& f, v3 Wo, v3 Wi, v3 N, float& pdf) {
       v3 WiN = normalize(Wi);
       float sI = SinTheta(WiN), sO = SinTheta(Wo);
       float maxCos = 0;
-      if ((double)sI > 1e-4 && (double)sO > 1e-4) {
+      if (sI > 1e-4f && sO > 1e-4f) {
         float sPI = SinPhi(WiN), cPI = CosPhi(WiN);
         float sPO = SinPhi(Wo), cPO = CosPhi(Wo);
         float dCos = cPI * cPO + sPI * sPO;

@@ -54,6 +54,15 @@ PTD unsigned long long quantize_weight(double w, double mx, double scale) {
   return (mx > 0.0) ? (unsigned long long)(w * scale) : 0ull;
 }
 
+// Bits per quantised texel weight: 40, fewer for maps of more than 2^23
+// texels, so that the total of W * H weights stays below 2^63.
+PTD int env_weight_bits(uint32_t W, uint32_t H) {
+  const unsigned long long n = (unsigned long long)W * H;
+  int lg = 0;
+  while (lg < 63 && (1ull << lg) < n) lg++;
+  return min(40, 63 - lg);
+}
+
 // inclusive scan of one value per thread over the block (Hillis-Steele in LDS)
 PTD unsigned long long block_inclusive_scan(unsigned long long v, unsigned long long* lds) {
   lds[threadIdx.x] = v;
@@ -75,7 +84,7 @@ __global__ __launch_bounds__(EB) void k_env_rows(const float4* tex, uint32_t W, 
   const double mx = __longlong_as_double((long long)*max_bits);
   int ex = 0;
   (void)frexp(mx, &ex);
-  const double scale = ldexp(1.0, 40 - ex);
+  const double scale = ldexp(1.0, env_weight_bits(W, H) - ex);
   const uint32_t chunk = (W + EB - 1) / EB;
   const uint32_t b = threadIdx.x * chunk, e = min(W, b + chunk);
   const float4* row = tex + (size_t)j * W;

@@ -9,8 +9,8 @@
 //     atan(a) = pi/6 + atan((a*sqrt3 - 1) / (a + sqrt3)) and the odd Taylor
 //     series to t^11 (Cody and Waite, "Software Manual for the Elementary
 //     Functions", 1980);
-//   pow: exp(y * ln x) in double precision (ln by the atanh series of
-//     (m - 1) / (m + 1), exp by reduction modulo ln 2 and Taylor);
+//   pow: 2^(y log2 x) in float (ln by the atanh series of (m - 1) / (m + 1),
+//     2^r by reduction to |r| <= 1/2 and Taylor, then exact scaling);
 //   pow5(x) = x^5 by products.
 #pragma once
 
@@ -76,43 +76,57 @@ PTL float patan2(float y, float x) {
 PTL float pasin(float x) { return patan2(x, sqrtf((1.0f - x) * (1.0f + x))); }
 PTL float pacos(float x) { return patan2(sqrtf((1.0f - x) * (1.0f + x)), x); }
 
-PTL double pln(float x) {
+// ln(m) of x = m 2^e, m in [sqrt(1/2), sqrt(2)): 2 atanh(s), s = (m - 1) /
+// (m + 1), by the odd series to s^11 (|s| <= 0.1716: the next term is below
+// 2^-30 relative); e returned
+PTL float pln_m(float x, int& e) {
   uint32_t b = __float_as_uint(x);
-  int e = 0;
-  if (b < 0x00800000u) {
+  e = 0;
+  if (b < 0x00800000u) {  // subnormal: scale by 2^24
     x = x * 16777216.0f;
     b = __float_as_uint(x);
     e = -24;
   }
   e += (int)((b >> 23) & 0xFFu) - 127;
-  double m = (double)__uint_as_float((b & 0x007FFFFFu) | 0x3F800000u);
-  if (m > 1.4142135623730951) {
-    m = m * 0.5;
+  float m = __uint_as_float((b & 0x007FFFFFu) | 0x3F800000u);
+  if (m > 0x1.6a09e6p+0f) {
+    m = m * 0.5f;
     e += 1;
   }
-  double s = (m - 1.0) / (m + 1.0);
-  double s2 = s * s;
-  double p = s2 * (1.0 / 3.0 + s2 * (1.0 / 5.0 + s2 * (1.0 / 7.0 + s2 * (1.0 / 9.0 + s2 * (1.0 / 11.0 + s2 * (1.0 / 13.0))))));
-  return (double)e * 0x1.62e42fefa39efp-1 + 2.0 * (s + s * p);
+  float s = (m - 1.0f) / (m + 1.0f);
+  float s2 = s * s;
+  float p = s2 * (0x1.555556p-2f + s2 * (0x1.99999ap-3f + s2 * (0x1.24924ap-3f + s2 * (0x1.c71c72p-4f + s2 * 0x1.745d18p-4f))));
+  return 2.0f * (s + s * p);
 }
-PTL float pexp_d(double z) {
-  if (z > 89.0) return __builtin_huge_valf();
-  if (z < -104.0) return 0.0f;
-  double k = floor(z * 0x1.71547652b82fep+0 + 0.5);
-  double r = z - k * 0x1.62e42fefa39efp-1;
-  double p = 1.0 + r * (1.0 + r * (1.0 / 2.0 + r * (1.0 / 6.0 + r * (1.0 / 24.0 + r * (1.0 / 120.0 + r * (1.0 / 720.0 + r * (1.0 / 5040.0 + r * (1.0 / 40320.0 + r * (1.0 / 362880.0 + r * (1.0 / 3628800.0))))))))));
-  return (float)ldexp(p, (int)k);
+// natural log of a positive finite float: e ln2 with ln2 split in a 16-bit
+// head (e * head exact) and its tail
+PTL float pln(float x) {
+  int e;
+  float lm = pln_m(x, e);
+  return (float)e * 0x1.62e4p-1f + ((float)e * 0x1.7f7d1cp-20f + lm);
 }
+// 2^z: z = k + r, |r| <= 1/2, e^(r ln 2) by Taylor to degree 8, scaled by 2^k
+PTL float pexp2(float z) {
+  if (z > 128.0f) return __builtin_huge_valf();
+  if (z < -151.0f) return 0.0f;
+  float k = floorf(z + 0.5f);
+  float t = (z - k) * 0x1.62e430p-1f;
+  float q = 1.0f + t * (1.0f + t * (0x1p-1f + t * (0x1.555556p-3f + t * (0x1.555556p-5f + t * (0x1.111112p-7f + t * (0x1.6c16c2p-10f + t * (0x1.a01a02p-13f + t * 0x1.a01a02p-16f)))))));
+  return ldexpf(q, (int)k);
+}
+// x^y = 2^(y log2 x), all in float
 PTL float ppow(float x, float y) {
   if (x != x || y != y) return x + y;
   if (y == 0.0f || x == 1.0f) return 1.0f;
   if (x < 0.0f) return __builtin_nanf("");
   if (x == 0.0f) return y > 0.0f ? 0.0f : __builtin_huge_valf();
   if (__builtin_isinf(x)) return y > 0.0f ? x : 0.0f;
-  return pexp_d((double)y * pln(x));
+  int e;
+  float lm = pln_m(x, e);
+  return pexp2(y * ((float)e + lm * 0x1.715476p+0f));
 }
 PTL float ppow5(float x) { return (x * x) * (x * x) * x; }
-// natural log in float through pln; -inf at 0
-PTL float plogf(float x) { return x > 0.0f ? (float)pln(x) : -__builtin_huge_valf(); }
+// natural log; -inf at 0
+PTL float plogf(float x) { return x > 0.0f ? pln(x) : -__builtin_huge_valf(); }
 
 }  // namespace pt

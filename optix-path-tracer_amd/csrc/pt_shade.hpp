@@ -295,8 +295,8 @@ PTD float env_pdf(const DevScene& sc, v3 d) {
   unsigned long long w = row[i] - (i ? row[i - 1] : 0ull);
   float st = psin(v * PI_F);
   if (w == 0 || !(st > 0.f)) return 0.f;
-  double p = (double)w / (double)sc.env_total * (double)sc.env_w * (double)sc.env_h;
-  return (float)(p / (2.0 * 3.14159265358979323846 * 3.14159265358979323846 * (double)st));
+  const float p = ((float)w / (float)sc.env_total) * ((float)sc.env_w * (float)sc.env_h);
+  return p / ((2.0f * PI_F * PI_F) * st);
 }
 // two draws: row from the marginal table, column from the row's table,
 // continuous offsets inside the texel
@@ -312,8 +312,9 @@ PTD v3 env_sample(const DevScene& sc, uint32_t& seed) {
   unsigned long long tc = scale24(rw, r2);
   uint32_t i = upper_index(row, sc.env_w, tc);
   unsigned long long cprev = i ? row[i - 1] : 0ull;
-  float fu = (float)((double)(tc - cprev) / (double)(row[i] - cprev));
-  float fv = (float)((double)(tr - rprev) / (double)rw);
+  // offsets inside the texel, below 1 (a ratio of two roundings may reach it)
+  const float fu = fminf((float)(tc - cprev) / (float)(row[i] - cprev), 0x1.fffffep-1f);
+  const float fv = fminf((float)(tr - rprev) / (float)rw, 0x1.fffffep-1f);
   float u = ((float)i + fu) / (float)sc.env_w;
   float v = ((float)j + fv) / (float)sc.env_h;
   float lon = 2.f * PI_F * u, lat = PI_F * v;

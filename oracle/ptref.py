@@ -46,10 +46,20 @@ def load():
                                          C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint64, C.c_int32,
                                          P, P, P, P, P]),
             "ptref_display": (None, [P, C.c_uint64, C.c_uint32, P]),
+            "ptref_render_pixels": (C.c_int32, [P, C.POINTER(ptamd.Camera), C.POINTER(ptamd.RenderParams),
+                                                C.c_uint32, C.c_uint32, P, C.c_uint64, C.c_int32, P, P]),
+            "ptref64_last_error": (C.c_char_p, []),
+            "ptref64_scene_create": (P, [C.POINTER(ptamd.SceneDesc)]),
+            "ptref64_scene_destroy": (None, [P]),
+            "ptref64_render_pixels": (C.c_int32, [P, C.POINTER(ptamd.Camera), C.POINTER(ptamd.RenderParams),
+                                                  C.c_uint32, C.c_uint32, P, C.c_uint64, C.c_int32, P, P]),
             "ptref_brdf_sample_eval": (None, [C.c_int32, P, P, C.c_float, C.c_float, P, P, P, P, P,
                                               P]),
             "ptref_brdf_eval": (None, [C.c_int32, P, P, C.c_float, C.c_float, P, P, P, P, P]),
             "ptref_cosine_sample": (None, [C.c_float, C.c_float, P, P]),
+            "ptref_brdf_sample_many": (None, [C.c_int32, P, P, C.c_float, C.c_float, P, P, C.c_uint32,
+                                              C.c_uint64, P, P, P, P]),
+            "ptref_metal_scatter_many": (None, [P, P, C.c_float, C.c_uint32, C.c_uint64, P]),
             "ptref_light_sample": (None, [P, C.c_int32, P, P, P, P]),
             "ptref_light_pdf": (C.c_float, [P, C.c_int32, P, P]),
             "ptref_miss": (None, [P, P, P]),
@@ -93,17 +103,43 @@ def rnd_sequence(seed, n):
 
 
 class OracleScene:
-    def __init__(self, desc):
+    """The oracle's copy of a scene: float (fp64=False, the GPU's arithmetic)
+    or its double build (fp64=True: converged references; only render_pixels
+    is available there)."""
+
+    def __init__(self, desc, fp64=False):
         lib = load()
         self.lib = lib
-        self.h = lib.ptref_scene_create(C.byref(desc))
+        self.fp64 = fp64
+        if fp64:
+            self.h = lib.ptref64_scene_create(C.byref(desc))
+            err = lib.ptref64_last_error
+        else:
+            self.h = lib.ptref_scene_create(C.byref(desc))
+            err = lib.ptref_last_error
         if not self.h:
-            raise RuntimeError("oracle scene: " + lib.ptref_last_error().decode())
+            raise RuntimeError("oracle scene: " + err().decode())
 
     def close(self):
         if self.h:
-            self.lib.ptref_scene_destroy(self.h)
+            (self.lib.ptref64_scene_destroy if self.fp64 else self.lib.ptref_scene_destroy)(self.h)
             self.h = None
+
+    def render_pixels(self, cam, params, frame_begin, frame_count, pix, threads=8, sums=None, stats=None):
+        """Sum of de_nan(color()) over frames [frame_begin, +frame_count) for
+        each listed pixel (launch index x + W*y): (len(pix), 3) float64."""
+        pix = np.ascontiguousarray(pix, np.uint32)
+        if sums is None:
+            sums = np.zeros((len(pix), 3), np.float64)
+        if stats is None:
+            stats = np.zeros(3, np.uint64)
+        fn = self.lib.ptref64_render_pixels if self.fp64 else self.lib.ptref_render_pixels
+        st = fn(self.h, C.byref(cam), C.byref(params), frame_begin, frame_count, p(pix), len(pix), threads,
+                p(sums), p(stats))
+        if st != 0:
+            err = self.lib.ptref64_last_error if self.fp64 else self.lib.ptref_last_error
+            raise RuntimeError("oracle render: " + err().decode())
+        return sums, stats
 
     def __del__(self):
         try:
@@ -208,6 +244,24 @@ def brdf_sample_eval(mtype, c0, c1, p0, p1, wo, n, seed):
     load().ptref_brdf_sample_eval(mtype, p(f3(c0)), p(f3(c1)), p0, p1, p(f3(wo)), p(f3(n)), C.byref(s),
                                   p(wi), p(f), p(pdf))
     return wi, f, float(pdf[0]), s.value
+
+
+def brdf_sample_many(mtype, c0, c1, p0, p1, wo, n, seed0, count):
+    """count samples (sample k seeded with tea(k, seed0, 8)): Wi, f, pdf and
+    the first rnd() each sample draws."""
+    wi = np.zeros((count, 3), np.float32)
+    f = np.zeros((count, 3), np.float32)
+    pdf = np.zeros(count, np.float32)
+    u0 = np.zeros(count, np.float32)
+    load().ptref_brdf_sample_many(mtype, p(f3(c0)), p(f3(c1)), p0, p1, p(f3(wo)), p(f3(n)), seed0, count,
+                                  p(wi), p(f), p(pdf), p(u0))
+    return wi, f, pdf, u0
+
+
+def metal_scatter_many(wo, n, fuzz, seed0, count):
+    d = np.zeros((count, 3), np.float32)
+    load().ptref_metal_scatter_many(p(f3(wo)), p(f3(n)), fuzz, seed0, count, p(d))
+    return d
 
 
 def brdf_eval(mtype, c0, c1, p0, p1, wo, wi, n):

@@ -13,7 +13,7 @@
 #include <thread>
 #include <vector>
 
-#include "ptref_render.hpp"
+#include "ptref_pixels.hpp"
 
 using namespace ptref;
 
@@ -195,6 +195,21 @@ int32_t ptref_render(const void* sp, const pt_camera* cam, const pt_render_param
   }
 }
 
+// The listed pixels (launch index) over frames [frame_begin, +frame_count):
+// sum[3k..3k+2] += sum of de_nan(color()) of pixel pix[k].
+int32_t ptref_render_pixels(const void* sp, const pt_camera* cam, const pt_render_params* rp, uint32_t frame_begin,
+                            uint32_t frame_count, const uint32_t* pix, uint64_t npix, int32_t nthreads, double* sum,
+                            uint64_t* stats) {
+  try {
+    render_pixel_list(*static_cast<const Scene*>(sp), *cam, *rp, frame_begin, frame_count, pix, npix, nthreads, sum,
+                      stats);
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
 // Display mapping of an accumulation (raygen.cu:136-146): rgba W*H*4.
 void ptref_display(const double* accum, uint64_t npix, uint32_t frames, uint8_t* rgba) {
   for (uint64_t i = 0; i < npix; i++) {
@@ -223,6 +238,35 @@ void ptref_brdf_eval(int32_t type, const float* c0, const float* c1, float p0, f
   f3 fv = brdf_eval(s, mk(wo[0], wo[1], wo[2]), mk(wi[0], wi[1], wi[2]), mk(n[0], n[1], n[2]), pd);
   f[0] = fv.x; f[1] = fv.y; f[2] = fv.z;
   *pdf = pd;
+}
+// `count` samples of brdf_sample + brdf_eval, sample k seeded with
+// tea(k, seed0, 8); u0[k] = the first rnd() that sample draws
+void ptref_brdf_sample_many(int32_t type, const float* c0, const float* c1, float p0, float p1, const float* wo,
+                            const float* n, uint32_t seed0, uint64_t count, float* wi, float* f, float* pdf,
+                            float* u0) {
+  Surface s{type, mk(c0[0], c0[1], c0[2]), mk(c1[0], c1[1], c1[2]), p0, p1};
+  f3 Wo = mk(wo[0], wo[1], wo[2]), N = mk(n[0], n[1], n[2]);
+  for (uint64_t k = 0; k < count; k++) {
+    uint32_t seed = tea((uint32_t)k, seed0, 8);
+    uint32_t peek = seed;
+    u0[k] = rnd(peek);
+    f3 Wi = brdf_sample(s, Wo, N, seed);
+    float pd = 1.f;
+    f3 fv = brdf_eval(s, Wo, Wi, N, pd);
+    wi[3 * k] = Wi.x; wi[3 * k + 1] = Wi.y; wi[3 * k + 2] = Wi.z;
+    f[3 * k] = fv.x; f[3 * k + 1] = fv.y; f[3 * k + 2] = fv.z;
+    pdf[k] = pd;
+  }
+}
+// `count` metal scatter directions (seeds as above)
+void ptref_metal_scatter_many(const float* wo, const float* n, float fuzz, uint32_t seed0, uint64_t count,
+                              float* dir) {
+  f3 Wo = mk(wo[0], wo[1], wo[2]), N = mk(n[0], n[1], n[2]);
+  for (uint64_t k = 0; k < count; k++) {
+    uint32_t seed = tea((uint32_t)k, seed0, 8);
+    f3 d = metal_scatter(Wo, N, fuzz, seed);
+    dir[3 * k] = d.x; dir[3 * k + 1] = d.y; dir[3 * k + 2] = d.z;
+  }
 }
 // cosine hemisphere sample about n: out = world direction
 void ptref_cosine_sample(float u1, float u2, const float* n, float* out) {

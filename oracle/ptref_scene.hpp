@@ -25,11 +25,11 @@
 
 namespace ptref {
 
-constexpr float INF = std::numeric_limits<float>::infinity();
+constexpr Real INF = std::numeric_limits<Real>::infinity();
 constexpr int BLAS_LEAF_MAX = 4;
 constexpr int TLAS_LEAF_MAX = 1;
 // Conservative slab test: tfar is scaled by 1 + 2^-22 before the compare.
-constexpr float SLAB_FAR_SCALE = 1.00000024f;
+constexpr Real SLAB_FAR_SCALE = 1.00000024f;
 
 struct Aabb {
   f3 lo{INF, INF, INF}, hi{-INF, -INF, -INF};
@@ -41,42 +41,42 @@ inline Aabb merge(const Aabb& a, const Aabb& b) {
   return r;
 }
 // half surface area, ((dx*dy) + (dy*dz)) + (dz*dx)
-inline float half_area(const Aabb& b) {
-  float dx = b.hi.x - b.lo.x, dy = b.hi.y - b.lo.y, dz = b.hi.z - b.lo.z;
+inline Real half_area(const Aabb& b) {
+  Real dx = b.hi.x - b.lo.x, dy = b.hi.y - b.lo.y, dz = b.hi.z - b.lo.z;
   return dx * dy + dy * dz + dz * dx;
 }
 
 // ------------------------------------------------------------ transforms
 // Row-major 3x4 affine transform as uploaded by Transform::setMatrix
 // (transforms.hpp:65); rtTransformPoint/Vector/Normal restated.
-inline f3 xform_point(const float* m, f3 p) {
+inline f3 xform_point(const Real* m, f3 p) {
   return mk(m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3],
             m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7],
             m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11]);
 }
-inline f3 xform_vector(const float* m, f3 v) {
+inline f3 xform_vector(const Real* m, f3 v) {
   return mk(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[4] * v.x + m[5] * v.y + m[6] * v.z,
             m[8] * v.x + m[9] * v.y + m[10] * v.z);
 }
 // normal: transpose of the world-to-object 3x3 (inverse transpose of M)
-inline f3 xform_normal(const float* w2o, f3 n) {
+inline f3 xform_normal(const Real* w2o, f3 n) {
   return mk(w2o[0] * n.x + w2o[4] * n.y + w2o[8] * n.z, w2o[1] * n.x + w2o[5] * n.y + w2o[9] * n.z,
             w2o[2] * n.x + w2o[6] * n.y + w2o[10] * n.z);
 }
 
 // ------------------------------------------------------------ primitives
-struct Sphere { f3 c; float r; int mat; };
-struct Rect { int axis, flip; float a0, a1, b0, b1, k; int mat; };
+struct Sphere { f3 c; Real r; int mat; };
+struct Rect { int axis, flip; Real a0, a1, b0, b1, k; int mat; };
 struct Box { f3 lo, hi; int mat; };
 // hitables.hpp:175-209 / :316-349: shape PT_VOLUME_SPHERE (a = center,
 // b.x = radius) or PT_VOLUME_BOX (a = lo, b = hi)
-struct Volume { int shape; f3 a, b; float density; int mat; };
+struct Volume { int shape; f3 a, b; Real density; int mat; };
 // hitables.hpp:411-454: lateral surface around the vertical through o
-struct Cylinder { f3 o; float L, R; int mat; };
+struct Cylinder { f3 o; Real L, R; int mat; };
 // hitables.hpp:131-172
-struct MovingSphere { f3 c0, c1; float r, t0, t1; int mat; };
+struct MovingSphere { f3 c0, c1; Real r, t0, t1; int mat; };
 // moving_sphere.cu center()
-inline f3 moving_center(const MovingSphere& m, float time) {
+inline f3 moving_center(const MovingSphere& m, Real time) {
   return m.c0 + ((time - m.t0) / (m.t1 - m.t0)) * (m.c1 - m.c0);
 }
 struct Mesh {
@@ -152,8 +152,8 @@ inline Aabb bounds_tri(f3 a, f3 bb, f3 c) {
 // zero direction component then never meets 0 * inf in the slab test, so
 // the structure never culls a primitive whose intersection program accepts
 // points on its (inclusive) bounds.
-inline float widen_lo(float x) { return x - (std::fabs(x) * 0x1p-23f + 0x1p-126f); }
-inline float widen_hi(float x) { return x + (std::fabs(x) * 0x1p-23f + 0x1p-126f); }
+inline Real widen_lo(Real x) { return x - (std::fabs(x) * 0x1p-23f + 0x1p-126f); }
+inline Real widen_hi(Real x) { return x + (std::fabs(x) * 0x1p-23f + 0x1p-126f); }
 inline Aabb widen(const Aabb& b) {
   Aabb r;
   r.lo = mk(widen_lo(b.lo.x), widen_lo(b.lo.y), widen_lo(b.lo.z));
@@ -164,10 +164,10 @@ inline Aabb widen(const Aabb& b) {
 // Candidate acceptance shared by every intersector: tmin < t and either
 // t < best or (t == best and the (instance, primitive) key is smaller).
 struct HitState {
-  float t;
+  Real t;
   int inst = -1, prim = -1;
   bool found() const { return inst >= 0; }
-  bool offer(float tc, float tmin, int ci, int cp) {
+  bool offer(Real tc, Real tmin, int ci, int cp) {
     if (!(tc > tmin)) return false;
     if (tc < t || (found() && tc == t && (ci < inst || (ci == inst && cp < prim)))) {
       t = tc; inst = ci; prim = cp;
@@ -178,14 +178,14 @@ struct HitState {
 };
 
 // sphere.cu:32-62 -- both roots offered in order
-inline void isect_sphere(const Sphere& s, f3 o, f3 d, float tmin, HitState& h, int inst, int prim) {
+inline void isect_sphere(const Sphere& s, f3 o, f3 d, Real tmin, HitState& h, int inst, int prim) {
   f3 oc = o - s.c;
-  float a = dot(d, d);
-  float b = dot(oc, d);
-  float c = dot(oc, oc) - s.r * s.r;
-  float disc = b * b - a * c;
+  Real a = dot(d, d);
+  Real b = dot(oc, d);
+  Real c = dot(oc, oc) - s.r * s.r;
+  Real disc = b * b - a * c;
   if (disc < 0.f) return;
-  float t = (-b - std::sqrt(disc)) / a;
+  Real t = (-b - std::sqrt(disc)) / a;
   h.offer(t, tmin, inst, prim);
   t = (-b + std::sqrt(disc)) / a;
   h.offer(t, tmin, inst, prim);
@@ -195,8 +195,8 @@ inline void isect_sphere(const Sphere& s, f3 o, f3 d, float tmin, HitState& h, i
 // reciprocal direction `inv` (1 / d per component, once per instance)
 // where the reference divides (it was compiled with --use_fast_math, whose
 // division is approximate anyway).
-inline void isect_rect(const Rect& r, f3 o, f3 d, f3 inv, float tmin, HitState& h, int inst, int prim) {
-  float t, a, b;
+inline void isect_rect(const Rect& r, f3 o, f3 d, f3 inv, Real tmin, HitState& h, int inst, int prim) {
+  Real t, a, b;
   if (r.axis == PT_AXIS_X) { t = (r.k - o.x) * inv.x; a = o.y + t * d.y; b = o.z + t * d.z; }
   else if (r.axis == PT_AXIS_Y) { t = (r.k - o.y) * inv.y; a = o.x + t * d.x; b = o.z + t * d.z; }
   else { t = (r.k - o.z) * inv.z; a = o.x + t * d.x; b = o.y + t * d.y; }
@@ -205,11 +205,11 @@ inline void isect_rect(const Rect& r, f3 o, f3 d, f3 inv, float tmin, HitState& 
 }
 // box.cu:36-53 -- slab test, tmin offered first, tmax only if tmin rejected
 inline f3 mulv(f3 a, f3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
-inline void isect_box(const Box& x, f3 o, f3 inv, float tmin, HitState& h, int inst, int prim) {
+inline void isect_box(const Box& x, f3 o, f3 inv, Real tmin, HitState& h, int inst, int prim) {
   f3 t0 = mulv(x.lo - o, inv);
   f3 t1 = mulv(x.hi - o, inv);
-  float tn = max_component(min_vec(t0, t1));
-  float tf = min_component(max_vec(t0, t1));
+  Real tn = max_component(min_vec(t0, t1));
+  Real tf = min_component(max_vec(t0, t1));
   if (tn <= tf) {
     if (!h.offer(tn, tmin, inst, prim)) h.offer(tf, tmin, inst, prim);
   }
@@ -230,29 +230,29 @@ inline uint32_t mix32(uint32_t x) {
 // order; the hash leaves the path's stream untouched.  Only the low 24 bits
 // of the state enter (the bits every rnd() output depends on), so a path
 // draws the same free flights whatever the state's top byte holds.
-inline float vol_xi(uint32_t seed, uint32_t vol, uint32_t inst) {
+inline Real vol_xi(uint32_t seed, uint32_t vol, uint32_t inst) {
   const uint32_t h = mix32((seed & 0xFFFFFFu) ^ mix32(vol * 0x9E3779B9u ^ mix32(inst + 0x632BE5ABu)));
-  return (float)(h >> 8) * (1.0f / 16777216.0f);
+  return (Real)(h >> 8) * (1.0f / 16777216.0f);
 }
 // hit_boundary of volume_sphere.cu:21-55 / volume_box.cu:20-40
-inline bool vol_boundary(const Volume& v, f3 o, f3 d, float lo, float hi, float& rec) {
+inline bool vol_boundary(const Volume& v, f3 o, f3 d, Real lo, Real hi, Real& rec) {
   if (v.shape == PT_VOLUME_BOX) {
     f3 t0 = (v.a - o) / d;
     f3 t1 = (v.b - o) / d;
-    float ta = max_component(min_vec(t0, t1));
-    float tb = min_component(max_vec(t0, t1));
+    Real ta = max_component(min_vec(t0, t1));
+    Real tb = min_component(max_vec(t0, t1));
     if (ta > tb) return false;
     if (ta < hi && ta > lo) { rec = ta; return true; }
     if (tb < hi && tb > lo) { rec = tb; return true; }
     return false;
   }
   f3 oc = o - v.a;
-  float a = dot(d, d);
-  float b = dot(oc, d);
-  float c = dot(oc, oc) - v.b.x * v.b.x;
-  float disc = b * b - a * c;
+  Real a = dot(d, d);
+  Real b = dot(oc, d);
+  Real c = dot(oc, oc) - v.b.x * v.b.x;
+  Real disc = b * b - a * c;
   if (disc < 0.f) return false;
-  float temp = (-b - std::sqrt(disc)) / a;
+  Real temp = (-b - std::sqrt(disc)) / a;
   if (temp < hi && temp > lo) { rec = temp; return true; }
   temp = (-b + std::sqrt(disc)) / a;
   if (temp < hi && temp > lo) { rec = temp; return true; }
@@ -261,16 +261,16 @@ inline bool vol_boundary(const Volume& v, f3 o, f3 d, float lo, float hi, float&
 // volume_sphere.cu:63-110 / volume_box.cu:42-85 with the boundary test of
 // Shirley's constant medium: the free flight -ln(xi) / density from the
 // entry (clamped to tmin) must end before the exit.
-inline void isect_volume(const Volume& v, f3 o, f3 d, float tmin, float xi, HitState& h, int inst, int prim) {
-  float r1 = 0.f, r2 = 0.f;
+inline void isect_volume(const Volume& v, f3 o, f3 d, Real tmin, Real xi, HitState& h, int inst, int prim) {
+  Real r1 = 0.f, r2 = 0.f;
   if (!vol_boundary(v, o, d, -FLT_MAX, FLT_MAX, r1)) return;
-  if (!vol_boundary(v, o, d, (float)((double)r1 + 0.0001), FLT_MAX, r2)) return;
+  if (!vol_boundary(v, o, d, r1 + (Real)0.0001f, FLT_MAX, r2)) return;
   if (r1 < tmin) r1 = tmin;
   if (r1 >= r2) return;
   if (r1 < 0.f) r1 = 0.f;
-  const float len = length(d);
-  const float inside = (r2 - r1) * len;
-  const float hd = -(1.f / v.density) * plogf(xi);
+  const Real len = length(d);
+  const Real inside = (r2 - r1) * len;
+  const Real hd = -(1.f / v.density) * plogf(xi);
   if (!(hd < inside)) return;
   h.offer(r1 + hd / len, tmin, inst, prim);
 }
@@ -279,16 +279,16 @@ inline void isect_volume(const Volume& v, f3 o, f3 d, float tmin, float xi, HitS
 // d.x P.x + d.z + P.z and P.x^2 + P.z^2 + R^2, and it rejects t < delta):
 // x^2 + z^2 = R^2 around the origin's vertical with 0 <= y - o.y <= L; the
 // nearer valid root; delta < 1e-8 misses as in the reference.
-inline void isect_cylinder(const Cylinder& cy, f3 o, f3 d, float tmin, HitState& h, int inst, int prim) {
+inline void isect_cylinder(const Cylinder& cy, f3 o, f3 d, Real tmin, HitState& h, int inst, int prim) {
   f3 P0 = o - cy.o;
-  float a = d.x * d.x + d.z * d.z;
-  float b = d.x * P0.x + d.z * P0.z;
-  float c = P0.x * P0.x + P0.z * P0.z - cy.R * cy.R;
-  float delta = b * b - a * c;
+  Real a = d.x * d.x + d.z * d.z;
+  Real b = d.x * P0.x + d.z * P0.z;
+  Real c = P0.x * P0.x + P0.z * P0.z - cy.R * cy.R;
+  Real delta = b * b - a * c;
   if (delta < 1e-8f) return;
-  float sq = std::sqrt(delta);
-  float t = (-b - sq) / a;
-  float y = P0.y + t * d.y;
+  Real sq = std::sqrt(delta);
+  Real t = (-b - sq) / a;
+  Real y = P0.y + t * d.y;
   if (t > tmin && y >= 0.f && y <= cy.L) { h.offer(t, tmin, inst, prim); return; }
   t = (-b + sq) / a;
   y = P0.y + t * d.y;
@@ -302,12 +302,12 @@ inline void isect_cylinder(const Cylinder& cy, f3 o, f3 d, float tmin, HitState&
 // weights of v1 (b1) and v2 (b2).
 struct TriRay {
   int kx, ky, kz;
-  float Sx, Sy, Sz;
+  Real Sx, Sy, Sz;
 };
-inline float comp(f3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+inline Real comp(f3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
 inline TriRay tri_ray_setup(f3 d) {
   TriRay r;
-  float ax = std::fabs(d.x), ay = std::fabs(d.y), az = std::fabs(d.z);
+  Real ax = std::fabs(d.x), ay = std::fabs(d.y), az = std::fabs(d.z);
   r.kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
   r.kx = r.kz + 1 == 3 ? 0 : r.kz + 1;
   r.ky = r.kx + 1 == 3 ? 0 : r.kx + 1;
@@ -317,33 +317,53 @@ inline TriRay tri_ray_setup(f3 d) {
   r.Sz = 1.0f / comp(d, r.kz);
   return r;
 }
-inline bool tri_test(const TriRay& r, f3 o, f3 v0, f3 v1, f3 v2, float& t, float& b1, float& b2) {
+// a * b = hi + lo exactly (Dekker's product over Veltkamp's 12-bit split; no
+// fused multiply-add)
+inline void two_prod(float a, float b, float& hi, float& lo) {
+  const float sa = a * 4097.0f, sb = b * 4097.0f;
+  const float ah = sa - (sa - a), al = a - ah;
+  const float bh = sb - (sb - b), bl = b - bh;
+  hi = a * b;
+  lo = (((ah * bh - hi) + ah * bl) + al * bh) + al * bl;
+}
+// a * b - c * d with the exact sign and a nearly correctly rounded value: the
+// edge functions of the watertight test when one of them rounds to zero
+// (Woop et al. 2013 recompute them at higher precision there; the double
+// build of the oracle needs no second pass)
+inline float diff_of_products(float a, float b, float c, float d) {
+  float h1, l1, h2, l2;
+  two_prod(a, b, h1, l1);
+  two_prod(c, d, h2, l2);
+  const float s = h1 - h2, z = s - h1;
+  const float t = (h1 - (s - z)) + (-h2 - z);
+  return s + (t + (l1 - l2));
+}
+inline bool tri_test(const TriRay& r, f3 o, f3 v0, f3 v1, f3 v2, Real& t, Real& b1, Real& b2) {
   f3 A = v0 - o, B = v1 - o, C = v2 - o;
-  float Ax = comp(A, r.kx) - r.Sx * comp(A, r.kz);
-  float Ay = comp(A, r.ky) - r.Sy * comp(A, r.kz);
-  float Bx = comp(B, r.kx) - r.Sx * comp(B, r.kz);
-  float By = comp(B, r.ky) - r.Sy * comp(B, r.kz);
-  float Cx = comp(C, r.kx) - r.Sx * comp(C, r.kz);
-  float Cy = comp(C, r.ky) - r.Sy * comp(C, r.kz);
-  float U = Cx * By - Cy * Bx;
-  float V = Ax * Cy - Ay * Cx;
-  float W = Bx * Ay - By * Ax;
+  Real Ax = comp(A, r.kx) - r.Sx * comp(A, r.kz);
+  Real Ay = comp(A, r.ky) - r.Sy * comp(A, r.kz);
+  Real Bx = comp(B, r.kx) - r.Sx * comp(B, r.kz);
+  Real By = comp(B, r.ky) - r.Sy * comp(B, r.kz);
+  Real Cx = comp(C, r.kx) - r.Sx * comp(C, r.kz);
+  Real Cy = comp(C, r.ky) - r.Sy * comp(C, r.kz);
+  Real U = Cx * By - Cy * Bx;
+  Real V = Ax * Cy - Ay * Cx;
+  Real W = Bx * Ay - By * Ax;
+#ifndef PTREF_DOUBLE
   if (U == 0.0f || V == 0.0f || W == 0.0f) {
-    double CxBy = (double)Cx * (double)By, CyBx = (double)Cy * (double)Bx;
-    U = (float)(CxBy - CyBx);
-    double AxCy = (double)Ax * (double)Cy, AyCx = (double)Ay * (double)Cx;
-    V = (float)(AxCy - AyCx);
-    double BxAy = (double)Bx * (double)Ay, ByAx = (double)By * (double)Ax;
-    W = (float)(BxAy - ByAx);
+    U = diff_of_products(Cx, By, Cy, Bx);
+    V = diff_of_products(Ax, Cy, Ay, Cx);
+    W = diff_of_products(Bx, Ay, By, Ax);
   }
+#endif
   if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
-  float det = U + V + W;
+  Real det = U + V + W;
   if (det == 0.0f) return false;
-  float Az = r.Sz * comp(A, r.kz);
-  float Bz = r.Sz * comp(B, r.kz);
-  float Cz = r.Sz * comp(C, r.kz);
-  float T = U * Az + V * Bz + W * Cz;
-  float rcp = 1.0f / det;
+  Real Az = r.Sz * comp(A, r.kz);
+  Real Bz = r.Sz * comp(B, r.kz);
+  Real Cz = r.Sz * comp(C, r.kz);
+  Real T = U * Az + V * Bz + W * Cz;
+  Real rcp = 1.0f / det;
   t = T * rcp;
   b1 = V * rcp;
   b2 = W * rcp;
@@ -359,8 +379,8 @@ inline uint32_t spread10(uint32_t v) {
   v = (v * 0x00000005u) & 0x49249249u;
   return v;
 }
-inline uint32_t quantize10(float c, float lo, float scale) {
-  float f = (c - lo) * scale;
+inline uint32_t quantize10(Real c, Real lo, Real scale) {
+  Real f = (c - lo) * scale;
   return f >= 1023.0f ? 1023u : (uint32_t)f;
 }
 
@@ -374,6 +394,16 @@ struct Bvh4 {
 
 inline uint32_t fbits(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
 inline float bitsf(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
+// Node boxes are stored as float: a wider Real (the double build) rounds
+// them outward; for Real = float both are the identity.
+inline float f_down(Real x) {
+  float f = (float)x;
+  return ((Real)f > x) ? std::nextafter(f, -std::numeric_limits<float>::infinity()) : f;
+}
+inline float f_up(Real x) {
+  float f = (float)x;
+  return ((Real)f < x) ? std::nextafter(f, std::numeric_limits<float>::infinity()) : f;
+}
 
 // Build: bounds -> centroids -> Morton codes -> stable sort -> Karras binary
 // hierarchy -> bottom-up bounds -> breadth-first collapse to 4-wide nodes.
@@ -390,10 +420,10 @@ inline Bvh4 build_bvh4(const std::vector<Aabb>& prim, int leaf_max) {
     cb = merge(cb, p);
     out.root = merge(out.root, prim[i]);
   }
-  float ex = cb.hi.x - cb.lo.x, ey = cb.hi.y - cb.lo.y, ez = cb.hi.z - cb.lo.z;
-  float sx = ex > 0.0f ? 1024.0f / ex : 0.0f;
-  float sy = ey > 0.0f ? 1024.0f / ey : 0.0f;
-  float sz = ez > 0.0f ? 1024.0f / ez : 0.0f;
+  Real ex = cb.hi.x - cb.lo.x, ey = cb.hi.y - cb.lo.y, ez = cb.hi.z - cb.lo.z;
+  Real sx = ex > 0.0f ? 1024.0f / ex : 0.0f;
+  Real sy = ey > 0.0f ? 1024.0f / ey : 0.0f;
+  Real sz = ez > 0.0f ? 1024.0f / ez : 0.0f;
   std::vector<uint32_t> code(n);
   for (uint32_t i = 0; i < n; i++) {
     code[i] = (spread10(quantize10(cen[i].x, cb.lo.x, sx)) << 2) |
@@ -485,12 +515,12 @@ inline Bvh4 build_bvh4(const std::vector<Aabb>& prim, int leaf_max) {
     return base;
   };
   auto set_slot = [&](size_t base, int c, const Aabb& b, uint32_t child, uint32_t count) {
-    out.words[base + 0 + c] = fbits(b.lo.x);
-    out.words[base + 4 + c] = fbits(b.hi.x);
-    out.words[base + 8 + c] = fbits(b.lo.y);
-    out.words[base + 12 + c] = fbits(b.hi.y);
-    out.words[base + 16 + c] = fbits(b.lo.z);
-    out.words[base + 20 + c] = fbits(b.hi.z);
+    out.words[base + 0 + c] = fbits(f_down(b.lo.x));
+    out.words[base + 4 + c] = fbits(f_up(b.hi.x));
+    out.words[base + 8 + c] = fbits(f_down(b.lo.y));
+    out.words[base + 12 + c] = fbits(f_up(b.hi.y));
+    out.words[base + 16 + c] = fbits(f_down(b.lo.z));
+    out.words[base + 20 + c] = fbits(f_up(b.hi.z));
     out.words[base + 24 + c] = child;
     out.words[base + 28 + c] = count;
   };
@@ -505,11 +535,11 @@ inline Bvh4 build_bvh4(const std::vector<Aabb>& prim, int leaf_max) {
     std::vector<uint32_t> slots{left[m], right[m]};
     while (slots.size() < 4) {
       int best = -1;
-      float best_area = -1.0f;
+      Real best_area = -1.0f;
       for (size_t s = 0; s < slots.size(); s++) {
         uint32_t r = slots[s];
         if ((r & LEAF) || ref_size(r) <= (uint32_t)leaf_max) continue;
-        float a = half_area(nb[r]);
+        Real a = half_area(nb[r]);
         if (a > best_area) { best_area = a; best = (int)s; }
       }
       if (best < 0) break;
@@ -537,7 +567,7 @@ struct Instance {
   int kind;
   uint32_t first, count;
   bool xf;
-  float o2w[12], w2o[12];
+  Real o2w[12], w2o[12];
   int blas;
 };
 struct Blas {
@@ -546,9 +576,9 @@ struct Blas {
   Bvh4 bvh;
 };
 
-struct Material { int type; int tex[2]; float p[4]; };
+struct Material { int type; int tex[2]; Real p[4]; };
 struct Texture { int type; f3 rgb; uint32_t w = 0, h = 0; std::vector<float> texels; };
-struct Light { int type, axis; float a0, a1, b0, b1, k; f3 c; float r; f3 e; };
+struct Light { int type, axis; Real a0, a1, b0, b1, k; f3 c; Real r; f3 e; };
 
 struct Counts {
   uint64_t nodes = 0, prims = 0, insts = 0;
@@ -568,15 +598,24 @@ struct EnvCdf {
 };
 
 // Texel weight of row j (theta at the row centre), FP64 luminance (Rec. 709
-// weights) times sin(theta) from the shared float libm.
+// weights) times sin(theta) from the shared Real libm.
 inline double env_texel_weight(const float* t, uint32_t j, uint32_t H) {
   double lum = 0.2126 * (double)t[0] + 0.7152 * (double)t[1] + 0.0722 * (double)t[2];
-  float st = psin(PI_F * (((float)j + 0.5f) / (float)H));
+  Real st = psin(PI_F * (((Real)j + 0.5f) / (Real)H));
   double w = lum * (double)st;
   return (w > 0.0) ? w : 0.0;  // NaN and negative texels get no weight
 }
-// weights are scaled by 2^(40 - e), e the binary exponent of the largest
-// weight, so every texel fits in 40 bits and the total in 61
+// Bits per quantised texel weight: 40, fewer for maps of more than 2^23
+// texels, so that the total of W * H weights stays below 2^63.
+inline int env_weight_bits(uint32_t W, uint32_t H) {
+  const uint64_t n = (uint64_t)W * H;
+  int lg = 0;
+  while (lg < 63 && (1ull << lg) < n) lg++;
+  return std::min(40, 63 - lg);
+}
+// weights are scaled by 2^(B - e), e the binary exponent of the largest
+// weight and B = env_weight_bits, so every texel fits in B bits and the
+// total below 2^63
 inline void build_env_cdf(const Texture& t, int tex, EnvCdf& e) {
   e.W = t.w;
   e.H = t.h;
@@ -586,7 +625,7 @@ inline void build_env_cdf(const Texture& t, int tex, EnvCdf& e) {
     for (uint32_t i = 0; i < e.W; i++) mx = std::fmax(mx, env_texel_weight(&t.texels[4 * ((size_t)j * e.W + i)], j, e.H));
   int ex = 0;
   std::frexp(mx, &ex);
-  double scale = std::ldexp(1.0, 40 - ex);
+  double scale = std::ldexp(1.0, env_weight_bits(e.W, e.H) - ex);
   e.cond.assign((size_t)e.W * e.H, 0);
   e.marg.assign(e.H, 0);
   uint64_t run = 0;
@@ -703,8 +742,10 @@ struct Scene {
       auto& in = d.instances[i];
       Instance I;
       I.kind = in.kind; I.first = in.first; I.count = in.count; I.xf = in.has_transform != 0;
-      std::memcpy(I.o2w, in.obj_to_world, sizeof(I.o2w));
-      std::memcpy(I.w2o, in.world_to_obj, sizeof(I.w2o));
+      for (int k = 0; k < 12; k++) {
+        I.o2w[k] = in.obj_to_world[k];
+        I.w2o[k] = in.world_to_obj[k];
+      }
       if (I.kind == PT_GEOM_MESH) I.count = meshes.at(I.first).ntri();
       I.blas = -1;
       for (size_t b = 0; b < blas.size(); b++)
@@ -763,8 +804,8 @@ struct Scene {
 
   // `vseed`: the ray's RNG state, for volume free flights; `time`: the
   // ray's time, for moving spheres
-  void isect_prim(const Instance& I, int inst, uint32_t local, f3 o, f3 d, const TriRay& tr, float tmin,
-                  HitState& h, uint32_t vseed, float time) const {
+  void isect_prim(const Instance& I, int inst, uint32_t local, f3 o, f3 d, const TriRay& tr, Real tmin,
+                  HitState& h, uint32_t vseed, Real time) const {
     uint32_t g = I.first + local;
     switch (I.kind) {
       case PT_GEOM_SPHERES: isect_sphere(spheres[g], o, d, tmin, h, inst, (int)g); break;
@@ -781,7 +822,7 @@ struct Scene {
       }
       default: {
         const Mesh& m = meshes[I.first];
-        float t, b1, b2;
+        Real t, b1, b2;
         if (tri_test(tr, o, m.v[m.idx[3 * local]], m.v[m.idx[3 * local + 1]], m.v[m.idx[3 * local + 2]], t, b1, b2))
           h.offer(t, tmin, inst, (int)local);
       }
@@ -789,23 +830,23 @@ struct Scene {
   }
 
   // slab test of one child slot; returns entry distance or +inf on a miss
-  static float slab(const uint32_t* w, int c, f3 o, f3 inv, float tmin, float tmax) {
-    float t0x = (bitsf(w[0 + c]) - o.x) * inv.x, t1x = (bitsf(w[4 + c]) - o.x) * inv.x;
-    float t0y = (bitsf(w[8 + c]) - o.y) * inv.y, t1y = (bitsf(w[12 + c]) - o.y) * inv.y;
-    float t0z = (bitsf(w[16 + c]) - o.z) * inv.z, t1z = (bitsf(w[20 + c]) - o.z) * inv.z;
-    float tn = std::fmax(std::fmax(std::fmin(t0x, t1x), std::fmin(t0y, t1y)), std::fmax(std::fmin(t0z, t1z), tmin));
-    float tf = std::fmin(std::fmin(std::fmax(t0x, t1x), std::fmax(t0y, t1y)), std::fmin(std::fmax(t0z, t1z), tmax));
+  static Real slab(const uint32_t* w, int c, f3 o, f3 inv, Real tmin, Real tmax) {
+    Real t0x = (bitsf(w[0 + c]) - o.x) * inv.x, t1x = (bitsf(w[4 + c]) - o.x) * inv.x;
+    Real t0y = (bitsf(w[8 + c]) - o.y) * inv.y, t1y = (bitsf(w[12 + c]) - o.y) * inv.y;
+    Real t0z = (bitsf(w[16 + c]) - o.z) * inv.z, t1z = (bitsf(w[20 + c]) - o.z) * inv.z;
+    Real tn = std::fmax(std::fmax(std::fmin(t0x, t1x), std::fmin(t0y, t1y)), std::fmax(std::fmin(t0z, t1z), tmin));
+    Real tf = std::fmin(std::fmin(std::fmax(t0x, t1x), std::fmax(t0y, t1y)), std::fmin(std::fmax(t0z, t1z), tmax));
     return (tn <= tf * SLAB_FAR_SCALE) ? tn : INF;
   }
 
   // Closest hit through the two-level BVH4.  Children are visited nearest
   // first (entry distance, then slot); popped entries farther than the
   // current hit are skipped.
-  HitState closest(f3 o, f3 d, float tmin, float tmax, Counts* cnt = nullptr, uint32_t vseed = 0,
-                   float time = 0.f) const {
+  HitState closest(f3 o, f3 d, Real tmin, Real tmax, Counts* cnt = nullptr, uint32_t vseed = 0,
+                   Real time = 0.f) const {
     HitState h;
     h.t = tmax;
-    struct Entry { uint32_t ref; float tn; };
+    struct Entry { uint32_t ref; Real tn; };
     const uint32_t LEAF = 1u << 31;
     std::vector<Entry> st;
     st.reserve(128);
@@ -851,7 +892,7 @@ struct Scene {
       int nh = 0;
       for (int c = 0; c < 4; c++) {
         if (w[24 + c] == 0xFFFFFFFFu) continue;
-        float tn = slab(w, c, ro, rinv, tmin, h.t);
+        Real tn = slab(w, c, ro, rinv, tmin, h.t);
         if (tn == INF) continue;
         uint32_t cnt_c = w[28 + c];
         uint32_t ref = cnt_c ? (LEAF | ((cnt_c - 1) << 28) | w[24 + c]) : w[24 + c];
@@ -867,7 +908,7 @@ struct Scene {
   }
 
   // Brute force over every instance and primitive.
-  HitState brute(f3 o, f3 d, float tmin, float tmax, uint32_t vseed = 0, float time = 0.f) const {
+  HitState brute(f3 o, f3 d, Real tmin, Real tmax, uint32_t vseed = 0, Real time = 0.f) const {
     HitState h;
     h.t = tmax;
     for (size_t i = 0; i < insts.size(); i++) {

@@ -1,0 +1,117 @@
+"""Pins the oracle's double build (Real = double, the RNG unchanged), which
+renders the converged references relMSE is measured against, and the
+committed references themselves (tests/golden/fp64/*.npz, written by
+tests/golden/make_fp64_refs.py).
+
+  * The double build takes the same discrete decisions as the float build
+    on almost every path, and agrees with it to float rounding there.
+  * White furnace: every path returns 1 in double as in float.
+  * Every committed reference regenerates bit for bit on a few pixels, lies
+    on the stratified subset of its view, and a low-spp render of the same
+    view has the relMSE an unbiased estimator must have (1/N decay).
+"""
+import math
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import ptamd
+import ptref
+from helpers import PyScene
+from ptamd.metrics import relmse, stratified_pixels
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+sys.path.insert(0, str(GOLDEN))
+import make_fp64_refs as refs  # noqa: E402
+
+FIXTURES = sorted((GOLDEN / "fp64").glob("*.npz"))
+
+
+# Final_Next_Week's spheres sit at coordinates near 500, where a float ray
+# leaving a sphere sometimes re-hits it just beyond tmin = 1e-3 (the
+# reference's epsilon); the double build does not, so fewer of its paths
+# match there.
+@pytest.mark.parametrize("cfg,W,H,variant,min_same", [(ptamd.PT_SCENE_CORNELL, 64, 64, 0, 0.98),
+                                                      (ptamd.PT_SCENE_RANDOM_SPHERES, 80, 54, 0, 0.98),
+                                                      (ptamd.PT_SCENE_MESHES, 96, 54, 3, 0.98),
+                                                      (ptamd.PT_SCENE_NEXT_WEEK, 48, 48, 0, 0.9)])
+def test_double_build_agrees_with_float_build(cfg, W, H, variant, min_same):
+    sc = ptamd.Scene(cfg, W, H, variant)
+    prm = sc.render_params()
+    o32, o64 = ptref.OracleScene(sc.desc), ptref.OracleScene(sc.desc, fp64=True)
+    pix = np.arange(W * H, dtype=np.uint32)
+    same = total = 0
+    for f in range(3):  # one frame per call: the sums are single paths
+        a, _ = o32.render_pixels(sc.camera, prm, f, 1, pix)
+        b, _ = o64.render_pixels(sc.camera, prm, f, 1, pix)
+        # float rounding grows along long specular chains (depth-50 glass),
+        # so a path that took the same decisions agrees to ~1e-4 .. 1e-3
+        close = np.all(np.abs(a - b) <= 2e-3 * np.abs(b) + 1e-6, axis=1)
+        same += int(close.sum())
+        total += len(pix)
+    assert same / total > min_same, same / total
+
+
+def test_double_build_white_furnace():
+    s = PyScene()
+    s.spheres.append(((0, 0, 0), 1.0, s.lambert((1, 1, 1))))
+    s.instance(ptamd.PT_GEOM_SPHERES, 0, 1)
+    s.miss = (ptamd.PT_MISS_CONSTANT, (1, 1, 1), (1, 1, 1), -1, 1)
+    desc = s.desc()
+    o64 = ptref.OracleScene(desc, fp64=True)
+    cam = ptamd.Camera()
+    cam.origin[:] = [0, 0, 4]
+    cam.lower_left[:] = [-0.5, -0.5, 3]
+    cam.horizontal[:] = [1, 0, 0]
+    cam.vertical[:] = [0, 1, 0]
+    cam.time1 = 1.0
+    prm = ptamd.RenderParams(32, 32, 8, 0, 11, 0, 0, 0)
+    sums, st = o64.render_pixels(cam, prm, 0, 4, np.arange(32 * 32, dtype=np.uint32))
+    # f / pdf = 1 to double rounding at every bounce
+    np.testing.assert_allclose(sums / 4, 1.0, atol=1e-12)
+    assert st[1] > 32 * 32 * 4  # the sphere is in view: some paths bounce
+
+
+@pytest.mark.parametrize("path", FIXTURES, ids=[p.stem for p in FIXTURES])
+def test_reference_regenerates(path):
+    z = np.load(path)
+    sc, cam, prm = refs.view_setup(path.stem)
+    W, H = sc.info.width, sc.info.height
+    assert (int(z["width"]), int(z["height"])) == (W, H)
+    assert np.array_equal(z["pix"], stratified_pixels(W, H))
+    cam_vals = np.array(list(cam.origin) + list(cam.lower_left) + list(cam.horizontal) + list(cam.vertical),
+                        np.float32)
+    assert np.array_equal(z["camera"], cam_vals)
+    assert int(z["flags"]) == prm.flags and int(z["max_depth"]) == prm.max_depth
+    assert int(z["frame_base"]) == refs.FRAME_BASE
+    spp = int(z["spp"])
+    assert spp >= 16384
+    # three pixels of the subset, bit for bit (per-pixel sums are in frame
+    # order whatever the thread count)
+    k = np.array([0, len(z["pix"]) // 2, len(z["pix"]) - 1])
+    o64 = ptref.OracleScene(sc.desc, fp64=True)
+    sums, _ = refs.reference_sums(o64, cam, prm, z["pix"][k], spp, threads=3)
+    assert np.array_equal((sums / spp).astype(np.float32), z["mean"][k])
+    assert np.all(np.isfinite(z["mean"])) and np.all(z["mean"] >= 0)
+
+
+@pytest.mark.parametrize("name", ["c1_cornell", "c0_random_spheres"])
+def test_relmse_of_unbiased_renders_decays_as_one_over_n(name):
+    """The float oracle (the GPU path's arithmetic) at 4 and 16 spp on the
+    reference's pixels: relMSE falls about 4x (the reference's own noise,
+    1/1024 of the 16-spp render's variance, does not show)."""
+    path = GOLDEN / "fp64" / f"{name}.npz"
+    if not path.exists():
+        pytest.skip("reference not generated")
+    z = np.load(path)
+    sc, cam, prm = refs.view_setup(name)
+    pix = z["pix"]
+    o32 = ptref.OracleScene(sc.desc)
+    e = {}
+    for spp in (4, 16):
+        sums, _ = o32.render_pixels(cam, prm, 0, spp, pix)
+        e[spp] = relmse(sums / spp, z["mean"])
+    ratio = e[4] / e[16]
+    assert 2.5 < ratio < 6.0, e
