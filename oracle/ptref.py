@@ -132,7 +132,7 @@ class OracleScene:
         if sums is None:
             sums = np.zeros((len(pix), 3), np.float64)
         if stats is None:
-            stats = np.zeros(3, np.uint64)
+            stats = np.zeros(4, np.uint64)
         fn = self.lib.ptref64_render_pixels if self.fp64 else self.lib.ptref_render_pixels
         st = fn(self.h, C.byref(cam), C.byref(params), frame_begin, frame_count, p(pix), len(pix), threads,
                 p(sums), p(stats))
@@ -180,7 +180,7 @@ class OracleScene:
         rgb = np.zeros((W * H, 3), np.float32) if keep_last else None
         sig = np.zeros(W * H, np.uint32) if keep_last else None
         bounces = np.zeros(W * H, np.uint32) if keep_last else None
-        stats = np.zeros(3, np.uint64)
+        stats = np.zeros(4, np.uint64)
         st = self.lib.ptref_render(self.h, C.byref(cam), C.byref(params), frame_begin, frame_count,
                                    pb, pe, threads, p(accum), p(rgb), p(sig), p(bounces), p(stats))
         if st != 0:

@@ -100,8 +100,8 @@ void ptref_trace(const void* sp, const pt_ray* rays, uint64_t n, pt_hit* hits, u
         f3 o = mk(r.org[0], r.org[1], r.org[2]), d = mk(r.dir[0], r.dir[1], r.dir[2]);
         Counts c;
         // volumes draw their free flight from the ray's index
-        HitState h = mode == 1 ? s.brute(o, d, r.tmin, r.tmax, (uint32_t)i)
-                               : s.closest(o, d, r.tmin, r.tmax, &c, (uint32_t)i);
+        HitState h = mode == 1 ? s.brute(o, d, r.tmin, r.tmax, query_vseed(i))
+                               : s.closest(o, d, r.tmin, r.tmax, &c, query_vseed(i));
         hits[i].t = h.found() ? h.t : INF;
         hits[i].inst = h.inst;
         hits[i].prim = h.prim;
@@ -127,7 +127,7 @@ void ptref_occluded(const void* sp, const pt_ray* rays, uint64_t n, uint8_t* occ
   for (uint64_t i = 0; i < n; i++) {
     const pt_ray& r = rays[i];
     HitState h = s.closest(mk(r.org[0], r.org[1], r.org[2]), mk(r.dir[0], r.dir[1], r.dir[2]), r.tmin, r.tmax,
-                           nullptr, (uint32_t)i);
+                           nullptr, query_vseed(i));
     occ[i] = h.found() ? (s.mats[s.material_of(h.inst, h.prim)].type == PT_MAT_DIFFUSE_LIGHT ? 0 : 1) : 0;
   }
 }
@@ -135,7 +135,8 @@ void ptref_occluded(const void* sp, const pt_ray* rays, uint64_t n, uint8_t* occ
 // Render frames [frame_begin, frame_begin + frame_count) of pixels
 // [pix_begin, pix_end) (launch index x + W*y) into accum (W*H*3 doubles,
 // row 0 = top).  Optional last-frame outputs are indexed by launch index.
-// stats: [0] paths, [1] extension rays, [2] shadow rays.
+// stats: [0] paths, [1] extension rays, [2] shadow rays, [3] paths whose
+// color has a NaN channel.
 int32_t ptref_render(const void* sp, const pt_camera* cam, const pt_render_params* rp, uint32_t frame_begin,
                      uint32_t frame_count, uint64_t pix_begin, uint64_t pix_end, int32_t nthreads, double* accum,
                      float* last_rgb, uint32_t* last_sig, uint32_t* last_bounces, uint64_t* stats) {
@@ -146,13 +147,13 @@ int32_t ptref_render(const void* sp, const pt_camera* cam, const pt_render_param
     uint64_t npix = (uint64_t)p.W * p.H;
     if (pix_end > npix) pix_end = npix;
     if (nthreads < 1) nthreads = 1;
-    std::atomic<uint64_t> ext{0}, shd{0}, paths{0};
+    std::atomic<uint64_t> ext{0}, shd{0}, paths{0}, nans{0};
     for (uint32_t f = frame_begin; f < frame_begin + frame_count; f++) {
       std::atomic<uint64_t> next{pix_begin};
       bool last = (f + 1 == frame_begin + frame_count);
       auto work = [&]() {
         PathCounters pc;
-        uint64_t np = 0;
+        uint64_t np = 0, nn = 0;
         while (true) {
           uint64_t b = next.fetch_add(256);
           if (b >= pix_end) break;
@@ -161,6 +162,7 @@ int32_t ptref_render(const void* sp, const pt_camera* cam, const pt_render_param
             uint32_t x = (uint32_t)(i % p.W), y = (uint32_t)(i / p.W);
             PathResult r = trace_path(s, *cam, p, x, y, f, pc);
             np++;
+            nn += (r.L.x == r.L.x && r.L.y == r.L.y && r.L.z == r.L.z) ? 0 : 1;
             f3 c = de_nan(r.L);
             size_t row = (size_t)(p.H - 1 - y);
             double* a = &accum[3 * (row * p.W + x)];
@@ -177,6 +179,7 @@ int32_t ptref_render(const void* sp, const pt_camera* cam, const pt_render_param
         ext += pc.ext;
         shd += pc.shadow;
         paths += np;
+        nans += nn;
       };
       std::vector<std::thread> th;
       for (int t = 1; t < nthreads; t++) th.emplace_back(work);
@@ -187,6 +190,7 @@ int32_t ptref_render(const void* sp, const pt_camera* cam, const pt_render_param
       stats[0] = paths;
       stats[1] = ext;
       stats[2] = shd;
+      stats[3] = nans;
     }
     return 0;
   } catch (const std::exception& e) {

@@ -138,8 +138,13 @@ inline f3 geometric_normal(const Scene& s, int inst, int prim, f3 n_shading) {
 
 // Bilinear lookup with repeat wrap, texel centres at (i + 0.5) / w
 // (textures.hpp:177-230 sampler settings; image_texture.cu:5-7).
+// Coordinates that are NaN or beyond 2^62 texels (a NaN direction reaching
+// a miss program) give a NaN colour, which de_nan removes and the render
+// statistics count.
 inline f3 tex_image(const Texture& t, Real u, Real v) {
   Real x = u * (Real)t.w - 0.5f, y = v * (Real)t.h - 0.5f;
+  if (!(std::fabs(x) < (Real)0x1p62f && std::fabs(y) < (Real)0x1p62f))
+    return mk(std::numeric_limits<Real>::quiet_NaN());
   Real x0 = std::floor(x), y0 = std::floor(y);
   Real fx = x - x0, fy = y - y0;
   auto wrap = [](int64_t i, int64_t n) { int64_t r = i % n; return r < 0 ? r + n : r; };
@@ -575,6 +580,7 @@ inline Real env_pdf(const Scene& s, f3 d) {
   Real u, v;
   env_uv(d, u, v);
   Real uw = u - std::floor(u);
+  if (!(uw == uw && v == v)) return 0.f;  // a NaN direction
   int64_t i = (int64_t)(uw * (Real)e.W), j = (int64_t)(v * (Real)e.H);
   if (i >= (int64_t)e.W) i = e.W - 1;
   if (j >= (int64_t)e.H) j = e.H - 1;

@@ -234,6 +234,9 @@ inline Real vol_xi(uint32_t seed, uint32_t vol, uint32_t inst) {
   const uint32_t h = mix32((seed & 0xFFFFFFu) ^ mix32(vol * 0x9E3779B9u ^ mix32(inst + 0x632BE5ABu)));
   return (Real)(h >> 8) * (1.0f / 16777216.0f);
 }
+// Free-flight seed of query ray i: a hash of the whole 64-bit index (vol_xi
+// reads 24 bits of it), so rays 2^24 apart do not share their flights.
+inline uint32_t query_vseed(uint64_t i) { return mix32((uint32_t)i ^ mix32((uint32_t)(i >> 32) + 0x7F4A7C15u)); }
 // hit_boundary of volume_sphere.cu:21-55 / volume_box.cu:20-40
 inline bool vol_boundary(const Volume& v, f3 o, f3 d, Real lo, Real hi, Real& rec) {
   if (v.shape == PT_VOLUME_BOX) {

@@ -61,7 +61,7 @@ def reference_sums(orc, cam, prm, pix, spp, threads, name=""):
     chunks of 1024 frames (each chunk summed in frame order, chunks added in
     order: the result does not depend on the thread count)."""
     sums = np.zeros((len(pix), 3), np.float64)
-    stats = np.zeros(3, np.uint64)
+    stats = np.zeros(4, np.uint64)
     t0 = time.time()
     chunk = 1024
     for f0 in range(0, spp, chunk):

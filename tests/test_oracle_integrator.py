@@ -174,3 +174,23 @@ def test_display_tone_map():
     np.testing.assert_array_equal(out[0, 0, :3], [0, int(255.99 * 0.5), 255])
     np.testing.assert_array_equal(out[0, 1, :3], [255, int(255.99 * np.float32(0.1)), int(255.99 * np.sqrt(np.float32(0.5)))])
     assert np.all(out[..., 3] == 255)
+
+
+def test_nan_paths_are_counted_and_removed():
+    """raygen.cu:126-134 de_nan zeroes NaN channels before accumulation; the
+    render statistics count those paths.  An Ashikhmin-Shirley floor facing
+    +Y has a NaN tangent frame (D9): its continuation is NaN and, under an
+    MIS environment (unclamped), so is the path.  The Cornell box has none."""
+    from test_gpu_fullsize import nan_scene
+
+    s = nan_scene()
+    W, H = 32, 24
+    prm = ptamd.RenderParams(W, H, 8, 1, 3, 0, ptamd.PT_FLAG_ENV_MIS, 0)
+    acc, (rgb, _, _), st = ptref.OracleScene(s.desc()).render(camera((0, 3, 8), (0, 0, 0), 40, W / H), prm, 0,
+                                                                2, keep_last=True)
+    nan_last = int(np.any(np.isnan(rgb), axis=1).sum())
+    assert 0 < nan_last <= int(st[3])
+    assert np.all(np.isfinite(acc))
+    sc = ptamd.Scene(ptamd.PT_SCENE_CORNELL, 32, 32)
+    _, _, st = ptref.OracleScene(sc.desc).render(sc.camera, sc.render_params(), 0, 2)
+    assert int(st[3]) == 0
