@@ -292,6 +292,12 @@ typedef struct pt_render_stats {
                                programs, shadow rays, continuation)        */
   uint32_t extend_launches;
   uint32_t shade_launches;
+  uint64_t nan_paths;       /* paths whose color() had a NaN channel, which
+                               de_nan zeroes (raygen.cu:126-134)           */
+  double cache_judge;       /* radiance cache, last batch: the judge's error
+                               ratio (-1: not judged, NaN: outputs not finite) */
+  uint32_t cache_fallbacks; /* batches of this call rendered again by the
+                               plain tracer because the judge failed them  */
 } pt_render_stats;
 
 typedef struct pt_accel_stats {
@@ -447,8 +453,11 @@ pt_status pt_autotune(struct pt_context* ctx, const pt_render_params* params, ui
  * weights, BF16 MFMA forward and backward with FP32 accumulation, Adam
  * (beta1 0.9, beta2 0.999, eps 1e-8).  Features are BF16 bit patterns,
  * PT_CACHE_INPUTS per sample; outputs and targets are PT_CACHE_OUTPUTS
- * floats per sample (the renderer uses log(1 + radiance)).  Loss: mean over
- * samples of the squared output error summed over the outputs. */
+ * floats per sample: outputs y stand for radiance r = expm1(min(y, 20)),
+ * targets are radiance.  Loss: mean over samples, summed over the outputs,
+ * of the relative L2 (r - t)^2 / (r^2 + 0.01) with the denominator held
+ * constant (Mueller et al. 2021): its minimiser over noisy targets of one
+ * input is their mean. */
 #define PT_CACHE_INPUTS 96
 #define PT_CACHE_OUTPUTS 3
 #define PT_CACHE_PARAMS 18883
@@ -466,17 +475,30 @@ pt_status pt_cache_train(struct pt_context* ctx, const uint16_t* features, const
 
 /* Render-loop use of the cache (PT_FLAG_RADIANCE_CACHE): paths not chosen
  * for training end at bounce `query_depth` (0 = the camera ray's hit) with
- * L += T * (exp(network) - 1); a `train_fraction` of the paths (hashed on
- * pixel and frame) are traced to the end and record every vertex; after
- * each batch of frames the network takes `steps` Adam steps of `batch`
- * vertices each (steps x batch vertices spread evenly over the batch's
- * records, each used once) with learning rate `lr`, then answers the
- * batch's queries.
+ * L += T * expm1(min(network, 20)) (negative and NaN answers count 0); a
+ * `train_fraction` of the paths (hashed on pixel and frame) are traced to
+ * the end by the plain tracer and record every vertex; after each batch of
+ * frames the network takes `steps` Adam steps of `batch` vertices each
+ * (steps x batch vertices spread evenly over the batch's records, each used
+ * once) with learning rate `lr`, is judged (pt_cache_judge_configure), then
+ * answers the batch's queries.
  * Defaults: 2, 0.03, 16384, 4, 1e-3. */
 pt_status pt_cache_configure(struct pt_context* ctx, uint32_t query_depth, float train_fraction,
                              uint32_t batch, uint32_t steps, float lr);
+/* The per-batch judge of the cache's error.  The plain tracer's training
+ * paths give, at `rows` of the batch's vertices (halfway between the ones
+ * trained on), an unbiased one-sample radiance t; with the network's answer
+ * c and the throughput T there,
+ *   ratio = sum T^2 (c - t)^2 / sum T^2 t^2,
+ * the squared error the answers would add to the image over that of ending
+ * the paths without one.  A batch whose ratio is >= max_ratio, or whose
+ * judged outputs are not finite, is rendered again by the plain tracer (its
+ * accumulation is then the plain render's, bit for bit); a network with
+ * non-finite outputs is re-initialised from the pt_cache_init seed.
+ * Defaults: max_ratio 1, rows 16384 (32 .. 2^20). */
+pt_status pt_cache_judge_configure(struct pt_context* ctx, float max_ratio, uint32_t rows);
 /* Records of the last render batch, for checks: which = 0 training
- * vertices (values: L before the vertex, T at it, target log(1 + radiance)),
+ * vertices (values: L before the vertex, T at it, target radiance),
  * 1 queries (values: T, 0, 0, 0, network output).  9 floats and one slot
  * per record, at most `max`; *count = records the batch produced. */
 pt_status pt_cache_read_records(struct pt_context* ctx, int32_t which, uint32_t max, uint16_t* features,

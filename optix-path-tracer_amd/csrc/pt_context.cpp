@@ -115,6 +115,19 @@ struct pt_context {
   uint32_t cache_batch = 16384;     // samples per Adam step
   uint32_t cache_steps = 4;         // Adam steps per render batch
   float cache_lr = 1e-3f;
+  uint32_t cache_seed = 0;          // of the last pt_cache_init (a diverged network restarts from it)
+  // per-batch judge: training vertices the plain tracer finished, checked
+  // against the network before its answers are used
+  float cache_judge_max = 1.f;      // fall back when error ratio >= this
+  uint32_t cache_judge_rows = 16384;
+  uint32_t cjudge_cap = 0;
+  std::vector<DevBuf> cjudge_bufs;
+  uint4* cjfeat = nullptr;          // [rows][12] features
+  float* cjvals = nullptr;          // [rows][9] L, T, target
+  float* cjout = nullptr;           // [rows][3] network outputs
+  double* cjres = nullptr;          // judge sums
+  double cache_last_judge = -1.0;   // error ratio of the last judged batch (-1: none)
+  uint32_t cache_last_fallbacks = 0;
   float scene_lo[3] = {0, 0, 0}, scene_hi[3] = {1, 1, 1};  // top-level box, for the position encoding
   uint64_t cache_rec_cap = 0;       // slots the record buffers were sized for
   std::vector<DevBuf> crec_bufs;
@@ -286,6 +299,7 @@ void pt_destroy(pt_context* c) {
   dfree(c->accel_bufs);
   dfree(c->path_bufs);
   dfree(c->cache_bufs);
+  dfree(c->cjudge_bufs);
   dfree(c->cwork_bufs);
   dfree(c->crec_bufs);
   dfree(c->batch_bufs);
@@ -987,6 +1001,8 @@ static pt_status ensure_paths(pt_context* c, uint64_t cap) {
 }
 
 static pt_status cache_work(pt_context* c, uint32_t n);
+static pt_status cache_reset(pt_context* c);
+static pt_status ensure_cache_judge(pt_context* c);
 
 // radiance-cache record buffers for `cap` path slots: one query per path;
 // training vertices for a quarter of the slots, split among the waves of a
@@ -1096,6 +1112,8 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
   if (ps != PT_OK) return ps;
   if (use_cache) {
     if ((ps = ensure_cache_records(c, (uint64_t)fpb * WH)) != PT_OK) return ps;
+    if ((ps = ensure_cache_judge(c)) != PT_OK) return ps;
+    c->cache_last_judge = -1.0;
     // dense training rows of a batch: steps x batch, at most every record
     const uint64_t rows = std::min<uint64_t>((uint64_t)c->cache_steps * c->cache_batch, c->paths.t_cap);
     if ((ps = cache_work(c, (uint32_t)std::max<uint64_t>(rows, c->cache_batch))) != PT_OK) return ps;
@@ -1146,6 +1164,7 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
     if (!c->frame_dev) HIPCHK(c, hipMalloc(&c->frame_dev, 64));
     launch_set_frame(c->frame_dev, frame_begin, c->stream);
   }
+  uint32_t fallbacks = 0;
   for (uint32_t f0 = 0; f0 < frame_count; f0 += fpb) {
     uint32_t nf = std::min(fpb, frame_count - f0);
     uint32_t nslots = nf * WH;
@@ -1176,7 +1195,7 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
         RenderConsts saved = rc;
         rc = grc;
         enqueue_batch(nslots, grid, false);
-        launch_accumulate(P, rc, nf, c->accum, grid_for(WH), c->stream);
+        launch_accumulate(P, rc, nf, c->accum, c->stats, grid_for(WH), c->stream);
         launch_next_frames(c->frame_dev, nf, c->stream);
         rc = saved;
         hipGraph_t g = nullptr;
@@ -1216,19 +1235,50 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
           c->cache_last_batch = n;
         }
       }
-      if (nq) {
+      // the judge: training vertices halfway between the ones trained on
+      // (the same ones when there are few), finished by the plain tracer,
+      // against the network that is about to answer the queries
+      bool use_answers = true;
+      const uint32_t nj = std::min(nt, c->cache_judge_rows);
+      if (nq && nj >= 32) {
+        launch_cache_gather(P, 0, nj, golden_stride(nj), grid, c->cjfeat, nullptr, c->cjvals, nullptr, c->stream,
+                            std::max<uint32_t>(rows, 1u));
+        launch_mlp_infer((const uint16_t*)c->cjfeat, nj, c->cfw, c->cbias, c->cjout,
+                         (int)std::min<uint32_t>(1024, (nj + 63) / 64), c->stream);
+        launch_cache_judge(c->cjvals, c->cjout, nj, c->cjres, c->stream);
+        double jr[3] = {0.0, 0.0, 0.0};
+        HIPCHK(c, hipMemcpyAsync(jr, c->cjres, sizeof(jr), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        const double ratio = jr[1] > 0.0 ? jr[0] / jr[1] : (jr[0] > 0.0 ? INFINITY : 0.0);
+        const bool finite = jr[2] == 0.0 && std::isfinite(jr[0]) && std::isfinite(jr[1]);
+        c->cache_last_judge = finite ? ratio : NAN;
+        use_answers = finite && ratio < (double)c->cache_judge_max;
+        // a network whose outputs are no longer finite starts again
+        if (!finite && (ps = cache_reset(c)) != PT_OK) return ps;
+      }
+      if (nq && use_answers) {
         launch_mlp_infer((const uint16_t*)P.q_feat, nslots, c->cfw, c->cbias, P.q_out, 2048, c->stream, P.qseg,
                          (uint32_t)grid * BLOCK);
         launch_cache_apply(P, grid, c->stream);
+      } else if (nq) {
+        // fallback: the batch again with the plain tracer, whose results
+        // replace the cut paths' (the same samples: seeds depend on pixel
+        // and frame only)
+        RenderConsts saved = rc;
+        rc.flags &= ~(uint32_t)PT_FLAG_RADIANCE_CACHE;
+        enqueue_batch(nslots, grid, kernel_events);
+        rc = saved;
+        fallbacks++;
       }
       c->cache_last_train = nt;
       c->cache_last_query = nq;
       c->cache_last_steps = steps;
       c->cache_last_grid = grid;
     }
-    launch_accumulate(P, rc, nf, c->accum, grid_for(WH), c->stream);
+    launch_accumulate(P, rc, nf, c->accum, c->stats, grid_for(WH), c->stream);
     c->last_fpb = nf;
   }
+  if (use_cache) c->cache_last_fallbacks = fallbacks;
   hipEvent_t t_end = want_stats ? mark() : nullptr;
   if (want_stats) launch_stats_reduce(c->stats, c->stats_total, c->stream);
   c->frames = frame_begin + frame_count;
@@ -1250,10 +1300,13 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
     out->shadow_prim_fetches = st[STAT_SH_PRIMS];
     out->shadow_instance_fetches = st[STAT_SH_INSTS];
     out->exhausted_paths = st[STAT_EXHAUSTED];
+    out->nan_paths = st[STAT_NAN];
     out->cache_queries = c->cache_last_query;
     out->cache_train_vertices = c->cache_last_train;
     out->cache_steps = c->cache_last_steps;
     out->cache_loss = 0.0;
+    out->cache_judge = use_cache ? c->cache_last_judge : -1.0;
+    out->cache_fallbacks = use_cache ? c->cache_last_fallbacks : 0u;
     if (use_cache && c->cache_last_steps) {
       const uint32_t n = c->cache_last_batch;
       const uint32_t waves = mlp_fwd_bwd_waves(n);
@@ -1372,7 +1425,7 @@ pt_status pt_cache_init(pt_context* c, uint32_t seed) {
   if (!c) return PT_ERR_INVALID_ARG;
   (void)hipSetDevice(c->device);
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (!c->cache_ready) {
+  if (!c->cp) {
     auto& B = c->cache_bufs;
     hipError_t e;
     if ((e = dalloc(B, (void**)&c->cp, MLP_PARAMS * 4)) != hipSuccess ||
@@ -1386,8 +1439,17 @@ pt_status pt_cache_init(pt_context* c, uint32_t seed) {
       return hip_fail(c, e, "hipMalloc (cache parameters)");
     }
   }
-  // He-uniform weights U(-sqrt(6 / fan_in), +sqrt(6 / fan_in)) from
-  // mt19937(seed) (24-bit uniforms), zero biases
+  c->cache_seed = seed;
+  pt_status ps = cache_reset(c);
+  if (ps != PT_OK) return ps;
+  c->cache_ready = true;
+  return PT_OK;
+}
+
+// He-uniform weights U(-sqrt(6 / fan_in), +sqrt(6 / fan_in)) from
+// mt19937(cache_seed) (24-bit uniforms), zero biases, zero Adam state
+static pt_status cache_reset(pt_context* c) {
+  const uint32_t seed = c->cache_seed;
   std::vector<float> p(MLP_PARAMS, 0.f);
   std::mt19937 g(seed);
   const int in[5] = {MLP_IN, MLP_H, MLP_H, MLP_H, MLP_H}, out[5] = {MLP_H, MLP_H, MLP_H, MLP_H, MLP_OUT};
@@ -1410,7 +1472,25 @@ pt_status pt_cache_init(pt_context* c, uint32_t seed) {
   launch_mlp_pack(c->cp, c->cfw, c->cbw, c->cbias, c->stream);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->cache_step = 0;
-  c->cache_ready = true;
+  return PT_OK;
+}
+
+static pt_status ensure_cache_judge(pt_context* c) {
+  const uint32_t n = c->cache_judge_rows;
+  if (n <= c->cjudge_cap && c->cjres) return PT_OK;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  dfree(c->cjudge_bufs);
+  c->cjudge_cap = 0;
+  auto& B = c->cjudge_bufs;
+  hipError_t e;
+  if ((e = dalloc(B, (void**)&c->cjfeat, (size_t)n * 12 * 16)) != hipSuccess ||
+      (e = dalloc(B, (void**)&c->cjvals, (size_t)n * 9 * 4)) != hipSuccess ||
+      (e = dalloc(B, (void**)&c->cjout, (size_t)n * 3 * 4)) != hipSuccess ||
+      (e = dalloc(B, (void**)&c->cjres, 4 * sizeof(double))) != hipSuccess) {
+    dfree(B);
+    return hip_fail(c, e, "hipMalloc (cache judge)");
+  }
+  c->cjudge_cap = n;
   return PT_OK;
 }
 
@@ -1619,7 +1699,7 @@ pt_status pt_render_batch(pt_context* const* ctxs, const pt_render_params* param
       DevPaths Pk = P;
       Pk.result = P.result + off[k];
       const uint32_t i = scene[k];
-      launch_accumulate(Pk, rcs[i], (uint32_t)(e - k), ctxs[i]->accum, grid_for(rcs[i].WH), B->stream);
+      launch_accumulate(Pk, rcs[i], (uint32_t)(e - k), ctxs[i]->accum, B->stats, grid_for(rcs[i].WH), B->stream);
       k = e;
     }
   }
@@ -1647,6 +1727,7 @@ pt_status pt_render_batch(pt_context* const* ctxs, const pt_render_params* param
     out->shadow_prim_fetches = st[STAT_SH_PRIMS];
     out->shadow_instance_fetches = st[STAT_SH_INSTS];
     out->exhausted_paths = st[STAT_EXHAUSTED];
+    out->nan_paths = st[STAT_NAN];
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, t0, t1);
     out->gpu_ms = ms;
@@ -1729,6 +1810,15 @@ pt_status pt_cache_configure(pt_context* c, uint32_t query_depth, float train_fr
   c->cache_batch = batch;
   c->cache_steps = steps;
   c->cache_lr = lr;
+  return PT_OK;
+}
+
+pt_status pt_cache_judge_configure(pt_context* c, float max_ratio, uint32_t rows) {
+  if (!c) return PT_ERR_INVALID_ARG;
+  if (!(max_ratio >= 0.f)) return fail(c, PT_ERR_INVALID_ARG, "max_ratio must be >= 0");
+  if (rows < 32 || rows > (1u << 20)) return fail(c, PT_ERR_INVALID_ARG, "rows not in [32, 2^20]");
+  c->cache_judge_max = max_ratio;
+  c->cache_judge_rows = rows;
   return PT_OK;
 }
 

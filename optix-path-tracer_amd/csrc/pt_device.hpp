@@ -221,6 +221,9 @@ PTD float vol_xi(uint32_t seed, uint32_t vol, uint32_t inst) {
   const uint32_t h = mix32((seed & 0xFFFFFFu) ^ mix32(vol * 0x9E3779B9u ^ mix32(inst + 0x632BE5ABu)));
   return (float)(h >> 8) * (1.0f / 16777216.0f);
 }
+// Free-flight seed of query ray i: a hash of the whole 64-bit index (vol_xi
+// reads 24 bits of it), so rays 2^24 apart do not share their flights.
+PTD uint32_t query_vseed(uint64_t i) { return mix32((uint32_t)i ^ mix32((uint32_t)(i >> 32) + 0x7F4A7C15u)); }
 constexpr float FLT_BIG = 3.40282347e+38f;
 // hit_boundary of volume_sphere.cu:21-55 and volume_box.cu:20-40: the first
 // boundary root in (lo, hi); A = (a, density), B = (b, index | box << 31)
@@ -622,5 +625,30 @@ PTD float power_heuristic(float fPdf, float gPdf) {
 }
 
 PTD uint32_t sig_mix(uint32_t h, uint32_t v) { return (h ^ v) * 16777619u; }
+
+// ------------------------------------------------------- radiance cache
+// The network's outputs y are log(1 + radiance) per channel; the radiance
+// it stands for is expm1(min(y, CACHE_Y_MAX)).
+constexpr float CACHE_Y_MAX = 20.f;     // e^20 ~ 4.9e8 per unit throughput
+constexpr float CACHE_REL_EPS = 0.01f;  // relative-loss denominator floor
+// Radiance of a query's answer: non-negative, 0 for a NaN output.
+PTD float cache_radiance(float y) {
+  const float r = expm1f(y > CACHE_Y_MAX ? CACHE_Y_MAX : y);
+  return r > 0.f ? r : 0.f;
+}
+// Relative L2 loss of Mueller et al. 2021 (neural radiance caching) on one
+// output: with yh = expm1(min(y, CACHE_Y_MAX)) and the target radiance t,
+//   loss = (yh - t)^2 / (yh^2 + CACHE_REL_EPS),  the denominator held
+//   constant, so dloss/dy = 2 (yh - t) (yh + 1) / (yh^2 + CACHE_REL_EPS);
+// its minimiser over noisy targets of one input is their mean.  Above
+// CACHE_Y_MAX the gradient is taken at the clamp (an over-bright output is
+// still pulled down); a NaN output gives a NaN loss.
+PTD void cache_loss_terms(float y, float t, float& loss, float& dy) {
+  const float yh = expm1f(y > CACHE_Y_MAX ? CACHE_Y_MAX : y);
+  const float den = yh * yh + CACHE_REL_EPS;
+  const float d = yh - t;
+  loss = d * d / den;
+  dy = 2.f * d * (yh + 1.f) / den;
+}
 
 }  // namespace pt

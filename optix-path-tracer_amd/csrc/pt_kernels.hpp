@@ -67,15 +67,21 @@ void launch_env_tables(const float4* tex, uint32_t W, uint32_t H, unsigned long 
 void launch_cache_scan(const DevPaths& P, int grid, hipStream_t st);
 // dense copies of `rows` records spread over the batch's records (which 0:
 // training vertices with targets, 1: queries), row r taking rank
-// (r * mul) mod rows (mul coprime to rows); tgt, vals, slots optional
+// (r * mul) mod rows (mul coprime to rows); tgt, vals, slots optional.
+// half_step = k > 0 moves every rank by total / (2 k): the records halfway
+// between those a gather of k rows takes.
 void launch_cache_gather(const DevPaths& P, int which, uint32_t rows, uint32_t mul, int grid, uint4* feat, float* tgt,
-                         float* vals, uint32_t* slots, hipStream_t st);
+                         float* vals, uint32_t* slots, hipStream_t st, uint32_t half_step = 0);
+// judge sums of n gathered training rows (vals) against the network's
+// outputs y: res[0] error of the answers, res[1] error of answering 0,
+// res[2] non-finite outputs (FP64, one block)
+void launch_cache_judge(const float* vals, const float* y, uint32_t n, double* res, hipStream_t st);
 void launch_cache_apply(const DevPaths& P, int grid, hipStream_t st);
 void launch_exhausted(const DevPaths& P, int in, uint64_t* stats, int grid, hipStream_t st);
 // per-wave statistics rows [MAX_WAVES][STAT_WORDS] -> totals [STAT_WORDS]
 void launch_stats_reduce(const uint64_t* part, uint64_t* total, hipStream_t st);
-void launch_accumulate(const DevPaths& P, const RenderConsts& rc, uint32_t nframes, double* accum, int grid,
-                       hipStream_t st);
+void launch_accumulate(const DevPaths& P, const RenderConsts& rc, uint32_t nframes, double* accum, uint64_t* stats,
+                       int grid, hipStream_t st);
 void launch_display(const double* accum, uint32_t npix, uint32_t frames, uchar4* rgba, int grid, hipStream_t st);
 void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st);
 void launch_fill_f64(double* p, uint64_t n, double v, hipStream_t st);

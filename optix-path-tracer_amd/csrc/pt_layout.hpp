@@ -207,7 +207,7 @@ constexpr int CTR_WORDS = 32;
 // [3] prims, [4] instances, ... kept per wave ([MAX_WAVES][STAT_WORDS],
 // each row updated by its own wave only) and summed once per render.
 constexpr int STAT_EXT = 0, STAT_SHADOW = 1, STAT_NODES = 2, STAT_PRIMS = 3, STAT_INSTS = 4;
-constexpr int STAT_SH_NODES = 5, STAT_SH_PRIMS = 6, STAT_SH_INSTS = 7, STAT_EXHAUSTED = 8;
+constexpr int STAT_SH_NODES = 5, STAT_SH_PRIMS = 6, STAT_SH_INSTS = 7, STAT_EXHAUSTED = 8, STAT_NAN = 9;
 constexpr int STAT_WORDS = 16;
 
 // ------------------------------------------------------- radiance cache

@@ -246,14 +246,15 @@ __global__ __launch_bounds__(MB) void k_mlp_fwd_bwd(const uint4* feats, const fl
   for (int s = 0; s < 3; s++) store_b(X, ld, X_ROW0[0], f.x[s], s, g, col, false);
   for (int L = 0; L < 4; L++)
     for (int s = 0; s < 2; s++) store_b(X, ld, X_ROW0[L + 1], f.h[L][s], s, g, col, true);
-  // loss 1/n sum (y - t)^2 and its gradient
+  // relative L2 loss and its gradient (cache_loss_terms)
   float dz5[4] = {0.f, 0.f, 0.f, 0.f};
   float loss = 0.f;
   for (int r = 0; r < 4; r++) {
     if (g == 0 && r < MLP_OUT && valid) {
-      float d = f.y[r] - targets[(size_t)col * MLP_OUT + r];
-      loss += d * d * inv_n;
-      dz5[r] = 2.f * d * inv_n;
+      float l, d;
+      cache_loss_terms(f.y[r], targets[(size_t)col * MLP_OUT + r], l, d);
+      loss += l * inv_n;
+      dz5[r] = d * inv_n;
     }
   }
   for (int r = 0; r < 4; r++) dZ[(size_t)(DZ_ROW0[4] + 4 * g + r) * ld + col] = to_bf16(dz5[r]);

@@ -153,7 +153,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_closest(DevScene sc, const pt_r
     const float4* r4 = reinterpret_cast<const float4*>(rays + i);
     float4 a = r4[0], b = r4[1];
     TraceCounts c{0, 0, 0};
-    HitOut h = trace_closest<COUNT, EXT>(sc, mk3(a), mk3(b), a.w, b.w, st, c, (uint32_t)i);
+    HitOut h = trace_closest<COUNT, EXT>(sc, mk3(a), mk3(b), a.w, b.w, st, c, EXT ? query_vseed(i) : 0u);
     pt_hit out;
     out.t = h.inst >= 0 ? h.t : FINF;
     out.inst = h.inst;
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_occluded(DevScene sc, const pt_
     const float4* r4 = reinterpret_cast<const float4*>(rays + i);
     float4 a = r4[0], b = r4[1];
     TraceCounts c{0, 0, 0};
-    HitOut h = trace_closest<false, EXT>(sc, mk3(a), mk3(b), a.w, b.w, st, c, (uint32_t)i);
+    HitOut h = trace_closest<false, EXT>(sc, mk3(a), mk3(b), a.w, b.w, st, c, EXT ? query_vseed(i) : 0u);
     uint8_t o = 0;
     if (h.inst >= 0) o = sc.mats[hit_material(sc, h.pos, h.prim)].type == PT_MAT_DIFFUSE_LIGHT ? 0 : 1;
     occ[i] = o;
@@ -878,20 +878,24 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_cache_scan(DevPaths P, uint32_
 // rows: with mul ~ 0.618 rows every run of rows, such as one Adam batch,
 // samples the whole image instead of a few waves' pixels): features, and
 // for training vertices the target
-//   log(1 + radiance leaving the vertex toward the camera side)
-//     = log(1 + (L_final - L_before) / T)
+//   radiance leaving the vertex toward the camera side
+//     = (L_final - L_before) / T
 // per channel, zero where T is zero or the value is not finite.  vals (9
 // per row: L, T, target for training vertices; T, 0, network output for
 // queries) and slots are optional.
 __global__ __launch_bounds__(BLOCK) void k_cache_gather(DevPaths P, int which, uint32_t rows, uint32_t mul,
-                                                        uint32_t waves, uint32_t stride, uint4* __restrict__ feat,
-                                                        float* __restrict__ tgt, float* __restrict__ vals,
-                                                        uint32_t* __restrict__ slots) {
+                                                        uint32_t half_step, uint32_t waves, uint32_t stride,
+                                                        uint4* __restrict__ feat, float* __restrict__ tgt,
+                                                        float* __restrict__ vals, uint32_t* __restrict__ slots) {
   const uint32_t total = P.counters[which == 0 ? CTR_TRAIN : CTR_QUERY];
   const uint32_t* pre = which == 0 ? P.tpre : P.qpre;
+  // half_step = k: ranks move by total / (2 k) (records between those a
+  // gather of k rows takes; the same ones when total < 2 k)
+  const uint32_t shift = half_step ? total / (2u * half_step) : 0u;
   for (uint32_t r = blockIdx.x * BLOCK + threadIdx.x; r < rows && r < total; r += gridDim.x * BLOCK) {
     const uint32_t rank = (uint32_t)((uint64_t)r * mul % rows);
-    const uint32_t idx = rows >= total ? rank : (uint32_t)((uint64_t)rank * total / rows);
+    uint32_t idx = rows >= total ? rank : (uint32_t)((uint64_t)rank * total / rows);
+    idx = (uint32_t)(((uint64_t)idx + shift) % total);
     uint32_t lo = 0, hi = waves - 1;  // last wave whose prefix is <= idx
     while (lo < hi) {
       const uint32_t mid = (lo + hi + 1) >> 1;
@@ -914,7 +918,7 @@ __global__ __launch_bounds__(BLOCK) void k_cache_gather(DevPaths P, int which, u
         v = (v > 0.f && v < 3.0e38f) ? v : 0.f;
         out[c] = L0[c];
         out[3 + c] = T0[c];
-        out[6 + c] = log1pf(v);
+        out[6 + c] = v;
       }
       if (tgt)
         for (int c = 0; c < 3; c++) tgt[(size_t)r * 3 + c] = out[6 + c];
@@ -933,7 +937,7 @@ __global__ __launch_bounds__(BLOCK) void k_cache_gather(DevPaths P, int which, u
   }
 }
 
-// The queries' answers: result += T * max(expm1(y), 0) per channel, over
+// The queries' answers: result += T * cache_radiance(y) per channel, over
 // each wave's query segment (same grid as the batch).
 __global__ __launch_bounds__(BLOCK) void k_cache_apply(DevPaths P) {
   const Seg g = my_seg();
@@ -942,8 +946,8 @@ __global__ __launch_bounds__(BLOCK) void k_cache_apply(DevPaths P) {
     const uint32_t i = g.pos(j);
     float4 m = P.q_meta[i];
     const uint32_t slot = __float_as_uint(m.w) & SLOT_MASK;
-    float y0 = fmaxf(expm1f(P.q_out[3 * (size_t)i]), 0.f), y1 = fmaxf(expm1f(P.q_out[3 * (size_t)i + 1]), 0.f),
-          y2 = fmaxf(expm1f(P.q_out[3 * (size_t)i + 2]), 0.f);
+    const float y0 = cache_radiance(P.q_out[3 * (size_t)i]), y1 = cache_radiance(P.q_out[3 * (size_t)i + 1]),
+                y2 = cache_radiance(P.q_out[3 * (size_t)i + 2]);
     float4 r = P.result[slot];
     r.x += m.x * y0;
     r.y += m.y * y1;
@@ -952,15 +956,62 @@ __global__ __launch_bounds__(BLOCK) void k_cache_apply(DevPaths P) {
   }
 }
 
+// The cache's per-batch judge, on n training vertices the plain tracer
+// finished (vals: 9 per row, T at 3..5 and the unbiased radiance sample t
+// at 6..8) and the network's outputs y for them:
+//   res[0] = sum T^2 (cache_radiance(y) - t)^2   squared error the answers
+//                                                would add to the image
+//   res[1] = sum T^2 t^2                          the same for answering 0
+//                                                (ending the paths there)
+//   res[2] = outputs that are not finite
+// per channel, summed in FP64 in a fixed order (one block).
+__global__ __launch_bounds__(BLOCK) void k_cache_judge(const float* __restrict__ vals, const float* __restrict__ y,
+                                                       uint32_t n, double* __restrict__ res) {
+  __shared__ double s_num[BLOCK], s_den[BLOCK], s_bad[BLOCK];
+  double num = 0.0, den = 0.0, bad = 0.0;
+  for (uint32_t r = threadIdx.x; r < n; r += BLOCK) {
+    for (int c = 0; c < 3; c++) {
+      const float yc = y[3 * (size_t)r + c];
+      const double T = vals[9 * (size_t)r + 3 + c], t = vals[9 * (size_t)r + 6 + c];
+      if (!(fabsf(yc) <= 3.0e38f)) bad += 1.0;
+      const double e = (double)cache_radiance(yc) - t;
+      num += T * T * e * e;
+      den += T * T * t * t;
+    }
+  }
+  s_num[threadIdx.x] = num;
+  s_den[threadIdx.x] = den;
+  s_bad[threadIdx.x] = bad;
+  __syncthreads();
+  for (int off = BLOCK / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      s_num[threadIdx.x] += s_num[threadIdx.x + off];
+      s_den[threadIdx.x] += s_den[threadIdx.x + off];
+      s_bad[threadIdx.x] += s_bad[threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    res[0] = s_num[0];
+    res[1] = s_den[0];
+    res[2] = s_bad[0];
+  }
+}
+
 // raygen.cu:126-134 de_nan and raygen.cu:166 acc_buffer += (rgb, 1), in FP64
 // and frame order; accumulation row 0 is the top of the image (raygen.cu:153).
-__global__ __launch_bounds__(BLOCK) void k_accumulate(DevPaths P, RenderConsts rc, uint32_t nframes, double* accum) {
+// Paths with a NaN channel are counted into the wave's statistics row.
+__global__ __launch_bounds__(BLOCK) void k_accumulate(DevPaths P, RenderConsts rc, uint32_t nframes, double* accum,
+                                                      uint64_t* stats) {
+  uint32_t nans = 0;
   for (uint32_t pix = blockIdx.x * BLOCK + threadIdx.x; pix < rc.WH; pix += gridDim.x * BLOCK) {
     uint32_t x = pix % rc.W, y = pix / rc.W;
     size_t o = 3 * ((size_t)(rc.H - 1 - y) * rc.W + x);
     double a0 = accum[o], a1 = accum[o + 1], a2 = accum[o + 2];
     for (uint32_t f = 0; f < nframes; f++) {
       float4 r = P.result[(size_t)f * rc.WH + pix];
+      const bool ok = (r.x == r.x) & (r.y == r.y) & (r.z == r.z);
+      nans += ok ? 0u : 1u;
       float cx = (r.x == r.x) ? r.x : 0.f, cy = (r.y == r.y) ? r.y : 0.f, cz = (r.z == r.z) ? r.z : 0.f;
       a0 += (double)cx;
       a1 += (double)cy;
@@ -970,6 +1021,8 @@ __global__ __launch_bounds__(BLOCK) void k_accumulate(DevPaths P, RenderConsts r
     accum[o + 1] = a1;
     accum[o + 2] = a2;
   }
+  for (int d = 32; d >= 1; d >>= 1) nans += __shfl_xor(nans, d);
+  if (__lane_id() == 0) stat_add(stats, my_seg(), STAT_NAN, nans);
 }
 
 // raygen.cu:136-146 make_Color on the FP64 accumulation
@@ -1164,10 +1217,13 @@ void launch_cache_scan(const DevPaths& P, int grid, hipStream_t st) {
   hipLaunchKernelGGL(k_cache_scan, dim3(1), dim3(SCAN_THREADS), 0, st, P, (uint32_t)(grid * WAVES));
 }
 void launch_cache_gather(const DevPaths& P, int which, uint32_t rows, uint32_t mul, int grid, uint4* feat, float* tgt,
-                         float* vals, uint32_t* slots, hipStream_t st) {
+                         float* vals, uint32_t* slots, hipStream_t st, uint32_t half_step) {
   const int g = (int)std::max<uint32_t>(1, std::min<uint32_t>((rows + BLOCK - 1) / BLOCK, 2048));
-  hipLaunchKernelGGL(k_cache_gather, dim3(g), dim3(BLOCK), 0, st, P, which, rows, mul, (uint32_t)(grid * WAVES),
-                     (uint32_t)(grid * BLOCK), feat, tgt, vals, slots);
+  hipLaunchKernelGGL(k_cache_gather, dim3(g), dim3(BLOCK), 0, st, P, which, rows, mul, half_step,
+                     (uint32_t)(grid * WAVES), (uint32_t)(grid * BLOCK), feat, tgt, vals, slots);
+}
+void launch_cache_judge(const float* vals, const float* y, uint32_t n, double* res, hipStream_t st) {
+  hipLaunchKernelGGL(k_cache_judge, dim3(1), dim3(BLOCK), 0, st, vals, y, n, res);
 }
 void launch_cache_apply(const DevPaths& P, int grid, hipStream_t st) {
   hipLaunchKernelGGL(k_cache_apply, dim3(grid), dim3(BLOCK), 0, st, P);
@@ -1178,9 +1234,9 @@ void launch_exhausted(const DevPaths& P, int in, uint64_t* stats, int grid, hipS
 void launch_stats_reduce(const uint64_t* part, uint64_t* total, hipStream_t st) {
   hipLaunchKernelGGL(k_stats_reduce, dim3(STAT_WORDS), dim3(BLOCK), 0, st, part, total);
 }
-void launch_accumulate(const DevPaths& P, const RenderConsts& rc, uint32_t nframes, double* accum, int grid,
-                       hipStream_t st) {
-  hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(BLOCK), 0, st, P, rc, nframes, accum);
+void launch_accumulate(const DevPaths& P, const RenderConsts& rc, uint32_t nframes, double* accum, uint64_t* stats,
+                       int grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(BLOCK), 0, st, P, rc, nframes, accum, stats);
 }
 void launch_display(const double* accum, uint32_t npix, uint32_t frames, uchar4* rgba, int grid, hipStream_t st) {
   hipLaunchKernelGGL(k_display, dim3(grid), dim3(BLOCK), 0, st, accum, npix, frames, rgba);

@@ -658,6 +658,11 @@ struct ContextObj : ObjBase {
       acc_host[i] = {(float)a[3 * i], (float)a[3 * i + 1], (float)a[3 * i + 2], (float)frames};
     return acc_host.data();
   }
+  // pt_save_image of the accumulation (Save_PNG / Save_HDR)
+  bool save_image(const char* path, int32_t format) {
+    if (!pt) return false;
+    return pt_sync(pt) == PT_OK && pt_save_image(pt, path, format) == PT_OK;
+  }
   void* map_display() {
     disp_host.resize((size_t)W * H);
     check(pt_read_display(pt, reinterpret_cast<uint8_t*>(disp_host.data())));

@@ -141,7 +141,8 @@ class RenderStats(C.Structure):
                 ("exhausted_paths", C.c_uint64), ("cache_queries", C.c_uint64),
                 ("cache_train_vertices", C.c_uint64), ("cache_loss", C.c_double), ("cache_steps", C.c_uint32), ("gpu_ms", C.c_double), ("extend_ms", C.c_double),
                 ("shade_ms", C.c_double), ("extend_launches", C.c_uint32),
-                ("shade_launches", C.c_uint32)]
+                ("shade_launches", C.c_uint32), ("nan_paths", C.c_uint64),
+                ("cache_judge", C.c_double), ("cache_fallbacks", C.c_uint32)]
 
 
 class AccelStats(C.Structure):
@@ -199,6 +200,7 @@ def _declare(lib):
         "pt_autotune": (C.c_int32, [P, C.POINTER(RenderParams), C.c_uint32, C.POINTER(TuneResult)]),
         "pt_cache_init": (C.c_int32, [P, C.c_uint32]),
         "pt_cache_configure": (C.c_int32, [P, C.c_uint32, C.c_float, C.c_uint32, C.c_uint32, C.c_float]),
+        "pt_cache_judge_configure": (C.c_int32, [P, C.c_float, C.c_uint32]),
         "pt_cache_read_records": (C.c_int32, [P, C.c_int32, C.c_uint32, P, P, P, C.POINTER(C.c_uint32)]),
         "pt_cache_params": (C.c_int32, [P, P, P]),
         "pt_cache_infer": (C.c_int32, [P, P, C.c_uint64, P]),
@@ -423,6 +425,9 @@ class Context:
     def cache_configure(self, query_depth=2, train_fraction=0.03, batch=16384, steps=4, lr=1e-3):
         self._check(self.lib.pt_cache_configure(self.h, query_depth, train_fraction, batch, steps, lr),
                     "pt_cache_configure")
+
+    def cache_judge_configure(self, max_ratio=1.0, rows=16384):
+        self._check(self.lib.pt_cache_judge_configure(self.h, max_ratio, rows), "pt_cache_judge_configure")
 
     def cache_records(self, which, max_records=1 << 20):
         """Last batch's training vertices (which=0) or queries (1):

@@ -67,17 +67,45 @@ def forward(params, feats, bf16_ops=True):
     raise AssertionError
 
 
-def loss_and_grad(params, feats, targets, bf16_ops=True):
-    """Mean over samples of the summed squared output error, and its
-    gradient with respect to every parameter (the GPU's rounding points
-    when bf16_ops)."""
+Y_MAX = 20.0        # outputs stand for radiance expm1(min(y, Y_MAX))
+REL_EPS = 0.01      # relative-loss denominator floor
+
+
+def radiance(y):
+    """Radiance an output stands for when it answers a query: expm1 of the
+    output clamped at Y_MAX, negative values and NaN as 0."""
+    r = np.expm1(np.minimum(np.asarray(y, np.float64), Y_MAX))
+    return np.where(r > 0, r, 0.0)
+
+
+def rel_l2_terms(y, t, den=None):
+    """Relative L2 loss (Mueller et al. 2021, neural radiance caching) of
+    outputs y against radiance targets t, elementwise: with
+    yh = expm1(min(y, Y_MAX)), loss = (yh - t)^2 / den, den = yh^2 + REL_EPS
+    held constant, so d loss / d y = 2 (yh - t) (yh + 1) / den (taken at the
+    clamp above Y_MAX).  `den` may be given (a denominator frozen elsewhere).
+    Returns (loss, dloss/dy, den)."""
+    y = np.asarray(y, np.float64)
+    yh = np.expm1(np.where(y > Y_MAX, Y_MAX, y))
+    if den is None:
+        den = yh * yh + REL_EPS
+    d = yh - np.asarray(t, np.float64)
+    return d * d / den, 2.0 * d * (yh + 1.0) / den, den
+
+
+def loss_and_grad(params, feats, targets, bf16_ops=True, den=None):
+    """Mean over samples of the relative L2 loss summed over the outputs,
+    and its gradient with respect to every parameter, the denominators held
+    constant (the GPU's rounding points when bf16_ops).  With `den` given the
+    denominators are those values (finite differences of the loss then see
+    the same constant the gradient does)."""
     params = np.asarray(params, np.float64)
     layers = unpack(params)
     n = len(feats)
     y, acts = forward(params, feats, bf16_ops)
-    d = y - np.asarray(targets, np.float64)
-    loss = float(np.sum(d * d) / n)
-    dz = 2.0 * d / n
+    lterm, dterm, _ = rel_l2_terms(y, targets, den)
+    loss = float(np.sum(lterm) / n)
+    dz = dterm / n
     grads = []
     for L in range(4, -1, -1):
         if bf16_ops:
