@@ -1,42 +1,64 @@
 """Benchmark: Mrays/s of the per-sample render path on one MI355X.
 
-Workload (one step): the Cornell box with light + BRDF MIS, 512x512, 64
-samples per pixel, max depth 16, Russian roulette from depth 3, rendered
-through the C ABI (pt_render, frames 0..63) with the scene, acceleration
-structure and path queues resident in HBM before the timed region.
+Default run (`python bench.py`): one JSON line whose headline is the
+triangle-mesh scene (8 displaced icospheres, 2^20 triangles, on a ground
+quad under an importance-sampled 2048x1024 HDRI with a sun lobe; 1920x1080,
+256 samples per pixel, max depth 12, material sort), the largest single-GPU
+configuration; beside it, in `configs`, the Cornell box with light + BRDF
+MIS (512x512, 64 spp, depth 16, Russian roulette from depth 3) and the
+random-spheres scene (400x267, 16 spp, depth 50).  One step renders every
+sample of the image (pt_render, frames 0..spp-1) with the scene,
+acceleration structure and path queues resident in HBM before the timed
+region.
 
-  value    = (closest-hit + shadow queries) / wall time of the timed steps
-  roofline = the dominant kernel, k_shade (hit/miss programs, light sample,
-             shadow-ray traversal, continuation, compaction of survivors):
-             algorithmic bytes = per shaded path 64 B state (32 B at depth
-             0, whose throughput and radiance are constants) + 16 B hit
-             read, 64 B survivor state or 20 B result written; per shadow ray
-             128 B per BVH4 node + record bytes per primitive + 64 B per
-             instance entered (counted by a traversal-counting pass over the
-             identical deterministic rays).  Material, light and texture
-             tables are not counted.  Divided by the kernel's summed duration
-             (HIP events on the render stream).
-  roofline_extend = the same for k_extend (32 B ray + 16 B hit + traversal;
-             the depth-0 launch starts the camera paths: it writes the ray
-             instead of reading it, and 16 B of result per path).
-  cpu_baseline = the scalar CPU restatement (oracle) on the host cores, on
-             all 64 frames on 16 threads, in Mrays/s.
+  value          = (closest-hit + shadow queries) / wall time of the timed steps
+  relmse_vs_fp64 = relMSE of the step's image against a converged FP64
+                   render of the oracle (16384 spp, tests/golden/fp64) on its
+                   stratified 1/64 pixel subset
+  roofline       = the dominant kernel: algorithmic bytes per launch (the
+                   traversal's node, primitive and instance fetches at their
+                   stored sizes, counted by a traversal-counting pass over the
+                   identical deterministic rays, plus the path state each
+                   launch reads and writes) / its average launch time, HIP
+                   events on the render stream; `traffic` = HBM bytes per
+                   launch from rocprofv3 FETCH_SIZE + WRITE_SIZE, each scaled
+                   by the same counter over a plain device copy
+  cpu_baseline   = the scalar CPU restatement (oracle) on the host's cores,
+                   on a bounded sample of the same workload (named in
+                   `sample`), in Mrays/s
+
+Other workloads: `--workload cache` (the learned radiance cache on the mesh
+scene's orbit: equal-error time of cache and plain tracer against the FP64
+references of the frozen views), `--workload batch` (64 uneven scenes in
+one pt_render_batch), `--workload meshes|cornell|random_spheres` (one plain
+configuration alone).
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
+import platform
 import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 REPO = Path(__file__).resolve().parent
 sys.path.insert(0, str(REPO / "optix-path-tracer_amd"))
+FIXTURES = REPO / "tests" / "golden" / "fp64"
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+CPU_THREADS_MAX = 16  # the host-core share of one GPU on the measurement box
 # stored record bytes per primitive kind (pt_layout.hpp)
 PRIM_BYTES = {0: 16, 1: 32, 2: 32, 3: 48}
+# algorithmic bytes of the LBVH build per primitive: bounds (32 B write +
+# read), Morton code and index (8 B write, 4 passes of 8 B read + 8 B write
+# in the radix sort), hierarchy and refit (64 B), BVH4 collapse (16 B)
+BUILD_BYTES_PER_PRIM = 130
 
 
 def log(msg):
@@ -61,119 +83,344 @@ def copy_calibration(mib: int):
     log(f"[bench] copied {mib} MiB three times")
 
 
-def load_traffic():
-    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on
-    this workload, scaled by the same counters over a plain copy."""
-    f = REPO / "profiles" / "traffic_cornell_512x512_64spp.json"
+def host_cpu():
+    """Model name and core counts of the host."""
+    model = platform.processor() or ""
     try:
-        return {k: v["hbm_bytes_per_launch"] for k, v in json.loads(f.read_text())["kernels"].items()}
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return {"model": model, "hardware_concurrency": os.cpu_count(), "available": avail}
+
+
+def load_profile(name):
+    """Per-kernel figures measured with rocprofv3 on this workload
+    (profiles/<name>.json): HBM bytes per launch and counter summaries."""
+    f = REPO / "profiles" / f"{name}.json"
+    try:
+        return json.loads(f.read_text())["kernels"]
     except Exception:
         return {}
 
 
-def orbit_camera(scene, frame, frames_per_turn=300, r=20.0, height=4.0, vfov=45.0):
-    """Camera of the radiance-cache workload: orbit of radius r around the
-    origin, one turn per `frames_per_turn` frames (frame 0 is the scene's own
-    pose at (20, 4, 0))."""
-    import math
+def load_fixture(name):
+    f = FIXTURES / f"{name}.npz"
+    if not f.exists():
+        return None
+    d = np.load(f)
+    return d["pix"], d["mean"].astype(np.float64), int(d["spp"])
 
+
+def relmse_at(ctx, W, H, fixture):
+    """relMSE of the context's accumulated image against a converged
+    reference on its pixel subset."""
+    from ptamd.metrics import accum_at, relmse
+
+    pix, ref, _ = fixture
+    acc, frames = ctx.read_accum(W, H)
+    return relmse(accum_at(acc, pix, W, H) / max(frames, 1), ref)
+
+
+# (scene config, reference image, record bytes of the scene's primitives,
+#  workload name, CPU-baseline sample)
+PLAIN = {
+    "meshes": ("PT_SCENE_MESHES", "c2_meshes", 48, "meshes_2p20_tris_hdri_envmis_{W}x{H}_{spp}spp_depth{d}",
+               "subset"),
+    "cornell": ("PT_SCENE_CORNELL", "c1_cornell", 32, "cornell_box_mis_{W}x{H}_{spp}spp_depth{d}_rr{rr}", "full"),
+    "random_spheres": ("PT_SCENE_RANDOM_SPHERES", "c0_random_spheres", 16,
+                       "random_spheres_{W}x{H}_{spp}spp_depth{d}_sky", "full"),
+}
+
+
+def cpu_baseline(scene, params, W, H, spp, how, fixture):
+    """The oracle on the host cores over a bounded sample of the workload:
+    `full` renders every frame of the step, `subset` the full sample count on
+    the stratified 1/64 pixel subset of the FP64 reference."""
+    sys.path.insert(0, str(REPO / "oracle"))
+    import ptref
+
+    cpu = host_cpu()
+    threads = max(1, min(CPU_THREADS_MAX, cpu["available"]))
+    orc = ptref.OracleScene(scene.desc)
+    t1 = time.perf_counter()
+    if how == "full":
+        _, _, st = orc.render(scene.camera, params, 0, spp, threads=threads)
+        rays = int(st[1]) + int(st[2])
+        sample = f"the whole step: {W}x{H}, frames 0-{spp - 1} ({W * H * spp} paths, {rays} rays)"
+    else:
+        pix = fixture[0]
+        _, st = orc.render_pixels(scene.camera, params, 0, spp, pix, threads=threads)
+        rays = int(st[1]) + int(st[2])
+        sample = (f"the stratified 1/64 pixel subset ({len(pix)} pixels) at the full {spp} spp "
+                  f"({len(pix) * spp} paths, {rays} rays)")
+    dt = time.perf_counter() - t1
+    return {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{sample}, {dt:.1f} s", "host": cpu}
+
+
+def plain_config(name, args, with_cpu=True):
+    """Measure one plain-tracer configuration; returns its result dict."""
     import ptamd
 
-    a = 2 * math.pi * frame / frames_per_turn
-    o = (r * math.cos(a), height, r * math.sin(a))
-    w = [o[k] for k in range(3)]
-    n = math.sqrt(sum(x * x for x in w))
-    w = [x / n for x in w]
-    up = (0.0, 1.0, 0.0)
-    u = [up[1] * w[2] - up[2] * w[1], up[2] * w[0] - up[0] * w[2], up[0] * w[1] - up[1] * w[0]]
-    n = math.sqrt(sum(x * x for x in u))
-    u = [x / n for x in u]
-    v = [w[1] * u[2] - w[2] * u[1], w[2] * u[0] - w[0] * u[2], w[0] * u[1] - w[1] * u[0]]
-    aspect = scene.info.width / scene.info.height
-    hh = math.tan(math.radians(vfov) / 2)
-    hw = aspect * hh
-    f = 10.0  # focus distance, as the scene camera
-    c = ptamd.Camera()
-    c.origin[:] = o
-    c.lower_left[:] = [o[k] - hw * f * u[k] - hh * f * v[k] - f * w[k] for k in range(3)]
-    c.horizontal[:] = [2 * hw * f * u[k] for k in range(3)]
-    c.vertical[:] = [2 * hh * f * v[k] for k in range(3)]
-    c.u[:] = u
-    c.v[:] = v
-    c.w[:] = w
-    c.lens_radius, c.time0, c.time1 = 0.0, 0.0, 1.0
-    return c
+    cfg, fixname, prim_b, wname, cpu_how = PLAIN[name]
+    scene = ptamd.Scene(getattr(ptamd, cfg))
+    info = scene.info
+    W, H = info.width, info.height
+    spp = args.spp or info.spp
+    params = scene.render_params()
+    ctx = ptamd.Context(0)
+    ctx.set_scene(scene.desc)
+    acc = ctx.build_accel()
+    ctx.set_camera(scene.camera)
+    log(f"[bench] {name}: {acc.num_primitives} primitives, {acc.num_nodes} nodes, {acc.num_instances} instances, "
+        f"build {acc.build_ms:.2f} ms")
+
+    # traversal-counting pass (identical rays: the render is deterministic)
+    cst = ctx.render(scene.render_params(flags=info.flags | ptamd.PT_FLAG_COUNT_TRAVERSAL), 0, spp)
+    log(f"[bench] {name}: counted pass {cst.extension_rays} closest-hit, {cst.shadow_rays} shadow rays")
+    for _ in range(args.warmup):
+        ctx.render(params, 0, spp)
+    ctx.sync()
+    steps = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        steps.append(ctx.render(params, 0, spp))
+        log(f"[bench] {name} step {k}: {steps[-1].gpu_ms:.2f} ms")
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    assert all(s.extension_rays == cst.extension_rays and s.shadow_rays == cst.shadow_rays for s in steps)
+    fixture = load_fixture(fixname)
+    err = relmse_at(ctx, W, H, fixture) if fixture else None
+    # the build again, timed alone (the scene and its structures are resident)
+    builds = [ctx.build_accel().build_ms for _ in range(3)]
+    ctx.close()
+
+    rays = sum(s.extension_rays + s.shadow_rays for s in steps)
+    ext, paths, exh = cst.extension_rays, cst.paths, cst.exhausted_paths
+    # k_shade: per shaded path 64 B state (32 B at depth 0) + 16 B hit read,
+    # 64 B survivor or 20 B result written; the shadow ray's traversal
+    shade_bytes = (ext * (64 + 16) - paths * 32 + (ext - paths + exh) * 64 + (paths - exh) * 20 +
+                   cst.shadow_node_fetches * 128 + cst.shadow_prim_fetches * prim_b +
+                   cst.shadow_instance_fetches * 64)
+    # k_extend: 32 B ray + 16 B hit per ray, 16 B result header per path at
+    # depth 0, the traversal's fetches
+    ext_bytes = (ext * (32 + 16) + paths * 16 + cst.node_fetches * 128 + cst.prim_fetches * prim_b +
+                 cst.instance_fetches * 64)
+    shade_ms = sum(s.shade_ms for s in steps) / len(steps)
+    ext_ms = sum(s.extend_ms for s in steps) / len(steps)
+    prof = load_profile(f"traffic_{name}")
+
+    def roof(kernel, key, bytes_step, ms, launches, extra):
+        achieved = bytes_step / (ms * 1e-3) / 1e9
+        p = prof.get(key, {})
+        r = {"kernel": kernel, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": p.get("hbm_bytes_per_launch"),
+             "algorithmic_bytes_per_launch": round(bytes_step / launches), "avg_launch_ms": round(ms / launches, 4)}
+        if p.get("hbm_bytes_per_launch"):
+            r["traffic_GBs"] = round(p["hbm_bytes_per_launch"] / (ms / launches * 1e-3) / 1e9, 1)
+        for k in ("valu_busy", "occupancy_waves_per_simd", "l2_hit", "wait_frac"):
+            if k in p:
+                r[k] = p[k]
+        r.update(extra)
+        return r
+
+    r_shade = roof("k_shade (shading + shadow-ray BVH4 traversal + compaction)", "k_shade", shade_bytes, shade_ms,
+                   steps[0].shade_launches, {"per_shadow_ray": {
+                       "nodes": round(cst.shadow_node_fetches / max(1, cst.shadow_rays), 3),
+                       "prims": round(cst.shadow_prim_fetches / max(1, cst.shadow_rays), 3),
+                       "instances": round(cst.shadow_instance_fetches / max(1, cst.shadow_rays), 3)}})
+    r_ext = roof("k_extend (closest-hit BVH4 traversal; camera paths started at depth 0)", "k_extend", ext_bytes,
+                 ext_ms, steps[0].extend_launches, {"per_ray": {
+                     "nodes": round(cst.node_fetches / ext, 3), "prims": round(cst.prim_fetches / ext, 3),
+                     "instances": round(cst.instance_fetches / ext, 3)}})
+    dominant, other = (r_ext, r_shade) if ext_ms >= shade_ms else (r_shade, r_ext)
+    build_ms = min(builds)
+    out = {
+        "metric": "Mrays/s",
+        "value": round(rays / wall / 1e6, 2),
+        "unit": "Mrays/s",
+        "ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "config": {"workload": wname.format(W=W, H=H, spp=spp, d=info.max_depth, rr=info.rr_min_depth),
+                   "width": W, "height": H, "spp": spp, "max_depth": info.max_depth,
+                   "rr_min_depth": info.rr_min_depth,
+                   "material_sort": bool(info.flags & ptamd.PT_FLAG_MATERIAL_SORT),
+                   "env_mis": bool(info.flags & ptamd.PT_FLAG_ENV_MIS)},
+        "samples_per_s": round(args.steps * W * H * spp / wall),
+        "relmse_vs_fp64": None if err is None else round(err, 6),
+        "relmse_reference": None if fixture is None else
+        f"oracle FP64 build, {fixture[2]} spp, {len(fixture[0])} pixels (stratified 1/64)",
+        "rays_per_step": {"closest_hit": cst.extension_rays, "shadow": cst.shadow_rays},
+        "nan_paths_per_step": cst.nan_paths,
+        "kernel_ms_per_step": {"extend": round(ext_ms, 3), "shade": round(shade_ms, 3),
+                               "render_events": round(sum(s.gpu_ms for s in steps) / len(steps), 3)},
+        "build": {"ms": round(build_ms, 3), "primitives": acc.num_primitives, "nodes": acc.num_nodes,
+                  "algorithmic_GBs": round(BUILD_BYTES_PER_PRIM * acc.num_primitives / (build_ms * 1e-3) / 1e9, 1)},
+        "roofline": dominant,
+        "roofline_other": other,
+    }
+    if with_cpu and not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline(scene, params, W, H, spp, cpu_how, fixture)
+        except Exception as e:  # the baseline is reported, never the target
+            log(f"[bench] cpu baseline failed: {e}")
+            out["cpu_baseline"] = None
+    return out
+
+
+def default_workload(args):
+    head = plain_config("meshes", args)
+    extra = {}
+    for name in ("cornell", "random_spheres"):
+        extra[name] = plain_config(name, args)
+    out = {"metric": "Mrays/s", "value": head["value"], "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic"}
+    out.update({k: v for k, v in head.items() if k not in out and k != "metric"})
+    out["configs"] = extra
+    print(json.dumps(out), flush=True)
+
+
+def single_workload(args):
+    r = plain_config(args.workload, args)
+    out = {"metric": "Mrays/s", "value": r["value"], "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic"}
+    out.update({k: v for k, v in r.items() if k not in out})
+    print(json.dumps(out), flush=True)
+
+
+# ------------------------------------------------------------------ cache
+
+def _time_to(curve, e, extrapolate):
+    """Time at which a relMSE-vs-time curve [(t, relmse)] first reaches
+    error e, interpolated linearly in log t / log relmse between
+    checkpoints; beyond the last checkpoint an unbiased estimator's
+    relMSE falls as 1/t (`extrapolate`), a biased one's is not assumed to
+    fall at all (None)."""
+    prev = None
+    for t, r in curve:
+        if r <= e:
+            if prev is None:  # reached at the first checkpoint
+                return t * r / e if extrapolate else t
+            (t0, r0) = prev
+            if r0 == r:
+                return t
+            s = (math.log(e) - math.log(r0)) / (math.log(r) - math.log(r0))
+            return math.exp(math.log(t0) + s * (math.log(t) - math.log(t0)))
+        prev = (t, r)
+    if extrapolate:
+        t, r = curve[-1]
+        return t * r / e
+    return None
 
 
 def cache_workload(args):
-    """Mesh scene, 1 spp per frame on an orbiting camera with the radiance
-    cache training online; then, on the frozen view of the last frame, the
-    plain tracer and the cache tracer each render 16 frames in one batch and
-    their relMSE against a high-spp plain render (the unbiased estimator the
-    oracle pins) is compared at the cache's GPU time."""
-    import numpy as np
-
+    """The mesh scene at 1920x1080 on an orbiting camera (radius 20, one
+    turn per 300 frames), 1 spp per frame, the radiance cache training
+    online (3 % of paths traced to the end as training paths, queries after
+    2 bounces).  At the frozen views of frames 50, 100, 200 and 300 the
+    plain tracer and the cache tracer (continuing to train) each render up
+    to `--eval-spp` frames progressively from a cleared image; relMSE
+    against the oracle's FP64 16384-spp reference of that view is taken
+    after 1, 2, 4, ... frames, against the wall time of each form (host
+    round trips included).  Reported per view: both curves and the time
+    the cache needs to reach the plain tracer's final error, divided by the
+    plain tracer's time (the target is at most 1/3)."""
     import ptamd
+    from ptamd.metrics import orbit_camera
 
-    W, H = 960, 540
-    scene = ptamd.Scene(ptamd.PT_SCENE_MESHES, W, H)
+    frozen = [f for f in (50, 100, 200, 300) if f <= args.frames]
+    scene = ptamd.Scene(ptamd.PT_SCENE_MESHES)
+    W, H = scene.info.width, scene.info.height
     ctx = ptamd.Context(0)
     ctx.set_scene(scene.desc)
     ctx.build_accel()
     ctx.cache_init(0)
     ctx.cache_configure()
     base = scene.info.flags
-    plain = scene.render_params(flags=base)
+    plain = scene.render_params(flags=base, frames_per_batch=1)
     cached = scene.render_params(flags=base | ptamd.PT_FLAG_RADIANCE_CACHE, frames_per_batch=1)
-    t_plain = t_cache = 0.0
-    for f in range(args.frames):
-        ctx.set_camera(orbit_camera(scene, f))
-        t_plain += ctx.render(plain, f, 1).gpu_ms
-        st = ctx.render(cached, f, 1)
-        t_cache += st.gpu_ms
-        if f % 10 == 0:
-            log(f"[bench] frame {f}: cache loss {st.cache_loss:.4f}, {st.cache_queries} queries, "
-                f"{st.cache_train_vertices} training vertices")
-    ctx.set_camera(orbit_camera(scene, args.frames))
-    ref_spp = 1024
-    ref_st = ctx.render(plain, 100000, ref_spp)
-    ref, _ = ctx.read_accum(W, H)
-    ref /= ref_spp
-    log(f"[bench] reference {ref_spp} spp in {ref_st.gpu_ms:.1f} ms")
-    # the frozen view: both forms render the same number of frames in one
-    # batch; the plain tracer is unbiased, so its error is scaled by its
-    # time ratio to the cache's GPU time (the reference's own noise is not
-    # scaled, which favours the plain tracer)
-    spp = 16
-    st_p = ctx.render(scene.render_params(flags=base, frames_per_batch=spp), 0, spp)
-    img_p, _ = ctx.read_accum(W, H)
-    img_p /= spp
-    st_c = ctx.render(scene.render_params(flags=base | ptamd.PT_FLAG_RADIANCE_CACHE, frames_per_batch=spp), 0, spp)
-    img_c, _ = ctx.read_accum(W, H)
-    img_c /= spp
+
+    def timed(p, f0, n):
+        ctx.sync()
+        t = time.perf_counter()
+        st = ctx.render(p, f0, n)
+        ctx.sync()
+        return (time.perf_counter() - t) * 1e3, st
+
+    def curve(p, fixture):
+        pts, t, done = [], 0.0, 0
+        n = 1
+        while done < args.eval_spp:
+            k = min(n, args.eval_spp) - done
+            ms, _ = timed(p, done, k)
+            t += ms
+            done += k
+            pts.append((round(t, 3), relmse_at(ctx, W, H, fixture), done))
+            n *= 2
+        return pts
+
+    # warm-up: kernels loaded, pools sized (frame 0 of both forms)
+    ctx.set_camera(orbit_camera(scene.info, 0))
+    timed(plain, 0, 1)
+    views, anim = [], {"plain_ms": 0.0, "cache_ms": 0.0, "fallbacks": 0, "frames": 0}
+    for f in range(args.frames + 1):
+        ctx.set_camera(orbit_camera(scene.info, f))
+        ms, st = timed(cached, f, 1)
+        anim["cache_ms"] += ms
+        anim["fallbacks"] += st.cache_fallbacks
+        anim["frames"] += 1
+        if f % 25 == 0:
+            ms_p, _ = timed(plain, f, 1)
+            log(f"[bench] frame {f}: cache {ms:.2f} ms (judge {st.cache_judge:.3f}, {st.cache_queries} queries, "
+                f"{st.cache_train_vertices} training vertices), plain {ms_p:.2f} ms")
+        if f in frozen:
+            fx = load_fixture(f"c4_orbit_f{f:03d}")
+            if fx is None:
+                raise SystemExit(f"missing reference tests/golden/fp64/c4_orbit_f{f:03d}.npz")
+            cp = curve(plain, fx)
+            cc = curve(cached, fx)
+            e_target = cp[-1][1]
+            t_c = _time_to([(t, r) for t, r, _ in cc], e_target, extrapolate=False)
+            t_p = cp[-1][0]
+            # and the other way round: the plain tracer's time to the
+            # cache's final error (1/t beyond its last checkpoint)
+            t_p2 = _time_to([(t, r) for t, r, _ in cp], cc[-1][1], extrapolate=True)
+            views.append({"frame": f, "plain": cp, "cache": cc,
+                          "time_ratio_at_plain_error": None if t_c is None else round(t_c / t_p, 4),
+                          "time_ratio_at_cache_error": round(cc[-1][0] / t_p2, 4) if t_p2 else None})
+            log(f"[bench] view {f}: plain {cp[-1]}, cache {cc[-1]}, ratio {views[-1]['time_ratio_at_plain_error']}")
+            ctx.set_camera(orbit_camera(scene.info, f))
     ctx.close()
-
-    def relmse(a):
-        return float(np.mean((a - ref) ** 2 / (ref ** 2 + 0.01)))
-
-    e_p, e_c = relmse(img_p), relmse(img_c)
-    e_eq = e_p * st_p.gpu_ms / st_c.gpu_ms
+    ratios = [v["time_ratio_at_plain_error"] for v in views]
+    finite = [r for r in ratios if r is not None]
+    value = math.exp(sum(math.log(r) for r in finite) / len(finite)) if finite and len(finite) == len(ratios) else None
     out = {
-        "metric": "equal-time relMSE, radiance cache / plain",
-        "value": round(e_c / e_eq, 4),
+        "metric": "equal-error time, radiance cache / plain (relMSE vs FP64 oracle)",
+        "value": None if value is None else round(value, 4),
         "unit": "ratio",
         "n_gpus": 1,
         "higher_is_better": False,
         "data": "synthetic",
-        "dtype": "f32 tracing, bf16 network",
-        "config": {"workload": f"meshes_hdri_orbit_{W}x{H}_1spp_per_frame_cache", "frames": args.frames},
-        "plain": {"spp": spp, "gpu_ms": round(st_p.gpu_ms, 2), "relmse": e_p, "relmse_at_cache_time": e_eq},
-        "cache": {"spp": spp, "gpu_ms": round(st_c.gpu_ms, 2), "relmse": e_c},
-        "animation_ms_per_frame": {"plain": round(t_plain / args.frames, 3), "cache": round(t_cache / args.frames, 3)},
-        "reference_spp": ref_spp,
+        "dtype": "f32 tracing, bf16 network (fp32 accumulate)",
+        "config": {"workload": f"meshes_hdri_orbit_{W}x{H}_1spp_per_frame_cache", "frames": args.frames,
+                   "frozen_views": frozen, "eval_spp": args.eval_spp},
+        "views": views,
+        "animation_ms_per_frame": round(anim["cache_ms"] / max(1, anim["frames"]), 3),
+        "animation_fallbacks": anim["fallbacks"],
+        "curves": "[wall ms, relMSE, frames] after 1, 2, 4, ... frames from a cleared image",
     }
     print(json.dumps(out), flush=True)
 
+
+# ------------------------------------------------------------------ batch
 
 def batch_workload(args):
     """The 64-scene batch: 32 sphere scenes (50..5000 spheres) and 32 mesh
@@ -245,15 +492,16 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--spp", type=int, default=0, help="override samples per pixel (0: the workload's own)")
-    ap.add_argument("--workload", choices=["cornell", "meshes", "cache", "batch"], default="cornell",
-                    help="cornell: Cornell box 512x512 64 spp (the reported metric); meshes: 8 displaced "
-                         "octaspheres (2^20 triangles) under an importance-sampled HDRI, 1920x1080; cache: the "
-                         "mesh scene on an orbiting camera with the learned radiance cache, equal-time relMSE; "
-                         "batch: 64 sphere and mesh scenes of uneven size in one pt_render_batch")
-    ap.add_argument("--frames", type=int, default=60, help="cache workload: animation frames before the evaluation")
+    ap.add_argument("--workload", choices=["default", "meshes", "cornell", "random_spheres", "cache", "batch"],
+                    default="default",
+                    help="default: the mesh scene (headline) plus the Cornell box and random spheres; meshes, "
+                         "cornell, random_spheres: one of them alone; cache: the mesh scene's orbit with the "
+                         "learned radiance cache against the plain tracer at equal error; batch: 64 sphere and "
+                         "mesh scenes of uneven size in one pt_render_batch")
+    ap.add_argument("--frames", type=int, default=300, help="cache workload: orbit frames (frozen views at 50, "
+                                                            "100, 200, 300)")
+    ap.add_argument("--eval-spp", type=int, default=64, help="cache workload: frames rendered per frozen view")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-frames", type=int, default=64)
-    ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -265,154 +513,14 @@ def main():
               "(samples would shard by frame with one reduction of the FP64 accumulation)",
               file=sys.stderr)
         sys.exit(2)
-
-    import ptamd
-
     if args.workload == "cache":
         cache_workload(args)
-        return
-    if args.workload == "batch":
+    elif args.workload == "batch":
         batch_workload(args)
-        return
-    meshes = args.workload == "meshes"
-    scene = ptamd.Scene(ptamd.PT_SCENE_MESHES if meshes else ptamd.PT_SCENE_CORNELL)
-    info = scene.info
-    W, H = info.width, info.height
-    spp = args.spp or info.spp
-    params = scene.render_params()
-    ctx = ptamd.Context(0)
-    ctx.set_scene(scene.desc)
-    acc = ctx.build_accel()
-    ctx.set_camera(scene.camera)
-    log(f"[bench] accel: {acc.num_nodes} nodes, {acc.num_instances} instances, build {acc.build_ms:.2f} ms")
-
-    # traversal-counting pass (identical rays: the render is deterministic)
-    cparams = scene.render_params(flags=info.flags | ptamd.PT_FLAG_COUNT_TRAVERSAL)
-    cst = ctx.render(cparams, 0, spp)
-    log(f"[bench] counted pass: {cst.extension_rays} closest-hit, {cst.shadow_rays} shadow rays")
-
-    for _ in range(args.warmup):
-        ctx.render(params, 0, spp)
-    ctx.sync()
-
-    steps = []
-    ctx.sync()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        steps.append(ctx.render(params, 0, spp))
-        log(f"[bench] step {k}: {steps[-1].gpu_ms:.2f} ms")
-    ctx.sync()
-    wall = time.perf_counter() - t0
-
-    rays = sum(s.extension_rays + s.shadow_rays for s in steps)
-    ext_rays = sum(s.extension_rays for s in steps)
-    mrays = rays / wall / 1e6
-    ms_per_step = wall * 1e3 / args.steps
-    samples_per_s = args.steps * W * H * spp / wall
-    # the counted pass saw exactly the rays of one step
-    assert all(s.extension_rays == cst.extension_rays and s.shadow_rays == cst.shadow_rays for s in steps)
-
-    # Roofline of the dominant kernel (k_shade: hit and miss programs, light
-    # sampling, the shadow ray's traversal, continuation and compaction) and
-    # of the closest-hit traversal kernel (k_extend).
-    # record bytes: rects and boxes (32 B) in the Cornell box, triangles
-    # (48 B; the one ground rect counted as a triangle) in the mesh scene
-    prim_b = PRIM_BYTES[3] if meshes else PRIM_BYTES[1]
-    ext, paths, exh = cst.extension_rays, cst.paths, cst.exhausted_paths
-    shade_bytes_step = (ext * (64 + 16) - paths * 32 + (ext - paths + exh) * 64 + (paths - exh) * 20 +
-                        cst.shadow_node_fetches * 128 + cst.shadow_prim_fetches * prim_b +
-                        cst.shadow_instance_fetches * 64)
-    ext_bytes_step = (ext * (32 + 16) + paths * 16 + cst.node_fetches * 128 + cst.prim_fetches * prim_b +
-                      cst.instance_fetches * 64)
-    shade_ms = sum(s.shade_ms for s in steps) / len(steps)
-    ext_ms = sum(s.extend_ms for s in steps) / len(steps)
-    traffic = {} if meshes else load_traffic()
-
-    def roof(kernel, key, bytes_step, ms, launches, extra):
-        achieved = bytes_step / (ms * 1e-3) / 1e9
-        t = traffic.get(key)
-        return dict({
-            "kernel": kernel,
-            "bound": "hbm",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": t,
-            "algorithmic_bytes_per_launch": round(bytes_step / launches),
-            "avg_launch_ms": round(ms / launches, 4),
-        }, **extra)
-
-    roofline = roof("k_shade (shading + shadow-ray BVH4 traversal + compaction)", "k_shade", shade_bytes_step,
-                    shade_ms, steps[0].shade_launches, {
-                        "shadow_per_ray": {
-                            "nodes": round(cst.shadow_node_fetches / max(1, cst.shadow_rays), 3),
-                            "prims": round(cst.shadow_prim_fetches / max(1, cst.shadow_rays), 3),
-                            "instances": round(cst.shadow_instance_fetches / max(1, cst.shadow_rays), 3)}})
-    roofline_extend = roof("k_extend (closest-hit BVH4 traversal; camera paths started at depth 0)", "k_extend",
-                           ext_bytes_step, ext_ms,
-                           steps[0].extend_launches, {
-                               "per_ray": {
-                                   "nodes": round(cst.node_fetches / ext, 3),
-                                   "prims": round(cst.prim_fetches / ext, 3),
-                                   "instances": round(cst.instance_fetches / ext, 3)}})
-
-    cpu = None
-    if not args.no_cpu and not meshes:
-        try:
-            sys.path.insert(0, str(REPO / "oracle"))
-            import ptref
-
-            orc = ptref.OracleScene(scene.desc)
-            nthreads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
-            fr = args.cpu_frames
-            t1 = time.perf_counter()
-            _, _, st = orc.render(scene.camera, params, 0, fr, threads=nthreads)
-            dt = time.perf_counter() - t1
-            cpu = {
-                "value": round((int(st[1]) + int(st[2])) / dt / 1e6, 3),
-                "unit": "Mrays/s",
-                "cores": nthreads,
-                "kind": "port",
-                "sample": f"Cornell 512x512, frames 0-{fr - 1} of 64 ({W * H * fr} paths, "
-                          f"{int(st[1]) + int(st[2])} rays, {dt:.1f} s)",
-            }
-        except Exception as e:  # the baseline is reported, never the target
-            log(f"[bench] cpu baseline failed: {e}")
-
-    ctx.close()
-    out = {
-        "metric": "Mrays/s",
-        "value": round(mrays, 2),
-        "unit": "Mrays/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": (f"meshes_2p20_tris_hdri_envmis_{W}x{H}_{spp}spp_depth{info.max_depth}" if meshes else
-                         f"cornell_box_mis_{W}x{H}_{spp}spp_depth{info.max_depth}_rr{info.rr_min_depth}"),
-            "width": W, "height": H, "spp": spp, "max_depth": info.max_depth,
-            "rr_min_depth": info.rr_min_depth, "material_sort": True,
-            "env_mis": bool(info.flags & ptamd.PT_FLAG_ENV_MIS),
-        },
-        "samples_per_s": round(samples_per_s),
-        "rays_per_step": {"closest_hit": cst.extension_rays, "shadow": cst.shadow_rays},
-        "kernel_ms_per_step": {
-            "extend": round(ext_ms, 3),
-            "shade": round(shade_ms, 3),
-            "render_events": round(sum(s.gpu_ms for s in steps) / len(steps), 3),
-        },
-        "roofline": roofline,
-        "roofline_extend": roofline_extend,
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(out), flush=True)
+    elif args.workload == "default":
+        default_workload(args)
+    else:
+        single_workload(args)
 
 
 if __name__ == "__main__":

@@ -356,18 +356,26 @@ pt_status pt_render(struct pt_context* ctx, const pt_render_params* params,
                     uint32_t frame_begin, uint32_t frame_count,
                     pt_render_stats* out_or_null);
 
-/* No counterpart in the reference (it renders one scene per OptiX context):
+/* No counterpart in the reference (it renders one scene per OptiX context;
+ * this replaces its per-scene launch loop, main.cpp:26-39,264-265):
  * frames [0, frame_counts[i]) of every ctxs[i], i < n <= 256, in one
- * wavefront.  The (scene, frame) work items are cut into batches of about
- * 16 Mi paths; within a batch all paths share one path pool and one launch
- * per kernel and bounce, so many small or uneven scenes fill the GPU
- * together.  Every context needs a built scene and a camera, all on one
+ * wavefront.  The (scene, frame) work items are cut into sub-batches of
+ * about 16 Mi paths; within a sub-batch all paths share one path pool and
+ * one launch per kernel and bounce, so many small or uneven scenes fill the
+ * GPU together.  Every context needs a built scene and a camera, all on one
  * device; ctxs[0] runs the batch.  Each context's accumulation restarts at
  * frame 0 and ends equal, bit for bit, to pt_render(ctxs[i], &params[i],
- * 0, frame_counts[i]).  The radiance cache is not available in a batch;
- * material sort and traversal counting follow params[0].flags.  stats (if
- * given) cover the whole batch; this call synchronises when they are asked
- * for, else it is asynchronous on ctxs[0]'s stream. */
+ * 0, frame_counts[i]).  The radiance cache is not available in a batch.
+ * Batch-wide flags are taken from params[0].flags alone (the other scenes'
+ * values of these bits are ignored): PT_FLAG_MATERIAL_SORT,
+ * PT_FLAG_COUNT_TRAVERSAL, PT_FLAG_GRAPH (the whole batch -- clears,
+ * sub-batches, bounce loops, accumulations -- captured once into a graph
+ * and replayed while contexts, parameters, frame counts, scenes and
+ * buffers are unchanged) and PT_FLAG_TUNED (the shape pt_autotune_batch
+ * picked for this batch's contexts and parameters, if any).  Per-scene
+ * flags: PT_FLAG_ENV_MIS, max_depth, Russian roulette, resolution.  stats
+ * (if given) cover the whole batch; this call synchronises when they are
+ * asked for, else it is asynchronous on ctxs[0]'s stream. */
 pt_status pt_render_batch(struct pt_context* const* ctxs, const pt_render_params* params,
                           const uint32_t* frame_counts, uint32_t n, pt_render_stats* stats_or_null);
 
@@ -437,14 +445,27 @@ pt_status pt_read_env_tables(struct pt_context* ctx, uint64_t* cond, uint64_t* m
  * pt_render uses it for renders with PT_FLAG_TUNED and the same resolution,
  * depth and flags.  Results are identical under every shape. */
 typedef struct pt_tune_result {
-  uint32_t frames_per_batch;
+  uint32_t frames_per_batch;  /* pt_autotune only (0 from pt_autotune_batch) */
   int32_t material_sort;
   int32_t graph;
   uint32_t candidates;
-  double ms_per_frame;
+  double ms_per_frame;        /* the fastest candidate's time per frame     */
+  uint64_t paths_per_batch;   /* paths in flight per batch                  */
 } pt_tune_result;
 pt_status pt_autotune(struct pt_context* ctx, const pt_render_params* params, uint32_t frames,
                       pt_tune_result* out_or_null);
+
+/* No counterpart in the reference.  The same for a batch of scenes: renders
+ * the batch (pt_render_batch semantics, with the given frame counts, which
+ * may be fewer than the production render's) under 4, 8, 16 and 32 Mi
+ * paths per sub-batch x material sort on/off (when the batch has more than
+ * one shading program) x graph replay on/off, twice each, and keeps the
+ * fastest second run in ctxs[0] for pt_render_batch calls whose params[0]
+ * has PT_FLAG_TUNED and whose contexts and parameters (other than the
+ * shape's own flags and frames_per_batch) are the same.  Every context's
+ * accumulation is left holding the tuning render. */
+pt_status pt_autotune_batch(struct pt_context* const* ctxs, const pt_render_params* params,
+                            const uint32_t* frame_counts, uint32_t n, pt_tune_result* out_or_null);
 
 /* ------------------------------------------------- learned radiance cache
  * No counterpart in the reference.  A 96 -> 64 -> 64 -> 64 -> 64 -> 3 MLP

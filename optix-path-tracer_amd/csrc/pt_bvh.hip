@@ -316,38 +316,101 @@ __global__ void k_karras(const uint32_t* code, uint32_t n, uint32_t* left, uint3
   }
 }
 
-// bottom-up bounds: the second child to arrive at a node merges both
-PTD void load_box(uint32_t ref, uint32_t n_int, const uint32_t* idx, const float4* plo, const float4* phi,
-                  const float4* nlo, const float4* nhi, float4& lo, float4& hi) {
-  if (ref & LEAF_BIT) {
-    uint32_t p = idx[ref & ~LEAF_BIT];
-    lo = plo[p];
-    hi = phi[p];
-  } else {
-    const volatile float* a = reinterpret_cast<const volatile float*>(nlo + ref);
-    const volatile float* b = reinterpret_cast<const volatile float*>(nhi + ref);
-    lo = make_float4(a[0], a[1], a[2], 0.f);
-    hi = make_float4(b[0], b[1], b[2], 0.f);
+// Bottom-up bounds: the second child to arrive at a node merges both.
+// A Karras node's index is one end of its leaf range, so a node whose range
+// lies inside a block's span of REFIT_SPAN leaves is indexed inside that
+// span: phase 1 (k_refit_local) climbs from every leaf of the span with
+// arrival flags and boxes in LDS, stopping below the first node whose range
+// leaves the span (that node's inside child is appended to a list); phase 2
+// (k_refit_top, one workgroup) climbs from the listed children through the
+// straddling nodes to the root with arrival flags in global memory.  Every
+// node box is min/max of its left and right child's, so the result does not
+// depend on who merges.  No device-scope fence is needed: LDS arrivals are
+// ordered at workgroup scope, and phase 2 runs in one workgroup.
+constexpr uint32_t REFIT_SPAN = 2048;
+constexpr uint32_t NO_PARENT = 0xFFFFFFFFu;
+
+PTD void leaf_box(uint32_t ref, const uint32_t* idx, const float4* plo, const float4* phi, float4& lo, float4& hi) {
+  uint32_t p = idx[ref & ~LEAF_BIT];
+  lo = plo[p];
+  hi = phi[p];
+}
+PTD void merge(const float4& al, const float4& ah, const float4& bl, const float4& bh, float4& lo, float4& hi) {
+  lo = make_float4(fminf(al.x, bl.x), fminf(al.y, bl.y), fminf(al.z, bl.z), 0.f);
+  hi = make_float4(fmaxf(ah.x, bh.x), fmaxf(ah.y, bh.y), fmaxf(ah.z, bh.z), 0.f);
+}
+
+__global__ void __launch_bounds__(BB) k_refit_local(uint32_t n, const uint32_t* idx, const float4* plo,
+                                                    const float4* phi, const uint32_t* left, const uint32_t* right,
+                                                    const uint32_t* first, const uint32_t* last,
+                                                    const uint32_t* parent, float4* nlo, float4* nhi,
+                                                    uint32_t* list, uint32_t* list_count) {
+  __shared__ uint32_t arrived[REFIT_SPAN];
+  __shared__ float4 blo[REFIT_SPAN], bhi[REFIT_SPAN];
+  const uint32_t base = blockIdx.x * REFIT_SPAN;
+  const uint32_t end = min(n, base + REFIT_SPAN);
+  for (uint32_t k = threadIdx.x; k < REFIT_SPAN; k += BB) arrived[k] = 0u;
+  __syncthreads();
+  auto inside = [&](uint32_t node) { return first[node] >= base && last[node] < end; };
+  for (uint32_t l = base + threadIdx.x; l < end; l += BB) {
+    uint32_t child = LEAF_BIT | l;
+    uint32_t node = parent[n - 1 + l];
+    for (int climb = 0; climb < 4096 && node != NO_PARENT; climb++) {
+      if (!inside(node)) {
+        list[atomicAdd(list_count, 1u)] = child;
+        break;
+      }
+      __threadfence_block();
+      if (atomicAdd(&arrived[node - base], 1u) == 0u) break;
+      __threadfence_block();
+      const uint32_t lr = left[node], rr = right[node];
+      float4 al, ah, bl, bh, lo, hi;
+      if (lr & LEAF_BIT) leaf_box(lr, idx, plo, phi, al, ah);
+      else { al = blo[lr - base]; ah = bhi[lr - base]; }
+      if (rr & LEAF_BIT) leaf_box(rr, idx, plo, phi, bl, bh);
+      else { bl = blo[rr - base]; bh = bhi[rr - base]; }
+      merge(al, ah, bl, bh, lo, hi);
+      blo[node - base] = lo;
+      bhi[node - base] = hi;
+      nlo[node] = lo;
+      nhi[node] = hi;
+      child = node;
+      node = parent[node];
+    }
   }
 }
-__global__ void k_refit(uint32_t n, const uint32_t* idx, const float4* plo, const float4* phi, const uint32_t* left,
-                        const uint32_t* right, const uint32_t* parent, uint32_t* flags, float4* nlo, float4* nhi) {
-  for (uint32_t l = blockIdx.x * BB + threadIdx.x; l < n; l += gridDim.x * BB) {
-    uint32_t node = parent[n - 1 + l];
-    for (int climb = 0; climb < 4096 && node < n - 1; climb++) {
-      __threadfence();
-      if (atomicAdd(&flags[node], 1u) == 0) break;
-      __threadfence();
-      float4 al, ah, bl, bh;
-      load_box(left[node], n - 1, idx, plo, phi, nlo, nhi, al, ah);
-      load_box(right[node], n - 1, idx, plo, phi, nlo, nhi, bl, bh);
-      float4 lo = make_float4(fminf(al.x, bl.x), fminf(al.y, bl.y), fminf(al.z, bl.z), 0.f);
-      float4 hi = make_float4(fmaxf(ah.x, bh.x), fmaxf(ah.y, bh.y), fmaxf(ah.z, bh.z), 0.f);
+
+__global__ void __launch_bounds__(1024) k_refit_top(uint32_t n, const uint32_t* idx, const float4* plo,
+                                                    const float4* phi, const uint32_t* left, const uint32_t* right,
+                                                    const uint32_t* parent, uint32_t* arrived, float4* nlo,
+                                                    float4* nhi, const uint32_t* list, const uint32_t* list_count) {
+  const uint32_t count = *list_count;
+  for (uint32_t k = threadIdx.x; k < count; k += blockDim.x) {
+    const uint32_t start = list[k];
+    uint32_t node = parent[(start & LEAF_BIT) ? (n - 1 + (start & ~LEAF_BIT)) : start];
+    for (int climb = 0; climb < 4096 && node != NO_PARENT; climb++) {
+      __threadfence_block();
+      if (atomicAdd(&arrived[node], 1u) == 0u) break;
+      __threadfence_block();
+      float4 al, ah, bl, bh, lo, hi;
+      const uint32_t refs[2] = {left[node], right[node]};
+      for (int c = 0; c < 2; c++) {
+        float4 l4, h4;
+        if (refs[c] & LEAF_BIT) {
+          leaf_box(refs[c], idx, plo, phi, l4, h4);
+        } else {
+          const volatile float* a = reinterpret_cast<const volatile float*>(nlo + refs[c]);
+          const volatile float* b = reinterpret_cast<const volatile float*>(nhi + refs[c]);
+          l4 = make_float4(a[0], a[1], a[2], 0.f);
+          h4 = make_float4(b[0], b[1], b[2], 0.f);
+        }
+        if (c == 0) { al = l4; ah = h4; } else { bl = l4; bh = h4; }
+      }
+      merge(al, ah, bl, bh, lo, hi);
       volatile float* wl = reinterpret_cast<volatile float*>(nlo + node);
       volatile float* wh = reinterpret_cast<volatile float*>(nhi + node);
       wl[0] = lo.x; wl[1] = lo.y; wl[2] = lo.z; wl[3] = 0.f;
       wh[0] = hi.x; wh[1] = hi.y; wh[2] = hi.z; wh[3] = 0.f;
-      if (node == 0) break;
       node = parent[node];
     }
   }
@@ -518,8 +581,13 @@ uint32_t build_bvh4(BuildScratch& sb, uint32_t n, int leaf_max, uint4* nodes_out
   launch_fill_u32(sb.parent, 2ull * n - 1, 0xFFFFFFFFu, st);
   hipLaunchKernelGGL(k_karras, dim3(grid_for(n - 1)), dim3(BB), 0, st, ci, n, sb.left, sb.right, sb.first, sb.last,
                      sb.parent);
-  hipLaunchKernelGGL(k_refit, dim3(g), dim3(BB), 0, st, n, ii, sb.lo, sb.hi, sb.left, sb.right, sb.parent, sb.flags,
-                     sb.nlo, sb.nhi);
+  // refit: block-local climbs, then the nodes straddling the blocks' spans
+  // (list of their inside children in frontier_b, its length in misc[0])
+  launch_fill_u32(sb.misc, 1, 0u, st);
+  hipLaunchKernelGGL(k_refit_local, dim3((n + REFIT_SPAN - 1) / REFIT_SPAN), dim3(BB), 0, st, n, ii, sb.lo, sb.hi,
+                     sb.left, sb.right, sb.first, sb.last, sb.parent, sb.nlo, sb.nhi, sb.frontier_b, sb.misc);
+  hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, st, n, ii, sb.lo, sb.hi, sb.left, sb.right, sb.parent,
+                     sb.flags, sb.nlo, sb.nhi, sb.frontier_b, sb.misc);
   hipLaunchKernelGGL(k_copy_root, dim3(1), dim3(64), 0, st, sb.nlo, sb.nhi, root_lo_hi);
   if (n <= (uint32_t)leaf_max) {
     hipLaunchKernelGGL(k_single_leaf, dim3(1), dim3(64), 0, st, sb.nlo, sb.nhi, n, nodes_out, root_lo_hi);

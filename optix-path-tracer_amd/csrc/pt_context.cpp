@@ -156,6 +156,16 @@ struct pt_context {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   std::vector<uint8_t> graph_key;   // launch state the graph was captured for
+  // pt_render_batch run by this context: its captured graph and tuned shape
+  hipGraph_t batch_graph = nullptr;
+  hipGraphExec_t batch_graph_exec = nullptr;
+  std::vector<uint8_t> batch_graph_key;
+  struct BatchTuned {
+    bool valid = false;
+    uint64_t paths = 0;
+    bool sort = true, graph = false;
+    std::vector<uint8_t> key;       // the batch composition it was tuned for
+  } batch_tuned;
   struct Tuned {
     bool valid = false;
     uint32_t W = 0, H = 0, max_depth = 0, flags = 0;
@@ -305,6 +315,8 @@ void pt_destroy(pt_context* c) {
   dfree(c->batch_bufs);
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
+  if (c->batch_graph_exec) (void)hipGraphExecDestroy(c->batch_graph_exec);
+  if (c->batch_graph) (void)hipGraphDestroy(c->batch_graph);
   if (c->frame_dev) (void)hipFree(c->frame_dev);
   if (c->accum) (void)hipFree(c->accum);
   if (c->rgba) (void)hipFree(c->rgba);
@@ -1555,11 +1567,34 @@ pt_status pt_cache_train(pt_context* c, const uint16_t* features, const float* t
 // slots; each batch's paths share one path pool and one launch per kernel
 // and bounce, each path finding its scene through the index kept in its RNG
 // state.  The results of every context are those of its own pt_render.
-pt_status pt_render_batch(pt_context* const* ctxs, const pt_render_params* params, const uint32_t* frame_counts,
-                          uint32_t n, pt_render_stats* out) {
-  if (!ctxs || !params || !frame_counts || n == 0 || !ctxs[0]) return PT_ERR_INVALID_ARG;
+namespace {
+
+// launch shape of a batch of scenes
+struct BatchShape {
+  uint64_t paths = PATHS_TARGET;  // slots per sub-batch (a scene's frame is never split)
+  bool sort = true;               // material sort (when the batch has more than one shading program)
+  bool graph = false;             // the whole batch replayed from one captured graph
+};
+
+// the batch's composition: what a tuned shape is valid for (contexts,
+// resolutions, depths and flags other than the shape's own)
+std::vector<uint8_t> batch_composition(pt_context* const* ctxs, const pt_render_params* params, uint32_t n) {
+  std::vector<uint8_t> k;
+  auto add = [&](const void* p, size_t b) { k.insert(k.end(), (const uint8_t*)p, (const uint8_t*)p + b); };
+  add(&n, sizeof(n));
+  for (uint32_t i = 0; i < n; i++) {
+    add(&ctxs[i], sizeof(ctxs[i]));
+    pt_render_params q = params[i];
+    q.flags &= ~(uint32_t)(PT_FLAG_TUNED | PT_FLAG_GRAPH | PT_FLAG_MATERIAL_SORT);
+    q.frames_per_batch = 0;
+    add(&q, sizeof(q));
+  }
+  return k;
+}
+
+pt_status render_batch_shaped(pt_context* const* ctxs, const pt_render_params* params, const uint32_t* frame_counts,
+                              uint32_t n, pt_render_stats* out, const BatchShape& shape) {
   pt_context* B = ctxs[0];
-  if (n > 256) return fail(B, PT_ERR_INVALID_ARG, "at most 256 scenes per batch");
   uint32_t union_types = 0, max_depth = 0;
   bool env = false, ext = false;  // environment MIS; media, cylinders, moving spheres or the normal shader
   for (uint32_t i = 0; i < n; i++) {
@@ -1582,7 +1617,7 @@ pt_status pt_render_batch(pt_context* const* ctxs, const pt_render_params* param
   }
   (void)hipSetDevice(B->device);
   for (uint32_t i = 0; i < n; i++) HIPCHK(B, hipStreamSynchronize(ctxs[i]->stream));
-  // accumulation buffers, cleared on the batch's stream
+  // accumulation buffers (cleared on the batch's stream, below)
   for (uint32_t i = 0; i < n; i++) {
     pt_context* c = ctxs[i];
     const pt_render_params* rp = params + i;
@@ -1599,19 +1634,20 @@ pt_status pt_render_batch(pt_context* const* ctxs, const pt_render_params* param
       c->W = rp->width;
       c->H = rp->height;
     }
-    launch_fill_f64(c->accum, (uint64_t)WH * 3, 0.0, B->stream);
     c->frames = 0;
     c->paths_valid = false;
   }
   // work items (scene, frame) in scene-then-frame order, cut into batches
+  // of at most shape.paths slots (or one frame, if larger)
   std::vector<uint32_t> off, scene, frame;
   std::vector<size_t> first;      // first item of each batch
   std::vector<uint32_t> bslots;   // slots of each batch
   uint64_t cur = 0;
+  const uint64_t cap = std::min<uint64_t>(shape.paths, SLOT_MASK);
   for (uint32_t i = 0; i < n; i++) {
     const uint32_t WH = params[i].width * params[i].height;
     for (uint32_t f = 0; f < frame_counts[i]; f++) {
-      if (first.empty() || (cur > 0 && cur + WH > std::max<uint64_t>(PATHS_TARGET, WH))) {
+      if (first.empty() || (cur > 0 && cur + WH > std::max<uint64_t>(cap, WH))) {
         if (!first.empty()) bslots.push_back((uint32_t)cur);
         first.push_back(off.size());
         cur = 0;
@@ -1664,7 +1700,8 @@ pt_status pt_render_batch(pt_context* const* ctxs, const pt_render_params* param
   }
   const bool want_stats = out != nullptr;
   const bool count_trav = (params[0].flags & PT_FLAG_COUNT_TRAVERSAL) != 0;
-  const bool sort = (params[0].flags & PT_FLAG_MATERIAL_SORT) && __builtin_popcount(union_types & ~(1u << PT_MAT_DIFFUSE_LIGHT)) > 1;
+  const bool sort = shape.sort && __builtin_popcount(union_types & ~(1u << PT_MAT_DIFFUSE_LIGHT)) > 1;
+  const int matset = shade_material_set(union_types);
   DevPaths& P = B->paths;
   launch_fill_u32((uint32_t*)B->stats, (uint64_t)MAX_WAVES * STAT_WORDS * 2, 0u, B->stream);
   hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -1675,33 +1712,75 @@ pt_status pt_render_batch(pt_context* const* ctxs, const pt_render_params* param
   }
   RenderConsts rc0 = rcs[0];
   rc0.flags &= ~(uint32_t)PT_FLAG_RADIANCE_CACHE;
-  for (size_t b = 0; b + 1 < first.size(); b++) {
-    const uint32_t nslots = bslots[b];
-    const size_t k0 = first[b], k1 = first[b + 1];
-    if (nslots == 0) continue;
-    const int grid = grid_for(nslots);
-    launch_raygen_batch(P, B->d_rcs, B->d_item_off + k0, B->d_item_scene + k0, B->d_item_frame + k0,
-                        (uint32_t)(k1 - k0), nslots, grid, B->stream);
-    int in = 0, o = 1;
-    for (uint32_t depth = 0; depth < max_depth; depth++) {
-      launch_extend(B->dsc, P, in, depth == 0 ? 1e-6f : 1e-3f, B->spill, B->stats, count_trav, sort, grid, B->stream,
-                    B->d_scenes, nullptr, 0u, ext);
-      if (sort) launch_sort(P, in, grid, B->stream);
-      launch_shade(B->dsc, P, in, sort, o, depth, rc0, shade_material_set(union_types), count_trav, B->spill, B->stats, grid, B->stream,
-                   B->d_scenes, B->d_rcs, env, ext);
-      std::swap(in, o);
+  // the whole batch: clear the accumulations, then per sub-batch one raygen,
+  // the bounce loop over every live path, the paths that outlived it, and
+  // each scene's frames of the sub-batch accumulated in frame order
+  auto enqueue = [&]() {
+    for (uint32_t i = 0; i < n; i++) launch_fill_f64(ctxs[i]->accum, (uint64_t)rcs[i].WH * 3, 0.0, B->stream);
+    for (size_t b = 0; b + 1 < first.size(); b++) {
+      const uint32_t nslots = bslots[b];
+      const size_t k0 = first[b], k1 = first[b + 1];
+      if (nslots == 0) continue;
+      const int grid = grid_for(nslots);
+      launch_raygen_batch(P, B->d_rcs, B->d_item_off + k0, B->d_item_scene + k0, B->d_item_frame + k0,
+                          (uint32_t)(k1 - k0), nslots, grid, B->stream);
+      int in = 0, o = 1;
+      for (uint32_t depth = 0; depth < max_depth; depth++) {
+        launch_extend(B->dsc, P, in, depth == 0 ? 1e-6f : 1e-3f, B->spill, B->stats, count_trav, sort, grid,
+                      B->stream, B->d_scenes, nullptr, 0u, ext);
+        if (sort) launch_sort(P, in, grid, B->stream);
+        launch_shade(B->dsc, P, in, sort, o, depth, rc0, matset, count_trav, B->spill, B->stats, grid, B->stream,
+                     B->d_scenes, B->d_rcs, env, ext);
+        std::swap(in, o);
+      }
+      launch_exhausted(P, in, B->stats, grid, B->stream);
+      for (size_t k = k0; k < k1;) {
+        size_t e = k;
+        while (e < k1 && scene[e] == scene[k]) e++;
+        DevPaths Pk = P;
+        Pk.result = P.result + off[k];
+        const uint32_t i = scene[k];
+        launch_accumulate(Pk, rcs[i], (uint32_t)(e - k), ctxs[i]->accum, B->stats, grid_for(rcs[i].WH), B->stream);
+        k = e;
+      }
     }
-    launch_exhausted(P, in, B->stats, grid, B->stream);
-    // each scene's frames of this batch, in frame order
-    for (size_t k = k0; k < k1;) {
-      size_t e = k;
-      while (e < k1 && scene[e] == scene[k]) e++;
-      DevPaths Pk = P;
-      Pk.result = P.result + off[k];
-      const uint32_t i = scene[k];
-      launch_accumulate(Pk, rcs[i], (uint32_t)(e - k), ctxs[i]->accum, B->stats, grid_for(rcs[i].WH), B->stream);
-      k = e;
+  };
+  if (shape.graph) {
+    // replay when everything the captured launches take by value or by
+    // address is unchanged: contexts, parameters, frame counts, the scene
+    // and constant tables, the path pool, accumulations and the shape
+    std::vector<uint8_t> key;
+    auto add = [&](const void* p, size_t b) { key.insert(key.end(), (const uint8_t*)p, (const uint8_t*)p + b); };
+    add(&n, sizeof(n));
+    add(ctxs, n * sizeof(ctxs[0]));
+    add(params, n * sizeof(params[0]));
+    add(frame_counts, n * sizeof(frame_counts[0]));
+    add(scenes.data(), n * sizeof(DevScene));
+    add(rcs.data(), n * sizeof(RenderConsts));
+    for (uint32_t i = 0; i < n; i++) add(&ctxs[i]->accum, sizeof(double*));
+    add(&B->dsc, sizeof(B->dsc));
+    add(&P, sizeof(P));
+    const void* ptrs[7] = {B->spill, B->stats, B->d_scenes, B->d_rcs, B->d_item_off, B->d_item_scene, B->d_item_frame};
+    add(ptrs, sizeof(ptrs));
+    const uint64_t sh[4] = {cap, (uint64_t)sort, (uint64_t)count_trav, (uint64_t)max_depth};
+    add(sh, sizeof(sh));
+    if (!B->batch_graph_exec || key != B->batch_graph_key) {
+      if (B->batch_graph_exec) (void)hipGraphExecDestroy(B->batch_graph_exec);
+      if (B->batch_graph) (void)hipGraphDestroy(B->batch_graph);
+      B->batch_graph_exec = nullptr;
+      B->batch_graph = nullptr;
+      B->batch_graph_key.clear();
+      HIPCHK(B, hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+      enqueue();
+      hipGraph_t g = nullptr;
+      HIPCHK(B, hipStreamEndCapture(B->stream, &g));
+      B->batch_graph = g;
+      HIPCHK(B, hipGraphInstantiate(&B->batch_graph_exec, g, nullptr, nullptr, 0));
+      B->batch_graph_key = key;
     }
+    HIPCHK(B, hipGraphLaunch(B->batch_graph_exec, B->stream));
+  } else {
+    enqueue();
   }
   for (uint32_t i = 0; i < n; i++) ctxs[i]->frames = frame_counts[i];
   // later work on any batch context (reads, renders) comes after the batch
@@ -1736,6 +1815,93 @@ pt_status pt_render_batch(pt_context* const* ctxs, const pt_render_params* param
     uint32_t ovf = 0;
     HIPCHK(B, hipMemcpy(&ovf, P.counters + CTR_STACK_OVERFLOW, 4, hipMemcpyDeviceToHost));
     if (ovf) return fail(B, PT_ERR_DEVICE, "traversal stack overflow");
+  }
+  return PT_OK;
+}
+
+}  // namespace
+
+pt_status pt_render_batch(pt_context* const* ctxs, const pt_render_params* params, const uint32_t* frame_counts,
+                          uint32_t n, pt_render_stats* out) {
+  if (!ctxs || !params || !frame_counts || n == 0 || !ctxs[0]) return PT_ERR_INVALID_ARG;
+  pt_context* B = ctxs[0];
+  if (n > 256) return fail(B, PT_ERR_INVALID_ARG, "at most 256 scenes per batch");
+  // the shape: the one pt_autotune_batch picked for this composition when
+  // params[0] asks for it, else ~16 Mi paths per sub-batch and
+  // params[0]'s material-sort and graph flags
+  BatchShape shape;
+  const uint32_t f0 = params[0].flags;
+  shape.sort = (f0 & PT_FLAG_MATERIAL_SORT) != 0;
+  shape.graph = (f0 & PT_FLAG_GRAPH) != 0;
+  if ((f0 & PT_FLAG_TUNED) && B->batch_tuned.valid && B->batch_tuned.key == batch_composition(ctxs, params, n)) {
+    shape.paths = B->batch_tuned.paths;
+    shape.sort = B->batch_tuned.sort;
+    shape.graph = B->batch_tuned.graph;
+  }
+  return render_batch_shaped(ctxs, params, frame_counts, n, out, shape);
+}
+
+// Launch-shape autotuner of a batch of scenes: renders the batch (with the
+// given frame counts, which may be fewer than the production render's)
+// under every candidate shape, twice each, and keeps the fastest second
+// run.  Candidates: 4, 8, 16 and 32 Mi paths per sub-batch, material sort
+// on and off (when the batch has more than one shading program), graph
+// replay on and off.  Results are identical under every shape.
+pt_status pt_autotune_batch(pt_context* const* ctxs, const pt_render_params* params, const uint32_t* frame_counts,
+                            uint32_t n, pt_tune_result* out) {
+  if (!ctxs || !params || !frame_counts || n == 0 || !ctxs[0]) return PT_ERR_INVALID_ARG;
+  pt_context* B = ctxs[0];
+  if (n > 256) return fail(B, PT_ERR_INVALID_ARG, "at most 256 scenes per batch");
+  uint32_t union_types = 0;
+  uint64_t frames = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    if (!ctxs[i]) return fail(B, PT_ERR_INVALID_ARG, "null context in the batch");
+    union_types |= ctxs[i]->mat_types;
+    frames += frame_counts[i];
+  }
+  if (frames == 0) return fail(B, PT_ERR_INVALID_ARG, "no frames to tune on");
+  const bool can_sort = __builtin_popcount(union_types & ~(1u << PT_MAT_DIFFUSE_LIGHT)) > 1;
+  B->batch_tuned.valid = false;
+  BatchShape best;
+  double best_ms = 1e300;
+  uint32_t cands = 0;
+  for (uint64_t paths : {1ull << 22, 1ull << 23, 1ull << 24, 1ull << 25})
+    for (int so = 0; so < (can_sort ? 2 : 1); so++)
+      for (int gr = 0; gr < 2; gr++) {
+        BatchShape s;
+        s.paths = paths;
+        s.sort = can_sort && so == 0;
+        s.graph = gr != 0;
+        pt_render_stats st{};
+        for (int rep = 0; rep < 2; rep++) {
+          pt_status ps = render_batch_shaped(ctxs, params, frame_counts, n, &st, s);
+          if (ps != PT_OK) return ps;
+        }
+        cands++;
+        if (st.gpu_ms < best_ms) {
+          best_ms = st.gpu_ms;
+          best = s;
+        }
+      }
+  // the candidates' graphs are not kept (the production render's frame
+  // counts usually differ)
+  if (B->batch_graph_exec) (void)hipGraphExecDestroy(B->batch_graph_exec);
+  if (B->batch_graph) (void)hipGraphDestroy(B->batch_graph);
+  B->batch_graph_exec = nullptr;
+  B->batch_graph = nullptr;
+  B->batch_graph_key.clear();
+  B->batch_tuned.valid = true;
+  B->batch_tuned.paths = best.paths;
+  B->batch_tuned.sort = best.sort;
+  B->batch_tuned.graph = best.graph;
+  B->batch_tuned.key = batch_composition(ctxs, params, n);
+  if (out) {
+    out->frames_per_batch = 0;
+    out->material_sort = best.sort ? 1 : 0;
+    out->graph = best.graph ? 1 : 0;
+    out->candidates = cands;
+    out->ms_per_frame = best_ms / (double)frames;
+    out->paths_per_batch = best.paths;
   }
   return PT_OK;
 }
@@ -1796,6 +1962,7 @@ pt_status pt_autotune(pt_context* c, const pt_render_params* rp, uint32_t frames
     out->graph = best.graph ? 1 : 0;
     out->candidates = (uint32_t)(fpbs.size() * (can_sort ? 2 : 1) * 2);
     out->ms_per_frame = best.ms_per_frame;
+    out->paths_per_batch = (uint64_t)best.frames_per_batch * WH;
   }
   return PT_OK;
 }

@@ -153,7 +153,7 @@ class AccelStats(C.Structure):
 
 class TuneResult(C.Structure):
     _fields_ = [("frames_per_batch", C.c_uint32), ("material_sort", C.c_int32), ("graph", C.c_int32),
-                ("candidates", C.c_uint32), ("ms_per_frame", C.c_double)]
+                ("candidates", C.c_uint32), ("ms_per_frame", C.c_double), ("paths_per_batch", C.c_uint64)]
 
 
 class SceneInfo(C.Structure):
@@ -198,6 +198,8 @@ def _declare(lib):
         "pt_read_env_tables": (C.c_int32, [P, P, P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                            C.POINTER(C.c_uint64)]),
         "pt_autotune": (C.c_int32, [P, C.POINTER(RenderParams), C.c_uint32, C.POINTER(TuneResult)]),
+        "pt_autotune_batch": (C.c_int32, [C.POINTER(C.c_void_p), C.POINTER(RenderParams), C.POINTER(C.c_uint32),
+                                          C.c_uint32, C.POINTER(TuneResult)]),
         "pt_cache_init": (C.c_int32, [P, C.c_uint32]),
         "pt_cache_configure": (C.c_int32, [P, C.c_uint32, C.c_float, C.c_uint32, C.c_uint32, C.c_float]),
         "pt_cache_judge_configure": (C.c_int32, [P, C.c_float, C.c_uint32]),
@@ -468,6 +470,19 @@ def render_batch(contexts, params, frames, stats=True):
     contexts[0]._check(lib.pt_render_batch(hs, ps, fs, n, C.byref(s) if s is not None else None),
                        "pt_render_batch")
     return s
+
+
+def autotune_batch(contexts, params, frames):
+    """pt_autotune_batch: the fastest launch shape of this batch (kept in
+    contexts[0] for renders whose params[0] has PT_FLAG_TUNED)."""
+    n = len(contexts)
+    lib = contexts[0].lib
+    hs = (C.c_void_p * n)(*[c.h for c in contexts])
+    ps = (RenderParams * n)(*params)
+    fs = (C.c_uint32 * n)(*frames)
+    r = TuneResult()
+    contexts[0]._check(lib.pt_autotune_batch(hs, ps, fs, n, C.byref(r)), "pt_autotune_batch")
+    return r
 
 
 def header_symbols():

@@ -15,6 +15,11 @@ Views:
   c4_orbit_fNNN       the mesh scene seen from the orbit camera of frame
                       NNN (50, 100, 200, 300): the frozen evaluation views
                       of the radiance-cache workload
+  c3_batch            the 64 scenes of the batch workload (32 random-sphere
+                      scenes, 32 mesh scenes under the HDRI), each on a
+                      stratified 1/256 subset (one pixel per 16 x 16 block)
+                      at 8192 spp; one file, arrays pix_NN / mean_NN per
+                      scene NN
 """
 from __future__ import annotations
 
@@ -92,14 +97,38 @@ def make(name, spp, threads):
     print(f"[fp64] {name}: {len(pix)} pixels x {spp} spp in {dt:.0f} s", flush=True)
 
 
+def make_batch(spp, threads, scenes=64):
+    import ptref
+
+    out = {}
+    t0 = time.time()
+    for i in range(scenes):
+        sc = ptamd.Scene(ptamd.PT_SCENE_BATCH, 0, 0, i)
+        W, H = sc.info.width, sc.info.height
+        pix = stratified_pixels(W, H, block=16)
+        orc = ptref.OracleScene(sc.desc, fp64=True)
+        sums, _ = reference_sums(orc, sc.camera, sc.render_params(), pix, spp, threads)
+        out[f"pix_{i:02d}"] = pix
+        out[f"mean_{i:02d}"] = (sums / spp).astype(np.float32)
+        out[f"size_{i:02d}"] = np.array([W, H], np.uint32)
+        print(f"[fp64] c3_batch scene {i}: {W}x{H}, {len(pix)} pixels x {spp} spp, {time.time() - t0:.0f} s",
+              flush=True)
+    OUT.mkdir(exist_ok=True)
+    np.savez_compressed(OUT / "c3_batch.npz", spp=spp, frame_base=FRAME_BASE, block=16, scenes=scenes,
+                        seconds=time.time() - t0, threads=threads, **out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--spp", type=int, default=16384)
     ap.add_argument("--threads", type=int, default=os.cpu_count() or 8)
-    ap.add_argument("names", nargs="*", default=list(VIEWS))
+    ap.add_argument("names", nargs="*", default=list(VIEWS) + ["c3_batch"])
     a = ap.parse_args()
     for n in a.names:
-        make(n, a.spp, a.threads)
+        if n == "c3_batch":
+            make_batch(min(a.spp, 8192), a.threads)
+        else:
+            make(n, a.spp, a.threads)
 
 
 if __name__ == "__main__":

@@ -67,18 +67,25 @@ def test_batch_equals_single_renders(scenes):
 def test_batch_with_media_and_moving_spheres_equals_single_renders():
     """Scenes with constant media, isotropic phase functions, moving spheres
     and noise textures (The Next Week, the moving-spheres scene) batched with
-    the Cornell box: each accumulation equals its scene's own render bit for
-    bit (the media's free flights hash only the RNG state's low 24 bits, so
-    the scene index in the top byte leaves them unchanged)."""
+    the Cornell box and the mesh scene under the importance-sampled HDRI
+    (environment MIS): each accumulation equals its scene's own render bit
+    for bit (the media's free flights hash only the RNG state's low 24 bits,
+    so the scene index in the top byte leaves them unchanged).  The batch
+    runs the extended-kind shading with environment sampling; run a second
+    time with traversal counting, its ray, exhaustion and fetch counts are
+    the sums of the single renders'."""
     made = [make(ptamd.PT_SCENE_NEXT_WEEK, 48, 48), make(ptamd.PT_SCENE_MOVING_SPHERES, 40, 27),
-            make(ptamd.PT_SCENE_CORNELL, 32, 32)]
+            make(ptamd.PT_SCENE_CORNELL, 32, 32), make(ptamd.PT_SCENE_MESHES, 64, 36, 6)]
     try:
-        frames = [3, 2, 2]
+        frames = [3, 2, 2, 3]
+        assert made[3][0].info.flags & ptamd.PT_FLAG_ENV_MIS
         alone = []
         for (sc, ctx), f in zip(made, frames):
             say(f"[gpu] single render {sc.info.width}x{sc.info.height} x {f}")
             alone.append(single(sc, ctx, f))
-        say("[gpu] batch: next week, moving spheres, cornell")
+        counted = [sc.render_params(flags=sc.info.flags | ptamd.PT_FLAG_COUNT_TRAVERSAL) for sc, _ in made]
+        alone_counted = [ctx.render(p, 0, f) for (_, ctx), p, f in zip(made, counted, frames)]
+        say("[gpu] batch: next week, moving spheres, cornell, meshes")
         st = ptamd.render_batch([c for _, c in made], [s.render_params() for s, _ in made], frames)
         for (sc, ctx), (acc, n, _), f in zip(made, alone, frames):
             got, m = ctx.read_accum(sc.info.width, sc.info.height)
@@ -86,6 +93,97 @@ def test_batch_with_media_and_moving_spheres_equals_single_renders():
             assert np.array_equal(got, acc)
         assert st.extension_rays == sum(a[2].extension_rays for a in alone)
         assert st.shadow_rays == sum(a[2].shadow_rays for a in alone)
+        assert st.exhausted_paths == sum(a[2].exhausted_paths for a in alone)
+        say("[gpu] the same batch, counting traversal")
+        stc = ptamd.render_batch([c for _, c in made], counted, frames)
+        for k in ("extension_rays", "shadow_rays", "exhausted_paths", "node_fetches", "prim_fetches",
+                  "instance_fetches", "shadow_node_fetches", "shadow_prim_fetches", "shadow_instance_fetches"):
+            assert getattr(stc, k) == sum(getattr(a, k) for a in alone_counted), k
+        assert stc.node_fetches > 0 and stc.shadow_node_fetches > 0
+        for (sc, ctx), (acc, _, _) in zip(made, alone):
+            got, _ = ctx.read_accum(sc.info.width, sc.info.height)
+            assert np.array_equal(got, acc)
+    finally:
+        for _, ctx in made:
+            ctx.close()
+
+
+def test_batch_graph_replay_is_bit_identical(scenes):
+    """PT_FLAG_GRAPH in params[0]: the whole batch is captured once and
+    replayed; every replay (with other renders of the same contexts in
+    between) leaves each accumulation equal to the direct batch's."""
+    frames = [5, 3, 4, 7]
+    ctxs = [c for _, c in scenes]
+    params = [s.render_params() for s, _ in scenes]
+    ptamd.render_batch(ctxs, params, frames, stats=False)
+    direct = [c.read_accum(s.info.width, s.info.height)[0] for s, c in scenes]
+    gparams = list(params)
+    gparams[0] = scenes[0][0].render_params(flags=scenes[0][0].info.flags | ptamd.PT_FLAG_GRAPH)
+    for rep in range(3):
+        say(f"[gpu] batch graph replay {rep}")
+        st = ptamd.render_batch(ctxs, gparams, frames)
+        for (s, c), d in zip(scenes, direct):
+            got, m = c.read_accum(s.info.width, s.info.height)
+            assert m == frames[scenes.index((s, c))]
+            assert np.array_equal(got, d)
+        assert st.paths == sum(s.info.width * s.info.height * f for (s, _), f in zip(scenes, frames))
+        # eager work on the contexts between replays
+        single(scenes[1][0], scenes[1][1], 2)
+        ctxs[0].sync()
+
+
+def test_batch_autotune_picks_a_shape_with_identical_results(scenes):
+    frames = [5, 3, 4, 7]
+    ctxs = [c for _, c in scenes]
+    params = [s.render_params() for s, _ in scenes]
+    ptamd.render_batch(ctxs, params, frames, stats=False)
+    direct = [c.read_accum(s.info.width, s.info.height)[0] for s, c in scenes]
+    say("[gpu] batch autotune")
+    r = ptamd.autotune_batch(ctxs, params, [1, 1, 1, 1])
+    assert r.candidates == 16 and r.paths_per_batch in (1 << 22, 1 << 23, 1 << 24, 1 << 25)
+    assert r.ms_per_frame > 0
+    tuned = list(params)
+    tuned[0] = scenes[0][0].render_params(flags=scenes[0][0].info.flags | ptamd.PT_FLAG_TUNED)
+    ptamd.render_batch(ctxs, tuned, frames, stats=False)
+    for (s, c), d in zip(scenes, direct):
+        assert np.array_equal(c.read_accum(s.info.width, s.info.height)[0], d)
+
+
+# the 64-scene workload's largest mesh scenes (1.31 M and 1.18 M triangles),
+# its largest sphere scene (4478 spheres) and five more, at full resolution
+C3_SUBSET = [49, 63, 44, 33, 20, 1, 11, 7]
+
+
+def test_workload_subset_matches_oracle():
+    """Eight scenes of the 64-scene workload at their own resolutions, 2
+    frames each, in one batch: on each scene's stratified 1/64 pixel subset
+    the accumulation equals the oracle's per-pixel sums bit for bit, and
+    its relMSE against the scene's FP64 reference is reported."""
+    import ptref
+    from pathlib import Path
+    from ptamd.metrics import accum_at, relmse, stratified_pixels
+
+    made = [make(ptamd.PT_SCENE_BATCH, 0, 0, i) for i in C3_SUBSET]
+    fx = Path(__file__).resolve().parent / "golden" / "fp64" / "c3_batch.npz"
+    refs = np.load(fx) if fx.exists() else None
+    try:
+        frames = [2] * len(made)
+        say("[gpu] 8 scenes of the batch workload, 2 frames each")
+        st = ptamd.render_batch([c for _, c in made], [s.render_params() for s, _ in made], frames)
+        assert st.paths == sum(s.info.width * s.info.height * 2 for s, _ in made)
+        for i, (sc, ctx) in zip(C3_SUBSET, made):
+            W, H = sc.info.width, sc.info.height
+            acc, m = ctx.read_accum(W, H)
+            assert m == 2
+            pix = stratified_pixels(W, H)
+            say(f"[cpu] oracle, scene {i} ({W}x{H}, {len(pix)} pixels)")
+            sums, _ = ptref.OracleScene(sc.desc).render_pixels(sc.camera, sc.render_params(), 0, 2, pix, threads=16)
+            assert np.array_equal(accum_at(acc, pix, W, H), sums), i
+            if refs is not None:
+                rp = refs[f"pix_{i:02d}"]
+                e = relmse(accum_at(acc, rp, W, H) / 2, refs[f"mean_{i:02d}"])
+                say(f"[gpu] scene {i}: relMSE at 2 spp vs FP64 reference {e:.4f}")
+                assert np.isfinite(e)
     finally:
         for _, ctx in made:
             ctx.close()

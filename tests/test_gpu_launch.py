@@ -70,3 +70,39 @@ def test_autotune_picks_a_shape_with_identical_results(ctx):
     t = sc.render_params(flags=p.flags | ptamd.PT_FLAG_TUNED)
     a, _ = image(ctx, t, 0, 128)
     assert np.array_equal(a, ref)
+
+
+DRIVER = ptamd.REPO / "tests" / "_bin" / "facade_driver"
+
+
+@pytest.mark.parametrize("scene,cfg,W,H,frames", [("Cornell", ptamd.PT_SCENE_CORNELL, 96, 80, 4),
+                                                  ("RandomSpheres", ptamd.PT_SCENE_RANDOM_SPHERES, 90, 60, 3)])
+def test_facade_launch_loop_equals_render(tmp_path, scene, cfg, W, H, frames):
+    """The reference's main.cpp pattern through the host scene API: the
+    scene function, acc_buffer / display_buffer bound as context variables,
+    an empty launch that builds, then context["frame"] and
+    Context::launch(0, W, H) per frame, accBuffer->map() and
+    displayBuffer->map() (tests/facade_driver.cpp).  The mapped buffers
+    equal one pt_render of the same frames."""
+    import subprocess
+
+    if not DRIVER.exists():
+        pytest.fail("tests/_bin/facade_driver is not built (run __graft_entry__.build())")
+    out = tmp_path / "maps.bin"
+    say(f"[gpu] facade driver {scene} {W}x{H} x {frames}")
+    subprocess.run([str(DRIVER), scene, str(W), str(H), str(frames), str(out)], check=True, timeout=120)
+    raw = out.read_bytes()
+    acc_map = np.frombuffer(raw[:W * H * 16], np.float32).reshape(H, W, 4)
+    disp_map = np.frombuffer(raw[W * H * 16:], np.uint8).reshape(H, W, 4)
+    sc = ptamd.Scene(cfg, W, H)
+    with ptamd.Context(0) as ctx:
+        ctx.set_scene(sc.desc)
+        ctx.build_accel()
+        ctx.set_camera(sc.camera)
+        ctx.render(sc.render_params(), 0, frames)
+        acc, n = ctx.read_accum(W, H)
+        disp = ctx.read_display(W, H)
+    assert n == frames
+    assert np.array_equal(acc_map[..., :3], acc.astype(np.float32))
+    assert np.all(acc_map[..., 3] == frames)
+    assert np.array_equal(disp_map, disp)
