@@ -52,6 +52,11 @@ FIXTURES = REPO / "tests" / "golden" / "fp64"
 
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
+# radiance-cache network 96 -> 64 -> 64 -> 64 -> 64 -> 3: multiply-adds of
+# the forward pass (6144 + 3 x 4096 + 192), of the backward pass through the
+# activations (layers 2-5: 3 x 4096 + 192) and of the weight gradients
+MLP_FWD_FLOP = 2 * 18624
+MLP_TRAIN_FLOP = 2 * (18624 + 12480 + 18624)
 CPU_THREADS_MAX = 16  # the host-core share of one GPU on the measurement box
 # stored record bytes per primitive kind (pt_layout.hpp)
 PRIM_BYTES = {0: 16, 1: 32, 2: 32, 3: 48}
@@ -119,13 +124,15 @@ def load_fixture(name):
 
 
 def relmse_at(ctx, W, H, fixture):
-    """relMSE of the context's accumulated image against a converged
-    reference on its pixel subset."""
-    from ptamd.metrics import accum_at, relmse
+    """(relMSE, trimmed relMSE) of the context's accumulated image against a
+    converged reference on its pixel subset (trimmed: without the 0.1 % of
+    pixels of largest error)."""
+    from ptamd.metrics import accum_at, relmse, relmse_trimmed
 
     pix, ref, _ = fixture
     acc, frames = ctx.read_accum(W, H)
-    return relmse(accum_at(acc, pix, W, H) / max(frames, 1), ref)
+    img = accum_at(acc, pix, W, H) / max(frames, 1)
+    return relmse(img, ref), relmse_trimmed(img, ref)
 
 
 # (scene config, reference image, record bytes of the scene's primitives,
@@ -197,7 +204,7 @@ def plain_config(name, args, with_cpu=True):
     wall = time.perf_counter() - t0
     assert all(s.extension_rays == cst.extension_rays and s.shadow_rays == cst.shadow_rays for s in steps)
     fixture = load_fixture(fixname)
-    err = relmse_at(ctx, W, H, fixture) if fixture else None
+    err, err_t = relmse_at(ctx, W, H, fixture) if fixture else (None, None)
     # the build again, timed alone (the scene and its structures are resident)
     builds = [ctx.build_accel().build_ms for _ in range(3)]
     ctx.close()
@@ -254,6 +261,7 @@ def plain_config(name, args, with_cpu=True):
                    "env_mis": bool(info.flags & ptamd.PT_FLAG_ENV_MIS)},
         "samples_per_s": round(args.steps * W * H * spp / wall),
         "relmse_vs_fp64": None if err is None else round(err, 6),
+        "relmse_trimmed_vs_fp64": None if err_t is None else round(err_t, 6),
         "relmse_reference": None if fixture is None else
         f"oracle FP64 build, {fixture[2]} spp, {len(fixture[0])} pixels (stratified 1/64)",
         "rays_per_step": {"closest_hit": cst.extension_rays, "shadow": cst.shadow_rays},
@@ -356,6 +364,8 @@ def cache_workload(args):
         return (time.perf_counter() - t) * 1e3, st
 
     def curve(p, fixture):
+        """[(wall ms, relMSE, trimmed relMSE, frames)] after 1, 2, 4, ...
+        frames from a cleared image"""
         pts, t, done = [], 0.0, 0
         n = 1
         while done < args.eval_spp:
@@ -363,7 +373,8 @@ def cache_workload(args):
             ms, _ = timed(p, done, k)
             t += ms
             done += k
-            pts.append((round(t, 3), relmse_at(ctx, W, H, fixture), done))
+            e, et = relmse_at(ctx, W, H, fixture)
+            pts.append((round(t, 3), round(e, 6), round(et, 6), done))
             n *= 2
         return pts
 
@@ -371,12 +382,17 @@ def cache_workload(args):
     ctx.set_camera(orbit_camera(scene.info, 0))
     timed(plain, 0, 1)
     views, anim = [], {"plain_ms": 0.0, "cache_ms": 0.0, "fallbacks": 0, "frames": 0}
+    mlp = {"train_ms": 0.0, "infer_ms": 0.0, "train_samples": 0, "infer_samples": 0}
     for f in range(args.frames + 1):
         ctx.set_camera(orbit_camera(scene.info, f))
         ms, st = timed(cached, f, 1)
         anim["cache_ms"] += ms
         anim["fallbacks"] += st.cache_fallbacks
         anim["frames"] += 1
+        mlp["train_ms"] += st.cache_train_ms
+        mlp["infer_ms"] += st.cache_infer_ms
+        mlp["train_samples"] += st.cache_train_samples
+        mlp["infer_samples"] += st.cache_infer_samples
         if f % 25 == 0:
             ms_p, _ = timed(plain, f, 1)
             log(f"[bench] frame {f}: cache {ms:.2f} ms (judge {st.cache_judge:.3f}, {st.cache_queries} queries, "
@@ -387,23 +403,39 @@ def cache_workload(args):
                 raise SystemExit(f"missing reference tests/golden/fp64/c4_orbit_f{f:03d}.npz")
             cp = curve(plain, fx)
             cc = curve(cached, fx)
-            e_target = cp[-1][1]
-            t_c = _time_to([(t, r) for t, r, _ in cc], e_target, extrapolate=False)
-            t_p = cp[-1][0]
-            # and the other way round: the plain tracer's time to the
-            # cache's final error (1/t beyond its last checkpoint)
-            t_p2 = _time_to([(t, r) for t, r, _ in cp], cc[-1][1], extrapolate=True)
-            views.append({"frame": f, "plain": cp, "cache": cc,
-                          "time_ratio_at_plain_error": None if t_c is None else round(t_c / t_p, 4),
-                          "time_ratio_at_cache_error": round(cc[-1][0] / t_p2, 4) if t_p2 else None})
-            log(f"[bench] view {f}: plain {cp[-1]}, cache {cc[-1]}, ratio {views[-1]['time_ratio_at_plain_error']}")
+            v = {"frame": f, "plain": cp, "cache": cc}
+            for name, col in (("trimmed", 2), ("full", 1)):
+                # the cache's time to reach the plain tracer's error after
+                # all its frames, over the plain tracer's time; and the
+                # plain tracer's time (1/t beyond its last checkpoint) to
+                # reach the cache's final error, over the cache's time
+                t_p = cp[-1][0]
+                t_c = _time_to([(q[0], q[col]) for q in cc], cp[-1][col], extrapolate=False)
+                t_p2 = _time_to([(q[0], q[col]) for q in cp], cc[-1][col], extrapolate=True)
+                v[f"time_ratio_at_plain_error_{name}"] = None if t_c is None else round(t_c / t_p, 4)
+                v[f"time_ratio_at_cache_error_{name}"] = round(cc[-1][0] / t_p2, 4) if t_p2 else None
+            views.append(v)
+            log(f"[bench] view {f}: plain {cp[-1]}, cache {cc[-1]}, ratio (trimmed) "
+                f"{v['time_ratio_at_plain_error_trimmed']}")
             ctx.set_camera(orbit_camera(scene.info, f))
     ctx.close()
-    ratios = [v["time_ratio_at_plain_error"] for v in views]
+    # MFMA roofline of the network kernels over the animation (HIP events
+    # around the training steps and the inference launches)
+    flop = mlp["train_samples"] * MLP_TRAIN_FLOP + mlp["infer_samples"] * MLP_FWD_FLOP
+    net_ms = mlp["train_ms"] + mlp["infer_ms"]
+    achieved = flop / (net_ms * 1e-3) / 1e12 if net_ms > 0 else 0.0
+    roofline = {"kernel": "k_mlp_* (BF16 MFMA forward, backward, weight gradients, Adam; inference)",
+                "bound": "mfma", "achieved": round(achieved, 3), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / BF16_PEAK_TFLOPS, 6), "traffic": None,
+                "train": {"samples": mlp["train_samples"], "ms": round(mlp["train_ms"], 3),
+                          "flop_per_sample": MLP_TRAIN_FLOP},
+                "infer": {"samples": mlp["infer_samples"], "ms": round(mlp["infer_ms"], 3),
+                          "flop_per_sample": MLP_FWD_FLOP}}
+    ratios = [v["time_ratio_at_plain_error_trimmed"] for v in views]
     finite = [r for r in ratios if r is not None]
     value = math.exp(sum(math.log(r) for r in finite) / len(finite)) if finite and len(finite) == len(ratios) else None
     out = {
-        "metric": "equal-error time, radiance cache / plain (relMSE vs FP64 oracle)",
+        "metric": "equal-error time, radiance cache / plain (trimmed relMSE vs FP64 oracle)",
         "value": None if value is None else round(value, 4),
         "unit": "ratio",
         "n_gpus": 1,
@@ -415,7 +447,10 @@ def cache_workload(args):
         "views": views,
         "animation_ms_per_frame": round(anim["cache_ms"] / max(1, anim["frames"]), 3),
         "animation_fallbacks": anim["fallbacks"],
-        "curves": "[wall ms, relMSE, frames] after 1, 2, 4, ... frames from a cleared image",
+        "roofline": roofline,
+        "curves": "[wall ms, relMSE, trimmed relMSE, frames] after 1, 2, 4, ... frames from a cleared image; "
+                  "value = geometric mean over the views of time_ratio_at_plain_error_trimmed (trimmed: without "
+                  "the 0.1 % of pixels of largest error, whose rare sun caustics decide the full relMSE)",
     }
     print(json.dumps(out), flush=True)
 
@@ -423,39 +458,62 @@ def cache_workload(args):
 # ------------------------------------------------------------------ batch
 
 def batch_workload(args):
-    """The 64-scene batch: 32 sphere scenes (50..5000 spheres) and 32 mesh
-    scenes (10k..2M triangles, sky light), square resolutions 256..1024, 4,
-    16 or 64 spp, every scene's frames rendered by one pt_render_batch per
-    step.  Beside it, the same scenes rendered one pt_render each, back to
-    back (the GPU time the batch saves on small and uneven scenes)."""
+    """The 64-scene batch: 32 sphere scenes (50..5000 spheres, sky) and 32
+    mesh scenes (14k..1.3M triangles under the HDRI), square resolutions
+    256..1024, 4, 16 or 64 spp, every scene's frames rendered by one
+    pt_render_batch per step in the launch shape pt_autotune_batch picked
+    (paths per sub-batch, material sort, graph replay; tuned on a quarter of
+    the frames).  Per-scene relMSE against the oracle's FP64 8192-spp
+    references on each scene's stratified 1/256 pixel subset.  Beside it,
+    the same scenes rendered one pt_render each, back to back."""
     import ptamd
+    from ptamd.metrics import accum_at, relmse, relmse_trimmed
 
     scenes = [ptamd.Scene(ptamd.PT_SCENE_BATCH, 0, 0, i) for i in range(64)]
     ctxs = [ptamd.Context(0) for _ in scenes]
     t0 = time.perf_counter()
     prims = 0
+    build_ms = 0.0
     for sc, ctx in zip(scenes, ctxs):
         ctx.set_scene(sc.desc)
         a = ctx.build_accel()
         ctx.set_camera(sc.camera)
         prims += a.num_primitives
-    log(f"[bench] 64 scenes built ({prims} primitives) in {time.perf_counter() - t0:.1f} s")
+        build_ms += a.build_ms
+    log(f"[bench] 64 scenes built ({prims} primitives, {build_ms:.1f} ms of GPU build) in "
+        f"{time.perf_counter() - t0:.1f} s")
     params = [sc.render_params() for sc in scenes]
     frames = [sc.info.spp for sc in scenes]
     paths = sum(sc.info.width * sc.info.height * sc.info.spp for sc in scenes)
-    st = ptamd.render_batch(ctxs, params, frames)  # rays of one step (and warm-up)
+    t0 = time.perf_counter()
+    tune = ptamd.autotune_batch(ctxs, params, [max(1, f // 4) for f in frames])
+    log(f"[bench] batch autotune: {tune.paths_per_batch >> 20} Mi paths per sub-batch, sort {tune.material_sort}, "
+        f"graph {tune.graph} ({tune.candidates} candidates, {time.perf_counter() - t0:.1f} s)")
+    tuned = list(params)
+    tuned[0] = scenes[0].render_params(flags=scenes[0].info.flags | ptamd.PT_FLAG_TUNED)
+    st = ptamd.render_batch(ctxs, tuned, frames)  # rays of one step (and warm-up)
     rays_step = st.extension_rays + st.shadow_rays
     log(f"[bench] batch: {paths} paths, {st.extension_rays} closest-hit + {st.shadow_rays} shadow rays, "
         f"{st.gpu_ms:.1f} ms")
     for _ in range(args.warmup):
-        ptamd.render_batch(ctxs, params, frames, stats=False)
+        ptamd.render_batch(ctxs, tuned, frames, stats=False)
     ctxs[0].sync()
     t0 = time.perf_counter()
     for k in range(args.steps):
-        ptamd.render_batch(ctxs, params, frames, stats=False)
+        ptamd.render_batch(ctxs, tuned, frames, stats=False)
         ctxs[0].sync()
         log(f"[bench] step {k} done")
     wall = time.perf_counter() - t0
+    errs, errs_t = [], []
+    fx = FIXTURES / "c3_batch.npz"
+    refs = np.load(fx) if fx.exists() else None
+    if refs is not None:
+        for i, (sc, ctx) in enumerate(zip(scenes, ctxs)):
+            W, H = sc.info.width, sc.info.height
+            acc, n = ctx.read_accum(W, H)
+            img, ref = accum_at(acc, refs[f"pix_{i:02d}"], W, H) / n, refs[f"mean_{i:02d}"]
+            errs.append(round(relmse(img, ref), 6))
+            errs_t.append(round(relmse_trimmed(img, ref), 6))
     # the same work as 64 separate renders (one scene at a time)
     seq_ms = 0.0
     for sc, ctx, p, f in zip(scenes, ctxs, params, frames):
@@ -476,12 +534,21 @@ def batch_workload(args):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "batch_64_scenes_32_spheres_50-5000_32_meshes_10k-2M_256-1024px_4-64spp",
+        "config": {"workload": "batch_64_scenes_32_spheres_50-5000_32_meshes_14k-1.3M_256-1024px_4-64spp",
                    "paths": paths, "primitives": prims},
         "samples_per_s": round(paths * args.steps / wall),
         "rays_per_step": {"closest_hit": st.extension_rays, "shadow": st.shadow_rays},
+        "tuned_shape": {"paths_per_batch": tune.paths_per_batch, "material_sort": bool(tune.material_sort),
+                        "graph": bool(tune.graph), "candidates": tune.candidates},
+        "build_ms_total": round(build_ms, 2),
         "one_render_per_scene_ms": round(seq_ms, 2),
         "batch_speedup": round(seq_ms / ms, 3),
+        "relmse_vs_fp64": None if not errs else {
+            "per_scene": errs, "mean": round(float(np.mean(errs)), 6), "median": round(float(np.median(errs)), 6),
+            "max": round(float(np.max(errs)), 6),
+            "trimmed_per_scene": errs_t, "trimmed_median": round(float(np.median(errs_t)), 6),
+            "reference": "oracle FP64 build, 8192 spp, stratified 1/256 pixel subset per scene; trimmed: without "
+                         "the 0.1 % of pixels of largest error"},
     }
     print(json.dumps(out), flush=True)
 

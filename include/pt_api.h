@@ -298,6 +298,12 @@ typedef struct pt_render_stats {
                                ratio (-1: not judged, NaN: outputs not finite) */
   uint32_t cache_fallbacks; /* batches of this call rendered again by the
                                plain tracer because the judge failed them  */
+  /* radiance cache, whole call (HIP events around the network kernels):  */
+  double cache_train_ms;    /* training steps (forward, backward, weight
+                               gradients, Adam)                            */
+  double cache_infer_ms;    /* inference over the queries and the judge's rows */
+  uint64_t cache_train_samples;  /* samples the training steps processed   */
+  uint64_t cache_infer_samples;  /* samples inference processed            */
 } pt_render_stats;
 
 typedef struct pt_accel_stats {
@@ -410,6 +416,7 @@ pt_status pt_trace_closest(struct pt_context* ctx, const pt_ray* rays,
 /* Same on device-resident arrays, asynchronous on the context stream. */
 pt_status pt_trace_closest_device(struct pt_context* ctx, const pt_ray* d_rays,
                                   uint64_t n, pt_hit* d_hits);
+
 /* Counted variant: per-ray node, primitive and instance fetch counts
  * (3 uint32 per ray), host arrays. */
 pt_status pt_trace_closest_counted(struct pt_context* ctx, const pt_ray* rays,
@@ -499,10 +506,11 @@ pt_status pt_cache_train(struct pt_context* ctx, const uint16_t* features, const
  * L += T * expm1(min(network, 20)) (negative and NaN answers count 0); a
  * `train_fraction` of the paths (hashed on pixel and frame) are traced to
  * the end by the plain tracer and record every vertex; after each batch of
- * frames the network takes `steps` Adam steps of `batch` vertices each
- * (steps x batch vertices spread evenly over the batch's records, each used
- * once) with learning rate `lr`, is judged (pt_cache_judge_configure), then
- * answers the batch's queries.
+ * frames the network takes `steps` Adam steps of up to `batch` vertices
+ * each (steps x batch vertices spread evenly over the batch's records,
+ * each used once; when the batch recorded fewer, the steps cycle through
+ * all of them) with learning rate `lr`, is judged
+ * (pt_cache_judge_configure), then answers the batch's queries.
  * Defaults: 2, 0.03, 16384, 4, 1e-3. */
 pt_status pt_cache_configure(struct pt_context* ctx, uint32_t query_depth, float train_fraction,
                              uint32_t batch, uint32_t steps, float lr);

@@ -440,73 +440,138 @@ PTD void ref_box(const BinTree& t, uint32_t r, float4& lo, float4& hi) {
   }
 }
 // expansion: repeatedly open the expandable slot of largest half area
-__global__ void k_collapse_expand(BinTree t, const uint32_t* frontier, uint32_t F, int leaf_max, uint32_t* slots,
-                                  uint32_t* nchild) {
-  for (uint32_t w = blockIdx.x * BB + threadIdx.x; w < F; w += gridDim.x * BB) {
-    uint32_t m = frontier[w];
-    uint32_t s[4] = {t.left[m], t.right[m], EMPTY_CHILD, EMPTY_CHILD};
-    int ns = 2;
-    while (ns < 4) {
-      int best = -1;
-      float best_area = -1.0f;
-      for (int k = 0; k < ns; k++) {
-        uint32_t r = s[k];
-        if ((r & LEAF_BIT) || ref_size(t, r) <= (uint32_t)leaf_max) continue;
-        float a = half_area(t.nlo[r], t.nhi[r]);
-        if (a > best_area) { best_area = a; best = k; }
-      }
-      if (best < 0) break;
-      uint32_t r = s[best];
-      for (int k = ns; k > best + 1; k--) s[k] = s[k - 1];
-      s[best] = t.left[r];
-      s[best + 1] = t.right[r];
-      ns++;
+PTD uint32_t collapse_expand(const BinTree& t, uint32_t m, int leaf_max, uint32_t s[4]) {
+  s[0] = t.left[m];
+  s[1] = t.right[m];
+  s[2] = s[3] = EMPTY_CHILD;
+  int ns = 2;
+  while (ns < 4) {
+    int best = -1;
+    float best_area = -1.0f;
+    for (int k = 0; k < ns; k++) {
+      uint32_t r = s[k];
+      if ((r & LEAF_BIT) || ref_size(t, r) <= (uint32_t)leaf_max) continue;
+      float a = half_area(t.nlo[r], t.nhi[r]);
+      if (a > best_area) { best_area = a; best = k; }
     }
-    uint32_t ni = 0;
-    for (int k = 0; k < 4; k++) {
-      slots[4 * w + k] = s[k];
-      if (s[k] != EMPTY_CHILD && !(s[k] & LEAF_BIT) && ref_size(t, s[k]) > (uint32_t)leaf_max) ni++;
-    }
-    nchild[w] = ni;
+    if (best < 0) break;
+    uint32_t r = s[best];
+    for (int k = ns; k > best + 1; k--) s[k] = s[k - 1];
+    s[best] = t.left[r];
+    s[best + 1] = t.right[r];
+    ns++;
   }
+  uint32_t ni = 0;
+  for (int k = 0; k < 4; k++)
+    if (s[k] != EMPTY_CHILD && !(s[k] & LEAF_BIT) && ref_size(t, s[k]) > (uint32_t)leaf_max) ni++;
+  return ni;
 }
-__global__ void k_collapse_emit(BinTree t, const uint32_t* frontier, uint32_t F, int leaf_max, const uint32_t* slots,
-                                const uint32_t* offs, uint32_t level_base, uint32_t next_base, uint4* nodes,
-                                uint32_t* next_frontier) {
-  for (uint32_t w = blockIdx.x * BB + threadIdx.x; w < F; w += gridDim.x * BB) {
-    uint32_t wl[6][4], ch[4], cn[4];
-    uint32_t o = offs[w];
-    for (int k = 0; k < 4; k++) {
-      uint32_t r = slots[4 * w + k];
-      if (r == EMPTY_CHILD) {
-        wl[0][k] = __float_as_uint(FINF); wl[1][k] = __float_as_uint(-FINF);
-        wl[2][k] = __float_as_uint(FINF); wl[3][k] = __float_as_uint(-FINF);
-        wl[4][k] = __float_as_uint(FINF); wl[5][k] = __float_as_uint(-FINF);
-        ch[k] = EMPTY_CHILD;
-        cn[k] = 0;
-        continue;
-      }
-      float4 lo, hi;
-      ref_box(t, r, lo, hi);
-      wl[0][k] = __float_as_uint(lo.x); wl[1][k] = __float_as_uint(hi.x);
-      wl[2][k] = __float_as_uint(lo.y); wl[3][k] = __float_as_uint(hi.y);
-      wl[4][k] = __float_as_uint(lo.z); wl[5][k] = __float_as_uint(hi.z);
-      uint32_t sz = ref_size(t, r);
-      if ((r & LEAF_BIT) || sz <= (uint32_t)leaf_max) {
-        ch[k] = ref_first(t, r);
-        cn[k] = sz;
-      } else {
-        ch[k] = next_base + o;
-        cn[k] = 0;
-        next_frontier[o] = r;
-        o++;
-      }
+// one BVH4 node from its four slots; internal children are numbered from
+// next_base + o in slot order and appended to the next frontier
+PTD void collapse_emit(const BinTree& t, const uint32_t s[4], uint32_t o, int leaf_max, uint32_t next_base,
+                       uint4* nd, uint32_t* next_frontier, uint32_t cap) {
+  uint32_t wl[6][4], ch[4], cn[4];
+  for (int k = 0; k < 4; k++) {
+    uint32_t r = s[k];
+    if (r == EMPTY_CHILD) {
+      wl[0][k] = __float_as_uint(FINF); wl[1][k] = __float_as_uint(-FINF);
+      wl[2][k] = __float_as_uint(FINF); wl[3][k] = __float_as_uint(-FINF);
+      wl[4][k] = __float_as_uint(FINF); wl[5][k] = __float_as_uint(-FINF);
+      ch[k] = EMPTY_CHILD;
+      cn[k] = 0;
+      continue;
     }
-    uint4* nd = nodes + (size_t)(level_base + w) * 8;
-    for (int q = 0; q < 6; q++) nd[q] = make_uint4(wl[q][0], wl[q][1], wl[q][2], wl[q][3]);
-    nd[6] = make_uint4(ch[0], ch[1], ch[2], ch[3]);
-    nd[7] = make_uint4(cn[0], cn[1], cn[2], cn[3]);
+    float4 lo, hi;
+    ref_box(t, r, lo, hi);
+    wl[0][k] = __float_as_uint(lo.x); wl[1][k] = __float_as_uint(hi.x);
+    wl[2][k] = __float_as_uint(lo.y); wl[3][k] = __float_as_uint(hi.y);
+    wl[4][k] = __float_as_uint(lo.z); wl[5][k] = __float_as_uint(hi.z);
+    uint32_t sz = ref_size(t, r);
+    if ((r & LEAF_BIT) || sz <= (uint32_t)leaf_max) {
+      ch[k] = ref_first(t, r);
+      cn[k] = sz;
+    } else {
+      ch[k] = next_base + o;
+      cn[k] = 0;
+      if (o < cap) next_frontier[o] = r;
+      o++;
+    }
   }
+  for (int q = 0; q < 6; q++) nd[q] = make_uint4(wl[q][0], wl[q][1], wl[q][2], wl[q][3]);
+  nd[6] = make_uint4(ch[0], ch[1], ch[2], ch[3]);
+  nd[7] = make_uint4(cn[0], cn[1], cn[2], cn[3]);
+}
+
+// Breadth-first collapse of the binary hierarchy into BVH4 nodes, every
+// level in one workgroup: expand each frontier node into up to four slots,
+// exclusive-scan the numbers of internal children (which numbers the next
+// level in frontier then slot order), emit the nodes.  One launch per
+// structure and no host round trip per level.  result[0] = node count, or
+// 0 for a malformed hierarchy (more than 128 levels or more than n nodes).
+constexpr int CB = 1024;
+__global__ void __launch_bounds__(CB) k_collapse_all(BinTree t, uint32_t n, int leaf_max, uint32_t cap,
+                                                     uint32_t* fa, uint32_t* fb, uint32_t* slots, uint32_t* nchild,
+                                                     uint4* nodes, uint32_t* result) {
+  __shared__ uint32_t wsum[CB / 64];
+  const uint32_t tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  if (tid == 0) fa[0] = 0u;  // the binary root
+  __syncthreads();
+  uint32_t F = 1, level_base = 0, next_base = 1;
+  uint32_t *cur = fa, *nxt = fb;
+  bool ok = true;
+  int levels = 0;
+  while (F > 0) {
+    if (++levels > 128 || next_base > n || F > cap) {
+      ok = false;
+      break;
+    }
+    for (uint32_t w = tid; w < F; w += CB) {
+      uint32_t sl[4];
+      nchild[w] = collapse_expand(t, cur[w], leaf_max, sl);
+      for (int k = 0; k < 4; k++) slots[4 * w + k] = sl[k];
+    }
+    // exclusive scan of nchild[0, F) in chunks of CB (each thread scans the
+    // entries it wrote above, so no other thread's writes are read)
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < F; b0 += CB) {
+      const uint32_t i = b0 + tid;
+      const uint32_t v = i < F ? nchild[i] : 0u;
+      uint32_t x = v;
+      for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= (uint32_t)d) x += y;
+      }
+      if (lane == 63) wsum[wave] = x;
+      __syncthreads();
+      if (wave == 0) {
+        uint32_t ws = lane < CB / 64 ? wsum[lane] : 0u;
+        for (int d = 1; d < CB / 64; d <<= 1) {
+          uint32_t y = __shfl_up(ws, d, 64);
+          if (lane >= (uint32_t)d) ws += y;
+        }
+        if (lane < CB / 64) wsum[lane] = ws;
+      }
+      __syncthreads();
+      const uint32_t woff = wave ? wsum[wave - 1] : 0u;
+      if (i < F) nchild[i] = carry + woff + x - v;
+      carry += wsum[CB / 64 - 1];
+      __syncthreads();
+    }
+    const uint32_t total = carry;
+    for (uint32_t w = tid; w < F; w += CB) {
+      uint32_t sl[4];
+      for (int k = 0; k < 4; k++) sl[k] = slots[4 * w + k];
+      collapse_emit(t, sl, nchild[w], leaf_max, next_base, nodes + (size_t)(level_base + w) * 8, nxt, cap);
+    }
+    __syncthreads();
+    level_base = next_base;
+    next_base += total;
+    F = total;
+    uint32_t* tmp = cur;
+    cur = nxt;
+    nxt = tmp;
+  }
+  if (tid == 0) result[0] = ok ? next_base : 0u;
 }
 // single-leaf root (n <= leaf_max)
 __global__ void k_single_leaf(const float4* lo_box, const float4* hi_box, uint32_t n, uint4* nodes, float4* root) {
@@ -593,31 +658,13 @@ uint32_t build_bvh4(BuildScratch& sb, uint32_t n, int leaf_max, uint4* nodes_out
     hipLaunchKernelGGL(k_single_leaf, dim3(1), dim3(64), 0, st, sb.nlo, sb.nhi, n, nodes_out, root_lo_hi);
     return 1;
   }
-  // breadth-first collapse, one level per iteration
-  uint32_t* fa = sb.frontier_a;
-  uint32_t* fb = sb.frontier_b;
-  launch_fill_u32(fa, 1, 0u, st);
-  uint32_t F = 1, level_base = 0, next_base = 1;
-  int levels = 0;
-  while (F > 0) {
-    if (++levels > 128 || next_base > n) return 0;  // malformed hierarchy
-    const int gf = grid_for(F);
-    hipLaunchKernelGGL(k_collapse_expand, dim3(gf), dim3(BB), 0, st, t, fa, F, leaf_max, sb.slots, sb.nchild);
-    // total = last exclusive prefix + last count
-    uint32_t last_cnt = 0, last_off = 0;
-    if (hipMemcpyAsync(&last_cnt, sb.nchild + (F - 1), 4, hipMemcpyDeviceToHost, st) != hipSuccess) return 0;
-    exclusive_scan(sb.nchild, F, sb.scan_tmp, st);
-    if (hipMemcpyAsync(&last_off, sb.nchild + (F - 1), 4, hipMemcpyDeviceToHost, st) != hipSuccess) return 0;
-    hipLaunchKernelGGL(k_collapse_emit, dim3(gf), dim3(BB), 0, st, t, fa, F, leaf_max, sb.slots, sb.nchild,
-                       level_base, next_base, nodes_out, fb);
-    if (hipStreamSynchronize(st) != hipSuccess) return 0;
-    uint32_t total = last_off + last_cnt;
-    level_base = next_base;
-    next_base += total;
-    F = total;
-    std::swap(fa, fb);
-  }
-  return next_base;
+  // breadth-first collapse to BVH4 nodes (one workgroup; misc[1] = node count)
+  hipLaunchKernelGGL(k_collapse_all, dim3(1), dim3(CB), 0, st, t, n, leaf_max, (uint32_t)sb.capacity, sb.frontier_a,
+                     sb.frontier_b, sb.slots, sb.nchild, nodes_out, sb.misc + 1);
+  uint32_t count = 0;
+  if (hipMemcpyAsync(&count, sb.misc + 1, 4, hipMemcpyDeviceToHost, st) != hipSuccess) return 0;
+  if (hipStreamSynchronize(st) != hipSuccess) return 0;
+  return count;
 }
 
 // ------------------------------------------------ sorted record pools

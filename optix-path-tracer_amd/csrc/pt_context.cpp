@@ -1148,7 +1148,8 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
     ev.push_back(e);
     return e;
   };
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> ext_ev, sh_ev;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ext_ev, sh_ev, train_ev, infer_ev;
+  uint64_t train_samples = 0, infer_samples = 0;
   launch_fill_u32((uint32_t*)c->stats, (uint64_t)MAX_WAVES * STAT_WORDS * 2, 0u, c->stream);
   hipEvent_t t_begin = want_stats ? mark() : nullptr;
   DevPaths& P = c->paths;
@@ -1235,17 +1236,22 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
       if (rows >= 32) {
         launch_cache_gather(P, 0, rows, golden_stride(rows), grid, (uint4*)c->cfeat, c->ctgt, nullptr, nullptr,
                             c->stream);
+        hipEvent_t ta = want_stats ? mark() : nullptr;
+        // steps x batch rows when there are that many; fewer rows are
+        // cycled through, so every batch takes its cache_steps steps
         for (uint32_t k = 0; k < c->cache_steps; k++) {
-          const uint32_t off = k * c->cache_batch;
-          if (off >= rows || rows - off < 32) break;
+          const uint32_t off = (uint32_t)(((uint64_t)k * c->cache_batch) % rows);
+          if (rows - off < 32) continue;
           const uint32_t n = std::min(c->cache_batch, rows - off);
           c->cache_step++;
           launch_mlp_train(c->cfeat + (size_t)off * MLP_IN, c->ctgt + (size_t)off * 3, n, c->cfw, c->cbw, c->cbias,
                            c->cX, c->cdZ, c->cdb, c->closs, c->cdw, c->cp, c->cm1, c->cm2, c->cgrad, c->cache_lr,
                            c->cache_step, c->cfw, c->cbw, c->cbias, c->stream);
           steps++;
+          train_samples += n;
           c->cache_last_batch = n;
         }
+        if (want_stats) train_ev.push_back({ta, mark()});
       }
       // the judge: training vertices halfway between the ones trained on
       // (the same ones when there are few), finished by the plain tracer,
@@ -1255,8 +1261,11 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
       if (nq && nj >= 32) {
         launch_cache_gather(P, 0, nj, golden_stride(nj), grid, c->cjfeat, nullptr, c->cjvals, nullptr, c->stream,
                             std::max<uint32_t>(rows, 1u));
+        hipEvent_t ia = want_stats ? mark() : nullptr;
         launch_mlp_infer((const uint16_t*)c->cjfeat, nj, c->cfw, c->cbias, c->cjout,
                          (int)std::min<uint32_t>(1024, (nj + 63) / 64), c->stream);
+        if (want_stats) infer_ev.push_back({ia, mark()});
+        infer_samples += nj;
         launch_cache_judge(c->cjvals, c->cjout, nj, c->cjres, c->stream);
         double jr[3] = {0.0, 0.0, 0.0};
         HIPCHK(c, hipMemcpyAsync(jr, c->cjres, sizeof(jr), hipMemcpyDeviceToHost, c->stream));
@@ -1269,8 +1278,11 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
         if (!finite && (ps = cache_reset(c)) != PT_OK) return ps;
       }
       if (nq && use_answers) {
+        hipEvent_t ia = want_stats ? mark() : nullptr;
         launch_mlp_infer((const uint16_t*)P.q_feat, nslots, c->cfw, c->cbias, P.q_out, 2048, c->stream, P.qseg,
                          (uint32_t)grid * BLOCK);
+        if (want_stats) infer_ev.push_back({ia, mark()});
+        infer_samples += nq;
         launch_cache_apply(P, grid, c->stream);
       } else if (nq) {
         // fallback: the batch again with the plain tracer, whose results
@@ -1341,6 +1353,18 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
       (void)hipEventElapsedTime(&m, p.first, p.second);
       out->shade_ms += m;
     }
+    for (auto& p : train_ev) {
+      float m = 0.f;
+      (void)hipEventElapsedTime(&m, p.first, p.second);
+      out->cache_train_ms += m;
+    }
+    for (auto& p : infer_ev) {
+      float m = 0.f;
+      (void)hipEventElapsedTime(&m, p.first, p.second);
+      out->cache_infer_ms += m;
+    }
+    out->cache_train_samples = train_samples;
+    out->cache_infer_samples = infer_samples;
     out->extend_launches = (uint32_t)ext_ev.size();
     out->shade_launches = (uint32_t)sh_ev.size();
     for (auto e : ev) (void)hipEventDestroy(e);
@@ -1567,6 +1591,8 @@ pt_status pt_cache_train(pt_context* c, const uint16_t* features, const float* t
 // slots; each batch's paths share one path pool and one launch per kernel
 // and bounce, each path finding its scene through the index kept in its RNG
 // state.  The results of every context are those of its own pt_render.
+}  // extern "C"
+
 namespace {
 
 // launch shape of a batch of scenes
@@ -1820,6 +1846,8 @@ pt_status render_batch_shaped(pt_context* const* ctxs, const pt_render_params* p
 }
 
 }  // namespace
+
+extern "C" {
 
 pt_status pt_render_batch(pt_context* const* ctxs, const pt_render_params* params, const uint32_t* frame_counts,
                           uint32_t n, pt_render_stats* out) {

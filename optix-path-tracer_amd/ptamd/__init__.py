@@ -142,7 +142,9 @@ class RenderStats(C.Structure):
                 ("cache_train_vertices", C.c_uint64), ("cache_loss", C.c_double), ("cache_steps", C.c_uint32), ("gpu_ms", C.c_double), ("extend_ms", C.c_double),
                 ("shade_ms", C.c_double), ("extend_launches", C.c_uint32),
                 ("shade_launches", C.c_uint32), ("nan_paths", C.c_uint64),
-                ("cache_judge", C.c_double), ("cache_fallbacks", C.c_uint32)]
+                ("cache_judge", C.c_double), ("cache_fallbacks", C.c_uint32),
+                ("cache_train_ms", C.c_double), ("cache_infer_ms", C.c_double),
+                ("cache_train_samples", C.c_uint64), ("cache_infer_samples", C.c_uint64)]
 
 
 class AccelStats(C.Structure):

@@ -49,6 +49,25 @@ def relmse(img: np.ndarray, ref: np.ndarray, eps: float = RELMSE_EPS) -> float:
     return float(np.mean((img - ref) ** 2 / (ref * ref + eps)))
 
 
+def relmse_trimmed(img: np.ndarray, ref: np.ndarray, trim: float = 1e-3, eps: float = RELMSE_EPS) -> float:
+    """relMSE without the pixels of largest error: per pixel the mean over
+    channels of (I - R)^2 / (R^2 + eps), the ceil(trim * n) largest dropped,
+    the rest averaged.  A scene lit by a small, very bright source (a sun
+    lobe of peak 5e4) gives rare samples of huge value (caustics through
+    glass); a handful of them decides the plain relMSE at any sample count,
+    the trimmed one follows the bulk of the image."""
+    img = np.asarray(img, np.float64)
+    ref = np.asarray(ref, np.float64)
+    if img.shape != ref.shape:
+        raise ValueError(f"relmse_trimmed: shapes {img.shape} and {ref.shape} differ")
+    e = ((img - ref) ** 2 / (ref * ref + eps)).reshape(len(img), -1).mean(1)
+    k = int(np.ceil(trim * len(e)))
+    if k >= len(e):
+        raise ValueError("relmse_trimmed: nothing left after trimming")
+    e = np.sort(e)[: len(e) - k]
+    return float(e.mean())
+
+
 def accum_at(accum: np.ndarray, pix: np.ndarray, W: int, H: int) -> np.ndarray:
     """Rows of an accumulation image (H x W x 3, row 0 = top, as
     pt_read_accum returns it) at launch indices pix (x + W*y, y up)."""

@@ -26,7 +26,7 @@ GOLDEN = Path(__file__).resolve().parent / "golden"
 sys.path.insert(0, str(GOLDEN))
 import make_fp64_refs as refs  # noqa: E402
 
-FIXTURES = sorted((GOLDEN / "fp64").glob("*.npz"))
+FIXTURES = sorted(f for f in (GOLDEN / "fp64").glob("*.npz") if f.stem != "c3_batch")
 
 
 # Final_Next_Week's spheres sit at coordinates near 500, where a float ray
@@ -95,6 +95,31 @@ def test_reference_regenerates(path):
     sums, _ = refs.reference_sums(o64, cam, prm, z["pix"][k], spp, threads=3)
     assert np.array_equal((sums / spp).astype(np.float32), z["mean"][k])
     assert np.all(np.isfinite(z["mean"])) and np.all(z["mean"] >= 0)
+
+
+def test_batch_references_regenerate():
+    """The 64-scene batch's references: every scene is present with its
+    1/256 subset; two pixels of a sphere scene and of the largest mesh
+    scene re-render bit for bit."""
+    path = GOLDEN / "fp64" / "c3_batch.npz"
+    if not path.exists():
+        pytest.skip("reference not generated")
+    z = np.load(path)
+    spp = int(z["spp"])
+    assert int(z["scenes"]) == 64 and spp >= 8192 and int(z["frame_base"]) == refs.FRAME_BASE
+    for i in range(64):
+        W, H = (int(v) for v in z[f"size_{i:02d}"])
+        assert np.array_equal(z[f"pix_{i:02d}"], stratified_pixels(W, H, block=16))
+        m = z[f"mean_{i:02d}"]
+        assert np.all(np.isfinite(m)) and np.all(m >= 0)
+    for i in (20, 49):
+        sc = ptamd.Scene(ptamd.PT_SCENE_BATCH, 0, 0, i)
+        assert [sc.info.width, sc.info.height] == [int(v) for v in z[f"size_{i:02d}"]]
+        pix = z[f"pix_{i:02d}"]
+        k = np.array([0, len(pix) - 1])
+        o64 = ptref.OracleScene(sc.desc, fp64=True)
+        sums, _ = refs.reference_sums(o64, sc.camera, sc.render_params(), pix[k], spp, threads=4)
+        assert np.array_equal((sums / spp).astype(np.float32), z[f"mean_{i:02d}"][k])
 
 
 @pytest.mark.parametrize("name", ["c1_cornell", "c0_random_spheres"])
