@@ -40,7 +40,20 @@ constexpr int INST_WORDS = 32;
 // lane in LDS, STK_SPILL more per thread in HBM.
 constexpr int BLOCK = 256;
 constexpr int STK_LDS = 16;
-constexpr int STK_SPILL = 48;
+// every thread owns STK_TOTAL entries in all: the ones beyond its kernel's
+// LDS entries spill to its HBM slice of STK_TOTAL entries
+constexpr int STK_TOTAL = 64;
+constexpr int STK_SPILL = STK_TOTAL - STK_LDS;
+// closest-hit kernel (k_extend): LDS entries per lane and occupancy target
+// (waves per SIMD); 16 entries x 8 B x 256 lanes is 32 KB per block, which
+// with up to 95 VGPRs holds 5 waves per SIMD
+#ifndef PT_EXTEND_STK
+#define PT_EXTEND_STK 16
+#endif
+#ifndef PT_EXTEND_WAVES
+#define PT_EXTEND_WAVES 5
+#endif
+constexpr int STK_LDS_EXTEND = PT_EXTEND_STK;
 // Grid of the path kernels: at most MAX_GRID blocks of BLOCK / 64 waves.
 constexpr int MAX_GRID = 2048;
 constexpr int WAVES = BLOCK / 64;

