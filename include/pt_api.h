@@ -417,6 +417,21 @@ pt_status pt_trace_closest(struct pt_context* ctx, const pt_ray* rays,
 pt_status pt_trace_closest_device(struct pt_context* ctx, const pt_ray* d_rays,
                                   uint64_t n, pt_hit* d_hits);
 
+/* No counterpart in the reference.  For callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): pt_trace_closest_device without any
+ * allocation.  Its traversal-stack spill lives in a fixed per-device
+ * scratch array of the library (48 MiB + 256 B, held by every GPU the
+ * library runs on until the process ends); the launch uses at most 384
+ * blocks.  n <= PT_NOALLOC_MAX_RAYS, else PT_ERR_INVALID_ARG.  Asynchronous
+ * on the context stream; if d_overflow is not null, the number of
+ * traversal-stack overflows of this call (0 for a valid result) is copied
+ * there (device memory) in stream order.  Calls on one GPU share its
+ * scratch: calls that overlap on one GPU (different streams or threads)
+ * race; serialise them. */
+#define PT_NOALLOC_MAX_RAYS (1ull << 31)
+pt_status pt_trace_closest_device_noalloc(struct pt_context* ctx, const pt_ray* d_rays, uint64_t n,
+                                          pt_hit* d_hits, uint32_t* d_overflow_or_null);
 /* Counted variant: per-ray node, primitive and instance fetch counts
  * (3 uint32 per ray), host arrays. */
 pt_status pt_trace_closest_counted(struct pt_context* ctx, const pt_ray* rays,

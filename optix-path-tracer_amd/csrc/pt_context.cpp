@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <random>
 #include <string>
 #include <tuple>
@@ -257,6 +258,34 @@ void launch_gather_tris(const float4* vtx, const uint4* idx, const uint32_t* ord
 void launch_gather_instances(const uint4* src, const uint32_t* order, uint32_t n, uint4* dst, hipStream_t st);
 }  // namespace pt
 
+// Scratch of pt_trace_closest_device_noalloc: the traversal-stack spill
+// of NOALLOC_GRID blocks (NOALLOC_SPILL_BYTES) followed by 256 bytes of
+// counters.  Every GPU the library runs on holds one copy of it until the
+// process ends; host code addresses it only through the pointer
+// hipGetSymbolAddress returned for that device (g_scratch), never through
+// the array's name.
+namespace {
+constexpr uint32_t NOALLOC_GRID = 384;
+constexpr size_t NOALLOC_SPILL_BYTES = (size_t)NOALLOC_GRID * BLOCK * STK_TOTAL * sizeof(uint2);
+constexpr size_t SCRATCH_BYTES = NOALLOC_SPILL_BYTES + 256;
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[MAX_DEVICES] = {};
+}  // namespace
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
+// once per device, with that device current
+static hipError_t resolve_scratch(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[device]) return hipSuccess;
+  void* ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  if (!ptr) return hipErrorInvalidSymbol;
+  g_scratch[device] = ptr;
+  return hipSuccess;
+}
+
 extern "C" {
 
 int32_t pt_api_version(void) { return PT_API_VERSION; }
@@ -276,9 +305,17 @@ pt_status pt_create(int32_t device, pt_context** out) {
     g_create_err = "device index out of range";
     return PT_ERR_INVALID_ARG;
   }
+  if (device >= MAX_DEVICES) {
+    g_create_err = "device index beyond the library's device table";
+    return PT_ERR_INVALID_ARG;
+  }
   e = hipSetDevice(device);
   if (e != hipSuccess) {
     g_create_err = std::string("hipSetDevice: ") + hipGetErrorString(e);
+    return PT_ERR_DEVICE;
+  }
+  if ((e = resolve_scratch(device)) != hipSuccess) {
+    g_create_err = std::string("hipGetSymbolAddress (scratch): ") + hipGetErrorString(e);
     return PT_ERR_DEVICE;
   }
   pt_context* c = new pt_context();
@@ -2143,6 +2180,31 @@ pt_status pt_trace_closest_device(pt_context* c, const pt_ray* d_rays, uint64_t 
   if (pt_status ps = ensure_spill(c); ps != PT_OK) return ps;
   launch_trace_closest(c->dsc, d_rays, n, d_hits, nullptr, c->spill, c->qcounters + CTR_STACK_OVERFLOW, grid_for(n),
                        c->stream);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(c, e, "trace launch");
+  return PT_OK;
+}
+
+pt_status pt_trace_closest_device_noalloc(pt_context* c, const pt_ray* d_rays, uint64_t n, pt_hit* d_hits,
+                                          uint32_t* d_overflow) {
+  if (!c || (n && (!d_rays || !d_hits))) return PT_ERR_INVALID_ARG;
+  if (!c->accel_ready) return fail(c, PT_ERR_STATE, "acceleration structure not built");
+  if (n > PT_NOALLOC_MAX_RAYS)
+    return fail(c, PT_ERR_INVALID_ARG, "pt_trace_closest_device_noalloc: more than PT_NOALLOC_MAX_RAYS (2^31) rays");
+  if (n == 0) return PT_OK;
+  (void)hipSetDevice(c->device);
+  unsigned char* base = static_cast<unsigned char*>(g_scratch[c->device]);
+  if (!base) return fail(c, PT_ERR_STATE, "scratch of this device was not resolved");
+  uint2* spill = reinterpret_cast<uint2*>(base);
+  uint32_t* overflow = reinterpret_cast<uint32_t*>(base + NOALLOC_SPILL_BYTES);
+  // the overflow counter starts at zero (a fill kernel, not a memset node)
+  launch_fill_u32(overflow, 1, 0u, c->stream);
+  launch_trace_closest(c->dsc, d_rays, n, d_hits, nullptr, spill, overflow,
+                       std::min<int>(grid_for(n), (int)NOALLOC_GRID), c->stream);
+  if (d_overflow) {
+    hipError_t e = hipMemcpyAsync(d_overflow, overflow, 4, hipMemcpyDeviceToDevice, c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "overflow copy");
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(c, e, "trace launch");
   return PT_OK;

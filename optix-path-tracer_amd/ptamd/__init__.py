@@ -193,6 +193,7 @@ def _declare(lib):
         "pt_read_paths": (C.c_int32, [P, P, P, P]),
         "pt_trace_closest": (C.c_int32, [P, P, C.c_uint64, P]),
         "pt_trace_closest_device": (C.c_int32, [P, P, C.c_uint64, P]),
+        "pt_trace_closest_device_noalloc": (C.c_int32, [P, P, C.c_uint64, P, P]),
         "pt_trace_closest_counted": (C.c_int32, [P, P, C.c_uint64, P, P]),
         "pt_trace_occluded": (C.c_int32, [P, P, C.c_uint64, P]),
         "pt_read_accel": (C.c_int32, [P, P, C.POINTER(C.c_uint64), P, P, C.POINTER(C.c_uint64), P,
