@@ -266,7 +266,10 @@ typedef struct pt_render_params {
   uint32_t max_depth;       /* maxDepth (main.cpp:63)                      */
   int32_t russian;          /* russian (main.cpp:62)                       */
   uint32_t rr_min_depth;    /* RR applies when depth >= this; reference 11 */
-  uint32_t frames_per_batch;/* frames traced together (0 = automatic)      */
+  uint32_t frames_per_batch;/* frames traced together; 0 = automatic: as
+                               many as 2^28 paths or a quarter of the
+                               device's memory (252 B of path state each)
+                               holds, whichever is less                    */
   uint32_t flags;           /* PT_FLAG_*                                   */
   uint32_t reserved;
 } pt_render_params;
@@ -365,8 +368,9 @@ pt_status pt_render(struct pt_context* ctx, const pt_render_params* params,
 /* No counterpart in the reference (it renders one scene per OptiX context;
  * this replaces its per-scene launch loop, main.cpp:26-39,264-265):
  * frames [0, frame_counts[i]) of every ctxs[i], i < n <= 256, in one
- * wavefront.  The (scene, frame) work items are cut into sub-batches of
- * about 16 Mi paths; within a sub-batch all paths share one path pool and
+ * wavefront.  The (scene, frame) work items are cut into sub-batches of at
+ * most the automatic batch size of ctxs[0] (see frames_per_batch); within a
+ * sub-batch all paths share one path pool and
  * one launch per kernel and bounce, so many small or uneven scenes fill the
  * GPU together.  Every context needs a built scene and a camera, all on one
  * device; ctxs[0] runs the batch.  Each context's accumulation restarts at
@@ -462,7 +466,8 @@ pt_status pt_read_env_tables(struct pt_context* ctx, uint64_t* cond, uint64_t* m
 
 /* ------------------------------------------------------------ autotuning
  * No counterpart in the reference.  pt_autotune renders `frames` frames of
- * the current scene under candidate launch shapes (paths in flight 1..16 Mi,
+ * the current scene under candidate launch shapes (paths in flight 1, 4,
+ * 16, 64 and 256 Mi, at most the automatic batch size,
  * material sort on/off, batch graphs on/off), keeps the fastest, and
  * pt_render uses it for renders with PT_FLAG_TUNED and the same resolution,
  * depth and flags.  Results are identical under every shape. */
@@ -479,8 +484,9 @@ pt_status pt_autotune(struct pt_context* ctx, const pt_render_params* params, ui
 
 /* No counterpart in the reference.  The same for a batch of scenes: renders
  * the batch (pt_render_batch semantics, with the given frame counts, which
- * may be fewer than the production render's) under 4, 8, 16 and 32 Mi
- * paths per sub-batch x material sort on/off (when the batch has more than
+ * may be fewer than the production render's) under 8, 16, 32, 64, 128 and
+ * 256 Mi paths per sub-batch (at most the automatic batch size) x material
+ * sort on/off (when the batch has more than
  * one shading program) x graph replay on/off, twice each, and keeps the
  * fastest second run in ctxs[0] for pt_render_batch calls whose params[0]
  * has PT_FLAG_TUNED and whose contexts and parameters (other than the

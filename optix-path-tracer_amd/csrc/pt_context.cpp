@@ -29,7 +29,16 @@ using namespace pt;
 
 namespace {
 
-constexpr uint32_t PATHS_TARGET = 1u << 24;  // paths in flight per batch (automatic)
+// Paths in flight per batch when the caller leaves it to the library: as
+// many as a quarter of the device's memory holds (PATH_BYTES of path state
+// each), at most 2^28.  Every batch ends in a dozen bounces whose launches
+// last as long as their slowest waves (a few paths trapped in glass or
+// between meshes), whatever the number of paths, so the fewer batches a
+// render takes, the less of that tail it pays: the 1080p 256-spp mesh scene
+// takes 487 ms in batches of 8 frames (16.6 M paths), 233 ms in batches of
+// 128 frames (265 M paths, 67 GB).
+constexpr uint64_t PATHS_MAX = 1ull << 28;
+constexpr uint64_t PATH_BYTES = 3 * 64 + 16 + 16 + 4 + 16 + 4 + 4;  // per path, ensure_paths
 
 struct DevBuf {
   void* p = nullptr;
@@ -176,6 +185,7 @@ struct pt_context {
   } tuned;
 
   // ---- path queues
+  uint64_t paths_target = 1u << 24;  // automatic paths in flight per batch (set at pt_create)
   uint64_t path_capacity = 0;
   std::vector<DevBuf> path_bufs;
   DevPaths paths{};
@@ -320,6 +330,11 @@ pt_status pt_create(int32_t device, pt_context** out) {
   }
   pt_context* c = new pt_context();
   c->device = device;
+  {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total_b > 0)
+      c->paths_target = std::max<uint64_t>(1u << 20, std::min<uint64_t>(PATHS_MAX, total_b / 4 / PATH_BYTES));
+  }
   e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     g_create_err = std::string("hipStreamCreate: ") + hipGetErrorString(e);
@@ -1149,7 +1164,8 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
   const bool tuned = (rp->flags & PT_FLAG_TUNED) && tu.valid && tu.W == rp->width && tu.H == rp->height &&
                      tu.max_depth == rp->max_depth && (tu.flags & ~tune_bits) == (rp->flags & ~tune_bits);
   uint32_t fpb = tuned ? tu.frames_per_batch
-                       : (rp->frames_per_batch ? rp->frames_per_batch : std::max<uint32_t>(1, PATHS_TARGET / WH));
+                       : (rp->frames_per_batch ? rp->frames_per_batch
+                                               : (uint32_t)std::max<uint64_t>(1, c->paths_target / WH));
   fpb = std::min(fpb, frame_count);
   if ((uint64_t)fpb * WH > SLOT_MASK) return fail(c, PT_ERR_INVALID_ARG, "frames_per_batch x W x H exceeds 2^30 - 1");
   const bool use_cache = (rp->flags & PT_FLAG_RADIANCE_CACHE) != 0;
@@ -1624,7 +1640,8 @@ pt_status pt_cache_train(pt_context* c, const uint16_t* features, const float* t
 }
 
 // A batch of scenes in one wavefront: the frames of every context form one
-// work list of (scene, frame) items, cut into batches of about PATHS_TARGET
+// work list of (scene, frame) items, cut into batches of about the
+// context's paths_target
 // slots; each batch's paths share one path pool and one launch per kernel
 // and bounce, each path finding its scene through the index kept in its RNG
 // state.  The results of every context are those of its own pt_render.
@@ -1634,7 +1651,7 @@ namespace {
 
 // launch shape of a batch of scenes
 struct BatchShape {
-  uint64_t paths = PATHS_TARGET;  // slots per sub-batch (a scene's frame is never split)
+  uint64_t paths = PATHS_MAX;     // slots per sub-batch (a scene's frame is never split)
   bool sort = true;               // material sort (when the batch has more than one shading program)
   bool graph = false;             // the whole batch replayed from one captured graph
 };
@@ -1895,6 +1912,7 @@ pt_status pt_render_batch(pt_context* const* ctxs, const pt_render_params* param
   // params[0] asks for it, else ~16 Mi paths per sub-batch and
   // params[0]'s material-sort and graph flags
   BatchShape shape;
+  shape.paths = B->paths_target;
   const uint32_t f0 = params[0].flags;
   shape.sort = (f0 & PT_FLAG_MATERIAL_SORT) != 0;
   shape.graph = (f0 & PT_FLAG_GRAPH) != 0;
@@ -1909,9 +1927,10 @@ pt_status pt_render_batch(pt_context* const* ctxs, const pt_render_params* param
 // Launch-shape autotuner of a batch of scenes: renders the batch (with the
 // given frame counts, which may be fewer than the production render's)
 // under every candidate shape, twice each, and keeps the fastest second
-// run.  Candidates: 4, 8, 16 and 32 Mi paths per sub-batch, material sort
-// on and off (when the batch has more than one shading program), graph
-// replay on and off.  Results are identical under every shape.
+// run.  Candidates: 8, 16, 32, 64, 128 and 256 Mi paths per sub-batch (at
+// most the context's paths_target), material sort on and off (when the
+// batch has more than one shading program), graph replay on and off.
+// Results are identical under every shape.
 pt_status pt_autotune_batch(pt_context* const* ctxs, const pt_render_params* params, const uint32_t* frame_counts,
                             uint32_t n, pt_tune_result* out) {
   if (!ctxs || !params || !frame_counts || n == 0 || !ctxs[0]) return PT_ERR_INVALID_ARG;
@@ -1930,7 +1949,12 @@ pt_status pt_autotune_batch(pt_context* const* ctxs, const pt_render_params* par
   BatchShape best;
   double best_ms = 1e300;
   uint32_t cands = 0;
-  for (uint64_t paths : {1ull << 22, 1ull << 23, 1ull << 24, 1ull << 25})
+  std::vector<uint64_t> sizes;
+  for (uint64_t paths : {1ull << 23, 1ull << 24, 1ull << 25, 1ull << 26, 1ull << 27, 1ull << 28}) {
+    paths = std::min(paths, B->paths_target);
+    if (std::find(sizes.begin(), sizes.end(), paths) == sizes.end()) sizes.push_back(paths);
+  }
+  for (uint64_t paths : sizes)
     for (int so = 0; so < (can_sort ? 2 : 1); so++)
       for (int gr = 0; gr < 2; gr++) {
         BatchShape s;
@@ -1973,17 +1997,18 @@ pt_status pt_autotune_batch(pt_context* const* ctxs, const pt_render_params* par
 
 // Launch-shape autotuner: renders `frames` frames (frames 0.. of the current
 // scene, camera and parameters) under every candidate shape, twice each,
-// and keeps the fastest second run.  Candidates: paths in flight 1, 2, 4, 8
-// and 16 Mi (frames per batch = paths / (W H), at least 1), material sort on
-// and off (when the scene has more than one shading program), batch graphs
-// on and off.
+// and keeps the fastest second run.  Candidates: paths in flight 1, 4, 16,
+// 64 and 256 Mi, at most the context's paths_target (frames per batch =
+// paths / (W H), at least 1), material sort on and off (when the scene has
+// more than one shading program), batch graphs on and off.
 pt_status pt_autotune(pt_context* c, const pt_render_params* rp, uint32_t frames, pt_tune_result* out) {
   if (!c || !rp || frames == 0) return PT_ERR_INVALID_ARG;
   if (rp->flags & PT_FLAG_RADIANCE_CACHE) return fail(c, PT_ERR_UNSUPPORTED, "autotune the plain path only");
   const uint64_t WH = (uint64_t)rp->width * rp->height;
   if (WH == 0) return fail(c, PT_ERR_INVALID_ARG, "empty image");
   std::vector<uint32_t> fpbs;
-  for (uint64_t paths : {1ull << 20, 1ull << 21, 1ull << 22, 1ull << 23, 1ull << 24}) {
+  for (uint64_t paths : {1ull << 20, 1ull << 22, 1ull << 24, 1ull << 26, 1ull << 28}) {
+    paths = std::min(paths, c->paths_target);
     uint32_t f = (uint32_t)std::max<uint64_t>(1, paths / WH);
     f = std::min(f, frames);
     if (f * WH > SLOT_MASK) continue;
