@@ -1217,7 +1217,7 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
                     nullptr, depth == 0 ? &rc : nullptr, depth == 0 ? nslots : 0u);
       hipEvent_t b = events ? mark() : nullptr;
       if (events) ext_ev.push_back({a, b});
-      if (sort) launch_sort(P, in, grid, c->stream);
+      if (sort) launch_sort(P, in, grid, c->stream, depth == 0);
       hipEvent_t a2 = events ? mark() : nullptr;
       launch_shade(c->dsc, P, in, sort, out, depth, rc, matset, count_trav, c->spill, c->stats, grid, c->stream);
       hipEvent_t b2 = events ? mark() : nullptr;
@@ -1808,7 +1808,7 @@ pt_status render_batch_shaped(pt_context* const* ctxs, const pt_render_params* p
       for (uint32_t depth = 0; depth < max_depth; depth++) {
         launch_extend(B->dsc, P, in, depth == 0 ? 1e-6f : 1e-3f, B->spill, B->stats, count_trav, sort, grid,
                       B->stream, B->d_scenes, nullptr, 0u, ext);
-        if (sort) launch_sort(P, in, grid, B->stream);
+        if (sort) launch_sort(P, in, grid, B->stream, depth == 0);
         launch_shade(B->dsc, P, in, sort, o, depth, rc0, matset, count_trav, B->spill, B->stats, grid, B->stream,
                      B->d_scenes, B->d_rcs, env, ext);
         std::swap(in, o);

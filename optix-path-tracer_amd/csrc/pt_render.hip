@@ -446,6 +446,10 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_sort_scan(DevPaths P, uint32_t
 }
 // Move path state and hit into material-bucket order (state set 2, a dense
 // set): each wave writes its bucket-b paths from its scanned base on.
+// At depth 0 (`first`) the paths' throughput and radiance are the
+// path-start constants, which the shading pass takes without reading them:
+// only origin, direction and hit move (80 of 144 bytes per path).
+template <bool FIRST>
 __global__ __launch_bounds__(BLOCK) void k_sort_scatter(DevPaths P, int in) {
   const Seg g = my_seg();
   const uint32_t n = seg_count(P, in, g);
@@ -466,8 +470,10 @@ __global__ __launch_bounds__(BLOCK) void k_sort_scatter(DevPaths P, int in) {
     if (key < 8) {
       D.o[dst] = S.o[i];
       D.d[dst] = S.d[i];
-      D.T[dst] = S.T[i];
-      D.L[dst] = S.L[i];
+      if (!FIRST) {
+        D.T[dst] = S.T[i];
+        D.L[dst] = S.L[i];
+      }
       P.hit_sorted[dst] = P.hit[i];
     }
   }
@@ -702,7 +708,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
                 s_keep[3 * BLOCK + kt] = L.x;
                 s_keep[4 * BLOCK + kt] = L.y;
                 s_keep[5 * BLOCK + kt] = L.z;
-                HitOut sh = trace_closest<COUNT, EXT, KINDS>(sc, r.P, Wi, 1e-3f, FINF, st, tc, seed, tm);
+                // an environment sample is lit iff its shadow ray hits
+                // nothing (any hit occludes): the first hit found decides it
+                HitOut sh = trace_closest<COUNT, EXT, KINDS>(sc, r.P, Wi, 1e-3f, FINF, st, tc, seed, tm, env);
                 T = mk3(s_keep[kt], s_keep[BLOCK + kt], s_keep[2 * BLOCK + kt]);
                 L = mk3(s_keep[3 * BLOCK + kt], s_keep[4 * BLOCK + kt], s_keep[5 * BLOCK + kt]);
                 const v3 emission = env ? env_radiance(sc, Wi) : light_emission(Lt);
@@ -1132,9 +1140,12 @@ void launch_extend(const DevScene& sc, const DevPaths& P, int in, float tmin, ui
 #undef PT_EXTEND4
 #undef PT_EXTEND
 }
-void launch_sort(const DevPaths& P, int in, int grid, hipStream_t st) {
+void launch_sort(const DevPaths& P, int in, int grid, hipStream_t st, bool first) {
   hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(SCAN_THREADS), 0, st, P, (uint32_t)(grid * WAVES));
-  hipLaunchKernelGGL(k_sort_scatter, dim3(grid), dim3(BLOCK), 0, st, P, in);
+  if (first)
+    hipLaunchKernelGGL(k_sort_scatter<true>, dim3(grid), dim3(BLOCK), 0, st, P, in);
+  else
+    hipLaunchKernelGGL(k_sort_scatter<false>, dim3(grid), dim3(BLOCK), 0, st, P, in);
 }
 template <int MS, int FEAT, int KINDS = KIND_ALL>
 static void launch_shade_v(const DevScene& sc, const DevPaths& P, int in, int src, const float4* hits, int out,
