@@ -502,14 +502,21 @@ PTD void collapse_emit(const BinTree& t, const uint32_t s[4], uint32_t o, int le
   nd[7] = make_uint4(cn[0], cn[1], cn[2], cn[3]);
 }
 
-// Breadth-first collapse of the binary hierarchy into BVH4 nodes, every
-// level in one workgroup: expand each frontier node into up to four slots,
-// exclusive-scan the numbers of internal children (which numbers the next
-// level in frontier then slot order), emit the nodes.  One launch per
-// structure and no host round trip per level.  result[0] = node count, or
-// 0 for a malformed hierarchy (more than 128 levels or more than n nodes).
+// Breadth-first collapse of the binary hierarchy into BVH4 nodes: expand
+// each frontier node into up to four slots, exclusive-scan the numbers of
+// internal children (which numbers the next level in frontier then slot
+// order), emit the nodes.  The narrow top levels run in one workgroup
+// (k_collapse_top: no launch or host round trip per level); once a level
+// holds more than COLLAPSE_WIDE nodes the workgroup stops and the wide
+// bottom levels (a few: each is about 3.5 times the last) run as multi-block
+// expand / scan / emit launches.  The node order is the same either way.
+// k_collapse_top's state: result[0] = 0 malformed (more than 128 levels or
+// more than n nodes), 1 done, 2 continue; [1] frontier size, [2] level
+// base, [3] next level base, [4] levels so far, [5] 1 if the frontier is in
+// fb.
 constexpr int CB = 1024;
-__global__ void __launch_bounds__(CB) k_collapse_all(BinTree t, uint32_t n, int leaf_max, uint32_t cap,
+constexpr uint32_t COLLAPSE_WIDE = 4096;
+__global__ void __launch_bounds__(CB) k_collapse_top(BinTree t, uint32_t n, int leaf_max, uint32_t cap,
                                                      uint32_t* fa, uint32_t* fb, uint32_t* slots, uint32_t* nchild,
                                                      uint4* nodes, uint32_t* result) {
   __shared__ uint32_t wsum[CB / 64];
@@ -520,7 +527,7 @@ __global__ void __launch_bounds__(CB) k_collapse_all(BinTree t, uint32_t n, int 
   uint32_t *cur = fa, *nxt = fb;
   bool ok = true;
   int levels = 0;
-  while (F > 0) {
+  while (F > 0 && F <= COLLAPSE_WIDE) {
     if (++levels > 128 || next_base > n || F > cap) {
       ok = false;
       break;
@@ -571,7 +578,32 @@ __global__ void __launch_bounds__(CB) k_collapse_all(BinTree t, uint32_t n, int 
     cur = nxt;
     nxt = tmp;
   }
-  if (tid == 0) result[0] = ok ? next_base : 0u;
+  if (tid == 0) {
+    result[0] = !ok ? 0u : (F == 0 ? 1u : 2u);
+    result[1] = F;
+    result[2] = level_base;
+    result[3] = next_base;
+    result[4] = (uint32_t)levels;
+    result[5] = cur == fb ? 1u : 0u;
+  }
+}
+// one wide level, multi-block
+__global__ void k_collapse_expand(BinTree t, const uint32_t* frontier, uint32_t F, int leaf_max, uint32_t* slots,
+                                  uint32_t* nchild) {
+  for (uint32_t w = blockIdx.x * BB + threadIdx.x; w < F; w += gridDim.x * BB) {
+    uint32_t sl[4];
+    nchild[w] = collapse_expand(t, frontier[w], leaf_max, sl);
+    for (int k = 0; k < 4; k++) slots[4 * w + k] = sl[k];
+  }
+}
+__global__ void k_collapse_emit(BinTree t, uint32_t F, int leaf_max, const uint32_t* slots, const uint32_t* offs,
+                                uint32_t level_base, uint32_t next_base, uint4* nodes, uint32_t* next_frontier,
+                                uint32_t cap) {
+  for (uint32_t w = blockIdx.x * BB + threadIdx.x; w < F; w += gridDim.x * BB) {
+    uint32_t sl[4];
+    for (int k = 0; k < 4; k++) sl[k] = slots[4 * w + k];
+    collapse_emit(t, sl, offs[w], leaf_max, next_base, nodes + (size_t)(level_base + w) * 8, next_frontier, cap);
+  }
 }
 // single-leaf root (n <= leaf_max)
 __global__ void k_single_leaf(const float4* lo_box, const float4* hi_box, uint32_t n, uint4* nodes, float4* root) {
@@ -658,13 +690,38 @@ uint32_t build_bvh4(BuildScratch& sb, uint32_t n, int leaf_max, uint4* nodes_out
     hipLaunchKernelGGL(k_single_leaf, dim3(1), dim3(64), 0, st, sb.nlo, sb.nhi, n, nodes_out, root_lo_hi);
     return 1;
   }
-  // breadth-first collapse to BVH4 nodes (one workgroup; misc[1] = node count)
-  hipLaunchKernelGGL(k_collapse_all, dim3(1), dim3(CB), 0, st, t, n, leaf_max, (uint32_t)sb.capacity, sb.frontier_a,
+  // breadth-first collapse to BVH4 nodes: the narrow levels in one
+  // workgroup (state in misc[1..6]), then the wide ones level by level
+  hipLaunchKernelGGL(k_collapse_top, dim3(1), dim3(CB), 0, st, t, n, leaf_max, (uint32_t)sb.capacity, sb.frontier_a,
                      sb.frontier_b, sb.slots, sb.nchild, nodes_out, sb.misc + 1);
-  uint32_t count = 0;
-  if (hipMemcpyAsync(&count, sb.misc + 1, 4, hipMemcpyDeviceToHost, st) != hipSuccess) return 0;
+  uint32_t state[6] = {0, 0, 0, 0, 0, 0};
+  if (hipMemcpyAsync(state, sb.misc + 1, sizeof(state), hipMemcpyDeviceToHost, st) != hipSuccess) return 0;
   if (hipStreamSynchronize(st) != hipSuccess) return 0;
-  return count;
+  if (state[0] == 0) return 0;  // malformed hierarchy
+  uint32_t F = state[1], level_base = state[2], next_base = state[3];
+  int levels = (int)state[4];
+  uint32_t* fa = state[5] ? sb.frontier_b : sb.frontier_a;
+  uint32_t* fb = state[5] ? sb.frontier_a : sb.frontier_b;
+  const uint32_t cap = (uint32_t)sb.capacity;
+  while (F > 0) {
+    if (++levels > 128 || next_base > n || F > cap) return 0;  // malformed hierarchy
+    const int gf = grid_for(F);
+    hipLaunchKernelGGL(k_collapse_expand, dim3(gf), dim3(BB), 0, st, t, fa, F, leaf_max, sb.slots, sb.nchild);
+    // total = last exclusive prefix + last count
+    uint32_t last_cnt = 0, last_off = 0;
+    if (hipMemcpyAsync(&last_cnt, sb.nchild + (F - 1), 4, hipMemcpyDeviceToHost, st) != hipSuccess) return 0;
+    exclusive_scan(sb.nchild, F, sb.scan_tmp, st);
+    if (hipMemcpyAsync(&last_off, sb.nchild + (F - 1), 4, hipMemcpyDeviceToHost, st) != hipSuccess) return 0;
+    hipLaunchKernelGGL(k_collapse_emit, dim3(gf), dim3(BB), 0, st, t, F, leaf_max, sb.slots, sb.nchild, level_base,
+                       next_base, nodes_out, fb, cap);
+    if (hipStreamSynchronize(st) != hipSuccess) return 0;
+    const uint32_t total = last_off + last_cnt;
+    level_base = next_base;
+    next_base += total;
+    F = total;
+    std::swap(fa, fb);
+  }
+  return next_base;
 }
 
 // ------------------------------------------------ sorted record pools

@@ -691,6 +691,7 @@ pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
   s.miss = c->miss;
   // environment importance-sampling tables, built on the GPU
   s.env_cond = s.env_marg = nullptr;
+  s.env_guide = nullptr;
   s.env_total = 0;
   s.env_w = s.env_h = 0;
   if (lts.size() != lts_plain.size()) {
@@ -701,7 +702,12 @@ pt_status pt_set_scene(pt_context* c, const pt_scene_desc* d) {
         (e = dalloc(L, (void**)&marg, (size_t)t.h * 8)) != hipSuccess ||
         (e = dalloc(L, (void**)&scratch, (size_t)(t.h + 1) * 8)) != hipSuccess)
       return hip_fail(c, e, "hipMalloc (environment tables)");
+    uint32_t* guide = nullptr;
+    if ((e = dalloc(L, (void**)&guide, (size_t)(t.h + 1) * (ENV_GUIDE + 1) * 4)) != hipSuccess)
+      return hip_fail(c, e, "hipMalloc (environment guide tables)");
     launch_env_tables(s.texels + t.offset, t.w, t.h, cond, marg, scratch, c->stream);
+    launch_env_guides(cond, marg, t.w, t.h, guide, c->stream);
+    s.env_guide = guide;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     unsigned long long total = 0;
     HIPCHK(c, hipMemcpy(&total, marg + (t.h - 1), 8, hipMemcpyDeviceToHost));

@@ -115,6 +115,44 @@ __global__ __launch_bounds__(EB) void k_env_marg(const unsigned long long* row_s
   }
 }
 
+// Guide tables (Chen & Asau 1974) of the sampler's two searches: for an
+// array a of n inclusive prefix sums with total a[n-1], entry b < G is the
+// first index whose prefix exceeds floor(b * total / G), entry G is n - 1.
+// A target x in [floor(b total / G), floor((b+1) total / G)) has its first
+// prefix above it between entries b and b + 1, so the sampler searches
+// only there and finds the same index as a search of the whole array.
+// floor(b * total / 64) = b (total >> 6) + (b (total & 63)) >> 6, exact in
+// 64-bit integers.
+PTD unsigned long long guide_bound(unsigned long long total, uint32_t b) {
+  return (total >> 6) * b + (((total & 63ull) * b) >> 6);
+}
+PTD uint32_t first_above(const unsigned long long* a, uint32_t lo, uint32_t hi, unsigned long long x) {
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] > x) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+static_assert(ENV_GUIDE == 64, "guide_bound divides by 64");
+__global__ void k_env_guides(const unsigned long long* cond, const unsigned long long* marg, uint32_t W, uint32_t H,
+                             uint32_t* guide) {
+  const uint64_t total_entries = (uint64_t)(H + 1) * (ENV_GUIDE + 1);
+  for (uint64_t e = (uint64_t)blockIdx.x * EB + threadIdx.x; e < total_entries; e += (uint64_t)gridDim.x * EB) {
+    const uint32_t r = (uint32_t)(e / (ENV_GUIDE + 1)), b = (uint32_t)(e % (ENV_GUIDE + 1));
+    const unsigned long long* a = r < H ? cond + (size_t)r * W : marg;
+    const uint32_t n = r < H ? W : H;
+    const unsigned long long total = r < H ? a[n - 1] : marg[H - 1];
+    guide[e] = b == ENV_GUIDE ? n - 1 : first_above(a, 0, n - 1, guide_bound(total, b));
+  }
+}
+void launch_env_guides(const unsigned long long* cond, const unsigned long long* marg, uint32_t W, uint32_t H,
+                       uint32_t* guide, hipStream_t st) {
+  const uint64_t n = (uint64_t)(H + 1) * (ENV_GUIDE + 1);
+  const int grid = (int)std::min<uint64_t>(2048, (n + EB - 1) / EB);
+  hipLaunchKernelGGL(k_env_guides, dim3(grid), dim3(EB), 0, st, cond, marg, W, H, guide);
+}
+
 void launch_env_tables(const float4* tex, uint32_t W, uint32_t H, unsigned long long* cond,
                        unsigned long long* marg, unsigned long long* scratch, hipStream_t st) {
   // scratch: [0] max weight bits, [1 .. H] row totals

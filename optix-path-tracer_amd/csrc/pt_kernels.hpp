@@ -62,6 +62,8 @@ int shade_material_set(uint32_t types_mask);
 // holds 1 + H 64-bit words
 void launch_env_tables(const float4* tex, uint32_t W, uint32_t H, unsigned long long* cond,
                        unsigned long long* marg, unsigned long long* scratch, hipStream_t st);
+void launch_env_guides(const unsigned long long* cond, const unsigned long long* marg, uint32_t W, uint32_t H,
+                       uint32_t* guide, hipStream_t st);
 // radiance cache in the render loop (pt_render.hip); `grid` is the batch's
 // grid (its waves own the record segments)
 void launch_cache_scan(const DevPaths& P, int grid, hipStream_t st);

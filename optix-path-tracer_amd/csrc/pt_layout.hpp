@@ -47,6 +47,8 @@ constexpr int STK_SPILL = STK_TOTAL - STK_LDS;
 // closest-hit kernel (k_extend): LDS entries per lane and occupancy target
 // (waves per SIMD); 16 entries x 8 B x 256 lanes is 32 KB per block, which
 // with up to 95 VGPRs holds 5 waves per SIMD
+// buckets per guide table of the environment sampler's searches
+constexpr uint32_t ENV_GUIDE = 64;
 #ifndef PT_EXTEND_STK
 #define PT_EXTEND_STK 16
 #endif
@@ -149,6 +151,9 @@ struct DevScene {
   const unsigned long long* env_marg;
   unsigned long long env_total;
   uint32_t env_w, env_h;
+  // guide tables of the two searches: (env_h + 1) rows of ENV_GUIDE + 1
+  // indices, row j < env_h for env_cond's row j, row env_h for env_marg
+  const uint32_t* env_guide;
 };
 
 struct DevCamera {

@@ -300,17 +300,36 @@ PTD float env_pdf(const DevScene& sc, v3 d) {
 }
 // two draws: row from the marginal table, column from the row's table,
 // continuous offsets inside the texel
+// first index of a (n inclusive prefix sums, total > x) whose prefix
+// exceeds x, searched between the guide entries around x's bucket (the
+// bucket is estimated in float; an estimate off by one still brackets the
+// answer, and the exact integer comparisons decide it): the same index as
+// upper_index over the whole array.
+PTD uint32_t guided_index(const unsigned long long* a, uint32_t n, unsigned long long total, unsigned long long x,
+                          const uint32_t* guide) {
+  const float est = (float)x * ((float)ENV_GUIDE / (float)total);
+  uint32_t b = est >= (float)(ENV_GUIDE - 1) ? ENV_GUIDE - 1 : (uint32_t)est;
+  uint32_t lo = guide[b > 0 ? b - 1 : 0], hi = guide[b + 2 < ENV_GUIDE ? b + 2 : ENV_GUIDE];
+  if (hi > n - 1) hi = n - 1;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] > x) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
 PTD v3 env_sample(const DevScene& sc, uint32_t& seed) {
   uint32_t r1 = lcg24(seed);
   uint32_t r2 = lcg24(seed);
   if (sc.env_total == 0) return mk3(0.f, 1.f, 0.f);
   unsigned long long tr = scale24(sc.env_total, r1);
-  uint32_t j = upper_index(sc.env_marg, sc.env_h, tr);
+  const uint32_t* gm = sc.env_guide + (size_t)sc.env_h * (ENV_GUIDE + 1);
+  uint32_t j = guided_index(sc.env_marg, sc.env_h, sc.env_total, tr, gm);
   unsigned long long rprev = j ? sc.env_marg[j - 1] : 0ull;
   unsigned long long rw = sc.env_marg[j] - rprev;
   const unsigned long long* row = sc.env_cond + (size_t)j * sc.env_w;
   unsigned long long tc = scale24(rw, r2);
-  uint32_t i = upper_index(row, sc.env_w, tc);
+  uint32_t i = guided_index(row, sc.env_w, rw, tc, sc.env_guide + (size_t)j * (ENV_GUIDE + 1));
   unsigned long long cprev = i ? row[i - 1] : 0ull;
   // offsets inside the texel, below 1 (a ratio of two roundings may reach it)
   const float fu = fminf((float)(tc - cprev) / (float)(row[i] - cprev), 0x1.fffffep-1f);
