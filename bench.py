@@ -9,7 +9,8 @@ MIS (512x512, 64 spp, depth 16, Russian roulette from depth 3) and the
 random-spheres scene (400x267, 16 spp, depth 50).  One step renders every
 sample of the image (pt_render, frames 0..spp-1) with the scene,
 acceleration structure and path queues resident in HBM before the timed
-region.
+region, in the launch shape pt_autotune picked for the view beforehand
+(paths per batch, material sort, graph replay; reported as `tuned_shape`).
 
   value          = (closest-hit + shadow queries) / wall time of the timed steps
   relmse_vs_fp64 = relMSE of the step's image against a converged FP64
@@ -189,20 +190,36 @@ def plain_config(name, args, with_cpu=True):
     log(f"[bench] {name}: {acc.num_primitives} primitives, {acc.num_nodes} nodes, {acc.num_instances} instances, "
         f"build {acc.build_ms:.2f} ms")
 
+    # launch shape: pt_autotune's pick for this view over the whole step
+    # (paths per batch, material sort, graph replay; untimed)
+    t_tune = time.perf_counter()
+    tune = ctx.autotune(params, spp)
+    t_tune = time.perf_counter() - t_tune
+    shape = {"frames_per_batch": tune.frames_per_batch, "paths_per_batch": tune.paths_per_batch,
+             "material_sort": bool(tune.material_sort), "graph": bool(tune.graph), "candidates": tune.candidates,
+             "tuning_s": round(t_tune, 1)}
+    log(f"[bench] {name}: tuned shape {shape}")
+    tuned = scene.render_params(flags=info.flags | ptamd.PT_FLAG_TUNED)
+    # direct launches of the same shape, for per-kernel HIP-event timings
+    sort_bit = ptamd.PT_FLAG_MATERIAL_SORT if tune.material_sort else 0
+    direct = scene.render_params(flags=(info.flags & ~ptamd.PT_FLAG_MATERIAL_SORT) | sort_bit,
+                                 frames_per_batch=tune.frames_per_batch)
     # traversal-counting pass (identical rays: the render is deterministic)
-    cst = ctx.render(scene.render_params(flags=info.flags | ptamd.PT_FLAG_COUNT_TRAVERSAL), 0, spp)
+    cst = ctx.render(scene.render_params(flags=(info.flags & ~ptamd.PT_FLAG_MATERIAL_SORT) | sort_bit |
+                                         ptamd.PT_FLAG_COUNT_TRAVERSAL, frames_per_batch=tune.frames_per_batch), 0, spp)
     log(f"[bench] {name}: counted pass {cst.extension_rays} closest-hit, {cst.shadow_rays} shadow rays")
     for _ in range(args.warmup):
-        ctx.render(params, 0, spp)
+        ctx.render(tuned, 0, spp)
     ctx.sync()
     steps = []
     t0 = time.perf_counter()
     for k in range(args.steps):
-        steps.append(ctx.render(params, 0, spp))
+        steps.append(ctx.render(tuned, 0, spp))
         log(f"[bench] {name} step {k}: {steps[-1].gpu_ms:.2f} ms")
     ctx.sync()
     wall = time.perf_counter() - t0
     assert all(s.extension_rays == cst.extension_rays and s.shadow_rays == cst.shadow_rays for s in steps)
+    kst = [ctx.render(direct, 0, spp) for _ in range(2)][-1]  # kernel timings (the image is the same)
     fixture = load_fixture(fixname)
     err, err_t = relmse_at(ctx, W, H, fixture) if fixture else (None, None)
     # the build again, timed alone (the scene and its structures are resident)
@@ -220,8 +237,8 @@ def plain_config(name, args, with_cpu=True):
     # depth 0, the traversal's fetches
     ext_bytes = (ext * (32 + 16) + paths * 16 + cst.node_fetches * 128 + cst.prim_fetches * prim_b +
                  cst.instance_fetches * 64)
-    shade_ms = sum(s.shade_ms for s in steps) / len(steps)
-    ext_ms = sum(s.extend_ms for s in steps) / len(steps)
+    shade_ms = kst.shade_ms
+    ext_ms = kst.extend_ms
     prof = load_profile(f"traffic_{name}")
 
     def roof(kernel, key, bytes_step, ms, launches, extra):
@@ -232,19 +249,21 @@ def plain_config(name, args, with_cpu=True):
              "algorithmic_bytes_per_launch": round(bytes_step / launches), "avg_launch_ms": round(ms / launches, 4)}
         if p.get("hbm_bytes_per_launch"):
             r["traffic_GBs"] = round(p["hbm_bytes_per_launch"] / (ms / launches * 1e-3) / 1e9, 1)
-        for k in ("valu_busy", "occupancy_waves_per_simd", "l2_hit", "wait_frac"):
+        r["node_bytes"] = 128  # BVH4 node: 6 x 4 child slab planes + 4 child refs + 4 counts, fp32
+        r["prim_bytes"] = prim_b
+        for k in ("valu_busy", "occupancy_waves_per_simd", "l2_hit", "mall_hit", "wait_frac"):
             if k in p:
                 r[k] = p[k]
         r.update(extra)
         return r
 
     r_shade = roof("k_shade (shading + shadow-ray BVH4 traversal + compaction)", "k_shade", shade_bytes, shade_ms,
-                   steps[0].shade_launches, {"per_shadow_ray": {
+                   kst.shade_launches, {"per_shadow_ray": {
                        "nodes": round(cst.shadow_node_fetches / max(1, cst.shadow_rays), 3),
                        "prims": round(cst.shadow_prim_fetches / max(1, cst.shadow_rays), 3),
                        "instances": round(cst.shadow_instance_fetches / max(1, cst.shadow_rays), 3)}})
     r_ext = roof("k_extend (closest-hit BVH4 traversal; camera paths started at depth 0)", "k_extend", ext_bytes,
-                 ext_ms, steps[0].extend_launches, {"per_ray": {
+                 ext_ms, kst.extend_launches, {"per_ray": {
                      "nodes": round(cst.node_fetches / ext, 3), "prims": round(cst.prim_fetches / ext, 3),
                      "instances": round(cst.instance_fetches / ext, 3)}})
     dominant, other = (r_ext, r_shade) if ext_ms >= shade_ms else (r_shade, r_ext)
@@ -278,7 +297,9 @@ def plain_config(name, args, with_cpu=True):
         "rays_per_step": {"closest_hit": cst.extension_rays, "shadow": cst.shadow_rays},
         "nan_paths_per_step": cst.nan_paths,
         "kernel_ms_per_step": {"extend": round(ext_ms, 3), "shade": round(shade_ms, 3),
-                               "render_events": round(sum(s.gpu_ms for s in steps) / len(steps), 3)},
+                               "render_events": round(sum(s.gpu_ms for s in steps) / len(steps), 3),
+                               "direct_launches_events": round(kst.gpu_ms, 3)},
+        "tuned_shape": shape,
         "build": {"ms": round(build_ms, 3), "primitives": acc.num_primitives, "nodes": acc.num_nodes,
                   "algorithmic_GBs": round(BUILD_BYTES_PER_PRIM * acc.num_primitives / (build_ms * 1e-3) / 1e9, 1)},
         "roofline": dominant,
@@ -392,13 +413,14 @@ def cache_workload(args):
     # warm-up: kernels loaded, pools sized (frame 0 of both forms)
     ctx.set_camera(orbit_camera(scene.info, 0))
     timed(plain, 0, 1)
-    views, anim = [], {"plain_ms": 0.0, "cache_ms": 0.0, "fallbacks": 0, "frames": 0}
+    views, anim = [], {"plain_ms": 0.0, "cache_ms": 0.0, "fallbacks": 0, "held": 0, "frames": 0}
     mlp = {"train_ms": 0.0, "infer_ms": 0.0, "train_samples": 0, "infer_samples": 0}
     for f in range(args.frames + 1):
         ctx.set_camera(orbit_camera(scene.info, f))
         ms, st = timed(cached, f, 1)
         anim["cache_ms"] += ms
         anim["fallbacks"] += st.cache_fallbacks
+        anim["held"] += st.cache_plain_batches
         anim["frames"] += 1
         mlp["train_ms"] += st.cache_train_ms
         mlp["infer_ms"] += st.cache_infer_ms
@@ -458,6 +480,7 @@ def cache_workload(args):
         "views": views,
         "animation_ms_per_frame": round(anim["cache_ms"] / max(1, anim["frames"]), 3),
         "animation_fallbacks": anim["fallbacks"],
+        "animation_held_plain": anim["held"],
         "roofline": roofline,
         "curves": "[wall ms, relMSE, trimmed relMSE, frames] after 1, 2, 4, ... frames from a cleared image; "
                   "value = geometric mean over the views of time_ratio_at_plain_error_trimmed (trimmed: without "

@@ -307,6 +307,9 @@ typedef struct pt_render_stats {
   double cache_infer_ms;    /* inference over the queries and the judge's rows */
   uint64_t cache_train_samples;  /* samples the training steps processed   */
   uint64_t cache_infer_samples;  /* samples inference processed            */
+  uint32_t cache_plain_batches;  /* batches of this call rendered without
+                                    queries because the network had not
+                                    passed the judge (new, or failed last)  */
 } pt_render_stats;
 
 typedef struct pt_accel_stats {
@@ -543,7 +546,10 @@ pt_status pt_cache_configure(struct pt_context* ctx, uint32_t query_depth, float
  * the squared error the answers would add to the image over that of ending
  * the paths without one.  A batch whose ratio is >= max_ratio, or whose
  * judged outputs are not finite, is rendered again by the plain tracer (its
- * accumulation is then the plain render's, bit for bit); a network with
+ * accumulation is then the plain render's, bit for bit), and the batches
+ * after it render plain (no queries; the training paths still train and
+ * the network is judged after each) until the network passes again; a new
+ * network (pt_cache_init) starts held the same way.  A network with
  * non-finite outputs is re-initialised from the pt_cache_init seed.
  * Defaults: max_ratio 1, rows 16384 (32 .. 2^20). */
 pt_status pt_cache_judge_configure(struct pt_context* ctx, float max_ratio, uint32_t rows);

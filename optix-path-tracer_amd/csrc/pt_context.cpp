@@ -137,7 +137,12 @@ struct pt_context {
   float* cjout = nullptr;           // [rows][3] network outputs
   double* cjres = nullptr;          // judge sums
   double cache_last_judge = -1.0;   // error ratio of the last judged batch (-1: none)
+  // the next batch renders plain (no queries; training paths still train
+  // and the network is still judged): a new network, or the last judge
+  // failed it
+  bool cache_hold = true;
   uint32_t cache_last_fallbacks = 0;
+  uint32_t cache_last_plain_batches = 0;
   float scene_lo[3] = {0, 0, 0}, scene_hi[3] = {1, 1, 1};  // top-level box, for the position encoding
   uint64_t cache_rec_cap = 0;       // slots the record buffers were sized for
   std::vector<DevBuf> crec_bufs;
@@ -1236,11 +1241,13 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
     if (!c->frame_dev) HIPCHK(c, hipMalloc(&c->frame_dev, 64));
     launch_set_frame(c->frame_dev, frame_begin, c->stream);
   }
-  uint32_t fallbacks = 0;
+  uint32_t fallbacks = 0, plain_batches = 0;
   for (uint32_t f0 = 0; f0 < frame_count; f0 += fpb) {
     uint32_t nf = std::min(fpb, frame_count - f0);
     uint32_t nslots = nf * WH;
     rc.frame0 = frame_begin + f0;
+    // a held batch asks nothing (no path reaches this query depth)
+    if (use_cache) rc.cache.query_depth = c->cache_hold ? 0xFFFFFFFFu : c->cache_query_depth;
     const int grid = grid_for(nslots);
     if (use_graph && nf == fpb) {
       // full batches replay a graph captured once for this launch state;
@@ -1314,10 +1321,13 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
       }
       // the judge: training vertices halfway between the ones trained on
       // (the same ones when there are few), finished by the plain tracer,
-      // against the network that is about to answer the queries
+      // against the network that is about to answer the queries (or, for a
+      // held batch, that would answer the next batch's)
       bool use_answers = true;
+      const bool held = c->cache_hold;
+      if (held) plain_batches++;
       const uint32_t nj = std::min(nt, c->cache_judge_rows);
-      if (nq && nj >= 32) {
+      if (nj >= 32) {
         launch_cache_gather(P, 0, nj, golden_stride(nj), grid, c->cjfeat, nullptr, c->cjvals, nullptr, c->stream,
                             std::max<uint32_t>(rows, 1u));
         hipEvent_t ia = want_stats ? mark() : nullptr;
@@ -1333,6 +1343,7 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
         const bool finite = jr[2] == 0.0 && std::isfinite(jr[0]) && std::isfinite(jr[1]);
         c->cache_last_judge = finite ? ratio : NAN;
         use_answers = finite && ratio < (double)c->cache_judge_max;
+        c->cache_hold = !use_answers;
         // a network whose outputs are no longer finite starts again
         if (!finite && (ps = cache_reset(c)) != PT_OK) return ps;
       }
@@ -1346,7 +1357,8 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
       } else if (nq) {
         // fallback: the batch again with the plain tracer, whose results
         // replace the cut paths' (the same samples: seeds depend on pixel
-        // and frame only)
+        // and frame only); the batches after it render plain until the
+        // network passes the judge again
         RenderConsts saved = rc;
         rc.flags &= ~(uint32_t)PT_FLAG_RADIANCE_CACHE;
         enqueue_batch(nslots, grid, kernel_events);
@@ -1361,7 +1373,10 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
     launch_accumulate(P, rc, nf, c->accum, c->stats, grid_for(WH), c->stream);
     c->last_fpb = nf;
   }
-  if (use_cache) c->cache_last_fallbacks = fallbacks;
+  if (use_cache) {
+    c->cache_last_fallbacks = fallbacks;
+    c->cache_last_plain_batches = plain_batches;
+  }
   hipEvent_t t_end = want_stats ? mark() : nullptr;
   if (want_stats) launch_stats_reduce(c->stats, c->stats_total, c->stream);
   c->frames = frame_begin + frame_count;
@@ -1390,6 +1405,7 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
     out->cache_loss = 0.0;
     out->cache_judge = use_cache ? c->cache_last_judge : -1.0;
     out->cache_fallbacks = use_cache ? c->cache_last_fallbacks : 0u;
+    out->cache_plain_batches = use_cache ? c->cache_last_plain_batches : 0u;
     if (use_cache && c->cache_last_steps) {
       const uint32_t n = c->cache_last_batch;
       const uint32_t waves = mlp_fwd_bwd_waves(n);
@@ -1567,6 +1583,7 @@ static pt_status cache_reset(pt_context* c) {
   launch_mlp_pack(c->cp, c->cfw, c->cbw, c->cbias, c->stream);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->cache_step = 0;
+  c->cache_hold = true;
   return PT_OK;
 }
 

@@ -710,7 +710,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
                 s_keep[5 * BLOCK + kt] = L.z;
                 // an environment sample is lit iff its shadow ray hits
                 // nothing (any hit occludes): the first hit found decides it
-                HitOut sh = trace_closest<COUNT, EXT, KINDS>(sc, r.P, Wi, 1e-3f, FINF, st, tc, seed, tm, env);
+                HitOut sh = trace_closest<COUNT, EXT, KINDS, true>(sc, r.P, Wi, 1e-3f, FINF, st, tc, seed, tm, env);
                 T = mk3(s_keep[kt], s_keep[BLOCK + kt], s_keep[2 * BLOCK + kt]);
                 L = mk3(s_keep[3 * BLOCK + kt], s_keep[4 * BLOCK + kt], s_keep[5 * BLOCK + kt]);
                 const v3 emission = env ? env_radiance(sc, Wi) : light_emission(Lt);

@@ -80,10 +80,13 @@ PTD float slab_test(float lox, float hix, float loy, float hiy, float loz, float
 // boxes, meshes) may occur; the tests of the other kinds (and the
 // triangle test's per-instance ray setup) are compiled out.
 constexpr int KIND_ALL = 15;
-// any_hit: return the first candidate accepted in (tmin, tmax) instead of
-// the closest (occlusion queries whose answer is only whether anything is
-// hit: a miss is a miss either way, and a hit is reported as one).
-template <bool COUNT, bool EXT = false, int KINDS = KIND_ALL, class ST>
+// ANY && any_hit: return the first candidate accepted in (tmin, tmax)
+// instead of the closest (occlusion queries whose answer is only whether
+// anything is hit: a miss is a miss either way, and a hit is reported as
+// one).  ANY is a template switch so that the closest-hit kernels carry no
+// trace of the early exit (a runtime-false flag still changes their
+// traversal loop's code and costs them about a tenth of their time).
+template <bool COUNT, bool EXT = false, int KINDS = KIND_ALL, bool ANY = false, class ST>
 PTD HitOut trace_closest(const DevScene& sc, v3 o, v3 d, float tmin, float tmax, ST& st, TraceCounts& cnt,
                          uint32_t vseed = 0u, float time = 0.f, bool any_hit = false) {
   constexpr bool SPH = (KINDS & 1) != 0, RECT = (KINDS & 2) != 0, BOX = (KINDS & 4) != 0, MESH = (KINDS & 8) != 0;
@@ -188,7 +191,7 @@ PTD HitOut trace_closest(const DevScene& sc, v3 o, v3 d, float tmin, float tmax,
           h.inst = (int)inst_id;
           h.prim = (int)orig;
           h.pos = cur;
-          if (any_hit) return h;
+          if (ANY && any_hit) return h;
         }
       }
       continue;

@@ -144,7 +144,8 @@ class RenderStats(C.Structure):
                 ("shade_launches", C.c_uint32), ("nan_paths", C.c_uint64),
                 ("cache_judge", C.c_double), ("cache_fallbacks", C.c_uint32),
                 ("cache_train_ms", C.c_double), ("cache_infer_ms", C.c_double),
-                ("cache_train_samples", C.c_uint64), ("cache_infer_samples", C.c_uint64)]
+                ("cache_train_samples", C.c_uint64), ("cache_infer_samples", C.c_uint64),
+                ("cache_plain_batches", C.c_uint32)]
 
 
 class AccelStats(C.Structure):

@@ -180,7 +180,7 @@ def test_queries_end_paths_and_training_converges(ctx):
         st = ctx.render(params, k, 1)
         losses.append(st.cache_loss)
         judged.append(round(st.cache_judge, 4))
-        fell.append(st.cache_fallbacks)
+        fell.append(st.cache_fallbacks + st.cache_plain_batches)  # frames not answered by the network
     say(f"[gpu] judge per frame {judged}")
     say(f"[gpu] fallbacks per frame {fell}")
     say(f"[gpu] loss first {np.round(losses[:5], 4)}, last {np.round(losses[-5:], 4)}")
@@ -195,10 +195,11 @@ def test_queries_end_paths_and_training_converges(ctx):
     # a batch whose queries were answered
     ctx.cache_judge_configure(max_ratio=1e30)
     try:
-        st = ctx.render(params, 60, 1)
+        ctx.render(params, 60, 1)  # its judge passes the network: the next frame asks it
+        st = ctx.render(params, 61, 1)
     finally:
         ctx.cache_judge_configure()
-    assert st.cache_fallbacks == 0
+    assert st.cache_fallbacks == 0 and st.cache_plain_batches == 0
     _, _, bounces = ctx.read_paths(W, H)
     _, _, tslots, _ = ctx.cache_records(0)
     qf, qv, qslots, nq = ctx.cache_records(1)
@@ -221,52 +222,78 @@ def test_queries_end_paths_and_training_converges(ctx):
 
 
 def test_judge_fallback_renders_the_plain_image(ctx):
-    """A judge that passes nothing (max ratio 0) renders every batch again
-    with the plain tracer: the accumulation equals the plain render's bit
-    for bit, and every batch is counted as a fallback."""
-    say("[gpu] cache judge: fallback to the plain tracer")
+    """A new network is held: until the judge passes it, batches render
+    plain (no queries; training paths still train).  A judge that passes
+    nothing (max ratio 0) therefore keeps every batch plain: the
+    accumulation equals the plain render's bit for bit, and no batch needs
+    rendering twice.  A network that passed once and then fails is caught
+    on the batch that asked it: that batch is rendered again by the plain
+    tracer (a fallback) and the next one is held."""
+    say("[gpu] cache judge: held batches and fallback to the plain tracer")
     W = H = 96
     sc = cornell(ctx, W, H)
     ctx.cache_init(8)
     ctx.cache_configure(query_depth=2, train_fraction=0.03, batch=1024, steps=2)
-    ctx.cache_judge_configure(max_ratio=0.0, rows=2048)
+    cached = sc.render_params(flags=ptamd.PT_FLAG_RADIANCE_CACHE, frames_per_batch=2)
+    plain = sc.render_params(flags=0, frames_per_batch=2)
+    a, _ = render_image(ctx, plain, 0, 6)
     try:
-        cached = sc.render_params(flags=ptamd.PT_FLAG_RADIANCE_CACHE, frames_per_batch=2)
-        plain = sc.render_params(flags=0, frames_per_batch=2)
+        ctx.cache_judge_configure(max_ratio=0.0, rows=2048)
         b, st = render_image(ctx, cached, 0, 6)
-        a, _ = render_image(ctx, plain, 0, 6)
+        assert st.cache_fallbacks == 0 and st.cache_plain_batches == 3, (st.cache_fallbacks, st.cache_plain_batches)
+        assert st.cache_queries == 0 and st.cache_judge >= 0.0
+        assert np.array_equal(a, b)
+        # pass the network once (any finite ratio), then fail it
+        ctx.cache_judge_configure(max_ratio=1e30, rows=2048)
+        st = ctx.render(cached, 0, 2)  # held (the judge above failed it); judged: passes
+        assert st.cache_plain_batches == 1 and st.cache_fallbacks == 0
+        ctx.cache_judge_configure(max_ratio=0.0, rows=2048)
+        st = ctx.render(cached, 2, 4)  # batch 1 asks and falls back, batch 2 is held
+        assert st.cache_fallbacks == 1 and st.cache_plain_batches == 1, (st.cache_fallbacks, st.cache_plain_batches)
+        b, _ = ctx.read_accum(W, H)
     finally:
         ctx.cache_judge_configure()
-    assert st.cache_fallbacks == 3 and st.cache_judge >= 0.0, (st.cache_fallbacks, st.cache_judge)
-    assert np.array_equal(a, b)
+    assert np.array_equal(a, b / 6)
 
 
 def test_forced_divergence_falls_back_to_plain(ctx):
     """A learning rate of 1e3 drives the network's outputs out of range
-    within a batch or two; the judge then refuses its answers and the frame
-    is the plain tracer's, path for path, with a finite accumulation."""
+    within a batch or two: the judge refuses the answers of the batch that
+    asked the diverged network (rendered again by the plain tracer) and
+    holds the later ones, and every such frame is the plain tracer's, path
+    for path, with a finite accumulation."""
     say("[gpu] cache judge: forced divergence")
     W = H = 96
     sc = cornell(ctx, W, H)
     ctx.cache_init(9)
-    ctx.cache_configure(query_depth=2, train_fraction=0.05, batch=1024, steps=4, lr=1e3)
+    ctx.cache_configure(query_depth=2, train_fraction=0.05, batch=1024, steps=4)
     cached = sc.render_params(flags=ptamd.PT_FLAG_RADIANCE_CACHE | ptamd.PT_FLAG_KEEP_PATHS, frames_per_batch=1)
     plain = sc.render_params(flags=ptamd.PT_FLAG_KEEP_PATHS, frames_per_batch=1)
-    fell, judged = [], []
-    for f in range(8):
+    # a sane network that the judge has passed: frame 1 asks it
+    ctx.cache_judge_configure(max_ratio=1e30)
+    ctx.render(cached, 0, 1)
+    ctx.cache_judge_configure()
+    ctx.cache_configure(query_depth=2, train_fraction=0.05, batch=1024, steps=4, lr=1e3)
+    fell, held, judged = [], [], []
+    for f in range(1, 9):
         st = ctx.render(cached, f, 1)
         fell.append(st.cache_fallbacks)
+        held.append(st.cache_plain_batches)
         judged.append(st.cache_judge)
-        if st.cache_fallbacks:
+        if st.cache_fallbacks or st.cache_plain_batches:
             rgb_c, sig_c, _ = ctx.read_paths(W, H)
             acc, _ = ctx.read_accum(W, H)
             assert np.all(np.isfinite(acc))
             ctx.render(plain, f, 1)  # frame f alone (f > 0 continues an accumulation, harmless here)
             rgb_p, sig_p, _ = ctx.read_paths(W, H)
             assert np.array_equal(sig_c, sig_p) and np.array_equal(rgb_c, rgb_p, equal_nan=True)
-    say(f"[gpu] fallbacks per frame {fell}, judge {judged}")
-    assert sum(fell[2:]) == len(fell) - 2, (fell, judged)
-    assert all(not (0.0 <= j < 1.0) for j in judged[2:]), judged
+    say(f"[gpu] fallbacks per frame {fell}, held {held}, judge {judged}")
+    # the first frame or the second asks the network that diverged in training
+    first = next((i for i, x in enumerate(fell) if x), None)
+    assert first is not None and first <= 1, (fell, held, judged)
+    assert all(a + b == 1 for a, b in zip(fell[first:], held[first:])), (fell, held, judged)
+    assert sum(fell[first + 1:]) == 0, (fell, held, judged)  # held after the fallback, not rendered twice
+    assert all(not (0.0 <= j < 1.0) for j in judged[first:]), judged
 
 
 def test_equal_time_gain_over_plain_tracer(ctx):
@@ -297,7 +324,7 @@ def test_equal_time_gain_over_plain_tracer(ctx):
     # steps each (the accumulation is cleared by the render below)
     ctx.render(sc.render_params(flags=ptamd.PT_FLAG_RADIANCE_CACHE, frames_per_batch=8), 5000, 256)
     img_c, st_c = render_image(ctx, cached, 0, fpb)
-    assert st_c.cache_fallbacks == 0, st_c.cache_judge
+    assert st_c.cache_fallbacks == 0 and st_c.cache_plain_batches == 0, st_c.cache_judge
     e_p, e_c = relmse(img_p, ref), relmse(img_c, ref)
     e_eq = e_p * st_p.gpu_ms / st_c.gpu_ms
     say(f"[gpu] {fpb} spp: plain {st_p.gpu_ms:.2f} ms relMSE {e_p:.5f}; cache {st_c.gpu_ms:.2f} ms "
@@ -343,9 +370,13 @@ def test_mesh_orbit_cache_records_targets_queries_and_gain():
         judged, fell = [], []
         for f in range(51):
             ctx.set_camera(orbit_camera(sc.info, f))
+            # frames 49 and 50 take any finite answers, so that frame 50 is
+            # answered and its query records exist
+            ctx.cache_judge_configure(max_ratio=1e30 if f >= 49 else 1.0)
             st = ctx.render(cached, f, 1)
             judged.append(round(st.cache_judge, 3))
-            fell.append(st.cache_fallbacks)
+            fell.append(st.cache_fallbacks + st.cache_plain_batches)
+        ctx.cache_judge_configure()
         say(f"[gpu] judge per frame {judged}")
         say(f"[gpu] fallbacks {sum(fell)} of 51; queries {st.cache_queries}, "
             f"training vertices {st.cache_train_vertices}")
@@ -363,10 +394,10 @@ def test_mesh_orbit_cache_records_targets_queries_and_gain():
         want = np.where(T0 > 0, (rgb[tslots] - L0) / np.where(T0 > 0, T0, 1), 0)
         want = np.where((want > 0) & (want < 3e38), want, 0)
         np.testing.assert_allclose(tgt[ok], want[ok].astype(np.float32), rtol=1e-5, atol=1e-6)
-        if fell[-1] == 0:
-            qf, qv, qslots, nq = ctx.cache_records(1)
-            assert nq == st.cache_queries and np.unique(qslots).size == nq
-            np.testing.assert_array_equal(qv[:, 6:9], ctx.cache_infer(qf))
+        assert fell[-1] == 0, (fell, judged)
+        qf, qv, qslots, nq = ctx.cache_records(1)
+        assert nq == st.cache_queries and np.unique(qslots).size == nq
+        np.testing.assert_array_equal(qv[:, 6:9], ctx.cache_infer(qf))
         # the frozen view of frame 50: 64 spp each from a cleared image
         z = np.load(fx)
         ctx.set_camera(orbit_camera(sc.info, 50))
@@ -379,5 +410,5 @@ def test_mesh_orbit_cache_records_targets_queries_and_gain():
     e_p, e_c = rel(img_p, z["mean"]), rel(img_c, z["mean"])
     t_p, t_c = relmse_trimmed(img_p, z["mean"]), relmse_trimmed(img_c, z["mean"])
     say(f"[gpu] frozen view 50 at 64 spp: relMSE plain {e_p:.4f}, cache {e_c:.4f}; trimmed plain {t_p:.5f}, "
-        f"cache {t_c:.5f}; {st.cache_fallbacks} of 64 cache frames fell back")
+        f"cache {t_c:.5f}; of 64 cache frames {st.cache_fallbacks} fell back, {st.cache_plain_batches} held")
     assert np.isfinite(e_c) and np.isfinite(t_c) and e_c < e_p
