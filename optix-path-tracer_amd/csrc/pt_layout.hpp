@@ -44,16 +44,23 @@ constexpr int STK_LDS = 16;
 // LDS entries spill to its HBM slice of STK_TOTAL entries
 constexpr int STK_TOTAL = 64;
 constexpr int STK_SPILL = STK_TOTAL - STK_LDS;
+// buckets per guide table of the environment sampler's searches
+constexpr uint32_t ENV_GUIDE = 64;
 // closest-hit kernel (k_extend): LDS entries per lane and occupancy target
 // (waves per SIMD); 16 entries x 8 B x 256 lanes is 32 KB per block, which
 // with up to 95 VGPRs holds 5 waves per SIMD
-// buckets per guide table of the environment sampler's searches
-constexpr uint32_t ENV_GUIDE = 64;
 #ifndef PT_EXTEND_STK
 #define PT_EXTEND_STK 16
 #endif
 #ifndef PT_EXTEND_WAVES
 #define PT_EXTEND_WAVES 5
+#endif
+// bounce rays (k_extend after depth 0): a lane whose ray is done takes the
+// wave's next path once at least this many of the wave's lanes are idle
+// (0: the wave's lanes take 64 paths at a time, each group waiting for its
+// longest ray)
+#ifndef PT_EXTEND_REFILL
+#define PT_EXTEND_REFILL 32
 #endif
 constexpr int STK_LDS_EXTEND = PT_EXTEND_STK;
 // Grid of the path kernels: at most MAX_GRID blocks of BLOCK / 64 waves.

@@ -331,7 +331,67 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PT_EXTEND
   const PathState S = P.st[in];
   uint32_t bc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   TraceCounts c{0, 0, 0};
-  for (uint32_t j0 = 0; j0 < n; j0 += 64) {
+  constexpr bool REFILL = PT_EXTEND_REFILL > 0 && !START && !MULTI && (KINDS & 8) != 0;
+  if (REFILL) {
+    // Bounce rays in scenes with meshes differ widely in length (the sky
+    // after one node, a mesh after dozens): each lane keeps tracing paths
+    // of the wave's segment one after another, and the idle lanes take the
+    // next paths together once PT_EXTEND_REFILL of them wait (few ray
+    // set-ups per refill).  Every path is traced exactly as alone, so hits
+    // and counts do not depend on which lane takes it.  (Scenes of a few
+    // primitives, whose rays are all short, run faster in lock-step: the
+    // lanes' steps stay alike.)
+    Tracer<COUNT, EXT, KINDS> tr;
+    uint32_t j = __lane_id();
+    bool busy = j < n;
+    uint32_t next = 64;  // wave-uniform: the next position to hand out
+    auto take = [&](uint32_t jj) {
+      const uint32_t i = g.pos(jj);
+      const float4 o = S.o[i], d = S.d[i];
+      const float tm = (EXT && sc0.mov_instances) ? P.ptime[__float_as_uint(d.w) & SLOT_MASK] : 0.f;
+      tr.begin(sc0, mk3(o), mk3(d), tmin, FINF, st, __float_as_uint(o.w), tm);
+    };
+    if (busy) take(j);
+    while (true) {
+      if (busy && !tr.step(sc0, st, c)) {
+        P.hit[g.pos(j)] = make_float4(tr.h.t, __int_as_float(tr.h.pos), __int_as_float(tr.h.prim),
+                                      __int_as_float(tr.h.inst));
+        busy = false;
+      }
+      const uint64_t idle = __ballot(!busy);
+      if (next >= n) {
+        if (idle == ~0ull) break;
+      } else if (__popcll(idle) >= (uint32_t)PT_EXTEND_REFILL) {
+        if (!busy) {
+          const uint32_t jn = next + lane_rank(idle);
+          if (jn < n) {
+            j = jn;
+            take(j);
+            busy = true;
+          }
+        }
+        next += (uint32_t)__popcll(idle);
+      }
+    }
+    if (SORT) {
+      // material keys and bucket counts of the wave's hits, 64 at a time
+      // (the material look-ups' latency paid once per 64 paths); the
+      // fence makes the hits other lanes stored visible to these loads
+      __threadfence_block();
+      for (uint32_t j0 = 0; j0 < n; j0 += 64) {
+        const uint32_t jj = j0 + __lane_id();
+        uint32_t key = 0xFFu;
+        if (jj < n) {
+          const uint32_t i = g.pos(jj);
+          key = material_key(sc0, P.hit[i]);
+          P.keys[i] = key;
+        }
+#pragma unroll
+        for (uint32_t b = 0; b < 8; b++) bc[b] += (uint32_t)__popcll(__ballot(key == b));
+      }
+    }
+  }
+  for (uint32_t j0 = 0; j0 < (REFILL ? 0u : n); j0 += 64) {
     const uint32_t j = j0 + __lane_id(), i = g.pos(j);
     uint32_t key = 0xFFu;
     if (j < n) {

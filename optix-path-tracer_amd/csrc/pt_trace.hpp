@@ -80,42 +80,80 @@ PTD float slab_test(float lox, float hix, float loy, float hiy, float loz, float
 // boxes, meshes) may occur; the tests of the other kinds (and the
 // triangle test's per-instance ray setup) are compiled out.
 constexpr int KIND_ALL = 15;
-// ANY && any_hit: return the first candidate accepted in (tmin, tmax)
-// instead of the closest (occlusion queries whose answer is only whether
-// anything is hit: a miss is a miss either way, and a hit is reported as
-// one).  ANY is a template switch so that the closest-hit kernels carry no
-// trace of the early exit (a runtime-false flag still changes their
-// traversal loop's code and costs them about a tenth of their time).
-template <bool COUNT, bool EXT = false, int KINDS = KIND_ALL, bool ANY = false, class ST>
-PTD HitOut trace_closest(const DevScene& sc, v3 o, v3 d, float tmin, float tmax, ST& st, TraceCounts& cnt,
-                         uint32_t vseed = 0u, float time = 0.f, bool any_hit = false) {
-  constexpr bool SPH = (KINDS & 1) != 0, RECT = (KINDS & 2) != 0, BOX = (KINDS & 4) != 0, MESH = (KINDS & 8) != 0;
-  HitOut h{tmax, -1, -1, -1};
-  const v3 winv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-  v3 ro = o, rd = d, rinv = winv;
-  TriRay tr{0, 1, 2, 0.f, 0.f, 0.f};
-  int cur = -1;     // instance slot in top-level order, -1 = top level
-  int base_sp = 0;  // stack depth when the instance was entered
-  uint32_t node_base = sc.tlas_root, prim_base = 0, kind = 0, inst_id = 0;
-  st.sp = 0;
+// The traversal of one ray as a resumable state: begin() sets it up, each
+// step() does one trip of the loop (a pop, one internal node, one leaf's
+// primitives or an instance entry) and returns false once the ray is done
+// (h then holds the result).  trace_closest runs the steps back to back;
+// kernels that refill lanes with new rays step several rays at once.
+//
+// ANY && any_hit: the ray ends at the first candidate accepted in
+// (tmin, tmax) instead of the closest (occlusion queries whose answer is
+// only whether anything is hit: a miss is a miss either way, and a hit is
+// reported as one).  ANY is a template switch so that the closest-hit
+// kernels carry no trace of the early exit (a runtime-false flag still
+// changes their traversal loop's code and costs them about a tenth of
+// their time).
+template <bool COUNT, bool EXT = false, int KINDS = KIND_ALL, bool ANY = false>
+struct Tracer {
+  static constexpr bool SPH = (KINDS & 1) != 0, RECT = (KINDS & 2) != 0, BOX = (KINDS & 4) != 0,
+                        MESH = (KINDS & 8) != 0;
+  HitOut h;
+  v3 o, d, winv, ro, rd, rinv;
+  float tmin, time;
+  uint32_t vseed;
+  TriRay tr;
+  int cur;      // instance slot in top-level order, -1 = top level
+  int base_sp;  // stack depth when the instance was entered
+  uint32_t node_base, prim_base, kind, inst_id;
   // the entry in hand: the top-level root first; the nearest child of each
   // node and an instance's root are taken directly instead of through the
   // stack, which visits the same nodes in the same order
-  uint32_t ref = 0u;
-  float etn = tmin;
-  bool have = true;
+  uint32_t ref;
+  float etn;
+  bool have, any_hit;
   // bound on loop trips: a well-formed structure never comes close
-  uint32_t trips = 0;
-  while (++trips < (1u << 22)) {
+  uint32_t trips;
+
+  template <class ST>
+  PTD void begin(const DevScene& sc, v3 o_, v3 d_, float tmin_, float tmax, ST& st, uint32_t vseed_ = 0u,
+                 float time_ = 0.f, bool any_hit_ = false) {
+    h = HitOut{tmax, -1, -1, -1};
+    o = o_;
+    d = d_;
+    tmin = tmin_;
+    time = time_;
+    vseed = vseed_;
+    any_hit = any_hit_;
+    winv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    ro = o;
+    rd = d;
+    rinv = winv;
+    tr = TriRay{0, 1, 2, 0.f, 0.f, 0.f};
+    cur = -1;
+    base_sp = 0;
+    node_base = sc.tlas_root;
+    prim_base = 0;
+    kind = 0;
+    inst_id = 0;
+    st.sp = 0;
+    ref = 0u;
+    etn = tmin;
+    have = true;
+    trips = 0;
+  }
+
+  template <class ST>
+  PTD bool step(const DevScene& sc, ST& st, TraceCounts& cnt) {
+    if (!(++trips < (1u << 22))) return false;
     if (!have) {
       if (cur >= 0 && st.sp == base_sp) {
         cur = -1;
         node_base = sc.tlas_root;
         ro = o; rd = d; rinv = winv;
       }
-      if (st.sp == 0) break;
+      if (st.sp == 0) return false;
       st.pop(ref, etn);
-      if (ref == EMPTY_CHILD || etn > h.t) continue;
+      if (ref == EMPTY_CHILD || etn > h.t) return true;
     }
     have = false;
     if (ref & LEAF_BIT) {
@@ -143,7 +181,7 @@ PTD HitOut trace_closest(const DevScene& sc, v3 o, v3 d, float tmin, float tmax,
         base_sp = st.sp;
         ref = 0u;  // the bottom-level root, next
         have = true;
-        continue;
+        return true;
       }
       for (uint32_t k = 0; k < count; k++) {
         uint32_t p = prim_base + first + k;
@@ -191,10 +229,10 @@ PTD HitOut trace_closest(const DevScene& sc, v3 o, v3 d, float tmin, float tmax,
           h.inst = (int)inst_id;
           h.prim = (int)orig;
           h.pos = cur;
-          if (ANY && any_hit) return h;
+          if (ANY && any_hit) return false;
         }
       }
-      continue;
+      return true;
     }
     // internal node
     if (COUNT) cnt.nodes++;
@@ -239,8 +277,18 @@ PTD HitOut trace_closest(const DevScene& sc, v3 o, v3 d, float tmin, float tmax,
       }
     }
     if (nh > 0) st.sp += nh - 1;
+    return true;
   }
-  return h;
+};
+
+template <bool COUNT, bool EXT = false, int KINDS = KIND_ALL, bool ANY = false, class ST>
+PTD HitOut trace_closest(const DevScene& sc, v3 o, v3 d, float tmin, float tmax, ST& st, TraceCounts& cnt,
+                         uint32_t vseed = 0u, float time = 0.f, bool any_hit = false) {
+  Tracer<COUNT, EXT, KINDS, ANY> t;
+  t.begin(sc, o, d, tmin, tmax, st, vseed, time, any_hit);
+  while (t.step(sc, st, cnt)) {
+  }
+  return t.h;
 }
 
 }  // namespace pt
