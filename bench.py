@@ -193,17 +193,24 @@ def plain_config(name, args, with_cpu=True):
     # launch shape: pt_autotune's pick for this view over the whole step
     # (paths per batch, material sort, graph replay; untimed)
     t_tune = time.perf_counter()
-    tune = ctx.autotune(params, spp)
+    if args.shape:
+        # a fixed shape of direct launches (profiler runs: rocprofv3's
+        # kernel trace has crashed inside graph capture)
+        fpb, so = (int(x) for x in args.shape.split(","))
+        tune = ptamd.TuneResult(frames_per_batch=fpb, material_sort=so, graph=0, candidates=0,
+                                paths_per_batch=fpb * W * H)
+    else:
+        tune = ctx.autotune(params, spp)
     t_tune = time.perf_counter() - t_tune
     shape = {"frames_per_batch": tune.frames_per_batch, "paths_per_batch": tune.paths_per_batch,
              "material_sort": bool(tune.material_sort), "graph": bool(tune.graph), "candidates": tune.candidates,
              "tuning_s": round(t_tune, 1)}
-    log(f"[bench] {name}: tuned shape {shape}")
-    tuned = scene.render_params(flags=info.flags | ptamd.PT_FLAG_TUNED)
+    log(f"[bench] {name}: {'fixed' if args.shape else 'tuned'} shape {shape}")
     # direct launches of the same shape, for per-kernel HIP-event timings
     sort_bit = ptamd.PT_FLAG_MATERIAL_SORT if tune.material_sort else 0
     direct = scene.render_params(flags=(info.flags & ~ptamd.PT_FLAG_MATERIAL_SORT) | sort_bit,
                                  frames_per_batch=tune.frames_per_batch)
+    tuned = direct if args.shape else scene.render_params(flags=info.flags | ptamd.PT_FLAG_TUNED)
     # traversal-counting pass (identical rays: the render is deterministic)
     cst = ctx.render(scene.render_params(flags=(info.flags & ~ptamd.PT_FLAG_MATERIAL_SORT) | sort_bit |
                                          ptamd.PT_FLAG_COUNT_TRAVERSAL, frames_per_batch=tune.frames_per_batch), 0, spp)
@@ -251,7 +258,7 @@ def plain_config(name, args, with_cpu=True):
             r["traffic_GBs"] = round(p["hbm_bytes_per_launch"] / (ms / launches * 1e-3) / 1e9, 1)
         r["node_bytes"] = 128  # BVH4 node: 6 x 4 child slab planes + 4 child refs + 4 counts, fp32
         r["prim_bytes"] = prim_b
-        for k in ("valu_busy", "occupancy_waves_per_simd", "l2_hit", "mall_hit", "wait_frac"):
+        for k in ("valu_busy", "occupancy_waves_per_simd", "l2_hit", "wait_frac"):
             if k in p:
                 r[k] = p[k]
         r.update(extra)
@@ -603,6 +610,9 @@ def main():
                                                             "100, 200, 300)")
     ap.add_argument("--eval-spp", type=int, default=64, help="cache workload: frames rendered per frozen view")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--shape", default="", metavar="FPB,SORT",
+                    help="plain workloads: frames per batch and material sort (0/1) of direct launches instead of "
+                         "the autotuner's pick")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
