@@ -171,11 +171,17 @@ struct Tracer {
         prim_base = info.y;
         kind = info.z & 0xFFu;
         inst_id = info.w;
-        if (info.z >> 8) {
-          ro = xf_point(w0, w1, w2, o);
-          rd = xf_vector(w0, w1, w2, d);
-          rinv = mk3(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
-        }
+        // the object-space ray is formed for every instance and kept where
+        // the instance has a transform: formed behind a branch on that flag,
+        // the matrix would be loaded only after the flag arrives (a second
+        // memory round trip per instance entry)
+        // (component selects: a select of whole structs keeps this state
+        // in scratch memory)
+        const bool xf = (info.z >> 8) != 0;
+        const v3 xo = xf_point(w0, w1, w2, o), xd = xf_vector(w0, w1, w2, d);
+        ro = mk3(xf ? xo.x : o.x, xf ? xo.y : o.y, xf ? xo.z : o.z);
+        rd = mk3(xf ? xd.x : d.x, xf ? xd.y : d.y, xf ? xd.z : d.z);
+        if (xf) rinv = mk3(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
         if (MESH && kind == PT_GEOM_MESH) tr = tri_setup(rd);
         cur = (int)first;
         base_sp = st.sp;
@@ -244,13 +250,15 @@ struct Tracer {
     const uint32_t chd[4] = {q6.x, q6.y, q6.z, q6.w}, cnc[4] = {q7.x, q7.y, q7.z, q7.w};
     float tn[4];
     uint32_t cref[4];
+    // every slot's test runs and empty slots are masked afterwards: a test
+    // behind a branch on the slot lets the compiler load that slot's planes
+    // only after the child words arrive, a second memory round trip per node
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-      tn[c] = FINF;
-      if (chd[c] != EMPTY_CHILD)
-        tn[c] = slab_test(__uint_as_float(lox[c]), __uint_as_float(hix[c]), __uint_as_float(loy[c]),
-                          __uint_as_float(hiy[c]), __uint_as_float(loz[c]), __uint_as_float(hiz[c]), ro, rinv, tmin,
-                          h.t);
+      const float ts = slab_test(__uint_as_float(lox[c]), __uint_as_float(hix[c]), __uint_as_float(loy[c]),
+                                 __uint_as_float(hiy[c]), __uint_as_float(loz[c]), __uint_as_float(hiz[c]), ro, rinv,
+                                 tmin, h.t);
+      tn[c] = chd[c] != EMPTY_CHILD ? ts : FINF;
       cref[c] = cnc[c] ? (LEAF_BIT | ((cnc[c] - 1u) << 28) | chd[c]) : chd[c];
     }
     // rank by (tn, slot): the nearest is taken next, the others go on the
