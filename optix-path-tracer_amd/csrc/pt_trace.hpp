@@ -114,6 +114,26 @@ struct Tracer {
   // bound on loop trips: a well-formed structure never comes close
   uint32_t trips;
 
+  // the watertight test of one triangle record and, when it is nearer (or
+  // as near with the smaller key), its acceptance; true when an any-hit
+  // query ends here
+  PTD bool offer_tri(float4 a, float4 b, float4 c, TraceCounts& cnt) {
+    if (COUNT) cnt.prims++;
+    float t = FINF, b1, b2;
+    if (!tri_test(tr, ro, mk3(a), mk3(b), mk3(c), t, b1, b2)) t = FINF;
+    const uint32_t orig = __float_as_uint(a.w);
+    if (!(t > tmin)) return false;
+    bool take = t < h.t;
+    if (!take && t == h.t && h.inst >= 0)
+      take = ((int)inst_id < h.inst) || ((int)inst_id == h.inst && (int)orig < h.prim);
+    if (!take) return false;
+    h.t = t;
+    h.inst = (int)inst_id;
+    h.prim = (int)orig;
+    h.pos = cur;
+    return ANY && any_hit;
+  }
+
   template <class ST>
   PTD void begin(const DevScene& sc, v3 o_, v3 d_, float tmin_, float tmax, ST& st, uint32_t vseed_ = 0u,
                  float time_ = 0.f, bool any_hit_ = false) {
@@ -187,6 +207,21 @@ struct Tracer {
         base_sp = st.sp;
         ref = 0u;  // the bottom-level root, next
         have = true;
+        return true;
+      }
+      if (MESH && kind == PT_GEOM_MESH) {
+        // triangles two at a time: both records are loaded before either
+        // test, one memory round trip per pair (a leaf holds up to 4); the
+        // candidates are still offered in leaf order
+        for (uint32_t k = 0; k < count; k += 2) {
+          const uint32_t p0 = prim_base + first + k;
+          const bool two = k + 1 < count;
+          const uint32_t p1 = two ? p0 + 1 : p0;
+          const float4 a0 = sc.tri[3 * p0], b0 = sc.tri[3 * p0 + 1], c0 = sc.tri[3 * p0 + 2];
+          const float4 a1 = sc.tri[3 * p1], b1 = sc.tri[3 * p1 + 1], c1 = sc.tri[3 * p1 + 2];
+          if (offer_tri(a0, b0, c0, cnt)) return false;
+          if (two && offer_tri(a1, b1, c1, cnt)) return false;
+        }
         return true;
       }
       for (uint32_t k = 0; k < count; k++) {
