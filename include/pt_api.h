@@ -267,9 +267,9 @@ typedef struct pt_render_params {
   int32_t russian;          /* russian (main.cpp:62)                       */
   uint32_t rr_min_depth;    /* RR applies when depth >= this; reference 11 */
   uint32_t frames_per_batch;/* frames traced together; 0 = automatic: as
-                               many as 2^28 paths or a quarter of the
-                               device's memory (252 B of path state each)
-                               holds, whichever is less                    */
+                               many as 2^29 paths or half of the device's
+                               memory (252 B of path state each) holds,
+                               whichever is less                           */
   uint32_t flags;           /* PT_FLAG_*                                   */
   uint32_t reserved;
 } pt_render_params;

@@ -30,14 +30,15 @@ using namespace pt;
 namespace {
 
 // Paths in flight per batch when the caller leaves it to the library: as
-// many as a quarter of the device's memory holds (PATH_BYTES of path state
-// each), at most 2^28.  Every batch ends in a dozen bounces whose launches
-// last as long as their slowest waves (a few paths trapped in glass or
-// between meshes), whatever the number of paths, so the fewer batches a
-// render takes, the less of that tail it pays: the 1080p 256-spp mesh scene
-// takes 487 ms in batches of 8 frames (16.6 M paths), 233 ms in batches of
-// 128 frames (265 M paths, 67 GB).
-constexpr uint64_t PATHS_MAX = 1ull << 28;
+// many as half of the device's memory holds (PATH_BYTES of path state
+// each), at most 2^29 (135 GB on the 288 GB MI355X).  Every batch ends in a
+// dozen bounces whose launches last as long as their slowest waves (a few
+// paths trapped in glass or between meshes), whatever the number of paths,
+// so the fewer batches a render takes, the less of that tail it pays: the
+// 1080p 256-spp mesh scene took 487 ms in batches of 8 frames (16.6 M
+// paths) and 233 ms in batches of 128 frames (265 M paths, 67 GB); all 256
+// frames fit one batch of 2^29.
+constexpr uint64_t PATHS_MAX = 1ull << 29;
 constexpr uint64_t PATH_BYTES = 3 * 64 + 16 + 16 + 4 + 16 + 4 + 4;  // per path, ensure_paths
 
 struct DevBuf {
@@ -338,7 +339,7 @@ pt_status pt_create(int32_t device, pt_context** out) {
   {
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total_b > 0)
-      c->paths_target = std::max<uint64_t>(1u << 20, std::min<uint64_t>(PATHS_MAX, total_b / 4 / PATH_BYTES));
+      c->paths_target = std::max<uint64_t>(1u << 20, std::min<uint64_t>(PATHS_MAX, total_b / 2 / PATH_BYTES));
   }
   e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -2030,7 +2031,7 @@ pt_status pt_autotune(pt_context* c, const pt_render_params* rp, uint32_t frames
   const uint64_t WH = (uint64_t)rp->width * rp->height;
   if (WH == 0) return fail(c, PT_ERR_INVALID_ARG, "empty image");
   std::vector<uint32_t> fpbs;
-  for (uint64_t paths : {1ull << 20, 1ull << 22, 1ull << 24, 1ull << 26, 1ull << 28}) {
+  for (uint64_t paths : {1ull << 20, 1ull << 22, 1ull << 24, 1ull << 26, 1ull << 28, 1ull << 29}) {
     paths = std::min(paths, c->paths_target);
     uint32_t f = (uint32_t)std::max<uint64_t>(1, paths / WH);
     f = std::min(f, frames);
