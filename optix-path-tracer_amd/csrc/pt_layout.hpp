@@ -62,10 +62,6 @@ constexpr uint32_t ENV_GUIDE = 64;
 #ifndef PT_EXTEND_REFILL
 #define PT_EXTEND_REFILL 32
 #endif
-// the same in the batches of scenes (lanes step in their path's scene)
-#ifndef PT_MULTI_REFILL
-#define PT_MULTI_REFILL 1
-#endif
 constexpr int STK_LDS_EXTEND = PT_EXTEND_STK;
 // Grid of the path kernels: at most MAX_GRID blocks of BLOCK / 64 waves.
 constexpr int MAX_GRID = 2048;

@@ -331,7 +331,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PT_EXTEND
   const PathState S = P.st[in];
   uint32_t bc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   TraceCounts c{0, 0, 0};
-  constexpr bool REFILL = PT_EXTEND_REFILL > 0 && !START && (KINDS & 8) != 0 && (!MULTI || PT_MULTI_REFILL);
+  constexpr bool REFILL = PT_EXTEND_REFILL > 0 && !START && !MULTI && (KINDS & 8) != 0;
   if (REFILL) {
     // Bounce rays in scenes with meshes differ widely in length (the sky
     // after one node, a mesh after dozens): each lane keeps tracing paths
@@ -341,22 +341,19 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PT_EXTEND
     // and counts do not depend on which lane takes it.  (Scenes of a few
     // primitives, whose rays are all short, run faster in lock-step: the
     // lanes' steps stay alike.)
-    // (a batch of scenes: each lane steps in its path's scene)
     Tracer<COUNT, EXT, KINDS> tr;
-    uint32_t j = __lane_id(), sid = 0;
+    uint32_t j = __lane_id();
     bool busy = j < n;
     uint32_t next = 64;  // wave-uniform: the next position to hand out
     auto take = [&](uint32_t jj) {
       const uint32_t i = g.pos(jj);
       const float4 o = S.o[i], d = S.d[i];
-      if (MULTI) sid = __float_as_uint(o.w) >> 24;
-      const DevScene& sc = MULTI ? scenes[sid] : sc0;
-      const float tm = (EXT && sc.mov_instances) ? P.ptime[__float_as_uint(d.w) & SLOT_MASK] : 0.f;
-      tr.begin(sc, mk3(o), mk3(d), tmin, FINF, st, __float_as_uint(o.w), tm);
+      const float tm = (EXT && sc0.mov_instances) ? P.ptime[__float_as_uint(d.w) & SLOT_MASK] : 0.f;
+      tr.begin(sc0, mk3(o), mk3(d), tmin, FINF, st, __float_as_uint(o.w), tm);
     };
     if (busy) take(j);
     while (true) {
-      if (busy && !tr.step(MULTI ? scenes[sid] : sc0, st, c)) {
+      if (busy && !tr.step(sc0, st, c)) {
         P.hit[g.pos(j)] = make_float4(tr.h.t, __int_as_float(tr.h.pos), __int_as_float(tr.h.prim),
                                       __int_as_float(tr.h.inst));
         busy = false;
@@ -386,7 +383,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PT_EXTEND
         uint32_t key = 0xFFu;
         if (jj < n) {
           const uint32_t i = g.pos(jj);
-          key = material_key(MULTI ? scenes[__float_as_uint(S.o[i].w) >> 24] : sc0, P.hit[i]);
+          key = material_key(sc0, P.hit[i]);
           P.keys[i] = key;
         }
 #pragma unroll
