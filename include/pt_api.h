@@ -470,8 +470,9 @@ pt_status pt_read_env_tables(struct pt_context* ctx, uint64_t* cond, uint64_t* m
 /* ------------------------------------------------------------ autotuning
  * No counterpart in the reference.  pt_autotune renders `frames` frames of
  * the current scene under candidate launch shapes (paths in flight 1, 4,
- * 16, 64 and 256 Mi, at most the automatic batch size,
- * material sort on/off, batch graphs on/off), keeps the fastest, and
+ * 16, 64, 256 and 512 Mi, at most the automatic batch size, in whole
+ * frames; material sort on/off, batch graphs on/off; a warm-up render and
+ * the faster of two timed ones each), keeps the fastest, and
  * pt_render uses it for renders with PT_FLAG_TUNED and the same resolution,
  * depth and flags.  Results are identical under every shape. */
 typedef struct pt_tune_result {
