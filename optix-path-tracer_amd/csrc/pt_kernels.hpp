@@ -52,7 +52,7 @@ void launch_next_frames(uint32_t* frame_dev, uint32_t step, hipStream_t st);
 void launch_extend(const DevScene& sc, const DevPaths& P, int in, float tmin, uint2* spill, uint64_t* stats,
                    bool count_traversal, bool sort, int grid, hipStream_t st, const DevScene* scenes = nullptr,
                    const RenderConsts* start = nullptr, uint32_t nstart = 0, bool multi_ext = false);
-void launch_sort(const DevPaths& P, int in, int grid, hipStream_t st, bool first);
+void launch_sort(const DevPaths& P, int in, int grid, hipStream_t st);
 void launch_shade(const DevScene& sc, const DevPaths& P, int in, bool sorted, int out, uint32_t depth,
                   const RenderConsts& rc, int matset, bool count, uint2* spill, uint64_t* stats, int grid,
                   hipStream_t st, const DevScene* scenes = nullptr, const RenderConsts* rcs = nullptr,

@@ -506,10 +506,6 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_sort_scan(DevPaths P, uint32_t
 }
 // Move path state and hit into material-bucket order (state set 2, a dense
 // set): each wave writes its bucket-b paths from its scanned base on.
-// At depth 0 (`first`) the paths' throughput and radiance are the
-// path-start constants, which the shading pass takes without reading them:
-// only origin, direction and hit move (80 of 144 bytes per path).
-template <bool FIRST>
 __global__ __launch_bounds__(BLOCK) void k_sort_scatter(DevPaths P, int in) {
   const Seg g = my_seg();
   const uint32_t n = seg_count(P, in, g);
@@ -530,10 +526,8 @@ __global__ __launch_bounds__(BLOCK) void k_sort_scatter(DevPaths P, int in) {
     if (key < 8) {
       D.o[dst] = S.o[i];
       D.d[dst] = S.d[i];
-      if (!FIRST) {
-        D.T[dst] = S.T[i];
-        D.L[dst] = S.L[i];
-      }
+      D.T[dst] = S.T[i];
+      D.L[dst] = S.L[i];
       P.hit_sorted[dst] = P.hit[i];
     }
   }
@@ -1173,17 +1167,23 @@ void launch_extend(const DevScene& sc, const DevPaths& P, int in, float tmin, ui
     else if (sort) PT_EXTEND(false, true, M, E, K, ST);                  \
     else PT_EXTEND(false, false, M, E, K, ST);                           \
   } while (0)
-#define PT_EXTEND_KINDS(ST)                                                                              \
+// the camera-ray pass never sorts (camera hits are shaded in pixel order)
+#define PT_EXTEND2(M, E, K, ST)                                     \
+  do {                                                              \
+    if (count_traversal) PT_EXTEND(true, false, M, E, K, ST);       \
+    else PT_EXTEND(false, false, M, E, K, ST);                      \
+  } while (0)
+#define PT_EXTEND_KINDS(ST, X)                                                                           \
   do {                                                                                                   \
     if (sc.ext_instances != 0) { /* volumes, cylinders, moving spheres */                              \
-      PT_EXTEND4(false, true, KIND_ALL, ST);                                                             \
+      X(false, true, KIND_ALL, ST);                                                                      \
     } else {                                                                                             \
       switch (extend_kinds(sc.kinds)) {                                                                  \
-        case 6: PT_EXTEND4(false, false, 6, ST); break;                                                  \
-        case 10: PT_EXTEND4(false, false, 10, ST); break;                                                \
-        case 1: PT_EXTEND4(false, false, 1, ST); break;                                                  \
-        case 7: PT_EXTEND4(false, false, 7, ST); break;                                                  \
-        default: PT_EXTEND4(false, false, KIND_ALL, ST);                                                 \
+        case 6: X(false, false, 6, ST); break;                                                           \
+        case 10: X(false, false, 10, ST); break;                                                         \
+        case 1: X(false, false, 1, ST); break;                                                           \
+        case 7: X(false, false, 7, ST); break;                                                           \
+        default: X(false, false, KIND_ALL, ST);                                                          \
       }                                                                                                  \
     }                                                                                                    \
   } while (0)
@@ -1192,20 +1192,18 @@ void launch_extend(const DevScene& sc, const DevPaths& P, int in, float tmin, ui
   } else if (scenes) {
     PT_EXTEND4(true, false, KIND_ALL, false);
   } else if (starts) {
-    PT_EXTEND_KINDS(true);
+    PT_EXTEND_KINDS(true, PT_EXTEND2);
   } else {
-    PT_EXTEND_KINDS(false);
+    PT_EXTEND_KINDS(false, PT_EXTEND4);
   }
 #undef PT_EXTEND_KINDS
+#undef PT_EXTEND2
 #undef PT_EXTEND4
 #undef PT_EXTEND
 }
-void launch_sort(const DevPaths& P, int in, int grid, hipStream_t st, bool first) {
+void launch_sort(const DevPaths& P, int in, int grid, hipStream_t st) {
   hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(SCAN_THREADS), 0, st, P, (uint32_t)(grid * WAVES));
-  if (first)
-    hipLaunchKernelGGL(k_sort_scatter<true>, dim3(grid), dim3(BLOCK), 0, st, P, in);
-  else
-    hipLaunchKernelGGL(k_sort_scatter<false>, dim3(grid), dim3(BLOCK), 0, st, P, in);
+  hipLaunchKernelGGL(k_sort_scatter, dim3(grid), dim3(BLOCK), 0, st, P, in);
 }
 template <int MS, int FEAT, int KINDS = KIND_ALL>
 static void launch_shade_v(const DevScene& sc, const DevPaths& P, int in, int src, const float4* hits, int out,
