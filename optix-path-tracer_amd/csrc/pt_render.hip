@@ -311,13 +311,23 @@ PTD uint32_t material_key(const DevScene& sc, float4 h) {
 // round seed hash) and the traversal latency-bound, so one launch hides
 // most of the former under the latter.  A separate instantiation: the
 // path-start code costs the traversal of the other depths a few spills.
+// Occupancy and LDS stack entries per lane of a closest-hit instantiation:
+// without meshes (short stacks, fewer registers) the bounce-ray kernels fit
+// 6 waves per SIMD with 12 LDS entries and no spills (Cornell box 10.2 ->
+// 10.1 ms, random spheres 2.48 -> 2.38 ms per step); the camera-ray ones
+// spill at 6 and keep 5.
+template <int KINDS, bool START>
+constexpr int extend_waves() { return (KINDS & 8) || START ? PT_EXTEND_WAVES : 6; }
+template <int KINDS>
+constexpr int extend_stk() { return (KINDS & 8) ? STK_LDS_EXTEND : 12; }
 template <bool COUNT, bool SORT, bool MULTI, bool EXT, int KINDS = KIND_ALL, bool START = false>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PT_EXTEND_WAVES, 8))) void k_extend(
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(extend_waves<KINDS, START>(), 8))) void k_extend(
     DevScene sc0, DevPaths P, int in, float tmin, uint2* spill, uint64_t* stats, const DevScene* __restrict__ scenes,
     RenderConsts rc, uint32_t nstart) {
-  __shared__ uint32_t s_ref[STK_LDS_EXTEND * BLOCK];
-  __shared__ float s_tn[STK_LDS_EXTEND * BLOCK];
-  StackT<STK_LDS_EXTEND> st = make_stack<STK_LDS_EXTEND>(s_ref, s_tn, spill, P.counters + CTR_STACK_OVERFLOW);
+  constexpr int NS = extend_stk<KINDS>();
+  __shared__ uint32_t s_ref[NS * BLOCK];
+  __shared__ float s_tn[NS * BLOCK];
+  StackT<NS> st = make_stack<NS>(s_ref, s_tn, spill, P.counters + CTR_STACK_OVERFLOW);
   const Seg g = my_seg();
   constexpr bool starts = START && !MULTI;
   uint32_t n;
