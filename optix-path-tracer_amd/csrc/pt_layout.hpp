@@ -46,9 +46,10 @@ constexpr int STK_TOTAL = 64;
 constexpr int STK_SPILL = STK_TOTAL - STK_LDS;
 // buckets per guide table of the environment sampler's searches
 constexpr uint32_t ENV_GUIDE = 64;
-// closest-hit kernel (k_extend): LDS entries per lane and occupancy target
-// (waves per SIMD); 16 entries x 8 B x 256 lanes is 32 KB per block, which
-// with up to 95 VGPRs holds 5 waves per SIMD
+// closest-hit kernel (k_extend) in scenes with meshes and in the camera-ray
+// pass: LDS entries per lane and occupancy target (waves per SIMD); 16
+// entries x 8 B x 256 lanes is 32 KB per block, which with up to 96 VGPRs
+// holds 5 waves per SIMD (the other instantiations: extend_waves)
 #ifndef PT_EXTEND_STK
 #define PT_EXTEND_STK 16
 #endif
