@@ -330,6 +330,12 @@ def default_workload(args):
            "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic"}
     out.update({k: v for k, v in head.items() if k not in out and k != "metric"})
+    # the 64-scene batch (its own line: bench.py --workload batch)
+    try:
+        extra["batch"] = batch_workload(args)
+    except Exception as e:
+        log(f"[bench] batch workload failed: {e}")
+        extra["batch"] = None
     out["configs"] = extra
     print(json.dumps(out), flush=True)
 
@@ -527,15 +533,36 @@ def batch_workload(args):
     frames = [sc.info.spp for sc in scenes]
     paths = sum(sc.info.width * sc.info.height * sc.info.spp for sc in scenes)
     t0 = time.perf_counter()
-    tune = ptamd.autotune_batch(ctxs, params, [max(1, f // 4) for f in frames])
-    log(f"[bench] batch autotune: {tune.paths_per_batch >> 20} Mi paths per sub-batch, sort {tune.material_sort}, "
-        f"graph {tune.graph} ({tune.candidates} candidates, {time.perf_counter() - t0:.1f} s)")
+    f0 = scenes[0].info.flags
     tuned = list(params)
-    tuned[0] = scenes[0].render_params(flags=scenes[0].info.flags | ptamd.PT_FLAG_TUNED)
-    st = ptamd.render_batch(ctxs, tuned, frames)  # rays of one step (and warm-up)
+    if args.shape:
+        # direct launches at the automatic sub-batch size with the given
+        # material sort (profiler runs: no graph capture)
+        so = int(args.shape.split(",")[-1])
+        f0 = (f0 & ~ptamd.PT_FLAG_MATERIAL_SORT) | (ptamd.PT_FLAG_MATERIAL_SORT if so else 0)
+        tuned[0] = scenes[0].render_params(flags=f0)
+        shape = {"fixed": True, "material_sort": bool(so), "graph": False}
+    else:
+        tune = ptamd.autotune_batch(ctxs, params, [max(1, f // 4) for f in frames])
+        log(f"[bench] batch autotune: {tune.paths_per_batch >> 20} Mi paths per sub-batch, sort "
+            f"{tune.material_sort}, graph {tune.graph} ({tune.candidates} candidates, "
+            f"{time.perf_counter() - t0:.1f} s)")
+        f0 |= ptamd.PT_FLAG_TUNED
+        tuned[0] = scenes[0].render_params(flags=f0)
+        shape = {"paths_per_batch": tune.paths_per_batch, "material_sort": bool(tune.material_sort),
+                 "graph": bool(tune.graph), "candidates": tune.candidates}
+    # counted pass (node / primitive fetches per ray) and a direct pass of
+    # the same shape for per-kernel HIP-event times; the images are the same
+    counted = list(tuned)
+    counted[0] = scenes[0].render_params(flags=f0 | ptamd.PT_FLAG_COUNT_TRAVERSAL | ptamd.PT_FLAG_DIRECT)
+    cst = ptamd.render_batch(ctxs, counted, frames)
+    direct = list(tuned)
+    direct[0] = scenes[0].render_params(flags=f0 | ptamd.PT_FLAG_DIRECT)
+    kst = [ptamd.render_batch(ctxs, direct, frames) for _ in range(2)][-1]
+    st = cst
     rays_step = st.extension_rays + st.shadow_rays
     log(f"[bench] batch: {paths} paths, {st.extension_rays} closest-hit + {st.shadow_rays} shadow rays, "
-        f"{st.gpu_ms:.1f} ms")
+        f"{kst.gpu_ms:.1f} ms direct ({kst.extend_ms:.1f} closest-hit, {kst.shade_ms:.1f} shading)")
     for _ in range(args.warmup):
         ptamd.render_batch(ctxs, tuned, frames, stats=False)
     ctxs[0].sync()
@@ -562,6 +589,44 @@ def batch_workload(args):
     for ctx in ctxs:
         ctx.close()
     ms = wall * 1e3 / args.steps
+    # rooflines of the batch's kernels from the counted pass and the direct
+    # pass's kernel times: algorithmic bytes as for the single scenes, with
+    # 16 B per primitive fetch (a sphere record; a triangle's is 48 B, so the
+    # achieved figures are lower bounds)
+    ext, npaths, exh = cst.extension_rays, cst.paths, cst.exhausted_paths
+    prim_b = 16
+    ext_bytes = (ext * (32 + 16) + npaths * 16 + cst.node_fetches * 128 + cst.prim_fetches * prim_b +
+                 cst.instance_fetches * 64)
+    shade_bytes = (ext * (64 + 16) - npaths * 32 + (ext - npaths + exh) * 64 + (npaths - exh) * 20 +
+                   cst.shadow_node_fetches * 128 + cst.shadow_prim_fetches * prim_b +
+                   cst.shadow_instance_fetches * 64)
+    prof = load_profile("traffic_batch")
+
+    def roof(kernel, key, b, kms, launches):
+        ach = b / (kms * 1e-3) / 1e9
+        p = prof.get(key, {})
+        r = {"kernel": kernel, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": p.get("hbm_bytes_per_launch"),
+             "algorithmic_bytes_per_launch": round(b / max(1, launches)),
+             "avg_launch_ms": round(kms / max(1, launches), 4), "node_bytes": 128, "prim_bytes": prim_b}
+        for k in ("valu_busy", "occupancy_waves_per_simd", "l2_hit", "wait_frac"):
+            if k in p:
+                r[k] = p[k]
+        return r
+
+    r_ext = roof("k_extend (closest hit, every scene's paths in one launch per bounce)", "k_extend", ext_bytes,
+                 kst.extend_ms, kst.extend_launches)
+    r_ext["per_ray"] = {"nodes": round(cst.node_fetches / ext, 3), "prims": round(cst.prim_fetches / ext, 3),
+                        "instances": round(cst.instance_fetches / ext, 3)}
+    r_shade = roof("k_shade (shading + shadow rays + compaction, every scene's paths)", "k_shade", shade_bytes,
+                   kst.shade_ms, kst.shade_launches)
+    dominant, other = (r_ext, r_shade) if kst.extend_ms >= kst.shade_ms else (r_shade, r_ext)
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = batch_cpu_baseline(scenes)
+        except Exception as e:  # the baseline is reported, never the target
+            log(f"[bench] batch cpu baseline failed: {e}")
     out = {
         "metric": "Mrays/s",
         "value": round(rays_step * args.steps / wall / 1e6, 2),
@@ -579,8 +644,12 @@ def batch_workload(args):
                    "paths": paths, "primitives": prims},
         "samples_per_s": round(paths * args.steps / wall),
         "rays_per_step": {"closest_hit": st.extension_rays, "shadow": st.shadow_rays},
-        "tuned_shape": {"paths_per_batch": tune.paths_per_batch, "material_sort": bool(tune.material_sort),
-                        "graph": bool(tune.graph), "candidates": tune.candidates},
+        "tuned_shape": shape,
+        "kernel_ms_per_step": {"extend": round(kst.extend_ms, 3), "shade": round(kst.shade_ms, 3),
+                               "direct_launches_events": round(kst.gpu_ms, 3)},
+        "roofline": dominant,
+        "roofline_other": other,
+        "cpu_baseline": cpu,
         "build_ms_total": round(build_ms, 2),
         "one_render_per_scene_ms": round(seq_ms, 2),
         "batch_speedup": round(seq_ms / ms, 3),
@@ -591,7 +660,31 @@ def batch_workload(args):
             "reference": "oracle FP64 build, 8192 spp, stratified 1/256 pixel subset per scene; trimmed: without "
                          "the 0.1 % of pixels of largest error"},
     }
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def batch_cpu_baseline(scenes, block=3):
+    """The oracle on the host cores over every scene of the batch, each on
+    its stratified 1/block^2 pixel subset at its full sample count."""
+    sys.path.insert(0, str(REPO / "oracle"))
+    import ptref
+    from ptamd.metrics import stratified_pixels
+
+    cpu = host_cpu()
+    threads = max(1, min(CPU_THREADS_MAX, cpu["available"]))
+    rays, npix, dt = 0, 0, 0.0
+    for sc in scenes:
+        pix = stratified_pixels(sc.info.width, sc.info.height, block)
+        orc = ptref.OracleScene(sc.desc)
+        t1 = time.perf_counter()
+        _, st = orc.render_pixels(sc.camera, sc.render_params(), 0, sc.info.spp, pix, threads=threads)
+        dt += time.perf_counter() - t1
+        rays += int(st[1]) + int(st[2])
+        npix += len(pix)
+    return {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"all {len(scenes)} scenes of the batch, stratified 1/{block * block} pixel subsets ({npix} "
+                      f"pixels) at each scene's full sample count ({rays} rays), {dt:.1f} s of tracing (the CPU "
+                      f"structure builds not timed)", "host": cpu}
 
 
 def main():
@@ -611,8 +704,8 @@ def main():
     ap.add_argument("--eval-spp", type=int, default=64, help="cache workload: frames rendered per frozen view")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--shape", default="", metavar="FPB,SORT",
-                    help="plain workloads: frames per batch and material sort (0/1) of direct launches instead of "
-                         "the autotuner's pick")
+                    help="frames per batch and material sort (0/1) of direct launches instead of the autotuner's "
+                         "pick (batch workload: only the sort; the automatic sub-batch size)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -627,7 +720,7 @@ def main():
     if args.workload == "cache":
         cache_workload(args)
     elif args.workload == "batch":
-        batch_workload(args)
+        print(json.dumps(batch_workload(args)), flush=True)
     elif args.workload == "default":
         default_workload(args)
     else:

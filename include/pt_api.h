@@ -110,6 +110,10 @@ typedef int32_t pt_status;
                                         timing in the stats)                */
 #define PT_FLAG_TUNED 0x40u          /* use pt_autotune's batch size, sort
                                         and graph choice when it matches     */
+#define PT_FLAG_DIRECT 0x80u         /* launch directly even where the tuned
+                                        shape replays graphs: per-kernel
+                                        times in the stats (extend_ms,
+                                        shade_ms)                            */
 
 /* ------------------------------------------------------------ scene input */
 typedef struct pt_sphere_desc {

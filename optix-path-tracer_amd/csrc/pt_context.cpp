@@ -1170,8 +1170,9 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
   }
   if (frame_count == 0) return PT_OK;
   // launch shape: the autotuned one for this scene and resolution when asked
-  // for and available, else frames_per_batch or ~16 Mi paths in flight
-  const uint32_t tune_bits = PT_FLAG_TUNED | PT_FLAG_GRAPH | PT_FLAG_MATERIAL_SORT;
+  // for and available, else frames_per_batch or the automatic batch size
+  const uint32_t tune_bits =
+      PT_FLAG_TUNED | PT_FLAG_GRAPH | PT_FLAG_MATERIAL_SORT | PT_FLAG_DIRECT | PT_FLAG_COUNT_TRAVERSAL;
   const pt_context::Tuned& tu = c->tuned;
   const bool tuned = (rp->flags & PT_FLAG_TUNED) && tu.valid && tu.W == rp->width && tu.H == rp->height &&
                      tu.max_depth == rp->max_depth && (tu.flags & ~tune_bits) == (rp->flags & ~tune_bits);
@@ -1203,7 +1204,8 @@ pt_status pt_render(pt_context* c, const pt_render_params* rp, uint32_t frame_be
   // sorting pays only when more than one shading program can meet in a wave
   const bool sort = (tuned ? tu.sort : (rp->flags & PT_FLAG_MATERIAL_SORT) != 0) && c->shading_kinds > 1;
   // whole batches replayed from one captured graph (no per-kernel timing)
-  const bool use_graph = (tuned ? tu.graph : (rp->flags & PT_FLAG_GRAPH) != 0) && !use_cache;
+  const bool use_graph =
+      (tuned ? tu.graph : (rp->flags & PT_FLAG_GRAPH) != 0) && !use_cache && !(rp->flags & PT_FLAG_DIRECT);
   const bool kernel_events = want_stats && !use_graph;
   std::vector<hipEvent_t> ev;
   auto mark = [&]() -> hipEvent_t {
@@ -1693,7 +1695,8 @@ std::vector<uint8_t> batch_composition(pt_context* const* ctxs, const pt_render_
   for (uint32_t i = 0; i < n; i++) {
     add(&ctxs[i], sizeof(ctxs[i]));
     pt_render_params q = params[i];
-    q.flags &= ~(uint32_t)(PT_FLAG_TUNED | PT_FLAG_GRAPH | PT_FLAG_MATERIAL_SORT);
+    q.flags &= ~(uint32_t)(PT_FLAG_TUNED | PT_FLAG_GRAPH | PT_FLAG_MATERIAL_SORT | PT_FLAG_DIRECT |
+                           PT_FLAG_COUNT_TRAVERSAL);
     q.frames_per_batch = 0;
     add(&q, sizeof(q));
   }
@@ -1820,6 +1823,18 @@ pt_status render_batch_shaped(pt_context* const* ctxs, const pt_render_params* p
   }
   RenderConsts rc0 = rcs[0];
   rc0.flags &= ~(uint32_t)PT_FLAG_RADIANCE_CACHE;
+  // per-kernel HIP events around the closest-hit and shading launches of
+  // direct (not replayed) batches with statistics
+  const bool kernel_events = want_stats && !shape.graph;
+  std::vector<hipEvent_t> kev;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ext_ev, sh_ev;
+  auto mark = [&]() -> hipEvent_t {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    (void)hipEventRecord(e, B->stream);
+    kev.push_back(e);
+    return e;
+  };
   // the whole batch: clear the accumulations, then per sub-batch one raygen,
   // the bounce loop over every live path, the paths that outlived it, and
   // each scene's frames of the sub-batch accumulated in frame order
@@ -1835,11 +1850,15 @@ pt_status render_batch_shaped(pt_context* const* ctxs, const pt_render_params* p
       int in = 0, o = 1;
       for (uint32_t depth = 0; depth < max_depth; depth++) {
         const bool sd = sort && depth > 0;  // from the first bounce on, as in pt_render
+        hipEvent_t a = kernel_events ? mark() : nullptr;
         launch_extend(B->dsc, P, in, depth == 0 ? 1e-6f : 1e-3f, B->spill, B->stats, count_trav, sd, grid,
                       B->stream, B->d_scenes, nullptr, 0u, ext);
+        if (kernel_events) ext_ev.push_back({a, mark()});
         if (sd) launch_sort(P, in, grid, B->stream);
+        hipEvent_t a2 = kernel_events ? mark() : nullptr;
         launch_shade(B->dsc, P, in, sd, o, depth, rc0, matset, count_trav, B->spill, B->stats, grid, B->stream,
                      B->d_scenes, B->d_rcs, env, ext);
+        if (kernel_events) sh_ev.push_back({a2, mark()});
         std::swap(in, o);
       }
       launch_exhausted(P, in, B->stats, grid, B->stream);
@@ -1921,6 +1940,17 @@ pt_status render_batch_shaped(pt_context* const* ctxs, const pt_render_params* p
     out->gpu_ms = ms;
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
+    for (auto& e : ext_ev) {
+      float m = 0.f;
+      if (e.first && e.second && hipEventElapsedTime(&m, e.first, e.second) == hipSuccess) out->extend_ms += m;
+    }
+    for (auto& e : sh_ev) {
+      float m = 0.f;
+      if (e.first && e.second && hipEventElapsedTime(&m, e.first, e.second) == hipSuccess) out->shade_ms += m;
+    }
+    out->extend_launches = (uint32_t)ext_ev.size();
+    out->shade_launches = (uint32_t)sh_ev.size();
+    for (hipEvent_t e : kev) (void)hipEventDestroy(e);
     uint32_t ovf = 0;
     HIPCHK(B, hipMemcpy(&ovf, P.counters + CTR_STACK_OVERFLOW, 4, hipMemcpyDeviceToHost));
     if (ovf) return fail(B, PT_ERR_DEVICE, "traversal stack overflow");
@@ -1950,6 +1980,7 @@ pt_status pt_render_batch(pt_context* const* ctxs, const pt_render_params* param
     shape.sort = B->batch_tuned.sort;
     shape.graph = B->batch_tuned.graph;
   }
+  if (f0 & PT_FLAG_DIRECT) shape.graph = false;
   return render_batch_shaped(ctxs, params, frame_counts, n, out, shape);
 }
 

@@ -39,6 +39,7 @@ PT_FLAG_COUNT_TRAVERSAL = 0x8
 PT_FLAG_RADIANCE_CACHE = 0x10
 PT_FLAG_GRAPH = 0x20
 PT_FLAG_TUNED = 0x40
+PT_FLAG_DIRECT = 0x80
 PT_SCENE_RANDOM_SPHERES, PT_SCENE_CORNELL, PT_SCENE_MESHES, PT_SCENE_BATCH, PT_SCENE_MOVING_SPHERES = 0, 1, 2, 3, 4
 PT_SCENE_NEXT_WEEK = 5
 PT_CACHE_INPUTS, PT_CACHE_OUTPUTS, PT_CACHE_PARAMS = 96, 3, 18883
