@@ -1845,14 +1845,15 @@ pt_status render_batch_shaped(pt_context* const* ctxs, const pt_render_params* p
       const size_t k0 = first[b], k1 = first[b + 1];
       if (nslots == 0) continue;
       const int grid = grid_for(nslots);
-      launch_raygen_batch(P, B->d_rcs, B->d_item_off + k0, B->d_item_scene + k0, B->d_item_frame + k0,
-                          (uint32_t)(k1 - k0), nslots, grid, B->stream);
+      // the depth-0 closest-hit launch starts the sub-batch's camera paths
+      const BatchStart bs{B->d_rcs, B->d_item_off + k0, B->d_item_scene + k0, B->d_item_frame + k0,
+                          (uint32_t)(k1 - k0)};
       int in = 0, o = 1;
       for (uint32_t depth = 0; depth < max_depth; depth++) {
         const bool sd = sort && depth > 0;  // from the first bounce on, as in pt_render
         hipEvent_t a = kernel_events ? mark() : nullptr;
         launch_extend(B->dsc, P, in, depth == 0 ? 1e-6f : 1e-3f, B->spill, B->stats, count_trav, sd, grid,
-                      B->stream, B->d_scenes, nullptr, 0u, ext);
+                      B->stream, B->d_scenes, nullptr, depth == 0 ? nslots : 0u, ext, depth == 0 ? &bs : nullptr);
         if (kernel_events) ext_ev.push_back({a, mark()});
         if (sd) launch_sort(P, in, grid, B->stream);
         hipEvent_t a2 = kernel_events ? mark() : nullptr;

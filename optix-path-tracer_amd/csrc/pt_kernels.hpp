@@ -36,22 +36,28 @@ void launch_trace_closest(const DevScene& sc, const pt_ray* rays, uint64_t n, pt
                           uint2* spill, uint32_t* overflow, int grid, hipStream_t st);
 void launch_trace_occluded(const DevScene& sc, const pt_ray* rays, uint64_t n, uint8_t* occ, uint2* spill,
                            uint32_t* overflow, int grid, hipStream_t st);
-// a batch of scenes: slots [item_off[k], item_off[k + 1]) are frame
-// item_frame[k] of scene item_scene[k] (tables in device memory)
-void launch_raygen_batch(const DevPaths& P, const RenderConsts* rcs, const uint32_t* item_off,
-                         const uint32_t* item_scene, const uint32_t* item_frame, uint32_t nitems, uint32_t nslots,
-                         int grid, hipStream_t st);
 // frame_dev[0] = value (k_set_frame) / frame_dev[0] += step (k_next_frames)
 void launch_set_frame(uint32_t* frame_dev, uint32_t value, hipStream_t st);
 void launch_next_frames(uint32_t* frame_dev, uint32_t step, hipStream_t st);
+// The camera paths of a batch of scenes (device tables): slots
+// [off[k], off[k + 1]) are frame frame[k] of scene scene[k], k < nitems.
+struct BatchStart {
+  const RenderConsts* rcs;
+  const uint32_t* off;
+  const uint32_t* scene;
+  const uint32_t* frame;
+  uint32_t nitems;
+};
 // `scenes` (device table, non-null for a batch of scenes): each path's
 // scene comes from its RNG state's top byte instead of `sc`.
 // `start` (single scenes, depth 0, in == 0): the launch first starts the
-// nstart camera paths of the dense state set 0 with these constants
-// (k_raygen_batch does it for a batch of scenes).
+// nstart camera paths of the dense state set 0 with these constants.
+// `bstart` (a batch of scenes, depth 0, in == 0): the same for the nstart
+// paths of the batch's items.
 void launch_extend(const DevScene& sc, const DevPaths& P, int in, float tmin, uint2* spill, uint64_t* stats,
                    bool count_traversal, bool sort, int grid, hipStream_t st, const DevScene* scenes = nullptr,
-                   const RenderConsts* start = nullptr, uint32_t nstart = 0, bool multi_ext = false);
+                   const RenderConsts* start = nullptr, uint32_t nstart = 0, bool multi_ext = false,
+                   const BatchStart* bstart = nullptr);
 void launch_sort(const DevPaths& P, int in, int grid, hipStream_t st);
 void launch_shade(const DevScene& sc, const DevPaths& P, int in, bool sorted, int out, uint32_t depth,
                   const RenderConsts& rc, int matset, bool count, uint2* spill, uint64_t* stats, int grid,

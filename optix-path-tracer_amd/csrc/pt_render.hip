@@ -269,29 +269,6 @@ PTD void reset_segments(const DevPaths& P, uint32_t nslots, const Seg& g) {
   }
 }
 
-// The same for a batch of scenes: slots [item_off[k], item_off[k + 1]) are
-// frame item_frame[k] of scene item_scene[k] (k < nitems, item_off[0] = 0,
-// item_off[nitems] = nslots); the scene index goes into the RNG state's top
-// byte.
-__global__ __launch_bounds__(BLOCK) void k_raygen_batch(DevPaths P, const RenderConsts* __restrict__ rcs,
-                                                        const uint32_t* __restrict__ item_off,
-                                                        const uint32_t* __restrict__ item_scene,
-                                                        const uint32_t* __restrict__ item_frame, uint32_t nitems,
-                                                        uint32_t nslots) {
-  reset_segments(P, nslots, my_seg());
-  for (uint32_t p = blockIdx.x * BLOCK + threadIdx.x; p < nslots; p += gridDim.x * BLOCK) {
-    const uint32_t s = slot_of(p, nslots);
-    uint32_t lo = 0, hi = nitems - 1;  // last item starting at or before s
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi + 1) >> 1;
-      if (item_off[mid] <= s) lo = mid;
-      else hi = mid - 1;
-    }
-    const uint32_t sid = item_scene[lo];
-    start_path(P, rcs[sid], p, s, item_frame[lo], s - item_off[lo], true, sid);
-  }
-}
-
 // material bucket of a hit: 0 = miss, 1 + material type otherwise
 PTD uint32_t material_key(const DevScene& sc, float4 h) {
   int pos = __float_as_int(h.y);
@@ -304,10 +281,11 @@ PTD uint32_t material_key(const DevScene& sc, float4 h) {
 // 5 waves per SIMD (at most 96 VGPRs, no spills): latency-bound traversal
 // of large meshes gains 3.5 % over 4 waves; the Cornell box is unchanged.
 //
-// START (depth 0 of a single scene, in == 0): the query starts the nstart
-// camera paths itself (raygen.cu:148-168, 50-74): position i of the
-// dense state set 0 is slot slot_of(i, nstart), its ray is stored for the
-// shading pass and traced at once.  The path start is VALU-bound (the 64
+// START (depth 0, in == 0): the query starts the nstart camera paths
+// itself (raygen.cu:148-168, 50-74): position i of the dense state set 0
+// is slot slot_of(i, nstart), its ray is stored for the shading pass and
+// traced at once.  A batch of scenes (MULTI) finds the slot's item
+// (scene and frame) in the BatchStart tables.  The path start is VALU-bound (the 64
 // round seed hash) and the traversal latency-bound, so one launch hides
 // most of the former under the latter.  A separate instantiation: the
 // path-start code costs the traversal of the other depths a few spills.
@@ -323,16 +301,16 @@ constexpr int extend_stk() { return (KINDS & 8) ? STK_LDS_EXTEND : 12; }
 template <bool COUNT, bool SORT, bool MULTI, bool EXT, int KINDS = KIND_ALL, bool START = false>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(extend_waves<KINDS, START>(), 8))) void k_extend(
     DevScene sc0, DevPaths P, int in, float tmin, uint2* spill, uint64_t* stats, const DevScene* __restrict__ scenes,
-    RenderConsts rc, uint32_t nstart) {
+    RenderConsts rc, uint32_t nstart, BatchStart bs) {
   constexpr int NS = extend_stk<KINDS>();
   __shared__ uint32_t s_ref[NS * BLOCK];
   __shared__ float s_tn[NS * BLOCK];
   StackT<NS> st = make_stack<NS>(s_ref, s_tn, spill, P.counters + CTR_STACK_OVERFLOW);
   const Seg g = my_seg();
-  constexpr bool starts = START && !MULTI;
+  constexpr bool starts = START;
   uint32_t n;
   if (starts) {
-    if (rc.frame_dev) rc.frame0 = *rc.frame_dev;  // replayed graphs take the batch's frame from memory
+    if (!MULTI && rc.frame_dev) rc.frame0 = *rc.frame_dev;  // replayed graphs take the batch's frame from memory
     n = dense_count(nstart, g);
     reset_segments(P, nstart, g);
   } else {
@@ -408,8 +386,21 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(extend_wa
       float4 o, d;
       float tm = 0.f;
       if (starts) {
-        const uint32_t s = slot_of(i, nstart), f = s / rc.WH;
-        const PathStart ps = start_path(P, rc, i, s, rc.frame0 + f, s - f * rc.WH, false, 0u);
+        const uint32_t s = slot_of(i, nstart);
+        PathStart ps;
+        if (MULTI) {
+          uint32_t lo = 0, hi = bs.nitems - 1;  // last item starting at or before s
+          while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (bs.off[mid] <= s) lo = mid;
+            else hi = mid - 1;
+          }
+          const uint32_t sid = bs.scene[lo];
+          ps = start_path(P, bs.rcs[sid], i, s, bs.frame[lo], s - bs.off[lo], true, sid);
+        } else {
+          const uint32_t f = s / rc.WH;
+          ps = start_path(P, rc, i, s, rc.frame0 + f, s - f * rc.WH, false, 0u);
+        }
         o = ps.o;
         d = ps.d;
         if (EXT && sc0.mov_instances) tm = ps.tm;
@@ -1149,12 +1140,6 @@ void launch_set_frame(uint32_t* frame_dev, uint32_t value, hipStream_t st) {
 void launch_next_frames(uint32_t* frame_dev, uint32_t step, hipStream_t st) {
   hipLaunchKernelGGL(k_next_frames, dim3(1), dim3(1), 0, st, frame_dev, step);
 }
-void launch_raygen_batch(const DevPaths& P, const RenderConsts* rcs, const uint32_t* item_off,
-                         const uint32_t* item_scene, const uint32_t* item_frame, uint32_t nitems, uint32_t nslots,
-                         int grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_raygen_batch, dim3(grid), dim3(BLOCK), 0, st, P, rcs, item_off, item_scene, item_frame, nitems,
-                     nslots);
-}
 // instantiation for the scene's kinds: rects + boxes (the Cornell box),
 // rects + meshes (meshes on a ground rect), spheres, analytic, all
 static int extend_kinds(uint32_t k) {
@@ -1164,12 +1149,14 @@ static int extend_kinds(uint32_t k) {
 }
 void launch_extend(const DevScene& sc, const DevPaths& P, int in, float tmin, uint2* spill, uint64_t* stats,
                    bool count_traversal, bool sort, int grid, hipStream_t st, const DevScene* scenes,
-                   const RenderConsts* start, uint32_t nstart, bool multi_ext) {
+                   const RenderConsts* start, uint32_t nstart, bool multi_ext, const BatchStart* bstart) {
   const RenderConsts rc = start ? *start : RenderConsts{};
   const bool starts = start != nullptr && scenes == nullptr && nstart != 0;
+  const bool bstarts = bstart != nullptr && scenes != nullptr && nstart != 0 && bstart->nitems != 0;
+  const BatchStart bs = bstarts ? *bstart : BatchStart{};
 #define PT_EXTEND(C, S, M, E, K, ST)                                                                                \
   hipLaunchKernelGGL((k_extend<C, S, M, E, K, ST>), dim3(grid), dim3(BLOCK), 0, st, sc, P, in, tmin, spill, stats, \
-                     scenes, rc, nstart)
+                     scenes, rc, nstart, bs)
 #define PT_EXTEND4(M, E, K, ST)                                          \
   do {                                                                   \
     if (count_traversal && sort) PT_EXTEND(true, true, M, E, K, ST);     \
@@ -1197,7 +1184,11 @@ void launch_extend(const DevScene& sc, const DevPaths& P, int in, float tmin, ui
       }                                                                                                  \
     }                                                                                                    \
   } while (0)
-  if (scenes && multi_ext) {  // a batch holding volumes, cylinders or moving spheres
+  if (bstarts && multi_ext) {
+    PT_EXTEND2(true, true, KIND_ALL, true);
+  } else if (bstarts) {
+    PT_EXTEND2(true, false, KIND_ALL, true);
+  } else if (scenes && multi_ext) {  // a batch holding volumes, cylinders or moving spheres
     PT_EXTEND4(true, true, KIND_ALL, false);
   } else if (scenes) {
     PT_EXTEND4(true, false, KIND_ALL, false);
