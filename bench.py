@@ -614,8 +614,8 @@ def batch_workload(args):
                 r[k] = p[k]
         return r
 
-    r_ext = roof("k_extend (closest hit, every scene's paths in one launch per bounce)", "k_extend", ext_bytes,
-                 kst.extend_ms, kst.extend_launches)
+    r_ext = roof("k_extend (closest hit, every scene's paths in one launch per bounce; camera paths started at "
+                 "depth 0)", "k_extend", ext_bytes, kst.extend_ms, kst.extend_launches)
     r_ext["per_ray"] = {"nodes": round(cst.node_fetches / ext, 3), "prims": round(cst.prim_fetches / ext, 3),
                         "instances": round(cst.instance_fetches / ext, 3)}
     r_shade = roof("k_shade (shading + shadow rays + compaction, every scene's paths)", "k_shade", shade_bytes,
