@@ -1835,9 +1835,10 @@ pt_status render_batch_shaped(pt_context* const* ctxs, const pt_render_params* p
     kev.push_back(e);
     return e;
   };
-  // the whole batch: clear the accumulations, then per sub-batch one raygen,
-  // the bounce loop over every live path, the paths that outlived it, and
-  // each scene's frames of the sub-batch accumulated in frame order
+  // the whole batch: clear the accumulations, then per sub-batch the bounce
+  // loop over every live path (its depth-0 launch starts the paths), the
+  // paths that outlived it, and each scene's frames of the sub-batch
+  // accumulated in frame order
   auto enqueue = [&]() {
     for (uint32_t i = 0; i < n; i++) launch_fill_f64(ctxs[i]->accum, (uint64_t)rcs[i].WH * 3, 0.0, B->stream);
     for (size_t b = 0; b + 1 < first.size(); b++) {

@@ -5,8 +5,8 @@
 // programs/materials/*.cu, miss programs, shadow any-hit).
 //
 // Wavefront, one bounce at a time over SoA path state in HBM:
-//   raygen -> [extend -> (material sort) -> shade -> shadow+connect]*
-//   -> accumulate
+//   [extend (depth 0: path start + camera rays) -> (material sort)
+//    -> shade + shadow ray + connect]* -> accumulate
 // Path sets are split into wave segments (pt_layout.hpp): each wave reads
 // its own segment and compacts its survivors into its own segment of the
 // next set with a ballot, so the bounce loop takes no global atomic.  The
@@ -827,8 +827,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
       }
       if (MULTI && alive && depth + 1 >= rc.max_depth) {
         // the last bounce of this path's scene: it ends like the paths that
-        // outlive a single scene's loop (k_exhausted), result left as raygen
-        // set it
+        // outlive a single scene's loop (k_exhausted), result left as the
+        // path start set it
         alive = false;
         exhausted++;
         P.sig_final[dw & SLOT_MASK] = sig;
