@@ -61,7 +61,7 @@ constexpr uint32_t ENV_GUIDE = 64;
 // (0: the wave's lanes take 64 paths at a time, each group waiting for its
 // longest ray)
 #ifndef PT_EXTEND_REFILL
-#define PT_EXTEND_REFILL 32
+#define PT_EXTEND_REFILL 16
 #endif
 constexpr int STK_LDS_EXTEND = PT_EXTEND_STK;
 // Grid of the path kernels: at most MAX_GRID blocks of BLOCK / 64 waves.
