@@ -253,7 +253,10 @@ inline float bf(uint32_t u) {
   return f;
 }
 
-int grid_for(uint64_t n) { return (int)std::max<uint64_t>(1, std::min<uint64_t>((n + BLOCK - 1) / BLOCK, MAX_GRID)); }
+int grid_for(uint64_t n) {
+  const uint64_t cap = n >= GRID_CAP_PATHS ? (uint64_t)GRID_CAP : (uint64_t)GRID_SMALL;
+  return (int)std::max<uint64_t>(1, std::min<uint64_t>((n + BLOCK - 1) / BLOCK, cap));
+}
 
 // an odd stride near 0.618 n that is coprime to n (a full-period
 // multiplicative order of 0..n-1)

@@ -64,8 +64,18 @@ constexpr uint32_t ENV_GUIDE = 64;
 #define PT_EXTEND_REFILL 16
 #endif
 constexpr int STK_LDS_EXTEND = PT_EXTEND_STK;
-// Grid of the path kernels: at most MAX_GRID blocks of BLOCK / 64 waves.
-constexpr int MAX_GRID = 2048;
+// Grid of the path kernels (blocks of BLOCK / 64 waves).  Pools of at
+// least GRID_CAP_PATHS paths take up to GRID_CAP blocks: 5120 blocks are
+// whole rounds of both the closest-hit kernel (5 waves per SIMD: 1280
+// resident blocks on 256 CUs) and the shading kernel (4 waves: 1024), so
+// neither ends a launch with part of the chip idle.  Smaller pools take
+// up to GRID_SMALL blocks: their deep bounces are short launches, which
+// run faster in fewer, fuller wave segments.  MAX_GRID sizes the per-wave
+// tables.
+constexpr int MAX_GRID = 8192;
+constexpr int GRID_CAP = 5120;
+constexpr int GRID_SMALL = 2048;
+constexpr uint64_t GRID_CAP_PATHS = 1ull << 27;
 constexpr int WAVES = BLOCK / 64;
 constexpr int MAX_WAVES = MAX_GRID * WAVES;
 

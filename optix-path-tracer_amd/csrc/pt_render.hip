@@ -434,7 +434,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(extend_wa
   if (__lane_id() == 0) {
     if (SORT) {
 #pragma unroll
-      for (int k = 0; k < 8; k++) P.bcount[(size_t)k * MAX_WAVES + g.w] = bc[k];
+      for (int k = 0; k < 8; k++) P.bcount[(size_t)k * (gridDim.x * WAVES) + g.w] = bc[k];
     }
     stat_add(stats, g, STAT_EXT, n);
     if (COUNT) {
@@ -447,27 +447,26 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(extend_wa
 
 // Exclusive scan of the (bucket, wave) counts in bucket-major order: the
 // first sorted position of each wave's paths of each bucket, and the size
-// of the sorted set.  The [8][MAX_WAVES] table is scanned whole, entries of
-// waves >= `waves` read as 0.  Each of the block's 16 waves owns a
-// contiguous chunk of 4096 entries, read coalesced 16 loads at a time: a
-// first pass sums the chunk, and after the chunk totals are exchanged a
-// second pass (re-reading from L2) scans 64 entries at a time across the
-// wave's lanes and writes the positions.
+// of the sorted set.  The table holds 8 rows of `waves` entries (entry
+// k * waves + w: bucket k of wave w, `waves` the launch grid's waves).
+// Each of the block's 16 waves owns a contiguous chunk of whole 64-entry
+// rows, read coalesced 16 loads at a time: a first pass sums the chunk,
+// and after the chunk totals are exchanged a second pass (re-reading from
+// L2) scans 64 entries at a time across the wave's lanes and writes the
+// positions.
 constexpr int SCAN_THREADS = 1024;
-constexpr int SCAN_ITERS = 8 * MAX_WAVES / SCAN_THREADS;  // 64-entry rows per wave
 constexpr int SCAN_BATCH = 16;
-static_assert(SCAN_ITERS * SCAN_THREADS == 8 * MAX_WAVES, "scan covers the table");
-static_assert(SCAN_ITERS % SCAN_BATCH == 0, "whole batches");
-static_assert((MAX_WAVES & (MAX_WAVES - 1)) == 0, "wave index is the low bits of an entry");
 __global__ __launch_bounds__(SCAN_THREADS) void k_sort_scan(DevPaths P, uint32_t waves) {
   constexpr int NW = SCAN_THREADS / 64;
   __shared__ uint32_t wsum[NW];
   const uint32_t lane = __lane_id(), wv = threadIdx.x >> 6;
-  const uint32_t e0 = wv * (SCAN_ITERS * 64) + lane;
-  auto count = [&](uint32_t e) -> uint32_t { return (e & (MAX_WAVES - 1)) < waves ? P.bcount[e] : 0u; };
+  const uint32_t total = 8u * waves, rows = (total + 63u) / 64u;
+  const uint32_t per = ((rows + NW - 1) / NW + SCAN_BATCH - 1) / SCAN_BATCH * SCAN_BATCH;  // rows per wave
+  const uint32_t e0 = wv * per * 64u + lane;
+  auto count = [&](uint32_t e) -> uint32_t { return e < total ? P.bcount[e] : 0u; };
   uint32_t sum = 0;
 #pragma unroll 1
-  for (int b0 = 0; b0 < SCAN_ITERS; b0 += SCAN_BATCH) {
+  for (uint32_t b0 = 0; b0 < per; b0 += SCAN_BATCH) {
     uint32_t v[SCAN_BATCH];
 #pragma unroll
     for (int i = 0; i < SCAN_BATCH; i++) v[i] = count(e0 + 64u * (b0 + i));
@@ -486,7 +485,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_sort_scan(DevPaths P, uint32_t
     all += t;
   }
 #pragma unroll 1
-  for (int b0 = 0; b0 < SCAN_ITERS; b0 += SCAN_BATCH) {
+  for (uint32_t b0 = 0; b0 < per; b0 += SCAN_BATCH) {
     uint32_t v[SCAN_BATCH];
 #pragma unroll
     for (int i = 0; i < SCAN_BATCH; i++) v[i] = count(e0 + 64u * (b0 + i));
@@ -499,7 +498,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_sort_scan(DevPaths P, uint32_t
         if (lane >= (uint32_t)off) s += t;
       }
       const uint32_t e = e0 + 64u * (b0 + i);
-      if ((e & (MAX_WAVES - 1)) < waves) P.bbase[e] = run + s - v[i];
+      if (e < total) P.bbase[e] = run + s - v[i];
       run += __shfl(s, 63);
     }
   }
@@ -513,7 +512,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_scatter(DevPaths P, int in) {
   const PathState S = P.st[in], D = P.st[2];
   uint32_t cur[8];
 #pragma unroll
-  for (int k = 0; k < 8; k++) cur[k] = uniform(P.bbase[(size_t)k * MAX_WAVES + g.w]);
+  for (int k = 0; k < 8; k++) cur[k] = uniform(P.bbase[(size_t)k * (gridDim.x * WAVES) + g.w]);
   for (uint32_t j0 = 0; j0 < n; j0 += 64) {
     const uint32_t j = j0 + __lane_id(), i = g.pos(j);
     const uint32_t key = j < n ? P.keys[i] : 0xFFu;
