@@ -1046,7 +1046,7 @@ pt_status pt_set_camera(pt_context* c, const pt_camera* cam) {
 // another context never needs one)
 static pt_status ensure_spill(pt_context* c) {
   if (c->spill) return PT_OK;
-  HIPCHK(c, hipMalloc(&c->spill, (size_t)MAX_GRID * BLOCK * STK_TOTAL * sizeof(uint2)));
+  HIPCHK(c, hipMalloc(&c->spill, (size_t)GRID_CAP * BLOCK * STK_TOTAL * sizeof(uint2)));
   return PT_OK;
 }
 
